@@ -1,0 +1,167 @@
+"""Benchmark of the explicit solid MPM step on one MI355X.
+
+A step is one pass of the hot path over the whole particle set:
+cell binning (keys + radix sort + cell ranges), P2G with the grid update,
+G2P velocity + MUSL remap, and the dF/F/J/x/V/stress update.
+Workload (default): the 3D Neo-Hookean cube on a 128^3 grid at 2x4x2
+points per cell = 4,194,304 particles, synthetic (seeded) data, FP32
+arithmetic with compensated node sums, FP64 det F and volume.
+
+Prints one JSON line: particle-steps/s over the timed steps, the roofline of
+the dominant kernel (algorithmic bytes per launch / HIP-event time on the
+engine's stream), and the FP64 CPU oracle timed on the host over a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hpc-mpm-code_amd"))
+
+import numpy as np  # noqa: E402
+
+from mpm_mi355x import capi, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic bytes per particle and per active node of each FP32 kernel
+# (compulsory loads/stores of that kernel's inputs and outputs in the
+# engine's storage format; the cell-partial buffer between a cells kernel
+# and its node pass is not counted).
+BYTES = {
+    # perm 4 + key 4 + x (hi,lo) 24 + v 12 + m 4 + V(f64) 8 + stress 24
+    "p2g_cells": (80, 0),
+    # node m 4 + v 12 + a 12 written
+    "p2g_nodes": (0, 28),
+    # perm 4 + key 4 + x 24 + m 4 read, v 12 written; node v, a read 24
+    "musl_cells": (48, 24),
+    # node m 4 read, MUSL v 12 written
+    "musl_nodes": (0, 16),
+    # key 4 + x 24 + H 36 + V0 4 read; x 24 + H 36 + stress 24 + J 8 + V 8 + key 4 written; node v 12
+    "update": (172, 12),
+    # radix sort (3 passes of 8 bits), cell ranges
+    "bin": (60, 0),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="cfg2-4M", choices=["cfg1", "cfg2", "cfg2-4M"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--relayout", type=int, default=32)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def make_scene(name):
+    if name == "cfg1":
+        return scenes.cfg1()
+    if name == "cfg2":
+        return scenes.cfg2(ppc=(2, 2, 2))
+    return scenes.cfg2(ppc=(2, 4, 2))
+
+
+def cpu_baseline(scene, nsteps):
+    """FP64 oracle (timing build, -O3 -march=native), one core, on the same
+    workload for a few steps; step loop only."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    o = pyoracle.Oracle(scene, fast=True)
+    t0 = time.perf_counter()
+    rc = o.run(1, nsteps)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        return None
+    return {"value": scene.n * nsteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "%s, %d particles, %d steps (FP64 oracle, -O3 -march=native, 1 thread), %.1f s" % (
+                scene.name, scene.n, nsteps, dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this engine has no multi-GPU path (see DESIGN.md for the sharding "
+              "design); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    scene = make_scene(a.config)
+    prec = capi.PREC_FP32 if a.precision == "fp32" else capi.PREC_FP64_EXACT
+    eng = capi.Engine(scene, precision=prec, relayout_every=a.relayout if prec == capi.PREC_FP32 else 0)
+    eng.upload(scene.parts)
+    eng.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
+    capi.check(eng.step(1, a.warmup))
+    eng.set_timing(True)
+    capi.check(eng.lib.mpm_sync(eng.h))
+    t0 = time.perf_counter()
+    capi.check(eng.step(1 + a.warmup, a.steps))
+    capi.check(eng.lib.mpm_sync(eng.h))
+    wall = time.perf_counter() - t0
+    kt = eng.kernel_times()
+    n = scene.n
+    value = n * a.steps / wall
+    ms_step = 1e3 * wall / a.steps
+
+    nact = kt["active_nodes"]
+    per = {
+        "bin": kt["bin_ms"], "p2g_cells": kt["p2g_cells_ms"], "p2g_nodes": kt["p2g_nodes_ms"],
+        "musl_cells": kt["musl_cells_ms"], "musl_nodes": kt["musl_nodes_ms"], "update": kt["g2p_upd_ms"],
+    }
+    kernels = {}
+    for k, ms in per.items():
+        bp, bn = BYTES[k]
+        byts = bp * n + bn * nact
+        avg_ms = ms / a.steps
+        gbs = byts / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        kernels[k] = {"avg_us": round(avg_ms * 1e3, 2), "alg_bytes": byts, "GBps": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max((k for k in kernels if k != "bin"), key=lambda k: kernels[k]["avg_us"])
+    d = kernels[dom]
+    roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d["frac"], "traffic": None, "kernel": dom}
+    # P2G and G2P as whole passes
+    def phase(keys):
+        b = sum(kernels[k]["alg_bytes"] for k in keys)
+        us = sum(kernels[k]["avg_us"] for k in keys)
+        return {"avg_us": round(us, 2), "GBps": round(b / (us * 1e-6) / 1e9, 1),
+                "frac": round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    phases = {"p2g": phase(["p2g_cells", "p2g_nodes"]), "g2p": phase(["musl_cells", "musl_nodes", "update"])}
+
+    cpu = None
+    if not a.no_cpu:
+        cpu_scene = make_scene(a.config)
+        cpu = cpu_baseline(cpu_scene, a.cpu_steps)
+
+    out = {
+        "metric": "particle-steps/sec on 1 MI355X (plain explicit MPM step)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if prec == capi.PREC_FP32 else "fp64",
+        "data": "synthetic",
+        "config": {"workload": scene.name, "particles": n, "grid": "x".join(map(str, scene.nelem)),
+                   "active_cells": kt["active_cells"], "active_nodes": nact, "method": "plain (no surrogate)",
+                   "scheme": "PIC/MUSL", "relayout_every": a.relayout},
+        "roofline": roofline,
+        "phases": phases,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

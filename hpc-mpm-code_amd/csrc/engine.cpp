@@ -1,0 +1,957 @@
+// Host side of the C ABI (include/mpm/mpm_capi.h): device memory, layout
+// conversion between the reference's AoS FP64 arrays and the device SoA,
+// and the orchestration of the per-pass and whole-step kernels.
+#include "mpm_common.h"
+#include "mpm_kernels.h"
+
+#include "../../include/mpm/mpm_capi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace mpm {
+
+static thread_local std::string g_last_error;
+
+static int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    g_last_error = buf;
+    return MPM_ERR_HIP;
+}
+
+static int bit_length(uint32_t v) {
+    int b = 0;
+    while (v) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+}  // namespace mpm
+
+using namespace mpm;
+
+enum { OPT_FULL_NODE_OUTPUT = 1, OPT_RELAYOUT_EVERY = 2 };
+
+struct mpm_engine {
+    GridGeom g{};
+    mpm_step_params prm{};
+    MaterialTable mt{};
+    int prec = MPM_PREC_FP64_EXACT;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // boundary conditions, host lists and dense device form
+    std::vector<int32_t> nbc[4];
+    std::vector<double> fbc[4];
+    uint8_t *d_bcmask = nullptr;
+    double *d_bcval = nullptr;
+    bool bc_dirty = true;
+
+    int n = 0;
+    size_t ld = 0;
+    bool has_trac = false, has_matid = false, has_epsp = false;
+
+    PartF64 P64{};
+    NodesF64 N64{};
+
+    PartF32 P32[2]{};
+    int *pid[2] = {nullptr, nullptr};
+    int cur = 0;
+    NodesF32 N32{};
+    float *partials = nullptr;
+    NodeTables T{};
+    float *d_tn = nullptr;
+    float m_small = 0.f;
+
+    uint32_t *keys64 = nullptr;  // FP64 mode cell keys by particle
+    SortBuffers sb{};
+    uint32_t *perm = nullptr;    // points into sb (or perm_id after relayout)
+    uint32_t *perm_id = nullptr;
+    uint32_t *cstart = nullptr, *cend = nullptr;
+    uint32_t *d_err = nullptr;
+    bool bins_valid = false;
+    bool keys_valid = false;
+    int last_pass = 0;  // 1 = p2g, 2 = g2p
+    int steps_since_relayout = 0;
+    int full_node_output = 1;
+    int relayout_every = 0;
+    int active_cells = 0;
+
+    bool timing = false;
+    mpm_kernel_times kt{};
+    std::vector<hipEvent_t> ev;
+
+    std::vector<void *> allocs;
+    std::vector<void *> part_allocs;
+
+    template <typename T>
+    int dalloc(T **p, size_t count, bool particle = false) {
+        *p = nullptr;
+        if (count == 0) return 0;
+        hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+        (particle ? part_allocs : allocs).push_back((void *)*p);
+        return 0;
+    }
+
+    void free_particles() {
+        for (void *p : part_allocs) (void)hipFree(p);
+        part_allocs.clear();
+        P64 = PartF64{};
+        P32[0] = PartF32{};
+        P32[1] = PartF32{};
+        pid[0] = pid[1] = nullptr;
+        keys64 = nullptr;
+        sb = SortBuffers{};
+        perm = nullptr;
+        perm_id = nullptr;
+        n = 0;
+        bins_valid = keys_valid = false;
+    }
+
+    ~mpm_engine() {
+        free_particles();
+        for (void *p : allocs) (void)hipFree(p);
+        for (auto e : ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    BcDense bc() const { return BcDense{d_bcmask, d_bcval}; }
+    CellLists lists() const { return CellLists{perm, cstart, cend}; }
+};
+
+namespace {
+
+int upload_bc(mpm_engine *e) {
+    if (!e->bc_dirty) return 0;
+    const int nodec = e->g.nodec;
+    std::vector<uint8_t> mask(nodec, 0);
+    std::vector<double> val(3 * (size_t)nodec, 0.0);
+    // BCSetVal order of var_trans_solid_explicit.cpp:91-96: u, v, w, then rigid on all comps
+    for (int l = 0; l < 4; l++) {
+        for (size_t i = 0; i < e->nbc[l].size(); i++) {
+            int nd = e->nbc[l][i];
+            for (int c = 0; c < 3; c++) {
+                if (l < 3 && c != l) continue;
+                mask[nd] |= (uint8_t)(1u << c);
+                val[(size_t)c * nodec + nd] = e->fbc[l][i];
+            }
+        }
+    }
+    MPM_HIP_CHECK(hipMemcpyAsync(e->d_bcmask, mask.data(), nodec, hipMemcpyHostToDevice, e->stream));
+    MPM_HIP_CHECK(hipMemcpyAsync(e->d_bcval, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->bc_dirty = false;
+    return 0;
+}
+
+double facl_of(const mpm_engine *e, int istep) {
+    // work/src_solid/explicit/main.cpp:43-48
+    if (istep <= e->prm.nlstep) return (1.0 / (double)e->prm.nlstep) * (double)istep;
+    return 1.0;
+}
+
+StepConsts consts(const mpm_engine *e, double facl) {
+    StepConsts c{};
+    c.bx = (float)(e->prm.bb[0] * facl);
+    c.by = (float)(e->prm.bb[1] * facl);
+    c.bz = (float)(e->prm.bb[2] * facl);
+    c.dt = (float)e->prm.dt;
+    c.dt64 = e->prm.dt;
+    c.mtol = e->prm.mtol;
+    c.facl = facl;
+    c.m_small = e->m_small;
+    c.scheme = e->prm.scheme;
+    return c;
+}
+
+int check_err(mpm_engine *e) {
+    uint32_t h = 0;
+    MPM_HIP_CHECK(hipMemcpyAsync(&h, e->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    if (h & kErrOutOfGrid) return set_error(MPM_ERR_OUT_OF_GRID, "a particle is outside the background grid");
+    if (h & kErrNonFinite) return set_error(MPM_ERR_NONFINITE, "non-finite particle state");
+    if (h & kErrJacobian) return set_error(MPM_ERR_JACOBIAN, "det F <= 0");
+    return 0;
+}
+
+int clear_err(mpm_engine *e) {
+    launch_fill_u32(e->stream, e->d_err, 1, 0u);
+    return 0;
+}
+
+// keys (if needed) + radix sort + cell ranges
+int do_bin(mpm_engine *e) {
+    const int n = e->n;
+    if (!e->keys_valid) {
+        if (e->prec == MPM_PREC_FP64_EXACT)
+            launch_keys_f64(e->stream, e->g, e->P64.x, e->ld, n, e->keys64, e->d_err);
+        else {
+            PartF32 &P = e->P32[e->cur];
+            launch_keys_hilo(e->stream, e->g, P.xh, P.xl, e->ld, n, P.key, e->d_err);
+        }
+        e->keys_valid = true;
+    }
+    const uint32_t *keys = (e->prec == MPM_PREC_FP64_EXACT) ? e->keys64 : e->P32[e->cur].key;
+    int nbits = bit_length((uint32_t)e->g.nelem);
+    uint32_t *sk = nullptr, *sv = nullptr;
+    int rc = radix_sort_pairs(e->stream, e->sb, keys, n, nbits, &sk, &sv);
+    if (rc) return set_error(MPM_ERR_HIP, "radix sort launch failed");
+    launch_fill_u32(e->stream, e->cstart, (size_t)e->g.nelem, 0u);
+    launch_fill_u32(e->stream, e->cend, (size_t)e->g.nelem, 0u);
+    launch_cell_ranges(e->stream, sk, n, e->g.nelem, e->cstart, e->cend);
+    e->perm = sv;
+    if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && ++e->steps_since_relayout >= e->relayout_every) {
+        // physically reorder particle slots into the sorted order
+        const int o = 1 - e->cur;
+        launch_f32_permute(e->stream, e->P32[e->cur], e->P32[o], e->perm, e->pid[e->cur], e->pid[o]);
+        e->cur = o;
+        launch_iota_u32(e->stream, e->perm_id, n);
+        e->perm = e->perm_id;
+        e->steps_since_relayout = 0;
+    }
+    e->bins_valid = true;
+    MPM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int do_p2g(mpm_engine *e, double facl, int diag, hipEvent_t mid = nullptr) {
+    if (e->prec == MPM_PREC_FP64_EXACT) {
+        launch_f64_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.bb[0] * facl,
+                       e->prm.bb[1] * facl, e->prm.bb[2] * facl, e->prm.mtol, facl);
+    } else {
+        launch_f32_p2g(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, consts(e, facl),
+                       e->partials, diag, mid);
+    }
+    e->last_pass = 1;
+    MPM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int do_solve(mpm_engine *e) {
+    if (e->prec == MPM_PREC_FP64_EXACT) launch_f64_solve_grid(e->stream, e->g, e->N64, e->bc(), e->prm.mtol);
+    // FP32: the acceleration is formed in the P2G node pass
+    MPM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2 = nullptr) {
+    if (e->prec == MPM_PREC_FP64_EXACT) {
+        launch_f64_g2p(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->mt, e->prm.scheme, e->prm.dt,
+                       e->prm.mtol, facl, e->d_err);
+        e->keys_valid = false;
+    } else {
+        StepConsts c = consts(e, facl);
+        launch_f32_g2p_vel_musl(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, c, e->partials,
+                                diag, mid2);
+        if (mid) MPM_HIP_CHECK(hipEventRecord(mid, e->stream));
+        launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err);
+        e->keys_valid = true;  // the update kernel writes the new keys
+    }
+    e->bins_valid = false;
+    e->last_pass = 2;
+    MPM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int require_particles(mpm_engine *e) {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    if (e->n <= 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "no particles uploaded");
+    return 0;
+}
+
+int alloc_particles(mpm_engine *e, int n) {
+    e->free_particles();
+    e->n = n;
+    e->ld = (size_t)n;
+    const size_t ld = e->ld;
+    int rc = 0;
+    if (e->prec == MPM_PREC_FP64_EXACT) {
+        PartF64 &P = e->P64;
+        if ((rc = e->dalloc(&P.x, 3 * ld, true)) || (rc = e->dalloc(&P.v, 3 * ld, true)) ||
+            (rc = e->dalloc(&P.m, ld, true)) || (rc = e->dalloc(&P.V, ld, true)) ||
+            (rc = e->dalloc(&P.V0, ld, true)) || (rc = e->dalloc(&P.sig, 6 * ld, true)) ||
+            (rc = e->dalloc(&P.F, 9 * ld, true)) || (rc = e->dalloc(&P.J, ld, true)) ||
+            (rc = e->dalloc(&P.ddg, 9 * ld, true)) || (rc = e->dalloc(&e->keys64, ld, true)))
+            return rc;
+        if (e->has_trac && (rc = e->dalloc(&P.trac, 3 * ld, true))) return rc;
+        if (e->has_matid && (rc = e->dalloc(&P.matid, ld, true))) return rc;
+        if (e->has_epsp && (rc = e->dalloc(&P.eps_p, ld, true))) return rc;
+        P.key = e->keys64;
+        P.ld = ld;
+        P.n = n;
+    } else {
+        int nbuf = e->relayout_every > 0 ? 2 : 1;
+        for (int b = 0; b < nbuf; b++) {
+            PartF32 &P = e->P32[b];
+            if ((rc = e->dalloc(&P.xh, 3 * ld, true)) || (rc = e->dalloc(&P.xl, 3 * ld, true)) ||
+                (rc = e->dalloc(&P.v, 3 * ld, true)) || (rc = e->dalloc(&P.m, ld, true)) ||
+                (rc = e->dalloc(&P.V0, ld, true)) || (rc = e->dalloc(&P.V, ld, true)) ||
+                (rc = e->dalloc(&P.J, ld, true)) || (rc = e->dalloc(&P.H, 9 * ld, true)) ||
+                (rc = e->dalloc(&P.sig, 6 * ld, true)) || (rc = e->dalloc(&P.key, ld, true)) ||
+                (rc = e->dalloc(&e->pid[b], ld, true)))
+                return rc;
+            if (e->has_trac && (rc = e->dalloc(&P.trac, 3 * ld, true))) return rc;
+            if (e->has_matid && (rc = e->dalloc(&P.matid, ld, true))) return rc;
+            if (e->has_epsp && (rc = e->dalloc(&P.eps_p, ld, true))) return rc;
+            P.ld = ld;
+            P.n = n;
+        }
+        e->cur = 0;
+        if ((rc = e->dalloc(&e->perm_id, ld, true))) return rc;
+    }
+    SortBuffers &sb = e->sb;
+    if ((rc = e->dalloc(&sb.keys, ld, true)) || (rc = e->dalloc(&sb.vals, ld, true)) ||
+        (rc = e->dalloc(&sb.keys_alt, ld, true)) || (rc = e->dalloc(&sb.vals_alt, ld, true)) ||
+        (rc = e->dalloc(&sb.hist, radix_hist_entries(n), true)) || (rc = e->dalloc(&sb.bsum, 4096, true)))
+        return rc;
+    return 0;
+}
+
+template <typename T>
+int h2d(mpm_engine *e, T *dst, const std::vector<T> &src) {
+    if (src.empty()) return 0;
+    MPM_HIP_CHECK(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, e->stream));
+    return 0;
+}
+
+template <typename T>
+int d2h(mpm_engine *e, std::vector<T> &dst, const T *src, size_t count) {
+    dst.resize(count);
+    if (count == 0) return 0;
+    MPM_HIP_CHECK(hipMemcpyAsync(dst.data(), src, count * sizeof(T), hipMemcpyDeviceToHost, e->stream));
+    return 0;
+}
+
+int validate_grid(const mpm_grid_desc *g, bool allow_quadratic) {
+    for (int i = 0; i < 3; i++) {
+        if (g->nelem[i] <= 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "nelem must be positive");
+        if (!(g->xmax[i] > g->xmin[i])) return set_error(MPM_ERR_INVALID_ARGUMENT, "xmax must exceed xmin");
+        if (g->idimc[i] != 1 && !(allow_quadratic && g->idimc[i] == 2))
+            return set_error(MPM_ERR_UNSUPPORTED, "shape-function order must be 1 (linear) for the step engine");
+    }
+    double ne = (double)g->nelem[0] * g->nelem[1] * g->nelem[2];
+    double nn = (double)(g->nelem[0] + g->idimc[0]) * (g->nelem[1] + g->idimc[1]) * (g->nelem[2] + g->idimc[2]);
+    if (ne >= 2147483647.0 || nn >= 2147483647.0) return set_error(MPM_ERR_INVALID_ARGUMENT, "grid too large");
+    return 0;
+}
+
+GridGeom make_geom(const mpm_grid_desc *gd) {
+    GridGeom g{};
+    for (int i = 0; i < 3; i++) {
+        g.xmin[i] = gd->xmin[i];
+        g.dx[i] = (gd->xmax[i] - gd->xmin[i]) / (double)gd->nelem[i];  // datain_para.cpp:87
+        g.nex[i] = gd->nelem[i];
+        g.idimc[i] = gd->idimc[i];
+        g.nxc[i] = gd->nelem[i] + gd->idimc[i];
+    }
+    g.nelem = g.nex[0] * g.nex[1] * g.nex[2];
+    g.nodec = g.nxc[0] * g.nxc[1] * g.nxc[2];
+    return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpm_version(void) { return "mpm-mi355x 0.1.0 gfx950"; }
+
+const char *mpm_last_error(void) { return g_last_error.c_str(); }
+
+int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, int32_t nmat,
+                      const mpm_step_params *params, mpm_engine **out) {
+    if (!grid || !params || !out || (nmat > 0 && !mats))
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (nmat < 1 || nmat > kMaxMat) return set_error(MPM_ERR_INVALID_ARGUMENT, "nmat must be in 1..8");
+    int rc = validate_grid(grid, false);
+    if (rc) return rc;
+    if (!(params->dt > 0.0)) return set_error(MPM_ERR_INVALID_ARGUMENT, "dt must be positive");
+    if (params->scheme != MPM_SCHEME_PIC && params->scheme != MPM_SCHEME_FLIP)
+        return set_error(MPM_ERR_UNSUPPORTED, "scheme must be PIC or FLIP");
+    if (params->precision != MPM_PREC_FP64_EXACT && params->precision != MPM_PREC_FP32)
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (params->nlstep < 1) return set_error(MPM_ERR_INVALID_ARGUMENT, "nlstep must be >= 1");
+    for (int i = 0; i < nmat; i++) {
+        int m = mats[i].model;
+        if (!(m == -1 || m == 0 || m == 1 || m == 2 || m == 3 || m == 10))
+            return set_error(MPM_ERR_UNSUPPORTED, "unknown constitutive model");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(MPM_ERR_NO_DEVICE, "no HIP device");
+
+    mpm_engine *e = new mpm_engine();
+    e->g = make_geom(grid);
+    e->prm = *params;
+    e->prec = params->precision;
+    e->relayout_every = params->precision == MPM_PREC_FP32 ? std::max(0, (int)params->relayout_every) : 0;
+    e->mt.nmat = nmat;
+    for (int i = 0; i < nmat; i++) {
+        e->mt.m[i].model = mats[i].model;
+        for (int k = 0; k < 8; k++) e->mt.m[i].prop[k] = mats[i].prop[k];
+    }
+    hipError_t he = hipGetDevice(&e->device);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        int r = hip_fail(he, "hipStreamCreate", __FILE__, __LINE__);
+        delete e;
+        return r;
+    }
+    const GridGeom &g = e->g;
+    const size_t nodec = (size_t)g.nodec, nelem = (size_t)g.nelem;
+    if ((rc = e->dalloc(&e->d_bcmask, nodec)) || (rc = e->dalloc(&e->d_bcval, 3 * nodec)) ||
+        (rc = e->dalloc(&e->cstart, nelem)) || (rc = e->dalloc(&e->cend, nelem)) || (rc = e->dalloc(&e->d_err, 1))) {
+        delete e;
+        return rc;
+    }
+    if (e->prec == MPM_PREC_FP64_EXACT) {
+        NodesF64 &N = e->N64;
+        if ((rc = e->dalloc(&N.nm, nodec)) || (rc = e->dalloc(&N.nvof, nodec)) || (rc = e->dalloc(&N.np, 3 * nodec)) ||
+            (rc = e->dalloc(&N.nf, 3 * nodec)) || (rc = e->dalloc(&N.nv, 3 * nodec)) ||
+            (rc = e->dalloc(&N.na, 3 * nodec))) {
+            delete e;
+            return rc;
+        }
+    } else {
+        NodesF32 &N = e->N32;
+        if ((rc = e->dalloc(&N.nm, nodec)) || (rc = e->dalloc(&N.nvof, nodec)) || (rc = e->dalloc(&N.np, 3 * nodec)) ||
+            (rc = e->dalloc(&N.nf, 3 * nodec)) || (rc = e->dalloc(&N.nv, 3 * nodec)) ||
+            (rc = e->dalloc(&N.na, 3 * nodec)) || (rc = e->dalloc(&N.nvm, 3 * nodec)) ||
+            (rc = e->dalloc(&e->partials, (size_t)kPartialsPerCell * nelem))) {
+            delete e;
+            return rc;
+        }
+        // node coordinate tables, FP32 (hi, lo) pairs of xmin + dx * i
+        int tld = std::max(g.nxc[0], std::max(g.nxc[1], g.nxc[2]));
+        std::vector<float> tn(6 * (size_t)tld, 0.f);
+        for (int a = 0; a < 3; a++)
+            for (int i = 0; i < g.nxc[a]; i++) {
+                double xn = g.xmin[a] + g.dx[a] * (double)i;
+                float hi = (float)xn;
+                tn[(size_t)a * tld + i] = hi;
+                tn[(size_t)(3 + a) * tld + i] = (float)(xn - (double)hi);
+            }
+        if ((rc = e->dalloc(&e->d_tn, tn.size()))) {
+            delete e;
+            return rc;
+        }
+        he = hipMemcpy(e->d_tn, tn.data(), tn.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (he != hipSuccess) {
+            int r = hip_fail(he, "hipMemcpy", __FILE__, __LINE__);
+            delete e;
+            return r;
+        }
+        e->T.tnh = e->d_tn;
+        e->T.tnl = e->d_tn + 3 * (size_t)tld;
+        e->T.ld = tld;
+        for (int a = 0; a < 3; a++) e->T.inv_dx[a] = (float)(1.0 / g.dx[a]);
+    }
+    *out = e;
+    return 0;
+}
+
+int mpm_engine_destroy(mpm_engine *eng) {
+    if (!eng) return 0;
+    (void)hipStreamSynchronize(eng->stream);
+    delete eng;
+    return 0;
+}
+
+int mpm_set_bc(mpm_engine *e, int32_t comp, const int32_t *nbc, const double *fbc, int32_t ibc) {
+    if (!e || comp < 0 || comp > 3 || ibc < 0 || (ibc > 0 && !nbc))
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "bad BC arguments");
+    for (int i = 0; i < ibc; i++)
+        if (nbc[i] < 0 || nbc[i] >= e->g.nodec) return set_error(MPM_ERR_INVALID_ARGUMENT, "BC node out of range");
+    e->nbc[comp].assign(nbc, nbc + ibc);
+    if (fbc)
+        e->fbc[comp].assign(fbc, fbc + ibc);
+    else
+        e->fbc[comp].assign(ibc, 0.0);
+    e->bc_dirty = true;
+    return 0;
+}
+
+int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
+    if (!e || !p || p->n <= 0 || !p->coord || !p->mass || !p->vol0)
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "particle view needs n > 0, coord, mass, vol0");
+    if (p->n >= 2147483647LL) return set_error(MPM_ERR_INVALID_ARGUMENT, "too many particles");
+    const int n = (int)p->n;
+    if (p->matid)
+        for (int i = 0; i < n; i++)
+            if (p->matid[i] < 0 || p->matid[i] >= e->mt.nmat)
+                return set_error(MPM_ERR_INVALID_ARGUMENT, "matid out of range");
+    e->has_trac = p->trac_force != nullptr;
+    e->has_matid = p->matid != nullptr;
+    e->has_epsp = true;
+    int rc = alloc_particles(e, n);
+    if (rc) return rc;
+    const size_t ld = e->ld;
+    auto F_at = [&](int i, int q) -> double {
+        if (p->def_grad) return p->def_grad[9 * (size_t)i + q];
+        return (q == 0 || q == 4 || q == 8) ? 1.0 : 0.0;
+    };
+    double msum = 0.0;
+    for (int i = 0; i < n; i++) msum += p->mass[i];
+    e->m_small = (float)(1e-2 * msum / (double)n);
+    if (e->prec == MPM_PREC_FP64_EXACT) {
+        std::vector<double> x(3 * ld), v(3 * ld, 0.0), sig(6 * ld, 0.0), F(9 * ld), m(ld), V(ld), V0(ld), J(ld),
+            tr, ep(ld, 0.0);
+        for (int i = 0; i < n; i++) {
+            for (int a = 0; a < 3; a++) {
+                x[a * ld + i] = p->coord[3 * (size_t)i + a];
+                if (p->vel) v[a * ld + i] = p->vel[3 * (size_t)i + a];
+            }
+            if (p->stress)
+                for (int q = 0; q < 6; q++) sig[q * ld + i] = p->stress[6 * (size_t)i + q];
+            for (int q = 0; q < 9; q++) F[q * ld + i] = F_at(i, q);
+            m[i] = p->mass[i];
+            V0[i] = p->vol0[i];
+            V[i] = p->vol ? p->vol[i] : p->vol0[i];
+            J[i] = p->det_def_grad ? p->det_def_grad[i] : 1.0;
+            if (p->eps_p) ep[i] = p->eps_p[i];
+        }
+        PartF64 &P = e->P64;
+        if ((rc = h2d(e, P.x, x)) || (rc = h2d(e, P.v, v)) || (rc = h2d(e, P.sig, sig)) || (rc = h2d(e, P.F, F)) ||
+            (rc = h2d(e, P.m, m)) || (rc = h2d(e, P.V, V)) || (rc = h2d(e, P.V0, V0)) || (rc = h2d(e, P.J, J)) ||
+            (rc = h2d(e, P.eps_p, ep)))
+            return rc;
+        if (e->has_trac) {
+            tr.assign(3 * ld, 0.0);
+            for (int i = 0; i < n; i++)
+                for (int a = 0; a < 3; a++) tr[a * ld + i] = p->trac_force[3 * (size_t)i + a];
+            if ((rc = h2d(e, P.trac, tr))) return rc;
+        }
+        if (e->has_matid) {
+            std::vector<int> mi(p->matid, p->matid + n);
+            if ((rc = h2d(e, P.matid, mi))) return rc;
+        }
+    } else {
+        std::vector<float> xh(3 * ld), xl(3 * ld), v(3 * ld, 0.f), sig(6 * ld, 0.f), H(9 * ld), m(ld), V0(ld),
+            ep(ld, 0.f), tr;
+        std::vector<double> V(ld), J(ld);
+        std::vector<int> pidv(ld);
+        for (int i = 0; i < n; i++) {
+            for (int a = 0; a < 3; a++) {
+                double xv = p->coord[3 * (size_t)i + a];
+                float h = (float)xv;
+                xh[a * ld + i] = h;
+                xl[a * ld + i] = (float)(xv - (double)h);
+                if (p->vel) v[a * ld + i] = (float)p->vel[3 * (size_t)i + a];
+            }
+            if (p->stress)
+                for (int q = 0; q < 6; q++) sig[q * ld + i] = (float)p->stress[6 * (size_t)i + q];
+            for (int q = 0; q < 9; q++)
+                H[q * ld + i] = (float)(F_at(i, q) - ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0));
+            m[i] = (float)p->mass[i];
+            V0[i] = (float)p->vol0[i];
+            V[i] = p->vol ? p->vol[i] : p->vol0[i];
+            J[i] = p->det_def_grad ? p->det_def_grad[i] : 1.0;
+            if (p->eps_p) ep[i] = (float)p->eps_p[i];
+            pidv[i] = i;
+        }
+        PartF32 &P = e->P32[0];
+        if ((rc = h2d(e, P.xh, xh)) || (rc = h2d(e, P.xl, xl)) || (rc = h2d(e, P.v, v)) ||
+            (rc = h2d(e, P.sig, sig)) || (rc = h2d(e, P.H, H)) || (rc = h2d(e, P.m, m)) || (rc = h2d(e, P.V0, V0)) ||
+            (rc = h2d(e, P.V, V)) || (rc = h2d(e, P.J, J)) || (rc = h2d(e, P.eps_p, ep)) ||
+            (rc = h2d(e, e->pid[0], pidv)))
+            return rc;
+        if (e->has_trac) {
+            tr.assign(3 * ld, 0.f);
+            for (int i = 0; i < n; i++)
+                for (int a = 0; a < 3; a++) tr[a * ld + i] = (float)p->trac_force[3 * (size_t)i + a];
+            if ((rc = h2d(e, P.trac, tr))) return rc;
+        }
+        if (e->has_matid) {
+            std::vector<int> mi(p->matid, p->matid + n);
+            if ((rc = h2d(e, P.matid, mi))) return rc;
+        }
+    }
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->keys_valid = false;
+    e->bins_valid = false;
+    e->steps_since_relayout = 0;
+    return 0;
+}
+
+int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    if (!p || p->n != e->n) return set_error(MPM_ERR_INVALID_ARGUMENT, "view size differs from the engine's");
+    const int n = e->n;
+    const size_t ld = e->ld;
+    if (e->prec == MPM_PREC_FP64_EXACT) {
+        const PartF64 &P = e->P64;
+        std::vector<double> x, v, sig, F, V, J, ep;
+        if ((rc = d2h(e, x, P.x, 3 * ld)) || (rc = d2h(e, v, P.v, 3 * ld)) || (rc = d2h(e, sig, P.sig, 6 * ld)) ||
+            (rc = d2h(e, F, P.F, 9 * ld)) || (rc = d2h(e, V, P.V, ld)) || (rc = d2h(e, J, P.J, ld)) ||
+            (rc = d2h(e, ep, P.eps_p, ld)))
+            return rc;
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        for (int i = 0; i < n; i++) {
+            for (int a = 0; a < 3; a++) {
+                if (p->coord) p->coord[3 * (size_t)i + a] = x[a * ld + i];
+                if (p->vel) p->vel[3 * (size_t)i + a] = v[a * ld + i];
+            }
+            if (p->stress)
+                for (int q = 0; q < 6; q++) p->stress[6 * (size_t)i + q] = sig[q * ld + i];
+            if (p->def_grad)
+                for (int q = 0; q < 9; q++) p->def_grad[9 * (size_t)i + q] = F[q * ld + i];
+            if (p->vol) p->vol[i] = V[i];
+            if (p->det_def_grad) p->det_def_grad[i] = J[i];
+            if (p->eps_p && !ep.empty()) p->eps_p[i] = ep[i];
+        }
+    } else {
+        const PartF32 &P = e->P32[e->cur];
+        std::vector<float> xh, xl, v, sig, H, ep;
+        std::vector<double> V, J;
+        std::vector<int> pidv;
+        if ((rc = d2h(e, xh, P.xh, 3 * ld)) || (rc = d2h(e, xl, P.xl, 3 * ld)) || (rc = d2h(e, v, P.v, 3 * ld)) ||
+            (rc = d2h(e, sig, P.sig, 6 * ld)) || (rc = d2h(e, H, P.H, 9 * ld)) || (rc = d2h(e, V, P.V, ld)) ||
+            (rc = d2h(e, J, P.J, ld)) || (rc = d2h(e, ep, P.eps_p, ld)) || (rc = d2h(e, pidv, e->pid[e->cur], ld)))
+            return rc;
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        for (int s = 0; s < n; s++) {
+            const size_t i = (size_t)pidv[s];
+            for (int a = 0; a < 3; a++) {
+                if (p->coord) p->coord[3 * i + a] = (double)xh[a * ld + s] + (double)xl[a * ld + s];
+                if (p->vel) p->vel[3 * i + a] = v[a * ld + s];
+            }
+            if (p->stress)
+                for (int q = 0; q < 6; q++) p->stress[6 * i + q] = sig[q * ld + s];
+            if (p->def_grad)
+                for (int q = 0; q < 9; q++)
+                    p->def_grad[9 * i + q] = (double)H[q * ld + s] + ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0);
+            if (p->vol) p->vol[i] = V[s];
+            if (p->det_def_grad) p->det_def_grad[i] = J[s];
+            if (p->eps_p && !ep.empty()) p->eps_p[i] = ep[s];
+        }
+    }
+    return 0;
+}
+
+int mpm_download_nodes(mpm_engine *e, mpm_nodes_view *nv) {
+    if (!e || !nv) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    if (nv->nodec != e->g.nodec) return set_error(MPM_ERR_INVALID_ARGUMENT, "nodec differs from the engine's");
+    const size_t nodec = (size_t)e->g.nodec;
+    int rc = 0;
+    if (e->prec == MPM_PREC_FP64_EXACT) {
+        const NodesF64 &N = e->N64;
+        struct {
+            double *dst;
+            const double *src;
+            size_t cnt;
+        } items[] = {{nv->nmass, N.nm, nodec},  {nv->nvof, N.nvof, nodec},     {nv->nmome, N.np, 3 * nodec},
+                     {nv->nforce, N.nf, 3 * nodec}, {nv->nvel, N.nv, 3 * nodec}, {nv->naccel, N.na, 3 * nodec}};
+        for (auto &it : items)
+            if (it.dst) MPM_HIP_CHECK(hipMemcpyAsync(it.dst, it.src, it.cnt * sizeof(double), hipMemcpyDeviceToHost,
+                                                     e->stream));
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    } else {
+        const NodesF32 &N = e->N32;
+        const float *vel_src = (e->last_pass == 2) ? N.nvm : N.nv;
+        struct {
+            double *dst;
+            const float *src;
+            size_t cnt;
+        } items[] = {{nv->nmass, N.nm, nodec},  {nv->nvof, N.nvof, nodec},     {nv->nmome, N.np, 3 * nodec},
+                     {nv->nforce, N.nf, 3 * nodec}, {nv->nvel, vel_src, 3 * nodec}, {nv->naccel, N.na, 3 * nodec}};
+        std::vector<float> tmp;
+        for (auto &it : items) {
+            if (!it.dst) continue;
+            if ((rc = d2h(e, tmp, it.src, it.cnt))) return rc;
+            MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+            for (size_t i = 0; i < it.cnt; i++) it.dst[i] = tmp[i];
+        }
+    }
+    return 0;
+}
+
+int mpm_bin(mpm_engine *e) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    clear_err(e);
+    if ((rc = do_bin(e))) return rc;
+    return check_err(e);
+}
+
+int mpm_p2g(mpm_engine *e, double facl) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "call mpm_bin before mpm_p2g");
+    if ((rc = upload_bc(e))) return rc;
+    if ((rc = do_p2g(e, facl, 1))) return rc;
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int mpm_solve_grid(mpm_engine *e) {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    int rc = upload_bc(e);
+    if (rc) return rc;
+    if ((rc = do_solve(e))) return rc;
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int mpm_g2p(mpm_engine *e, double facl) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "call mpm_bin/mpm_p2g before mpm_g2p");
+    if ((rc = upload_bc(e))) return rc;
+    clear_err(e);
+    if ((rc = do_g2p(e, facl, 1, nullptr))) return rc;
+    return check_err(e);
+}
+
+int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *numep, int32_t *cell_of) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "no valid binning (call mpm_bin)");
+    const int n = e->n, nelem = e->g.nelem;
+    std::vector<uint32_t> perm, cs, ce, keys;
+    std::vector<int> pidv;
+    if ((rc = d2h(e, perm, e->perm, n)) || (rc = d2h(e, cs, e->cstart, nelem)) || (rc = d2h(e, ce, e->cend, nelem)))
+        return rc;
+    const uint32_t *kd = (e->prec == MPM_PREC_FP64_EXACT) ? e->keys64 : e->P32[e->cur].key;
+    if ((rc = d2h(e, keys, kd, n))) return rc;
+    if (e->prec == MPM_PREC_FP32) {
+        if ((rc = d2h(e, pidv, e->pid[e->cur], n))) return rc;
+    }
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    auto to_pid = [&](uint32_t slot) -> int { return pidv.empty() ? (int)slot : pidv[slot]; };
+    if (idp2p)
+        for (int i = 0; i < n; i++) idp2p[i] = -1;
+    for (int c = 0; c < nelem; c++) {
+        uint32_t a = cs[c], b = ce[c];
+        if (numep) numep[c] = (int32_t)(b - a);
+        if (idepf) idepf[c] = (b > a) ? to_pid(perm[a]) : -1;
+        if (idp2p)
+            for (uint32_t s = a; s + 1 < b; s++) idp2p[to_pid(perm[s])] = to_pid(perm[s + 1]);
+    }
+    if (cell_of)
+        for (int s = 0; s < n; s++) cell_of[to_pid((uint32_t)s)] = keys[s] >= (uint32_t)nelem ? -1 : (int32_t)keys[s];
+    return 0;
+}
+
+int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    if (nsteps < 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "nsteps < 0");
+    if ((rc = upload_bc(e))) return rc;
+    clear_err(e);
+    const int nev = 7;
+    if (e->timing) {
+        size_t need = (size_t)nev * nsteps;
+        while (e->ev.size() < need) {
+            hipEvent_t ev;
+            MPM_HIP_CHECK(hipEventCreate(&ev));
+            e->ev.push_back(ev);
+        }
+    }
+    for (int s = 0; s < nsteps; s++) {
+        const int istep = istep0 + s;
+        const double facl = facl_of(e, istep);
+        const int diag = (e->full_node_output && s == nsteps - 1) ? 1 : 0;
+        hipEvent_t *ev = e->timing ? &e->ev[(size_t)nev * s] : nullptr;
+        // events: 0 | bin | 1 | p2g cells | 2 | p2g nodes | 3 | musl cells | 4 | musl nodes | 5 | update | 6
+        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[0], e->stream));
+        if ((rc = do_bin(e))) return rc;
+        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[1], e->stream));
+        if ((rc = do_p2g(e, facl, diag, ev ? ev[2] : nullptr))) return rc;
+        if ((rc = do_solve(e))) return rc;
+        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[3], e->stream));
+        if ((rc = do_g2p(e, facl, diag, ev ? ev[5] : nullptr, ev ? ev[4] : nullptr))) return rc;
+        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[6], e->stream));
+    }
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    if (e->timing) {
+        mpm_kernel_times kt{};
+        const bool f32 = e->prec == MPM_PREC_FP32;
+        for (int s = 0; s < nsteps; s++) {
+            hipEvent_t *ev = &e->ev[(size_t)nev * s];
+            float t[6] = {0, 0, 0, 0, 0, 0};
+            if (f32) {
+                for (int k = 0; k < 6; k++) MPM_HIP_CHECK(hipEventElapsedTime(&t[k], ev[k], ev[k + 1]));
+            } else {
+                MPM_HIP_CHECK(hipEventElapsedTime(&t[0], ev[0], ev[1]));
+                MPM_HIP_CHECK(hipEventElapsedTime(&t[1], ev[1], ev[3]));
+                MPM_HIP_CHECK(hipEventElapsedTime(&t[5], ev[3], ev[6]));
+            }
+            kt.bin_ms += t[0];
+            kt.p2g_cells_ms += t[1];
+            kt.p2g_nodes_ms += t[2];
+            kt.musl_cells_ms += t[3];
+            kt.musl_nodes_ms += t[4];
+            kt.g2p_upd_ms += t[5];
+        }
+        kt.p2g_ms = kt.p2g_cells_ms + kt.p2g_nodes_ms;
+        kt.g2p_vel_ms = kt.musl_cells_ms + kt.musl_nodes_ms;
+        kt.total_ms = kt.bin_ms + kt.p2g_ms + kt.g2p_vel_ms + kt.g2p_upd_ms;
+        kt.steps = nsteps;
+        e->kt = kt;
+    }
+    return check_err(e);
+}
+
+int mpm_sync(mpm_engine *e) {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int mpm_get_kernel_times(mpm_engine *e, mpm_kernel_times *out) {
+    if (!e || !out) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    *out = e->kt;
+    if (e->bins_valid || e->n > 0) {
+        // count of non-empty cells of the last binning
+        std::vector<uint32_t> ce;
+        int rc = d2h(e, ce, e->cend, (size_t)e->g.nelem);
+        if (rc) return rc;
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        const GridGeom &g = e->g;
+        std::vector<uint8_t> act((size_t)g.nodec, 0);
+        int cnt = 0;
+        for (int c = 0; c < g.nelem; c++) {
+            if (!ce[c]) continue;
+            cnt++;
+            int iz = c / (g.nex[0] * g.nex[1]);
+            int iy = (c / g.nex[0]) % g.nex[1];
+            int ix = c % g.nex[0];
+            for (int k = 0; k <= g.idimc[2]; k++)
+                for (int j = 0; j <= g.idimc[1]; j++)
+                    for (int i = 0; i <= g.idimc[0]; i++)
+                        act[(size_t)(ix + i) + (size_t)g.nxc[0] * (iy + j) + (size_t)g.nxc[0] * g.nxc[1] * (iz + k)] = 1;
+        }
+        int na = 0;
+        for (uint8_t a : act) na += a;
+        out->active_cells = cnt;
+        out->active_nodes = na;
+    }
+    return 0;
+}
+
+int mpm_set_timing(mpm_engine *e, int32_t enabled) {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    e->timing = enabled != 0;
+    return 0;
+}
+
+int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    switch (key) {
+    case OPT_FULL_NODE_OUTPUT: e->full_node_output = value ? 1 : 0; return 0;
+    default: return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown option");
+    }
+}
+
+int mpm_maksf_batch(const mpm_grid_desc *grid, int64_t n, const int32_t *cell, const double *xyg, int32_t *nc,
+                    double *sf, double *dsf) {
+    if (!grid || n < 0 || (n > 0 && (!cell || !xyg || !nc || !sf || !dsf)))
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_grid(grid, true);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    if (n >= (1LL << 31) / 27) return set_error(MPM_ERR_INVALID_ARGUMENT, "batch too large");
+    GridGeom g = make_geom(grid);
+    for (int64_t i = 0; i < n; i++)
+        if (cell[i] < 0 || cell[i] >= g.nelem) return set_error(MPM_ERR_INVALID_ARGUMENT, "cell out of range");
+    const int nen = (g.idimc[0] + 1) * (g.idimc[1] + 1) * (g.idimc[2] + 1);
+    int32_t *dc = nullptr, *dnc = nullptr;
+    double *dx = nullptr, *dsfd = nullptr, *ddsf = nullptr;
+    hipStream_t s = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(dc);
+        (void)hipFree(dnc);
+        (void)hipFree(dx);
+        (void)hipFree(dsfd);
+        (void)hipFree(ddsf);
+        if (s) (void)hipStreamDestroy(s);
+    };
+    hipError_t he = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipMalloc((void **)&dc, n * sizeof(int32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&dnc, n * nen * sizeof(int32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&dx, 3 * n * sizeof(double));
+    if (he == hipSuccess) he = hipMalloc((void **)&dsfd, n * nen * sizeof(double));
+    if (he == hipSuccess) he = hipMalloc((void **)&ddsf, 3 * n * nen * sizeof(double));
+    if (he == hipSuccess) he = hipMemcpyAsync(dc, cell, n * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(dx, xyg, 3 * n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (he == hipSuccess) he = (hipError_t)launch_f64_maksf(s, g, (int)n, dc, dx, dnc, dsfd, ddsf);
+    if (he == hipSuccess) he = hipMemcpyAsync(nc, dnc, n * nen * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(sf, dsfd, n * nen * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(dsf, ddsf, 3 * n * nen * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) rc = hip_fail(he, "mpm_maksf_batch", __FILE__, __LINE__);
+    cleanup();
+    return rc;
+}
+
+int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, int32_t *cell_of, int32_t *order) {
+    if (!grid || n < 0 || (n > 0 && (!xyg || !cell_of || !order)))
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_grid(grid, false);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    if (n >= (1LL << 31)) return set_error(MPM_ERR_INVALID_ARGUMENT, "too many positions");
+    GridGeom g = make_geom(grid);
+    const size_t ld = (size_t)n;
+    std::vector<double> x(3 * ld);
+    for (size_t i = 0; i < ld; i++)
+        for (int a = 0; a < 3; a++) x[a * ld + i] = xyg[3 * i + a];
+    double *dx = nullptr;
+    uint32_t *keys = nullptr, *err = nullptr;
+    SortBuffers sb{};
+    hipStream_t s = nullptr;
+    hipError_t he = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipMalloc((void **)&dx, 3 * ld * sizeof(double));
+    if (he == hipSuccess) he = hipMalloc((void **)&keys, ld * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&err, sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.keys, ld * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.vals, ld * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.keys_alt, ld * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.vals_alt, ld * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.hist, radix_hist_entries((int)n) * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.bsum, 4096 * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemcpyAsync(dx, x.data(), 3 * ld * sizeof(double), hipMemcpyHostToDevice, s);
+    uint32_t *sk = nullptr, *sv = nullptr;
+    std::vector<uint32_t> kh(ld), vh(ld);
+    if (he == hipSuccess) {
+        launch_fill_u32(s, err, 1, 0u);
+        launch_keys_f64(s, g, dx, ld, (int)n, keys, err);
+        if (radix_sort_pairs(s, sb, keys, (int)n, bit_length((uint32_t)g.nelem), &sk, &sv)) he = hipErrorUnknown;
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(kh.data(), keys, ld * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(vh.data(), sv, ld * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) rc = hip_fail(he, "mpm_bin_positions", __FILE__, __LINE__);
+    (void)hipFree(dx);
+    (void)hipFree(keys);
+    (void)hipFree(err);
+    (void)hipFree(sb.keys);
+    (void)hipFree(sb.vals);
+    (void)hipFree(sb.keys_alt);
+    (void)hipFree(sb.vals_alt);
+    (void)hipFree(sb.hist);
+    (void)hipFree(sb.bsum);
+    if (s) (void)hipStreamDestroy(s);
+    if (rc) return rc;
+    for (size_t i = 0; i < ld; i++) {
+        cell_of[i] = kh[i] >= (uint32_t)g.nelem ? -1 : (int32_t)kh[i];
+        order[i] = (int32_t)vh[i];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,553 @@
+// FP64 kernels that follow the reference's arithmetic operation by operation.
+//
+// Compiled with -ffp-contract=off: no fused multiply-adds, so every product
+// and sum is rounded as in the reference's x86-64 build. Node sums are
+// gathered per node in the reference's order (cells ascending, particles in
+// linked-list order), which makes the P2G and MUSL sums identical to the
+// sequential scatter loops of var_trans_solid_explicit.cpp:35-70 and :109-127.
+// Particle-side passes are one thread per particle and visit the stencil in
+// MakSf's k, j, i order. IEEE division and sqrt are correctly rounded on the
+// device as on the host; only libm functions (log in Neo-Hookean, cbrt/log in
+// Mooney-Rivlin) may differ in the last bit.
+#include "mpm_common.h"
+#include "mpm_kernels.h"
+
+namespace mpm {
+
+namespace {
+
+// One axis of MakSf's linear branch (shape_function.cpp:40-51) for control
+// point index ixyn.
+__device__ __forceinline__ void sf_linear(double xyg, double xmin, double dxi, int ixyn, double &s, double &d) {
+    double xy = xmin + dxi * (double)ixyn;
+    double dxnpi = xyg - xy;
+    double xi = fabs(dxnpi) / dxi;
+    s = 1.0 - xi;
+    d = (dxnpi < 0.0) ? (1.0 / dxi) : (-1.0 / dxi);
+}
+
+// Quadratic B-spline branch (shape_function.cpp:53-104), single rank.
+__device__ __forceinline__ void sf_bspline(double xyg, double xmin, double dxi, int ixyn, int nw, double &s,
+                                           double &d) {
+    int ncxy = ixyn;
+    double xy = xmin + dxi * (double)(ixyn - 2);
+    double dxnpi = xyg - xy;
+    double dxh = dxnpi / dxi;
+    double s1, s2, s3, d1, d2;
+    if (ncxy == 0) {
+        dxh -= 2.0;
+        s1 = 1.0; s2 = -2.0; s3 = 1.0; d1 = 2.0; d2 = -2.0;
+    } else if (ncxy == 1) {
+        dxh -= 1.0;
+        if (dxh <= 1.0) {
+            s1 = -1.5; s2 = 2.0; s3 = 0.0; d1 = -3.0; d2 = 2.0;
+        } else {
+            s1 = 0.5; s2 = -2.0; s3 = 2.0; d1 = 1.0; d2 = -2.0;
+        }
+    } else if (ncxy == nw - 1) {
+        s1 = 1.0; s2 = 0.0; s3 = 0.0; d1 = 2.0; d2 = 0.0;
+    } else if (ncxy == nw - 2) {
+        if (dxh <= 1.0) {
+            s1 = 0.5; s2 = 0.0; s3 = 0.0; d1 = 1.0; d2 = 0.0;
+        } else {
+            s1 = -1.5; s2 = 4.0; s3 = -2.0; d1 = -3.0; d2 = 4.0;
+        }
+    } else {
+        if (dxh <= 1.0) {
+            s1 = 0.5; s2 = 0.0; s3 = 0.0; d1 = 1.0; d2 = 0.0;
+        } else if (dxh <= 2.0) {
+            s1 = -1.0; s2 = 3.0; s3 = -1.5; d1 = -2.0; d2 = 3.0;
+        } else {
+            s1 = 0.5; s2 = -3.0; s3 = 4.5; d1 = 1.0; d2 = -3.0;
+        }
+    }
+    s = (s1 * dxh + s2) * dxh + s3;
+    d = (d1 * dxh + d2) / dxi;
+}
+
+__device__ __forceinline__ void sf_axis(const GridGeom &g, int ii, double xyg, int ixyn, double &s, double &d) {
+    if (g.idimc[ii] == 1)
+        sf_linear(xyg, g.xmin[ii], g.dx[ii], ixyn, s, d);
+    else
+        sf_bspline(xyg, g.xmin[ii], g.dx[ii], ixyn, g.nxc[ii], s, d);
+}
+
+// Stencil entry of MakSf for control point (ix, iy, iz) and position xyg
+// (tensor product of shape_function.cpp:107-110).
+__device__ __forceinline__ void sf_node(const GridGeom &g, const double xyg[3], int ix, int iy, int iz, double &sfi,
+                                        double ds[3]) {
+    double s0, s1, s2, d0, d1, d2;
+    sf_axis(g, 0, xyg[0], ix, s0, d0);
+    sf_axis(g, 1, xyg[1], iy, s1, d1);
+    sf_axis(g, 2, xyg[2], iz, s2, d2);
+    sfi = s0 * s1 * s2;
+    ds[0] = d0 * s1 * s2;
+    ds[1] = s0 * d1 * s2;
+    ds[2] = s0 * s1 * d2;
+}
+
+__device__ __forceinline__ void cell_ijk(const GridGeom &g, int m, int &ixe, int &iye, int &ize) {
+    int nex = g.nex[0], ney = g.nex[1];
+    ize = m / (nex * ney);
+    iye = (m - ize * (nex * ney)) / nex;
+    ixe = m - ize * (nex * ney) - iye * nex;
+}
+
+__device__ __forceinline__ void apply_vel_bc(const BcDense &bc, int n, int nodec, double facl, double &v0, double &v1,
+                                             double &v2) {
+    uint8_t mk = bc.mask ? bc.mask[n] : 0;
+    if (mk) {
+        if (mk & 1) v0 = bc.val[n] * facl;
+        if (mk & 2) v1 = bc.val[nodec + n] * facl;
+        if (mk & 4) v2 = bc.val[2 * nodec + n] * facl;
+    }
+}
+
+// Cells adjacent to control point (i, j, k), ascending cell index; returns count.
+__device__ __forceinline__ int node_cells(const GridGeom &g, int i, int j, int k, int *cells) {
+    int cnt = 0;
+    const int r0 = g.idimc[0], r1 = g.idimc[1], r2 = g.idimc[2];
+    for (int ck = k - r2; ck <= k; ck++) {
+        if (ck < 0 || ck >= g.nex[2]) continue;
+        for (int cj = j - r1; cj <= j; cj++) {
+            if (cj < 0 || cj >= g.nex[1]) continue;
+            for (int ci = i - r0; ci <= i; ci++) {
+                if (ci < 0 || ci >= g.nex[0]) continue;
+                cells[cnt++] = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
+            }
+        }
+    }
+    return cnt;
+}
+
+// Particle2NodeS (var_trans_solid_explicit.cpp:20-96) for one node.
+__global__ void k64_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, double fx, double fy, double fz,
+                        double mtol, double facl) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    const int nodec = g.nodec;
+    const size_t ld = P.ld;
+    int i = n % g.nxc[0];
+    int j = (n / g.nxc[0]) % g.nxc[1];
+    int k = n / (g.nxc[0] * g.nxc[1]);
+    int cells[27];
+    int nc = node_cells(g, i, j, k, cells);
+    double nm = 0.0, nvof = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = 0; q < nc; q++) {
+        int c = cells[q];
+        uint32_t s0 = L.cstart[c], s1 = L.cend[c];
+        for (uint32_t s = s0; s < s1; s++) {
+            uint32_t pid = L.perm[s];
+            double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+            double sfi, ds[3];
+            sf_node(g, xyg, i, j, k, sfi, ds);
+            double mp = P.m[pid], vp = P.V[pid];
+            double s_0 = P.sig[pid], s_1 = P.sig[ld + pid], s_2 = P.sig[2 * ld + pid];
+            double s_3 = P.sig[3 * ld + pid], s_4 = P.sig[4 * ld + pid], s_5 = P.sig[5 * ld + pid];
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+            if (P.trac) {
+                t0 = P.trac[pid];
+                t1 = P.trac[ld + pid];
+                t2 = P.trac[2 * ld + pid];
+            }
+            nm += sfi * mp;
+            nvof += sfi * vp;
+            p0 += sfi * mp * P.v[pid];
+            p1 += sfi * mp * P.v[ld + pid];
+            p2 += sfi * mp * P.v[2 * ld + pid];
+            double fi0 = -vp * (s_0 * ds[0] + s_5 * ds[1] + s_4 * ds[2]);
+            double fi1 = -vp * (s_5 * ds[0] + s_1 * ds[1] + s_3 * ds[2]);
+            double fi2 = -vp * (s_4 * ds[0] + s_3 * ds[1] + s_2 * ds[2]);
+            double fe0 = sfi * (mp * fx + t0);
+            double fe1 = sfi * (mp * fy + t1);
+            double fe2 = sfi * (mp * fz + t2);
+            f0 += fi0 + fe0;
+            f1 += fi1 + fe1;
+            f2 += fi2 + fe2;
+        }
+    }
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (nm > mtol) {
+        v0 = p0 / nm;
+        v1 = p1 / nm;
+        v2 = p2 / nm;
+    }
+    if (nvof < mtol) nvof = 0.0;
+    apply_vel_bc(bc, n, nodec, facl, v0, v1, v2);
+    N.nm[n] = nm;
+    N.nvof[n] = nvof;
+    N.np[n] = p0;
+    N.np[nodec + n] = p1;
+    N.np[2 * nodec + n] = p2;
+    N.nf[n] = f0;
+    N.nf[nodec + n] = f1;
+    N.nf[2 * nodec + n] = f2;
+    N.nv[n] = v0;
+    N.nv[nodec + n] = v1;
+    N.nv[2 * nodec + n] = v2;
+}
+
+// SolveSolid (solve_solid_explicit.cpp:8-31)
+__global__ void k64_solve_grid(GridGeom g, NodesF64 N, BcDense bc, double mtol) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    const int nodec = g.nodec;
+    double nm = N.nm[n];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (nm > mtol) {
+        a0 = N.nf[n] / nm;
+        a1 = N.nf[nodec + n] / nm;
+        a2 = N.nf[2 * nodec + n] / nm;
+    }
+    uint8_t mk = bc.mask ? bc.mask[n] : 0;
+    if (mk & 1) a0 = 0.0;
+    if (mk & 2) a1 = 0.0;
+    if (mk & 4) a2 = 0.0;
+    N.na[n] = a0;
+    N.na[nodec + n] = a1;
+    N.na[2 * nodec + n] = a2;
+}
+
+// var_trans_solid_explicit.cpp:194-200
+__global__ void k64_vel_plus_dt_acc(int n3, double *nv, const double *na, double dt) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < n3) nv[n] += dt * na[n];
+}
+
+// var_trans_solid_explicit.cpp:202-240 (PIC / FLIP velocity G2P)
+__global__ void k64_g2p_vel(GridGeom g, PartF64 P, NodesF64 N, int scheme, double dt) {
+    int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P.n) return;
+    uint32_t m = P.key[pid];
+    if (m >= (uint32_t)g.nelem) return;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    int ixe, iye, ize;
+    cell_ijk(g, (int)m, ixe, iye, ize);
+    double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+    double v0, v1, v2;
+    if (scheme != 1) {
+        v0 = 0.0;
+        v1 = 0.0;
+        v2 = 0.0;
+    } else {
+        v0 = P.v[pid];
+        v1 = P.v[ld + pid];
+        v2 = P.v[2 * ld + pid];
+    }
+    for (int k = 0; k <= g.idimc[2]; k++)
+        for (int j = 0; j <= g.idimc[1]; j++)
+            for (int i = 0; i <= g.idimc[0]; i++) {
+                int ix = ixe + i, iy = iye + j, iz = ize + k;
+                int nid = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+                double sfi, ds[3];
+                sf_node(g, xyg, ix, iy, iz, sfi, ds);
+                if (scheme != 1) {
+                    v0 += sfi * N.nv[nid];
+                    v1 += sfi * N.nv[nodec + nid];
+                    v2 += sfi * N.nv[2 * nodec + nid];
+                } else {
+                    v0 += sfi * dt * N.na[nid];
+                    v1 += sfi * dt * N.na[nodec + nid];
+                    v2 += sfi * dt * N.na[2 * nodec + nid];
+                }
+            }
+    P.v[pid] = v0;
+    P.v[ld + pid] = v1;
+    P.v[2 * ld + pid] = v2;
+}
+
+// MUSL (var_trans_solid_explicit.cpp:101-148) for one node
+__global__ void k64_musl(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, double mtol, double facl) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    const int nodec = g.nodec;
+    const size_t ld = P.ld;
+    int i = n % g.nxc[0];
+    int j = (n / g.nxc[0]) % g.nxc[1];
+    int k = n / (g.nxc[0] * g.nxc[1]);
+    int cells[27];
+    int nc = node_cells(g, i, j, k, cells);
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    for (int q = 0; q < nc; q++) {
+        int c = cells[q];
+        uint32_t s0 = L.cstart[c], s1 = L.cend[c];
+        for (uint32_t s = s0; s < s1; s++) {
+            uint32_t pid = L.perm[s];
+            double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+            double sfi, ds[3];
+            sf_node(g, xyg, i, j, k, sfi, ds);
+            double mp = P.m[pid];
+            p0 += sfi * mp * P.v[pid];
+            p1 += sfi * mp * P.v[ld + pid];
+            p2 += sfi * mp * P.v[2 * ld + pid];
+        }
+    }
+    double nm = N.nm[n];
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (nm > mtol) {
+        v0 = p0 / nm;
+        v1 = p1 / nm;
+        v2 = p2 / nm;
+    }
+    apply_vel_bc(bc, n, nodec, facl, v0, v1, v2);
+    N.np[n] = p0;
+    N.np[nodec + n] = p1;
+    N.np[2 * nodec + n] = p2;
+    N.nv[n] = v0;
+    N.nv[nodec + n] = v1;
+    N.nv[2 * nodec + n] = v2;
+}
+
+__device__ __forceinline__ double det3(const double a[9]) {
+    return a[0] * (a[4] * a[8] - a[5] * a[7]) + a[1] * (a[5] * a[6] - a[3] * a[8]) +
+           a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+__device__ __forceinline__ void store_voigt(double *sig, size_t ld, int pid, const double s[9]) {
+    sig[pid] = s[0];
+    sig[ld + pid] = s[4];
+    sig[2 * ld + pid] = s[8];
+    sig[3 * ld + pid] = s[5];
+    sig[4 * ld + pid] = s[2];
+    sig[5 * ld + pid] = s[1];
+}
+
+__device__ __forceinline__ void bmat(const double F[9], double be[9]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            be[3 * i + j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
+}
+
+// ConstitutiveModel::UpdateStress (constitutive_model.cpp:287-305), FP64,
+// same expression order as the reference. J2 has no reference counterpart.
+__device__ void update_stress64(const MaterialDev &md, double *sig, size_t ld, int pid, const double F[9],
+                                const double dF[9], double jac, double *eps_p) {
+    const double *p = md.prop;
+    switch (md.model) {
+    case -1:
+        for (int q = 0; q < 6; q++) sig[q * ld + pid] = 0.0;
+        break;
+    case 0: {  // MatLinElast :16-60
+        double EE = p[0], nu = p[1];
+        double C0 = (EE * (1.0 - nu)) / ((1.0 - 2.0 * nu) * (1.0 + nu));
+        double C1 = (EE * nu) / ((1.0 - 2.0 * nu) * (1.0 + nu));
+        double C2 = EE / (2.0 * (1.0 + nu));
+        double e0 = dF[0] - 1.0, e1 = dF[4] - 1.0, e2 = dF[8] - 1.0;
+        double e3 = dF[5] + dF[7], e4 = dF[6] + dF[2], e5 = dF[1] + dF[3];
+        double d0 = C0 * e0 + C1 * e1 + C1 * e2;
+        double d1 = C1 * e0 + C0 * e1 + C1 * e2;
+        double d2 = C1 * e0 + C1 * e1 + C0 * e2;
+        sig[pid] += d0;
+        sig[ld + pid] += d1;
+        sig[2 * ld + pid] += d2;
+        sig[3 * ld + pid] += C2 * e3;
+        sig[4 * ld + pid] += C2 * e4;
+        sig[5 * ld + pid] += C2 * e5;
+        break;
+    }
+    case 1: {  // MatNeoHookean :62-120
+        double yng = p[0], nu = p[1];
+        double lmd = nu * yng / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        double mu = yng / (2.0 * (1.0 + nu));
+        double lnjac = log(jac);
+        double be[9], s[9];
+        bmat(F, be);
+        for (int q = 0; q < 9; q++) {
+            double eye = (q == 0 || q == 4 || q == 8) ? 1.0 : 0.0;
+            s[q] = (mu * (be[q] - eye) + lmd * lnjac * eye) / jac;
+        }
+        store_voigt(sig, ld, pid, s);
+        break;
+    }
+    case 2: {  // MatStVk :122-185
+        double yng = p[0], nu = p[1];
+        double lmd = nu * yng / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        double mu = yng / (2.0 * (1.0 + nu));
+        double be[9], be2[9], s[9];
+        bmat(F, be);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                be2[3 * i + j] = be[3 * i] * be[j] + be[3 * i + 1] * be[3 + j] + be[3 * i + 2] * be[6 + j];
+        double tr_be = 0.0;
+        for (int i = 0; i < 3; i++) tr_be += be[4 * i];
+        double tr_gls = 0.5 * (tr_be - 3.0);
+        for (int q = 0; q < 9; q++) s[q] = (mu * be2[q] + (lmd * tr_gls - mu) * be[q]) / jac;
+        store_voigt(sig, ld, pid, s);
+        break;
+    }
+    case 3: {  // MatMooneyRivlin :187-285 (pow(I1,2) written as I1*I1)
+        double c1 = p[0], c2 = p[1], bulk = p[2];
+        double Fi[9], be[9], be2[9], s[9];
+        double cj = cbrt(jac);
+        for (int q = 0; q < 9; q++) Fi[q] = 1.0 / cj * F[q];
+        bmat(Fi, be);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                be2[3 * i + j] = be[3 * i] * be[j] + be[3 * i + 1] * be[3 + j] + be[3 * i + 2] * be[6 + j];
+        double I1 = 0.0, trb2 = 0.0;
+        for (int i = 0; i < 3; i++) I1 += be[4 * i];
+        for (int i = 0; i < 3; i++) trb2 += be2[4 * i];
+        double I2 = 0.5 * (I1 * I1 - trb2);
+        double hp = bulk * log(jac) / jac;
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) s[3 * i + j] = 2.0 * ((c1 + c2 * I1) * be[3 * i + j] - c2 * be2[3 * i + j]) / jac;
+            s[4 * i] += hp - 2.0 / (3.0 * jac) * (c1 * I1 + 2.0 * c2 * I2);
+        }
+        store_voigt(sig, ld, pid, s);
+        break;
+    }
+    case 10: {  // J2 radial return, small-increment form (Simo & Hughes 1998, Box 3.2)
+        double E = p[0], nu = p[1], sy = p[2], Hh = p[3];
+        double lmd = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        double mu = E / (2.0 * (1.0 + nu));
+        double de[6] = {dF[0] - 1.0, dF[4] - 1.0, dF[8] - 1.0, dF[5] + dF[7], dF[6] + dF[2], dF[1] + dF[3]};
+        double tr = de[0] + de[1] + de[2];
+        double st[6];
+        st[0] = sig[pid] + lmd * tr + 2.0 * mu * de[0];
+        st[1] = sig[ld + pid] + lmd * tr + 2.0 * mu * de[1];
+        st[2] = sig[2 * ld + pid] + lmd * tr + 2.0 * mu * de[2];
+        st[3] = sig[3 * ld + pid] + mu * de[3];
+        st[4] = sig[4 * ld + pid] + mu * de[4];
+        st[5] = sig[5 * ld + pid] + mu * de[5];
+        double pm = (st[0] + st[1] + st[2]) / 3.0;
+        double d0 = st[0] - pm, d1 = st[1] - pm, d2 = st[2] - pm;
+        double j2 = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2) + st[3] * st[3] + st[4] * st[4] + st[5] * st[5];
+        double q = sqrt(3.0 * j2);
+        double ep = eps_p ? eps_p[pid] : 0.0;
+        double f = q - (sy + Hh * ep);
+        if (f <= 0.0) {
+            for (int c = 0; c < 6; c++) sig[c * ld + pid] = st[c];
+        } else {
+            double dg = f / (3.0 * mu + Hh);
+            double scale = 1.0 - 3.0 * mu * dg / q;
+            sig[pid] = pm + d0 * scale;
+            sig[ld + pid] = pm + d1 * scale;
+            sig[2 * ld + pid] = pm + d2 * scale;
+            sig[3 * ld + pid] = st[3] * scale;
+            sig[4 * ld + pid] = st[4] * scale;
+            sig[5 * ld + pid] = st[5] * scale;
+            if (eps_p) eps_p[pid] = ep + dg;
+        }
+        break;
+    }
+    default: break;
+    }
+}
+
+// Deformation gradient, coordinates, volume and stress
+// (var_trans_solid_explicit.cpp:244-296 and :150-185), one thread per particle.
+__global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err) {
+    int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P.n) return;
+    uint32_t m = P.key[pid];
+    if (m >= (uint32_t)g.nelem) return;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    int ixe, iye, ize;
+    cell_ijk(g, (int)m, ixe, iye, ize);
+    double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+    double dd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double dx0 = xyg[0], dx1 = xyg[1], dx2 = xyg[2];
+    for (int k = 0; k <= g.idimc[2]; k++)
+        for (int j = 0; j <= g.idimc[1]; j++)
+            for (int i = 0; i <= g.idimc[0]; i++) {
+                int ix = ixe + i, iy = iye + j, iz = ize + k;
+                int nid = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+                double sfi, ds[3];
+                sf_node(g, xyg, ix, iy, iz, sfi, ds);
+                double vu = N.nv[nid], vv = N.nv[nodec + nid], vw = N.nv[2 * nodec + nid];
+                dd[0] += ds[0] * vu * dt;
+                dd[1] += ds[1] * vu * dt;
+                dd[2] += ds[2] * vu * dt;
+                dd[3] += ds[0] * vv * dt;
+                dd[4] += ds[1] * vv * dt;
+                dd[5] += ds[2] * vv * dt;
+                dd[6] += ds[0] * vw * dt;
+                dd[7] += ds[1] * vw * dt;
+                dd[8] += ds[2] * vw * dt;
+                dx0 += sfi * dt * vu;
+                dx1 += sfi * dt * vv;
+                dx2 += sfi * dt * vw;
+            }
+    dd[0] += 1.0;
+    dd[4] += 1.0;
+    dd[8] += 1.0;
+    double Fo[9], F[9];
+    for (int q = 0; q < 9; q++) Fo[q] = P.F[q * ld + pid];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int kk = 0; kk < 3; kk++) acc += dd[3 * i + kk] * Fo[3 * kk + j];
+            F[3 * i + j] = acc;
+        }
+    for (int q = 0; q < 9; q++) {
+        P.F[q * ld + pid] = F[q];
+        P.ddg[q * ld + pid] = dd[q];
+    }
+    double J = det3(F);
+    P.J[pid] = J;
+    P.x[pid] = dx0;
+    P.x[ld + pid] = dx1;
+    P.x[2 * ld + pid] = dx2;
+    P.V[pid] = J * P.V0[pid];
+    int mat = P.matid ? P.matid[pid] : 0;
+    update_stress64(mt.m[mat], P.sig, ld, pid, F, dd, J, P.eps_p);
+    if (!(J > 0.0)) atomicOr(err, kErrJacobian);
+    if (!isfinite(dx0) || !isfinite(dx1) || !isfinite(dx2)) atomicOr(err, kErrNonFinite);
+}
+
+__global__ void k64_maksf(GridGeom g, int n, const int32_t *cell, const double *xyg, int32_t *nc, double *sf,
+                          double *dsf) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int nen = (g.idimc[0] + 1) * (g.idimc[1] + 1) * (g.idimc[2] + 1);
+    int ixe, iye, ize;
+    cell_ijk(g, cell[p], ixe, iye, ize);
+    double x[3] = {xyg[3 * p], xyg[3 * p + 1], xyg[3 * p + 2]};
+    int id = 0;
+    for (int k = 0; k <= g.idimc[2]; k++)
+        for (int j = 0; j <= g.idimc[1]; j++)
+            for (int i = 0; i <= g.idimc[0]; i++) {
+                int ix = ixe + i, iy = iye + j, iz = ize + k;
+                size_t o = (size_t)p * nen + id;
+                nc[o] = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+                double s, ds[3];
+                sf_node(g, x, ix, iy, iz, s, ds);
+                sf[o] = s;
+                dsf[3 * o] = ds[0];
+                dsf[3 * o + 1] = ds[1];
+                dsf[3 * o + 2] = ds[2];
+                id++;
+            }
+}
+
+}  // namespace
+
+void launch_f64_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
+                    const BcDense &bc, double fx, double fy, double fz, double mtol, double facl) {
+    hipLaunchKernelGGL(k64_p2g, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, fx, fy, fz, mtol, facl);
+}
+
+void launch_f64_solve_grid(hipStream_t s, const GridGeom &g, const NodesF64 &N, const BcDense &bc, double mtol) {
+    hipLaunchKernelGGL(k64_solve_grid, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, N, bc, mtol);
+}
+
+void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
+                    const BcDense &bc, const MaterialTable &mt, int scheme, double dt, double mtol, double facl,
+                    uint32_t *err) {
+    if (scheme != 1)
+        hipLaunchKernelGGL(k64_vel_plus_dt_acc, dim3(cdiv(3 * g.nodec, 256)), dim3(256), 0, s, 3 * g.nodec, N.nv,
+                           (const double *)N.na, dt);
+    if (P.n > 0) hipLaunchKernelGGL(k64_g2p_vel, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, scheme, dt);
+    hipLaunchKernelGGL(k64_musl, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, mtol, facl);
+    if (P.n > 0) hipLaunchKernelGGL(k64_g2p_update, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, mt, dt, err);
+}
+
+int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cell, const double *xyg, int32_t *nc,
+                     double *sf, double *dsf) {
+    if (n > 0) hipLaunchKernelGGL(k64_maksf, dim3(cdiv(n, 128)), dim3(128), 0, s, g, n, cell, xyg, nc, sf, dsf);
+    return (int)hipGetLastError();
+}
+
+}  // namespace mpm

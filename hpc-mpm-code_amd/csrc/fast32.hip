@@ -1,0 +1,812 @@
+// FP32 explicit-MPM step kernels for gfx950 (wave64, 160 KB LDS per CU).
+//
+// Data layout (HBM, structure of arrays, one slot per particle): position as
+// an FP32 pair (xh + xl, compensated), velocity, mass, reference volume, the
+// displacement gradient H = F - I (so small strains do not cancel against
+// the identity), Cauchy stress, and FP64 det F and volume. Nodes are FP32
+// [u|v|w] blocks like the reference's node vectors.
+//
+// P2G is deterministic without float atomics: a 64-thread workgroup owns a
+// row segment of 64 consecutive cells along x. Their particles are one
+// contiguous range of the cell-sorted order, which is staged through LDS in
+// chunks with coalesced loads; each thread then folds the particles of its
+// own cell, in sorted order, into the 8 corner partials of that cell
+// (registers). A node pass adds the partials of the <= 8 cells around each
+// node in ascending cell order with Neumaier-compensated FP32 sums and
+// applies the grid update (velocity, acceleration, BCs) in the same pass.
+// The MUSL remap uses the same structure, with the G2P velocity of every
+// particle computed from the 8 node values of its cell held in registers.
+#include "mpm_common.h"
+#include "mpm_kernels.h"
+
+namespace mpm {
+
+namespace {
+
+constexpr int kSeg = 64;      // cells per row segment = threads per workgroup
+constexpr int kChunk = 256;   // particles staged per LDS pass
+constexpr int kChunkPad = kChunk + kChunk / 8;
+
+// LDS slot of staged particle i: one pad word every 8 particles spreads the
+// per-cell start offsets of neighbouring threads over the banks.
+__device__ __forceinline__ int lds_idx(int i) { return i + (i >> 3); }
+
+__device__ __forceinline__ float local_coord(float xh, float xl, const float *tnh, const float *tnl, int idx,
+                                             float inv) {
+    return ((xh - tnh[idx]) + (xl - tnl[idx])) * inv;
+}
+
+// Neumaier (improved Kahan) compensated sum.
+__device__ __forceinline__ void neu_add(float &s, float &c, float x) {
+    float t = s + x;
+    if (fabsf(s) >= fabsf(x))
+        c += (s - t) + x;
+    else
+        c += (x - t) + s;
+    s = t;
+}
+
+struct SegInfo {
+    int iy, iz, rowbase, ix, c;
+    bool vcell;
+    uint32_t cs, ce;
+};
+
+__device__ __forceinline__ bool seg_setup(const GridGeom &g, const CellLists &L, int nseg_x, SegInfo &si,
+                                          uint32_t &S, uint32_t &E) {
+    const int t = threadIdx.x;
+    const int seg = blockIdx.x;
+    const int sx = seg % nseg_x;
+    const int row = seg / nseg_x;
+    si.iy = row % g.nex[1];
+    si.iz = row / g.nex[1];
+    si.rowbase = si.iy * g.nex[0] + si.iz * g.nex[0] * g.nex[1];
+    si.ix = sx * kSeg + t;
+    si.vcell = si.ix < g.nex[0];
+    si.c = si.rowbase + si.ix;
+    si.cs = 0;
+    si.ce = 0;
+    if (si.vcell) {
+        si.cs = L.cstart[si.c];
+        si.ce = L.cend[si.c];
+    }
+    bool ne = si.ce > si.cs;
+    uint32_t s = ne ? si.cs : 0xFFFFFFFFu, e = ne ? si.ce : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s = min(s, (uint32_t)__shfl_xor((int)s, o, 64));
+        e = max(e, (uint32_t)__shfl_xor((int)e, o, 64));
+    }
+    S = s;
+    E = e;
+    return e != 0u;
+}
+
+// ------------------------------------------------------------------ P2G
+template <bool DIAG, bool TRAC>
+__global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T,
+                                                       float *__restrict__ part, int nseg_x) {
+    constexpr int NF = 13 + (DIAG ? 1 : 0) + (TRAC ? 3 : 0);
+    __shared__ float rec[NF][kChunkPad];
+    SegInfo si;
+    uint32_t S, E;
+    if (!seg_setup(g, L, nseg_x, si, S, E)) return;
+    const int t = threadIdx.x;
+    const size_t ld = P.ld;
+    const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
+
+    float acc[8][7];
+    float vof[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+#pragma unroll
+        for (int f = 0; f < 7; f++) acc[q][f] = 0.f;
+        vof[q] = 0.f;
+    }
+
+    for (uint32_t base = S; base < E; base += kChunk) {
+        const int nch = (int)min((uint32_t)kChunk, E - base);
+        for (int i = t; i < nch; i += kSeg) {
+            uint32_t slot = L.perm[base + i];
+            int pix = (int)P.key[slot] - si.rowbase;
+            float fx = local_coord(P.xh[slot], P.xl[slot], T.tnh, T.tnl, pix, inv0);
+            float fy = local_coord(P.xh[ld + slot], P.xl[ld + slot], T.tnh + T.ld, T.tnl + T.ld, si.iy, inv1);
+            float fz =
+                local_coord(P.xh[2 * ld + slot], P.xl[2 * ld + slot], T.tnh + 2 * T.ld, T.tnl + 2 * T.ld, si.iz, inv2);
+            float m = P.m[slot];
+            float Vf = (float)P.V[slot];
+            int li = lds_idx(i);
+            rec[0][li] = fx;
+            rec[1][li] = fy;
+            rec[2][li] = fz;
+            rec[3][li] = m;
+            rec[4][li] = m * P.v[slot];
+            rec[5][li] = m * P.v[ld + slot];
+            rec[6][li] = m * P.v[2 * ld + slot];
+            // -V sigma (negated so that the force update is a plain FMA)
+#pragma unroll
+            for (int q = 0; q < 6; q++) rec[7 + q][li] = -Vf * P.sig[q * ld + slot];
+            if (DIAG) rec[13][li] = Vf;
+            if (TRAC) {
+                rec[NF - 3][li] = P.trac[slot];
+                rec[NF - 2][li] = P.trac[ld + slot];
+                rec[NF - 1][li] = P.trac[2 * ld + slot];
+            }
+        }
+        __syncthreads();
+        const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
+        for (uint32_t pos = lo; pos < hi; pos++) {
+            const int li = lds_idx((int)(pos - base));
+            const float fx = rec[0][li], fy = rec[1][li], fz = rec[2][li];
+            const float m = rec[3][li], px = rec[4][li], py = rec[5][li], pz = rec[6][li];
+            // Voigt xx,yy,zz,yz,xz,xy of -V sigma
+            const float sxx = rec[7][li], syy = rec[8][li], szz = rec[9][li];
+            const float syz = rec[10][li], sxz = rec[11][li], sxy = rec[12][li];
+            const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const int q = i + 2 * j + 4 * k;
+                        const float wyz = wy[j] * wz[k];
+                        const float N = wx[i] * wyz;
+                        const float gx = (i ? inv0 : -inv0) * wyz;
+                        const float gy = (j ? inv1 : -inv1) * (wx[i] * wz[k]);
+                        const float gz = (k ? inv2 : -inv2) * (wx[i] * wy[j]);
+                        acc[q][0] += N * m;
+                        acc[q][1] += N * px;
+                        acc[q][2] += N * py;
+                        acc[q][3] += N * pz;
+                        acc[q][4] += sxx * gx + sxy * gy + sxz * gz;
+                        acc[q][5] += sxy * gx + syy * gy + syz * gz;
+                        acc[q][6] += sxz * gx + syz * gy + szz * gz;
+                        if (TRAC) {
+                            acc[q][4] += N * rec[NF - 3][li];
+                            acc[q][5] += N * rec[NF - 2][li];
+                            acc[q][6] += N * rec[NF - 1][li];
+                        }
+                        if (DIAG) vof[q] += N * rec[13][li];
+                    }
+        }
+        __syncthreads();
+    }
+    if (si.ce > si.cs) {
+        const size_t ne = (size_t)g.nelem;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int f = 0; f < 7; f++) part[(size_t)(q * 7 + f) * ne + si.c] = acc[q][f];
+        if (DIAG) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) part[(size_t)(56 + q) * ne + si.c] = vof[q];
+        }
+    }
+    (void)t;
+}
+
+__device__ __forceinline__ void node_ijk(const GridGeom &g, int n, int &i, int &j, int &k) {
+    i = n % g.nxc[0];
+    j = (n / g.nxc[0]) % g.nxc[1];
+    k = n / (g.nxc[0] * g.nxc[1]);
+}
+
+__device__ __forceinline__ float safe_div(float p, float m, float m_small) {
+    if (m >= m_small) return p / m;
+    return (float)((double)p / (double)m);
+}
+
+template <bool DIAG>
+__global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, const float *__restrict__ part,
+                                                      NodesF32 N, BcDense bc, StepConsts sc) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    const size_t ne = (size_t)g.nelem;
+    constexpr int NQ = DIAG ? 8 : 7;
+    float s[NQ], cp[NQ];
+#pragma unroll
+    for (int f = 0; f < NQ; f++) {
+        s[f] = 0.f;
+        cp[f] = 0.f;
+    }
+    bool any = false;
+    for (int ck = k - 1; ck <= k; ck++) {
+        if (ck < 0 || ck >= g.nex[2]) continue;
+        for (int cj = j - 1; cj <= j; cj++) {
+            if (cj < 0 || cj >= g.nex[1]) continue;
+            for (int ci = i - 1; ci <= i; ci++) {
+                if (ci < 0 || ci >= g.nex[0]) continue;
+                int c = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
+                if (L.cend[c] == 0u) continue;
+                int corner = (i - ci) + 2 * (j - cj) + 4 * (k - ck);
+                any = true;
+#pragma unroll
+                for (int f = 0; f < 7; f++) neu_add(s[f], cp[f], part[(size_t)(corner * 7 + f) * ne + c]);
+                if (DIAG) neu_add(s[7], cp[7], part[(size_t)(56 + corner) * ne + c]);
+            }
+        }
+    }
+    const int nodec = g.nodec;
+    if (!any) {
+        if (DIAG) {
+            N.nm[n] = 0.f;
+            N.nvof[n] = 0.f;
+            for (int c = 0; c < 3; c++) {
+                N.nv[c * nodec + n] = 0.f;
+                N.na[c * nodec + n] = 0.f;
+                N.np[c * nodec + n] = 0.f;
+                N.nf[c * nodec + n] = 0.f;
+                N.nvm[c * nodec + n] = 0.f;
+            }
+        }
+        return;
+    }
+    float m = s[0] + cp[0];
+    float p[3], f[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) p[c] = s[1 + c] + cp[1 + c];
+    f[0] = (s[4] + cp[4]) + m * sc.bx;
+    f[1] = (s[5] + cp[5]) + m * sc.by;
+    f[2] = (s[6] + cp[6]) + m * sc.bz;
+    float v[3] = {0.f, 0.f, 0.f}, a[3] = {0.f, 0.f, 0.f};
+    if ((double)m > sc.mtol) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v[c] = safe_div(p[c], m, sc.m_small);
+            a[c] = safe_div(f[c], m, sc.m_small);
+        }
+    }
+    uint8_t mk = bc.mask ? bc.mask[n] : 0;
+    if (mk) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            if (mk & (1 << c)) {
+                v[c] = (float)(bc.val[c * nodec + n] * sc.facl);
+                a[c] = 0.f;
+            }
+    }
+    N.nm[n] = m;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        N.nv[c * nodec + n] = v[c];
+        N.na[c * nodec + n] = a[c];
+    }
+    if (DIAG) {
+        float vof = s[7] + cp[7];
+        if ((double)vof < sc.mtol) vof = 0.f;
+        N.nvof[n] = vof;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            N.np[c * nodec + n] = p[c];
+            N.nf[c * nodec + n] = f[c];
+        }
+    }
+}
+
+// ------------------------------------------------------ G2P velocity + MUSL
+template <bool FLIP>
+__global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T, NodesF32 N,
+                                                        StepConsts sc, float *__restrict__ part, int nseg_x) {
+    constexpr int NF = 8 + (FLIP ? 3 : 0);  // fx fy fz m vx vy vz slot [v_old]
+    __shared__ float rec[NF][kChunkPad];
+    SegInfo si;
+    uint32_t S, E;
+    if (!seg_setup(g, L, nseg_x, si, S, E)) return;
+    const int t = threadIdx.x;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
+    const bool ne = si.ce > si.cs;
+
+    // node values of this cell: PIC uses v + dt a (var_trans_solid_explicit.cpp:194-200),
+    // FLIP uses a
+    float nvl[8][3];
+    if (ne) {
+        const int n0 = si.ix + g.nxc[0] * si.iy + g.nxc[0] * g.nxc[1] * si.iz;
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    const int q = i + 2 * j + 4 * k;
+                    const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        float a = N.na[c * nodec + nid];
+                        nvl[q][c] = FLIP ? a : (N.nv[c * nodec + nid] + sc.dt * a);
+                    }
+                }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) nvl[q][c] = 0.f;
+    }
+
+    float acc[8][3];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[q][c] = 0.f;
+
+    for (uint32_t base = S; base < E; base += kChunk) {
+        const int nch = (int)min((uint32_t)kChunk, E - base);
+        for (int i = t; i < nch; i += kSeg) {
+            uint32_t slot = L.perm[base + i];
+            int pix = (int)P.key[slot] - si.rowbase;
+            int li = lds_idx(i);
+            rec[0][li] = local_coord(P.xh[slot], P.xl[slot], T.tnh, T.tnl, pix, inv0);
+            rec[1][li] = local_coord(P.xh[ld + slot], P.xl[ld + slot], T.tnh + T.ld, T.tnl + T.ld, si.iy, inv1);
+            rec[2][li] =
+                local_coord(P.xh[2 * ld + slot], P.xl[2 * ld + slot], T.tnh + 2 * T.ld, T.tnl + 2 * T.ld, si.iz, inv2);
+            rec[3][li] = P.m[slot];
+            rec[7][li] = __uint_as_float(slot);
+            if (FLIP) {
+                rec[8][li] = P.v[slot];
+                rec[9][li] = P.v[ld + slot];
+                rec[10][li] = P.v[2 * ld + slot];
+            }
+        }
+        __syncthreads();
+        const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
+        for (uint32_t pos = lo; pos < hi; pos++) {
+            const int li = lds_idx((int)(pos - base));
+            const float fx = rec[0][li], fy = rec[1][li], fz = rec[2][li], m = rec[3][li];
+            const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
+            float Nw[8];
+            float vp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const int q = i + 2 * j + 4 * k;
+                        Nw[q] = wx[i] * (wy[j] * wz[k]);
+#pragma unroll
+                        for (int c = 0; c < 3; c++) vp[c] += Nw[q] * nvl[q][c];
+                    }
+            if (FLIP) {
+                vp[0] = rec[8][li] + sc.dt * vp[0];
+                vp[1] = rec[9][li] + sc.dt * vp[1];
+                vp[2] = rec[10][li] + sc.dt * vp[2];
+            }
+            rec[4][li] = vp[0];
+            rec[5][li] = vp[1];
+            rec[6][li] = vp[2];
+            const float mv[3] = {m * vp[0], m * vp[1], m * vp[2]};
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) acc[q][c] += Nw[q] * mv[c];
+        }
+        __syncthreads();
+        for (int i = t; i < nch; i += kSeg) {
+            int li = lds_idx(i);
+            uint32_t slot = __float_as_uint(rec[7][li]);
+            P.v[slot] = rec[4][li];
+            P.v[ld + slot] = rec[5][li];
+            P.v[2 * ld + slot] = rec[6][li];
+        }
+    }
+    if (ne) {
+        const size_t nel = (size_t)g.nelem;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) part[(size_t)(q * 3 + c) * nel + si.c] = acc[q][c];
+    }
+}
+
+template <bool DIAG>
+__global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, const float *__restrict__ part,
+                                                       NodesF32 N, BcDense bc, StepConsts sc) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    const size_t ne = (size_t)g.nelem;
+    float s[3] = {0.f, 0.f, 0.f}, cp[3] = {0.f, 0.f, 0.f};
+    bool any = false;
+    for (int ck = k - 1; ck <= k; ck++) {
+        if (ck < 0 || ck >= g.nex[2]) continue;
+        for (int cj = j - 1; cj <= j; cj++) {
+            if (cj < 0 || cj >= g.nex[1]) continue;
+            for (int ci = i - 1; ci <= i; ci++) {
+                if (ci < 0 || ci >= g.nex[0]) continue;
+                int c = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
+                if (L.cend[c] == 0u) continue;
+                int corner = (i - ci) + 2 * (j - cj) + 4 * (k - ck);
+                any = true;
+#pragma unroll
+                for (int f = 0; f < 3; f++) neu_add(s[f], cp[f], part[(size_t)(corner * 3 + f) * ne + c]);
+            }
+        }
+    }
+    const int nodec = g.nodec;
+    if (!any) {
+        if (DIAG)
+            for (int c = 0; c < 3; c++) N.nvm[c * nodec + n] = 0.f;
+        return;
+    }
+    float m = N.nm[n];
+    float v[3] = {0.f, 0.f, 0.f};
+    float p[3] = {s[0] + cp[0], s[1] + cp[1], s[2] + cp[2]};
+    if ((double)m > sc.mtol) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = safe_div(p[c], m, sc.m_small);
+    }
+    uint8_t mk = bc.mask ? bc.mask[n] : 0;
+    if (mk) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            if (mk & (1 << c)) v[c] = (float)(bc.val[c * nodec + n] * sc.facl);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) N.nvm[c * nodec + n] = v[c];
+    if (DIAG) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) N.np[c * nodec + n] = p[c];
+    }
+}
+
+// ------------------------------------------------------ particle update
+__device__ __forceinline__ void two_sum_update(float &xh, float &xl, float d) {
+    float s = xh + d;
+    float bp = s - xh;
+    float e = (xh - (s - bp)) + (d - bp);
+    float l = xl + e;
+    float t = s + l;
+    xl = l - (t - s);
+    xh = t;
+}
+
+// Stress update in FP32 from H = F - I and the increment D = dF - I
+// (semantics of ConstitutiveModel::UpdateStress, constitutive_model.cpp:287-305,
+// rewritten without cancellation against the identity).
+__device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9], const float D[9], double J,
+                                         float s[6], float *eps_p) {
+    const double *p = md.prop;
+    switch (md.model) {
+    case -1:
+        for (int q = 0; q < 6; q++) s[q] = 0.f;
+        break;
+    case 0: {  // hypoelastic increment, dF - I = D
+        float E = (float)p[0], nu = (float)p[1];
+        float C0 = (E * (1.f - nu)) / ((1.f - 2.f * nu) * (1.f + nu));
+        float C1 = (E * nu) / ((1.f - 2.f * nu) * (1.f + nu));
+        float C2 = E / (2.f * (1.f + nu));
+        float e0 = D[0], e1 = D[4], e2 = D[8];
+        s[0] += C0 * e0 + C1 * (e1 + e2);
+        s[1] += C0 * e1 + C1 * (e0 + e2);
+        s[2] += C0 * e2 + C1 * (e0 + e1);
+        s[3] += C2 * (D[5] + D[7]);
+        s[4] += C2 * (D[6] + D[2]);
+        s[5] += C2 * (D[1] + D[3]);
+        break;
+    }
+    case 1: {  // Neo-Hookean: sigma = (mu (b - I) + lambda ln J I) / J, b - I = H + H^T + H H^T
+        float E = (float)p[0], nu = (float)p[1];
+        float lmd = nu * E / ((1.f + nu) * (1.f - 2.f * nu));
+        float mu = E / (2.f * (1.f + nu));
+        float lnj = log1pf((float)(J - 1.0));
+        float invJ = (float)(1.0 / J);
+        float B[6];
+        // B = H + H^T + H H^T in Voigt order xx,yy,zz,yz,xz,xy
+        B[0] = 2.f * H[0] + (H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
+        B[1] = 2.f * H[4] + (H[3] * H[3] + H[4] * H[4] + H[5] * H[5]);
+        B[2] = 2.f * H[8] + (H[6] * H[6] + H[7] * H[7] + H[8] * H[8]);
+        B[3] = (H[5] + H[7]) + (H[3] * H[6] + H[4] * H[7] + H[5] * H[8]);
+        B[4] = (H[2] + H[6]) + (H[0] * H[6] + H[1] * H[7] + H[2] * H[8]);
+        B[5] = (H[1] + H[3]) + (H[0] * H[3] + H[1] * H[4] + H[2] * H[5]);
+        float vol = lmd * lnj;
+        s[0] = (mu * B[0] + vol) * invJ;
+        s[1] = (mu * B[1] + vol) * invJ;
+        s[2] = (mu * B[2] + vol) * invJ;
+        s[3] = mu * B[3] * invJ;
+        s[4] = mu * B[4] * invJ;
+        s[5] = mu * B[5] * invJ;
+        break;
+    }
+    case 2: {  // St.Venant-Kirchhoff: sigma = (mu (B + B^2) + lambda trE (I + B)) / J with b = I + B
+        float E = (float)p[0], nu = (float)p[1];
+        float lmd = nu * E / ((1.f + nu) * (1.f - 2.f * nu));
+        float mu = E / (2.f * (1.f + nu));
+        float invJ = (float)(1.0 / J);
+        float Bm[9];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                Bm[3 * a + b] = H[3 * a + b] + H[3 * b + a] +
+                                (H[3 * a] * H[3 * b] + H[3 * a + 1] * H[3 * b + 1] + H[3 * a + 2] * H[3 * b + 2]);
+        float trE = 0.5f * (Bm[0] + Bm[4] + Bm[8]);
+        float sg[9];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                float b2 = Bm[3 * a] * Bm[b] + Bm[3 * a + 1] * Bm[3 + b] + Bm[3 * a + 2] * Bm[6 + b];
+                float eye = (a == b) ? 1.f : 0.f;
+                sg[3 * a + b] = (mu * (Bm[3 * a + b] + b2) + lmd * trE * (eye + Bm[3 * a + b])) * invJ;
+            }
+        s[0] = sg[0];
+        s[1] = sg[4];
+        s[2] = sg[8];
+        s[3] = sg[5];
+        s[4] = sg[2];
+        s[5] = sg[1];
+        break;
+    }
+    case 3: {  // Mooney-Rivlin (constitutive_model.cpp:187-285), direct FP32 form
+        float c1 = (float)p[0], c2 = (float)p[1], bulk = (float)p[2];
+        float Jf = (float)J;
+        float sc = 1.f / cbrtf(Jf);
+        float F[9];
+        for (int q = 0; q < 9; q++) F[q] = sc * (H[q] + ((q == 0 || q == 4 || q == 8) ? 1.f : 0.f));
+        float be[9], be2[9];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                be[3 * a + b] = F[3 * a] * F[3 * b] + F[3 * a + 1] * F[3 * b + 1] + F[3 * a + 2] * F[3 * b + 2];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                be2[3 * a + b] = be[3 * a] * be[b] + be[3 * a + 1] * be[3 + b] + be[3 * a + 2] * be[6 + b];
+        float I1 = be[0] + be[4] + be[8];
+        float I2 = 0.5f * (I1 * I1 - (be2[0] + be2[4] + be2[8]));
+        float hp = bulk * log1pf((float)(J - 1.0)) / Jf;
+        float sg[9];
+        for (int a = 0; a < 3; a++) {
+            for (int b = 0; b < 3; b++) sg[3 * a + b] = 2.f * ((c1 + c2 * I1) * be[3 * a + b] - c2 * be2[3 * a + b]) / Jf;
+            sg[4 * a] += hp - 2.f / (3.f * Jf) * (c1 * I1 + 2.f * c2 * I2);
+        }
+        s[0] = sg[0];
+        s[1] = sg[4];
+        s[2] = sg[8];
+        s[3] = sg[5];
+        s[4] = sg[2];
+        s[5] = sg[1];
+        break;
+    }
+    case 10: {  // J2 radial return, small-increment form
+        float E = (float)p[0], nu = (float)p[1], sy = (float)p[2], Hh = (float)p[3];
+        float lmd = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
+        float mu = E / (2.f * (1.f + nu));
+        float de[6] = {D[0], D[4], D[8], D[5] + D[7], D[6] + D[2], D[1] + D[3]};
+        float tr = de[0] + de[1] + de[2];
+        float st[6];
+        st[0] = s[0] + lmd * tr + 2.f * mu * de[0];
+        st[1] = s[1] + lmd * tr + 2.f * mu * de[1];
+        st[2] = s[2] + lmd * tr + 2.f * mu * de[2];
+        st[3] = s[3] + mu * de[3];
+        st[4] = s[4] + mu * de[4];
+        st[5] = s[5] + mu * de[5];
+        float pm = (st[0] + st[1] + st[2]) * (1.f / 3.f);
+        float d0 = st[0] - pm, d1 = st[1] - pm, d2 = st[2] - pm;
+        float j2 = 0.5f * (d0 * d0 + d1 * d1 + d2 * d2) + st[3] * st[3] + st[4] * st[4] + st[5] * st[5];
+        float q = sqrtf(3.f * j2);
+        float ep = eps_p ? *eps_p : 0.f;
+        float f = q - (sy + Hh * ep);
+        if (f <= 0.f) {
+            for (int c = 0; c < 6; c++) s[c] = st[c];
+        } else {
+            float dg = f / (3.f * mu + Hh);
+            float scl = 1.f - 3.f * mu * dg / q;
+            s[0] = pm + d0 * scl;
+            s[1] = pm + d1 * scl;
+            s[2] = pm + d2 * scl;
+            s[3] = st[3] * scl;
+            s[4] = st[4] * scl;
+            s[5] = st[5] * scl;
+            if (eps_p) *eps_p = ep + dg;
+        }
+        break;
+    }
+    default: break;
+    }
+}
+
+__global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
+                                                       MaterialTable mt, StepConsts sc, uint32_t *err) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= P.n) return;
+    const uint32_t key = P.key[slot];
+    if (key >= (uint32_t)g.nelem) return;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    const int nexy = g.nex[0] * g.nex[1];
+    const int iz = (int)key / nexy;
+    const int rem = (int)key - iz * nexy;
+    const int iy = rem / g.nex[0];
+    const int ix = rem - iy * g.nex[0];
+    float xh[3], xl[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        xh[a] = P.xh[a * ld + slot];
+        xl[a] = P.xl[a * ld + slot];
+    }
+    const float fx = local_coord(xh[0], xl[0], T.tnh, T.tnl, ix, T.inv_dx[0]);
+    const float fy = local_coord(xh[1], xl[1], T.tnh + T.ld, T.tnl + T.ld, iy, T.inv_dx[1]);
+    const float fz = local_coord(xh[2], xl[2], T.tnh + 2 * T.ld, T.tnl + 2 * T.ld, iz, T.inv_dx[2]);
+    const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
+    const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
+    const int n0 = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+    float Dv[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) Dv[q] = 0.f;
+    float dxp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
+                const float v0 = N.nvm[nid], v1 = N.nvm[nodec + nid], v2 = N.nvm[2 * nodec + nid];
+                const float wyz = wy[j] * wz[k];
+                const float Nw = wx[i] * wyz;
+                const float gx = (i ? inv0 : -inv0) * wyz;
+                const float gy = (j ? inv1 : -inv1) * (wx[i] * wz[k]);
+                const float gz = (k ? inv2 : -inv2) * (wx[i] * wy[j]);
+                Dv[0] += v0 * gx;
+                Dv[1] += v0 * gy;
+                Dv[2] += v0 * gz;
+                Dv[3] += v1 * gx;
+                Dv[4] += v1 * gy;
+                Dv[5] += v1 * gz;
+                Dv[6] += v2 * gx;
+                Dv[7] += v2 * gy;
+                Dv[8] += v2 * gz;
+                dxp[0] += Nw * v0;
+                dxp[1] += Nw * v1;
+                dxp[2] += Nw * v2;
+            }
+    float D[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) D[q] = sc.dt * Dv[q];
+    // H_new = (I + D)(I + H) - I = H + D + D H
+    float H[9], Hn[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) H[q] = P.H[q * ld + slot];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            Hn[3 * a + b] = (H[3 * a + b] + D[3 * a + b]) +
+                            (D[3 * a] * H[b] + D[3 * a + 1] * H[3 + b] + D[3 * a + 2] * H[6 + b]);
+    // J = det(I + H_new) in FP64
+    double A[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) A[q] = (double)Hn[q] + ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0);
+    const double J = A[0] * (A[4] * A[8] - A[5] * A[7]) + A[1] * (A[5] * A[6] - A[3] * A[8]) +
+                     A[2] * (A[3] * A[7] - A[4] * A[6]);
+    const double V = J * (double)P.V0[slot];
+    // position, compensated
+#pragma unroll
+    for (int a = 0; a < 3; a++) two_sum_update(xh[a], xl[a], sc.dt * dxp[a]);
+    const int mat = P.matid ? P.matid[slot] : 0;
+    const MaterialDev &md = mt.m[mat];
+    // only the rate forms (linear elastic increment, J2) read the old stress
+    const bool rate_form = (md.model == 0 || md.model == 10);
+    float s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (rate_form) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) s[q] = P.sig[q * ld + slot];
+    }
+    float ep = (md.model == 10 && P.eps_p) ? P.eps_p[slot] : 0.f;
+    stress32(md, Hn, D, J, s, &ep);
+    if (md.model == 10 && P.eps_p) P.eps_p[slot] = ep;
+#pragma unroll
+    for (int q = 0; q < 9; q++) P.H[q * ld + slot] = Hn[q];
+#pragma unroll
+    for (int q = 0; q < 6; q++) P.sig[q * ld + slot] = s[q];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        P.xh[a * ld + slot] = xh[a];
+        P.xl[a * ld + slot] = xl[a];
+    }
+    P.J[slot] = J;
+    P.V[slot] = V;
+    const uint32_t nk = cell_key_f64(g, (double)xh[0] + (double)xl[0], (double)xh[1] + (double)xl[1],
+                                     (double)xh[2] + (double)xl[2]);
+    P.key[slot] = nk;
+    uint32_t e = 0;
+    if (nk == (uint32_t)g.nelem) e |= kErrOutOfGrid;
+    if (!(J > 0.0)) e |= kErrJacobian;
+    if (!isfinite(xh[0] + xh[1] + xh[2] + s[0] + s[1] + s[2])) e |= kErrNonFinite;
+    if (e) atomicOr(err, e);
+}
+
+__global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict__ perm, const int *__restrict__ pid_src,
+                            int *__restrict__ pid_dst) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= src.n) return;
+    const uint32_t o = perm[s];
+    const size_t ls = src.ld, lt = dst.ld;
+    for (int a = 0; a < 3; a++) {
+        dst.xh[a * lt + s] = src.xh[a * ls + o];
+        dst.xl[a * lt + s] = src.xl[a * ls + o];
+        dst.v[a * lt + s] = src.v[a * ls + o];
+        if (src.trac) dst.trac[a * lt + s] = src.trac[a * ls + o];
+    }
+    for (int q = 0; q < 9; q++) dst.H[q * lt + s] = src.H[q * ls + o];
+    for (int q = 0; q < 6; q++) dst.sig[q * lt + s] = src.sig[q * ls + o];
+    dst.m[s] = src.m[o];
+    dst.V0[s] = src.V0[o];
+    dst.V[s] = src.V[o];
+    dst.J[s] = src.J[o];
+    dst.key[s] = src.key[o];
+    if (src.eps_p) dst.eps_p[s] = src.eps_p[o];
+    if (src.matid) dst.matid[s] = src.matid[o];
+    pid_dst[s] = pid_src[o];
+}
+
+}  // namespace
+
+static int segs_x(const GridGeom &g) { return cdiv(g.nex[0], kSeg); }
+
+void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                    const BcDense &bc, const NodeTables &T, const StepConsts &c, float *partials, int diag,
+                    hipEvent_t mid) {
+    const int nsx = segs_x(g);
+    const int nseg = nsx * g.nex[1] * g.nex[2];
+    const bool trac = P.trac != nullptr;
+    if (diag) {
+        if (trac)
+            hipLaunchKernelGGL((k32_p2g_cells<true, true>), dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, partials, nsx);
+        else
+            hipLaunchKernelGGL((k32_p2g_cells<true, false>), dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, partials, nsx);
+        if (mid) (void)hipEventRecord(mid, s);
+        hipLaunchKernelGGL(k32_p2g_nodes<true>, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, L,
+                           (const float *)partials, N, bc, c);
+    } else {
+        if (trac)
+            hipLaunchKernelGGL((k32_p2g_cells<false, true>), dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, partials, nsx);
+        else
+            hipLaunchKernelGGL((k32_p2g_cells<false, false>), dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, partials,
+                               nsx);
+        if (mid) (void)hipEventRecord(mid, s);
+        hipLaunchKernelGGL(k32_p2g_nodes<false>, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, L,
+                           (const float *)partials, N, bc, c);
+    }
+}
+
+void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
+                             const CellLists &L, const BcDense &bc, const NodeTables &T, const StepConsts &c,
+                             float *partials, int diag, hipEvent_t mid) {
+    const int nsx = segs_x(g);
+    const int nseg = nsx * g.nex[1] * g.nex[2];
+    if (c.scheme == 1)
+        hipLaunchKernelGGL(k32_musl_cells<true>, dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, N, c, partials, nsx);
+    else
+        hipLaunchKernelGGL(k32_musl_cells<false>, dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, N, c, partials, nsx);
+    if (mid) (void)hipEventRecord(mid, s);
+    if (diag)
+        hipLaunchKernelGGL(k32_musl_nodes<true>, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, L,
+                           (const float *)partials, N, bc, c);
+    else
+        hipLaunchKernelGGL(k32_musl_nodes<false>, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, L,
+                           (const float *)partials, N, bc, c);
+}
+
+void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
+                           const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k32_g2p_update, dim3(cdiv(P.n, 256)), dim3(256), 0, s, g, P, N, T, mt, c, err);
+}
+
+__global__ void k_iota(uint32_t *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
+void launch_iota_u32(hipStream_t s, uint32_t *p, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_iota, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n);
+}
+
+void launch_f32_permute(hipStream_t s, const PartF32 &src, const PartF32 &dst, const uint32_t *perm,
+                        const int *pid_src, int *pid_dst) {
+    if (src.n <= 0) return;
+    hipLaunchKernelGGL(k32_permute, dim3(cdiv(src.n, 256)), dim3(256), 0, s, src, dst, perm, pid_src, pid_dst);
+}
+
+}  // namespace mpm

@@ -1,0 +1,145 @@
+// Launchers shared between the kernel translation units and the engine.
+#pragma once
+
+#include "mpm_common.h"
+
+namespace mpm {
+
+// ---------------------------------------------------------------- binning
+struct SortBuffers {
+    uint32_t *keys;      // output buffers of alternating passes
+    uint32_t *vals;
+    uint32_t *keys_alt;
+    uint32_t *vals_alt;
+    uint32_t *hist;      // radix_hist_entries(n)
+    uint32_t *bsum;      // >= 4096 entries
+};
+
+size_t radix_hist_entries(int n);
+int excl_scan_u32(hipStream_t s, uint32_t *data, int n, uint32_t *bsum);
+// Stable LSD sort of (keys_in[i], i) over nbits key bits. keys_in is not
+// modified. Results point into the SortBuffers arrays.
+int radix_sort_pairs(hipStream_t s, SortBuffers &b, const uint32_t *keys_in, int n, int nbits, uint32_t **keys_out,
+                     uint32_t **vals_out);
+void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
+void launch_cell_ranges(hipStream_t s, const uint32_t *skeys, int n, int nelem, uint32_t *cstart, uint32_t *cend);
+void launch_keys_f64(hipStream_t s, const GridGeom &g, const double *x, size_t ld, int n, uint32_t *keys,
+                     uint32_t *err);
+void launch_keys_hilo(hipStream_t s, const GridGeom &g, const float *xh, const float *xl, size_t ld, int n,
+                      uint32_t *keys, uint32_t *err);
+
+// Cell lists produced by the binning: particles of cell c are
+// perm[cstart[c] .. cend[c]) (both 0 for an empty cell).
+struct CellLists {
+    const uint32_t *perm;
+    const uint32_t *cstart;
+    const uint32_t *cend;
+};
+
+// ------------------------------------------------------ FP64 exact kernels
+struct PartF64 {
+    double *x;      // [3][ld]
+    double *v;      // [3][ld]
+    double *m;
+    double *V;
+    double *V0;
+    double *sig;    // [6][ld]
+    double *F;      // [9][ld]
+    double *J;
+    double *trac;   // [3][ld] or null
+    int *matid;     // or null
+    double *eps_p;  // or null
+    double *ddg;    // [9][ld] deformation-gradient increment
+    const uint32_t *key;
+    size_t ld;
+    int n;
+};
+
+struct NodesF64 {
+    double *nm;
+    double *nvof;
+    double *np;  // [3][nodec]
+    double *nf;
+    double *nv;
+    double *na;
+};
+
+void launch_f64_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
+                    const BcDense &bc, double fx, double fy, double fz, double mtol, double facl);
+void launch_f64_solve_grid(hipStream_t s, const GridGeom &g, const NodesF64 &N, const BcDense &bc, double mtol);
+void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
+                    const BcDense &bc, const MaterialTable &mt, int scheme, double dt, double mtol, double facl,
+                    uint32_t *err);
+int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cell, const double *xyg, int32_t *nc,
+                     double *sf, double *dsf);
+
+// ------------------------------------------------------------ FP32 kernels
+struct PartF32 {
+    float *xh;      // [3][ld] position, leading part
+    float *xl;      // [3][ld] position, compensation part
+    float *v;       // [3][ld]
+    float *m;
+    float *V0;
+    double *V;      // FP64 volume V = J V0
+    double *J;      // FP64 det F
+    float *H;       // [9][ld] displacement gradient F - I
+    float *sig;     // [6][ld]
+    float *eps_p;   // J2 equivalent plastic strain (or null)
+    float *trac;    // [3][ld] or null
+    int *matid;     // or null
+    uint32_t *key;  // cell of each slot
+    size_t ld;
+    int n;
+};
+
+struct NodesF32 {
+    float *nm;
+    float *nv;    // [3][nodec] velocity after the P2G grid update (+dt a for PIC)
+    float *na;    // [3][nodec] acceleration (FLIP and diagnostics)
+    float *nvm;   // [3][nodec] MUSL velocity
+    float *np;    // [3][nodec] momentum (diagnostics, may be null)
+    float *nf;    // [3][nodec] force (diagnostics, may be null)
+    float *nvof;  // [nodec] volume (diagnostics, may be null)
+};
+
+struct StepConsts {
+    float bx, by, bz;  // body force * facl
+    float dt;
+    double dt64;
+    double mtol;
+    double facl;
+    float m_small;     // below this node mass the division runs in FP64
+    int scheme;
+};
+
+// FP32 node coordinates as (hi, lo) float pairs per axis, [3][ld], so that
+// particle positions relative to a node keep FP32 precision of the offset.
+struct NodeTables {
+    const float *tnh;
+    const float *tnl;
+    int ld;
+    float inv_dx[3];
+};
+
+// Partials buffer: 64 floats per cell: 8 corners x {m, px, py, pz, fx, fy, fz}
+// (MUSL uses the first 24 as 8 corners x {px, py, pz}), then 8 corner volumes
+// in diagnostic mode.
+constexpr int kPartialsPerCell = 64;
+
+// P2G: corner partials per cell + node pass with the grid update (v = p/m,
+// a = f/m, BCs). diag != 0 also writes momentum, force and nodal volume.
+void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                    const BcDense &bc, const NodeTables &T, const StepConsts &c, float *partials, int diag,
+                    hipEvent_t mid = nullptr);
+// G2P velocity fused with the MUSL momentum remap + MUSL node pass.
+void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
+                             const CellLists &L, const BcDense &bc, const NodeTables &T, const StepConsts &c,
+                             float *partials, int diag, hipEvent_t mid = nullptr);
+// dF, H, J, V, x, stress and the new cell key of every slot.
+void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
+                           const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err);
+void launch_f32_permute(hipStream_t s, const PartF32 &src, const PartF32 &dst, const uint32_t *perm,
+                        const int *pid_src, int *pid_dst);
+void launch_iota_u32(hipStream_t s, uint32_t *p, int n);
+
+}  // namespace mpm

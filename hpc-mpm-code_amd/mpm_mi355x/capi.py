@@ -1,0 +1,318 @@
+"""ctypes binding of the engine's C ABI (include/mpm/mpm_capi.h).
+
+The structures below mirror the header field for field. The loaded library
+is the in-tree ``libmpm_mi355x.so``; there is no fallback: if it is missing
+or the GPU is absent, calls fail with an exception.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libmpm_mi355x.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "mpm", "mpm_capi.h")
+
+MPM_OK = 0
+MPM_ERR_INVALID_ARGUMENT = -1
+MPM_ERR_OUT_OF_GRID = -2
+MPM_ERR_NONFINITE = -3
+MPM_ERR_JACOBIAN = -4
+MPM_ERR_HIP = -5
+MPM_ERR_UNSUPPORTED = -6
+MPM_ERR_NO_DEVICE = -7
+
+SCHEME_PIC, SCHEME_FLIP = 0, 1
+PREC_FP64_EXACT, PREC_FP32 = 0, 1
+MODEL_RIGID, MODEL_LINELAST, MODEL_NEOHOOKEAN, MODEL_STVK, MODEL_MOONEYRIVLIN, MODEL_J2 = -1, 0, 1, 2, 3, 10
+OPT_FULL_NODE_OUTPUT = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("xmin", C.c_double * 3), ("xmax", C.c_double * 3), ("nelem", C.c_int32 * 3),
+                ("idimc", C.c_int32 * 3)]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("model", C.c_int32), ("nprop", C.c_int32), ("prop", C.c_double * 8)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("mtol", C.c_double), ("bb", C.c_double * 3), ("scheme", C.c_int32),
+                ("nlstep", C.c_int32), ("precision", C.c_int32), ("relayout_every", C.c_int32)]
+
+
+class ParticlesView(C.Structure):
+    _fields_ = [("n", C.c_int64), ("coord", _dp), ("vel", _dp), ("mass", _dp), ("vol", _dp), ("vol0", _dp),
+                ("stress", _dp), ("def_grad", _dp), ("det_def_grad", _dp), ("trac_force", _dp), ("matid", _ip),
+                ("eps_p", _dp)]
+
+
+class NodesView(C.Structure):
+    _fields_ = [("nodec", C.c_int64), ("nmass", _dp), ("nvof", _dp), ("nmome", _dp), ("nforce", _dp),
+                ("nvel", _dp), ("naccel", _dp)]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("bin_ms", C.c_double), ("p2g_ms", C.c_double), ("p2g_cells_ms", C.c_double),
+                ("p2g_nodes_ms", C.c_double), ("g2p_vel_ms", C.c_double), ("musl_cells_ms", C.c_double),
+                ("musl_nodes_ms", C.c_double), ("g2p_upd_ms", C.c_double), ("total_ms", C.c_double),
+                ("steps", C.c_int32), ("active_cells", C.c_int32), ("active_nodes", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+# name -> (restype, argtypes); every function declared in mpm_capi.h
+SIGNATURES = {
+    "mpm_version": (C.c_char_p, []),
+    "mpm_last_error": (C.c_char_p, []),
+    "mpm_engine_create": (C.c_int, [C.POINTER(GridDesc), C.POINTER(MaterialDesc), C.c_int32,
+                                    C.POINTER(StepParams), C.POINTER(C.c_void_p)]),
+    "mpm_engine_destroy": (C.c_int, [C.c_void_p]),
+    "mpm_set_bc": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, C.c_int32]),
+    "mpm_upload_particles": (C.c_int, [C.c_void_p, C.POINTER(ParticlesView)]),
+    "mpm_download_particles": (C.c_int, [C.c_void_p, C.POINTER(ParticlesView)]),
+    "mpm_download_nodes": (C.c_int, [C.c_void_p, C.POINTER(NodesView)]),
+    "mpm_bin": (C.c_int, [C.c_void_p]),
+    "mpm_p2g": (C.c_int, [C.c_void_p, C.c_double]),
+    "mpm_solve_grid": (C.c_int, [C.c_void_p]),
+    "mpm_g2p": (C.c_int, [C.c_void_p, C.c_double]),
+    "mpm_download_bins": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip]),
+    "mpm_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "mpm_sync": (C.c_int, [C.c_void_p]),
+    "mpm_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
+    "mpm_set_option": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mpm_maksf_batch": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _ip, _dp, _ip, _dp, _dp]),
+    "mpm_bin_positions": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _dp, _ip, _ip]),
+}
+
+_lib = None
+
+
+class MPMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mpm error {code}: {msg}")
+        self.code = code
+
+
+def load(path=None):
+    """Loads the engine library (no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is not built; run hpc-mpm-code_amd/build.py")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != MPM_OK:
+        raise MPMError(rc, load().mpm_last_error().decode())
+    return rc
+
+
+def _ptr(a, ct=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def grid_desc(xmin, xmax, nelem, idimc=(1, 1, 1)):
+    g = GridDesc()
+    for i in range(3):
+        g.xmin[i] = xmin[i]
+        g.xmax[i] = xmax[i]
+        g.nelem[i] = int(nelem[i])
+        g.idimc[i] = int(idimc[i])
+    return g
+
+
+class Particles:
+    """Host particle arrays in the reference AoS FP64 layout."""
+
+    FIELDS = ("coord", "vel", "mass", "vol", "vol0", "stress", "def_grad", "det_def_grad", "trac_force", "matid",
+              "eps_p")
+
+    def __init__(self, n, coord=None, vel=None, mass=None, vol0=None, vol=None, stress=None, def_grad=None,
+                 det_def_grad=None, trac_force=None, matid=None, eps_p=None):
+        self.n = int(n)
+        z = lambda *s: np.zeros(s, dtype=np.float64)
+        self.coord = np.ascontiguousarray(coord, dtype=np.float64) if coord is not None else z(n, 3)
+        self.vel = np.ascontiguousarray(vel, dtype=np.float64) if vel is not None else z(n, 3)
+        self.mass = np.ascontiguousarray(mass, dtype=np.float64) if mass is not None else z(n)
+        self.vol0 = np.ascontiguousarray(vol0, dtype=np.float64) if vol0 is not None else z(n)
+        self.vol = np.ascontiguousarray(vol, dtype=np.float64) if vol is not None else self.vol0.copy()
+        self.stress = np.ascontiguousarray(stress, dtype=np.float64) if stress is not None else z(n, 6)
+        if def_grad is None:
+            def_grad = np.tile(np.eye(3), (n, 1, 1))
+        self.def_grad = np.ascontiguousarray(def_grad, dtype=np.float64).reshape(n, 3, 3)
+        self.det_def_grad = (np.ascontiguousarray(det_def_grad, dtype=np.float64) if det_def_grad is not None
+                             else np.ones(n))
+        self.trac_force = (np.ascontiguousarray(trac_force, dtype=np.float64) if trac_force is not None else None)
+        self.matid = np.ascontiguousarray(matid, dtype=np.int32) if matid is not None else None
+        self.eps_p = np.ascontiguousarray(eps_p, dtype=np.float64) if eps_p is not None else z(n)
+
+    def copy(self):
+        p = Particles(self.n)
+        for f in self.FIELDS:
+            v = getattr(self, f)
+            setattr(p, f, None if v is None else v.copy())
+        return p
+
+    def view(self):
+        v = ParticlesView()
+        v.n = self.n
+        v.coord = _ptr(self.coord)
+        v.vel = _ptr(self.vel)
+        v.mass = _ptr(self.mass)
+        v.vol = _ptr(self.vol)
+        v.vol0 = _ptr(self.vol0)
+        v.stress = _ptr(self.stress)
+        v.def_grad = _ptr(self.def_grad)
+        v.det_def_grad = _ptr(self.det_def_grad)
+        v.trac_force = _ptr(self.trac_force)
+        v.matid = _ptr(self.matid, C.c_int32)
+        v.eps_p = _ptr(self.eps_p)
+        return v
+
+
+class Nodes:
+    def __init__(self, nodec):
+        self.nodec = int(nodec)
+        self.nmass = np.zeros(nodec)
+        self.nvof = np.zeros(nodec)
+        self.nmome = np.zeros(3 * nodec)
+        self.nforce = np.zeros(3 * nodec)
+        self.nvel = np.zeros(3 * nodec)
+        self.naccel = np.zeros(3 * nodec)
+
+    def view(self):
+        v = NodesView()
+        v.nodec = self.nodec
+        for f in ("nmass", "nvof", "nmome", "nforce", "nvel", "naccel"):
+            setattr(v, f, _ptr(getattr(self, f)))
+        return v
+
+
+class Engine:
+    """Thin owner of an mpm_engine handle."""
+
+    def __init__(self, scene, precision=PREC_FP64_EXACT, relayout_every=0):
+        lib = load()
+        self.lib = lib
+        self.scene = scene
+        g = grid_desc(scene.xmin, scene.xmax, scene.nelem)
+        self.nodec = int(np.prod(np.asarray(scene.nelem) + 1))
+        self.nelem = int(np.prod(scene.nelem))
+        mats = (MaterialDesc * len(scene.materials))()
+        for i, (model, props) in enumerate(scene.materials):
+            mats[i].model = model
+            mats[i].nprop = len(props)
+            for k, v in enumerate(props):
+                mats[i].prop[k] = v
+        p = StepParams()
+        p.dt = scene.dt
+        p.mtol = scene.mtol
+        for i in range(3):
+            p.bb[i] = scene.bb[i]
+        p.scheme = scene.scheme
+        p.nlstep = scene.nlstep
+        p.precision = precision
+        p.relayout_every = relayout_every
+        h = C.c_void_p()
+        check(lib.mpm_engine_create(C.byref(g), mats, len(scene.materials), C.byref(p), C.byref(h)))
+        self.h = h
+        for comp, (nbc, fbc) in scene.bcs.items():
+            nbc = np.ascontiguousarray(nbc, dtype=np.int32)
+            fbc = np.ascontiguousarray(fbc, dtype=np.float64)
+            check(lib.mpm_set_bc(self.h, comp, _ptr(nbc, C.c_int32), _ptr(fbc), len(nbc)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mpm_engine_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def upload(self, parts):
+        check(self.lib.mpm_upload_particles(self.h, C.byref(parts.view())))
+
+    def download(self, n):
+        out = Particles(n)
+        check(self.lib.mpm_download_particles(self.h, C.byref(out.view())))
+        return out
+
+    def nodes(self):
+        nd = Nodes(self.nodec)
+        check(self.lib.mpm_download_nodes(self.h, C.byref(nd.view())))
+        return nd
+
+    def bin(self):
+        check(self.lib.mpm_bin(self.h))
+
+    def p2g(self, facl=1.0):
+        check(self.lib.mpm_p2g(self.h, facl))
+
+    def solve_grid(self):
+        check(self.lib.mpm_solve_grid(self.h))
+
+    def g2p(self, facl=1.0):
+        check(self.lib.mpm_g2p(self.h, facl))
+
+    def step(self, istep0, nsteps):
+        return self.lib.mpm_step(self.h, istep0, nsteps)
+
+    def bins(self, n):
+        idepf = np.zeros(self.nelem, np.int32)
+        idp2p = np.zeros(n, np.int32)
+        numep = np.zeros(self.nelem, np.int32)
+        cell = np.zeros(n, np.int32)
+        check(self.lib.mpm_download_bins(self.h, _ptr(idepf, C.c_int32), _ptr(idp2p, C.c_int32),
+                                         _ptr(numep, C.c_int32), _ptr(cell, C.c_int32)))
+        return idepf, idp2p, numep, cell
+
+    def set_timing(self, on=True):
+        check(self.lib.mpm_set_timing(self.h, 1 if on else 0))
+
+    def set_option(self, key, value):
+        check(self.lib.mpm_set_option(self.h, key, value))
+
+    def kernel_times(self):
+        kt = KernelTimes()
+        check(self.lib.mpm_get_kernel_times(self.h, C.byref(kt)))
+        return {f: getattr(kt, f) for f, _ in KernelTimes._fields_}
+
+
+def maksf_batch(grid, cells, xyg):
+    lib = load()
+    cells = np.ascontiguousarray(cells, dtype=np.int32)
+    xyg = np.ascontiguousarray(xyg, dtype=np.float64)
+    n = len(cells)
+    nen = int(np.prod(np.asarray(list(grid.idimc)) + 1))
+    nc = np.zeros((n, nen), np.int32)
+    sf = np.zeros((n, nen))
+    dsf = np.zeros((n, nen, 3))
+    check(lib.mpm_maksf_batch(C.byref(grid), n, _ptr(cells, C.c_int32), _ptr(xyg), _ptr(nc, C.c_int32), _ptr(sf),
+                              _ptr(dsf)))
+    return nc, sf, dsf
+
+
+def bin_positions(grid, xyg):
+    lib = load()
+    xyg = np.ascontiguousarray(xyg, dtype=np.float64)
+    n = len(xyg)
+    cell = np.zeros(n, np.int32)
+    order = np.zeros(n, np.int32)
+    check(lib.mpm_bin_positions(C.byref(grid), n, _ptr(xyg), _ptr(cell, C.c_int32), _ptr(order, C.c_int32)))
+    return cell, order

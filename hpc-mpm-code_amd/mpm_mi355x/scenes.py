@@ -1,0 +1,191 @@
+"""Synthetic scenes for the explicit solid MPM step.
+
+Particles are seeded like the reference's generators
+(data/generate/solid/solid_generator.cpp:92-188, module/material_point.cpp:16-50):
+cells in (k, j, i) order, then sub-cell points in (kp, jp, ip) order at the
+offsets of GaussianDistribution (module/mesh.cpp:12-54), each with mass
+rho * dx*dy*dz / npe. Node-list BCs are built in node-id order like the
+generator's lists. Random velocity noise comes from a SplitMix64 stream
+(Steele, Lea & Flood 2014) with a fixed seed per scene.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+
+
+def splitmix64(seed, count):
+    """count uint64 draws of SplitMix64 starting from state `seed`."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        state = (np.uint64(seed) + np.arange(1, count + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)) & M
+        z = state
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, count, lo, hi):
+    u = (splitmix64(seed, count) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def gauss_offsets(npx, dx):
+    """GaussianDistribution (module/mesh.cpp:12-54) for one axis."""
+    table = {
+        1: [0.0],
+        2: [-0.25, 0.25],
+        3: [-1.0 / 3.0, 0.0, 1.0 / 3.0],
+        4: [-0.375, -0.125, 0.125, 0.375],
+        5: [-0.4, -0.2, 0.0, 0.2, 0.4],
+        6: [-5.0 / 12.0, -3.0 / 12.0, -1.0 / 12.0, 1.0 / 12.0, 3.0 / 12.0, 5.0 / 12.0],
+    }
+    return [c * dx for c in table[npx]]
+
+
+@dataclass
+class Scene:
+    name: str
+    xmin: tuple
+    xmax: tuple
+    nelem: tuple
+    materials: list           # [(model, [props])]
+    rho: float
+    dt: float
+    steps: int
+    bb: tuple
+    mtol: float = 1e-12
+    scheme: int = capi.SCHEME_PIC
+    nlstep: int = 1
+    bcs: dict = field(default_factory=dict)   # comp -> (nbc, fbc)
+    parts: object = None
+
+    @property
+    def dx(self):
+        return tuple((self.xmax[i] - self.xmin[i]) / self.nelem[i] for i in range(3))
+
+    @property
+    def nodec(self):
+        return int(np.prod(np.asarray(self.nelem) + 1))
+
+    @property
+    def nelem_total(self):
+        return int(np.prod(self.nelem))
+
+    @property
+    def n(self):
+        return self.parts.n
+
+
+def seed_block(xmin, dx, nelem, lo_cells, hi_cells, npxye):
+    """Particle coordinates of the cells [lo, hi) per axis, reference order."""
+    ox = gauss_offsets(npxye[0], dx[0])
+    oy = gauss_offsets(npxye[1], dx[1])
+    oz = gauss_offsets(npxye[2], dx[2])
+    ks = np.arange(lo_cells[2], hi_cells[2])
+    js = np.arange(lo_cells[1], hi_cells[1])
+    is_ = np.arange(lo_cells[0], hi_cells[0])
+    K, J, I = np.meshgrid(ks, js, is_, indexing="ij")
+    ecx = dx[0] * (I.ravel() + 0.5) + xmin[0]
+    ecy = dx[1] * (J.ravel() + 0.5) + xmin[1]
+    ecz = dx[2] * (K.ravel() + 0.5) + xmin[2]
+    pts = []
+    for kp in range(npxye[2]):
+        for jp in range(npxye[1]):
+            for ip in range(npxye[0]):
+                pts.append((ox[ip], oy[jp], oz[kp]))
+    pts = np.asarray(pts)  # (npe, 3)
+    x = ecx[:, None] + pts[None, :, 0]
+    y = ecy[:, None] + pts[None, :, 1]
+    z = ecz[:, None] + pts[None, :, 2]
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+
+
+def node_lists(nelem, select):
+    """Node ids (reference numbering) in id order where select(i, j, k) holds."""
+    nx, ny, nz = nelem[0] + 1, nelem[1] + 1, nelem[2] + 1
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ids = (I + nx * J + nx * ny * K).ravel()
+    mask = select(I.ravel(), J.ravel(), K.ravel(), nx, ny, nz)
+    return ids[mask].astype(np.int32)
+
+
+def _bc(ids):
+    return (ids, np.zeros(len(ids)))
+
+
+def cfg1(steps=1000, seed=0x5EED0001):
+    """2D elastic block (reference slab form: one cell through the thickness,
+    vertical = z): 64x64 cells on the unit square, block [0.25,0.75]x[0,0.5]
+    at 2x2 points per cell = 4,096 particles, linear elastic E=1e6, nu=0.3,
+    rho=1000, dt=1e-5. Bottom nodes fixed, side nodes roller, v = 0
+    everywhere (plane strain)."""
+    h = 1.0 / 64.0
+    xmin, xmax, nelem = (0.0, 0.0, 0.0), (1.0, h, 1.0), (64, 1, 64)
+    dx = (h, h, h)
+    coord = seed_block(xmin, dx, nelem, (16, 0, 0), (48, 1, 32), (2, 1, 2))
+    n = len(coord)
+    rho = 1000.0
+    volp = dx[0] * dx[1] * dx[2] / 4.0
+    vel = uniform(seed, 3 * n, -1e-3, 1e-3).reshape(n, 3)
+    vel[:, 1] = 0.0
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    bottom = node_lists(nelem, lambda i, j, k, nx, ny, nz: k == 0)
+    sides = node_lists(nelem, lambda i, j, k, nx, ny, nz: (i == 0) | (i == nx - 1) | (k == 0))
+    allv = node_lists(nelem, lambda i, j, k, nx, ny, nz: np.ones_like(i, dtype=bool))
+    return Scene("cfg1-2d-linelast", xmin, xmax, nelem, [(capi.MODEL_LINELAST, [1e6, 0.3])], rho, 1e-5, steps,
+                 (0.0, 0.0, -9.81), bcs={0: _bc(sides), 1: _bc(allv), 2: _bc(bottom)}, parts=parts)
+
+
+def cfg2(steps=1000, ppc=(2, 2, 2), n_cells=128, seed=0x5EED0002):
+    """3D Neo-Hookean cube: 128^3 cells on the unit cube, cube
+    [0.25,0.75]x[0,0.5]x[0.25,0.75]; ppc (2,2,2) gives 2,097,152 particles
+    (geometry as stated), (2,4,2) gives 4,194,304. E=5e5, nu=0.35, rho=1000,
+    angular velocity 2 rad/s about the y axis through (0.5, y, 0.5) plus
+    +-1e-3 m/s noise, gravity -y, floor nodes (y = 0) fixed, dt = 5e-6."""
+    h = 1.0 / n_cells
+    xmin, xmax, nelem = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (n_cells,) * 3
+    dx = (h, h, h)
+    q = n_cells // 4
+    coord = seed_block(xmin, dx, nelem, (q, 0, q), (3 * q, 2 * q, 3 * q), ppc)
+    n = len(coord)
+    rho = 1000.0
+    volp = h ** 3 / float(np.prod(ppc))
+    omega = 2.0
+    vel = np.zeros((n, 3))
+    vel[:, 0] = omega * (coord[:, 2] - 0.5)
+    vel[:, 2] = -omega * (coord[:, 0] - 0.5)
+    vel += uniform(seed, 3 * n, -1e-3, 1e-3).reshape(n, 3)
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    floor = node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0)
+    bcs = {0: _bc(floor), 1: _bc(floor), 2: _bc(floor)}
+    name = "cfg2-3d-neohookean-%dppc" % int(np.prod(ppc))
+    return Scene(name, xmin, xmax, nelem, [(capi.MODEL_NEOHOOKEAN, [5e5, 0.35])], rho, 5e-6, steps,
+                 (0.0, -9.81, 0.0), bcs=bcs, parts=parts)
+
+
+def tiny(n_cells=8, ppc=(2, 2, 2), model=capi.MODEL_NEOHOOKEAN, props=(5e5, 0.35), scheme=capi.SCHEME_PIC,
+         seed=0x5EED0009, dt=1e-5, omega=2.0, gravity=(0.0, -9.81, 0.0), nlstep=1):
+    """Small 3D scene for parity tests: the cfg2 geometry on an n_cells^3 grid."""
+    h = 1.0 / n_cells
+    xmin, xmax, nelem = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (n_cells,) * 3
+    dx = (h, h, h)
+    q = max(1, n_cells // 4)
+    coord = seed_block(xmin, dx, nelem, (q, 0, q), (3 * q, 2 * q, 3 * q), ppc)
+    n = len(coord)
+    rho = 1000.0
+    volp = h ** 3 / float(np.prod(ppc))
+    vel = np.zeros((n, 3))
+    vel[:, 0] = omega * (coord[:, 2] - 0.5)
+    vel[:, 2] = -omega * (coord[:, 0] - 0.5)
+    vel += uniform(seed, 3 * n, -1e-3, 1e-3).reshape(n, 3)
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    floor = node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0)
+    bcs = {0: _bc(floor), 1: _bc(floor), 2: _bc(floor)}
+    return Scene("tiny", xmin, xmax, nelem, [(model, list(props))], rho, dt, 10, gravity, scheme=scheme,
+                 nlstep=nlstep, bcs=bcs, parts=parts)
+
+
+CONFIGS = {"cfg1": cfg1, "cfg2": cfg2}

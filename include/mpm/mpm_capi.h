@@ -1,0 +1,221 @@
+/*
+ * mpm_capi.h -- C ABI of the MI355X explicit-MPM step engine.
+ *
+ * Plain C types only: pointers, sizes and status codes. Host buffers are
+ * owned by the caller and only read or written during a call; the engine
+ * owns its device memory (allocated by mpm_engine_create and
+ * mpm_upload_particles, freed by mpm_engine_destroy).
+ *
+ * Reference layouts honoured at the boundary (reference = HPC-MPM-Code):
+ *   particle arrays  AoS FP64 as MaterialPoint keeps them
+ *                    (module/material_point.h:113-121): coord[n][3],
+ *                    vel[n][3], stress[n][6] in Voigt order xx,yy,zz,yz,xz,xy
+ *                    (module/dataset.h:15), def_grad[n][3][3] row-major
+ *   node arrays      component blocks [u | v | w] of length nodec
+ *                    (nuc = 0, nvc = nodec, nwc = 2*nodec; module/data_io.cpp:53-56)
+ *   node id          i + nxc*j + nxc*nyc*k (module/shape_function.cpp:34)
+ *   cell id          ix + iy*nex + iz*nex*ney (module/material_point.cpp:65)
+ *
+ * Every entry point returns MPM_OK (0) or a negative mpm_status. No C++
+ * exception crosses this boundary. Calls on one engine must be serialised
+ * by the caller; distinct engines are independent.
+ */
+#ifndef MPM_CAPI_H
+#define MPM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mpm_status {
+    MPM_OK = 0,
+    MPM_ERR_INVALID_ARGUMENT = -1,
+    MPM_ERR_OUT_OF_GRID = -2,   /* a particle left the background grid */
+    MPM_ERR_NONFINITE = -3,     /* NaN/Inf in particle state */
+    MPM_ERR_JACOBIAN = -4,      /* det F <= 0 */
+    MPM_ERR_HIP = -5,           /* a HIP runtime call failed */
+    MPM_ERR_UNSUPPORTED = -6,   /* option not implemented for this precision/shape */
+    MPM_ERR_NO_DEVICE = -7
+} mpm_status;
+
+/* Transfer scheme; same meaning as MapScheme (module/dataset.h:19). */
+enum { MPM_SCHEME_PIC = 0, MPM_SCHEME_FLIP = 1 };
+
+/* Constitutive models; codes -1..3 are the reference's iprop values
+ * (module/solid/constitutive_model.cpp:287-305). MPM_MODEL_J2 has no
+ * reference counterpart (J2 plasticity with linear hardening,
+ * prop = {E, nu, sigma_y, H}). */
+enum {
+    MPM_MODEL_RIGID = -1,
+    MPM_MODEL_LINELAST = 0,
+    MPM_MODEL_NEOHOOKEAN = 1,
+    MPM_MODEL_STVK = 2,
+    MPM_MODEL_MOONEYRIVLIN = 3,
+    MPM_MODEL_J2 = 10
+};
+
+/* Arithmetic of the step kernels.
+ * MPM_PREC_FP64_EXACT: FP64 throughout, in the reference's summation order
+ *   (cells ascending, particles in index order, stencil nodes k,j,i); with
+ *   libm-free models (linear elastic) the result equals the reference's
+ *   arithmetic bit for bit.
+ * MPM_PREC_FP32: FP32 particle/grid arithmetic with compensated (Kahan) node
+ *   sums; det F and the particle volume in FP64, FP64 division at
+ *   near-empty nodes. Positions carry a compensation term. */
+enum { MPM_PREC_FP64_EXACT = 0, MPM_PREC_FP32 = 1 };
+
+typedef struct mpm_grid_desc {
+    double xmin[3];    /* xyminw */
+    double xmax[3];    /* xymaxw */
+    int32_t nelem[3];  /* xyelemw: cells per axis */
+    int32_t idimc[3];  /* shape-function order per axis (1 = linear tent, 2 = quadratic B-spline) */
+} mpm_grid_desc;
+
+typedef struct mpm_material_desc {
+    int32_t model;     /* MPM_MODEL_* (reference iprop) */
+    int32_t nprop;
+    double prop[8];    /* mat_prop: E, nu (LinElast/NeoHookean/StVK); c1, c2, bulk (Mooney-Rivlin) */
+} mpm_material_desc;
+
+typedef struct mpm_step_params {
+    double dt;
+    double mtol;       /* node mass threshold (input.txt) */
+    double bb[3];      /* body force per unit mass (gravity) */
+    int32_t scheme;    /* MPM_SCHEME_PIC / MPM_SCHEME_FLIP */
+    int32_t nlstep;    /* load ramp: facl = istep/nlstep for istep <= nlstep, else 1 */
+    int32_t precision; /* MPM_PREC_* */
+    int32_t relayout_every; /* FP32: physically reorder particles every k steps (0 = never) */
+} mpm_step_params;
+
+/* Host view of particle arrays in the reference layout. Optional pointers
+ * may be NULL: vel (zero), stress (zero), def_grad (identity), det_def_grad
+ * (one), vol (vol0), trac_force (zero), matid (all 0), eps_p (zero). */
+typedef struct mpm_particles_view {
+    int64_t n;
+    double *coord;        /* [n][3] */
+    double *vel;          /* [n][3] */
+    double *mass;         /* [n]    */
+    double *vol;          /* [n]    */
+    double *vol0;         /* [n]    */
+    double *stress;       /* [n][6] */
+    double *def_grad;     /* [n][9] */
+    double *det_def_grad; /* [n]    */
+    double *trac_force;   /* [n][3] */
+    int32_t *matid;       /* [n]    */
+    double *eps_p;        /* [n] equivalent plastic strain (J2) */
+} mpm_particles_view;
+
+/* Host view of node arrays, [u|v|w] blocks of length nodec. Any pointer may
+ * be NULL (not downloaded). */
+typedef struct mpm_nodes_view {
+    int64_t nodec;
+    double *nmass;   /* [nodec]   */
+    double *nvof;    /* [nodec]   */
+    double *nmome;   /* [3*nodec] */
+    double *nforce;  /* [3*nodec] */
+    double *nvel;    /* [3*nodec] */
+    double *naccel;  /* [3*nodec] */
+} mpm_nodes_view;
+
+/* Per-kernel timing of the last mpm_step call (HIP events recorded on the
+ * engine's stream between the kernels), milliseconds summed over the steps
+ * of that call. In FP64-exact mode only bin_ms, p2g_ms and g2p_upd_ms (the
+ * whole G2P pass) are filled. */
+typedef struct mpm_kernel_times {
+    double bin_ms;        /* cell keys + radix sort + cell ranges (+ relayout) */
+    double p2g_ms;        /* p2g_cells_ms + p2g_nodes_ms */
+    double p2g_cells_ms;  /* per-cell corner partials */
+    double p2g_nodes_ms;  /* node sums + grid update */
+    double g2p_vel_ms;    /* musl_cells_ms + musl_nodes_ms */
+    double musl_cells_ms; /* G2P velocity + MUSL momentum partials */
+    double musl_nodes_ms; /* MUSL node sums + velocity + BCs */
+    double g2p_upd_ms;    /* dF, F, J, x, V, stress update */
+    double total_ms;
+    int32_t steps;
+    int32_t active_cells; /* cells holding particles at the last bin */
+    int32_t active_nodes; /* nodes of those cells */
+    int32_t pad;
+} mpm_kernel_times;
+
+typedef struct mpm_engine mpm_engine;
+
+/* Version string of the library ("mpm-mi355x <semver> gfx950"). */
+const char *mpm_version(void);
+/* Human-readable text of the last error on this thread. */
+const char *mpm_last_error(void);
+
+/* Creates an engine on the current HIP device. Replaces the global set-up
+ * of datain_para + BuildMesh + BuildControlPoint
+ * (work/src_solid/explicit/datain_para.cpp:18-162, module/mesh.cpp:56-158). */
+int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, int32_t nmat,
+                      const mpm_step_params *params, mpm_engine **out);
+int mpm_engine_destroy(mpm_engine *eng);
+
+/* Node-list Dirichlet BC; comp 0/1/2 = usbc/vsbc/wsbc, 3 = rigid_bc (all
+ * three components). Semantics of BoundaryCondition::BCSetVal/BCSetZero
+ * (module/bc.cpp:45-67): velocity set to fbc*facl, acceleration zeroed.
+ * Lists are applied in the order u, v, w, rigid; a later entry wins. */
+int mpm_set_bc(mpm_engine *eng, int32_t comp, const int32_t *nbc, const double *fbc, int32_t ibc);
+
+/* Copies particles host -> device (replaces InputPointData + the initial
+ * state set-up of datain_para.cpp:104-126). */
+int mpm_upload_particles(mpm_engine *eng, const mpm_particles_view *p);
+int mpm_download_particles(mpm_engine *eng, mpm_particles_view *p);
+int mpm_download_nodes(mpm_engine *eng, mpm_nodes_view *nodes);
+
+/* Per-pass entry points (one call = one reference pass on the device):
+ *   mpm_bin          MaterialPoint::MeshPointLinklist (material_point.cpp:52-77)
+ *   mpm_p2g          Particle2NodeS (var_trans_solid_explicit.cpp:20-99)
+ *   mpm_solve_grid   SolveSolid (solve_solid_explicit.cpp:8-31)
+ *   mpm_g2p          Node2ParticleS incl. MUSL and the coordinate/stress
+ *                    update (var_trans_solid_explicit.cpp:101-311)
+ * facl is the load factor of the current step (main.cpp:43-48). */
+int mpm_bin(mpm_engine *eng);
+int mpm_p2g(mpm_engine *eng, double facl);
+int mpm_solve_grid(mpm_engine *eng);
+int mpm_g2p(mpm_engine *eng, double facl);
+
+/* Downloads the binning in the reference's linked-list form:
+ * idepf[nelem] (first particle per cell or -1), idp2p[n] (next particle or
+ * -1), numep[nelem] (count); cell_of[n] may be NULL. */
+int mpm_download_bins(mpm_engine *eng, int32_t *idepf, int32_t *idp2p, int32_t *numep, int32_t *cell_of);
+
+/* Runs steps istep0 .. istep0+nsteps-1 of the time loop
+ * (work/src_solid/explicit/main.cpp:39-73: bin, P2G, grid, G2P) with
+ * everything resident on the device. Returns MPM_ERR_OUT_OF_GRID /
+ * MPM_ERR_NONFINITE when the device-side checks fired. */
+int mpm_step(mpm_engine *eng, int32_t istep0, int32_t nsteps);
+
+/* Waits for the engine's stream. */
+int mpm_sync(mpm_engine *eng);
+
+int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
+
+/* Engine options. MPM_OPT_FULL_NODE_OUTPUT (default 1): the last step of
+ * each mpm_step call also writes node momentum, force and volume, so that
+ * mpm_download_nodes returns every field; 0 skips that work in every step
+ * (FP32 precision only; node arrays are then valid at nodes next to
+ * particles). */
+enum { MPM_OPT_FULL_NODE_OUTPUT = 1 };
+int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
+/* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
+int mpm_set_timing(mpm_engine *eng, int32_t enabled);
+
+/* MakSf on the device for a batch (module/shape_function.cpp:8-119):
+ * cell[n], xyg[n][3] -> nc[n][nen], sf[n][nen], dsf[n][nen][3], FP64,
+ * with nen = (idimc0+1)(idimc1+1)(idimc2+1). Host buffers in and out. */
+int mpm_maksf_batch(const mpm_grid_desc *grid, int64_t n, const int32_t *cell, const double *xyg, int32_t *nc,
+                    double *sf, double *dsf);
+
+/* Stable cell-key sort on the device (the binning kernel alone):
+ * xyg[n][3] host FP64 -> cell_of[n] (-1 outside the grid), order[n] =
+ * particle indices sorted by (cell, index). */
+int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, int32_t *cell_of, int32_t *order);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM_CAPI_H */

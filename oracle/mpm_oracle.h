@@ -1,0 +1,145 @@
+/*
+ * FP64 CPU oracle for the explicit solid MPM time step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a scalar restatement of the reference
+ * explicit update-stress-last / MUSL step, written from the reference's
+ * semantics. It exists to check the HIP engine; only tests, the smoke check
+ * and the CPU-baseline leg of the benchmark may load it. The product path
+ * never calls it.
+ *
+ * Layouts follow the reference (module/dataset.h, module/mesh.h,
+ * module/data_io.cpp:53-56):
+ *   particles  AoS FP64: coord[n][3], vel[n][3], stress[n][6] (Voigt
+ *              xx,yy,zz,yz,xz,xy per module/dataset.h:15), def_grad[n][3][3]
+ *   nodes      component blocks [u | v | w] of length nodec (nuc=0,
+ *              nvc=nodec, nwc=2*nodec)
+ *   node id    i + nxc*j + nxc*nyc*k   (module/shape_function.cpp:34)
+ *   cell id    ix + iy*nex + iz*nex*ney (module/material_point.cpp:65)
+ *
+ * Parity status per function (how each one is pinned) is listed in the
+ * comment above its definition in mpm_oracle.c.
+ */
+#ifndef MPM_ORACLE_H
+#define MPM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXMAT 8
+#define ORC_MAXPROP 8
+
+/* Transfer schemes, same meaning as MapScheme in module/dataset.h:19. */
+enum { ORC_PIC = 0, ORC_FLIP = 1 };
+
+/* Constitutive model codes, same as iprop in module/solid/constitutive_model.cpp:287-305,
+ * plus J2 plasticity (no reference counterpart). */
+enum {
+    ORC_RIGID = -1,
+    ORC_LINELAST = 0,
+    ORC_NEOHOOKEAN = 1,
+    ORC_STVK = 2,
+    ORC_MOONEYRIVLIN = 3,
+    ORC_J2 = 10
+};
+
+typedef struct orc_state {
+    /* grid: single rank, so xymin == xyminw */
+    double xmin[3];
+    double xmax[3];
+    double dx[3];
+    int32_t nex[3];   /* cells per axis (xyelem) */
+    int32_t idimc[3]; /* 1 = linear tent, 2 = quadratic B-spline */
+    int32_t nxc[3];   /* control points per axis (xynodec = nex + idimc) */
+    int32_t nelem;
+    int32_t nodec;
+
+    /* step parameters */
+    double dt;
+    double mtol;
+    double bb[3];
+    int32_t scheme;   /* ORC_PIC / ORC_FLIP */
+    int32_t nlstep;
+    int32_t shifting; /* 0: particle shifting off (declared default) */
+    int32_t pad0;
+
+    /* materials */
+    int32_t nmat;
+    int32_t iprop[ORC_MAXMAT];
+    double mat_prop[ORC_MAXMAT][ORC_MAXPROP];
+
+    /* boundary-condition lists: 0=u, 1=v, 2=w, 3=rigid (all comps) */
+    int32_t ibc[4];
+    int32_t *nbc[4];
+    double *fbc[4];
+
+    /* particles */
+    int32_t num;
+    double *coord;
+    double *vel;
+    double *mass;
+    double *vol;
+    double *vol0;
+    double *stress;
+    double *def_grad;
+    double *det_def_grad;
+    double *trac_force; /* may be NULL (zero traction) */
+    int32_t *matid;     /* may be NULL (all material 0) */
+    double *eps_p;      /* J2 equivalent plastic strain, may be NULL */
+
+    /* binning (module/material_point.cpp:52-77) */
+    int32_t *idepf;
+    int32_t *idp2p;
+    int32_t *numep;
+
+    /* nodes, [u|v|w] blocks */
+    double *nmass;
+    double *nvof;
+    double *nmome;
+    double *nforce;
+    double *nvel;
+    double *naccel;
+    double *nvol; /* nodal volume for shifting, may be NULL when shifting=0 */
+
+    /* scratch for the deformation-gradient increment, [num][9] */
+    double *ddg;
+} orc_state;
+
+/* grid set-up helper: dx = (xmax - xmin) / nex as in datain_para.cpp:87 */
+void orc_grid_init(orc_state *st);
+
+int orc_maksf(const orc_state *st, int m, const double xyg[3], int32_t *nc, double *sf, double *dsf);
+int orc_maksf_batch(const orc_state *st, int n, const int32_t *cell, const double *xyg, int32_t *nc, double *sf,
+                    double *dsf);
+
+/* cell of a position, FP64 floor formula of material_point.cpp:59-65 */
+int orc_cell_of(const orc_state *st, const double x[3]);
+
+int orc_bin(orc_state *st);
+void orc_p2g(orc_state *st, double facl);
+void orc_solve_solid(orc_state *st);
+void orc_g2p(orc_state *st, double facl);
+double orc_facl(const orc_state *st, int istep);
+int orc_step(orc_state *st, int istep);
+int orc_run(orc_state *st, int ista, int iend);
+
+void orc_bc_set_val(const orc_state *st, int comp_list, int nn, double *var, double facl);
+void orc_bc_set_zero(const orc_state *st, int comp_list, int nn, double *var);
+
+/* constitutive update, semantics of ConstitutiveModel::UpdateStress */
+void orc_update_stress(int model, double stress[6], const double F[9], const double dF[9], const double *prop,
+                       double jac, double jac_bar);
+/* J2 radial return (Simo & Hughes 1998, Box 3.2), small-strain increment form */
+int orc_j2_return(const double prop[8], double stress[6], double *eps_p, const double deps[6], double *dgamma,
+                  double *resid);
+
+double orc_det3(const double m[9]);
+void orc_inv3(const double m[9], double out[9]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

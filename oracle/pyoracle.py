@@ -1,0 +1,177 @@
+"""ctypes wrapper of the FP64 CPU oracle (mpm_oracle.c) -- TEST INFRASTRUCTURE.
+
+Only tests, the smoke check and the CPU-baseline leg of the benchmark use
+this module, as the checker. It also wraps oracle/_ref/libmpm_ref.so, the
+harness around the reference's own constitutive_model.cpp, bc.cpp and
+cal_mat.h / map_and_interpolate.h templates, when that library was built.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libmpm_oracle.so")
+LIB_FAST = os.path.join(HERE, "_build", "libmpm_oracle_fast.so")
+REF_LIB = os.path.join(HERE, "_ref", "libmpm_ref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+PIC, FLIP = 0, 1
+
+
+class OrcState(C.Structure):
+    _fields_ = [
+        ("xmin", C.c_double * 3), ("xmax", C.c_double * 3), ("dx", C.c_double * 3),
+        ("nex", C.c_int32 * 3), ("idimc", C.c_int32 * 3), ("nxc", C.c_int32 * 3),
+        ("nelem", C.c_int32), ("nodec", C.c_int32),
+        ("dt", C.c_double), ("mtol", C.c_double), ("bb", C.c_double * 3),
+        ("scheme", C.c_int32), ("nlstep", C.c_int32), ("shifting", C.c_int32), ("pad0", C.c_int32),
+        ("nmat", C.c_int32), ("iprop", C.c_int32 * 8), ("mat_prop", (C.c_double * 8) * 8),
+        ("ibc", C.c_int32 * 4), ("nbc", _ip * 4), ("fbc", _dp * 4),
+        ("num", C.c_int32),
+        ("coord", _dp), ("vel", _dp), ("mass", _dp), ("vol", _dp), ("vol0", _dp), ("stress", _dp),
+        ("def_grad", _dp), ("det_def_grad", _dp), ("trac_force", _dp), ("matid", _ip), ("eps_p", _dp),
+        ("idepf", _ip), ("idp2p", _ip), ("numep", _ip),
+        ("nmass", _dp), ("nvof", _dp), ("nmome", _dp), ("nforce", _dp), ("nvel", _dp), ("naccel", _dp),
+        ("nvol", _dp), ("ddg", _dp),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "oracle", "timing"], check=True)
+
+
+def load(fast=False):
+    path = LIB_FAST if fast else LIB
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    lib.orc_grid_init.argtypes = [C.POINTER(OrcState)]
+    lib.orc_maksf.argtypes = [C.POINTER(OrcState), C.c_int, _dp, _ip, _dp, _dp]
+    lib.orc_maksf_batch.argtypes = [C.POINTER(OrcState), C.c_int, _ip, _dp, _ip, _dp, _dp]
+    lib.orc_cell_of.argtypes = [C.POINTER(OrcState), _dp]
+    for f in ("orc_bin", "orc_solve_solid"):
+        getattr(lib, f).argtypes = [C.POINTER(OrcState)]
+    lib.orc_p2g.argtypes = [C.POINTER(OrcState), C.c_double]
+    lib.orc_g2p.argtypes = [C.POINTER(OrcState), C.c_double]
+    lib.orc_step.argtypes = [C.POINTER(OrcState), C.c_int]
+    lib.orc_run.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int]
+    lib.orc_facl.argtypes = [C.POINTER(OrcState), C.c_int]
+    lib.orc_facl.restype = C.c_double
+    lib.orc_update_stress.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
+    lib.orc_j2_return.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_det3.argtypes = [_dp]
+    lib.orc_det3.restype = C.c_double
+    lib.orc_inv3.argtypes = [_dp, _dp]
+    lib.orc_bc_set_val.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, _dp, C.c_double]
+    lib.orc_bc_set_zero.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, _dp]
+    return lib
+
+
+def load_ref():
+    """The reference-numerics harness, or None when it was not built."""
+    if not os.path.exists(REF_LIB):
+        return None
+    lib = C.CDLL(REF_LIB)
+    lib.ref_update_stress.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double]
+    lib.ref_bc_apply.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_double]
+    lib.ref_det3.argtypes = [_dp]
+    lib.ref_det3.restype = C.c_double
+    lib.ref_inv3.argtypes = [_dp, _dp]
+    lib.ref_var_g2p.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, _ip, _dp, C.c_double]
+    lib.ref_trace3.argtypes = [_dp, _dp]
+    return lib
+
+
+def P(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class Oracle:
+    """Holds an orc_state and the numpy arrays it points into."""
+
+    def __init__(self, scene, fast=False):
+        self.lib = load(fast)
+        st = OrcState()
+        for i in range(3):
+            st.xmin[i] = scene.xmin[i]
+            st.xmax[i] = scene.xmax[i]
+            st.nex[i] = scene.nelem[i]
+            st.idimc[i] = 1
+        self.lib.orc_grid_init(C.byref(st))
+        st.dt = scene.dt
+        st.mtol = scene.mtol
+        for i in range(3):
+            st.bb[i] = scene.bb[i]
+        st.scheme = scene.scheme
+        st.nlstep = scene.nlstep
+        st.shifting = 0
+        st.nmat = len(scene.materials)
+        for i, (model, props) in enumerate(scene.materials):
+            st.iprop[i] = model
+            for k, v in enumerate(props):
+                st.mat_prop[i][k] = v
+        self._keep = []
+        for comp in range(4):
+            if comp in scene.bcs:
+                nbc, fbc = scene.bcs[comp]
+                nbc = np.ascontiguousarray(nbc, np.int32)
+                fbc = np.ascontiguousarray(fbc, np.float64)
+                self._keep += [nbc, fbc]
+                st.ibc[comp] = len(nbc)
+                st.nbc[comp] = P(nbc, C.c_int32)
+                st.fbc[comp] = P(fbc)
+            else:
+                st.ibc[comp] = 0
+        p = scene.parts.copy()
+        self.parts = p
+        n = p.n
+        st.num = n
+        for f in ("coord", "vel", "mass", "vol", "vol0", "stress", "def_grad", "det_def_grad", "eps_p"):
+            setattr(st, f, P(getattr(p, f)))
+        if p.trac_force is not None:
+            st.trac_force = P(p.trac_force)
+        if p.matid is not None:
+            st.matid = P(p.matid, C.c_int32)
+        nelem, nodec = st.nelem, st.nodec
+        self.idepf = np.zeros(nelem, np.int32)
+        self.idp2p = np.zeros(n, np.int32)
+        self.numep = np.zeros(nelem, np.int32)
+        st.idepf, st.idp2p, st.numep = P(self.idepf, C.c_int32), P(self.idp2p, C.c_int32), P(self.numep, C.c_int32)
+        self.nmass = np.zeros(nodec)
+        self.nvof = np.zeros(nodec)
+        self.nmome = np.zeros(3 * nodec)
+        self.nforce = np.zeros(3 * nodec)
+        self.nvel = np.zeros(3 * nodec)
+        self.naccel = np.zeros(3 * nodec)
+        self.ddg = np.zeros(9 * n)
+        for f in ("nmass", "nvof", "nmome", "nforce", "nvel", "naccel", "ddg"):
+            setattr(st, f, P(getattr(self, f)))
+        self.st = st
+        self.nodec = nodec
+        self.nelem = nelem
+
+    def bin(self):
+        return self.lib.orc_bin(C.byref(self.st))
+
+    def p2g(self, facl=1.0):
+        self.lib.orc_p2g(C.byref(self.st), facl)
+
+    def solve_grid(self):
+        self.lib.orc_solve_solid(C.byref(self.st))
+
+    def g2p(self, facl=1.0):
+        self.lib.orc_g2p(C.byref(self.st), facl)
+
+    def step(self, istep):
+        return self.lib.orc_step(C.byref(self.st), istep)
+
+    def run(self, ista, iend):
+        return self.lib.orc_run(C.byref(self.st), ista, iend)
+
+    def cell_of(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        return self.lib.orc_cell_of(C.byref(self.st), P(x))
