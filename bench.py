@@ -32,18 +32,19 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # engine's storage format; the cell-partial buffer between a cells kernel
 # and its node pass is not counted).
 BYTES = {
-    # perm 4 + key 4 + x (hi,lo) 24 + v 12 + m 4 + V(f64) 8 + stress 24
-    "p2g_cells": (80, 0),
+    # perm 4 + x (hi,lo) 24 + v 12 + m 4 + V(f64) 8 + stress 24
+    "p2g_cells": (76, 0, 0),
     # node m 4 + v 12 + a 12 written
-    "p2g_nodes": (0, 28),
-    # perm 4 + key 4 + x 24 + m 4 read, v 12 written; node v, a read 24
-    "musl_cells": (48, 24),
+    "p2g_nodes": (0, 28, 0),
+    # perm 4 + x 24 + m 4 read, v 12 written; node v, a read 24
+    "musl_cells": (44, 24, 0),
     # node m 4 read, MUSL v 12 written
-    "musl_nodes": (0, 16),
+    "musl_nodes": (0, 16, 0),
     # key 4 + x 24 + H 36 + V0 4 read; x 24 + H 36 + stress 24 + J 8 + V 8 + key 4 written; node v 12
-    "update": (172, 12),
-    # radix sort (3 passes of 8 bits), cell ranges
-    "bin": (60, 0),
+    "update": (172, 12, 0),
+    # counting sort: keys read twice, perm written (12 per particle); per cell: count read twice,
+    # start and end written (16)
+    "bin": (12, 0, 16),
 }
 
 
@@ -115,8 +116,8 @@ def main():
     }
     kernels = {}
     for k, ms in per.items():
-        bp, bn = BYTES[k]
-        byts = bp * n + bn * nact
+        bp, bn, bc = BYTES[k]
+        byts = bp * n + bn * nact + bc * scene.nelem_total
         avg_ms = ms / a.steps
         gbs = byts / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
         kernels[k] = {"avg_us": round(avg_ms * 1e3, 2), "alg_bytes": byts, "GBps": round(gbs, 1),

@@ -20,6 +20,7 @@ COMMON = [
 ]
 UNITS = [
     ("sort.hip", []),
+    ("bin.hip", []),
     ("exact64.hip", ["-ffp-contract=off"]),
     ("fast32.hip", []),
     ("engine.cpp", ["-x", "hip"]),

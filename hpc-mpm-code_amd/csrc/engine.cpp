@@ -76,10 +76,11 @@ struct mpm_engine {
     float m_small = 0.f;
 
     uint32_t *keys64 = nullptr;  // FP64 mode cell keys by particle
-    SortBuffers sb{};
-    uint32_t *perm = nullptr;    // points into sb (or perm_id after relayout)
+    BinBuffers bb{};             // counting-sort binning
+    TileGeom tg{};
+    uint32_t stamp = 0;
+    uint32_t *perm = nullptr;    // bb.perm (or perm_id right after a relayout)
     uint32_t *perm_id = nullptr;
-    uint32_t *cstart = nullptr, *cend = nullptr;
     uint32_t *d_err = nullptr;
     bool bins_valid = false;
     bool keys_valid = false;
@@ -114,7 +115,7 @@ struct mpm_engine {
         P32[1] = PartF32{};
         pid[0] = pid[1] = nullptr;
         keys64 = nullptr;
-        sb = SortBuffers{};
+        bb.perm = nullptr;
         perm = nullptr;
         perm_id = nullptr;
         n = 0;
@@ -129,7 +130,7 @@ struct mpm_engine {
     }
 
     BcDense bc() const { return BcDense{d_bcmask, d_bcval}; }
-    CellLists lists() const { return CellLists{perm, cstart, cend}; }
+    CellLists lists() const { return CellLists{perm, bb.cstart, bb.cend, bb.active, bb.ctr, bb.tiles}; }
 };
 
 namespace {
@@ -185,6 +186,7 @@ int check_err(mpm_engine *e) {
     if (h & kErrOutOfGrid) return set_error(MPM_ERR_OUT_OF_GRID, "a particle is outside the background grid");
     if (h & kErrNonFinite) return set_error(MPM_ERR_NONFINITE, "non-finite particle state");
     if (h & kErrJacobian) return set_error(MPM_ERR_JACOBIAN, "det F <= 0");
+    if (h & kErrCellOverflow) return set_error(MPM_ERR_UNSUPPORTED, "more than 4096 particles in one cell");
     return 0;
 }
 
@@ -206,14 +208,9 @@ int do_bin(mpm_engine *e) {
         e->keys_valid = true;
     }
     const uint32_t *keys = (e->prec == MPM_PREC_FP64_EXACT) ? e->keys64 : e->P32[e->cur].key;
-    int nbits = bit_length((uint32_t)e->g.nelem);
-    uint32_t *sk = nullptr, *sv = nullptr;
-    int rc = radix_sort_pairs(e->stream, e->sb, keys, n, nbits, &sk, &sv);
-    if (rc) return set_error(MPM_ERR_HIP, "radix sort launch failed");
-    launch_fill_u32(e->stream, e->cstart, (size_t)e->g.nelem, 0u);
-    launch_fill_u32(e->stream, e->cend, (size_t)e->g.nelem, 0u);
-    launch_cell_ranges(e->stream, sk, n, e->g.nelem, e->cstart, e->cend);
-    e->perm = sv;
+    int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, ++e->stamp, e->d_err);
+    if (rc) return set_error(MPM_ERR_HIP, "binning launch failed");
+    e->perm = e->bb.perm;
     if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && ++e->steps_since_relayout >= e->relayout_every) {
         // physically reorder particle slots into the sorted order
         const int o = 1 - e->cur;
@@ -233,7 +230,7 @@ int do_p2g(mpm_engine *e, double facl, int diag, hipEvent_t mid = nullptr) {
         launch_f64_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.bb[0] * facl,
                        e->prm.bb[1] * facl, e->prm.bb[2] * facl, e->prm.mtol, facl);
     } else {
-        launch_f32_p2g(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, consts(e, facl),
+        launch_f32_p2g(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, consts(e, facl),
                        e->partials, diag, mid);
     }
     e->last_pass = 1;
@@ -255,8 +252,8 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         e->keys_valid = false;
     } else {
         StepConsts c = consts(e, facl);
-        launch_f32_g2p_vel_musl(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, c, e->partials,
-                                diag, mid2);
+        launch_f32_g2p_vel_musl(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, c,
+                                e->partials, diag, mid2);
         if (mid) MPM_HIP_CHECK(hipEventRecord(mid, e->stream));
         launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err);
         e->keys_valid = true;  // the update kernel writes the new keys
@@ -313,11 +310,7 @@ int alloc_particles(mpm_engine *e, int n) {
         e->cur = 0;
         if ((rc = e->dalloc(&e->perm_id, ld, true))) return rc;
     }
-    SortBuffers &sb = e->sb;
-    if ((rc = e->dalloc(&sb.keys, ld, true)) || (rc = e->dalloc(&sb.vals, ld, true)) ||
-        (rc = e->dalloc(&sb.keys_alt, ld, true)) || (rc = e->dalloc(&sb.vals_alt, ld, true)) ||
-        (rc = e->dalloc(&sb.hist, radix_hist_entries(n), true)) || (rc = e->dalloc(&sb.bsum, 4096, true)))
-        return rc;
+    if ((rc = e->dalloc(&e->bb.perm, ld, true))) return rc;
     return 0;
 }
 
@@ -412,10 +405,31 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     }
     const GridGeom &g = e->g;
     const size_t nodec = (size_t)g.nodec, nelem = (size_t)g.nelem;
+    e->tg = make_tile_geom(e->g);
+    BinBuffers &b = e->bb;
+    const size_t ncap = bin_cell_capacity(e->g.nelem);
     if ((rc = e->dalloc(&e->d_bcmask, nodec)) || (rc = e->dalloc(&e->d_bcval, 3 * nodec)) ||
-        (rc = e->dalloc(&e->cstart, nelem)) || (rc = e->dalloc(&e->cend, nelem)) || (rc = e->dalloc(&e->d_err, 1))) {
+        (rc = e->dalloc(&b.cnt, ncap)) || (rc = e->dalloc(&b.cstart, ncap)) || (rc = e->dalloc(&b.cend, ncap)) ||
+        (rc = e->dalloc(&b.active, nelem)) || (rc = e->dalloc(&b.stamp, nelem)) ||
+        (rc = e->dalloc(&b.fixlist, nelem)) || (rc = e->dalloc(&b.tiles, (size_t)e->tg.ntiles)) ||
+        (rc = e->dalloc(&b.bsum, bin_scan_blocks(e->g.nelem))) || (rc = e->dalloc(&b.ctr, 1)) ||
+        (rc = e->dalloc(&e->d_err, 1))) {
         delete e;
         return rc;
+    }
+    if (bin_scan_blocks(e->g.nelem) > 4096) {
+        delete e;
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "grid too large for the binning scan (> 16.7M cells)");
+    }
+    he = hipMemset(b.cnt, 0, ncap * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(b.stamp, 0, nelem * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(b.cstart, 0, ncap * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(b.cend, 0, ncap * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(b.ctr, 0, sizeof(BinCounters));
+    if (he != hipSuccess) {
+        int r = hip_fail(he, "hipMemset", __FILE__, __LINE__);
+        delete e;
+        return r;
     }
     if (e->prec == MPM_PREC_FP64_EXACT) {
         NodesF64 &N = e->N64;
@@ -723,7 +737,8 @@ int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *nu
     const int n = e->n, nelem = e->g.nelem;
     std::vector<uint32_t> perm, cs, ce, keys;
     std::vector<int> pidv;
-    if ((rc = d2h(e, perm, e->perm, n)) || (rc = d2h(e, cs, e->cstart, nelem)) || (rc = d2h(e, ce, e->cend, nelem)))
+    if ((rc = d2h(e, perm, e->perm, n)) || (rc = d2h(e, cs, e->bb.cstart, nelem)) ||
+        (rc = d2h(e, ce, e->bb.cend, nelem)))
         return rc;
     const uint32_t *kd = (e->prec == MPM_PREC_FP64_EXACT) ? e->keys64 : e->P32[e->cur].key;
     if ((rc = d2h(e, keys, kd, n))) return rc;
@@ -817,15 +832,16 @@ int mpm_get_kernel_times(mpm_engine *e, mpm_kernel_times *out) {
     *out = e->kt;
     if (e->bins_valid || e->n > 0) {
         // count of non-empty cells of the last binning
-        std::vector<uint32_t> ce;
-        int rc = d2h(e, ce, e->cend, (size_t)e->g.nelem);
+        std::vector<uint32_t> ce, cs;
+        int rc = d2h(e, ce, e->bb.cend, (size_t)e->g.nelem);
+        if (!rc) rc = d2h(e, cs, e->bb.cstart, (size_t)e->g.nelem);
         if (rc) return rc;
         MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
         const GridGeom &g = e->g;
         std::vector<uint8_t> act((size_t)g.nodec, 0);
         int cnt = 0;
         for (int c = 0; c < g.nelem; c++) {
-            if (!ce[c]) continue;
+            if (ce[c] <= cs[c]) continue;
             cnt++;
             int iz = c / (g.nex[0] * g.nex[1]);
             int iy = (c / g.nex[0]) % g.nex[1];

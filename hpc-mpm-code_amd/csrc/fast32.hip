@@ -25,11 +25,6 @@ namespace {
 
 constexpr int kSeg = 64;      // cells per row segment = threads per workgroup
 constexpr int kChunk = 256;   // particles staged per LDS pass
-constexpr int kChunkPad = kChunk + kChunk / 8;
-
-// LDS slot of staged particle i: one pad word every 8 particles spreads the
-// per-cell start offsets of neighbouring threads over the banks.
-__device__ __forceinline__ int lds_idx(int i) { return i + (i >> 3); }
 
 __device__ __forceinline__ float local_coord(float xh, float xl, const float *tnh, const float *tnl, int idx,
                                              float inv) {
@@ -46,144 +41,188 @@ __device__ __forceinline__ void neu_add(float &s, float &c, float x) {
     s = t;
 }
 
-struct SegInfo {
-    int iy, iz, rowbase, ix, c;
-    bool vcell;
+// A group is 64 consecutive entries of the active-cell list (one per thread).
+// Because the list is in ascending cell order and skips only empty cells, the
+// particles of a group are one contiguous range [S, E) of the sorted order.
+struct CellInfo {
+    int ix, iy, iz, c;
     uint32_t cs, ce;
 };
 
-__device__ __forceinline__ bool seg_setup(const GridGeom &g, const CellLists &L, int nseg_x, SegInfo &si,
-                                          uint32_t &S, uint32_t &E) {
+__device__ __forceinline__ void group_setup(const GridGeom &g, const CellLists &L, uint32_t n_active, int grp,
+                                            CellInfo &ci, uint32_t &S, uint32_t &E) {
     const int t = threadIdx.x;
-    const int seg = blockIdx.x;
-    const int sx = seg % nseg_x;
-    const int row = seg / nseg_x;
-    si.iy = row % g.nex[1];
-    si.iz = row / g.nex[1];
-    si.rowbase = si.iy * g.nex[0] + si.iz * g.nex[0] * g.nex[1];
-    si.ix = sx * kSeg + t;
-    si.vcell = si.ix < g.nex[0];
-    si.c = si.rowbase + si.ix;
-    si.cs = 0;
-    si.ce = 0;
-    if (si.vcell) {
-        si.cs = L.cstart[si.c];
-        si.ce = L.cend[si.c];
+    const uint32_t a = (uint32_t)grp * kSeg + t;
+    ci.cs = 0xFFFFFFFFu;
+    ci.ce = 0u;
+    ci.c = -1;
+    ci.ix = ci.iy = ci.iz = 0;
+    if (a < n_active) {
+        ci.c = (int)L.active[a];
+        ci.cs = L.cstart[ci.c];
+        ci.ce = L.cend[ci.c];
+        const int nexy = g.nex[0] * g.nex[1];
+        ci.iz = ci.c / nexy;
+        const int r = ci.c - ci.iz * nexy;
+        ci.iy = r / g.nex[0];
+        ci.ix = r - ci.iy * g.nex[0];
     }
-    bool ne = si.ce > si.cs;
-    uint32_t s = ne ? si.cs : 0xFFFFFFFFu, e = ne ? si.ce : 0u;
+    uint32_t smin = ci.cs, emax = ci.ce;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        s = min(s, (uint32_t)__shfl_xor((int)s, o, 64));
-        e = max(e, (uint32_t)__shfl_xor((int)e, o, 64));
+        smin = min(smin, (uint32_t)__shfl_xor((int)smin, o, 64));
+        emax = max(emax, (uint32_t)__shfl_xor((int)emax, o, 64));
     }
-    S = s;
-    E = e;
-    return e != 0u;
+    S = smin;
+    E = emax;
+    if (ci.c < 0) ci.cs = ci.ce = 0u;
 }
 
 // ------------------------------------------------------------------ P2G
+// Particle fields are gathered (through the sorted order) straight from HBM
+// into LDS with gfx950's global_load_lds_dword: lane l of a wave writes the
+// dword for chunk particle 64*r + l at stage block r. Nothing is held in
+// VGPRs while the loads are in flight, so a wave keeps all of a chunk's loads
+// outstanding. Each field's 64-particle blocks are 65 dwords apart, which
+// spreads the thread-per-cell reads (stride = particles per cell) over the
+// LDS banks.
+constexpr int kBlk = kChunk / 64;
+__device__ __forceinline__ int st_idx(int f, int i) { return (f * kBlk + (i >> 6)) * 65 + (i & 63); }
+
+__device__ __forceinline__ void lds_gather(const void *field_base, uint32_t slot, float *stage, int f, int r) {
+    __builtin_amdgcn_global_load_lds((const uint32_t *)field_base + slot, stage + st_idx(f, 64 * r), 4, 0, 0);
+}
+
+__device__ __forceinline__ void lds_gather2(const double *field_base, uint32_t slot, float *stage, int f, int r) {
+    const uint32_t *p = (const uint32_t *)field_base + 2 * (size_t)slot;
+    __builtin_amdgcn_global_load_lds(p, stage + st_idx(f, 64 * r), 4, 0, 0);
+    __builtin_amdgcn_global_load_lds(p + 1, stage + st_idx(f + 1, 64 * r), 4, 0, 0);
+}
+
+// wait for every outstanding vector-memory load (including LDS DMA) of this wave
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
+
 template <bool DIAG, bool TRAC>
 __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T,
-                                                       float *__restrict__ part, int nseg_x) {
-    constexpr int NF = 13 + (DIAG ? 1 : 0) + (TRAC ? 3 : 0);
-    __shared__ float rec[NF][kChunkPad];
-    SegInfo si;
-    uint32_t S, E;
-    if (!seg_setup(g, L, nseg_x, si, S, E)) return;
+                                                       float *__restrict__ part) {
+    // fields: 0-2 xh, 3-5 xl, 6-8 v, 9 m, 10-11 V (lo, hi dword), 12-17 sigma, [18-20 traction]
+    constexpr int NF = 18 + (TRAC ? 3 : 0);
+    __shared__ float stage[NF * kBlk * 65];
+    const uint32_t n_active = L.ctr->n_active;
+    const int ngroups = (int)((n_active + kSeg - 1) / kSeg);
     const int t = threadIdx.x;
     const size_t ld = P.ld;
     const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
+    for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        CellInfo si;
+        uint32_t S, E;
+        __syncthreads();
+        group_setup(g, L, n_active, grp, si, S, E);
+        __syncthreads();
+        // node coordinates of this thread's cell (lower corner)
+        const float cxh = T.tnh[si.ix], cxl = T.tnl[si.ix];
+        const float cyh = T.tnh[T.ld + si.iy], cyl = T.tnl[T.ld + si.iy];
+        const float czh = T.tnh[2 * T.ld + si.iz], czl = T.tnl[2 * T.ld + si.iz];
 
-    float acc[8][7];
-    float vof[8];
+        float acc[8][7];
+        float vof[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+        for (int q = 0; q < 8; q++) {
 #pragma unroll
-        for (int f = 0; f < 7; f++) acc[q][f] = 0.f;
-        vof[q] = 0.f;
-    }
+            for (int f = 0; f < 7; f++) acc[q][f] = 0.f;
+            vof[q] = 0.f;
+        }
 
-    for (uint32_t base = S; base < E; base += kChunk) {
-        const int nch = (int)min((uint32_t)kChunk, E - base);
-        for (int i = t; i < nch; i += kSeg) {
-            uint32_t slot = L.perm[base + i];
-            int pix = (int)P.key[slot] - si.rowbase;
-            float fx = local_coord(P.xh[slot], P.xl[slot], T.tnh, T.tnl, pix, inv0);
-            float fy = local_coord(P.xh[ld + slot], P.xl[ld + slot], T.tnh + T.ld, T.tnl + T.ld, si.iy, inv1);
-            float fz =
-                local_coord(P.xh[2 * ld + slot], P.xl[2 * ld + slot], T.tnh + 2 * T.ld, T.tnl + 2 * T.ld, si.iz, inv2);
-            float m = P.m[slot];
-            float Vf = (float)P.V[slot];
-            int li = lds_idx(i);
-            rec[0][li] = fx;
-            rec[1][li] = fy;
-            rec[2][li] = fz;
-            rec[3][li] = m;
-            rec[4][li] = m * P.v[slot];
-            rec[5][li] = m * P.v[ld + slot];
-            rec[6][li] = m * P.v[2 * ld + slot];
-            // -V sigma (negated so that the force update is a plain FMA)
+        for (uint32_t base = S; base < E; base += kChunk) {
+            const int nch = (int)min((uint32_t)kChunk, E - base);
+            uint32_t slot[kBlk];
 #pragma unroll
-            for (int q = 0; q < 6; q++) rec[7 + q][li] = -Vf * P.sig[q * ld + slot];
-            if (DIAG) rec[13][li] = Vf;
-            if (TRAC) {
-                rec[NF - 3][li] = P.trac[slot];
-                rec[NF - 2][li] = P.trac[ld + slot];
-                rec[NF - 1][li] = P.trac[2 * ld + slot];
+            for (int r = 0; r < kBlk; r++) slot[r] = (64 * r + t < nch) ? L.perm[base + 64 * r + t] : 0u;
+#pragma unroll
+            for (int r = 0; r < kBlk; r++) {
+                if (64 * r + t < nch) {
+                    const uint32_t sl = slot[r];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        lds_gather(P.xh + a * ld, sl, stage, a, r);
+                        lds_gather(P.xl + a * ld, sl, stage, 3 + a, r);
+                        lds_gather(P.v + a * ld, sl, stage, 6 + a, r);
+                    }
+                    lds_gather(P.m, sl, stage, 9, r);
+                    lds_gather2(P.V, sl, stage, 10, r);
+#pragma unroll
+                    for (int q = 0; q < 6; q++) lds_gather(P.sig + q * ld, sl, stage, 12 + q, r);
+                    if (TRAC) {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) lds_gather(P.trac + a * ld, sl, stage, 18 + a, r);
+                    }
+                }
+            }
+            wait_vm_all();
+            __syncthreads();
+            const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
+            for (uint32_t pos = lo; pos < hi; pos++) {
+                const int i = (int)(pos - base);
+                const float fx = ((stage[st_idx(0, i)] - cxh) + (stage[st_idx(3, i)] - cxl)) * inv0;
+                const float fy = ((stage[st_idx(1, i)] - cyh) + (stage[st_idx(4, i)] - cyl)) * inv1;
+                const float fz = ((stage[st_idx(2, i)] - czh) + (stage[st_idx(5, i)] - czl)) * inv2;
+                const float m = stage[st_idx(9, i)];
+                const float px = m * stage[st_idx(6, i)], py = m * stage[st_idx(7, i)], pz = m * stage[st_idx(8, i)];
+                const double Vd = __hiloint2double(__float_as_int(stage[st_idx(11, i)]),
+                                                   __float_as_int(stage[st_idx(10, i)]));
+                const float nV = -(float)Vd;
+                // Voigt xx,yy,zz,yz,xz,xy of -V sigma
+                const float sxx = nV * stage[st_idx(12, i)], syy = nV * stage[st_idx(13, i)];
+                const float szz = nV * stage[st_idx(14, i)], syz = nV * stage[st_idx(15, i)];
+                const float sxz = nV * stage[st_idx(16, i)], sxy = nV * stage[st_idx(17, i)];
+                float tx = 0.f, ty = 0.f, tz = 0.f;
+                if (TRAC) {
+                    tx = stage[st_idx(18, i)];
+                    ty = stage[st_idx(19, i)];
+                    tz = stage[st_idx(20, i)];
+                }
+                const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
+#pragma unroll
+                for (int k = 0; k < 2; k++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+#pragma unroll
+                        for (int ii = 0; ii < 2; ii++) {
+                            const int q = ii + 2 * j + 4 * k;
+                            const float wyz = wy[j] * wz[k];
+                            const float N = wx[ii] * wyz;
+                            const float gx = (ii ? inv0 : -inv0) * wyz;
+                            const float gy = (j ? inv1 : -inv1) * (wx[ii] * wz[k]);
+                            const float gz = (k ? inv2 : -inv2) * (wx[ii] * wy[j]);
+                            acc[q][0] += N * m;
+                            acc[q][1] += N * px;
+                            acc[q][2] += N * py;
+                            acc[q][3] += N * pz;
+                            acc[q][4] += sxx * gx + sxy * gy + sxz * gz;
+                            acc[q][5] += sxy * gx + syy * gy + syz * gz;
+                            acc[q][6] += sxz * gx + syz * gy + szz * gz;
+                            if (TRAC) {
+                                acc[q][4] += N * tx;
+                                acc[q][5] += N * ty;
+                                acc[q][6] += N * tz;
+                            }
+                            if (DIAG) vof[q] += N * (float)Vd;
+                        }
+            }
+            __syncthreads();
+        }
+        if (si.ce > si.cs) {
+            const size_t ne = (size_t)g.nelem;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int f = 0; f < 7; f++) part[(size_t)(q * 7 + f) * ne + si.c] = acc[q][f];
+            if (DIAG) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) part[(size_t)(56 + q) * ne + si.c] = vof[q];
             }
         }
-        __syncthreads();
-        const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
-        for (uint32_t pos = lo; pos < hi; pos++) {
-            const int li = lds_idx((int)(pos - base));
-            const float fx = rec[0][li], fy = rec[1][li], fz = rec[2][li];
-            const float m = rec[3][li], px = rec[4][li], py = rec[5][li], pz = rec[6][li];
-            // Voigt xx,yy,zz,yz,xz,xy of -V sigma
-            const float sxx = rec[7][li], syy = rec[8][li], szz = rec[9][li];
-            const float syz = rec[10][li], sxz = rec[11][li], sxy = rec[12][li];
-            const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
-#pragma unroll
-            for (int k = 0; k < 2; k++)
-#pragma unroll
-                for (int j = 0; j < 2; j++)
-#pragma unroll
-                    for (int i = 0; i < 2; i++) {
-                        const int q = i + 2 * j + 4 * k;
-                        const float wyz = wy[j] * wz[k];
-                        const float N = wx[i] * wyz;
-                        const float gx = (i ? inv0 : -inv0) * wyz;
-                        const float gy = (j ? inv1 : -inv1) * (wx[i] * wz[k]);
-                        const float gz = (k ? inv2 : -inv2) * (wx[i] * wy[j]);
-                        acc[q][0] += N * m;
-                        acc[q][1] += N * px;
-                        acc[q][2] += N * py;
-                        acc[q][3] += N * pz;
-                        acc[q][4] += sxx * gx + sxy * gy + sxz * gz;
-                        acc[q][5] += sxy * gx + syy * gy + syz * gz;
-                        acc[q][6] += sxz * gx + syz * gy + szz * gz;
-                        if (TRAC) {
-                            acc[q][4] += N * rec[NF - 3][li];
-                            acc[q][5] += N * rec[NF - 2][li];
-                            acc[q][6] += N * rec[NF - 1][li];
-                        }
-                        if (DIAG) vof[q] += N * rec[13][li];
-                    }
-        }
-        __syncthreads();
-    }
-    if (si.ce > si.cs) {
-        const size_t ne = (size_t)g.nelem;
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-#pragma unroll
-            for (int f = 0; f < 7; f++) part[(size_t)(q * 7 + f) * ne + si.c] = acc[q][f];
-        if (DIAG) {
-#pragma unroll
-            for (int q = 0; q < 8; q++) part[(size_t)(56 + q) * ne + si.c] = vof[q];
-        }
-    }
-    (void)t;
+    }  // groups
 }
 
 __device__ __forceinline__ void node_ijk(const GridGeom &g, int n, int &i, int &j, int &k) {
@@ -197,13 +236,33 @@ __device__ __forceinline__ float safe_div(float p, float m, float m_small) {
     return (float)((double)p / (double)m);
 }
 
+// Node of this thread in the node tile of this block; false when the tile has
+// no non-empty cell next to it (all threads of the block agree) or the node
+// lies outside the grid.
+__device__ __forceinline__ bool tile_node(const GridGeom &g, const CellLists &L, const TileGeom &tg, bool all_tiles,
+                                          int &n, int &i, int &j, int &k) {
+    const int tile = blockIdx.x;
+    if (!all_tiles && L.tiles[tile] == 0u) return false;
+    const int tx = tile % tg.ntx;
+    const int ty = (tile / tg.ntx) % tg.nty;
+    const int tz = tile / (tg.ntx * tg.nty);
+    const int l = threadIdx.x;
+    i = tx * kNodeTileX + (l % kNodeTileX);
+    j = ty * kNodeTileY + (l / kNodeTileX) % kNodeTileY;
+    k = tz * kNodeTileZ + l / (kNodeTileX * kNodeTileY);
+    if (i >= g.nxc[0] || j >= g.nxc[1] || k >= g.nxc[2]) return false;
+    n = i + g.nxc[0] * (j + g.nxc[1] * k);
+    return true;
+}
+
+__device__ __forceinline__ bool cell_nonempty(const CellLists &L, int c) { return L.cend[c] > L.cstart[c]; }
+
 template <bool DIAG>
-__global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, const float *__restrict__ part,
-                                                      NodesF32 N, BcDense bc, StepConsts sc) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.nodec) return;
-    int i, j, k;
-    node_ijk(g, n, i, j, k);
+__global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, TileGeom tg,
+                                                      const float *__restrict__ part, NodesF32 N, BcDense bc,
+                                                      StepConsts sc) {
+    int n, i, j, k;
+    if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
     const size_t ne = (size_t)g.nelem;
     constexpr int NQ = DIAG ? 8 : 7;
     float s[NQ], cp[NQ];
@@ -220,7 +279,7 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, co
             for (int ci = i - 1; ci <= i; ci++) {
                 if (ci < 0 || ci >= g.nex[0]) continue;
                 int c = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
-                if (L.cend[c] == 0u) continue;
+                if (!cell_nonempty(L, c)) continue;
                 int corner = (i - ci) + 2 * (j - cj) + 4 * (k - ck);
                 any = true;
 #pragma unroll
@@ -289,76 +348,33 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, co
 // ------------------------------------------------------ G2P velocity + MUSL
 template <bool FLIP>
 __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T, NodesF32 N,
-                                                        StepConsts sc, float *__restrict__ part, int nseg_x) {
-    constexpr int NF = 8 + (FLIP ? 3 : 0);  // fx fy fz m vx vy vz slot [v_old]
-    __shared__ float rec[NF][kChunkPad];
-    SegInfo si;
-    uint32_t S, E;
-    if (!seg_setup(g, L, nseg_x, si, S, E)) return;
+                                                        StepConsts sc, float *__restrict__ part) {
+    // staged fields: 0-2 xh, 3-5 xl, 6 m, [7-9 old v for FLIP]; results: G2P velocity per particle
+    constexpr int NF = 7 + (FLIP ? 3 : 0);
+    __shared__ float stage[NF * kBlk * 65];
+    __shared__ float vout[3 * kBlk * 65];
+    const uint32_t n_active = L.ctr->n_active;
+    const int ngroups = (int)((n_active + kSeg - 1) / kSeg);
     const int t = threadIdx.x;
     const size_t ld = P.ld;
     const int nodec = g.nodec;
     const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
-    const bool ne = si.ce > si.cs;
-
-    // node values of this cell: PIC uses v + dt a (var_trans_solid_explicit.cpp:194-200),
-    // FLIP uses a
-    float nvl[8][3];
-    if (ne) {
-        const int n0 = si.ix + g.nxc[0] * si.iy + g.nxc[0] * g.nxc[1] * si.iz;
-#pragma unroll
-        for (int k = 0; k < 2; k++)
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-#pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    const int q = i + 2 * j + 4 * k;
-                    const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        float a = N.na[c * nodec + nid];
-                        nvl[q][c] = FLIP ? a : (N.nv[c * nodec + nid] + sc.dt * a);
-                    }
-                }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) nvl[q][c] = 0.f;
-    }
-
-    float acc[8][3];
-#pragma unroll
-    for (int q = 0; q < 8; q++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) acc[q][c] = 0.f;
-
-    for (uint32_t base = S; base < E; base += kChunk) {
-        const int nch = (int)min((uint32_t)kChunk, E - base);
-        for (int i = t; i < nch; i += kSeg) {
-            uint32_t slot = L.perm[base + i];
-            int pix = (int)P.key[slot] - si.rowbase;
-            int li = lds_idx(i);
-            rec[0][li] = local_coord(P.xh[slot], P.xl[slot], T.tnh, T.tnl, pix, inv0);
-            rec[1][li] = local_coord(P.xh[ld + slot], P.xl[ld + slot], T.tnh + T.ld, T.tnl + T.ld, si.iy, inv1);
-            rec[2][li] =
-                local_coord(P.xh[2 * ld + slot], P.xl[2 * ld + slot], T.tnh + 2 * T.ld, T.tnl + 2 * T.ld, si.iz, inv2);
-            rec[3][li] = P.m[slot];
-            rec[7][li] = __uint_as_float(slot);
-            if (FLIP) {
-                rec[8][li] = P.v[slot];
-                rec[9][li] = P.v[ld + slot];
-                rec[10][li] = P.v[2 * ld + slot];
-            }
-        }
+    for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        CellInfo si;
+        uint32_t S, E;
         __syncthreads();
-        const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
-        for (uint32_t pos = lo; pos < hi; pos++) {
-            const int li = lds_idx((int)(pos - base));
-            const float fx = rec[0][li], fy = rec[1][li], fz = rec[2][li], m = rec[3][li];
-            const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
-            float Nw[8];
-            float vp[3] = {0.f, 0.f, 0.f};
+        group_setup(g, L, n_active, grp, si, S, E);
+        __syncthreads();
+        const bool ne = si.ce > si.cs;
+        const float cxh = T.tnh[si.ix], cxl = T.tnl[si.ix];
+        const float cyh = T.tnh[T.ld + si.iy], cyl = T.tnl[T.ld + si.iy];
+        const float czh = T.tnh[2 * T.ld + si.iz], czl = T.tnl[2 * T.ld + si.iz];
+
+        // node values of this cell: PIC uses v + dt a (var_trans_solid_explicit.cpp:194-200),
+        // FLIP uses a
+        float nvl[8][3];
+        if (ne) {
+            const int n0 = si.ix + g.nxc[0] * si.iy + g.nxc[0] * g.nxc[1] * si.iz;
 #pragma unroll
             for (int k = 0; k < 2; k++)
 #pragma unroll
@@ -366,49 +382,110 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
 #pragma unroll
                     for (int i = 0; i < 2; i++) {
                         const int q = i + 2 * j + 4 * k;
-                        Nw[q] = wx[i] * (wy[j] * wz[k]);
+                        const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
 #pragma unroll
-                        for (int c = 0; c < 3; c++) vp[c] += Nw[q] * nvl[q][c];
+                        for (int c = 0; c < 3; c++) {
+                            float a = N.na[c * nodec + nid];
+                            nvl[q][c] = FLIP ? a : (N.nv[c * nodec + nid] + sc.dt * a);
+                        }
                     }
-            if (FLIP) {
-                vp[0] = rec[8][li] + sc.dt * vp[0];
-                vp[1] = rec[9][li] + sc.dt * vp[1];
-                vp[2] = rec[10][li] + sc.dt * vp[2];
-            }
-            rec[4][li] = vp[0];
-            rec[5][li] = vp[1];
-            rec[6][li] = vp[2];
-            const float mv[3] = {m * vp[0], m * vp[1], m * vp[2]};
+        } else {
 #pragma unroll
             for (int q = 0; q < 8; q++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) acc[q][c] += Nw[q] * mv[c];
+                for (int c = 0; c < 3; c++) nvl[q][c] = 0.f;
         }
-        __syncthreads();
-        for (int i = t; i < nch; i += kSeg) {
-            int li = lds_idx(i);
-            uint32_t slot = __float_as_uint(rec[7][li]);
-            P.v[slot] = rec[4][li];
-            P.v[ld + slot] = rec[5][li];
-            P.v[2 * ld + slot] = rec[6][li];
-        }
-    }
-    if (ne) {
-        const size_t nel = (size_t)g.nelem;
+
+        float acc[8][3];
 #pragma unroll
         for (int q = 0; q < 8; q++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) part[(size_t)(q * 3 + c) * nel + si.c] = acc[q][c];
-    }
+            for (int c = 0; c < 3; c++) acc[q][c] = 0.f;
+
+        for (uint32_t base = S; base < E; base += kChunk) {
+            const int nch = (int)min((uint32_t)kChunk, E - base);
+            uint32_t slot[kBlk];
+#pragma unroll
+            for (int r = 0; r < kBlk; r++) slot[r] = (64 * r + t < nch) ? L.perm[base + 64 * r + t] : 0u;
+#pragma unroll
+            for (int r = 0; r < kBlk; r++) {
+                if (64 * r + t < nch) {
+                    const uint32_t sl = slot[r];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        lds_gather(P.xh + a * ld, sl, stage, a, r);
+                        lds_gather(P.xl + a * ld, sl, stage, 3 + a, r);
+                        if (FLIP) lds_gather(P.v + a * ld, sl, stage, 7 + a, r);
+                    }
+                    lds_gather(P.m, sl, stage, 6, r);
+                }
+            }
+            wait_vm_all();
+            __syncthreads();
+            const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
+            for (uint32_t pos = lo; pos < hi; pos++) {
+                const int i = (int)(pos - base);
+                const float fx = ((stage[st_idx(0, i)] - cxh) + (stage[st_idx(3, i)] - cxl)) * inv0;
+                const float fy = ((stage[st_idx(1, i)] - cyh) + (stage[st_idx(4, i)] - cyl)) * inv1;
+                const float fz = ((stage[st_idx(2, i)] - czh) + (stage[st_idx(5, i)] - czl)) * inv2;
+                const float m = stage[st_idx(6, i)];
+                const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
+                float Nw[8];
+                float vp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < 2; k++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+#pragma unroll
+                        for (int ii = 0; ii < 2; ii++) {
+                            const int q = ii + 2 * j + 4 * k;
+                            Nw[q] = wx[ii] * (wy[j] * wz[k]);
+#pragma unroll
+                            for (int c = 0; c < 3; c++) vp[c] += Nw[q] * nvl[q][c];
+                        }
+                if (FLIP) {
+                    vp[0] = stage[st_idx(7, i)] + sc.dt * vp[0];
+                    vp[1] = stage[st_idx(8, i)] + sc.dt * vp[1];
+                    vp[2] = stage[st_idx(9, i)] + sc.dt * vp[2];
+                }
+                vout[st_idx(0, i)] = vp[0];
+                vout[st_idx(1, i)] = vp[1];
+                vout[st_idx(2, i)] = vp[2];
+                const float mv[3] = {m * vp[0], m * vp[1], m * vp[2]};
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) acc[q][c] += Nw[q] * mv[c];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kBlk; r++) {
+                const int i = 64 * r + t;
+                if (i < nch) {
+                    const uint32_t sl = slot[r];
+                    P.v[sl] = vout[st_idx(0, i)];
+                    P.v[ld + sl] = vout[st_idx(1, i)];
+                    P.v[2 * ld + sl] = vout[st_idx(2, i)];
+                }
+            }
+            __syncthreads();
+        }
+        if (ne) {
+            const size_t nel = (size_t)g.nelem;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) part[(size_t)(q * 3 + c) * nel + si.c] = acc[q][c];
+        }
+    }  // groups
 }
 
 template <bool DIAG>
-__global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, const float *__restrict__ part,
-                                                       NodesF32 N, BcDense bc, StepConsts sc) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.nodec) return;
-    int i, j, k;
-    node_ijk(g, n, i, j, k);
+__global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, TileGeom tg,
+                                                       const float *__restrict__ part, NodesF32 N, BcDense bc,
+                                                       StepConsts sc) {
+    int n, i, j, k;
+    if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
     const size_t ne = (size_t)g.nelem;
     float s[3] = {0.f, 0.f, 0.f}, cp[3] = {0.f, 0.f, 0.f};
     bool any = false;
@@ -419,7 +496,7 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, c
             for (int ci = i - 1; ci <= i; ci++) {
                 if (ci < 0 || ci >= g.nex[0]) continue;
                 int c = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
-                if (L.cend[c] == 0u) continue;
+                if (!cell_nonempty(L, c)) continue;
                 int corner = (i - ci) + 2 * (j - cj) + 4 * (k - ck);
                 any = true;
 #pragma unroll
@@ -742,50 +819,50 @@ __global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict
 
 }  // namespace
 
-static int segs_x(const GridGeom &g) { return cdiv(g.nex[0], kSeg); }
+static int cell_groups_grid(const GridGeom &g) {
+    int ng = cdiv(g.nelem, kSeg);
+    return ng < 4096 ? ng : 4096;
+}
 
 void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
-                    const BcDense &bc, const NodeTables &T, const StepConsts &c, float *partials, int diag,
-                    hipEvent_t mid) {
-    const int nsx = segs_x(g);
-    const int nseg = nsx * g.nex[1] * g.nex[2];
+                    const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
+                    int diag, hipEvent_t mid) {
+    const int ng = cell_groups_grid(g);
     const bool trac = P.trac != nullptr;
     if (diag) {
         if (trac)
-            hipLaunchKernelGGL((k32_p2g_cells<true, true>), dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, partials, nsx);
+            hipLaunchKernelGGL((k32_p2g_cells<true, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
         else
-            hipLaunchKernelGGL((k32_p2g_cells<true, false>), dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, partials, nsx);
+            hipLaunchKernelGGL((k32_p2g_cells<true, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
         if (mid) (void)hipEventRecord(mid, s);
-        hipLaunchKernelGGL(k32_p2g_nodes<true>, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, L,
-                           (const float *)partials, N, bc, c);
+        hipLaunchKernelGGL(k32_p2g_nodes<true>, dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials, N,
+                           bc, c);
     } else {
         if (trac)
-            hipLaunchKernelGGL((k32_p2g_cells<false, true>), dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, partials, nsx);
+            hipLaunchKernelGGL((k32_p2g_cells<false, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
         else
-            hipLaunchKernelGGL((k32_p2g_cells<false, false>), dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, partials,
-                               nsx);
+            hipLaunchKernelGGL((k32_p2g_cells<false, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
         if (mid) (void)hipEventRecord(mid, s);
-        hipLaunchKernelGGL(k32_p2g_nodes<false>, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, L,
-                           (const float *)partials, N, bc, c);
+        hipLaunchKernelGGL(k32_p2g_nodes<false>, dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials,
+                           N, bc, c);
     }
 }
 
 void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
-                             const CellLists &L, const BcDense &bc, const NodeTables &T, const StepConsts &c,
-                             float *partials, int diag, hipEvent_t mid) {
-    const int nsx = segs_x(g);
-    const int nseg = nsx * g.nex[1] * g.nex[2];
+                             const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
+                             const StepConsts &c, float *partials, int diag, hipEvent_t mid) {
+    const int ng = cell_groups_grid(g);
     if (c.scheme == 1)
-        hipLaunchKernelGGL(k32_musl_cells<true>, dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, N, c, partials, nsx);
+        hipLaunchKernelGGL(k32_musl_cells<true>, dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
     else
-        hipLaunchKernelGGL(k32_musl_cells<false>, dim3(nseg), dim3(kSeg), 0, s, g, P, L, T, N, c, partials, nsx);
+        hipLaunchKernelGGL(k32_musl_cells<false>, dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
     if (mid) (void)hipEventRecord(mid, s);
     if (diag)
-        hipLaunchKernelGGL(k32_musl_nodes<true>, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, L,
-                           (const float *)partials, N, bc, c);
+        hipLaunchKernelGGL(k32_musl_nodes<true>, dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials, N,
+                           bc, c);
     else
-        hipLaunchKernelGGL(k32_musl_nodes<false>, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, L,
-                           (const float *)partials, N, bc, c);
+        hipLaunchKernelGGL(k32_musl_nodes<false>, dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials,
+                           N, bc, c);
 }
 
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,

@@ -45,7 +45,14 @@ enum : uint32_t {
     kErrOutOfGrid = 1u,
     kErrNonFinite = 2u,
     kErrJacobian = 4u,
+    kErrCellOverflow = 8u,  // more than 4096 particles in one cell
 };
+
+// Node tiles of the node passes (x fastest); a tile is processed only when a
+// cell touching one of its nodes holds particles.
+constexpr int kNodeTileX = 16;
+constexpr int kNodeTileY = 4;
+constexpr int kNodeTileZ = 4;
 
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 

@@ -17,24 +17,60 @@ struct SortBuffers {
 
 size_t radix_hist_entries(int n);
 int excl_scan_u32(hipStream_t s, uint32_t *data, int n, uint32_t *bsum);
-// Stable LSD sort of (keys_in[i], i) over nbits key bits. keys_in is not
+// Stable LSD radix sort of (keys_in[i], i) over nbits key bits (standalone
+// binning entry point). keys_in is not
 // modified. Results point into the SortBuffers arrays.
 int radix_sort_pairs(hipStream_t s, SortBuffers &b, const uint32_t *keys_in, int n, int nbits, uint32_t **keys_out,
                      uint32_t **vals_out);
 void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
-void launch_cell_ranges(hipStream_t s, const uint32_t *skeys, int n, int nelem, uint32_t *cstart, uint32_t *cend);
 void launch_keys_f64(hipStream_t s, const GridGeom &g, const double *x, size_t ld, int n, uint32_t *keys,
                      uint32_t *err);
 void launch_keys_hilo(hipStream_t s, const GridGeom &g, const float *xh, const float *xl, size_t ld, int n,
                       uint32_t *keys, uint32_t *err);
 
 // Cell lists produced by the binning: particles of cell c are
-// perm[cstart[c] .. cend[c]) (both 0 for an empty cell).
+// perm[cstart[c] .. cend[c]) (cstart == cend for an empty cell); active[] lists
+// the non-empty cells in ascending order, ctr->n_active of them; tiles[]
+// flags the node tiles next to a non-empty cell.
+struct BinCounters {
+    uint32_t n_active;
+    uint32_t n_fix;
+    uint32_t pad[2];
+};
+
 struct CellLists {
     const uint32_t *perm;
     const uint32_t *cstart;
     const uint32_t *cend;
+    const uint32_t *active;
+    const BinCounters *ctr;
+    const uint32_t *tiles;
 };
+
+struct TileGeom {
+    int ntx, nty, ntz, ntiles;
+};
+
+struct BinBuffers {
+    uint32_t *cnt;      // [bin_cell_capacity], zero between calls
+    uint32_t *cstart;   // [bin_cell_capacity]
+    uint32_t *cend;     // [bin_cell_capacity]
+    uint32_t *active;   // [nelem]
+    uint32_t *stamp;    // [nelem], fix-up queue marks
+    uint32_t *fixlist;  // [nelem]
+    uint32_t *perm;     // [n]
+    uint32_t *tiles;    // [ntiles]
+    uint64_t *bsum;     // [bin_scan_blocks(nelem)]
+    BinCounters *ctr;
+};
+
+TileGeom make_tile_geom(const GridGeom &g);
+size_t bin_scan_blocks(int nelem);
+// per-cell arrays (cnt, cstart, cend) must hold this many entries (padding zeroed)
+size_t bin_cell_capacity(int nelem);
+// Counting-sort binning of keys[0..n) (sentinel nelem = outside the grid).
+int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
+               uint32_t stamp, uint32_t *err);
 
 // ------------------------------------------------------ FP64 exact kernels
 struct PartF64 {
@@ -129,12 +165,12 @@ constexpr int kPartialsPerCell = 64;
 // P2G: corner partials per cell + node pass with the grid update (v = p/m,
 // a = f/m, BCs). diag != 0 also writes momentum, force and nodal volume.
 void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
-                    const BcDense &bc, const NodeTables &T, const StepConsts &c, float *partials, int diag,
-                    hipEvent_t mid = nullptr);
+                    const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
+                    int diag, hipEvent_t mid = nullptr);
 // G2P velocity fused with the MUSL momentum remap + MUSL node pass.
 void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
-                             const CellLists &L, const BcDense &bc, const NodeTables &T, const StepConsts &c,
-                             float *partials, int diag, hipEvent_t mid = nullptr);
+                             const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
+                             const StepConsts &c, float *partials, int diag, hipEvent_t mid = nullptr);
 // dF, H, J, V, x, stress and the new cell key of every slot.
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                            const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err);

@@ -240,3 +240,25 @@ def test_fp32_relayout_changes_only_summation_order():
         outs.append(e.download(s.n))
     assert np.max(np.abs(outs[0].coord - outs[1].coord)) < 1e-9
     assert rel(outs[0].stress, outs[1].stress) < 1e-5
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_FP64_EXACT, capi.PREC_FP32])
+def test_engine_bins_crowded_cells(prec):
+    """Cells holding 17..400 particles in interleaved slot order: every
+    ordering path of the binning (register networks and the long-cell path)
+    must still give the stable order of the reference's linked lists."""
+    s = scenes.tiny(n_cells=8)
+    rng = np.random.default_rng(5)
+    n = s.n
+    centers = np.array([[0.3, 0.2, 0.3], [0.55, 0.1, 0.6], [0.7, 0.3, 0.2], [0.1, 0.1, 0.1]])
+    which = rng.integers(0, len(centers), n)
+    s.parts.coord[:] = centers[which] + rng.random((n, 3)) * 0.1
+    e = engine(s, prec)
+    e.bin()
+    idepf, idp2p, numep, cell = e.bins(n)
+    o = pyoracle.Oracle(s)
+    assert o.bin() == 0
+    assert numep.max() > 32
+    assert np.array_equal(numep, o.numep)
+    assert np.array_equal(idepf, o.idepf)
+    assert np.array_equal(idp2p, o.idp2p)
