@@ -1,11 +1,12 @@
 """Benchmark of the explicit solid MPM step on one MI355X.
 
 A step is one pass of the hot path over the whole particle set:
-cell binning (keys + radix sort + cell ranges), P2G with the grid update,
+cell binning (counting sort by cell key), P2G with the grid update,
 G2P velocity + MUSL remap, and the dF/F/J/x/V/stress update.
 Workload (default): the 3D Neo-Hookean cube on a 128^3 grid at 2x4x2
 points per cell = 4,194,304 particles, synthetic (seeded) data, FP32
-arithmetic with compensated node sums, FP64 det F and volume.
+arithmetic with compensated node sums, FP64 det F and volume, positions
+as cell index + FP32 offset.
 
 Prints one JSON line: particle-steps/s over the timed steps, the roofline of
 the dominant kernel (algorithmic bytes per launch / HIP-event time on the
@@ -32,16 +33,16 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # engine's storage format; the cell-partial buffer between a cells kernel
 # and its node pass is not counted).
 BYTES = {
-    # perm 4 + x (hi,lo) 24 + v 12 + m 4 + V(f64) 8 + stress 24
-    "p2g_cells": (76, 0, 0),
+    # perm 4 + cell offset 12 + v 12 + m 4 + V(f64) 8 + stress 24
+    "p2g_cells": (64, 0, 0),
     # node m 4 + v 12 + a 12 written
     "p2g_nodes": (0, 28, 0),
-    # perm 4 + x 24 + m 4 read, v 12 written; node v, a read 24
-    "musl_cells": (44, 24, 0),
+    # perm 4 + cell offset 12 + m 4 read, v 12 written; node v, a read 24
+    "musl_cells": (32, 24, 0),
     # node m 4 read, MUSL v 12 written
     "musl_nodes": (0, 16, 0),
-    # key 4 + x 24 + H 36 + V0 4 read; x 24 + H 36 + stress 24 + J 8 + V 8 + key 4 written; node v 12
-    "update": (172, 12, 0),
+    # key 4 + offset 12 + H 36 + V0 4 read; offset 12 + H 36 + stress 24 + J 8 + V 8 + key 4 written; node v 12
+    "update": (148, 12, 0),
     # counting sort: keys read twice, perm written (12 per particle); per cell: count read twice,
     # start and end written (16)
     "bin": (12, 0, 16),
@@ -58,6 +59,8 @@ def parse():
     ap.add_argument("--relayout", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes-per-cell", type=int, default=0, help="override (1, 2, 4, 8); 0 = automatic")
+    ap.add_argument("--no-probe", action="store_true", help="skip the copy-bandwidth probe")
     return ap.parse_args()
 
 
@@ -97,6 +100,8 @@ def main():
     eng = capi.Engine(scene, precision=prec, relayout_every=a.relayout if prec == capi.PREC_FP32 else 0)
     eng.upload(scene.parts)
     eng.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
+    if a.lanes_per_cell:
+        eng.set_option(capi.OPT_LANES_PER_CELL, a.lanes_per_cell)
     capi.check(eng.step(1, a.warmup))
     eng.set_timing(True)
     capi.check(eng.lib.mpm_sync(eng.h))
@@ -134,6 +139,9 @@ def main():
                 "frac": round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
 
     phases = {"p2g": phase(["p2g_cells", "p2g_nodes"]), "g2p": phase(["musl_cells", "musl_nodes", "update"])}
+
+    copy_gbps = None if a.no_probe else round(capi.bandwidth_probe(1 << 30, 20), 1)
+    roofline["copy_ceiling_GBps"] = copy_gbps
 
     cpu = None
     if not a.no_cpu:

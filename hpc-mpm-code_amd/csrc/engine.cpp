@@ -42,7 +42,7 @@ static int bit_length(uint32_t v) {
 
 using namespace mpm;
 
-enum { OPT_FULL_NODE_OUTPUT = 1, OPT_RELAYOUT_EVERY = 2 };
+enum { OPT_FULL_NODE_OUTPUT = 1, OPT_LANES_PER_CELL = 2 };
 
 struct mpm_engine {
     GridGeom g{};
@@ -72,8 +72,9 @@ struct mpm_engine {
     NodesF32 N32{};
     float *partials = nullptr;
     NodeTables T{};
-    float *d_tn = nullptr;
     float m_small = 0.f;
+    int lanes_per_cell = 1;  // from the particles per non-empty cell at upload
+    int fp32_outside_at_upload = 0;
 
     uint32_t *keys64 = nullptr;  // FP64 mode cell keys by particle
     BinBuffers bb{};             // counting-sort binning
@@ -176,6 +177,7 @@ StepConsts consts(const mpm_engine *e, double facl) {
     c.facl = facl;
     c.m_small = e->m_small;
     c.scheme = e->prm.scheme;
+    c.lanes_per_cell = e->lanes_per_cell;
     return c;
 }
 
@@ -183,7 +185,8 @@ int check_err(mpm_engine *e) {
     uint32_t h = 0;
     MPM_HIP_CHECK(hipMemcpyAsync(&h, e->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
-    if (h & kErrOutOfGrid) return set_error(MPM_ERR_OUT_OF_GRID, "a particle is outside the background grid");
+    if ((h & kErrOutOfGrid) || e->fp32_outside_at_upload)
+        return set_error(MPM_ERR_OUT_OF_GRID, "a particle is outside the background grid");
     if (h & kErrNonFinite) return set_error(MPM_ERR_NONFINITE, "non-finite particle state");
     if (h & kErrJacobian) return set_error(MPM_ERR_JACOBIAN, "det F <= 0");
     if (h & kErrCellOverflow) return set_error(MPM_ERR_UNSUPPORTED, "more than 4096 particles in one cell");
@@ -201,11 +204,7 @@ int do_bin(mpm_engine *e) {
     if (!e->keys_valid) {
         if (e->prec == MPM_PREC_FP64_EXACT)
             launch_keys_f64(e->stream, e->g, e->P64.x, e->ld, n, e->keys64, e->d_err);
-        else {
-            PartF32 &P = e->P32[e->cur];
-            launch_keys_hilo(e->stream, e->g, P.xh, P.xl, e->ld, n, P.key, e->d_err);
-        }
-        e->keys_valid = true;
+        e->keys_valid = true;  // FP32 keys are uploaded and then kept by the update kernel
     }
     const uint32_t *keys = (e->prec == MPM_PREC_FP64_EXACT) ? e->keys64 : e->P32[e->cur].key;
     int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, ++e->stamp, e->d_err);
@@ -294,8 +293,7 @@ int alloc_particles(mpm_engine *e, int n) {
         int nbuf = e->relayout_every > 0 ? 2 : 1;
         for (int b = 0; b < nbuf; b++) {
             PartF32 &P = e->P32[b];
-            if ((rc = e->dalloc(&P.xh, 3 * ld, true)) || (rc = e->dalloc(&P.xl, 3 * ld, true)) ||
-                (rc = e->dalloc(&P.v, 3 * ld, true)) || (rc = e->dalloc(&P.m, ld, true)) ||
+            if ((rc = e->dalloc(&P.r, 3 * ld, true)) || (rc = e->dalloc(&P.v, 3 * ld, true)) || (rc = e->dalloc(&P.m, ld, true)) ||
                 (rc = e->dalloc(&P.V0, ld, true)) || (rc = e->dalloc(&P.V, ld, true)) ||
                 (rc = e->dalloc(&P.J, ld, true)) || (rc = e->dalloc(&P.H, 9 * ld, true)) ||
                 (rc = e->dalloc(&P.sig, 6 * ld, true)) || (rc = e->dalloc(&P.key, ld, true)) ||
@@ -448,30 +446,12 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
             delete e;
             return rc;
         }
-        // node coordinate tables, FP32 (hi, lo) pairs of xmin + dx * i
-        int tld = std::max(g.nxc[0], std::max(g.nxc[1], g.nxc[2]));
-        std::vector<float> tn(6 * (size_t)tld, 0.f);
-        for (int a = 0; a < 3; a++)
-            for (int i = 0; i < g.nxc[a]; i++) {
-                double xn = g.xmin[a] + g.dx[a] * (double)i;
-                float hi = (float)xn;
-                tn[(size_t)a * tld + i] = hi;
-                tn[(size_t)(3 + a) * tld + i] = (float)(xn - (double)hi);
-            }
-        if ((rc = e->dalloc(&e->d_tn, tn.size()))) {
-            delete e;
-            return rc;
+        for (int a = 0; a < 3; a++) {
+            e->T.inv_dx[a] = (float)(1.0 / g.dx[a]);
+            e->T.dx[a] = (float)g.dx[a];
         }
-        he = hipMemcpy(e->d_tn, tn.data(), tn.size() * sizeof(float), hipMemcpyHostToDevice);
-        if (he != hipSuccess) {
-            int r = hip_fail(he, "hipMemcpy", __FILE__, __LINE__);
-            delete e;
-            return r;
-        }
-        e->T.tnh = e->d_tn;
-        e->T.tnl = e->d_tn + 3 * (size_t)tld;
-        e->T.ld = tld;
-        for (int a = 0; a < 3; a++) e->T.inv_dx[a] = (float)(1.0 / g.dx[a]);
+        e->T.inv_nex = (float)(1.0 / (double)g.nex[0]);
+        e->T.inv_nexy = (float)(1.0 / ((double)g.nex[0] * g.nex[1]));
     }
     *out = e;
     return 0;
@@ -520,6 +500,32 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
     double msum = 0.0;
     for (int i = 0; i < n; i++) msum += p->mass[i];
     e->m_small = (float)(1e-2 * msum / (double)n);
+    {
+        // particles per non-empty cell picks how many lanes fold one cell: a
+        // 256-particle LDS chunk should cover the 64 / L cells of a group
+        const GridGeom &g = e->g;
+        std::vector<uint8_t> seen((size_t)g.nelem, 0);
+        long long nact = 0;
+        for (int i = 0; i < n; i++) {
+            double q[3];
+            int ok = 1, idx[3];
+            for (int a = 0; a < 3; a++) {
+                q[a] = std::floor((p->coord[3 * (size_t)i + a] - g.xmin[a]) / g.dx[a]);
+                if (!(q[a] >= 0.0 && q[a] < (double)g.nex[a])) ok = 0;
+                idx[a] = ok ? (int)q[a] : 0;
+            }
+            if (!ok) continue;
+            size_t c = (size_t)idx[0] + (size_t)g.nex[0] * ((size_t)idx[1] + (size_t)g.nex[1] * idx[2]);
+            if (!seen[c]) {
+                seen[c] = 1;
+                nact++;
+            }
+        }
+        double ppc = nact > 0 ? (double)n / (double)nact : 1.0;
+        int lpc = 1;
+        while (lpc < 8 && 64.0 * ppc / lpc > 256.0) lpc *= 2;
+        e->lanes_per_cell = lpc;
+    }
     if (e->prec == MPM_PREC_FP64_EXACT) {
         std::vector<double> x(3 * ld), v(3 * ld, 0.0), sig(6 * ld, 0.0), F(9 * ld), m(ld), V(ld), V0(ld), J(ld),
             tr, ep(ld, 0.0);
@@ -553,18 +559,32 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
             if ((rc = h2d(e, P.matid, mi))) return rc;
         }
     } else {
-        std::vector<float> xh(3 * ld), xl(3 * ld), v(3 * ld, 0.f), sig(6 * ld, 0.f), H(9 * ld), m(ld), V0(ld),
-            ep(ld, 0.f), tr;
+        std::vector<float> rr(3 * ld), v(3 * ld, 0.f), sig(6 * ld, 0.f), H(9 * ld), m(ld), V0(ld), ep(ld, 0.f), tr;
         std::vector<double> V(ld), J(ld);
         std::vector<int> pidv(ld);
+        std::vector<uint32_t> keys(ld);
+        const GridGeom &g = e->g;
+        int outside = 0;
         for (int i = 0; i < n; i++) {
+            // cell with the reference's FP64 formula (material_point.cpp:59-65), offset inside it
+            int ijk[3];
+            bool ok = true;
             for (int a = 0; a < 3; a++) {
                 double xv = p->coord[3 * (size_t)i + a];
-                float h = (float)xv;
-                xh[a * ld + i] = h;
-                xl[a * ld + i] = (float)(xv - (double)h);
+                double q = std::floor((xv - g.xmin[a]) / g.dx[a]);
+                if (!(q >= 0.0 && q < (double)g.nex[a])) {
+                    ok = false;
+                    q = 0.0;
+                }
+                ijk[a] = (int)q;
+                float ra = (float)((xv - (g.xmin[a] + g.dx[a] * (double)ijk[a])) / g.dx[a]);
+                if (ra >= 1.f) ra = std::nextafter(1.f, 0.f);
+                if (ra < 0.f) ra = 0.f;
+                rr[a * ld + i] = ra;
                 if (p->vel) v[a * ld + i] = (float)p->vel[3 * (size_t)i + a];
             }
+            if (!ok) outside++;
+            keys[i] = ok ? (uint32_t)(ijk[0] + ijk[1] * g.nex[0] + ijk[2] * g.nex[0] * g.nex[1]) : (uint32_t)g.nelem;
             if (p->stress)
                 for (int q = 0; q < 6; q++) sig[q * ld + i] = (float)p->stress[6 * (size_t)i + q];
             for (int q = 0; q < 9; q++)
@@ -577,11 +597,12 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
             pidv[i] = i;
         }
         PartF32 &P = e->P32[0];
-        if ((rc = h2d(e, P.xh, xh)) || (rc = h2d(e, P.xl, xl)) || (rc = h2d(e, P.v, v)) ||
+        if ((rc = h2d(e, P.r, rr)) || (rc = h2d(e, P.v, v)) || (rc = h2d(e, P.key, keys)) ||
             (rc = h2d(e, P.sig, sig)) || (rc = h2d(e, P.H, H)) || (rc = h2d(e, P.m, m)) || (rc = h2d(e, P.V0, V0)) ||
             (rc = h2d(e, P.V, V)) || (rc = h2d(e, P.J, J)) || (rc = h2d(e, P.eps_p, ep)) ||
             (rc = h2d(e, e->pid[0], pidv)))
             return rc;
+        e->fp32_outside_at_upload = outside;
         if (e->has_trac) {
             tr.assign(3 * ld, 0.f);
             for (int i = 0; i < n; i++)
@@ -594,7 +615,7 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
         }
     }
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
-    e->keys_valid = false;
+    e->keys_valid = e->prec == MPM_PREC_FP32;
     e->bins_valid = false;
     e->steps_since_relayout = 0;
     return 0;
@@ -629,18 +650,24 @@ int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) {
         }
     } else {
         const PartF32 &P = e->P32[e->cur];
-        std::vector<float> xh, xl, v, sig, H, ep;
+        std::vector<float> rr, v, sig, H, ep;
         std::vector<double> V, J;
         std::vector<int> pidv;
-        if ((rc = d2h(e, xh, P.xh, 3 * ld)) || (rc = d2h(e, xl, P.xl, 3 * ld)) || (rc = d2h(e, v, P.v, 3 * ld)) ||
+        std::vector<uint32_t> keys;
+        if ((rc = d2h(e, rr, P.r, 3 * ld)) || (rc = d2h(e, v, P.v, 3 * ld)) || (rc = d2h(e, keys, P.key, ld)) ||
             (rc = d2h(e, sig, P.sig, 6 * ld)) || (rc = d2h(e, H, P.H, 9 * ld)) || (rc = d2h(e, V, P.V, ld)) ||
             (rc = d2h(e, J, P.J, ld)) || (rc = d2h(e, ep, P.eps_p, ld)) || (rc = d2h(e, pidv, e->pid[e->cur], ld)))
             return rc;
         MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        const GridGeom &g = e->g;
         for (int s = 0; s < n; s++) {
             const size_t i = (size_t)pidv[s];
+            const uint32_t k = keys[s] < (uint32_t)g.nelem ? keys[s] : 0u;
+            const int ijk[3] = {(int)(k % (uint32_t)g.nex[0]), (int)((k / (uint32_t)g.nex[0]) % (uint32_t)g.nex[1]),
+                                (int)(k / ((uint32_t)g.nex[0] * (uint32_t)g.nex[1]))};
             for (int a = 0; a < 3; a++) {
-                if (p->coord) p->coord[3 * i + a] = (double)xh[a * ld + s] + (double)xl[a * ld + s];
+                if (p->coord)
+                    p->coord[3 * i + a] = g.xmin[a] + g.dx[a] * ((double)ijk[a] + (double)rr[a * ld + s]);
                 if (p->vel) p->vel[3 * i + a] = v[a * ld + s];
             }
             if (p->stress)
@@ -869,6 +896,11 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     switch (key) {
     case OPT_FULL_NODE_OUTPUT: e->full_node_output = value ? 1 : 0; return 0;
+    case OPT_LANES_PER_CELL:
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+            return set_error(MPM_ERR_INVALID_ARGUMENT, "lanes per cell must be 1, 2, 4 or 8");
+        e->lanes_per_cell = (int)value;
+        return 0;
     default: return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown option");
     }
 }
@@ -968,6 +1000,40 @@ int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, i
         order[i] = (int32_t)vh[i];
     }
     return 0;
+}
+
+int mpm_bandwidth_probe(int64_t nbytes, int32_t iters, double *gbps) {
+    if (nbytes < 4096 || iters < 1 || !gbps) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad probe arguments");
+    const size_t n16 = (size_t)nbytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    hipError_t he = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipMalloc(&a, n16 * 16);
+    if (he == hipSuccess) he = hipMalloc(&b, n16 * 16);
+    if (he == hipSuccess) he = hipMemsetAsync(a, 1, n16 * 16, s);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he == hipSuccess) {
+        launch_copy16(s, a, b, n16);  // warm-up
+        he = hipEventRecord(e0, s);
+    }
+    if (he == hipSuccess) {
+        for (int i = 0; i < iters; i++) launch_copy16(s, (i & 1) ? b : a, (i & 1) ? a : b, n16);
+        he = hipEventRecord(e1, s);
+    }
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+    if (he != hipSuccess) rc = hip_fail(he, "mpm_bandwidth_probe", __FILE__, __LINE__);
+    else *gbps = 2.0 * (double)n16 * 16.0 * iters / (ms * 1e-3) / 1e9;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
 }
 
 }  // extern "C"

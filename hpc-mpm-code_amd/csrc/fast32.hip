@@ -1,10 +1,12 @@
 // FP32 explicit-MPM step kernels for gfx950 (wave64, 160 KB LDS per CU).
 //
 // Data layout (HBM, structure of arrays, one slot per particle): position as
-// an FP32 pair (xh + xl, compensated), velocity, mass, reference volume, the
-// displacement gradient H = F - I (so small strains do not cancel against
-// the identity), Cauchy stress, and FP64 det F and volume. Nodes are FP32
-// [u|v|w] blocks like the reference's node vectors.
+// the cell key plus an FP32 offset inside the cell in cell units
+// (x = xmin + dx (i + r), r in [0, 1), resolution 2^-24 of a cell),
+// velocity, mass, reference volume, the displacement gradient H = F - I (so
+// small strains do not cancel against the identity), Cauchy stress, and FP64
+// det F and volume. Nodes are FP32 [u|v|w] blocks like the reference's node
+// vectors.
 //
 // P2G is deterministic without float atomics: a 64-thread workgroup owns a
 // row segment of 64 consecutive cells along x. Their particles are one
@@ -26,9 +28,19 @@ namespace {
 constexpr int kSeg = 64;      // cells per row segment = threads per workgroup
 constexpr int kChunk = 256;   // particles staged per LDS pass
 
-__device__ __forceinline__ float local_coord(float xh, float xl, const float *tnh, const float *tnl, int idx,
-                                             float inv) {
-    return ((xh - tnh[idx]) + (xl - tnl[idx])) * inv;
+// q = a / n, rem = a % n for a < 2^24 via the float reciprocal, corrected to exact
+__device__ __forceinline__ int fdivmod(int a, int n, float inv_n, int &rem) {
+    int q = (int)((float)a * inv_n);
+    int r = a - q * n;
+    if (r < 0) {
+        q--;
+        r += n;
+    } else if (r >= n) {
+        q++;
+        r -= n;
+    }
+    rem = r;
+    return q;
 }
 
 // Neumaier (improved Kahan) compensated sum.
@@ -49,18 +61,24 @@ struct CellInfo {
     uint32_t cs, ce;
 };
 
-__device__ __forceinline__ void group_setup(const GridGeom &g, const CellLists &L, uint32_t n_active, int grp,
+// A group is G = 64 / L consecutive entries of the active-cell list; L lanes
+// work on each cell (lane = cell-in-group * L + sub). Because the list is in
+// ascending cell order and skips only empty cells, the particles of a group
+// are one contiguous range [S, E) of the sorted order.
+template <int L>
+__device__ __forceinline__ void group_setup(const GridGeom &g, const CellLists &L_, uint32_t n_active, int grp,
                                             CellInfo &ci, uint32_t &S, uint32_t &E) {
+    constexpr int G = kSeg / L;
     const int t = threadIdx.x;
-    const uint32_t a = (uint32_t)grp * kSeg + t;
+    const uint32_t a = (uint32_t)grp * G + t / L;
     ci.cs = 0xFFFFFFFFu;
     ci.ce = 0u;
     ci.c = -1;
     ci.ix = ci.iy = ci.iz = 0;
     if (a < n_active) {
-        ci.c = (int)L.active[a];
-        ci.cs = L.cstart[ci.c];
-        ci.ce = L.cend[ci.c];
+        ci.c = (int)L_.active[a];
+        ci.cs = L_.cstart[ci.c];
+        ci.ce = L_.cend[ci.c];
         const int nexy = g.nex[0] * g.nex[1];
         ci.iz = ci.c / nexy;
         const int r = ci.c - ci.iz * nexy;
@@ -76,6 +94,23 @@ __device__ __forceinline__ void group_setup(const GridGeom &g, const CellLists &
     S = smin;
     E = emax;
     if (ci.c < 0) ci.cs = ci.ce = 0u;
+}
+
+// first position >= lo of the arithmetic sequence cs + sub + k L
+template <int L>
+__device__ __forceinline__ uint32_t first_pos(uint32_t cs, uint32_t lo, int sub) {
+    uint32_t f = cs + (uint32_t)sub;
+    if (f < lo) f += ((lo - f + L - 1) / L) * L;
+    return f;
+}
+
+// combine the L lanes of a cell (fixed xor butterfly; every lane ends with the
+// same, order-independent value)
+template <int L>
+__device__ __forceinline__ float lane_combine(float v) {
+#pragma unroll
+    for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 // ------------------------------------------------------------------ P2G
@@ -102,14 +137,16 @@ __device__ __forceinline__ void lds_gather2(const double *field_base, uint32_t s
 // wait for every outstanding vector-memory load (including LDS DMA) of this wave
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
 
-template <bool DIAG, bool TRAC>
+template <int LPC, bool DIAG, bool TRAC>
 __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T,
                                                        float *__restrict__ part) {
-    // fields: 0-2 xh, 3-5 xl, 6-8 v, 9 m, 10-11 V (lo, hi dword), 12-17 sigma, [18-20 traction]
-    constexpr int NF = 18 + (TRAC ? 3 : 0);
+    constexpr int G = kSeg / LPC;
+    // fields: 0-2 r, 3-5 v, 6 m, 7-8 V (lo, hi dword), 9-14 sigma, [15-17 traction]
+    constexpr int NF = 15 + (TRAC ? 3 : 0);
     __shared__ float stage[NF * kBlk * 65];
     const uint32_t n_active = L.ctr->n_active;
-    const int ngroups = (int)((n_active + kSeg - 1) / kSeg);
+    const int ngroups = (int)((n_active + G - 1) / G);
+    const int sub = threadIdx.x % LPC;
     const int t = threadIdx.x;
     const size_t ld = P.ld;
     const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
@@ -117,13 +154,8 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
         CellInfo si;
         uint32_t S, E;
         __syncthreads();
-        group_setup(g, L, n_active, grp, si, S, E);
+        group_setup<LPC>(g, L, n_active, grp, si, S, E);
         __syncthreads();
-        // node coordinates of this thread's cell (lower corner)
-        const float cxh = T.tnh[si.ix], cxl = T.tnl[si.ix];
-        const float cyh = T.tnh[T.ld + si.iy], cyl = T.tnl[T.ld + si.iy];
-        const float czh = T.tnh[2 * T.ld + si.iz], czl = T.tnl[2 * T.ld + si.iz];
-
         float acc[8][7];
         float vof[8];
 #pragma unroll
@@ -144,42 +176,41 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                     const uint32_t sl = slot[r];
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
-                        lds_gather(P.xh + a * ld, sl, stage, a, r);
-                        lds_gather(P.xl + a * ld, sl, stage, 3 + a, r);
-                        lds_gather(P.v + a * ld, sl, stage, 6 + a, r);
+                        lds_gather(P.r + a * ld, sl, stage, a, r);
+                        lds_gather(P.v + a * ld, sl, stage, 3 + a, r);
                     }
-                    lds_gather(P.m, sl, stage, 9, r);
-                    lds_gather2(P.V, sl, stage, 10, r);
+                    lds_gather(P.m, sl, stage, 6, r);
+                    lds_gather2(P.V, sl, stage, 7, r);
 #pragma unroll
-                    for (int q = 0; q < 6; q++) lds_gather(P.sig + q * ld, sl, stage, 12 + q, r);
+                    for (int q = 0; q < 6; q++) lds_gather(P.sig + q * ld, sl, stage, 9 + q, r);
                     if (TRAC) {
 #pragma unroll
-                        for (int a = 0; a < 3; a++) lds_gather(P.trac + a * ld, sl, stage, 18 + a, r);
+                        for (int a = 0; a < 3; a++) lds_gather(P.trac + a * ld, sl, stage, 15 + a, r);
                     }
                 }
             }
             wait_vm_all();
             __syncthreads();
             const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
-            for (uint32_t pos = lo; pos < hi; pos++) {
+            for (uint32_t pos = first_pos<LPC>(si.cs, lo, sub); pos < hi; pos += LPC) {
                 const int i = (int)(pos - base);
-                const float fx = ((stage[st_idx(0, i)] - cxh) + (stage[st_idx(3, i)] - cxl)) * inv0;
-                const float fy = ((stage[st_idx(1, i)] - cyh) + (stage[st_idx(4, i)] - cyl)) * inv1;
-                const float fz = ((stage[st_idx(2, i)] - czh) + (stage[st_idx(5, i)] - czl)) * inv2;
-                const float m = stage[st_idx(9, i)];
-                const float px = m * stage[st_idx(6, i)], py = m * stage[st_idx(7, i)], pz = m * stage[st_idx(8, i)];
-                const double Vd = __hiloint2double(__float_as_int(stage[st_idx(11, i)]),
-                                                   __float_as_int(stage[st_idx(10, i)]));
+                const float fx = stage[st_idx(0, i)];
+                const float fy = stage[st_idx(1, i)];
+                const float fz = stage[st_idx(2, i)];
+                const float m = stage[st_idx(6, i)];
+                const float px = m * stage[st_idx(3, i)], py = m * stage[st_idx(4, i)], pz = m * stage[st_idx(5, i)];
+                const double Vd = __hiloint2double(__float_as_int(stage[st_idx(8, i)]),
+                                                   __float_as_int(stage[st_idx(7, i)]));
                 const float nV = -(float)Vd;
                 // Voigt xx,yy,zz,yz,xz,xy of -V sigma
-                const float sxx = nV * stage[st_idx(12, i)], syy = nV * stage[st_idx(13, i)];
-                const float szz = nV * stage[st_idx(14, i)], syz = nV * stage[st_idx(15, i)];
-                const float sxz = nV * stage[st_idx(16, i)], sxy = nV * stage[st_idx(17, i)];
+                const float sxx = nV * stage[st_idx(9, i)], syy = nV * stage[st_idx(10, i)];
+                const float szz = nV * stage[st_idx(11, i)], syz = nV * stage[st_idx(12, i)];
+                const float sxz = nV * stage[st_idx(13, i)], sxy = nV * stage[st_idx(14, i)];
                 float tx = 0.f, ty = 0.f, tz = 0.f;
                 if (TRAC) {
-                    tx = stage[st_idx(18, i)];
-                    ty = stage[st_idx(19, i)];
-                    tz = stage[st_idx(20, i)];
+                    tx = stage[st_idx(15, i)];
+                    ty = stage[st_idx(16, i)];
+                    tz = stage[st_idx(17, i)];
                 }
                 const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
 #pragma unroll
@@ -211,15 +242,25 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
             }
             __syncthreads();
         }
+        if (LPC > 1) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+#pragma unroll
+                for (int f = 0; f < 7; f++) acc[q][f] = lane_combine<LPC>(acc[q][f]);
+                if (DIAG) vof[q] = lane_combine<LPC>(vof[q]);
+            }
+        }
         if (si.ce > si.cs) {
             const size_t ne = (size_t)g.nelem;
 #pragma unroll
             for (int q = 0; q < 8; q++)
 #pragma unroll
-                for (int f = 0; f < 7; f++) part[(size_t)(q * 7 + f) * ne + si.c] = acc[q][f];
+                for (int f = 0; f < 7; f++)
+                    if ((q * 7 + f) % LPC == sub) part[(size_t)(q * 7 + f) * ne + si.c] = acc[q][f];
             if (DIAG) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) part[(size_t)(56 + q) * ne + si.c] = vof[q];
+                for (int q = 0; q < 8; q++)
+                    if (q % LPC == sub) part[(size_t)(56 + q) * ne + si.c] = vof[q];
             }
         }
     }  // groups
@@ -346,29 +387,27 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
 }
 
 // ------------------------------------------------------ G2P velocity + MUSL
-template <bool FLIP>
+template <int LPC, bool FLIP>
 __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T, NodesF32 N,
                                                         StepConsts sc, float *__restrict__ part) {
-    // staged fields: 0-2 xh, 3-5 xl, 6 m, [7-9 old v for FLIP]; results: G2P velocity per particle
-    constexpr int NF = 7 + (FLIP ? 3 : 0);
+    constexpr int G = kSeg / LPC;
+    // staged fields: 0-2 r, 3 m, [4-6 old v for FLIP]; results: G2P velocity per particle
+    constexpr int NF = 4 + (FLIP ? 3 : 0);
     __shared__ float stage[NF * kBlk * 65];
     __shared__ float vout[3 * kBlk * 65];
     const uint32_t n_active = L.ctr->n_active;
-    const int ngroups = (int)((n_active + kSeg - 1) / kSeg);
+    const int ngroups = (int)((n_active + G - 1) / G);
+    const int sub = threadIdx.x % LPC;
     const int t = threadIdx.x;
     const size_t ld = P.ld;
     const int nodec = g.nodec;
-    const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
     for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         CellInfo si;
         uint32_t S, E;
         __syncthreads();
-        group_setup(g, L, n_active, grp, si, S, E);
+        group_setup<LPC>(g, L, n_active, grp, si, S, E);
         __syncthreads();
         const bool ne = si.ce > si.cs;
-        const float cxh = T.tnh[si.ix], cxl = T.tnl[si.ix];
-        const float cyh = T.tnh[T.ld + si.iy], cyl = T.tnl[T.ld + si.iy];
-        const float czh = T.tnh[2 * T.ld + si.iz], czl = T.tnl[2 * T.ld + si.iz];
 
         // node values of this cell: PIC uses v + dt a (var_trans_solid_explicit.cpp:194-200),
         // FLIP uses a
@@ -413,22 +452,21 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                     const uint32_t sl = slot[r];
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
-                        lds_gather(P.xh + a * ld, sl, stage, a, r);
-                        lds_gather(P.xl + a * ld, sl, stage, 3 + a, r);
-                        if (FLIP) lds_gather(P.v + a * ld, sl, stage, 7 + a, r);
+                        lds_gather(P.r + a * ld, sl, stage, a, r);
+                        if (FLIP) lds_gather(P.v + a * ld, sl, stage, 4 + a, r);
                     }
-                    lds_gather(P.m, sl, stage, 6, r);
+                    lds_gather(P.m, sl, stage, 3, r);
                 }
             }
             wait_vm_all();
             __syncthreads();
             const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
-            for (uint32_t pos = lo; pos < hi; pos++) {
+            for (uint32_t pos = first_pos<LPC>(si.cs, lo, sub); pos < hi; pos += LPC) {
                 const int i = (int)(pos - base);
-                const float fx = ((stage[st_idx(0, i)] - cxh) + (stage[st_idx(3, i)] - cxl)) * inv0;
-                const float fy = ((stage[st_idx(1, i)] - cyh) + (stage[st_idx(4, i)] - cyl)) * inv1;
-                const float fz = ((stage[st_idx(2, i)] - czh) + (stage[st_idx(5, i)] - czl)) * inv2;
-                const float m = stage[st_idx(6, i)];
+                const float fx = stage[st_idx(0, i)];
+                const float fy = stage[st_idx(1, i)];
+                const float fz = stage[st_idx(2, i)];
+                const float m = stage[st_idx(3, i)];
                 const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
                 float Nw[8];
                 float vp[3] = {0.f, 0.f, 0.f};
@@ -444,9 +482,9 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                             for (int c = 0; c < 3; c++) vp[c] += Nw[q] * nvl[q][c];
                         }
                 if (FLIP) {
-                    vp[0] = stage[st_idx(7, i)] + sc.dt * vp[0];
-                    vp[1] = stage[st_idx(8, i)] + sc.dt * vp[1];
-                    vp[2] = stage[st_idx(9, i)] + sc.dt * vp[2];
+                    vp[0] = stage[st_idx(4, i)] + sc.dt * vp[0];
+                    vp[1] = stage[st_idx(5, i)] + sc.dt * vp[1];
+                    vp[2] = stage[st_idx(6, i)] + sc.dt * vp[2];
                 }
                 vout[st_idx(0, i)] = vp[0];
                 vout[st_idx(1, i)] = vp[1];
@@ -470,12 +508,19 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
             }
             __syncthreads();
         }
+        if (LPC > 1) {
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) acc[q][c] = lane_combine<LPC>(acc[q][c]);
+        }
         if (ne) {
             const size_t nel = (size_t)g.nelem;
 #pragma unroll
             for (int q = 0; q < 8; q++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) part[(size_t)(q * 3 + c) * nel + si.c] = acc[q][c];
+                for (int c = 0; c < 3; c++)
+                    if ((q * 3 + c) % LPC == sub) part[(size_t)(q * 3 + c) * nel + si.c] = acc[q][c];
         }
     }  // groups
 }
@@ -532,16 +577,6 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, T
 }
 
 // ------------------------------------------------------ particle update
-__device__ __forceinline__ void two_sum_update(float &xh, float &xl, float d) {
-    float s = xh + d;
-    float bp = s - xh;
-    float e = (xh - (s - bp)) + (d - bp);
-    float l = xl + e;
-    float t = s + l;
-    xl = l - (t - s);
-    xh = t;
-}
-
 // Stress update in FP32 from H = F - I and the increment D = dF - I
 // (semantics of ConstitutiveModel::UpdateStress, constitutive_model.cpp:287-305,
 // rewritten without cancellation against the identity).
@@ -691,19 +726,13 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     const size_t ld = P.ld;
     const int nodec = g.nodec;
     const int nexy = g.nex[0] * g.nex[1];
-    const int iz = (int)key / nexy;
-    const int rem = (int)key - iz * nexy;
-    const int iy = rem / g.nex[0];
-    const int ix = rem - iy * g.nex[0];
-    float xh[3], xl[3];
+    int rem, ix, iy;
+    const int iz = fdivmod((int)key, nexy, T.inv_nexy, rem);
+    iy = fdivmod(rem, g.nex[0], T.inv_nex, ix);
+    float r[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        xh[a] = P.xh[a * ld + slot];
-        xl[a] = P.xl[a * ld + slot];
-    }
-    const float fx = local_coord(xh[0], xl[0], T.tnh, T.tnl, ix, T.inv_dx[0]);
-    const float fy = local_coord(xh[1], xl[1], T.tnh + T.ld, T.tnl + T.ld, iy, T.inv_dx[1]);
-    const float fz = local_coord(xh[2], xl[2], T.tnh + 2 * T.ld, T.tnl + 2 * T.ld, iz, T.inv_dx[2]);
+    for (int a = 0; a < 3; a++) r[a] = P.r[a * ld + slot];
+    const float fx = r[0], fy = r[1], fz = r[2];
     const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
     const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
     const int n0 = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
@@ -757,9 +786,22 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     const double J = A[0] * (A[4] * A[8] - A[5] * A[7]) + A[1] * (A[5] * A[6] - A[3] * A[8]) +
                      A[2] * (A[3] * A[7] - A[4] * A[6]);
     const double V = J * (double)P.V0[slot];
-    // position, compensated
+    // position: offset in cell units, carried into the cell index when it leaves [0, 1)
+    int ci[3] = {ix, iy, iz};
+    bool inside = true;
 #pragma unroll
-    for (int a = 0; a < 3; a++) two_sum_update(xh[a], xl[a], sc.dt * dxp[a]);
+    for (int a = 0; a < 3; a++) {
+        float ra = r[a] + (sc.dt * dxp[a]) * T.inv_dx[a];
+        const float fl = floorf(ra);
+        ra -= fl;  // exact for |fl| <= 1 (Sterbenz)
+        ci[a] += (int)fl;
+        if (ra >= 1.f) {  // -tiny + 1 rounded up to 1: the particle sits on the next cell's lower face
+            ra = 0.f;
+            ci[a] += 1;
+        }
+        r[a] = ra;
+        inside = inside && ci[a] >= 0 && ci[a] < g.nex[a];
+    }
     const int mat = P.matid ? P.matid[slot] : 0;
     const MaterialDev &md = mt.m[mat];
     // only the rate forms (linear elastic increment, J2) read the old stress
@@ -777,19 +819,15 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
 #pragma unroll
     for (int q = 0; q < 6; q++) P.sig[q * ld + slot] = s[q];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        P.xh[a * ld + slot] = xh[a];
-        P.xl[a * ld + slot] = xl[a];
-    }
+    for (int a = 0; a < 3; a++) P.r[a * ld + slot] = r[a];
     P.J[slot] = J;
     P.V[slot] = V;
-    const uint32_t nk = cell_key_f64(g, (double)xh[0] + (double)xl[0], (double)xh[1] + (double)xl[1],
-                                     (double)xh[2] + (double)xl[2]);
+    const uint32_t nk = inside ? (uint32_t)(ci[0] + ci[1] * g.nex[0] + ci[2] * nexy) : (uint32_t)g.nelem;
     P.key[slot] = nk;
     uint32_t e = 0;
-    if (nk == (uint32_t)g.nelem) e |= kErrOutOfGrid;
+    if (!inside) e |= kErrOutOfGrid;
     if (!(J > 0.0)) e |= kErrJacobian;
-    if (!isfinite(xh[0] + xh[1] + xh[2] + s[0] + s[1] + s[2])) e |= kErrNonFinite;
+    if (!isfinite(r[0] + r[1] + r[2] + s[0] + s[1] + s[2])) e |= kErrNonFinite;
     if (e) atomicOr(err, e);
 }
 
@@ -800,8 +838,7 @@ __global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict
     const uint32_t o = perm[s];
     const size_t ls = src.ld, lt = dst.ld;
     for (int a = 0; a < 3; a++) {
-        dst.xh[a * lt + s] = src.xh[a * ls + o];
-        dst.xl[a * lt + s] = src.xl[a * ls + o];
+        dst.r[a * lt + s] = src.r[a * ls + o];
         dst.v[a * lt + s] = src.v[a * ls + o];
         if (src.trac) dst.trac[a * lt + s] = src.trac[a * ls + o];
     }
@@ -820,8 +857,24 @@ __global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict
 }  // namespace
 
 static int cell_groups_grid(const GridGeom &g) {
-    int ng = cdiv(g.nelem, kSeg);
-    return ng < 4096 ? ng : 4096;
+    int ng = cdiv(g.nelem, 16);
+    return ng < 8192 ? ng : 8192;
+}
+
+template <int LPC>
+static void p2g_cells_launch(hipStream_t s, int ng, bool diag, bool trac, const GridGeom &g, const PartF32 &P,
+                             const CellLists &L, const NodeTables &T, float *partials) {
+    if (diag) {
+        if (trac)
+            hipLaunchKernelGGL((k32_p2g_cells<LPC, true, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
+        else
+            hipLaunchKernelGGL((k32_p2g_cells<LPC, true, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
+    } else {
+        if (trac)
+            hipLaunchKernelGGL((k32_p2g_cells<LPC, false, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
+        else
+            hipLaunchKernelGGL((k32_p2g_cells<LPC, false, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
+    }
 }
 
 void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
@@ -829,33 +882,40 @@ void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const No
                     int diag, hipEvent_t mid) {
     const int ng = cell_groups_grid(g);
     const bool trac = P.trac != nullptr;
-    if (diag) {
-        if (trac)
-            hipLaunchKernelGGL((k32_p2g_cells<true, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
-        else
-            hipLaunchKernelGGL((k32_p2g_cells<true, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
-        if (mid) (void)hipEventRecord(mid, s);
+    switch (c.lanes_per_cell) {
+    case 8: p2g_cells_launch<8>(s, ng, diag, trac, g, P, L, T, partials); break;
+    case 4: p2g_cells_launch<4>(s, ng, diag, trac, g, P, L, T, partials); break;
+    case 2: p2g_cells_launch<2>(s, ng, diag, trac, g, P, L, T, partials); break;
+    default: p2g_cells_launch<1>(s, ng, diag, trac, g, P, L, T, partials); break;
+    }
+    if (mid) (void)hipEventRecord(mid, s);
+    if (diag)
         hipLaunchKernelGGL(k32_p2g_nodes<true>, dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials, N,
                            bc, c);
-    } else {
-        if (trac)
-            hipLaunchKernelGGL((k32_p2g_cells<false, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
-        else
-            hipLaunchKernelGGL((k32_p2g_cells<false, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
-        if (mid) (void)hipEventRecord(mid, s);
+    else
         hipLaunchKernelGGL(k32_p2g_nodes<false>, dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials,
                            N, bc, c);
-    }
+}
+
+template <int LPC>
+static void musl_cells_launch(hipStream_t s, int ng, const GridGeom &g, const PartF32 &P, const CellLists &L,
+                              const NodeTables &T, const NodesF32 &N, const StepConsts &c, float *partials) {
+    if (c.scheme == 1)
+        hipLaunchKernelGGL((k32_musl_cells<LPC, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
+    else
+        hipLaunchKernelGGL((k32_musl_cells<LPC, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
 }
 
 void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                              const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
                              const StepConsts &c, float *partials, int diag, hipEvent_t mid) {
     const int ng = cell_groups_grid(g);
-    if (c.scheme == 1)
-        hipLaunchKernelGGL(k32_musl_cells<true>, dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
-    else
-        hipLaunchKernelGGL(k32_musl_cells<false>, dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
+    switch (c.lanes_per_cell) {
+    case 8: musl_cells_launch<8>(s, ng, g, P, L, T, N, c, partials); break;
+    case 4: musl_cells_launch<4>(s, ng, g, P, L, T, N, c, partials); break;
+    case 2: musl_cells_launch<2>(s, ng, g, P, L, T, N, c, partials); break;
+    default: musl_cells_launch<1>(s, ng, g, P, L, T, N, c, partials); break;
+    }
     if (mid) (void)hipEventRecord(mid, s);
     if (diag)
         hipLaunchKernelGGL(k32_musl_nodes<true>, dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials, N,

@@ -50,9 +50,9 @@ enum : uint32_t {
 
 // Node tiles of the node passes (x fastest); a tile is processed only when a
 // cell touching one of its nodes holds particles.
-constexpr int kNodeTileX = 16;
-constexpr int kNodeTileY = 4;
-constexpr int kNodeTileZ = 4;
+constexpr int kNodeTileX = 64;
+constexpr int kNodeTileY = 2;
+constexpr int kNodeTileZ = 2;
 
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 

@@ -16,6 +16,7 @@ struct SortBuffers {
 };
 
 size_t radix_hist_entries(int n);
+void launch_copy16(hipStream_t s, const void *src, void *dst, size_t n16);
 int excl_scan_u32(hipStream_t s, uint32_t *data, int n, uint32_t *bsum);
 // Stable LSD radix sort of (keys_in[i], i) over nbits key bits (standalone
 // binning entry point). keys_in is not
@@ -25,8 +26,6 @@ int radix_sort_pairs(hipStream_t s, SortBuffers &b, const uint32_t *keys_in, int
 void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
 void launch_keys_f64(hipStream_t s, const GridGeom &g, const double *x, size_t ld, int n, uint32_t *keys,
                      uint32_t *err);
-void launch_keys_hilo(hipStream_t s, const GridGeom &g, const float *xh, const float *xl, size_t ld, int n,
-                      uint32_t *keys, uint32_t *err);
 
 // Cell lists produced by the binning: particles of cell c are
 // perm[cstart[c] .. cend[c]) (cstart == cend for an empty cell); active[] lists
@@ -111,8 +110,7 @@ int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cel
 
 // ------------------------------------------------------------ FP32 kernels
 struct PartF32 {
-    float *xh;      // [3][ld] position, leading part
-    float *xl;      // [3][ld] position, compensation part
+    float *r;       // [3][ld] position inside the cell, in cell units: x = xmin + dx (i + r)
     float *v;       // [3][ld]
     float *m;
     float *V0;
@@ -146,15 +144,15 @@ struct StepConsts {
     double facl;
     float m_small;     // below this node mass the division runs in FP64
     int scheme;
+    int lanes_per_cell;  // 1, 2, 4 or 8 lanes fold one cell's particles in the cells kernels
 };
 
-// FP32 node coordinates as (hi, lo) float pairs per axis, [3][ld], so that
-// particle positions relative to a node keep FP32 precision of the offset.
+// Grid spacing constants of the FP32 kernels.
 struct NodeTables {
-    const float *tnh;
-    const float *tnl;
-    int ld;
     float inv_dx[3];
+    float dx[3];
+    float inv_nex;   // 1 / nex and 1 / (nex ney) for the cell-index split
+    float inv_nexy;
 };
 
 // Partials buffer: 64 floats per cell: 8 corners x {m, px, py, pz, fx, fy, fz}

@@ -193,19 +193,21 @@ __global__ void k_keys_f64(GridGeom g, const double *__restrict__ x, size_t ld, 
     if (k == (uint32_t)g.nelem) atomicOr(err, kErrOutOfGrid);
 }
 
-__global__ void k_keys_hilo(GridGeom g, const float *__restrict__ xh, const float *__restrict__ xl, size_t ld, int n,
-                            uint32_t *__restrict__ keys, uint32_t *__restrict__ err) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    double x = (double)xh[p] + (double)xl[p];
-    double y = (double)xh[ld + p] + (double)xl[ld + p];
-    double z = (double)xh[2 * ld + p] + (double)xl[2 * ld + p];
-    uint32_t k = cell_key_f64(g, x, y, z);
-    keys[p] = k;
-    if (k == (uint32_t)g.nelem) atomicOr(err, kErrOutOfGrid);
+
+// Plain copy of n 16-byte words (HBM bandwidth probe).
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
 }
 
 }  // namespace
+
+void launch_copy16(hipStream_t s, const void *src, void *dst, size_t n16) {
+    int blocks = (int)((n16 + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, n16);
+}
 
 size_t radix_hist_entries(int n) { return (size_t)256 * (size_t)cdiv(n > 0 ? n : 1, kTile); }
 
@@ -275,10 +277,5 @@ void launch_keys_f64(hipStream_t s, const GridGeom &g, const double *x, size_t l
     hipLaunchKernelGGL(k_keys_f64, dim3(cdiv(n, 256)), dim3(256), 0, s, g, x, ld, n, keys, err);
 }
 
-void launch_keys_hilo(hipStream_t s, const GridGeom &g, const float *xh, const float *xl, size_t ld, int n,
-                      uint32_t *keys, uint32_t *err) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_keys_hilo, dim3(cdiv(n, 256)), dim3(256), 0, s, g, xh, xl, ld, n, keys, err);
-}
 
 }  // namespace mpm

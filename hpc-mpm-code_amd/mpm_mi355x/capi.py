@@ -26,6 +26,7 @@ SCHEME_PIC, SCHEME_FLIP = 0, 1
 PREC_FP64_EXACT, PREC_FP32 = 0, 1
 MODEL_RIGID, MODEL_LINELAST, MODEL_NEOHOOKEAN, MODEL_STVK, MODEL_MOONEYRIVLIN, MODEL_J2 = -1, 0, 1, 2, 3, 10
 OPT_FULL_NODE_OUTPUT = 1
+OPT_LANES_PER_CELL = 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -87,6 +88,7 @@ SIGNATURES = {
     "mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "mpm_maksf_batch": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _ip, _dp, _ip, _dp, _dp]),
     "mpm_bin_positions": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _dp, _ip, _ip]),
+    "mpm_bandwidth_probe": (C.c_int, [C.c_int64, C.c_int32, _dp]),
 }
 
 _lib = None
@@ -316,3 +318,10 @@ def bin_positions(grid, xyg):
     order = np.zeros(n, np.int32)
     check(lib.mpm_bin_positions(C.byref(grid), n, _ptr(xyg), _ptr(cell, C.c_int32), _ptr(order, C.c_int32)))
     return cell, order
+
+
+def bandwidth_probe(nbytes=1 << 30, iters=20):
+    """Achieved copy bandwidth in GB/s (read + write) of a plain copy kernel."""
+    out = C.c_double()
+    check(load().mpm_bandwidth_probe(int(nbytes), int(iters), C.byref(out)))
+    return out.value

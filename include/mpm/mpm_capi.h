@@ -197,8 +197,10 @@ int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
  * each mpm_step call also writes node momentum, force and volume, so that
  * mpm_download_nodes returns every field; 0 skips that work in every step
  * (FP32 precision only; node arrays are then valid at nodes next to
- * particles). */
-enum { MPM_OPT_FULL_NODE_OUTPUT = 1 };
+ * particles). MPM_OPT_LANES_PER_CELL (1, 2, 4, 8; set from the particles per
+ * cell at upload): lanes of a wave that fold one cell's particles in the FP32
+ * cells kernels. It changes only the FP32 summation order. */
+enum { MPM_OPT_FULL_NODE_OUTPUT = 1, MPM_OPT_LANES_PER_CELL = 2 };
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 /* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
 int mpm_set_timing(mpm_engine *eng, int32_t enabled);
@@ -213,6 +215,13 @@ int mpm_maksf_batch(const mpm_grid_desc *grid, int64_t n, const int32_t *cell, c
  * xyg[n][3] host FP64 -> cell_of[n] (-1 outside the grid), order[n] =
  * particle indices sorted by (cell, index). */
 int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, int32_t *cell_of, int32_t *order);
+
+/* Measurement helper with no counterpart in the reference: copies nbytes
+ * device-to-device iters times with a plain 16-byte-per-lane copy kernel and
+ * returns the achieved bandwidth (bytes read + written per second / 1e9).
+ * Gives the measured HBM ceiling next to the vendor peak, and the reference
+ * kernel for scaling rocprof byte counters. */
+int mpm_bandwidth_probe(int64_t nbytes, int32_t iters, double *gbps);
 
 #ifdef __cplusplus
 }

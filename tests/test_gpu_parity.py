@@ -101,10 +101,11 @@ def test_engine_bins_equal_reference_linked_lists(prec):
     assert np.array_equal(idp2p, o.idp2p)
 
 
-def test_out_of_grid_is_an_error():
+@pytest.mark.parametrize("prec", [capi.PREC_FP64_EXACT, capi.PREC_FP32])
+def test_out_of_grid_is_an_error(prec):
     s = scenes.tiny(n_cells=8)
     s.parts.coord[5] = [1.5, 0.2, 0.2]
-    e = engine(s)
+    e = engine(s, prec)
     with pytest.raises(capi.MPMError) as ei:
         e.bin()
     assert ei.value.code == capi.MPM_ERR_OUT_OF_GRID
@@ -201,7 +202,8 @@ def test_fp32_per_pass_against_oracle():
     o.g2p(1.0)
     e.g2p(1.0)
     got = e.download(s.n)
-    assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-9
+    # FP32 positions are a cell-relative offset: resolution h * 2^-24 (5e-9 on this 12^3 grid)
+    assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-8
     assert rel(got.vel, o.parts.vel) < 1e-5
     assert rel(got.stress, o.parts.stress) < 1e-4
 
@@ -218,6 +220,16 @@ def test_fp32_cfg1_1000_steps():
     assert rel(got.stress, o.parts.stress) < 1e-3   # stress, relative Frobenius over all particles
     nd = e.nodes()
     assert rel(nd.nmome, o.nmome) < 1e-3            # node momentum, relative L2
+
+
+def test_fp32_particle_leaving_the_grid_is_an_error():
+    s = scenes.tiny(n_cells=8)
+    s.bcs = {}
+    s.parts.coord[7] = [0.999, 0.5, 0.5]
+    s.parts.vel[7] = [50.0, 0.0, 0.0]   # crosses x = 1 within a few steps
+    e = engine(s, capi.PREC_FP32)
+    rc = e.step(1, 10)
+    assert rc == capi.MPM_ERR_OUT_OF_GRID
 
 
 def test_fp32_deterministic():
@@ -262,3 +274,20 @@ def test_engine_bins_crowded_cells(prec):
     assert np.array_equal(numep, o.numep)
     assert np.array_equal(idepf, o.idepf)
     assert np.array_equal(idp2p, o.idp2p)
+
+
+def test_fp32_lanes_per_cell_variants_agree():
+    """1, 2, 4 or 8 lanes per cell change only the FP32 summation order."""
+    s = scenes.tiny(n_cells=16, ppc=(2, 4, 2))
+    o = oracle_run(s, 20)
+    outs = []
+    for lpc in (1, 2, 4, 8):
+        e = engine(s, capi.PREC_FP32)
+        e.set_option(capi.OPT_LANES_PER_CELL, lpc)
+        assert e.step(1, 20) == 0
+        outs.append(e.download(s.n))
+    for g in outs:
+        # offset resolution h * 2^-24 = 3.7e-9 on this 16^3 grid, 20 steps
+        assert np.max(np.abs(g.coord - o.parts.coord)) < 5e-8
+        assert rel(g.stress, o.parts.stress) < 1e-4
+        assert np.max(np.abs(g.coord - outs[0].coord)) < 2e-8
