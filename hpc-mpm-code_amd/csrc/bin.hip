@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(kBlock) k_bin_reduce(const uint32_t *__restric
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(kBlock) k_bin_top(uint64_t *__restrict__ bsum, int nb, BinCounters *ctr) {
+__global__ void __launch_bounds__(kBlock) k_bin_top(uint64_t *__restrict__ bsum, int nb, BinCounters *ctr,
+                                                     uint32_t *__restrict__ astart) {
     const int base = threadIdx.x * kItems;
     uint64_t v[kItems];
     uint64_t s = 0;
@@ -131,7 +132,10 @@ __global__ void __launch_bounds__(kBlock) k_bin_top(uint64_t *__restrict__ bsum,
         if (base + i < nb) bsum[base + i] = pre;
         pre += v[i];
     }
-    if (threadIdx.x == 0) ctr->n_active = (uint32_t)(tot >> 32);
+    if (threadIdx.x == 0) {
+        ctr->n_active = (uint32_t)(tot >> 32);
+        astart[tot >> 32] = (uint32_t)tot;
+    }
 }
 
 __device__ __forceinline__ void mark_cell(const GridGeom &g, const TileGeom &tg, uint32_t *tiles, int c) {
@@ -154,8 +158,8 @@ __device__ __forceinline__ void mark_cell(const GridGeom &g, const TileGeom &tg,
 __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *__restrict__ cnt,
                                                           const uint64_t *__restrict__ bsum,
                                                           uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend,
-                                                          uint32_t *__restrict__ active, uint32_t *__restrict__ tiles,
-                                                          TileGeom tg) {
+                                                          uint32_t *__restrict__ active, uint32_t *__restrict__ astart,
+                                                          uint32_t *__restrict__ tiles, TileGeom tg) {
     const size_t tbase = (size_t)blockIdx.x * kTile;
     uint4 *c4 = (uint4 *)(cnt + tbase);
     uint4 *s4 = (uint4 *)(cstart + tbase);
@@ -176,6 +180,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
             if (vv[q]) {
                 const int c = (int)(tbase + 4 * idx + q);
                 active[(uint32_t)(pre >> 32)] = (uint32_t)c;
+                astart[(uint32_t)(pre >> 32)] = st[q];
                 mark_cell(g, tg, tiles, c);
                 any = true;
             }
@@ -295,9 +300,9 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
     if (pblocks > 8192) pblocks = 8192;
     hipLaunchKernelGGL(k_bin_count, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, b.cnt, b.tiles, tg.ntiles, b.ctr);
     hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
-    hipLaunchKernelGGL(k_bin_top, dim3(1), dim3(kBlock), 0, s, b.bsum, nb, b.ctr);
+    hipLaunchKernelGGL(k_bin_top, dim3(1), dim3(kBlock), 0, s, b.bsum, nb, b.ctr, b.astart);
     hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, b.cstart,
-                       b.cend, b.active, b.tiles, tg);
+                       b.cend, b.active, b.astart, b.tiles, tg);
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, (const uint32_t *)b.cstart,
                            b.cend, b.perm, b.stamp, stamp, b.fixlist, b.ctr);

@@ -131,7 +131,7 @@ struct mpm_engine {
     }
 
     BcDense bc() const { return BcDense{d_bcmask, d_bcval}; }
-    CellLists lists() const { return CellLists{perm, bb.cstart, bb.cend, bb.active, bb.ctr, bb.tiles}; }
+    CellLists lists() const { return CellLists{perm, bb.cstart, bb.cend, bb.active, bb.astart, bb.ctr, bb.tiles}; }
 };
 
 namespace {
@@ -408,7 +408,8 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     const size_t ncap = bin_cell_capacity(e->g.nelem);
     if ((rc = e->dalloc(&e->d_bcmask, nodec)) || (rc = e->dalloc(&e->d_bcval, 3 * nodec)) ||
         (rc = e->dalloc(&b.cnt, ncap)) || (rc = e->dalloc(&b.cstart, ncap)) || (rc = e->dalloc(&b.cend, ncap)) ||
-        (rc = e->dalloc(&b.active, nelem)) || (rc = e->dalloc(&b.stamp, nelem)) ||
+        (rc = e->dalloc(&b.active, nelem)) || (rc = e->dalloc(&b.astart, nelem + 1)) ||
+        (rc = e->dalloc(&b.stamp, nelem)) ||
         (rc = e->dalloc(&b.fixlist, nelem)) || (rc = e->dalloc(&b.tiles, (size_t)e->tg.ntiles)) ||
         (rc = e->dalloc(&b.bsum, bin_scan_blocks(e->g.nelem))) || (rc = e->dalloc(&b.ctr, 1)) ||
         (rc = e->dalloc(&e->d_err, 1))) {

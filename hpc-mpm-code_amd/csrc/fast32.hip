@@ -70,30 +70,24 @@ __device__ __forceinline__ void group_setup(const GridGeom &g, const CellLists &
                                             CellInfo &ci, uint32_t &S, uint32_t &E) {
     constexpr int G = kSeg / L;
     const int t = threadIdx.x;
-    const uint32_t a = (uint32_t)grp * G + t / L;
-    ci.cs = 0xFFFFFFFFu;
-    ci.ce = 0u;
+    const uint32_t a0 = (uint32_t)grp * G;
+    const uint32_t a = a0 + t / L;
+    const uint32_t a_end = min(a0 + (uint32_t)G, n_active);
+    S = L_.astart[a0];
+    E = L_.astart[a_end];
     ci.c = -1;
+    ci.cs = ci.ce = 0u;
     ci.ix = ci.iy = ci.iz = 0;
     if (a < n_active) {
         ci.c = (int)L_.active[a];
-        ci.cs = L_.cstart[ci.c];
-        ci.ce = L_.cend[ci.c];
+        ci.cs = L_.astart[a];
+        ci.ce = L_.astart[a + 1];
         const int nexy = g.nex[0] * g.nex[1];
         ci.iz = ci.c / nexy;
         const int r = ci.c - ci.iz * nexy;
         ci.iy = r / g.nex[0];
         ci.ix = r - ci.iy * g.nex[0];
     }
-    uint32_t smin = ci.cs, emax = ci.ce;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        smin = min(smin, (uint32_t)__shfl_xor((int)smin, o, 64));
-        emax = max(emax, (uint32_t)__shfl_xor((int)emax, o, 64));
-    }
-    S = smin;
-    E = emax;
-    if (ci.c < 0) ci.cs = ci.ce = 0u;
 }
 
 // first position >= lo of the arithmetic sequence cs + sub + k L
@@ -202,10 +196,14 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                 const double Vd = __hiloint2double(__float_as_int(stage[st_idx(8, i)]),
                                                    __float_as_int(stage[st_idx(7, i)]));
                 const float nV = -(float)Vd;
-                // Voigt xx,yy,zz,yz,xz,xy of -V sigma
-                const float sxx = nV * stage[st_idx(9, i)], syy = nV * stage[st_idx(10, i)];
-                const float szz = nV * stage[st_idx(11, i)], syz = nV * stage[st_idx(12, i)];
-                const float sxz = nV * stage[st_idx(13, i)], sxy = nV * stage[st_idx(14, i)];
+                // -V sigma (Voigt xx,yy,zz,yz,xz,xy) scaled by 1/dx of the derivative direction
+                const float s0 = nV * stage[st_idx(9, i)], s1 = nV * stage[st_idx(10, i)];
+                const float s2 = nV * stage[st_idx(11, i)], s3 = nV * stage[st_idx(12, i)];
+                const float s4 = nV * stage[st_idx(13, i)], s5 = nV * stage[st_idx(14, i)];
+                // force on a corner: f_a = sum_b (-V sigma_ab) dN/dx_b, dN/dx = (sx/dx wy wz, sy/dy wx wz, sz/dz wx wy)
+                const float X0 = s0 * inv0, X1 = s5 * inv0, X2 = s4 * inv0;  // b = x: sigma_xx, sigma_yx, sigma_zx
+                const float Y0 = s5 * inv1, Y1 = s1 * inv1, Y2 = s3 * inv1;  // b = y
+                const float Z0 = s4 * inv2, Z1 = s3 * inv2, Z2 = s2 * inv2;  // b = z
                 float tx = 0.f, ty = 0.f, tz = 0.f;
                 if (TRAC) {
                     tx = stage[st_idx(15, i)];
@@ -213,6 +211,15 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                     tz = stage[st_idx(17, i)];
                 }
                 const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
+                float wyz[2][2], wxz[2][2], wxy[2][2];
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                    for (int w = 0; w < 2; w++) {
+                        wyz[u][w] = wy[u] * wz[w];
+                        wxz[u][w] = wx[u] * wz[w];
+                        wxy[u][w] = wx[u] * wy[w];
+                    }
 #pragma unroll
                 for (int k = 0; k < 2; k++)
 #pragma unroll
@@ -220,18 +227,17 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
 #pragma unroll
                         for (int ii = 0; ii < 2; ii++) {
                             const int q = ii + 2 * j + 4 * k;
-                            const float wyz = wy[j] * wz[k];
-                            const float N = wx[ii] * wyz;
-                            const float gx = (ii ? inv0 : -inv0) * wyz;
-                            const float gy = (j ? inv1 : -inv1) * (wx[ii] * wz[k]);
-                            const float gz = (k ? inv2 : -inv2) * (wx[ii] * wy[j]);
+                            const float N = wx[ii] * wyz[j][k];
+                            const float gx = ii ? wyz[j][k] : -wyz[j][k];
+                            const float gy = j ? wxz[ii][k] : -wxz[ii][k];
+                            const float gz = k ? wxy[ii][j] : -wxy[ii][j];
                             acc[q][0] += N * m;
                             acc[q][1] += N * px;
                             acc[q][2] += N * py;
                             acc[q][3] += N * pz;
-                            acc[q][4] += sxx * gx + sxy * gy + sxz * gz;
-                            acc[q][5] += sxy * gx + syy * gy + syz * gz;
-                            acc[q][6] += sxz * gx + syz * gy + szz * gz;
+                            acc[q][4] += X0 * gx + Y0 * gy + Z0 * gz;
+                            acc[q][5] += X1 * gx + Y1 * gy + Z1 * gz;
+                            acc[q][6] += X2 * gx + Y2 * gy + Z2 * gz;
                             if (TRAC) {
                                 acc[q][4] += N * tx;
                                 acc[q][5] += N * ty;

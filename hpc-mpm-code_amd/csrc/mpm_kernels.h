@@ -42,6 +42,7 @@ struct CellLists {
     const uint32_t *cstart;
     const uint32_t *cend;
     const uint32_t *active;
+    const uint32_t *astart;  // [n_active + 1]: start of each active cell, then the binned total
     const BinCounters *ctr;
     const uint32_t *tiles;
 };
@@ -55,6 +56,7 @@ struct BinBuffers {
     uint32_t *cstart;   // [bin_cell_capacity]
     uint32_t *cend;     // [bin_cell_capacity]
     uint32_t *active;   // [nelem]
+    uint32_t *astart;   // [nelem + 1]
     uint32_t *stamp;    // [nelem], fix-up queue marks
     uint32_t *fixlist;  // [nelem]
     uint32_t *perm;     // [n]
