@@ -61,6 +61,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes-per-cell", type=int, default=0, help="override (1, 2, 4, 8); 0 = automatic")
     ap.add_argument("--no-probe", action="store_true", help="skip the copy-bandwidth probe")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the J2 surrogate vs plain comparison")
+    ap.add_argument("--sur-collect", type=int, default=200, help="logged plain steps before the fit")
+    ap.add_argument("--sur-samples", type=int, default=5_000_000)
+    ap.add_argument("--sur-steps", type=int, default=100, help="timed steps of each method")
+    ap.add_argument("--sur-epochs", type=int, default=40)
+    ap.add_argument("--sur-batch", type=int, default=4096)
+    ap.add_argument("--sur-tol", type=float, default=1e-4, help="the stated fallback tolerance")
+    ap.add_argument("--sur-sweep", default="1e-3,1e-2,1e30", help="further tolerances reported")
     return ap.parse_args()
 
 
@@ -87,6 +95,76 @@ def cpu_baseline(scene, nsteps):
     return {"value": scene.n * nsteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": "%s, %d particles, %d steps (FP64 oracle, -O3 -march=native, 1 thread), %.1f s" % (
                 scene.name, scene.n, nsteps, dt)}
+
+
+def timed_steps(eng, istep0, nsteps):
+    capi.check(eng.lib.mpm_sync(eng.h))
+    t0 = time.perf_counter()
+    capi.check(eng.step(istep0, nsteps))
+    capi.check(eng.lib.mpm_sync(eng.h))
+    return time.perf_counter() - t0
+
+
+def surrogate_vs_plain(a):
+    """J2 column (192x96x96, 2.43 M particles): the plain method logs plastic
+    trial states for the first steps, the network is fitted on the GPU, then
+    both methods advance the same state; stress error is that of the
+    surrogate run against the plain run (Frobenius over all particles)."""
+    scene = scenes.cfg3()
+    n = scene.n
+    eng = capi.Engine(scene, precision=capi.PREC_FP32, relayout_every=a.relayout)
+    eng.upload(scene.parts)
+    eng.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
+    rate = min(1.0, a.sur_samples / float(n * a.sur_collect))
+    eng.surrogate_collect(True, capacity=a.sur_samples, rate=rate)
+    t_collect = timed_steps(eng, 1, a.sur_collect)
+    t0 = time.perf_counter()
+    loss = eng.surrogate_fit(epochs=a.sur_epochs, batch=a.sur_batch, lr=2e-3, lr_final=5e-5, seed=1)
+    t_fit = time.perf_counter() - t0
+    st = eng.surrogate_stats()
+    params, scaling = eng.surrogate_params()
+    state = eng.download(n)
+    eng.close()
+    i0 = 1 + a.sur_collect
+
+    def advance(tol):
+        e = capi.Engine(scene, precision=capi.PREC_FP32, relayout_every=a.relayout)
+        e.upload(state)
+        e.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
+        if tol is not None:
+            e.surrogate_set_params(params, scaling)
+            e.surrogate_enable(True, tol)
+        e.set_timing(True)
+        wall = timed_steps(e, i0, a.sur_steps)
+        kt = e.kernel_times()
+        ss = e.surrogate_stats() if tol is not None else None
+        out = e.download(n)
+        e.close()
+        upd = round(1e3 * kt["g2p_upd_ms"] / a.sur_steps, 2)
+        if ss is not None:
+            ss["update_us"] = upd
+        return n * a.sur_steps / wall, ss, out.stress, upd
+
+    plain_rate, _, sp, plain_upd = advance(None)
+    runs = []
+    tols = [a.sur_tol] + [float(t) for t in a.sur_sweep.split(",") if t]
+    for tol in tols:
+        rate_s, ss, su, _ = advance(tol)
+        runs.append({"tol": tol, "particle_steps_per_s": rate_s, "ratio": rate_s / plain_rate,
+                     "rel_stress_err": float(np.linalg.norm(su - sp) / np.linalg.norm(sp)),
+                     "max_abs_stress_err": float(np.max(np.abs(su - sp))), "evaluated": ss["evaluated"],
+                     "fallback_rate": ss["fallbacks"] / max(1, ss["evaluated"]), "update_us": ss["update_us"]})
+    head = runs[0]
+    return {
+        "workload": scene.name, "particles": n, "grid": "x".join(map(str, scene.nelem)),
+        "plain_particle_steps_per_s": plain_rate, "surrogate_particle_steps_per_s": head["particle_steps_per_s"],
+        "plain_update_us": plain_upd, "surrogate_update_us": head["update_us"],
+        "ratio": head["ratio"], "rel_stress_err": head["rel_stress_err"], "tol": head["tol"],
+        "fallback_rate": head["fallback_rate"], "sweep": runs[1:],
+        "samples": st["samples"], "sample_rate": rate, "collect_steps": a.sur_collect,
+        "collect_s": round(t_collect, 3), "fit_s": round(t_fit, 3), "fit_epochs": a.sur_epochs,
+        "fit_batch": a.sur_batch, "fit_loss": loss, "timed_steps": a.sur_steps,
+    }
 
 
 def main():
@@ -143,6 +221,10 @@ def main():
     copy_gbps = None if a.no_probe else round(capi.bandwidth_probe(1 << 30, 20), 1)
     roofline["copy_ceiling_GBps"] = copy_gbps
 
+    sur = None
+    if not a.no_surrogate:
+        sur = surrogate_vs_plain(a)
+
     cpu = None
     if not a.no_cpu:
         cpu_scene = make_scene(a.config)
@@ -168,6 +250,7 @@ def main():
         "phases": phases,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "surrogate": sur,
     }
     print(json.dumps(out), flush=True)
 

@@ -23,6 +23,8 @@ UNITS = [
     ("bin.hip", []),
     ("exact64.hip", ["-ffp-contract=off"]),
     ("fast32.hip", []),
+    ("mlp.hip", []),
+    ("surrogate.cpp", ["-x", "hip"]),
     ("engine.cpp", ["-x", "hip"]),
 ]
 

@@ -3,8 +3,10 @@
 // and the orchestration of the per-pass and whole-step kernels.
 #include "mpm_common.h"
 #include "mpm_kernels.h"
+#include "surrogate_host.h"
 
 #include "../../include/mpm/mpm_capi.h"
+#include "../../include/mpm/mpm_surrogate.h"
 
 #include <algorithm>
 #include <cmath>
@@ -21,6 +23,8 @@ static int set_error(int code, const std::string &msg) {
     g_last_error = msg;
     return code;
 }
+
+int set_error_msg(int code, const char *msg) { return set_error(code, msg); }
 
 int hip_fail(hipError_t e, const char *what, const char *file, int line) {
     char buf[512];
@@ -97,6 +101,31 @@ struct mpm_engine {
 
     std::vector<void *> allocs;
     std::vector<void *> part_allocs;
+
+    // J2 constitutive surrogate
+    struct Surrogate {
+        bool logging = false, enabled = false, have_net = false;
+        MlpDev net;
+        float *log_in = nullptr, *log_tgt = nullptr;
+        uint32_t *log_key = nullptr;
+        int64_t cap = 0, nsamples = 0, logged_steps = 0;
+        unsigned long long *ctr = nullptr;  // evaluated, fallbacks, offered
+        uint32_t thr = 0, step_tag = 0;
+        bool sorted = false;
+        float loss = 0.f, tol = 1e-4f;
+        void free_log() {
+            for (void *p : {(void *)log_in, (void *)log_tgt, (void *)log_key})
+                if (p) (void)hipFree(p);
+            log_in = log_tgt = nullptr;
+            log_key = nullptr;
+            cap = nsamples = 0;
+        }
+        ~Surrogate() {
+            free_log();
+            if (ctr) (void)hipFree(ctr);
+            net.release();
+        }
+    } sur;
 
     template <typename T>
     int dalloc(T **p, size_t count, bool particle = false) {
@@ -254,7 +283,29 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         launch_f32_g2p_vel_musl(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, c,
                                 e->partials, diag, mid2);
         if (mid) MPM_HIP_CHECK(hipEventRecord(mid, e->stream));
-        launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err);
+        SurrogateArgs sa{};
+        auto &su = e->sur;
+        if (su.logging && su.log_in) {
+            // the sample key step_tag * n + id must fit 32 bits
+            if ((uint64_t)(su.step_tag + 1) * (uint64_t)e->n <= 0xFFFFFFFFull) {
+                sa.mode = 1;
+                sa.log_in = su.log_in;
+                sa.log_tgt = su.log_tgt;
+                sa.log_key = su.log_key;
+                sa.cap = (uint32_t)su.cap;
+                sa.sel_threshold = su.thr;
+                sa.step_tag = su.step_tag++;
+                su.logged_steps++;
+            }
+        } else if (su.enabled && su.have_net) {
+            sa.mode = 2;
+            sa.tol = su.tol;
+            sa.net = su.net.view();
+            sa.net_scaling = su.net.scaling;
+        }
+        sa.ctr = su.ctr;
+        sa.pid = e->pid[e->cur];
+        launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa);
         e->keys_valid = true;  // the update kernel writes the new keys
     }
     e->bins_valid = false;
@@ -455,6 +506,200 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         e->T.inv_nexy = (float)(1.0 / ((double)g.nex[0] * g.nex[1]));
     }
     *out = e;
+    return 0;
+}
+
+static int sur_require(mpm_engine *e) {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    if (e->prec != MPM_PREC_FP32)
+        return set_error(MPM_ERR_UNSUPPORTED, "the constitutive surrogate runs in the FP32 engine only");
+    if (!e->sur.ctr) {
+        MPM_HIP_CHECK(hipMalloc((void **)&e->sur.ctr, 4 * sizeof(unsigned long long)));
+        MPM_HIP_CHECK(hipMemsetAsync(e->sur.ctr, 0, 4 * sizeof(unsigned long long), e->stream));
+    }
+    if (!e->sur.net.master) {
+        int rc = e->sur.net.init();
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int mpm_surrogate_collect(mpm_engine *e, int32_t on, int64_t capacity, double rate) {
+    int rc = sur_require(e);
+    if (rc) return rc;
+    auto &su = e->sur;
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    if (!on) {
+        su.logging = false;
+        return 0;
+    }
+    if (capacity <= 0 || capacity >= (1ll << 31) || !(rate > 0.0))
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "capacity must be in (0, 2^31) and rate > 0");
+    if (capacity != su.cap) {
+        su.free_log();
+        MPM_HIP_CHECK(hipMalloc((void **)&su.log_in, (size_t)capacity * 16 * sizeof(float)));
+        MPM_HIP_CHECK(hipMalloc((void **)&su.log_tgt, (size_t)capacity * 16 * sizeof(float)));
+        MPM_HIP_CHECK(hipMalloc((void **)&su.log_key, (size_t)capacity * sizeof(uint32_t)));
+        su.cap = capacity;
+    }
+    MPM_HIP_CHECK(hipMemsetAsync(su.ctr, 0, 4 * sizeof(unsigned long long), e->stream));
+    su.thr = rate >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(rate * 4294967296.0);
+    su.step_tag = 0;
+    su.logged_steps = 0;
+    su.nsamples = 0;
+    su.sorted = false;
+    su.logging = true;
+    return 0;
+}
+
+static int sur_samples(mpm_engine *e, int64_t *n) {
+    unsigned long long c[4];
+    MPM_HIP_CHECK(hipMemcpyAsync(c, e->sur.ctr, sizeof c, hipMemcpyDeviceToHost, e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    *n = (int64_t)std::min<unsigned long long>(c[2], (unsigned long long)e->sur.cap);
+    return 0;
+}
+
+// order the logged samples by (step, particle id): the log is filled with
+// atomics, so its raw order varies from run to run
+static int sur_sort(mpm_engine *e) {
+    auto &su = e->sur;
+    if (su.sorted) return 0;
+    int64_t n = 0;
+    int rc = sur_samples(e, &n);
+    if (rc) return rc;
+    su.nsamples = n;
+    if (n == 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "no samples were logged (no plastic J2 trial states)");
+    SortBuffers sb{};
+    float *tmp = nullptr;
+    hipError_t he = hipMalloc((void **)&sb.keys, n * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.vals, n * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.keys_alt, n * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.vals_alt, n * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.hist, radix_hist_entries((int)n) * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&sb.bsum, 4096 * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc((void **)&tmp, (size_t)n * 16 * sizeof(float));
+    uint32_t *sk = nullptr, *sv = nullptr;
+    if (he == hipSuccess && radix_sort_pairs(e->stream, sb, su.log_key, (int)n, 32, &sk, &sv)) he = hipErrorUnknown;
+    if (he == hipSuccess) {
+        launch_gather_rows(e->stream, su.log_in, sv, n, tmp);
+        he = hipMemcpyAsync(su.log_in, tmp, (size_t)n * 16 * sizeof(float), hipMemcpyDeviceToDevice, e->stream);
+    }
+    if (he == hipSuccess) {
+        launch_gather_rows(e->stream, su.log_tgt, sv, n, tmp);
+        he = hipMemcpyAsync(su.log_tgt, tmp, (size_t)n * 16 * sizeof(float), hipMemcpyDeviceToDevice, e->stream);
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(su.log_key, sk, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    for (void *p : {(void *)sb.keys, (void *)sb.vals, (void *)sb.keys_alt, (void *)sb.vals_alt, (void *)sb.hist,
+                    (void *)sb.bsum, (void *)tmp})
+        if (p) (void)hipFree(p);
+    if (he != hipSuccess) return hip_fail(he, "surrogate sample sort", __FILE__, __LINE__);
+    su.sorted = true;
+    return 0;
+}
+
+int mpm_surrogate_fit(mpm_engine *e, const mpm_train_params *tp, float *final_loss) {
+    int rc = sur_require(e);
+    if (rc) return rc;
+    if (!tp) return set_error(MPM_ERR_INVALID_ARGUMENT, "null training parameters");
+    auto &su = e->sur;
+    if (!su.log_in) return set_error(MPM_ERR_INVALID_ARGUMENT, "no sample log (call mpm_surrogate_collect)");
+    su.logging = false;
+    if ((rc = sur_sort(e))) return rc;
+    float scaling[64];
+    if ((rc = mlp_fit_scaling(su.log_in, su.log_tgt, su.nsamples, scaling, e->stream))) return rc;
+    std::vector<float> params(kMlpNParams);
+    mlp_init_params(params.data(), tp->seed);
+    if ((rc = mlp_set(su.net, params.data(), scaling, e->stream))) return rc;
+    std::vector<float> el;
+    if ((rc = mlp_train(su.net, su.log_in, su.log_tgt, su.nsamples, *tp, el, e->stream))) return rc;
+    su.loss = el.empty() ? 0.f : el.back();
+    su.have_net = true;
+    if (final_loss) *final_loss = su.loss;
+    return 0;
+}
+
+int mpm_surrogate_enable(mpm_engine *e, int32_t on, double tol) {
+    int rc = sur_require(e);
+    if (rc) return rc;
+    if (on && !e->sur.have_net) return set_error(MPM_ERR_INVALID_ARGUMENT, "no trained or loaded network");
+    if (on && !(tol >= 0.0)) return set_error(MPM_ERR_INVALID_ARGUMENT, "tolerance must be >= 0");
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->sur.enabled = on != 0;
+    if (on) {
+        e->sur.tol = (float)std::min(tol, 1e30);
+        e->sur.logging = false;
+    }
+    return 0;
+}
+
+int mpm_surrogate_reset_counters(mpm_engine *e) {
+    int rc = sur_require(e);
+    if (rc) return rc;
+    unsigned long long c[4];
+    MPM_HIP_CHECK(hipMemcpyAsync(c, e->sur.ctr, sizeof c, hipMemcpyDeviceToHost, e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    c[0] = c[1] = 0;  // keep the log count
+    MPM_HIP_CHECK(hipMemcpyAsync(e->sur.ctr, c, sizeof c, hipMemcpyHostToDevice, e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int mpm_surrogate_stats_get(mpm_engine *e, mpm_surrogate_stats *out) {
+    int rc = sur_require(e);
+    if (rc) return rc;
+    if (!out) return set_error(MPM_ERR_INVALID_ARGUMENT, "null output");
+    unsigned long long c[4];
+    MPM_HIP_CHECK(hipMemcpyAsync(c, e->sur.ctr, sizeof c, hipMemcpyDeviceToHost, e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    out->samples = (int64_t)std::min<unsigned long long>(c[2], (unsigned long long)e->sur.cap);
+    out->logged_steps = e->sur.logged_steps;
+    out->evaluated = (int64_t)c[0];
+    out->fallbacks = (int64_t)c[1];
+    out->loss = e->sur.loss;
+    out->tol = e->sur.tol;
+    return 0;
+}
+
+int mpm_surrogate_get_params(mpm_engine *e, float *params, float *scaling) {
+    int rc = sur_require(e);
+    if (rc) return rc;
+    if (!e->sur.have_net) return set_error(MPM_ERR_INVALID_ARGUMENT, "no trained or loaded network");
+    return mlp_get(e->sur.net, params, scaling, e->stream);
+}
+
+int mpm_surrogate_set_params(mpm_engine *e, const float *params, const float *scaling) {
+    int rc = sur_require(e);
+    if (rc) return rc;
+    if (!params || !scaling) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    if ((rc = mlp_set(e->sur.net, params, scaling, e->stream))) return rc;
+    e->sur.have_net = true;
+    return 0;
+}
+
+int mpm_surrogate_download_samples(mpm_engine *e, float *in, float *tgt, int64_t max, int64_t *n) {
+    int rc = sur_require(e);
+    if (rc) return rc;
+    if (!n || max < 0 || (max > 0 && (!in || !tgt))) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (!e->sur.log_in) {
+        *n = 0;
+        return 0;
+    }
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    if (!e->sur.logging && (rc = sur_sort(e))) return rc;
+    int64_t have = 0;
+    if ((rc = sur_samples(e, &have))) return rc;
+    const int64_t k = std::min(have, max);
+    if (k > 0) {
+        MPM_HIP_CHECK(hipMemcpyAsync(in, e->sur.log_in, (size_t)k * 16 * sizeof(float), hipMemcpyDeviceToHost,
+                                     e->stream));
+        MPM_HIP_CHECK(hipMemcpyAsync(tgt, e->sur.log_tgt, (size_t)k * 16 * sizeof(float), hipMemcpyDeviceToHost,
+                                     e->stream));
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    }
+    *n = k;
     return 0;
 }
 

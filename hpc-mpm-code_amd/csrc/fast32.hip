@@ -586,6 +586,61 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, T
 // Stress update in FP32 from H = F - I and the increment D = dF - I
 // (semantics of ConstitutiveModel::UpdateStress, constitutive_model.cpp:287-305,
 // rewritten without cancellation against the identity).
+// J2 plasticity with linear isotropic hardening, small-increment form:
+// elastic predictor sigma_tr = sigma_n + C:de, radial return onto
+// q = sy + H eps_p (Simo & Hughes 1998, Box 3.2).
+struct J2Trial {
+    float st[6];          // trial stress
+    float pm, d0, d1, d2;  // trial pressure and normal deviators
+    float q, f;           // von Mises stress and yield function of the trial state
+    float mu, lmd, E, nu, sy, Hh, ep;
+};
+
+__device__ __forceinline__ void j2_trial(const double *p, const float D[9], const float s[6], float ep, J2Trial &t) {
+    const float E = (float)p[0], nu = (float)p[1];
+    t.E = E;
+    t.nu = nu;
+    t.sy = (float)p[2];
+    t.Hh = (float)p[3];
+    t.ep = ep;
+    t.lmd = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
+    t.mu = E / (2.f * (1.f + nu));
+    const float lmd = t.lmd, mu = t.mu;
+    float de[6] = {D[0], D[4], D[8], D[5] + D[7], D[6] + D[2], D[1] + D[3]};
+    float tr = de[0] + de[1] + de[2];
+    t.st[0] = s[0] + lmd * tr + 2.f * mu * de[0];
+    t.st[1] = s[1] + lmd * tr + 2.f * mu * de[1];
+    t.st[2] = s[2] + lmd * tr + 2.f * mu * de[2];
+    t.st[3] = s[3] + mu * de[3];
+    t.st[4] = s[4] + mu * de[4];
+    t.st[5] = s[5] + mu * de[5];
+    t.pm = (t.st[0] + t.st[1] + t.st[2]) * (1.f / 3.f);
+    t.d0 = t.st[0] - t.pm;
+    t.d1 = t.st[1] - t.pm;
+    t.d2 = t.st[2] - t.pm;
+    float j2 = 0.5f * (t.d0 * t.d0 + t.d1 * t.d1 + t.d2 * t.d2) + t.st[3] * t.st[3] + t.st[4] * t.st[4] +
+               t.st[5] * t.st[5];
+    t.q = sqrtf(3.f * j2);
+    t.f = t.q - (t.sy + t.Hh * ep);
+}
+
+// exact return (or the elastic trial state); eps_p receives the new value
+__device__ __forceinline__ void j2_return(const J2Trial &t, float s[6], float *eps_p) {
+    if (t.f <= 0.f) {
+        for (int c = 0; c < 6; c++) s[c] = t.st[c];
+    } else {
+        float dg = t.f / (3.f * t.mu + t.Hh);
+        float scl = 1.f - 3.f * t.mu * dg / t.q;
+        s[0] = t.pm + t.d0 * scl;
+        s[1] = t.pm + t.d1 * scl;
+        s[2] = t.pm + t.d2 * scl;
+        s[3] = t.st[3] * scl;
+        s[4] = t.st[4] * scl;
+        s[5] = t.st[5] * scl;
+        if (eps_p) *eps_p = t.ep + dg;
+    }
+}
+
 __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9], const float D[9], double J,
                                          float s[6], float *eps_p) {
     const double *p = md.prop;
@@ -686,49 +741,93 @@ __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9]
         break;
     }
     case 10: {  // J2 radial return, small-increment form
-        float E = (float)p[0], nu = (float)p[1], sy = (float)p[2], Hh = (float)p[3];
-        float lmd = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
-        float mu = E / (2.f * (1.f + nu));
-        float de[6] = {D[0], D[4], D[8], D[5] + D[7], D[6] + D[2], D[1] + D[3]};
-        float tr = de[0] + de[1] + de[2];
-        float st[6];
-        st[0] = s[0] + lmd * tr + 2.f * mu * de[0];
-        st[1] = s[1] + lmd * tr + 2.f * mu * de[1];
-        st[2] = s[2] + lmd * tr + 2.f * mu * de[2];
-        st[3] = s[3] + mu * de[3];
-        st[4] = s[4] + mu * de[4];
-        st[5] = s[5] + mu * de[5];
-        float pm = (st[0] + st[1] + st[2]) * (1.f / 3.f);
-        float d0 = st[0] - pm, d1 = st[1] - pm, d2 = st[2] - pm;
-        float j2 = 0.5f * (d0 * d0 + d1 * d1 + d2 * d2) + st[3] * st[3] + st[4] * st[4] + st[5] * st[5];
-        float q = sqrtf(3.f * j2);
-        float ep = eps_p ? *eps_p : 0.f;
-        float f = q - (sy + Hh * ep);
-        if (f <= 0.f) {
-            for (int c = 0; c < 6; c++) s[c] = st[c];
-        } else {
-            float dg = f / (3.f * mu + Hh);
-            float scl = 1.f - 3.f * mu * dg / q;
-            s[0] = pm + d0 * scl;
-            s[1] = pm + d1 * scl;
-            s[2] = pm + d2 * scl;
-            s[3] = st[3] * scl;
-            s[4] = st[4] * scl;
-            s[5] = st[5] * scl;
-            if (eps_p) *eps_p = ep + dg;
-        }
+        J2Trial t;
+        j2_trial(p, D, s, eps_p ? *eps_p : 0.f, t);
+        j2_return(t, s, eps_p);
         break;
     }
     default: break;
     }
 }
 
+template <bool ON>
+struct SurSmem {
+    MlpSmem w;
+    float scaling[64];
+    int wcount[4];
+    __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];  // queue of up to 256 staged rows
+};
+template <>
+struct SurSmem<false> {
+    int unused;
+};
+
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// 13 surrogate inputs: strain increment (engineering shear), elastic strain
+// of the old stress, old equivalent plastic strain
+__device__ __forceinline__ void j2_features(const J2Trial &t, const float D[9], const float s_old[6], float x[16]) {
+    x[0] = D[0];
+    x[1] = D[4];
+    x[2] = D[8];
+    x[3] = D[5] + D[7];
+    x[4] = D[6] + D[2];
+    x[5] = D[1] + D[3];
+    const float tr = s_old[0] + s_old[1] + s_old[2];
+    const float a = (1.f + t.nu) / t.E, b = t.nu / t.E, im = 1.f / t.mu;
+#pragma unroll
+    for (int c = 0; c < 3; c++) x[6 + c] = a * s_old[c] - b * tr;
+#pragma unroll
+    for (int c = 3; c < 6; c++) x[6 + c] = s_old[c] * im;
+    x[12] = t.ep;
+    x[13] = x[14] = x[15] = 0.f;
+}
+
+// Residual of the discrete return-mapping equations at a predicted state
+// (stress s, plastic increment dg), relative to the yield stress:
+// max(|q(s) - sy - H (ep + dg)|, max_c |s_c - s_rr(dg)_c|) / sy, where
+// s_rr(dg) is the radial-return stress for that increment.
+__device__ __forceinline__ float j2_residual(const J2Trial &t, const float s[6], float dg) {
+    if (!(dg >= 0.f)) return 1e30f;
+    const float pm = (s[0] + s[1] + s[2]) * (1.f / 3.f);
+    const float e0 = s[0] - pm, e1 = s[1] - pm, e2 = s[2] - pm;
+    const float j2 = 0.5f * (e0 * e0 + e1 * e1 + e2 * e2) + s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
+    const float q = sqrtf(3.f * j2);
+    float r = fabsf(q - (t.sy + t.Hh * (t.ep + dg)));
+    const float scl = 1.f - 3.f * t.mu * dg / t.q;
+    const float rr[6] = {t.pm + t.d0 * scl, t.pm + t.d1 * scl, t.pm + t.d2 * scl,
+                         t.st[3] * scl,     t.st[4] * scl,     t.st[5] * scl};
+#pragma unroll
+    for (int c = 0; c < 6; c++) r = fmaxf(r, fabsf(s[c] - rr[c]));
+    return r / t.sy;
+}
+
+template <int SUR>
 __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
-                                                       MaterialTable mt, StepConsts sc, uint32_t *err) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= P.n) return;
-    const uint32_t key = P.key[slot];
-    if (key >= (uint32_t)g.nelem) return;
+                                                       MaterialTable mt, StepConsts sc, uint32_t *err,
+                                                       SurrogateArgs sa) {
+    __shared__ SurSmem<SUR == 2> sm;
+    const int slot0 = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = slot0 < P.n;
+    // the network runs on whole waves: with SUR == 2 no lane leaves early,
+    // invalid lanes compute on slot 0 / cell 0 and store nothing
+    if constexpr (SUR != 2) {
+        if (!valid) return;
+    }
+    const int slot = valid ? slot0 : 0;
+    uint32_t key = P.key[slot];
+    if (key >= (uint32_t)g.nelem) valid = false;
+    if constexpr (SUR != 2) {
+        if (!valid) return;
+    }
+    if (!valid) key = 0;
     const size_t ld = P.ld;
     const int nodec = g.nodec;
     const int nexy = g.nex[0] * g.nex[1];
@@ -818,7 +917,105 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
         for (int q = 0; q < 6; q++) s[q] = P.sig[q * ld + slot];
     }
     float ep = (md.model == 10 && P.eps_p) ? P.eps_p[slot] : 0.f;
-    stress32(md, Hn, D, J, s, &ep);
+    if constexpr (SUR == 0) {
+        stress32(md, Hn, D, J, s, &ep);
+    } else {
+        const bool j2 = md.model == 10;
+        J2Trial t;
+        bool plastic = false;
+        if (j2) {
+            j2_trial(md.prop, D, s, ep, t);
+            plastic = valid && t.f > 0.f;
+        } else {
+            stress32(md, Hn, D, J, s, &ep);
+        }
+        const int lane = threadIdx.x & 63;
+        if constexpr (SUR == 1) {
+            float s_old[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) s_old[c] = s[c];
+            if (j2) j2_return(t, s, &ep);
+            const uint32_t id = (uint32_t)sa.pid[slot];
+            const bool want = plastic && mix32(id * 0x9E3779B9u ^ mix32(sa.step_tag + 0x51ED27u)) <= sa.sel_threshold;
+            const unsigned long long m = __ballot(want);
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(&sa.ctr[2], (unsigned long long)__popcll(m));
+                base = __shfl(base, leader, 64);
+                if (want) {
+                    const unsigned long long idx = base + __popcll(m & ((1ull << lane) - 1ull));
+                    if (idx < sa.cap) {
+                        float x[16];
+                        j2_features(t, D, s_old, x);
+                        float y[16];
+#pragma unroll
+                        for (int c = 0; c < 6; c++) y[c] = t.st[c] - s[c];
+                        y[6] = ep - t.ep;
+#pragma unroll
+                        for (int c = 7; c < 16; c++) y[c] = 0.f;
+                        float4 *xi = (float4 *)(sa.log_in + idx * 16), *yo = (float4 *)(sa.log_tgt + idx * 16);
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            xi[c] = make_float4(x[4 * c], x[4 * c + 1], x[4 * c + 2], x[4 * c + 3]);
+                            yo[c] = make_float4(y[4 * c], y[4 * c + 1], y[4 * c + 2], y[4 * c + 3]);
+                        }
+                        sa.log_key[idx] = sa.step_tag * (uint32_t)P.n + id;
+                    }
+                }
+            }
+        } else {
+            // The block's plastic trial states are queued in LDS (block
+            // order) and the network runs on full waves of the queue, so a
+            // block with k plastic states makes ceil(k / 64) wave passes.
+            bool accept = false;
+            if (__syncthreads_or(plastic)) {
+                const int w = threadIdx.x >> 6;
+                mlp_load_smem(sm.w, sa.net);
+                if (threadIdx.x < 64) sm.scaling[threadIdx.x] = sa.net_scaling[threadIdx.x];
+                const unsigned long long m = __ballot(plastic);
+                if (lane == 0) sm.wcount[w] = __popcll(m);
+                __syncthreads();
+                int base = 0, k = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    base += (i < w) ? sm.wcount[i] : 0;
+                    k += sm.wcount[i];
+                }
+                const int qi = base + __popcll(m & ((1ull << lane) - 1ull));
+                if (plastic) {
+                    float x[16], xn[16];
+                    j2_features(t, D, s, x);
+#pragma unroll
+                    for (int f = 0; f < 16; f++) xn[f] = (x[f] - sm.scaling[f]) * sm.scaling[16 + f];
+                    mlp_stage_row(&sm.xs[qi >> 6][(qi & 63) * kMlpIn], xn);
+                }
+                __syncthreads();
+                if (w * 64 < k) mlp_wave_run(sm.w, sm.xs[w], lane);
+                __syncthreads();
+                if (plastic) {
+                    const float *y = (const float *)sm.xs[qi >> 6] + (qi & 63) * 8;
+                    float ss[6];
+#pragma unroll
+                    for (int c = 0; c < 6; c++) ss[c] = t.st[c] - (y[c] * sm.scaling[48 + c] + sm.scaling[32 + c]);
+                    const float dg = y[6] * sm.scaling[54] + sm.scaling[38];
+                    if (j2_residual(t, ss, dg) <= sa.tol) {
+                        accept = true;
+#pragma unroll
+                        for (int c = 0; c < 6; c++) s[c] = ss[c];
+                        ep = t.ep + dg;
+                    }
+                }
+                const unsigned long long rej = __ballot(plastic && !accept);
+                if (lane == 0 && m) {
+                    atomicAdd(&sa.ctr[0], (unsigned long long)__popcll(m));
+                    if (rej) atomicAdd(&sa.ctr[1], (unsigned long long)__popcll(rej));
+                }
+            }
+            if (j2 && !accept) j2_return(t, s, &ep);
+        }
+    }
+    if (!valid) return;
     if (md.model == 10 && P.eps_p) P.eps_p[slot] = ep;
 #pragma unroll
     for (int q = 0; q < 9; q++) P.H[q * ld + slot] = Hn[q];
@@ -932,9 +1129,16 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
 }
 
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
-                           const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err) {
+                           const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
+                           const SurrogateArgs &sa) {
     if (P.n <= 0) return;
-    hipLaunchKernelGGL(k32_g2p_update, dim3(cdiv(P.n, 256)), dim3(256), 0, s, g, P, N, T, mt, c, err);
+    const dim3 grid(cdiv(P.n, 256)), block(256);
+    if (sa.mode == 1)
+        hipLaunchKernelGGL(k32_g2p_update<1>, grid, block, 0, s, g, P, N, T, mt, c, err, sa);
+    else if (sa.mode == 2)
+        hipLaunchKernelGGL(k32_g2p_update<2>, grid, block, 0, s, g, P, N, T, mt, c, err, sa);
+    else
+        hipLaunchKernelGGL(k32_g2p_update<0>, grid, block, 0, s, g, P, N, T, mt, c, err, sa);
 }
 
 __global__ void k_iota(uint32_t *p, int n) {

@@ -55,6 +55,7 @@ constexpr int kNodeTileY = 2;
 constexpr int kNodeTileZ = 2;
 
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Cell key of a position with the reference's FP64 formula
 // (material_point.cpp:59-65); returns nelem (sentinel) outside the grid.

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "mpm_common.h"
+#include "mpm_mlp.h"
 
 namespace mpm {
 
@@ -172,8 +173,26 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
                              const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
                              const StepConsts &c, float *partials, int diag, hipEvent_t mid = nullptr);
 // dF, H, J, V, x, stress and the new cell key of every slot.
+// J2 constitutive surrogate in the FP32 update: mode 0 = exact update only,
+// 1 = exact update and sample logging, 2 = network for plastic trial states
+// with residual check and exact fallback.
+struct SurrogateArgs {
+    int mode;
+    float tol;                     // mode 2: accepted residual / yield stress
+    MlpParams net;                 // mode 2
+    const float *net_scaling;      // mode 2: [in_mu | in_isd | out_mu | out_sd], 64 floats
+    unsigned long long *ctr;       // [0] evaluated, [1] fallbacks, [2] samples offered to the log
+    float *log_in, *log_tgt;       // mode 1: [cap][16]
+    uint32_t *log_key;             // mode 1: step_tag * n + particle id
+    uint32_t cap;
+    uint32_t sel_threshold;        // mode 1: keep when hash(pid, step) <= threshold
+    uint32_t step_tag;
+    const int *pid;                // particle id by slot
+};
+
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
-                           const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err);
+                           const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
+                           const SurrogateArgs &sa);
 void launch_f32_permute(hipStream_t s, const PartF32 &src, const PartF32 &dst, const uint32_t *perm,
                         const int *pid_src, int *pid_dst);
 void launch_iota_u32(hipStream_t s, uint32_t *p, int n);

@@ -1,0 +1,164 @@
+// Shapes and parameter layout of the constitutive-surrogate MLP.
+#pragma once
+
+#include <stdint.h>
+
+namespace mpm {
+
+constexpr int kMlpIn = 16;       // padded input width (13 used)
+constexpr int kMlpInUsed = 13;
+constexpr int kMlpHid = 64;
+constexpr int kMlpOut = 16;      // padded output width (7 used)
+constexpr int kMlpOutUsed = 7;
+
+// flat FP32 parameter vector: [w1 (64x16) | b1 (64) | w2 (64x64) | b2 (64) | w3 (16x64) | b3 (16)],
+// weights row-major [out][in]
+constexpr int kMlpOffW1 = 0;
+constexpr int kMlpOffB1 = kMlpOffW1 + kMlpHid * kMlpIn;
+constexpr int kMlpOffW2 = kMlpOffB1 + kMlpHid;
+constexpr int kMlpOffB2 = kMlpOffW2 + kMlpHid * kMlpHid;
+constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHid;
+constexpr int kMlpOffB3 = kMlpOffW3 + kMlpOut * kMlpHid;
+constexpr int kMlpNParams = kMlpOffB3 + kMlpOut;
+
+// Device view: BF16 weights (bit patterns), FP32 biases, feature scaling
+// (x_norm = (x - in_mu) * in_isd; y = y_norm * out_sd + out_mu).
+struct MlpParams {
+    const uint16_t *w1, *w2, *w3;
+    const float *b1, *b2, *b3;
+    const float *in_mu, *in_isd, *out_mu, *out_sd;
+};
+
+__device__ void mlp_publish_param(const MlpParams &p, int i, float v);
+
+typedef short mlp_v4s __attribute__((ext_vector_type(4)));
+typedef float mlp_v4f __attribute__((ext_vector_type(4)));
+
+// Weights of one network staged in LDS by a whole block (bf16 bit patterns,
+// [out][in] row-major) and the FP32 biases.
+struct MlpSmem {
+    uint16_t w1[kMlpHid * kMlpIn];
+    uint16_t w2[kMlpHid * kMlpHid];
+    uint16_t w3[kMlpOut * kMlpHid];
+    float b1[kMlpHid], b2[kMlpHid], b3[kMlpOut];
+};
+
+// Per-wave exchange buffer: 64 samples x 16 bf16 inputs, reused for 64 x 8
+// FP32 outputs.
+constexpr int kMlpWaveScratchBytes = 64 * kMlpIn * 2;
+
+__device__ __forceinline__ uint16_t mlp_f2bf(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ void mlp_load_smem(MlpSmem &s, const MlpParams &p) {
+    for (int i = threadIdx.x; i < kMlpHid * kMlpIn / 4; i += blockDim.x)
+        ((uint2 *)s.w1)[i] = ((const uint2 *)p.w1)[i];
+    for (int i = threadIdx.x; i < kMlpHid * kMlpHid / 4; i += blockDim.x)
+        ((uint2 *)s.w2)[i] = ((const uint2 *)p.w2)[i];
+    for (int i = threadIdx.x; i < kMlpOut * kMlpHid / 4; i += blockDim.x)
+        ((uint2 *)s.w3)[i] = ((const uint2 *)p.w3)[i];
+    for (int i = threadIdx.x; i < kMlpHid; i += blockDim.x) {
+        s.b1[i] = p.b1[i];
+        s.b2[i] = p.b2[i];
+    }
+    for (int i = threadIdx.x; i < kMlpOut; i += blockDim.x) s.b3[i] = p.b3[i];
+}
+
+__device__ __forceinline__ void mlp_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Forward pass of the 64 samples staged in xs ([64][16] bf16 normalised
+// inputs) on the 16x16x16 BF16 MFMA; on return xs holds [64][8] FP32
+// normalised outputs. The layers run transposed (hidden x samples) so that
+// the accumulator fragment of one layer (lane l: rows 4(l/16)+r, column
+// l%16) is exactly the B operand fragment of the next; only the inputs and
+// outputs pass through LDS. Must be called by all 64 lanes of the wave after
+// the rows are visible to the wave.
+__device__ __forceinline__ void mlp_wave_run(const MlpSmem &s, uint16_t *xs, int lane) {
+    const int c16 = lane & 15, q4 = 4 * (lane >> 4);
+    mlp_v4s xb[4];
+#pragma unroll
+    for (int si = 0; si < 4; si++) xb[si] = *(const mlp_v4s *)(xs + (si * 16 + c16) * kMlpIn + q4);
+    mlp_wave_sync();
+    float *ys = (float *)xs;  // [64][8]
+#pragma unroll
+    for (int si = 0; si < 4; si++) {
+        mlp_v4s h1[4], h2[4];
+#pragma unroll
+        for (int hb = 0; hb < 4; hb++) {
+            const mlp_v4s a = *(const mlp_v4s *)(s.w1 + (hb * 16 + c16) * kMlpIn + q4);
+            mlp_v4f acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb[si], mlp_v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) h1[hb][r] = (short)mlp_f2bf(fmaxf(acc[r] + s.b1[hb * 16 + q4 + r], 0.f));
+        }
+#pragma unroll
+        for (int hb = 0; hb < 4; hb++) {
+            mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++) {
+                const mlp_v4s a = *(const mlp_v4s *)(s.w2 + (hb * 16 + c16) * kMlpHid + kb * 16 + q4);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kb], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) h2[hb][r] = (short)mlp_f2bf(fmaxf(acc[r] + s.b2[hb * 16 + q4 + r], 0.f));
+        }
+        mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+            const mlp_v4s a = *(const mlp_v4s *)(s.w3 + c16 * kMlpHid + kb * 16 + q4);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[kb], acc, 0, 0, 0);
+        }
+        if (q4 < 8) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) ys[(si * 16 + c16) * 8 + q4 + r] = acc[r] + s.b3[q4 + r];
+        }
+    }
+    mlp_wave_sync();
+}
+
+// row of 16 normalised inputs as bf16 into a staging row
+__device__ __forceinline__ void mlp_stage_row(uint16_t *row, const float xn[kMlpIn]) {
+    uint32_t pk[kMlpIn / 2];
+#pragma unroll
+    for (int f = 0; f < kMlpIn / 2; f++)
+        pk[f] = (uint32_t)mlp_f2bf(xn[2 * f]) | ((uint32_t)mlp_f2bf(xn[2 * f + 1]) << 16);
+    uint4 *r4 = (uint4 *)row;
+    r4[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+    r4[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+}
+
+// one sample per lane: xn this lane's normalised inputs, y its normalised outputs
+__device__ __forceinline__ void mlp_wave_infer(const MlpSmem &s, const float xn[kMlpIn], float y[kMlpOutUsed],
+                                               uint16_t *xs, int lane) {
+    mlp_stage_row(xs + lane * kMlpIn, xn);
+    mlp_wave_sync();
+    mlp_wave_run(s, xs, lane);
+    const float *ys = (const float *)xs;
+#pragma unroll
+    for (int o = 0; o < kMlpOutUsed; o++) y[o] = ys[lane * 8 + o];
+    mlp_wave_sync();
+}
+
+void launch_mlp_infer(hipStream_t s, const MlpParams &p, const float *in, float *out, int n);
+int mlp_grad_waves(int n, int tiles_per_wave);
+// partial: [cdiv(waves,4)*4][kMlpNParams], loss_partial: same number of waves
+void launch_mlp_grad(hipStream_t s, const MlpParams &p, const float *in, const float *tgt, const uint32_t *order, int n,
+                     int tiles_per_wave, float *partial, float *loss_partial, float *g, float *loss);
+void launch_mlp_adam(hipStream_t s, float *master, float *m1, float *m2, const float *g, const MlpParams &p, float lr,
+                     int t);
+void launch_mlp_publish(hipStream_t s, const float *master, const MlpParams &p);
+// order[i] = (a i + c) mod n, a coprime to n
+void launch_affine_perm(hipStream_t s, uint32_t *order, int64_t n, uint64_t a, uint64_t c);
+// dst row i = src row idx[i] (rows of 16 floats)
+void launch_gather_rows(hipStream_t s, const float *src, const uint32_t *idx, int64_t n, float *dst);
+// per-block FP64 sums of (x - shift) and (x - shift)^2 of each of the 16 columns;
+// partial: [kMomentBlocks][16][2]
+constexpr int kMomentBlocks = 256;
+void launch_moments(hipStream_t s, const float *rows, int64_t n, const float *shift, double *partial);
+
+}  // namespace mpm

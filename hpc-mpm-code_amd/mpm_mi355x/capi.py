@@ -66,6 +66,18 @@ class KernelTimes(C.Structure):
 
 
 # name -> (restype, argtypes); every function declared in mpm_capi.h
+class TrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("seed", C.c_uint32), ("pad", C.c_int32)]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("logged_steps", C.c_int64), ("evaluated", C.c_int64),
+                ("fallbacks", C.c_int64), ("loss", C.c_float), ("tol", C.c_float)]
+
+
+_fp = C.POINTER(C.c_float)
+
 SIGNATURES = {
     "mpm_version": (C.c_char_p, []),
     "mpm_last_error": (C.c_char_p, []),
@@ -89,7 +101,21 @@ SIGNATURES = {
     "mpm_maksf_batch": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _ip, _dp, _ip, _dp, _dp]),
     "mpm_bin_positions": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _dp, _ip, _ip]),
     "mpm_bandwidth_probe": (C.c_int, [C.c_int64, C.c_int32, _dp]),
+    # include/mpm/mpm_surrogate.h
+    "mpm_mlp_forward": (C.c_int, [_fp, _fp, _fp, C.c_int64, _fp]),
+    "mpm_mlp_gradient": (C.c_int, [_fp, _fp, _fp, _fp, C.c_int64, _fp, _fp]),
+    "mpm_mlp_train": (C.c_int, [_fp, _fp, _fp, _fp, C.c_int64, C.POINTER(TrainParams), _fp]),
+    "mpm_surrogate_collect": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_double]),
+    "mpm_surrogate_fit": (C.c_int, [C.c_void_p, C.POINTER(TrainParams), _fp]),
+    "mpm_surrogate_enable": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "mpm_surrogate_stats_get": (C.c_int, [C.c_void_p, C.POINTER(SurrogateStats)]),
+    "mpm_surrogate_reset_counters": (C.c_int, [C.c_void_p]),
+    "mpm_surrogate_get_params": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "mpm_surrogate_set_params": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "mpm_surrogate_download_samples": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int64, C.POINTER(C.c_int64)]),
 }
+
+MLP_NPARAMS = 6288
 
 _lib = None
 
@@ -294,6 +320,84 @@ class Engine:
         kt = KernelTimes()
         check(self.lib.mpm_get_kernel_times(self.h, C.byref(kt)))
         return {f: getattr(kt, f) for f, _ in KernelTimes._fields_}
+
+
+    # ---- J2 constitutive surrogate
+    def surrogate_collect(self, on=True, capacity=1 << 20, rate=1.0):
+        check(self.lib.mpm_surrogate_collect(self.h, int(on), int(capacity), float(rate)))
+
+    def surrogate_fit(self, epochs=20, batch=8192, lr=2e-3, lr_final=5e-5, seed=1):
+        tp = TrainParams(epochs, batch, lr, lr_final, seed, 0)
+        loss = C.c_float(0)
+        check(self.lib.mpm_surrogate_fit(self.h, C.byref(tp), C.byref(loss)))
+        return loss.value
+
+    def surrogate_enable(self, on=True, tol=1e-4):
+        check(self.lib.mpm_surrogate_enable(self.h, int(on), float(tol)))
+
+    def surrogate_stats(self):
+        st = SurrogateStats()
+        check(self.lib.mpm_surrogate_stats_get(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in SurrogateStats._fields_}
+
+    def surrogate_reset_counters(self):
+        check(self.lib.mpm_surrogate_reset_counters(self.h))
+
+    def surrogate_params(self):
+        p = np.zeros(MLP_NPARAMS, np.float32)
+        sc = np.zeros(64, np.float32)
+        check(self.lib.mpm_surrogate_get_params(self.h, _ptr(p, C.c_float), _ptr(sc, C.c_float)))
+        return p, sc
+
+    def surrogate_set_params(self, params, scaling):
+        p = np.ascontiguousarray(params, np.float32)
+        sc = np.ascontiguousarray(scaling, np.float32)
+        check(self.lib.mpm_surrogate_set_params(self.h, _ptr(p, C.c_float), _ptr(sc, C.c_float)))
+
+    def surrogate_samples(self, max_rows=1 << 20):
+        xin = np.zeros((max_rows, 16), np.float32)
+        tgt = np.zeros((max_rows, 16), np.float32)
+        n = C.c_int64(0)
+        check(self.lib.mpm_surrogate_download_samples(self.h, _ptr(xin, C.c_float), _ptr(tgt, C.c_float), max_rows,
+                                                      C.byref(n)))
+        return xin[:n.value], tgt[:n.value]
+
+
+def mlp_forward(params, scaling, inp):
+    lib = load()
+    p = np.ascontiguousarray(params, np.float32)
+    sc = np.ascontiguousarray(scaling, np.float32)
+    x = np.ascontiguousarray(inp, np.float32)
+    out = np.zeros((x.shape[0], 16), np.float32)
+    check(lib.mpm_mlp_forward(_ptr(p, C.c_float), _ptr(sc, C.c_float), _ptr(x, C.c_float), x.shape[0],
+                              _ptr(out, C.c_float)))
+    return out
+
+
+def mlp_gradient(params, scaling, inp, tgt):
+    lib = load()
+    p = np.ascontiguousarray(params, np.float32)
+    sc = np.ascontiguousarray(scaling, np.float32)
+    x = np.ascontiguousarray(inp, np.float32)
+    t = np.ascontiguousarray(tgt, np.float32)
+    g = np.zeros(MLP_NPARAMS, np.float32)
+    loss = C.c_float(0)
+    check(lib.mpm_mlp_gradient(_ptr(p, C.c_float), _ptr(sc, C.c_float), _ptr(x, C.c_float), _ptr(t, C.c_float),
+                               x.shape[0], _ptr(g, C.c_float), C.byref(loss)))
+    return loss.value, g
+
+
+def mlp_train(params, scaling, inp, tgt, epochs=10, batch=4096, lr=2e-3, lr_final=5e-5, seed=1):
+    lib = load()
+    p = np.array(params, np.float32)
+    sc = np.ascontiguousarray(scaling, np.float32)
+    x = np.ascontiguousarray(inp, np.float32)
+    t = np.ascontiguousarray(tgt, np.float32)
+    hist = np.zeros(epochs, np.float32)
+    tp = TrainParams(epochs, batch, lr, lr_final, seed, 0)
+    check(lib.mpm_mlp_train(_ptr(p, C.c_float), _ptr(sc, C.c_float), _ptr(x, C.c_float), _ptr(t, C.c_float),
+                            x.shape[0], C.byref(tp), _ptr(hist, C.c_float)))
+    return p, hist
 
 
 def maksf_batch(grid, cells, xyg):

@@ -188,4 +188,51 @@ def tiny(n_cells=8, ppc=(2, 2, 2), model=capi.MODEL_NEOHOOKEAN, props=(5e5, 0.35
                  nlstep=nlstep, bcs=bcs, parts=parts)
 
 
-CONFIGS = {"cfg1": cfg1, "cfg2": cfg2}
+def cfg3(steps=2000, h=1.0 / 150.0, rho=2000.0, dt=4e-5, seed=0x5EED0003):
+    """Column collapse with J2 plasticity (linear hardening E=2e6, nu=0.3,
+    sigma_y=2e3, H=1e4): 192x96x96 cells. The spacing is not given for this
+    case; h = 1/150 makes the domain [0,1.28]x[0,0.64]x[0,0.64] and the column
+    [0,0.3]x[0,0.6]x[0,0.5] exactly 45x90x75 cells, at 2x2x2 points per cell
+    2,430,000 particles. rho = 2000 and dt = 4e-5 (about 0.22 of the elastic
+    CFL limit h/c) are likewise choices of this build. Gravity -y; floor nodes
+    (y = 0) fixed; x = 0 and z = 0 planes are symmetry rollers (u = 0, w = 0);
+    the far walls are rollers too."""
+    nelem = (192, 96, 96)
+    xmin = (0.0, 0.0, 0.0)
+    xmax = (192 * h, 96 * h, 96 * h)
+    dx = (h, h, h)
+    hi = (int(round(0.3 / h)), int(round(0.6 / h)), int(round(0.5 / h)))
+    coord = seed_block(xmin, dx, nelem, (0, 0, 0), hi, (2, 2, 2))
+    n = len(coord)
+    volp = h ** 3 / 8.0
+    vel = uniform(seed, 3 * n, -1e-3, 1e-3).reshape(n, 3)
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    floor = node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0)
+    xwalls = node_lists(nelem, lambda i, j, k, nx, ny, nz: (i == 0) | (i == nx - 1) | (j == 0))
+    zwalls = node_lists(nelem, lambda i, j, k, nx, ny, nz: (k == 0) | (k == nz - 1) | (j == 0))
+    bcs = {0: _bc(xwalls), 1: _bc(floor), 2: _bc(zwalls)}
+    return Scene("cfg3-j2-column-%dk" % (n // 1000), xmin, xmax, nelem,
+                 [(capi.MODEL_J2, [2e6, 0.3, 2e3, 1e4])], rho, dt, steps, (0.0, -9.81, 0.0), bcs=bcs, parts=parts)
+
+
+def tiny_j2(n_cells=12, dt=4e-5, sigma_y=2e3, shear=0.0):
+    """Small J2 column for parity tests: the cfg3 column on an n_cells^3 grid
+    of spacing 1/n_cells; `shear` adds an initial velocity v_x = shear * y."""
+    h = 1.0 / n_cells
+    nelem = (n_cells,) * 3
+    xmin, xmax = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    hi = (max(1, int(0.3 * n_cells)), max(1, int(0.6 * n_cells)), max(1, int(0.5 * n_cells)))
+    coord = seed_block(xmin, (h, h, h), nelem, (1, 0, 1), (1 + hi[0], hi[1], 1 + hi[2]), (2, 2, 2))
+    n = len(coord)
+    rho = 2000.0
+    volp = h ** 3 / 8.0
+    vel = uniform(0x5EED0013, 3 * n, -1e-3, 1e-3).reshape(n, 3)
+    vel[:, 0] += shear * coord[:, 1]
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    floor = node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0)
+    bcs = {0: _bc(floor), 1: _bc(floor), 2: _bc(floor)}
+    return Scene("tiny-j2", xmin, xmax, nelem, [(capi.MODEL_J2, [2e6, 0.3, sigma_y, 1e4])], rho, dt, 10,
+                 (0.0, -9.81, 0.0), bcs=bcs, parts=parts)
+
+
+CONFIGS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3}

@@ -135,6 +135,20 @@ void orc_update_stress(int model, double stress[6], const double F[9], const dou
 int orc_j2_return(const double prop[8], double stress[6], double *eps_p, const double deps[6], double *dgamma,
                   double *resid);
 
+/* Constitutive-surrogate MLP 16 -> 64 -> 64 -> 16 (ReLU), no reference
+ * counterpart. params: 6288 values [w1 64x16 | b1 64 | w2 64x64 | b2 64 |
+ * w3 16x64 | b3 16], weights [out][in]; scaling: 64 values [in_mu 16 |
+ * in_isd 16 | out_mu 16 | out_sd 16]; in/out/tgt rows of 16 floats.
+ * bf16 = 1 rounds weights, inputs, hidden activations and back-propagated
+ * activation gradients to bfloat16 where the GPU kernels do; bf16 = 0 is the
+ * exact FP64 network. */
+#define ORC_MLP_NPARAMS 6288
+void orc_mlp_forward(const double *params, const double *scaling, const float *in, long n, int bf16, double *out);
+/* gradient of L = 1/(2n) sum_i sum_{o<7} (y_io - t_io)^2 in normalised
+ * output units; returns L */
+double orc_mlp_grad(const double *params, const double *scaling, const float *in, const float *tgt, long n, int bf16,
+                    double *grad);
+
 double orc_det3(const double m[9]);
 void orc_inv3(const double m[9], double out[9]);
 

@@ -68,7 +68,37 @@ def load(fast=False):
     lib.orc_inv3.argtypes = [_dp, _dp]
     lib.orc_bc_set_val.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, _dp, C.c_double]
     lib.orc_bc_set_zero.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, _dp]
+    _fp = C.POINTER(C.c_float)
+    lib.orc_mlp_forward.argtypes = [_dp, _dp, _fp, C.c_long, C.c_int, _dp]
+    lib.orc_mlp_grad.argtypes = [_dp, _dp, _fp, _fp, C.c_long, C.c_int, _dp]
+    lib.orc_mlp_grad.restype = C.c_double
     return lib
+
+
+MLP_NPARAMS = 6288
+
+
+def mlp_forward(params, scaling, inp, bf16=True):
+    """Oracle forward pass: params (6288,), scaling (64,), inp (n, 16) float32 -> (n, 16) float64."""
+    lib = load()
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    sc = np.ascontiguousarray(scaling, dtype=np.float64)
+    x = np.ascontiguousarray(inp, dtype=np.float32)
+    out = np.zeros((x.shape[0], 16), dtype=np.float64)
+    lib.orc_mlp_forward(P(p), P(sc), P(x, C.c_float), x.shape[0], int(bf16), P(out))
+    return out
+
+
+def mlp_grad(params, scaling, inp, tgt, bf16=True):
+    """Oracle loss and gradient (6288,) of the mean squared error in normalised units."""
+    lib = load()
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    sc = np.ascontiguousarray(scaling, dtype=np.float64)
+    x = np.ascontiguousarray(inp, dtype=np.float32)
+    t = np.ascontiguousarray(tgt, dtype=np.float32)
+    g = np.zeros(MLP_NPARAMS, dtype=np.float64)
+    loss = lib.orc_mlp_grad(P(p), P(sc), P(x, C.c_float), P(t, C.c_float), x.shape[0], int(bf16), P(g))
+    return loss, g
 
 
 def load_ref():

@@ -1,5 +1,5 @@
 """The engine library loads on a machine without a GPU and exports every
-entry point declared in include/mpm/mpm_capi.h (no compute calls here)."""
+entry point declared in include/mpm/*.h (no compute calls here)."""
 import ctypes as C
 import os
 import re
@@ -11,15 +11,21 @@ from mpm_mi355x import capi
 
 
 def declared():
-    src = open(capi.HEADER_PATH).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(mpm_[a-z0-9_]+)\s*\(", src)))
+    inc = os.path.dirname(capi.HEADER_PATH)
+    names = set()
+    for f in sorted(os.listdir(inc)):
+        if f.endswith(".h"):
+            src = open(os.path.join(inc, f)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names |= set(re.findall(r"\b(mpm_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
 
 
 def test_header_declares_the_reference_pass_surface():
     names = declared()
     for n in ("mpm_bin", "mpm_p2g", "mpm_solve_grid", "mpm_g2p", "mpm_step", "mpm_maksf_batch",
-              "mpm_engine_create", "mpm_set_bc", "mpm_download_bins"):
+              "mpm_engine_create", "mpm_set_bc", "mpm_download_bins", "mpm_mlp_forward", "mpm_mlp_gradient",
+              "mpm_mlp_train", "mpm_surrogate_fit", "mpm_surrogate_enable"):
         assert n in names
 
 

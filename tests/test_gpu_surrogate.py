@@ -1,0 +1,207 @@
+"""J2 constitutive surrogate on the GPU (no reference counterpart).
+
+- The BF16-MFMA forward and backward kernels against the FP64 oracle with
+  the same bfloat16 rounding points (tolerances below, FP32 accumulation).
+- Training on the GPU lowers the loss.
+- Engine: logged samples obey the exact radial return (oracle), a fitted
+  network is used only where its return-mapping residual is within the
+  tolerance: tol = 0 reproduces the plain method bit for bit, and the
+  stress error against the plain method is measured and bounded.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle
+from mpm_mi355x import capi, scenes
+from test_oracle_mlp import rand_params, rand_rows, rand_scaling
+
+pytestmark = pytest.mark.gpu
+
+OFF = {"w1": (0, 1024), "b1": (1024, 1088), "w2": (1088, 5184), "b2": (5184, 5248), "w3": (5248, 6272),
+       "b3": (6272, 6279)}
+
+
+@pytest.mark.parametrize("n", [1, 64, 65, 1000])
+def test_mlp_forward_matches_oracle(n):
+    p, sc = rand_params(21), rand_scaling(21)
+    x, _ = rand_rows(21, n, sc)
+    print("forward n=%d" % n, flush=True)
+    got = capi.mlp_forward(p.astype(np.float32), sc.astype(np.float32), x)
+    ref = pyoracle.mlp_forward(p.astype(np.float32).astype(np.float64), sc.astype(np.float32).astype(np.float64), x,
+                               bf16=True)
+    gn = (got[:, :7] - sc[32:39]) / sc[48:55]
+    rn = (ref[:, :7] - sc[32:39]) / sc[48:55]
+    # normalised outputs are O(1); bf16 hidden activations may round
+    # differently after FP32 (GPU) vs FP64 (oracle) accumulation
+    assert np.max(np.abs(gn - rn)) < 2e-2
+    assert np.sqrt(np.mean((gn - rn) ** 2)) < 2e-3
+    assert np.all(got[:, 7:] == 0.0)
+
+
+@pytest.mark.parametrize("n", [64, 333, 5000])
+def test_mlp_gradient_matches_oracle(n):
+    p, sc = rand_params(31), rand_scaling(31)
+    x, t = rand_rows(31, n, sc)
+    print("gradient n=%d" % n, flush=True)
+    p32, sc32 = p.astype(np.float32), sc.astype(np.float32)
+    loss, g = capi.mlp_gradient(p32, sc32, x, t)
+    lref, gref = pyoracle.mlp_grad(p32.astype(np.float64), sc32.astype(np.float64), x, t, bf16=True)
+    assert abs(loss - lref) <= 5e-3 * lref
+    for name, (a, b) in OFF.items():
+        ref = gref[a:b]
+        assert np.linalg.norm(g[a:b] - ref) <= 3e-2 * np.linalg.norm(ref), name
+    assert np.all(g[6279:] == 0.0)
+    w1 = g[0:1024].reshape(64, 16)
+    assert np.all(w1[:, 13:] == 0.0)
+
+
+def test_mlp_training_lowers_the_loss():
+    # a smooth target that a 64-64 ReLU network can fit: y = A tanh(B x)
+    rng = np.random.default_rng(3)
+    n = 20000
+    x = np.zeros((n, 16), np.float32)
+    x[:, :13] = rng.normal(0, 1, (n, 13))
+    A = rng.normal(0, 1, (7, 5))
+    B = rng.normal(0, 0.5, (5, 13))
+    t = np.zeros((n, 16), np.float32)
+    t[:, :7] = (np.tanh(x[:, :13] @ B.T) @ A.T).astype(np.float32)
+    sc = np.zeros(64, np.float32)
+    sc[16:29] = 1.0
+    sc[32:39] = t[:, :7].mean(0)
+    sc[48:55] = t[:, :7].std(0)
+    sc[55:64] = 1.0
+    p0 = rand_params(41).astype(np.float32)
+    l0, _ = capi.mlp_gradient(p0, sc, x, t)
+    p1, hist = capi.mlp_train(p0, sc, x, t, epochs=30, batch=512, lr=3e-3, lr_final=1e-4, seed=5)
+    l1, _ = capi.mlp_gradient(p1, sc, x, t)
+    print("loss %.4g -> %.4g, epochs %s" % (l0, l1, np.array2string(hist, precision=4)), flush=True)
+    assert np.all(np.isfinite(hist))
+    assert l1 < 0.05 * l0
+    assert hist[-1] < hist[0]
+    # deterministic: same seed, same result
+    p2, hist2 = capi.mlp_train(p0, sc, x, t, epochs=30, batch=512, lr=3e-3, lr_final=1e-4, seed=5)
+    assert np.array_equal(p1, p2) and np.array_equal(hist, hist2)
+
+
+# ------------------------------------------------------------------ engine
+J2_PROP = (2e6, 0.3, 200.0, 1e4)
+
+
+def j2_scene():
+    return scenes.tiny_j2(n_cells=12, sigma_y=J2_PROP[2], shear=5.0)
+
+
+def fp32_engine(s):
+    e = capi.Engine(s, precision=capi.PREC_FP32, relayout_every=0)
+    e.upload(s.parts)
+    return e
+
+
+@pytest.fixture(scope="module")
+def trained():
+    s = j2_scene()
+    e = fp32_engine(s)
+    e.surrogate_collect(True, capacity=1 << 18, rate=1.0)
+    assert e.step(1, 60) == 0
+    st = e.surrogate_stats()
+    assert st["logged_steps"] == 60 and st["samples"] > 1000
+    loss = e.surrogate_fit(epochs=40, batch=2048, lr=3e-3, lr_final=5e-5, seed=1)
+    xin, tgt = e.surrogate_samples(1 << 18)
+    params, scaling = e.surrogate_params()
+    e.close()
+    return {"loss": loss, "x": xin, "t": tgt, "params": params, "scaling": scaling, "samples": st["samples"]}
+
+
+def test_logged_samples_follow_the_exact_return(trained):
+    lib = pyoracle.load()
+    x, t = trained["x"], trained["t"]
+    assert len(x) == trained["samples"]
+    E, nu, sy, H = J2_PROP
+    mu = E / (2 * (1 + nu))
+    prop = np.array([E, nu, sy, H, 0, 0, 0, 0], np.float64)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(x), size=min(400, len(x)), replace=False)
+    scale = np.max(np.abs(t[:, :6]))
+    gscale = np.max(np.abs(t[:, 6]))
+    for i in idx:
+        de = x[i, 0:6].astype(np.float64)
+        ee = x[i, 6:12].astype(np.float64)
+        ep = np.array([float(x[i, 12])])
+        tr = ee[:3].sum()
+        lmd = E * nu / ((1 + nu) * (1 - 2 * nu))
+        s = np.empty(6)
+        s[:3] = lmd * tr + 2 * mu * ee[:3]
+        s[3:] = mu * ee[3:]
+        s_tr = s.copy()
+        s_tr[:3] += lmd * de[:3].sum() + 2 * mu * de[:3]
+        s_tr[3:] += mu * de[3:]
+        dg, res = C.c_double(0), C.c_double(0)
+        plastic = lib.orc_j2_return(pyoracle.P(prop), pyoracle.P(s), pyoracle.P(ep), pyoracle.P(de), C.byref(dg),
+                                    C.byref(res))
+        # a trial state within FP32 rounding of the yield surface may come out
+        # elastic in FP64: then its logged correction is at rounding level
+        assert plastic == 1 or t[i, 6] <= 1e-4 * gscale
+        assert np.max(np.abs((s_tr - s) - t[i, :6])) <= 1e-3 * scale
+        assert abs(dg.value - t[i, 6]) <= 1e-3 * gscale
+    # samples are sorted by (step, particle id): keys are not stored, but the
+    # order is reproducible
+    assert np.all(t[:, 6] > 0.0)
+
+
+def test_fit_reduces_the_loss(trained):
+    # predicting the mean gives 0.5 per used output = 3.5
+    print("fit: %d samples, final loss %.4g" % (trained["samples"], trained["loss"]), flush=True)
+    assert trained["loss"] < 0.1
+
+
+def run_pair(params, scaling, tol, nsteps=40):
+    s = j2_scene()
+    plain = fp32_engine(s)
+    assert plain.step(1, nsteps) == 0
+    a = plain.download(s.n)
+    plain.close()
+    sur = fp32_engine(s)
+    sur.surrogate_set_params(params, scaling)
+    sur.surrogate_enable(True, tol)
+    assert sur.step(1, nsteps) == 0
+    b = sur.download(s.n)
+    st = sur.surrogate_stats()
+    sur.close()
+    return a, b, st
+
+
+def test_tolerance_zero_reproduces_the_plain_method(trained):
+    a, b, st = run_pair(trained["params"], trained["scaling"], 0.0)
+    print("tol 0: evaluated %d, fallbacks %d" % (st["evaluated"], st["fallbacks"]), flush=True)
+    assert st["evaluated"] > 0 and st["fallbacks"] == st["evaluated"]
+    for f in ("coord", "vel", "stress", "eps_p", "def_grad"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def stress_err(a, b):
+    return np.linalg.norm(b.stress - a.stress) / np.linalg.norm(a.stress)
+
+
+def test_residual_tolerance_bounds_the_stress_error(trained):
+    a, b, st = run_pair(trained["params"], trained["scaling"], 1e-4)
+    err = stress_err(a, b)
+    rate = st["fallbacks"] / max(1, st["evaluated"])
+    print("tol 1e-4: evaluated %d, fallback rate %.4f, rel stress err %.3g" % (st["evaluated"], rate, err),
+          flush=True)
+    assert st["evaluated"] > 0
+    assert err <= 1e-3
+
+
+def test_network_alone_error_is_measured(trained):
+    # every plastic state takes the network's answer: the error compounds
+    # through the stress history, so this only checks that the path runs and
+    # stays finite and prints the figure
+    a, b, st = run_pair(trained["params"], trained["scaling"], 1e30)
+    err = stress_err(a, b)
+    print("tol inf: evaluated %d, rel stress err %.3g, max abs %.3g" % (
+        st["evaluated"], err, np.max(np.abs(b.stress - a.stress))), flush=True)
+    assert st["fallbacks"] == 0 and st["evaluated"] > 0
+    assert np.all(np.isfinite(b.stress))
+    assert err < 2.0
