@@ -383,7 +383,10 @@ int validate_grid(const mpm_grid_desc *g, bool allow_quadratic) {
         if (g->nelem[i] <= 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "nelem must be positive");
         if (!(g->xmax[i] > g->xmin[i])) return set_error(MPM_ERR_INVALID_ARGUMENT, "xmax must exceed xmin");
         if (g->idimc[i] != 1 && !(allow_quadratic && g->idimc[i] == 2))
-            return set_error(MPM_ERR_UNSUPPORTED, "shape-function order must be 1 (linear) for the step engine");
+            return set_error(MPM_ERR_UNSUPPORTED, allow_quadratic
+                                                      ? "shape-function order must be 1 or 2"
+                                                      : "the FP32 engine takes the linear shape function only "
+                                                        "(idimc = 1); the FP64-exact engine also takes 2");
     }
     double ne = (double)g->nelem[0] * g->nelem[1] * g->nelem[2];
     double nn = (double)(g->nelem[0] + g->idimc[0]) * (g->nelem[1] + g->idimc[1]) * (g->nelem[2] + g->idimc[2]);
@@ -419,7 +422,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (nmat < 1 || nmat > kMaxMat) return set_error(MPM_ERR_INVALID_ARGUMENT, "nmat must be in 1..8");
-    int rc = validate_grid(grid, false);
+    int rc = validate_grid(grid, params && params->precision == MPM_PREC_FP64_EXACT);
     if (rc) return rc;
     if (!(params->dt > 0.0)) return set_error(MPM_ERR_INVALID_ARGUMENT, "dt must be positive");
     if (params->scheme != MPM_SCHEME_PIC && params->scheme != MPM_SCHEME_FLIP)

@@ -240,8 +240,9 @@ class Engine:
         lib = load()
         self.lib = lib
         self.scene = scene
-        g = grid_desc(scene.xmin, scene.xmax, scene.nelem)
-        self.nodec = int(np.prod(np.asarray(scene.nelem) + 1))
+        idimc = tuple(getattr(scene, "idimc", (1, 1, 1)))
+        g = grid_desc(scene.xmin, scene.xmax, scene.nelem, idimc)
+        self.nodec = int(np.prod(np.asarray(scene.nelem) + np.asarray(idimc)))
         self.nelem = int(np.prod(scene.nelem))
         mats = (MaterialDesc * len(scene.materials))()
         for i, (model, props) in enumerate(scene.materials):

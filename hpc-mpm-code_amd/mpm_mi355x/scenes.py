@@ -61,6 +61,7 @@ class Scene:
     nlstep: int = 1
     bcs: dict = field(default_factory=dict)   # comp -> (nbc, fbc)
     parts: object = None
+    idimc: tuple = (1, 1, 1)                 # 1 = linear tent, 2 = quadratic B-spline (FP64-exact engine)
 
     @property
     def dx(self):
@@ -68,7 +69,7 @@ class Scene:
 
     @property
     def nodec(self):
-        return int(np.prod(np.asarray(self.nelem) + 1))
+        return int(np.prod(np.asarray(self.nelem) + np.asarray(self.idimc)))
 
     @property
     def nelem_total(self):
@@ -103,9 +104,9 @@ def seed_block(xmin, dx, nelem, lo_cells, hi_cells, npxye):
     return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
 
 
-def node_lists(nelem, select):
+def node_lists(nelem, select, idimc=(1, 1, 1)):
     """Node ids (reference numbering) in id order where select(i, j, k) holds."""
-    nx, ny, nz = nelem[0] + 1, nelem[1] + 1, nelem[2] + 1
+    nx, ny, nz = nelem[0] + idimc[0], nelem[1] + idimc[1], nelem[2] + idimc[2]
     K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     ids = (I + nx * J + nx * ny * K).ravel()
     mask = select(I.ravel(), J.ravel(), K.ravel(), nx, ny, nz)
@@ -167,7 +168,7 @@ def cfg2(steps=1000, ppc=(2, 2, 2), n_cells=128, seed=0x5EED0002):
 
 
 def tiny(n_cells=8, ppc=(2, 2, 2), model=capi.MODEL_NEOHOOKEAN, props=(5e5, 0.35), scheme=capi.SCHEME_PIC,
-         seed=0x5EED0009, dt=1e-5, omega=2.0, gravity=(0.0, -9.81, 0.0), nlstep=1):
+         seed=0x5EED0009, dt=1e-5, omega=2.0, gravity=(0.0, -9.81, 0.0), nlstep=1, idimc=(1, 1, 1)):
     """Small 3D scene for parity tests: the cfg2 geometry on an n_cells^3 grid."""
     h = 1.0 / n_cells
     xmin, xmax, nelem = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (n_cells,) * 3
@@ -182,10 +183,10 @@ def tiny(n_cells=8, ppc=(2, 2, 2), model=capi.MODEL_NEOHOOKEAN, props=(5e5, 0.35
     vel[:, 2] = -omega * (coord[:, 0] - 0.5)
     vel += uniform(seed, 3 * n, -1e-3, 1e-3).reshape(n, 3)
     parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
-    floor = node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0)
+    floor = node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0, idimc)
     bcs = {0: _bc(floor), 1: _bc(floor), 2: _bc(floor)}
     return Scene("tiny", xmin, xmax, nelem, [(model, list(props))], rho, dt, 10, gravity, scheme=scheme,
-                 nlstep=nlstep, bcs=bcs, parts=parts)
+                 nlstep=nlstep, bcs=bcs, parts=parts, idimc=tuple(idimc))
 
 
 def cfg3(steps=2000, h=1.0 / 150.0, rho=2000.0, dt=4e-5, seed=0x5EED0003):

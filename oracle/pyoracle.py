@@ -130,7 +130,7 @@ class Oracle:
             st.xmin[i] = scene.xmin[i]
             st.xmax[i] = scene.xmax[i]
             st.nex[i] = scene.nelem[i]
-            st.idimc[i] = 1
+            st.idimc[i] = int(getattr(scene, "idimc", (1, 1, 1))[i])
         self.lib.orc_grid_init(C.byref(st))
         st.dt = scene.dt
         st.mtol = scene.mtol

@@ -316,3 +316,26 @@ def test_fp32_j2_run():
     assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-7
     assert rel(got.stress, o.parts.stress) < 2e-3    # plastic states, FP32
     assert rel(got.eps_p, o.parts.eps_p) < 1e-2
+
+
+# ------------------------------------------------------------------ quadratic B-spline (idimc = 2)
+@pytest.mark.parametrize("idimc", [(2, 2, 2), (2, 1, 2)])
+@pytest.mark.parametrize("scheme", [capi.SCHEME_PIC, capi.SCHEME_FLIP])
+def test_fp64_quadratic_bspline_run_bitexact(idimc, scheme):
+    # the reference's production shape (input.txt idimc = 2 1 2); the block
+    # sits on the y = 0 boundary, so the boundary branches of MakSf are used
+    s = scenes.tiny(n_cells=8, model=capi.MODEL_LINELAST, props=(1e6, 0.3), scheme=scheme, idimc=idimc)
+    print("quadratic idimc=%s scheme=%d n=%d" % (idimc, scheme, s.n), flush=True)
+    o = oracle_run(s, 20)
+    e = engine(s, capi.PREC_FP64_EXACT)
+    assert e.step(1, 20) == 0
+    assert_parts_equal(e.download(s.n), o.parts)
+    nd = e.nodes()
+    assert np.array_equal(nd.nmass, o.nmass)
+
+
+def test_fp32_rejects_quadratic_bspline():
+    s = scenes.tiny(n_cells=8, idimc=(2, 2, 2))
+    with pytest.raises(capi.MPMError) as ei:
+        capi.Engine(s, precision=capi.PREC_FP32)
+    assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
