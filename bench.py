@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="cfg2-4M", choices=["cfg1", "cfg2", "cfg2-4M"])
+    ap.add_argument("--config", default="cfg2-4M", choices=["cfg1", "cfg2", "cfg2-4M", "cfg3", "cfg5"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--relayout", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -75,6 +75,10 @@ def parse():
 def make_scene(name):
     if name == "cfg1":
         return scenes.cfg1()
+    if name == "cfg3":
+        return scenes.cfg3()
+    if name == "cfg5":
+        return scenes.cfg5()
     if name == "cfg2":
         return scenes.cfg2(ppc=(2, 2, 2))
     return scenes.cfg2(ppc=(2, 4, 2))

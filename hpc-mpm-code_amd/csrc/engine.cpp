@@ -432,7 +432,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     if (params->nlstep < 1) return set_error(MPM_ERR_INVALID_ARGUMENT, "nlstep must be >= 1");
     for (int i = 0; i < nmat; i++) {
         int m = mats[i].model;
-        if (!(m == -1 || m == 0 || m == 1 || m == 2 || m == 3 || m == 10))
+        if (!(m == -1 || m == 0 || m == 1 || m == 2 || m == 3 || m == 10 || m == 20))
             return set_error(MPM_ERR_UNSUPPORTED, "unknown constitutive model");
     }
     int ndev = 0;

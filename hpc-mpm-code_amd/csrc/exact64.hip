@@ -322,7 +322,7 @@ __device__ __forceinline__ void bmat(const double F[9], double be[9]) {
 // ConstitutiveModel::UpdateStress (constitutive_model.cpp:287-305), FP64,
 // same expression order as the reference. J2 has no reference counterpart.
 __device__ void update_stress64(const MaterialDev &md, double *sig, size_t ld, int pid, const double F[9],
-                                const double dF[9], double jac, double *eps_p) {
+                                const double dF[9], double jac, double *eps_p, double dt) {
     const double *p = md.prop;
     switch (md.model) {
     case -1:
@@ -395,6 +395,22 @@ __device__ void update_stress64(const MaterialDev &md, double *sig, size_t ld, i
             s[4 * i] += hp - 2.0 / (3.0 * jac) * (c1 * I1 + 2.0 * c2 * I2);
         }
         store_voigt(sig, ld, pid, s);
+        break;
+    }
+    case 20: {  // Tait equation of state + Newtonian viscosity (no reference counterpart)
+        const double rho0 = p[0], c = p[1], gam = p[2], mu = p[3];
+        const double B = rho0 * c * c / gam;
+        const double pr = B * expm1(-gam * log1p(jac - 1.0));
+        const double idt = 1.0 / dt;
+        double l[9];
+        for (int q = 0; q < 9; q++) l[q] = (dF[q] - ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0)) * idt;
+        const double tr = (l[0] + l[4] + l[8]) / 3.0;
+        sig[pid] = -pr + 2.0 * mu * (l[0] - tr);
+        sig[ld + pid] = -pr + 2.0 * mu * (l[4] - tr);
+        sig[2 * ld + pid] = -pr + 2.0 * mu * (l[8] - tr);
+        sig[3 * ld + pid] = mu * (l[5] + l[7]);
+        sig[4 * ld + pid] = mu * (l[6] + l[2]);
+        sig[5 * ld + pid] = mu * (l[1] + l[3]);
         break;
     }
     case 10: {  // J2 radial return, small-increment form (Simo & Hughes 1998, Box 3.2)
@@ -492,7 +508,7 @@ __global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable 
     P.x[2 * ld + pid] = dx2;
     P.V[pid] = J * P.V0[pid];
     int mat = P.matid ? P.matid[pid] : 0;
-    update_stress64(mt.m[mat], P.sig, ld, pid, F, dd, J, P.eps_p);
+    update_stress64(mt.m[mat], P.sig, ld, pid, F, dd, J, P.eps_p, dt);
     if (!(J > 0.0)) atomicOr(err, kErrJacobian);
     if (!isfinite(dx0) || !isfinite(dx1) || !isfinite(dx2)) atomicOr(err, kErrNonFinite);
 }

@@ -642,7 +642,7 @@ __device__ __forceinline__ void j2_return(const J2Trial &t, float s[6], float *e
 }
 
 __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9], const float D[9], double J,
-                                         float s[6], float *eps_p) {
+                                         float s[6], float *eps_p, float inv_dt) {
     const double *p = md.prop;
     switch (md.model) {
     case -1:
@@ -738,6 +738,20 @@ __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9]
         s[3] = sg[5];
         s[4] = sg[2];
         s[5] = sg[1];
+        break;
+    }
+    case 20: {  // Tait equation of state + Newtonian viscosity; p without cancellation near J = 1
+        const float rho0 = (float)p[0], c = (float)p[1], gam = (float)p[2], mu = (float)p[3];
+        const float B = rho0 * c * c / gam;
+        const float pr = B * expm1f(-gam * log1pf((float)(J - 1.0)));
+        const float tr = (D[0] + D[4] + D[8]) * (1.f / 3.f);
+        const float m2 = 2.f * mu * inv_dt, m1 = mu * inv_dt;
+        s[0] = -pr + m2 * (D[0] - tr);
+        s[1] = -pr + m2 * (D[4] - tr);
+        s[2] = -pr + m2 * (D[8] - tr);
+        s[3] = m1 * (D[5] + D[7]);
+        s[4] = m1 * (D[6] + D[2]);
+        s[5] = m1 * (D[1] + D[3]);
         break;
     }
     case 10: {  // J2 radial return, small-increment form
@@ -918,7 +932,7 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     }
     float ep = (md.model == 10 && P.eps_p) ? P.eps_p[slot] : 0.f;
     if constexpr (SUR == 0) {
-        stress32(md, Hn, D, J, s, &ep);
+        stress32(md, Hn, D, J, s, &ep, 1.f / sc.dt);
     } else {
         const bool j2 = md.model == 10;
         J2Trial t;
@@ -927,7 +941,7 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
             j2_trial(md.prop, D, s, ep, t);
             plastic = valid && t.f > 0.f;
         } else {
-            stress32(md, Hn, D, J, s, &ep);
+            stress32(md, Hn, D, J, s, &ep, 1.f / sc.dt);
         }
         const int lane = threadIdx.x & 63;
         if constexpr (SUR == 1) {

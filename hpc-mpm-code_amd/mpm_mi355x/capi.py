@@ -25,6 +25,7 @@ MPM_ERR_NO_DEVICE = -7
 SCHEME_PIC, SCHEME_FLIP = 0, 1
 PREC_FP64_EXACT, PREC_FP32 = 0, 1
 MODEL_RIGID, MODEL_LINELAST, MODEL_NEOHOOKEAN, MODEL_STVK, MODEL_MOONEYRIVLIN, MODEL_J2 = -1, 0, 1, 2, 3, 10
+MODEL_TAIT = 20
 OPT_FULL_NODE_OUTPUT = 1
 OPT_LANES_PER_CELL = 2
 

@@ -237,3 +237,51 @@ def tiny_j2(n_cells=12, dt=4e-5, sigma_y=2e3, shear=0.0):
 
 
 CONFIGS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3}
+
+
+TAIT_PROPS = [1000.0, 30.0, 7.0, 1e-3]  # rho0, c, gamma, mu
+
+
+def _walls(nelem, idimc=(1, 1, 1)):
+    """Free-slip box: u = 0 on x walls, v = 0 on floor and ceiling, w = 0 on z walls."""
+    xw = node_lists(nelem, lambda i, j, k, nx, ny, nz: (i == 0) | (i == nx - 1), idimc)
+    yw = node_lists(nelem, lambda i, j, k, nx, ny, nz: (j == 0) | (j == ny - 1), idimc)
+    zw = node_lists(nelem, lambda i, j, k, nx, ny, nz: (k == 0) | (k == nz - 1), idimc)
+    return {0: _bc(xw), 1: _bc(yw), 2: _bc(zw)}
+
+
+def cfg5(steps=2000, seed=0x5EED0005):
+    """Dam break, weakly compressible Tait fluid (rho0 = 1000, c = 30,
+    gamma = 7, mu = 1e-3): 256x128x128 cells of h = 1/128 on [0,2]x[0,1]x[0,1];
+    water column [0,0.5]x[0,0.5]x[0,1] = 64x64x128 cells at 2x2x2 points per
+    cell = 4,194,304 particles; gravity -y; free-slip walls; dt = 2e-5."""
+    h = 1.0 / 128.0
+    nelem = (256, 128, 128)
+    xmin, xmax = (0.0, 0.0, 0.0), (2.0, 1.0, 1.0)
+    coord = seed_block(xmin, (h, h, h), nelem, (0, 0, 0), (64, 64, 128), (2, 2, 2))
+    n = len(coord)
+    volp = h ** 3 / 8.0
+    rho = TAIT_PROPS[0]
+    vel = uniform(seed, 3 * n, -1e-3, 1e-3).reshape(n, 3)
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    return Scene("cfg5-tait-dam-break-%dk" % (n // 1000), xmin, xmax, nelem, [(capi.MODEL_TAIT, list(TAIT_PROPS))],
+                 rho, 2e-5, steps, (0.0, -9.81, 0.0), bcs=_walls(nelem), parts=parts)
+
+
+def tiny_tait(n_cells=12, dt=2e-5, idimc=(1, 1, 1)):
+    """Small dam break for parity tests: water [0,1/4]x[0,1/2]x[0,1) in a unit
+    box of n_cells^3 cells."""
+    h = 1.0 / n_cells
+    nelem = (n_cells,) * 3
+    xmin, xmax = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    coord = seed_block(xmin, (h, h, h), nelem, (0, 0, 0), (max(1, n_cells // 4), n_cells // 2, n_cells), (2, 2, 2))
+    n = len(coord)
+    volp = h ** 3 / 8.0
+    rho = TAIT_PROPS[0]
+    vel = uniform(0x5EED0015, 3 * n, -1e-3, 1e-3).reshape(n, 3)
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    return Scene("tiny-tait", xmin, xmax, nelem, [(capi.MODEL_TAIT, list(TAIT_PROPS))], rho, dt, 10,
+                 (0.0, -9.81, 0.0), bcs=_walls(nelem, idimc), parts=parts, idimc=tuple(idimc))
+
+
+CONFIGS["cfg5"] = cfg5

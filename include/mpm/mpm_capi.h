@@ -44,16 +44,20 @@ typedef enum mpm_status {
 enum { MPM_SCHEME_PIC = 0, MPM_SCHEME_FLIP = 1 };
 
 /* Constitutive models; codes -1..3 are the reference's iprop values
- * (module/solid/constitutive_model.cpp:287-305). MPM_MODEL_J2 has no
- * reference counterpart (J2 plasticity with linear hardening,
- * prop = {E, nu, sigma_y, H}). */
+ * (module/solid/constitutive_model.cpp:287-305). MPM_MODEL_J2 and
+ * MPM_MODEL_TAIT have no reference counterpart: J2 plasticity with linear
+ * hardening, prop = {E, nu, sigma_y, H}; weakly compressible Newtonian fluid
+ * with the Tait equation of state, prop = {rho0, c, gamma, mu},
+ * p = (rho0 c^2 / gamma) ((rho / rho0)^gamma - 1) with rho = rho0 / J,
+ * sigma = -p I + 2 mu dev(sym(grad v)). */
 enum {
     MPM_MODEL_RIGID = -1,
     MPM_MODEL_LINELAST = 0,
     MPM_MODEL_NEOHOOKEAN = 1,
     MPM_MODEL_STVK = 2,
     MPM_MODEL_MOONEYRIVLIN = 3,
-    MPM_MODEL_J2 = 10
+    MPM_MODEL_J2 = 10,
+    MPM_MODEL_TAIT = 20
 };
 
 /* Arithmetic of the step kernels.
@@ -63,7 +67,8 @@ enum {
  *   arithmetic bit for bit.
  * MPM_PREC_FP32: FP32 particle/grid arithmetic with compensated (Kahan) node
  *   sums; det F and the particle volume in FP64, FP64 division at
- *   near-empty nodes. Positions carry a compensation term. */
+ *   near-empty nodes. Positions are stored as cell index + FP32 offset in
+ *   the cell. */
 enum { MPM_PREC_FP64_EXACT = 0, MPM_PREC_FP32 = 1 };
 
 typedef struct mpm_grid_desc {

@@ -577,11 +577,31 @@ void orc_g2p(orc_state *st, double facl) {
             const double *prop = st->mat_prop[mat];
             if (model == ORC_J2)
                 j2_update(st, pid, prop, st->ddg + 9 * pid);
+            else if (model == ORC_TAIT)
+                orc_tait_stress(prop, st->det_def_grad[pid], st->ddg + 9 * pid, dt, st->stress + 6 * pid);
             else
                 orc_update_stress(model, st->stress + 6 * pid, st->def_grad + 9 * pid, st->ddg + 9 * pid, prop,
                                   st->det_def_grad[pid], st->det_def_grad[pid]);
         }
     }
+}
+
+/* Tait equation of state + Newtonian viscosity (textbook weakly
+ * compressible fluid; pinned by closed forms in tests/test_oracle_invariants.py) */
+void orc_tait_stress(const double *prop, double jac, const double dF[9], double dt, double s[6]) {
+    const double rho0 = prop[0], c = prop[1], gam = prop[2], mu = prop[3];
+    const double B = rho0 * c * c / gam;
+    const double p = B * expm1(-gam * log1p(jac - 1.0));
+    const double idt = 1.0 / dt;
+    double l[9];
+    for (int q = 0; q < 9; q++) l[q] = (dF[q] - ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0)) * idt;
+    const double tr = (l[0] + l[4] + l[8]) / 3.0;
+    s[0] = -p + 2.0 * mu * (l[0] - tr);
+    s[1] = -p + 2.0 * mu * (l[4] - tr);
+    s[2] = -p + 2.0 * mu * (l[8] - tr);
+    s[3] = mu * (l[5] + l[7]);
+    s[4] = mu * (l[6] + l[2]);
+    s[5] = mu * (l[1] + l[3]);
 }
 
 /* work/src_solid/explicit/main.cpp:43-48 */

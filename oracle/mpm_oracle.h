@@ -42,7 +42,8 @@ enum {
     ORC_NEOHOOKEAN = 1,
     ORC_STVK = 2,
     ORC_MOONEYRIVLIN = 3,
-    ORC_J2 = 10
+    ORC_J2 = 10,
+    ORC_TAIT = 20
 };
 
 typedef struct orc_state {
@@ -148,6 +149,12 @@ void orc_mlp_forward(const double *params, const double *scaling, const float *i
  * output units; returns L */
 double orc_mlp_grad(const double *params, const double *scaling, const float *in, const float *tgt, long n, int bf16,
                     double *grad);
+
+/* Weakly compressible Newtonian fluid, no reference counterpart:
+ * prop = {rho0, c, gamma, mu}; p = (rho0 c^2 / gamma) ((rho/rho0)^gamma - 1)
+ * with rho = rho0 / J (Tait), evaluated as B expm1(-gamma log1p(J - 1));
+ * sigma = -p I + 2 mu dev(sym(L)), L = (dF - I) / dt. */
+void orc_tait_stress(const double *prop, double jac, const double dF[9], double dt, double s[6]);
 
 double orc_det3(const double m[9]);
 void orc_inv3(const double m[9], double out[9]);

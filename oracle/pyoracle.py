@@ -68,6 +68,7 @@ def load(fast=False):
     lib.orc_inv3.argtypes = [_dp, _dp]
     lib.orc_bc_set_val.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, _dp, C.c_double]
     lib.orc_bc_set_zero.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, _dp]
+    lib.orc_tait_stress.argtypes = [_dp, C.c_double, _dp, C.c_double, _dp]
     _fp = C.POINTER(C.c_float)
     lib.orc_mlp_forward.argtypes = [_dp, _dp, _fp, C.c_long, C.c_int, _dp]
     lib.orc_mlp_grad.argtypes = [_dp, _dp, _fp, _fp, C.c_long, C.c_int, _dp]

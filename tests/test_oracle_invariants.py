@@ -366,3 +366,42 @@ def test_j2_uniaxial_hardening_closed_form():
     tangent = 3 * mu * H / (3 * mu + H)
     # in pure shear q = sqrt(3)|tau| and dq = tangent * d(eps_eq) with d(eps_eq) = gam / sqrt(3)
     np.testing.assert_allclose(dq, tangent * gam / np.sqrt(3.0), rtol=1e-9)
+
+
+# ------------------------------------------------------------------ Tait fluid (no reference counterpart)
+def _tait(prop, jac, dF, dt):
+    lib = pyoracle.load()
+    s = np.zeros(6)
+    lib.orc_tait_stress(pyoracle.P(np.asarray(prop, np.float64)), jac, pyoracle.P(np.asarray(dF, np.float64)), dt,
+                        pyoracle.P(s))
+    return s
+
+
+@pytest.mark.parametrize("jac", [0.97, 0.999, 1.0, 1.004])
+def test_tait_pressure_closed_form(jac):
+    rho0, c, gam, mu = 1000.0, 30.0, 7.0, 1e-3
+    s = _tait([rho0, c, gam, mu], jac, np.eye(3).ravel(), 1e-5)
+    # p = rho0 c^2 / gamma ((rho / rho0)^gamma - 1), rho = rho0 / J; at rest no viscous part
+    p = rho0 * c * c / gam * ((1.0 / jac) ** gam - 1.0)
+    assert np.allclose(s[:3], -p, rtol=1e-12, atol=1e-9)
+    assert np.all(s[3:] == 0.0)
+    if jac < 1:
+        assert p > 0  # compression pushes back
+
+
+def test_tait_viscous_shear_and_dilatation():
+    rho0, c, gam, mu, dt = 1000.0, 30.0, 7.0, 0.5, 1e-4
+    g = 3.0  # shear rate dv_x/dy
+    dF = np.eye(3)
+    dF[0, 1] = g * dt  # L_xy = g
+    s = _tait([rho0, c, gam, mu], 1.0, dF.ravel(), dt)
+    assert np.allclose(s, [0, 0, 0, 0, 0, mu * g], rtol=1e-12, atol=1e-12)
+    # a pure dilatation rate has no deviatoric part: isotropic stress only
+    dF = np.eye(3) * (1 + 2e-3 * dt)
+    s = _tait([rho0, c, gam, mu], 1.0, dF.ravel(), dt)
+    assert np.allclose(s, 0.0, atol=1e-9)
+    # a deviatoric stretch rate: sigma_xx - sigma_yy = 2 mu (L_xx - L_yy)
+    dF = np.diag([1 + 1e-3 * dt, 1 - 1e-3 * dt, 1.0])
+    s = _tait([rho0, c, gam, mu], 1.0, dF.ravel(), dt)
+    assert np.isclose(s[0] - s[1], 2 * mu * 2e-3, rtol=1e-9)
+    assert np.isclose(s[0] + s[1] + s[2], 0.0, atol=1e-15)
