@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--lanes-per-cell", type=int, default=0, help="override (1, 2, 4, 8); 0 = automatic")
     ap.add_argument("--no-probe", action="store_true", help="skip the copy-bandwidth probe")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the J2 surrogate vs plain comparison")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-scene ensemble measurement")
+    ap.add_argument("--ens-scenes", type=int, default=256)
+    ap.add_argument("--ens-steps", type=int, default=500)
     ap.add_argument("--sur-collect", type=int, default=200, help="logged plain steps before the fit")
     ap.add_argument("--sur-samples", type=int, default=5_000_000)
     ap.add_argument("--sur-steps", type=int, default=100, help="timed steps of each method")
@@ -171,6 +174,43 @@ def surrogate_vs_plain(a):
     }
 
 
+def ensemble_run(a):
+    """256 independent 2D drop scenes (three grid-size buckets) advanced in
+    one captured HIP graph per 32-step block, after the autotuner picked the
+    cells-kernel shape per bucket; the eager (graph-free) rate is reported
+    beside it."""
+    scs, bucket = scenes.cfg4(a.ens_scenes)
+    ntot = sum(s.n for s in scs)
+    ens = capi.Ensemble(scs, bucket=bucket, precision=capi.PREC_FP32, relayout_every=16)
+    step = 1
+    ens.step(step, 2)
+    step += 2
+    tune = ens.autotune(step, trial_steps=4)
+    step += 9 * 4
+    t_eager = ens.step(step, 32)
+    step += 32
+    block = 32
+    ens.capture(step, block)
+    nlaunch = max(1, (a.ens_steps - (step - 1)) // block)
+    t_graph = ens.launch(nlaunch)
+    step += nlaunch * block
+    ens.close()
+    per_bucket = []
+    for r in tune:
+        nb = sum(s.n for s, b in zip(scs, bucket) if b == r["bucket"])
+        per_bucket.append({"grid": ("64x64", "128x128", "256x256")[r["bucket"]], "scenes": r["scenes"],
+                           "particles": nb, "lanes_per_cell": r["lanes_per_cell"],
+                           "relayout_every": r["relayout_every"],
+                           "particle_steps_per_s_eager": nb / (r["ms_per_step"] * 1e-3),
+                           "autotune_gain": r["ms_per_step_worst"] / r["ms_per_step"]})
+    return {"workload": "cfg4-ensemble-%d-scenes" % len(scs), "particles": ntot, "steps": step - 1,
+            "graph_block_steps": block, "graph_launches": nlaunch,
+            "particle_steps_per_s_graph": ntot * nlaunch * block / (t_graph * 1e-3),
+            "particle_steps_per_s_eager": ntot * 32 / (t_eager * 1e-3),
+            "ms_per_step_graph": t_graph / (nlaunch * block), "ms_per_step_eager": t_eager / 32,
+            "per_bucket": per_bucket}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -228,6 +268,9 @@ def main():
     sur = None
     if not a.no_surrogate:
         sur = surrogate_vs_plain(a)
+    ens = None
+    if not a.no_ensemble:
+        ens = ensemble_run(a)
 
     cpu = None
     if not a.no_cpu:
@@ -255,6 +298,7 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
         "surrogate": sur,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

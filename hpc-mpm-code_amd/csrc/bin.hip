@@ -133,6 +133,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_top(uint64_t *__restrict__ bsum,
         pre += v[i];
     }
     if (threadIdx.x == 0) {
+        ctr->stamp += 1u;
         ctr->n_active = (uint32_t)(tot >> 32);
         astart[tot >> 32] = (uint32_t)tot;
     }
@@ -196,9 +197,9 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
 __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict__ keys, int n, int nelem,
                                                       const uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend,
                                                       uint32_t *__restrict__ perm, uint32_t *__restrict__ stamp_arr,
-                                                      uint32_t stamp, uint32_t *__restrict__ fixlist,
-                                                      BinCounters *ctr) {
+                                                      uint32_t *__restrict__ fixlist, BinCounters *ctr) {
     const int lane = threadIdx.x & 63;
+    const uint32_t stamp = ctr->stamp;
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int p = base + threadIdx.x;
         const uint32_t k = (p < n) ? keys[p] : 0xFFFFFFFFu;
@@ -292,7 +293,7 @@ size_t bin_scan_blocks(int nelem) { return (size_t)cdiv(nelem, kTile); }
 size_t bin_cell_capacity(int nelem) { return bin_scan_blocks(nelem) * kTile; }
 
 int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
-               uint32_t stamp, uint32_t *err) {
+               uint32_t *err) {
     const int nelem = g.nelem;
     const int nb = cdiv(nelem, kTile);
     if (nb > kTile) return -1;
@@ -305,7 +306,7 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
                        b.cend, b.active, b.astart, b.tiles, tg);
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, (const uint32_t *)b.cstart,
-                           b.cend, b.perm, b.stamp, stamp, b.fixlist, b.ctr);
+                           b.cend, b.perm, b.stamp, b.fixlist, b.ctr);
         hipLaunchKernelGGL(k_bin_fixup, dim3(256), dim3(256), 0, s, (const uint32_t *)b.fixlist,
                            (const BinCounters *)b.ctr, (const uint32_t *)b.cstart, (const uint32_t *)b.cend, b.perm,
                            err);

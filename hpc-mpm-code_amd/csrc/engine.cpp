@@ -6,6 +6,7 @@
 #include "surrogate_host.h"
 
 #include "../../include/mpm/mpm_capi.h"
+#include "../../include/mpm/mpm_ensemble.h"
 #include "../../include/mpm/mpm_surrogate.h"
 
 #include <algorithm>
@@ -46,7 +47,7 @@ static int bit_length(uint32_t v) {
 
 using namespace mpm;
 
-enum { OPT_FULL_NODE_OUTPUT = 1, OPT_LANES_PER_CELL = 2 };
+enum { OPT_FULL_NODE_OUTPUT = 1, OPT_LANES_PER_CELL = 2, OPT_RELAYOUT_EVERY = 3 };
 
 struct mpm_engine {
     GridGeom g{};
@@ -83,7 +84,6 @@ struct mpm_engine {
     uint32_t *keys64 = nullptr;  // FP64 mode cell keys by particle
     BinBuffers bb{};             // counting-sort binning
     TileGeom tg{};
-    uint32_t stamp = 0;
     uint32_t *perm = nullptr;    // bb.perm (or perm_id right after a relayout)
     uint32_t *perm_id = nullptr;
     uint32_t *d_err = nullptr;
@@ -236,7 +236,7 @@ int do_bin(mpm_engine *e) {
         e->keys_valid = true;  // FP32 keys are uploaded and then kept by the update kernel
     }
     const uint32_t *keys = (e->prec == MPM_PREC_FP64_EXACT) ? e->keys64 : e->P32[e->cur].key;
-    int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, ++e->stamp, e->d_err);
+    int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, e->d_err);
     if (rc) return set_error(MPM_ERR_HIP, "binning launch failed");
     e->perm = e->bb.perm;
     if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && ++e->steps_since_relayout >= e->relayout_every) {
@@ -1037,6 +1037,30 @@ int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *nu
     return 0;
 }
 
+// Enqueues nsteps whole steps on the engine's stream: kernels only (no
+// host synchronisation, no memset), so a stream capture can record it.
+// ev: 7 events per step for per-kernel timing, or null.
+static int enqueue_steps(mpm_engine *e, int32_t istep0, int32_t nsteps, bool diag_last, hipEvent_t *evs) {
+    int rc;
+    const int nev = 7;
+    for (int s = 0; s < nsteps; s++) {
+        const int istep = istep0 + s;
+        const double facl = facl_of(e, istep);
+        const int diag = (diag_last && s == nsteps - 1) ? 1 : 0;
+        hipEvent_t *ev = evs ? &evs[(size_t)nev * s] : nullptr;
+        // events: 0 | bin | 1 | p2g cells | 2 | p2g nodes | 3 | musl cells | 4 | musl nodes | 5 | update | 6
+        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[0], e->stream));
+        if ((rc = do_bin(e))) return rc;
+        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[1], e->stream));
+        if ((rc = do_p2g(e, facl, diag, ev ? ev[2] : nullptr))) return rc;
+        if ((rc = do_solve(e))) return rc;
+        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[3], e->stream));
+        if ((rc = do_g2p(e, facl, diag, ev ? ev[5] : nullptr, ev ? ev[4] : nullptr))) return rc;
+        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[6], e->stream));
+    }
+    return 0;
+}
+
 int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
     int rc = require_particles(e);
     if (rc) return rc;
@@ -1052,21 +1076,8 @@ int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
             e->ev.push_back(ev);
         }
     }
-    for (int s = 0; s < nsteps; s++) {
-        const int istep = istep0 + s;
-        const double facl = facl_of(e, istep);
-        const int diag = (e->full_node_output && s == nsteps - 1) ? 1 : 0;
-        hipEvent_t *ev = e->timing ? &e->ev[(size_t)nev * s] : nullptr;
-        // events: 0 | bin | 1 | p2g cells | 2 | p2g nodes | 3 | musl cells | 4 | musl nodes | 5 | update | 6
-        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[0], e->stream));
-        if ((rc = do_bin(e))) return rc;
-        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[1], e->stream));
-        if ((rc = do_p2g(e, facl, diag, ev ? ev[2] : nullptr))) return rc;
-        if ((rc = do_solve(e))) return rc;
-        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[3], e->stream));
-        if ((rc = do_g2p(e, facl, diag, ev ? ev[5] : nullptr, ev ? ev[4] : nullptr))) return rc;
-        if (ev) MPM_HIP_CHECK(hipEventRecord(ev[6], e->stream));
-    }
+    if ((rc = enqueue_steps(e, istep0, nsteps, e->full_node_output != 0, e->timing ? e->ev.data() : nullptr)))
+        return rc;
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     if (e->timing) {
         mpm_kernel_times kt{};
@@ -1149,6 +1160,12 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
         if (value != 1 && value != 2 && value != 4 && value != 8)
             return set_error(MPM_ERR_INVALID_ARGUMENT, "lanes per cell must be 1, 2, 4 or 8");
         e->lanes_per_cell = (int)value;
+        return 0;
+    case OPT_RELAYOUT_EVERY:
+        if (value < 0 || value > (1 << 20)) return set_error(MPM_ERR_INVALID_ARGUMENT, "relayout interval out of range");
+        if (e->prec != MPM_PREC_FP32) return value ? set_error(MPM_ERR_UNSUPPORTED, "relayout is an FP32 option") : 0;
+        e->relayout_every = (int)value;
+        e->steps_since_relayout = 0;
         return 0;
     default: return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown option");
     }
@@ -1286,3 +1303,264 @@ int mpm_bandwidth_probe(int64_t nbytes, int32_t iters, double *gbps) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ ensembles
+struct mpm_ensemble {
+    std::vector<mpm_engine *> eng;
+    std::vector<int32_t> bucket;
+    hipStream_t s0 = nullptr;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int block_steps = 0;
+
+    ~mpm_ensemble() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto ev : join) (void)hipEventDestroy(ev);
+        for (auto ev : {fork, t0, t1})
+            if (ev) (void)hipEventDestroy(ev);
+        if (s0) (void)hipStreamDestroy(s0);
+    }
+};
+
+namespace {
+
+// fork: every scene's stream waits for s0; join: s0 waits for every scene
+int ens_fork(mpm_ensemble *x, const std::vector<int> &idx) {
+    MPM_HIP_CHECK(hipEventRecord(x->fork, x->s0));
+    for (int i : idx) MPM_HIP_CHECK(hipStreamWaitEvent(x->eng[i]->stream, x->fork, 0));
+    return 0;
+}
+
+int ens_join(mpm_ensemble *x, const std::vector<int> &idx) {
+    for (int i : idx) {
+        MPM_HIP_CHECK(hipEventRecord(x->join[i], x->eng[i]->stream));
+        MPM_HIP_CHECK(hipStreamWaitEvent(x->s0, x->join[i], 0));
+    }
+    return 0;
+}
+
+int ens_prepare(mpm_ensemble *x) {
+    int rc;
+    for (auto *e : x->eng) {
+        if ((rc = require_particles(e)) || (rc = upload_bc(e))) return rc;
+        clear_err(e);
+    }
+    for (auto *e : x->eng) MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int ens_check(mpm_ensemble *x) {
+    MPM_HIP_CHECK(hipStreamSynchronize(x->s0));
+    for (size_t i = 0; i < x->eng.size(); i++) {
+        int rc = check_err(x->eng[i]);
+        if (rc) {
+            std::string m = "scene " + std::to_string(i) + ": " + g_last_error;
+            return set_error(rc, m);
+        }
+    }
+    return 0;
+}
+
+int ens_eager(mpm_ensemble *x, const std::vector<int> &idx, int32_t istep0, int32_t nsteps, double *ms) {
+    int rc;
+    MPM_HIP_CHECK(hipEventRecord(x->t0, x->s0));
+    if ((rc = ens_fork(x, idx))) return rc;
+    for (int i : idx)
+        if ((rc = enqueue_steps(x->eng[i], istep0, nsteps, false, nullptr))) return rc;
+    if ((rc = ens_join(x, idx))) return rc;
+    MPM_HIP_CHECK(hipEventRecord(x->t1, x->s0));
+    MPM_HIP_CHECK(hipEventSynchronize(x->t1));
+    float t = 0.f;
+    MPM_HIP_CHECK(hipEventElapsedTime(&t, x->t0, x->t1));
+    if (ms) *ms = t;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_ensemble_create(mpm_engine *const *engines, int32_t n, const int32_t *bucket, mpm_ensemble **out) {
+    if (!engines || n <= 0 || !out) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad ensemble arguments");
+    *out = nullptr;
+    auto *x = new mpm_ensemble();
+    for (int i = 0; i < n; i++) {
+        if (!engines[i]) {
+            delete x;
+            return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine in ensemble");
+        }
+        x->eng.push_back(engines[i]);
+        x->bucket.push_back(bucket ? bucket[i] : 0);
+    }
+    hipError_t he = hipSetDevice(x->eng[0]->device);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&x->s0, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&x->fork, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreate(&x->t0);
+    if (he == hipSuccess) he = hipEventCreate(&x->t1);
+    x->join.assign(n, nullptr);
+    for (int i = 0; i < n && he == hipSuccess; i++) he = hipEventCreateWithFlags(&x->join[i], hipEventDisableTiming);
+    if (he != hipSuccess) {
+        int r = hip_fail(he, "ensemble create", __FILE__, __LINE__);
+        delete x;
+        return r;
+    }
+    *out = x;
+    return 0;
+}
+
+int mpm_ensemble_destroy(mpm_ensemble *x) {
+    if (!x) return 0;
+    if (x->s0) (void)hipStreamSynchronize(x->s0);
+    delete x;
+    return 0;
+}
+
+int mpm_ensemble_step(mpm_ensemble *x, int32_t istep0, int32_t nsteps, double *elapsed_ms) {
+    if (!x || nsteps < 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
+    int rc = ens_prepare(x);
+    if (rc) return rc;
+    std::vector<int> all(x->eng.size());
+    for (size_t i = 0; i < all.size(); i++) all[i] = (int)i;
+    if ((rc = ens_eager(x, all, istep0, nsteps, elapsed_ms))) return rc;
+    return ens_check(x);
+}
+
+int mpm_ensemble_capture(mpm_ensemble *x, int32_t istep0, int32_t block_steps) {
+    if (!x || block_steps <= 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
+    for (auto *e : x->eng) {
+        if (istep0 <= e->prm.nlstep && e->prm.nlstep > 1)
+            return set_error(MPM_ERR_INVALID_ARGUMENT, "a captured block must start after the load ramp (nlstep)");
+        if (e->sur.logging) return set_error(MPM_ERR_INVALID_ARGUMENT, "sample logging cannot be captured");
+        if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && block_steps % (2 * e->relayout_every) != 0)
+            return set_error(MPM_ERR_INVALID_ARGUMENT,
+                             "block_steps must be a multiple of twice every scene's relayout interval");
+    }
+    int rc = ens_prepare(x);
+    if (rc) return rc;
+    if (x->exec) {
+        (void)hipGraphExecDestroy(x->exec);
+        x->exec = nullptr;
+    }
+    if (x->graph) {
+        (void)hipGraphDestroy(x->graph);
+        x->graph = nullptr;
+    }
+    std::vector<int> all(x->eng.size());
+    for (size_t i = 0; i < all.size(); i++) all[i] = (int)i;
+    // host-side step state that a replay must find unchanged
+    std::vector<int> cur0, since0;
+    for (auto *e : x->eng) {
+        if (e->prec == MPM_PREC_FP64_EXACT) e->keys_valid = false;  // every replay recomputes the FP64 cell keys
+        cur0.push_back(e->cur);
+        since0.push_back(e->steps_since_relayout);
+    }
+    MPM_HIP_CHECK(hipStreamBeginCapture(x->s0, hipStreamCaptureModeThreadLocal));
+    rc = ens_fork(x, all);
+    for (int i : all)
+        if (!rc) rc = enqueue_steps(x->eng[i], istep0, block_steps, false, nullptr);
+    if (!rc) rc = ens_join(x, all);
+    hipGraph_t g = nullptr;
+    hipError_t he = hipStreamEndCapture(x->s0, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (he != hipSuccess) return hip_fail(he, "hipStreamEndCapture", __FILE__, __LINE__);
+    x->graph = g;
+    for (size_t i = 0; i < x->eng.size(); i++)
+        if (x->eng[i]->cur != cur0[i] || x->eng[i]->steps_since_relayout != since0[i])
+            return set_error(MPM_ERR_INVALID_ARGUMENT, "captured block does not return to the same particle buffer");
+    MPM_HIP_CHECK(hipGraphInstantiate(&x->exec, x->graph, nullptr, nullptr, 0));
+    x->block_steps = block_steps;
+    // the capture did not run anything: the state is where it was
+    return 0;
+}
+
+int mpm_ensemble_launch(mpm_ensemble *x, int32_t nlaunch, double *elapsed_ms) {
+    if (!x || nlaunch < 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (!x->exec) return set_error(MPM_ERR_INVALID_ARGUMENT, "no captured graph");
+    int rc = ens_prepare(x);
+    if (rc) return rc;
+    MPM_HIP_CHECK(hipEventRecord(x->t0, x->s0));
+    for (int k = 0; k < nlaunch; k++) MPM_HIP_CHECK(hipGraphLaunch(x->exec, x->s0));
+    MPM_HIP_CHECK(hipEventRecord(x->t1, x->s0));
+    MPM_HIP_CHECK(hipEventSynchronize(x->t1));
+    float t = 0.f;
+    MPM_HIP_CHECK(hipEventElapsedTime(&t, x->t0, x->t1));
+    if (elapsed_ms) *elapsed_ms = t;
+    for (auto *e : x->eng) {
+        e->bins_valid = false;
+        e->keys_valid = true;
+    }
+    return ens_check(x);
+}
+
+int mpm_ensemble_autotune(mpm_ensemble *x, int32_t istep0, int32_t trial_steps, mpm_tune_result *results,
+                          int32_t max_results, int32_t *nresults) {
+    if (!x || trial_steps <= 0 || (max_results > 0 && !results) || !nresults)
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
+    int rc = ens_prepare(x);
+    if (rc) return rc;
+    std::vector<int32_t> ids = x->bucket;
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    const int lpcs[3] = {1, 2, 4}, rels[3] = {0, 8, 16};
+    int nres = 0;
+    for (int32_t b : ids) {
+        std::vector<int> idx;
+        bool f32 = true;
+        for (size_t i = 0; i < x->eng.size(); i++)
+            if (x->bucket[i] == b) {
+                idx.push_back((int)i);
+                f32 = f32 && x->eng[i]->prec == MPM_PREC_FP32;
+            }
+        double best = 1e300, worst = 0.0;
+        int best_l = 1, best_r = 0, step = istep0;
+        for (int li = 0; li < 3; li++)
+            for (int ri = 0; ri < 3; ri++) {
+                for (int i : idx) {
+                    auto *e = x->eng[i];
+                    if (f32) {
+                        e->lanes_per_cell = lpcs[li];
+                        e->relayout_every = rels[ri];
+                        e->steps_since_relayout = 0;
+                    }
+                }
+                double ms = 0.0;
+                if ((rc = ens_eager(x, idx, step, trial_steps, &ms))) return rc;
+                step += trial_steps;
+                const double per = ms / trial_steps;
+                if (per < best) {
+                    best = per;
+                    best_l = lpcs[li];
+                    best_r = rels[ri];
+                }
+                worst = std::max(worst, per);
+            }
+        for (int i : idx) {
+            auto *e = x->eng[i];
+            if (f32) {
+                e->lanes_per_cell = best_l;
+                e->relayout_every = best_r;
+                e->steps_since_relayout = 0;
+            }
+        }
+        if (nres < max_results) {
+            results[nres].bucket = b;
+            results[nres].lanes_per_cell = f32 ? best_l : 0;
+            results[nres].relayout_every = f32 ? best_r : 0;
+            results[nres].scenes = (int32_t)idx.size();
+            results[nres].ms_per_step = best;
+            results[nres].ms_per_step_worst = worst;
+        }
+        nres++;
+    }
+    *nresults = std::min(nres, max_results);
+    return ens_check(x);
+}
+
+}  // extern "C"
+

@@ -35,7 +35,8 @@ void launch_keys_f64(hipStream_t s, const GridGeom &g, const double *x, size_t l
 struct BinCounters {
     uint32_t n_active;
     uint32_t n_fix;
-    uint32_t pad[2];
+    uint32_t stamp;  // binning generation, advanced on the device (graph replays stay distinct)
+    uint32_t pad;
 };
 
 struct CellLists {
@@ -72,7 +73,7 @@ size_t bin_scan_blocks(int nelem);
 size_t bin_cell_capacity(int nelem);
 // Counting-sort binning of keys[0..n) (sentinel nelem = outside the grid).
 int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
-               uint32_t stamp, uint32_t *err);
+               uint32_t *err);
 
 // ------------------------------------------------------ FP64 exact kernels
 struct PartF64 {

@@ -27,6 +27,7 @@ PREC_FP64_EXACT, PREC_FP32 = 0, 1
 MODEL_RIGID, MODEL_LINELAST, MODEL_NEOHOOKEAN, MODEL_STVK, MODEL_MOONEYRIVLIN, MODEL_J2 = -1, 0, 1, 2, 3, 10
 MODEL_TAIT = 20
 OPT_FULL_NODE_OUTPUT = 1
+OPT_RELAYOUT_EVERY = 3
 OPT_LANES_PER_CELL = 2
 
 _dp = C.POINTER(C.c_double)
@@ -77,6 +78,11 @@ class SurrogateStats(C.Structure):
                 ("fallbacks", C.c_int64), ("loss", C.c_float), ("tol", C.c_float)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("bucket", C.c_int32), ("lanes_per_cell", C.c_int32), ("relayout_every", C.c_int32),
+                ("scenes", C.c_int32), ("ms_per_step", C.c_double), ("ms_per_step_worst", C.c_double)]
+
+
 _fp = C.POINTER(C.c_float)
 
 SIGNATURES = {
@@ -114,6 +120,14 @@ SIGNATURES = {
     "mpm_surrogate_get_params": (C.c_int, [C.c_void_p, _fp, _fp]),
     "mpm_surrogate_set_params": (C.c_int, [C.c_void_p, _fp, _fp]),
     "mpm_surrogate_download_samples": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int64, C.POINTER(C.c_int64)]),
+    # include/mpm/mpm_ensemble.h
+    "mpm_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_void_p)]),
+    "mpm_ensemble_destroy": (C.c_int, [C.c_void_p]),
+    "mpm_ensemble_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
+    "mpm_ensemble_capture": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "mpm_ensemble_launch": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "mpm_ensemble_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneResult), C.c_int32,
+                                        C.POINTER(C.c_int32)]),
 }
 
 MLP_NPARAMS = 6288
@@ -363,6 +377,59 @@ class Engine:
         check(self.lib.mpm_surrogate_download_samples(self.h, _ptr(xin, C.c_float), _ptr(tgt, C.c_float), max_rows,
                                                       C.byref(n)))
         return xin[:n.value], tgt[:n.value]
+
+
+class Ensemble:
+    """Independent scenes advanced together (include/mpm/mpm_ensemble.h). The
+    engines stay owned by this object and are destroyed with it."""
+
+    def __init__(self, scene_list, bucket=None, precision=PREC_FP32, relayout_every=16):
+        self.lib = load()
+        self.engines = []
+        for sc in scene_list:
+            e = Engine(sc, precision=precision, relayout_every=relayout_every if precision == PREC_FP32 else 0)
+            e.upload(sc.parts)
+            if precision == PREC_FP32:
+                e.set_option(OPT_FULL_NODE_OUTPUT, 0)
+            self.engines.append(e)
+        arr = (C.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        b = None if bucket is None else (C.c_int32 * len(bucket))(*bucket)
+        h = C.c_void_p()
+        check(self.lib.mpm_ensemble_create(arr, len(self.engines), b, C.byref(h)))
+        self.h = h
+
+    def step(self, istep0, nsteps):
+        ms = C.c_double(0)
+        check(self.lib.mpm_ensemble_step(self.h, istep0, nsteps, C.byref(ms)))
+        return ms.value
+
+    def capture(self, istep0, block_steps):
+        check(self.lib.mpm_ensemble_capture(self.h, istep0, block_steps))
+
+    def launch(self, nlaunch):
+        ms = C.c_double(0)
+        check(self.lib.mpm_ensemble_launch(self.h, nlaunch, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, istep0, trial_steps=5):
+        res = (TuneResult * 16)()
+        nres = C.c_int32(0)
+        check(self.lib.mpm_ensemble_autotune(self.h, istep0, trial_steps, res, 16, C.byref(nres)))
+        return [{k: getattr(res[i], k) for k, _ in TuneResult._fields_} for i in range(nres.value)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mpm_ensemble_destroy(self.h)
+            self.h = None
+        for e in self.engines:
+            e.close()
+        self.engines = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def mlp_forward(params, scaling, inp):

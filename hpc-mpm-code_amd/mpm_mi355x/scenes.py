@@ -285,3 +285,50 @@ def tiny_tait(n_cells=12, dt=2e-5, idimc=(1, 1, 1)):
 
 
 CONFIGS["cfg5"] = cfg5
+
+
+def cfg4_scene(k, seed=0x5EED0004):
+    """Scene k of the ensemble: a 2D elastic block dropped onto a fixed floor
+    (reference slab form, vertical = z, v = 0 everywhere). Seeded per scene:
+    grid G^2 with G in {64, 128, 256}; particle count N ~ U[2k, min(64k, G^2)]
+    at 2x2 points per cell (a w_x x w_z block of cells, w_x*w_z*4 ~ N); Young's
+    modulus log-uniform in [1e5, 1e7], nu ~ U[0.2, 0.45], linear elastic or
+    Neo-Hookean (alternating), drop height (block bottom) ~ U[0.2, 0.6],
+    rho = 1000; dt = 0.3 h / c with c the P-wave speed."""
+    r = uniform(seed ^ (k * 0x9E3779B1), 8, 0.0, 1.0)
+    G = (64, 128, 256)[min(2, int(r[0] * 3))]
+    nmax = min(64000, G * G)
+    N = 2000 + r[1] * (nmax - 2000)
+    cells = max(1, int(N / 4))
+    aspect = 0.5 + r[2]
+    wx = max(1, min(G // 2, int(round(np.sqrt(cells * aspect)))))
+    wz = max(1, min(G // 2, cells // wx))
+    E = 10 ** (5 + 2 * r[3])
+    nu = 0.2 + 0.25 * r[4]
+    model = capi.MODEL_LINELAST if k % 2 == 0 else capi.MODEL_NEOHOOKEAN
+    h = 1.0 / G
+    z0 = int(round((0.2 + 0.4 * r[5]) * G))
+    z0 = min(z0, G - wz - 2)
+    x0 = (G - wx) // 2
+    xmin, xmax, nelem = (0.0, 0.0, 0.0), (1.0, h, 1.0), (G, 1, G)
+    coord = seed_block(xmin, (h, h, h), nelem, (x0, 0, z0), (x0 + wx, 1, z0 + wz), (2, 1, 2))
+    n = len(coord)
+    rho = 1000.0
+    volp = h ** 3 / 4.0
+    vel = uniform(seed + k, 3 * n, -1e-3, 1e-3).reshape(n, 3)
+    vel[:, 1] = 0.0
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    c = np.sqrt((lam + 2 * mu) / rho)
+    bottom = node_lists(nelem, lambda i, j, k_, nx, ny, nz: k_ == 0)
+    allv = node_lists(nelem, lambda i, j, k_, nx, ny, nz: np.ones_like(i, dtype=bool))
+    return Scene("cfg4-scene%03d-%d" % (k, G), xmin, xmax, nelem, [(model, [E, nu])], rho, 0.3 * h / c, 500,
+                 (0.0, 0.0, -9.81), bcs={0: _bc(bottom), 1: _bc(allv), 2: _bc(bottom)}, parts=parts)
+
+
+def cfg4(n_scenes=256, seed=0x5EED0004):
+    """The ensemble: n_scenes seeded scenes (cfg4_scene); returns (scenes, bucket ids by grid size)."""
+    sc = [cfg4_scene(k, seed) for k in range(n_scenes)]
+    bucket = [{64: 0, 128: 1, 256: 2}[s.nelem[0]] for s in sc]
+    return sc, bucket

@@ -204,8 +204,10 @@ int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
  * (FP32 precision only; node arrays are then valid at nodes next to
  * particles). MPM_OPT_LANES_PER_CELL (1, 2, 4, 8; set from the particles per
  * cell at upload): lanes of a wave that fold one cell's particles in the FP32
- * cells kernels. It changes only the FP32 summation order. */
-enum { MPM_OPT_FULL_NODE_OUTPUT = 1, MPM_OPT_LANES_PER_CELL = 2 };
+ * cells kernels. It changes only the FP32 summation order.
+ * MPM_OPT_RELAYOUT_EVERY (FP32, >= 0): physically reorder the particle slots
+ * into cell order every k steps (0 = never); resets the interval counter. */
+enum { MPM_OPT_FULL_NODE_OUTPUT = 1, MPM_OPT_LANES_PER_CELL = 2, MPM_OPT_RELAYOUT_EVERY = 3 };
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 /* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
 int mpm_set_timing(mpm_engine *eng, int32_t enabled);
