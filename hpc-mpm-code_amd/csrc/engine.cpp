@@ -421,7 +421,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     if (!grid || !params || !out || (nmat > 0 && !mats))
         return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    if (nmat < 1 || nmat > kMaxMat) return set_error(MPM_ERR_INVALID_ARGUMENT, "nmat must be in 1..8");
+    if (nmat < 1 || nmat > kMaxMat) return set_error(MPM_ERR_INVALID_ARGUMENT, "nmat must be in 1..1024");
     int rc = validate_grid(grid, params && params->precision == MPM_PREC_FP64_EXACT);
     if (rc) return rc;
     if (!(params->dt > 0.0)) return set_error(MPM_ERR_INVALID_ARGUMENT, "dt must be positive");
@@ -444,11 +444,24 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     e->prec = params->precision;
     e->relayout_every = params->precision == MPM_PREC_FP32 ? std::max(0, (int)params->relayout_every) : 0;
     e->mt.nmat = nmat;
+    std::vector<MaterialDev> mh(nmat);
     for (int i = 0; i < nmat; i++) {
-        e->mt.m[i].model = mats[i].model;
-        for (int k = 0; k < 8; k++) e->mt.m[i].prop[k] = mats[i].prop[k];
+        mh[i].model = mats[i].model;
+        mh[i].pad = 0;
+        for (int k = 0; k < 8; k++) mh[i].prop[k] = mats[i].prop[k];
     }
     hipError_t he = hipGetDevice(&e->device);
+    {
+        MaterialDev *dm = nullptr;
+        int r = e->dalloc(&dm, nmat);
+        if (r) {
+            delete e;
+            return r;
+        }
+        he = hipMemcpy(dm, mh.data(), sizeof(MaterialDev) * nmat, hipMemcpyHostToDevice);
+        e->mt.m = dm;
+    }
+    if (he == hipSuccess) he = hipGetDevice(&e->device);
     if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) {
         int r = hip_fail(he, "hipStreamCreate", __FILE__, __LINE__);

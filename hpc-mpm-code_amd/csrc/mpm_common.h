@@ -33,10 +33,11 @@ struct MaterialDev {
     double prop[8];
 };
 
-constexpr int kMaxMat = 8;
+constexpr int kMaxMat = 1024;
 
+// materials in device memory, indexed by particle matid
 struct MaterialTable {
-    MaterialDev m[kMaxMat];
+    const MaterialDev *m;
     int nmat;
 };
 

@@ -332,3 +332,59 @@ def cfg4(n_scenes=256, seed=0x5EED0004):
     sc = [cfg4_scene(k, seed) for k in range(n_scenes)]
     bucket = [{64: 0, 128: 1, 256: 2}[s.nelem[0]] for s in sc]
     return sc, bucket
+
+
+def atlas(scene_list, name="atlas"):
+    """Packs 2D slab scenes of one grid (nelem (G, 1, G), same spacing, density
+    and gravity) into one grid so that a single engine advances them all:
+    scene k sits in tile (k mod T, k div T) of (G+1) x (G+1) cells, one empty
+    gap cell between tiles so that no node is shared; each scene keeps its
+    material (matid = k), its floor (the tile's bottom node row: u = w = 0)
+    and v = 0 everywhere. All scenes take the smallest of their time steps.
+    Returns (scene, layout) with layout[k] = (first particle, count, origin)."""
+    G = scene_list[0].nelem[0]
+    h = scene_list[0].dx[0]
+    for sc in scene_list:
+        assert sc.nelem == (G, 1, G) and sc.dx[0] == h and sc.rho == scene_list[0].rho
+    ns = len(scene_list)
+    T = int(np.ceil(np.sqrt(ns)))
+    rows = (ns + T - 1) // T
+    stride = G + 1
+    nelem = (T * stride, 1, rows * stride)
+    xmin = (0.0, 0.0, 0.0)
+    xmax = (nelem[0] * h, h, nelem[2] * h)
+    coords, vels, masses, vols, mat, layout = [], [], [], [], [], []
+    first = 0
+    for k, sc in enumerate(scene_list):
+        ox, oz = (k % T) * stride * h, (k // T) * stride * h
+        c = sc.parts.coord.copy()
+        c[:, 0] += ox
+        c[:, 2] += oz
+        coords.append(c)
+        vels.append(sc.parts.vel)
+        masses.append(sc.parts.mass)
+        vols.append(sc.parts.vol0)
+        mat.append(np.full(sc.n, k, np.int32))
+        layout.append((first, sc.n, (ox, 0.0, oz)))
+        first += sc.n
+    n = first
+    parts = capi.Particles(n, coord=np.concatenate(coords), vel=np.concatenate(vels), mass=np.concatenate(masses),
+                           vol0=np.concatenate(vols), matid=np.concatenate(mat))
+    floor = node_lists(nelem, lambda i, j, k_, nx, ny, nz: (k_ % stride) == 0)
+    allv = node_lists(nelem, lambda i, j, k_, nx, ny, nz: np.ones_like(i, dtype=bool))
+    mats = [sc.materials[0] for sc in scene_list]
+    dt = min(sc.dt for sc in scene_list)
+    return (Scene(name, xmin, xmax, nelem, mats, scene_list[0].rho, dt, scene_list[0].steps, scene_list[0].bb,
+                  bcs={0: _bc(floor), 1: _bc(allv), 2: _bc(floor)}, parts=parts), layout)
+
+
+def cfg4_atlases(n_scenes=256, seed=0x5EED0004):
+    """The ensemble packed per grid-size bucket: [(atlas scene, layout, member scene indices)]."""
+    sc, bucket = cfg4(n_scenes, seed)
+    out = []
+    for b in range(3):
+        idx = [i for i, x in enumerate(bucket) if x == b]
+        if idx:
+            a, lay = atlas([sc[i] for i in idx], "cfg4-atlas-%d" % (64 << b))
+            out.append((a, lay, idx))
+    return out

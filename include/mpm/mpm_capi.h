@@ -153,7 +153,8 @@ const char *mpm_last_error(void);
 
 /* Creates an engine on the current HIP device. Replaces the global set-up
  * of datain_para + BuildMesh + BuildControlPoint
- * (work/src_solid/explicit/datain_para.cpp:18-162, module/mesh.cpp:56-158). */
+ * (work/src_solid/explicit/datain_para.cpp:18-162, module/mesh.cpp:56-158).
+ * nmat: 1..1024 materials, selected per particle by matid. */
 int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, int32_t nmat,
                       const mpm_step_params *params, mpm_engine **out);
 int mpm_engine_destroy(mpm_engine *eng);

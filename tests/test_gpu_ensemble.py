@@ -73,3 +73,29 @@ def test_failing_scene_is_named():
         ens.step(1, 200)
     assert ei.value.code == capi.MPM_ERR_OUT_OF_GRID and "scene 1" in str(ei.value)
     ens.close()
+
+
+def test_atlas_matches_individual_scenes():
+    # scenes of one grid size packed into one engine give, scene by scene,
+    # the bits of separate engines run with the same time step
+    members = [scenes.cfg4_scene(k) for k in (4, 7, 8)]   # 128 x 128 grids
+    a, layout = scenes.atlas(members)
+    print("atlas %s n=%d dt=%.3e" % (a.nelem, a.n, a.dt), flush=True)
+    e = capi.Engine(a, precision=capi.PREC_FP32, relayout_every=16)
+    e.upload(a.parts)
+    e.set_option(capi.OPT_LANES_PER_CELL, 2)
+    assert e.step(1, 40) == 0
+    got = e.download(a.n)
+    e.close()
+    for sc, (first, cnt, origin) in zip(members, layout):
+        sc.dt = a.dt
+        s1 = capi.Engine(sc, precision=capi.PREC_FP32, relayout_every=16)
+        s1.upload(sc.parts)
+        s1.set_option(capi.OPT_LANES_PER_CELL, 2)
+        assert s1.step(1, 40) == 0
+        ref = s1.download(sc.n)
+        s1.close()
+        sl = slice(first, first + cnt)
+        assert np.array_equal(got.coord[sl] - np.asarray(origin), ref.coord), sc.name
+        for f in ("vel", "stress", "def_grad", "det_def_grad", "vol"):
+            assert np.array_equal(getattr(got, f)[sl], getattr(ref, f)), (sc.name, f)
