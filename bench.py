@@ -49,6 +49,16 @@ BYTES = {
 }
 
 
+# HBM traffic per launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on the 4,194,304-particle cube (bytes; copy-kernel calibrated,
+# profiles/hbm_traffic_cfg2_4M_fp32.md). Reported as roofline.traffic for
+# that workload only.
+MEASURED_TRAFFIC = {
+    "cfg2-3d-neohookean-16ppc": {"update": 640058167, "p2g_cells": 332576121, "musl_cells": 205078433,
+                                 "p2g_nodes": 98112156, "musl_nodes": 51089087},
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +73,8 @@ def parse():
     ap.add_argument("--no-probe", action="store_true", help="skip the copy-bandwidth probe")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the J2 surrogate vs plain comparison")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-scene ensemble measurement")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="FP32 vs FP64-exact comparison after this many steps (0 = skip)")
     ap.add_argument("--ens-scenes", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=500)
     ap.add_argument("--sur-collect", type=int, default=200, help="logged plain steps before the fit")
@@ -174,6 +186,35 @@ def surrogate_vs_plain(a):
     }
 
 
+def accuracy_run(a, scene_name):
+    """Per-field error of the FP32 engine against the FP64-exact engine (the
+    oracle's arithmetic in the reference's summation order) on the same
+    workload after the same number of steps."""
+    out = {}
+    t = {}
+    for name, prec, rel in (("fp64", capi.PREC_FP64_EXACT, 0), ("fp32", capi.PREC_FP32, a.relayout)):
+        sc = make_scene(scene_name)
+        e = capi.Engine(sc, precision=prec, relayout_every=rel)
+        e.upload(sc.parts)
+        t0 = time.perf_counter()
+        capi.check(e.step(1, a.accuracy_steps))
+        t[name] = time.perf_counter() - t0
+        out[name] = (e.download(sc.n), e.nodes())
+        e.close()
+    (p64, n64), (p32, n32) = out["fp64"], out["fp32"]
+    L = 1.0
+    return {
+        "workload": sc.name, "steps": a.accuracy_steps, "reference": "FP64-exact engine",
+        "position_max_abs_over_L": float(np.max(np.abs(p32.coord - p64.coord)) / L),
+        "stress_rel_fro": float(np.linalg.norm(p32.stress - p64.stress) / np.linalg.norm(p64.stress)),
+        "stress_max_abs_Pa": float(np.max(np.abs(p32.stress - p64.stress))),
+        "velocity_rel_l2": float(np.linalg.norm(p32.vel - p64.vel) / np.linalg.norm(p64.vel)),
+        "J_max_abs": float(np.max(np.abs(p32.det_def_grad - p64.det_def_grad))),
+        "node_momentum_rel_l2": float(np.linalg.norm(n32.nmome - n64.nmome) / np.linalg.norm(n64.nmome)),
+        "fp64_engine_s": round(t["fp64"], 2), "fp32_engine_s": round(t["fp32"], 2),
+    }
+
+
 def ensemble_run(a):
     """cfg4: 256 independent 2D drop scenes in three grid-size buckets. Each
     bucket's scenes are packed into one atlas grid (one engine per bucket,
@@ -265,8 +306,11 @@ def main():
                       "frac": round(gbs / HBM_PEAK_GBS, 4)}
     dom = max((k for k in kernels if k != "bin"), key=lambda k: kernels[k]["avg_us"])
     d = kernels[dom]
+    traffic = MEASURED_TRAFFIC.get(scene.name, {}).get(dom)
     roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": d["frac"], "traffic": None, "kernel": dom}
+                "frac": d["frac"], "traffic": traffic, "kernel": dom,
+                "alg_bytes": d["alg_bytes"],
+                "traffic_source": "profiles/hbm_traffic_cfg2_4M_fp32.md" if traffic else None}
     # P2G and G2P as whole passes
     def phase(keys):
         b = sum(kernels[k]["alg_bytes"] for k in keys)
@@ -285,6 +329,9 @@ def main():
     ens = None
     if not a.no_ensemble:
         ens = ensemble_run(a)
+    acc = None
+    if a.accuracy_steps > 0:
+        acc = accuracy_run(a, a.config)
 
     cpu = None
     if not a.no_cpu:
@@ -313,6 +360,7 @@ def main():
         "cpu_baseline": cpu,
         "surrogate": sur,
         "ensemble": ens,
+        "accuracy": acc,
     }
     print(json.dumps(out), flush=True)
 
