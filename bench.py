@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--config", default="cfg2-4M", choices=["cfg1", "cfg2", "cfg2-4M", "cfg3", "cfg5"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--relayout", type=int, default=32)
+    ap.add_argument("--graph-block", type=int, default=64,
+                    help="steps per recorded HIP graph block (0 = eager launches)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes-per-cell", type=int, default=0, help="override (1, 2, 4, 8); 0 = automatic")
@@ -279,13 +281,19 @@ def main():
     eng.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
     if a.lanes_per_cell:
         eng.set_option(capi.OPT_LANES_PER_CELL, a.lanes_per_cell)
+    if a.graph_block:
+        eng.set_option(capi.OPT_GRAPH_BLOCK, a.graph_block)
     capi.check(eng.step(1, a.warmup))
-    eng.set_timing(True)
     capi.check(eng.lib.mpm_sync(eng.h))
     t0 = time.perf_counter()
     capi.check(eng.step(1 + a.warmup, a.steps))
     capi.check(eng.lib.mpm_sync(eng.h))
     wall = time.perf_counter() - t0
+    # per-kernel breakdown: a separate eager run with HIP events between the kernels
+    kt_steps = max(10, min(a.steps, 100))
+    eng.set_timing(True)
+    capi.check(eng.step(1 + a.warmup + a.steps, kt_steps))
+    eng.set_timing(False)
     kt = eng.kernel_times()
     n = scene.n
     value = n * a.steps / wall
@@ -300,7 +308,7 @@ def main():
     for k, ms in per.items():
         bp, bn, bc = BYTES[k]
         byts = bp * n + bn * nact + bc * scene.nelem_total
-        avg_ms = ms / a.steps
+        avg_ms = ms / kt_steps
         gbs = byts / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
         kernels[k] = {"avg_us": round(avg_ms * 1e3, 2), "alg_bytes": byts, "GBps": round(gbs, 1),
                       "frac": round(gbs / HBM_PEAK_GBS, 4)}
@@ -353,7 +361,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": scene.name, "particles": n, "grid": "x".join(map(str, scene.nelem)),
                    "active_cells": kt["active_cells"], "active_nodes": nact, "method": "plain (no surrogate)",
-                   "scheme": "PIC/MUSL", "relayout_every": a.relayout},
+                   "scheme": "PIC/MUSL", "relayout_every": a.relayout, "graph_block": a.graph_block,
+                   "kernel_timing_steps": kt_steps},
         "roofline": roofline,
         "phases": phases,
         "kernels": kernels,

@@ -33,24 +33,6 @@ constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
 constexpr int kWaves = kBlock / 64;
 
-__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (lane == 0) ? 0ull : (~0ull >> (64 - lane)); }
-
-// Run structure of equal keys among the lanes of a wave: returns true for
-// the lane that heads its run; *len = run length, *head = head lane.
-__device__ __forceinline__ bool wave_runs(uint32_t k, int lane, int *len, int *head) {
-    uint32_t prev = (uint32_t)__shfl_up((int)k, 1, 64);
-    bool is_head = (lane == 0) || (k != prev);
-    uint64_t H = __ballot(is_head);
-    // head of this lane's run: highest set bit of H at or below lane
-    uint64_t below = H & (lanemask_lt(lane) | (1ull << lane));
-    *head = 63 - __clzll((long long)below);
-    // next head strictly above this lane's head
-    uint64_t above = (*head == 63) ? 0ull : (H & (~0ull << (*head + 1)));
-    int nxt = above ? (__ffsll((unsigned long long)above) - 1) : 64;
-    *len = nxt - *head;
-    return is_head;
-}
-
 __global__ void __launch_bounds__(256) k_bin_count(const uint32_t *__restrict__ keys, int n, int nelem,
                                                     uint32_t *__restrict__ cnt, uint32_t *__restrict__ tiles,
                                                     int ntiles, BinCounters *ctr) {
@@ -292,14 +274,23 @@ TileGeom make_tile_geom(const GridGeom &g) {
 size_t bin_scan_blocks(int nelem) { return (size_t)cdiv(nelem, kTile); }
 size_t bin_cell_capacity(int nelem) { return bin_scan_blocks(nelem) * kTile; }
 
+void launch_bin_count(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n,
+                      BinBuffers &b) {
+    int pblocks = cdiv(n > 0 ? n : 1, 256);
+    if (pblocks > 8192) pblocks = 8192;
+    hipLaunchKernelGGL(k_bin_count, dim3(pblocks), dim3(256), 0, s, keys, n, g.nelem, b.cnt, b.tiles, tg.ntiles, b.ctr);
+}
+
 int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
-               uint32_t *err) {
+               uint32_t *err, bool counted) {
     const int nelem = g.nelem;
     const int nb = cdiv(nelem, kTile);
     if (nb > kTile) return -1;
     int pblocks = cdiv(n > 0 ? n : 1, 256);
     if (pblocks > 8192) pblocks = 8192;
-    hipLaunchKernelGGL(k_bin_count, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, b.cnt, b.tiles, tg.ntiles, b.ctr);
+    if (!counted)
+        hipLaunchKernelGGL(k_bin_count, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, b.cnt, b.tiles, tg.ntiles,
+                           b.ctr);
     hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
     hipLaunchKernelGGL(k_bin_top, dim3(1), dim3(kBlock), 0, s, b.bsum, nb, b.ctr, b.astart);
     hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, b.cstart,

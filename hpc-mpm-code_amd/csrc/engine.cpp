@@ -47,7 +47,7 @@ static int bit_length(uint32_t v) {
 
 using namespace mpm;
 
-enum { OPT_FULL_NODE_OUTPUT = 1, OPT_LANES_PER_CELL = 2, OPT_RELAYOUT_EVERY = 3 };
+enum { OPT_FULL_NODE_OUTPUT = 1, OPT_LANES_PER_CELL = 2, OPT_RELAYOUT_EVERY = 3, OPT_GRAPH_BLOCK = 4 };
 
 struct mpm_engine {
     GridGeom g{};
@@ -89,11 +89,24 @@ struct mpm_engine {
     uint32_t *d_err = nullptr;
     bool bins_valid = false;
     bool keys_valid = false;
+    bool counts_ready = false;  // per-cell counts of the current keys are in bb.cnt (fused into the update)
     int last_pass = 0;  // 1 = p2g, 2 = g2p
     int steps_since_relayout = 0;
     int full_node_output = 1;
     int relayout_every = 0;
     int active_cells = 0;
+
+    // whole-step graph: blocks of graph_block steps captured once and replayed
+    int graph_block = 0;
+    int graph_cur = -1, graph_since = -1;  // particle buffer and relayout counter the graph starts from
+    hipGraph_t step_graph = nullptr;
+    hipGraphExec_t step_exec = nullptr;
+    void drop_graph() {
+        if (step_exec) (void)hipGraphExecDestroy(step_exec);
+        if (step_graph) (void)hipGraphDestroy(step_graph);
+        step_exec = nullptr;
+        step_graph = nullptr;
+    }
 
     bool timing = false;
     mpm_kernel_times kt{};
@@ -150,9 +163,11 @@ struct mpm_engine {
         perm_id = nullptr;
         n = 0;
         bins_valid = keys_valid = false;
+        counts_ready = false;
     }
 
     ~mpm_engine() {
+        drop_graph();
         free_particles();
         for (void *p : allocs) (void)hipFree(p);
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -236,7 +251,8 @@ int do_bin(mpm_engine *e) {
         e->keys_valid = true;  // FP32 keys are uploaded and then kept by the update kernel
     }
     const uint32_t *keys = (e->prec == MPM_PREC_FP64_EXACT) ? e->keys64 : e->P32[e->cur].key;
-    int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, e->d_err);
+    int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, e->d_err, e->counts_ready);
+    e->counts_ready = false;
     if (rc) return set_error(MPM_ERR_HIP, "binning launch failed");
     e->perm = e->bb.perm;
     if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && ++e->steps_since_relayout >= e->relayout_every) {
@@ -305,7 +321,12 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         }
         sa.ctr = su.ctr;
         sa.pid = e->pid[e->cur];
-        launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa);
+        // count the new keys for the next binning; counts left from an update
+        // that no binning consumed are cleared first
+        if (e->counts_ready) launch_fill_u32(e->stream, e->bb.cnt, bin_cell_capacity(e->g.nelem), 0u);
+        BinFuse bf{e->bb.cnt, e->bb.tiles, e->tg.ntiles, e->bb.ctr};
+        launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa, bf);
+        e->counts_ready = true;
         e->keys_valid = true;  // the update kernel writes the new keys
     }
     e->bins_valid = false;
@@ -543,6 +564,7 @@ static int sur_require(mpm_engine *e) {
 int mpm_surrogate_collect(mpm_engine *e, int32_t on, int64_t capacity, double rate) {
     int rc = sur_require(e);
     if (rc) return rc;
+    e->drop_graph();
     auto &su = e->sur;
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     if (!on) {
@@ -618,6 +640,7 @@ static int sur_sort(mpm_engine *e) {
 int mpm_surrogate_fit(mpm_engine *e, const mpm_train_params *tp, float *final_loss) {
     int rc = sur_require(e);
     if (rc) return rc;
+    e->drop_graph();
     if (!tp) return set_error(MPM_ERR_INVALID_ARGUMENT, "null training parameters");
     auto &su = e->sur;
     if (!su.log_in) return set_error(MPM_ERR_INVALID_ARGUMENT, "no sample log (call mpm_surrogate_collect)");
@@ -639,6 +662,7 @@ int mpm_surrogate_fit(mpm_engine *e, const mpm_train_params *tp, float *final_lo
 int mpm_surrogate_enable(mpm_engine *e, int32_t on, double tol) {
     int rc = sur_require(e);
     if (rc) return rc;
+    e->drop_graph();
     if (on && !e->sur.have_net) return set_error(MPM_ERR_INVALID_ARGUMENT, "no trained or loaded network");
     if (on && !(tol >= 0.0)) return set_error(MPM_ERR_INVALID_ARGUMENT, "tolerance must be >= 0");
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
@@ -688,6 +712,7 @@ int mpm_surrogate_get_params(mpm_engine *e, float *params, float *scaling) {
 int mpm_surrogate_set_params(mpm_engine *e, const float *params, const float *scaling) {
     int rc = sur_require(e);
     if (rc) return rc;
+    e->drop_graph();
     if (!params || !scaling) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     if ((rc = mlp_set(e->sur.net, params, scaling, e->stream))) return rc;
@@ -741,6 +766,7 @@ int mpm_set_bc(mpm_engine *e, int32_t comp, const int32_t *nbc, const double *fb
 }
 
 int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
+    if (e) e->drop_graph();
     if (!e || !p || p->n <= 0 || !p->coord || !p->mass || !p->vol0)
         return set_error(MPM_ERR_INVALID_ARGUMENT, "particle view needs n > 0, coord, mass, vol0");
     if (p->n >= 2147483647LL) return set_error(MPM_ERR_INVALID_ARGUMENT, "too many particles");
@@ -785,7 +811,7 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
         }
         double ppc = nact > 0 ? (double)n / (double)nact : 1.0;
         int lpc = 1;
-        while (lpc < 8 && 64.0 * ppc / lpc > 256.0) lpc *= 2;
+        while (lpc < 8 && 64.0 * ppc / lpc > 512.0) lpc *= 2;
         e->lanes_per_cell = lpc;
     }
     if (e->prec == MPM_PREC_FP64_EXACT) {
@@ -878,6 +904,10 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
     }
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     e->keys_valid = e->prec == MPM_PREC_FP32;
+    if (e->counts_ready) {
+        launch_fill_u32(e->stream, e->bb.cnt, bin_cell_capacity(e->g.nelem), 0u);
+        e->counts_ready = false;
+    }
     e->bins_valid = false;
     e->steps_since_relayout = 0;
     return 0;
@@ -1050,6 +1080,16 @@ int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *nu
     return 0;
 }
 
+// Before recording a graph: make the per-cell counts of the current keys
+// ready (as after an update kernel), so that every recorded block starts and
+// ends in the same state.
+static void prime_counts(mpm_engine *e) {
+    if (e->prec == MPM_PREC_FP32 && !e->counts_ready) {
+        launch_bin_count(e->stream, e->g, e->tg, e->P32[e->cur].key, e->n, e->bb);
+        e->counts_ready = true;
+    }
+}
+
 // Enqueues nsteps whole steps on the engine's stream: kernels only (no
 // host synchronisation, no memset), so a stream capture can record it.
 // ev: 7 events per step for per-kernel timing, or null.
@@ -1089,8 +1129,45 @@ int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
             e->ev.push_back(ev);
         }
     }
-    if ((rc = enqueue_steps(e, istep0, nsteps, e->full_node_output != 0, e->timing ? e->ev.data() : nullptr)))
+    const int B = e->graph_block;
+    const bool ramp_over = e->prm.nlstep <= 1 || istep0 > e->prm.nlstep;
+    const bool relayout_ok = !(e->prec == MPM_PREC_FP32 && e->relayout_every > 0) || B % (2 * e->relayout_every) == 0;
+    if (B > 0 && !e->timing && ramp_over && relayout_ok && !e->sur.logging && nsteps > B) {
+        // all but the last step in whole graph blocks; the rest eagerly (the
+        // last step may write the diagnostic node fields)
+        const int nblk = (nsteps - 1) / B;
+        if (e->step_exec && (e->graph_cur != e->cur || e->graph_since != e->steps_since_relayout)) e->drop_graph();
+        if (e->step_exec) prime_counts(e);
+        if (!e->step_exec) {
+            if (e->prec == MPM_PREC_FP64_EXACT) e->keys_valid = false;
+            prime_counts(e);
+            const int cur0 = e->cur, since0 = e->steps_since_relayout;
+            MPM_HIP_CHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_steps(e, istep0, B, false, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t he = hipStreamEndCapture(e->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (he != hipSuccess) return hip_fail(he, "hipStreamEndCapture", __FILE__, __LINE__);
+            e->step_graph = g;
+            if (e->cur != cur0 || e->steps_since_relayout != since0 ||
+                (e->prec == MPM_PREC_FP32 && !e->counts_ready)) {
+                e->drop_graph();
+                return set_error(MPM_ERR_INVALID_ARGUMENT, "graph block does not return to the same particle buffer");
+            }
+            MPM_HIP_CHECK(hipGraphInstantiate(&e->step_exec, e->step_graph, nullptr, nullptr, 0));
+            e->graph_cur = cur0;
+            e->graph_since = since0;
+        }
+        for (int b = 0; b < nblk; b++) MPM_HIP_CHECK(hipGraphLaunch(e->step_exec, e->stream));
+        if (e->prec == MPM_PREC_FP64_EXACT) e->keys_valid = false;
+        const int done = nblk * B;
+        if ((rc = enqueue_steps(e, istep0 + done, nsteps - done, e->full_node_output != 0, nullptr))) return rc;
+    } else if ((rc = enqueue_steps(e, istep0, nsteps, e->full_node_output != 0, e->timing ? e->ev.data() : nullptr))) {
         return rc;
+    }
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     if (e->timing) {
         mpm_kernel_times kt{};
@@ -1167,6 +1244,7 @@ int mpm_set_timing(mpm_engine *e, int32_t enabled) {
 
 int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    e->drop_graph();
     switch (key) {
     case OPT_FULL_NODE_OUTPUT: e->full_node_output = value ? 1 : 0; return 0;
     case OPT_LANES_PER_CELL:
@@ -1179,6 +1257,10 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
         if (e->prec != MPM_PREC_FP32) return value ? set_error(MPM_ERR_UNSUPPORTED, "relayout is an FP32 option") : 0;
         e->relayout_every = (int)value;
         e->steps_since_relayout = 0;
+        return 0;
+    case OPT_GRAPH_BLOCK:
+        if (value < 0 || value > 4096) return set_error(MPM_ERR_INVALID_ARGUMENT, "graph block out of range");
+        e->graph_block = (int)value;
         return 0;
     default: return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown option");
     }
@@ -1360,6 +1442,7 @@ int ens_prepare(mpm_ensemble *x) {
     for (auto *e : x->eng) {
         if ((rc = require_particles(e)) || (rc = upload_bc(e))) return rc;
         clear_err(e);
+        prime_counts(e);  // recorded blocks start from counted keys
     }
     for (auto *e : x->eng) MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return 0;
@@ -1466,6 +1549,7 @@ int mpm_ensemble_capture(mpm_ensemble *x, int32_t istep0, int32_t block_steps) {
     // host-side step state that a replay must find unchanged
     std::vector<int> cur0, since0;
     for (auto *e : x->eng) {
+        prime_counts(e);
         if (e->prec == MPM_PREC_FP64_EXACT) e->keys_valid = false;  // every replay recomputes the FP64 cell keys
         cur0.push_back(e->cur);
         since0.push_back(e->steps_since_relayout);

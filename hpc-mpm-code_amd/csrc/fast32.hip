@@ -826,21 +826,23 @@ __device__ __forceinline__ float j2_residual(const J2Trial &t, const float s[6],
 template <int SUR>
 __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
                                                        MaterialTable mt, StepConsts sc, uint32_t *err,
-                                                       SurrogateArgs sa) {
+                                                       SurrogateArgs sa, BinFuse bf) {
     __shared__ SurSmem<SUR == 2> sm;
     const int slot0 = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = slot0 < P.n;
-    // the network runs on whole waves: with SUR == 2 no lane leaves early,
-    // invalid lanes compute on slot 0 / cell 0 and store nothing
-    if constexpr (SUR != 2) {
-        if (!valid) return;
+    if (bf.cnt) {
+        for (int i = slot0; i < bf.ntiles; i += gridDim.x * blockDim.x) bf.tiles[i] = 0u;
+        if (slot0 == 0) {
+            bf.ctr->n_active = 0u;
+            bf.ctr->n_fix = 0u;
+        }
     }
+    // Wave-wide steps (the network, the run-aggregated cell counts) need every
+    // lane: no lane leaves early; invalid lanes compute on slot 0 / cell 0
+    // and store nothing.
+    bool valid = slot0 < P.n;
     const int slot = valid ? slot0 : 0;
     uint32_t key = P.key[slot];
     if (key >= (uint32_t)g.nelem) valid = false;
-    if constexpr (SUR != 2) {
-        if (!valid) return;
-    }
     if (!valid) key = 0;
     const size_t ld = P.ld;
     const int nodec = g.nodec;
@@ -1029,6 +1031,13 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
             if (j2 && !accept) j2_return(t, s, &ep);
         }
     }
+    const uint32_t nk = inside ? (uint32_t)(ci[0] + ci[1] * g.nex[0] + ci[2] * nexy) : (uint32_t)g.nelem;
+    if (bf.cnt) {
+        const uint32_t ck = valid ? nk : 0xFFFFFFFFu;
+        int len, head;
+        const bool h = wave_runs(ck, threadIdx.x & 63, &len, &head);
+        if (h && ck < (uint32_t)g.nelem) atomicAdd(&bf.cnt[ck], (uint32_t)len);
+    }
     if (!valid) return;
     if (md.model == 10 && P.eps_p) P.eps_p[slot] = ep;
 #pragma unroll
@@ -1039,7 +1048,6 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     for (int a = 0; a < 3; a++) P.r[a * ld + slot] = r[a];
     P.J[slot] = J;
     P.V[slot] = V;
-    const uint32_t nk = inside ? (uint32_t)(ci[0] + ci[1] * g.nex[0] + ci[2] * nexy) : (uint32_t)g.nelem;
     P.key[slot] = nk;
     uint32_t e = 0;
     if (!inside) e |= kErrOutOfGrid;
@@ -1144,15 +1152,15 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
 
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                            const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
-                           const SurrogateArgs &sa) {
+                           const SurrogateArgs &sa, const BinFuse &bf) {
     if (P.n <= 0) return;
     const dim3 grid(cdiv(P.n, 256)), block(256);
     if (sa.mode == 1)
-        hipLaunchKernelGGL(k32_g2p_update<1>, grid, block, 0, s, g, P, N, T, mt, c, err, sa);
+        hipLaunchKernelGGL(k32_g2p_update<1>, grid, block, 0, s, g, P, N, T, mt, c, err, sa, bf);
     else if (sa.mode == 2)
-        hipLaunchKernelGGL(k32_g2p_update<2>, grid, block, 0, s, g, P, N, T, mt, c, err, sa);
+        hipLaunchKernelGGL(k32_g2p_update<2>, grid, block, 0, s, g, P, N, T, mt, c, err, sa, bf);
     else
-        hipLaunchKernelGGL(k32_g2p_update<0>, grid, block, 0, s, g, P, N, T, mt, c, err, sa);
+        hipLaunchKernelGGL(k32_g2p_update<0>, grid, block, 0, s, g, P, N, T, mt, c, err, sa, bf);
 }
 
 __global__ void k_iota(uint32_t *p, int n) {

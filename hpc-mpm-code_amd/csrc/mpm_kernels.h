@@ -53,6 +53,33 @@ struct TileGeom {
     int ntx, nty, ntz, ntiles;
 };
 
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (lane == 0) ? 0ull : (~0ull >> (64 - lane)); }
+
+// Run structure of equal keys among the lanes of a wave: returns true for
+// the lane that heads its run; *len = run length, *head = head lane.
+__device__ __forceinline__ bool wave_runs(uint32_t k, int lane, int *len, int *head) {
+    uint32_t prev = (uint32_t)__shfl_up((int)k, 1, 64);
+    bool is_head = (lane == 0) || (k != prev);
+    uint64_t H = __ballot(is_head);
+    // head of this lane's run: highest set bit of H at or below lane
+    uint64_t below = H & (lanemask_lt(lane) | (1ull << lane));
+    *head = 63 - __clzll((long long)below);
+    // next head strictly above this lane's head
+    uint64_t above = (*head == 63) ? 0ull : (H & (~0ull << (*head + 1)));
+    int nxt = above ? (__ffsll((unsigned long long)above) - 1) : 64;
+    *len = nxt - *head;
+    return is_head;
+}
+
+// Cell counting fused into the FP32 update kernel (the next binning's first
+// pass): counts of the new keys, and the clearing of tile flags and counters.
+struct BinFuse {
+    uint32_t *cnt;  // null: not fused
+    uint32_t *tiles;
+    int ntiles;
+    BinCounters *ctr;
+};
+
 struct BinBuffers {
     uint32_t *cnt;      // [bin_cell_capacity], zero between calls
     uint32_t *cstart;   // [bin_cell_capacity]
@@ -72,8 +99,13 @@ size_t bin_scan_blocks(int nelem);
 // per-cell arrays (cnt, cstart, cend) must hold this many entries (padding zeroed)
 size_t bin_cell_capacity(int nelem);
 // Counting-sort binning of keys[0..n) (sentinel nelem = outside the grid).
+// count pass alone (per-cell counts of keys, cleared tiles and counters)
+void launch_bin_count(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n,
+                      BinBuffers &b);
+// counted: the per-cell counts, cleared tiles and counters were produced by
+// the previous update kernel (fused counting), so the count pass is skipped
 int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
-               uint32_t *err);
+               uint32_t *err, bool counted = false);
 
 // ------------------------------------------------------ FP64 exact kernels
 struct PartF64 {
@@ -193,7 +225,7 @@ struct SurrogateArgs {
 
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                            const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
-                           const SurrogateArgs &sa);
+                           const SurrogateArgs &sa, const BinFuse &bf);
 void launch_f32_permute(hipStream_t s, const PartF32 &src, const PartF32 &dst, const uint32_t *perm,
                         const int *pid_src, int *pid_dst);
 void launch_iota_u32(hipStream_t s, uint32_t *p, int n);

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libmpm_mi355x.so")
+LIB_PATH = os.environ.get("MPM_MI355X_LIB") or os.path.join(PKG_DIR, "libmpm_mi355x.so")
 HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "mpm", "mpm_capi.h")
 
 MPM_OK = 0
@@ -28,6 +28,7 @@ MODEL_RIGID, MODEL_LINELAST, MODEL_NEOHOOKEAN, MODEL_STVK, MODEL_MOONEYRIVLIN, M
 MODEL_TAIT = 20
 OPT_FULL_NODE_OUTPUT = 1
 OPT_RELAYOUT_EVERY = 3
+OPT_GRAPH_BLOCK = 4
 OPT_LANES_PER_CELL = 2
 
 _dp = C.POINTER(C.c_double)

@@ -207,8 +207,13 @@ int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
  * cell at upload): lanes of a wave that fold one cell's particles in the FP32
  * cells kernels. It changes only the FP32 summation order.
  * MPM_OPT_RELAYOUT_EVERY (FP32, >= 0): physically reorder the particle slots
- * into cell order every k steps (0 = never); resets the interval counter. */
-enum { MPM_OPT_FULL_NODE_OUTPUT = 1, MPM_OPT_LANES_PER_CELL = 2, MPM_OPT_RELAYOUT_EVERY = 3 };
+ * into cell order every k steps (0 = never); resets the interval counter.
+ * MPM_OPT_GRAPH_BLOCK (default 0 = off): mpm_step records a block of this
+ * many steps into a HIP graph once and relaunches it (no launch gaps); used
+ * when timing is off, the load ramp is over, no samples are logged and the
+ * block is a multiple of twice the relayout interval. Uploads, options and
+ * surrogate changes drop the recorded graph. */
+enum { MPM_OPT_FULL_NODE_OUTPUT = 1, MPM_OPT_LANES_PER_CELL = 2, MPM_OPT_RELAYOUT_EVERY = 3, MPM_OPT_GRAPH_BLOCK = 4 };
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 /* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
 int mpm_set_timing(mpm_engine *eng, int32_t enabled);

@@ -365,3 +365,23 @@ def test_fp32_tait_run():
     assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-7
     assert rel(got.stress, o.parts.stress) < 1e-3
     assert np.all(got.stress[:, 1] <= 1.0)  # compression (or free surface) only
+
+
+# ------------------------------------------------------------------ whole-step graph replay
+@pytest.mark.parametrize("prec", [capi.PREC_FP32, capi.PREC_FP64_EXACT])
+def test_graph_block_matches_eager(prec):
+    s = scenes.tiny(n_cells=8)
+    rel = 8 if prec == capi.PREC_FP32 else 0
+    runs = []
+    for block in (0, 16):
+        print("graph block %d prec %d" % (block, prec), flush=True)
+        e = engine(s, prec, relayout=rel)
+        e.set_option(capi.OPT_GRAPH_BLOCK, block)
+        # remainders cross relayouts, so later calls must re-record the graph
+        assert e.step(1, 37) == 0
+        assert e.step(38, 50) == 0
+        assert e.step(88, 3) == 0
+        runs.append((e.download(s.n), e.nodes()))
+    (a, na), (b, nb) = runs
+    assert_parts_equal(a, b)
+    assert np.array_equal(na.nmass, nb.nmass) and np.array_equal(na.nmome, nb.nmome)
