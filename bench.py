@@ -67,7 +67,8 @@ def parse():
     ap.add_argument("--config", default="cfg2-4M", choices=["cfg1", "cfg2", "cfg2-4M", "cfg3", "cfg5"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--relayout", type=int, default=32)
-    ap.add_argument("--graph-block", type=int, default=64,
+    ap.add_argument("--fused-count", type=int, default=1, help="count cells in the update kernel (1) or not (0)")
+    ap.add_argument("--graph-block", type=int, default=0,
                     help="steps per recorded HIP graph block (0 = eager launches)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -283,6 +284,7 @@ def main():
         eng.set_option(capi.OPT_LANES_PER_CELL, a.lanes_per_cell)
     if a.graph_block:
         eng.set_option(capi.OPT_GRAPH_BLOCK, a.graph_block)
+    eng.set_option(capi.OPT_FUSED_COUNT, a.fused_count)
     capi.check(eng.step(1, a.warmup))
     capi.check(eng.lib.mpm_sync(eng.h))
     t0 = time.perf_counter()

@@ -47,7 +47,7 @@ static int bit_length(uint32_t v) {
 
 using namespace mpm;
 
-enum { OPT_FULL_NODE_OUTPUT = 1, OPT_LANES_PER_CELL = 2, OPT_RELAYOUT_EVERY = 3, OPT_GRAPH_BLOCK = 4 };
+enum { OPT_FULL_NODE_OUTPUT = 1, OPT_LANES_PER_CELL = 2, OPT_RELAYOUT_EVERY = 3, OPT_GRAPH_BLOCK = 4, OPT_FUSED_COUNT = 5 };
 
 struct mpm_engine {
     GridGeom g{};
@@ -90,6 +90,7 @@ struct mpm_engine {
     bool bins_valid = false;
     bool keys_valid = false;
     bool counts_ready = false;  // per-cell counts of the current keys are in bb.cnt (fused into the update)
+    bool fused_count = true;    // FP32: the update kernel counts the new keys for the next binning
     int last_pass = 0;  // 1 = p2g, 2 = g2p
     int steps_since_relayout = 0;
     int full_node_output = 1;
@@ -324,9 +325,9 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         // count the new keys for the next binning; counts left from an update
         // that no binning consumed are cleared first
         if (e->counts_ready) launch_fill_u32(e->stream, e->bb.cnt, bin_cell_capacity(e->g.nelem), 0u);
-        BinFuse bf{e->bb.cnt, e->bb.tiles, e->tg.ntiles, e->bb.ctr};
+        BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->bb.tiles, e->tg.ntiles, e->bb.ctr};
         launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa, bf);
-        e->counts_ready = true;
+        e->counts_ready = e->fused_count;
         e->keys_valid = true;  // the update kernel writes the new keys
     }
     e->bins_valid = false;
@@ -1084,7 +1085,7 @@ int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *nu
 // ready (as after an update kernel), so that every recorded block starts and
 // ends in the same state.
 static void prime_counts(mpm_engine *e) {
-    if (e->prec == MPM_PREC_FP32 && !e->counts_ready) {
+    if (e->prec == MPM_PREC_FP32 && e->fused_count && !e->counts_ready) {
         launch_bin_count(e->stream, e->g, e->tg, e->P32[e->cur].key, e->n, e->bb);
         e->counts_ready = true;
     }
@@ -1153,7 +1154,7 @@ int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
             if (he != hipSuccess) return hip_fail(he, "hipStreamEndCapture", __FILE__, __LINE__);
             e->step_graph = g;
             if (e->cur != cur0 || e->steps_since_relayout != since0 ||
-                (e->prec == MPM_PREC_FP32 && !e->counts_ready)) {
+                (e->prec == MPM_PREC_FP32 && e->fused_count && !e->counts_ready)) {
                 e->drop_graph();
                 return set_error(MPM_ERR_INVALID_ARGUMENT, "graph block does not return to the same particle buffer");
             }
@@ -1257,6 +1258,13 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
         if (e->prec != MPM_PREC_FP32) return value ? set_error(MPM_ERR_UNSUPPORTED, "relayout is an FP32 option") : 0;
         e->relayout_every = (int)value;
         e->steps_since_relayout = 0;
+        return 0;
+    case OPT_FUSED_COUNT:
+        if (e->counts_ready) {
+            launch_fill_u32(e->stream, e->bb.cnt, bin_cell_capacity(e->g.nelem), 0u);
+            e->counts_ready = false;
+        }
+        e->fused_count = value != 0;
         return 0;
     case OPT_GRAPH_BLOCK:
         if (value < 0 || value > 4096) return set_error(MPM_ERR_INVALID_ARGUMENT, "graph block out of range");

@@ -29,6 +29,7 @@ MODEL_TAIT = 20
 OPT_FULL_NODE_OUTPUT = 1
 OPT_RELAYOUT_EVERY = 3
 OPT_GRAPH_BLOCK = 4
+OPT_FUSED_COUNT = 5
 OPT_LANES_PER_CELL = 2
 
 _dp = C.POINTER(C.c_double)

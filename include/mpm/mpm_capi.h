@@ -212,8 +212,16 @@ int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
  * many steps into a HIP graph once and relaunches it (no launch gaps); used
  * when timing is off, the load ramp is over, no samples are logged and the
  * block is a multiple of twice the relayout interval. Uploads, options and
- * surrogate changes drop the recorded graph. */
-enum { MPM_OPT_FULL_NODE_OUTPUT = 1, MPM_OPT_LANES_PER_CELL = 2, MPM_OPT_RELAYOUT_EVERY = 3, MPM_OPT_GRAPH_BLOCK = 4 };
+ * surrogate changes drop the recorded graph.
+ * MPM_OPT_FUSED_COUNT (FP32, default 1): the update kernel counts particles
+ * per cell for the next binning, which then skips its count pass. */
+enum {
+    MPM_OPT_FULL_NODE_OUTPUT = 1,
+    MPM_OPT_LANES_PER_CELL = 2,
+    MPM_OPT_RELAYOUT_EVERY = 3,
+    MPM_OPT_GRAPH_BLOCK = 4,
+    MPM_OPT_FUSED_COUNT = 5
+};
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 /* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
 int mpm_set_timing(mpm_engine *eng, int32_t enabled);
