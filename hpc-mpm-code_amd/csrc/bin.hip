@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_reduce(const uint32_t *__restric
 }
 
 __global__ void __launch_bounds__(kBlock) k_bin_top(uint64_t *__restrict__ bsum, int nb, BinCounters *ctr,
-                                                     uint32_t *__restrict__ astart) {
+                                                     uint32_t *__restrict__ astart, int n, uint32_t *err) {
     const int base = threadIdx.x * kItems;
     uint64_t v[kItems];
     uint64_t s = 0;
@@ -116,8 +116,13 @@ __global__ void __launch_bounds__(kBlock) k_bin_top(uint64_t *__restrict__ bsum,
     }
     if (threadIdx.x == 0) {
         ctr->stamp += 1u;
-        ctr->n_active = (uint32_t)(tot >> 32);
         astart[tot >> 32] = (uint32_t)tot;
+        // more counted particles than exist: nothing of this binning may be
+        // used (the later kernels would index past the permutation)
+        const bool bad = (uint64_t)(uint32_t)tot > (uint64_t)n;
+        ctr->bad = bad ? 1u : 0u;
+        ctr->n_active = bad ? 0u : (uint32_t)(tot >> 32);
+        if (bad) atomicOr(err, kErrBinCounts);
     }
 }
 
@@ -182,6 +187,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict_
                                                       uint32_t *__restrict__ fixlist, BinCounters *ctr) {
     const int lane = threadIdx.x & 63;
     const uint32_t stamp = ctr->stamp;
+    if (ctr->bad) return;
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int p = base + threadIdx.x;
         const uint32_t k = (p < n) ? keys[p] : 0xFFFFFFFFu;
@@ -206,6 +212,7 @@ __global__ void __launch_bounds__(256) k_bin_fixup(const uint32_t *__restrict__ 
                                                     const uint32_t *__restrict__ cstart,
                                                     const uint32_t *__restrict__ cend, uint32_t *__restrict__ perm,
                                                     uint32_t *__restrict__ err) {
+    if (ctr->bad) return;
     const uint32_t nfix = ctr->n_fix;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nfix; i += gridDim.x * blockDim.x) {
         const uint32_t c = fixlist[i];
@@ -292,7 +299,7 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
         hipLaunchKernelGGL(k_bin_count, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, b.cnt, b.tiles, tg.ntiles,
                            b.ctr);
     hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
-    hipLaunchKernelGGL(k_bin_top, dim3(1), dim3(kBlock), 0, s, b.bsum, nb, b.ctr, b.astart);
+    hipLaunchKernelGGL(k_bin_top, dim3(1), dim3(kBlock), 0, s, b.bsum, nb, b.ctr, b.astart, n, err);
     hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, b.cstart,
                        b.cend, b.active, b.astart, b.tiles, tg);
     if (n > 0) {

@@ -151,7 +151,15 @@ struct mpm_engine {
         return 0;
     }
 
+    // counts left by a fused update kernel that no binning consumed
+    void clear_counts() {
+        if (counts_ready && bb.cnt) launch_fill_u32(stream, bb.cnt, bin_cell_capacity(g.nelem), 0u);
+        counts_ready = false;
+    }
+
     void free_particles() {
+        clear_counts();
+        if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : part_allocs) (void)hipFree(p);
         part_allocs.clear();
         P64 = PartF64{};
@@ -164,7 +172,6 @@ struct mpm_engine {
         perm_id = nullptr;
         n = 0;
         bins_valid = keys_valid = false;
-        counts_ready = false;
     }
 
     ~mpm_engine() {
@@ -235,6 +242,7 @@ int check_err(mpm_engine *e) {
     if (h & kErrNonFinite) return set_error(MPM_ERR_NONFINITE, "non-finite particle state");
     if (h & kErrJacobian) return set_error(MPM_ERR_JACOBIAN, "det F <= 0");
     if (h & kErrCellOverflow) return set_error(MPM_ERR_UNSUPPORTED, "more than 4096 particles in one cell");
+    if (h & kErrBinCounts) return set_error(MPM_ERR_INTERNAL, "binning: cell counts exceed the particle count");
     return 0;
 }
 
@@ -324,7 +332,7 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         sa.pid = e->pid[e->cur];
         // count the new keys for the next binning; counts left from an update
         // that no binning consumed are cleared first
-        if (e->counts_ready) launch_fill_u32(e->stream, e->bb.cnt, bin_cell_capacity(e->g.nelem), 0u);
+        e->clear_counts();
         BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->bb.tiles, e->tg.ntiles, e->bb.ctr};
         launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa, bf);
         e->counts_ready = e->fused_count;
@@ -905,10 +913,7 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
     }
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     e->keys_valid = e->prec == MPM_PREC_FP32;
-    if (e->counts_ready) {
-        launch_fill_u32(e->stream, e->bb.cnt, bin_cell_capacity(e->g.nelem), 0u);
-        e->counts_ready = false;
-    }
+    e->clear_counts();
     e->bins_valid = false;
     e->steps_since_relayout = 0;
     return 0;
@@ -1260,10 +1265,7 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
         e->steps_since_relayout = 0;
         return 0;
     case OPT_FUSED_COUNT:
-        if (e->counts_ready) {
-            launch_fill_u32(e->stream, e->bb.cnt, bin_cell_capacity(e->g.nelem), 0u);
-            e->counts_ready = false;
-        }
+        e->clear_counts();
         e->fused_count = value != 0;
         return 0;
     case OPT_GRAPH_BLOCK:

@@ -47,6 +47,7 @@ enum : uint32_t {
     kErrNonFinite = 2u,
     kErrJacobian = 4u,
     kErrCellOverflow = 8u,  // more than 4096 particles in one cell
+    kErrBinCounts = 16u,    // per-cell counts add up to more than the particle count
 };
 
 // Node tiles of the node passes (x fastest); a tile is processed only when a

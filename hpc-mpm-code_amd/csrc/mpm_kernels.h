@@ -36,7 +36,7 @@ struct BinCounters {
     uint32_t n_active;
     uint32_t n_fix;
     uint32_t stamp;  // binning generation, advanced on the device (graph replays stay distinct)
-    uint32_t pad;
+    uint32_t bad;    // counts inconsistent with the particle count: this binning placed nothing
 };
 
 struct CellLists {

@@ -21,6 +21,7 @@ MPM_ERR_JACOBIAN = -4
 MPM_ERR_HIP = -5
 MPM_ERR_UNSUPPORTED = -6
 MPM_ERR_NO_DEVICE = -7
+MPM_ERR_INTERNAL = -8
 
 SCHEME_PIC, SCHEME_FLIP = 0, 1
 PREC_FP64_EXACT, PREC_FP32 = 0, 1

@@ -385,3 +385,22 @@ def test_graph_block_matches_eager(prec):
     (a, na), (b, nb) = runs
     assert_parts_equal(a, b)
     assert np.array_equal(na.nmass, nb.nmass) and np.array_equal(na.nmome, nb.nmome)
+
+
+# ------------------------------------------------------------------ re-upload into a used engine
+@pytest.mark.parametrize("prec", [capi.PREC_FP32, capi.PREC_FP64_EXACT])
+def test_reupload_after_steps_matches_fresh_engine(prec):
+    # the update kernel leaves the next binning's cell counts behind; a new
+    # upload must discard them (regression: they were added to the new counts)
+    s = scenes.tiny(n_cells=8)
+    rel = 8 if prec == capi.PREC_FP32 else 0
+    print("reupload prec %d" % prec, flush=True)
+    a = engine(s, prec, relayout=rel)
+    assert a.step(1, 5) == 0
+    a.upload(s.parts)
+    assert a.step(1, 20) == 0
+    a.upload(s.parts)
+    assert a.step(1, 20) == 0
+    b = engine(s, prec, relayout=rel)
+    assert b.step(1, 20) == 0
+    assert_parts_equal(a.download(s.n), b.download(s.n))
