@@ -97,6 +97,12 @@ struct mpm_engine {
     int relayout_every = 0;
     int active_cells = 0;
 
+    // reference particle shifting (FP64-exact engine)
+    bool shifting = false;
+    double *d_nvol = nullptr, *d_nei = nullptr, *d_part_eu = nullptr, *d_b0 = nullptr;
+    double *d_geup = nullptr, *d_part_dot = nullptr;  // particle-sized
+    int geup_n = 0;
+
     // whole-step graph: blocks of graph_block steps captured once and replayed
     int graph_block = 0;
     int graph_cur = -1, graph_since = -1;  // particle buffer and relayout counter the graph starts from
@@ -176,6 +182,8 @@ struct mpm_engine {
 
     ~mpm_engine() {
         drop_graph();
+        for (void *p : {(void *)d_geup, (void *)d_part_dot})
+            if (p) (void)hipFree(p);
         free_particles();
         for (void *p : allocs) (void)hipFree(p);
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -300,8 +308,21 @@ int do_solve(mpm_engine *e) {
 
 int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2 = nullptr) {
     if (e->prec == MPM_PREC_FP64_EXACT) {
+        ShiftF64 sh{};
+        if (e->shifting) {
+            if (e->geup_n != e->n) {
+                for (double **p : {&e->d_geup, &e->d_part_dot}) {
+                    if (*p) (void)hipFree(*p);
+                    *p = nullptr;
+                }
+                MPM_HIP_CHECK(hipMalloc((void **)&e->d_geup, 3 * (size_t)e->ld * sizeof(double)));
+                MPM_HIP_CHECK(hipMalloc((void **)&e->d_part_dot, (size_t)cdiv(e->n, kShiftBlock) * sizeof(double)));
+                e->geup_n = e->n;
+            }
+            sh = ShiftF64{e->d_nvol, e->d_nei, e->d_geup, e->d_part_eu, e->d_part_dot, e->d_b0};
+        }
         launch_f64_g2p(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->mt, e->prm.scheme, e->prm.dt,
-                       e->prm.mtol, facl, e->d_err);
+                       e->prm.mtol, facl, e->d_err, e->shifting ? &sh : nullptr);
         e->keys_valid = false;
     } else {
         StepConsts c = consts(e, facl);
@@ -750,6 +771,58 @@ int mpm_surrogate_download_samples(mpm_engine *e, float *in, float *tgt, int64_t
         MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     }
     *n = k;
+    return 0;
+}
+
+int mpm_set_shifting(mpm_engine *e, int32_t on, const int32_t *npxye) {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    if (on && e->prec != MPM_PREC_FP64_EXACT)
+        return set_error(MPM_ERR_UNSUPPORTED, "particle shifting runs in the FP64-exact engine only");
+    e->drop_graph();
+    if (!on) {
+        e->shifting = false;
+        return 0;
+    }
+    if (!npxye) return set_error(MPM_ERR_INVALID_ARGUMENT, "null points-per-cell");
+    for (int a = 0; a < 3; a++)
+        if (npxye[a] < 1 || npxye[a] > 6) return set_error(MPM_ERR_INVALID_ARGUMENT, "points per cell must be 1..6");
+    int rc;
+    const size_t nodec = (size_t)e->g.nodec;
+    if (!e->d_nvol &&
+        ((rc = e->dalloc(&e->d_nvol, nodec)) || (rc = e->dalloc(&e->d_nei, nodec)) ||
+         (rc = e->dalloc(&e->d_part_eu, (size_t)cdiv(e->g.nodec, kShiftBlock))) || (rc = e->dalloc(&e->d_b0, 1))))
+        return rc;
+    // GaussianDistribution offsets (module/mesh.cpp:12-54)
+    GaussOffsets dec{};
+    for (int a = 0; a < 3; a++) {
+        const double h = e->g.dx[a];
+        double *d = &dec.d[0][a];
+        auto set = [&](int i, double v) { d[3 * i] = v; };
+        switch (npxye[a]) {
+        case 1: set(0, 0.0); break;
+        case 2: set(0, -0.25 * h); set(1, 0.25 * h); break;
+        case 3: set(0, -1.0 / 3.0 * h); set(1, 0.0); set(2, 1.0 / 3.0 * h); break;
+        case 4: set(0, -0.375 * h); set(1, -0.125 * h); set(2, 0.125 * h); set(3, 0.375 * h); break;
+        case 5: set(0, -0.4 * h); set(1, -0.2 * h); set(2, 0.0); set(3, 0.2 * h); set(4, 0.4 * h); break;
+        default:
+            set(0, -5.0 / 12.0 * h); set(1, -3.0 / 12.0 * h); set(2, -1.0 / 12.0 * h);
+            set(3, 1.0 / 12.0 * h); set(4, 3.0 / 12.0 * h); set(5, 5.0 / 12.0 * h);
+            break;
+        }
+    }
+    const double volp = e->g.dx[0] * e->g.dx[1] * e->g.dx[2] / (double)(npxye[0] * npxye[1] * npxye[2]);
+    launch_f64_nodal_vol(e->stream, e->g, npxye[0], npxye[1], npxye[2], dec, volp, e->d_nvol);
+    MPM_HIP_CHECK(hipGetLastError());
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->shifting = true;
+    return 0;
+}
+
+int mpm_download_nodal_volume(mpm_engine *e, double *nvol) {
+    if (!e || !nvol) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!e->d_nvol) return set_error(MPM_ERR_INVALID_ARGUMENT, "no nodal volume (call mpm_set_shifting)");
+    MPM_HIP_CHECK(hipMemcpyAsync(nvol, e->d_nvol, sizeof(double) * e->g.nodec, hipMemcpyDeviceToHost, e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return 0;
 }
 

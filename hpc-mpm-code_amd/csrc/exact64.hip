@@ -453,7 +453,8 @@ __device__ void update_stress64(const MaterialDev &md, double *sig, size_t ld, i
 
 // Deformation gradient, coordinates, volume and stress
 // (var_trans_solid_explicit.cpp:244-296 and :150-185), one thread per particle.
-__global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err) {
+__global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err,
+                               const double *__restrict__ geup, const double *__restrict__ b0p) {
     int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P.n) return;
     uint32_t m = P.key[pid];
@@ -503,6 +504,12 @@ __global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable 
     }
     double J = det3(F);
     P.J[pid] = J;
+    if (geup) {  // shifting correction after the position update (var_trans_solid_explicit.cpp:176)
+        const double b0 = *b0p;
+        dx0 += -b0 * geup[pid];
+        dx1 += -b0 * geup[ld + pid];
+        dx2 += -b0 * geup[2 * ld + pid];
+    }
     P.x[pid] = dx0;
     P.x[ld + pid] = dx1;
     P.x[2 * ld + pid] = dx2;
@@ -549,15 +556,135 @@ void launch_f64_solve_grid(hipStream_t s, const GridGeom &g, const NodesF64 &N, 
     hipLaunchKernelGGL(k64_solve_grid, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, N, bc, mtol);
 }
 
+// MakNodalVol (module/mesh.cpp:160-202) for one node: cells around it in
+// ascending order, points of each cell in (z, y, x) order, as the reference's
+// scatter adds them.
+__global__ void k64_nodal_vol(GridGeom g, int npx, int npy, int npz, GaussOffsets dec, double volp,
+                              double *__restrict__ nvol) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    const int i = n % g.nxc[0], j = (n / g.nxc[0]) % g.nxc[1], k = n / (g.nxc[0] * g.nxc[1]);
+    int cells[27];
+    const int nc = node_cells(g, i, j, k, cells);
+    double acc = 0.0;
+    for (int q = 0; q < nc; q++) {
+        int ixe, iye, ize;
+        cell_ijk(g, cells[q], ixe, iye, ize);
+        double xye[3], xyp[3];
+        xye[0] = g.xmin[0] + g.dx[0] * ((double)ixe + 0.5);
+        xye[1] = g.xmin[1] + g.dx[1] * ((double)iye + 0.5);
+        xye[2] = g.xmin[2] + g.dx[2] * ((double)ize + 0.5);
+        for (int iz = 0; iz < npz; iz++) {
+            xyp[2] = xye[2] + dec.d[iz][2];
+            for (int iy = 0; iy < npy; iy++) {
+                xyp[1] = xye[1] + dec.d[iy][1];
+                for (int ix = 0; ix < npx; ix++) {
+                    xyp[0] = xye[0] + dec.d[ix][0];
+                    double sfi, ds[3];
+                    sf_node(g, xyp, i, j, k, sfi, ds);
+                    acc += sfi * volp;
+                }
+            }
+        }
+    }
+    nvol[n] = acc;
+}
+
+// excess nodal volume nei = max(0, nvof - nvol) and the block sums of nei^2
+__global__ void k64_shift_nei(int nodec, const double *__restrict__ nvof, ShiftF64 S) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = b * kShiftBlock;
+    if (i0 >= nodec) return;
+    double s = 0.0;
+    for (int i = i0; i < nodec && i < i0 + kShiftBlock; i++) {
+        const double e = -S.nvol[i] + nvof[i];
+        const double ne = e > 0.0 ? e : 0.0;
+        S.nei[i] = ne;
+        s += ne * ne;
+    }
+    S.part_eu[b] = s;
+}
+
+// gradient of nei at the particle (old position), times 2 V (old volume)
+__global__ void k64_shift_geup(GridGeom g, PartF64 P, ShiftF64 S) {
+    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P.n) return;
+    const size_t ld = P.ld;
+    const uint32_t m = P.key[pid];
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (m < (uint32_t)g.nelem) {
+        int ixe, iye, ize;
+        cell_ijk(g, (int)m, ixe, iye, ize);
+        const double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+        for (int k = 0; k <= g.idimc[2]; k++)
+            for (int j = 0; j <= g.idimc[1]; j++)
+                for (int i = 0; i <= g.idimc[0]; i++) {
+                    const int ix = ixe + i, iy = iye + j, iz = ize + k;
+                    const int nid = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+                    double sfi, ds[3];
+                    sf_node(g, xyg, ix, iy, iz, sfi, ds);
+                    g0 += ds[0] * S.nei[nid];
+                    g1 += ds[1] * S.nei[nid];
+                    g2 += ds[2] * S.nei[nid];
+                }
+        const double s2 = 2.0 * P.V[pid];
+        g0 *= s2;
+        g1 *= s2;
+        g2 *= s2;
+    }
+    S.geup[pid] = g0;
+    S.geup[ld + pid] = g1;
+    S.geup[2 * ld + pid] = g2;
+}
+
+__global__ void k64_shift_dot(int n, size_t ld, ShiftF64 S) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = b * kShiftBlock;
+    if (i0 >= n) return;
+    double s = 0.0;
+    for (int i = i0; i < n && i < i0 + kShiftBlock; i++)
+        for (int c = 0; c < 3; c++) {
+            const double v = S.geup[c * ld + i];
+            s += v * v;
+        }
+    S.part_dot[b] = s;
+}
+
+__global__ void k64_shift_b0(int nbe, int nbd, double mtol, ShiftF64 S) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double eu = 0.0, dot = 0.0;
+    for (int b = 0; b < nbe; b++) eu += S.part_eu[b];
+    for (int b = 0; b < nbd; b++) dot += S.part_dot[b];
+    *S.b0 = (dot < mtol) ? 0.0 : (eu / dot);
+}
+
+void launch_f64_nodal_vol(hipStream_t s, const GridGeom &g, int npx, int npy, int npz, const GaussOffsets &dec,
+                          double volp, double *nvol) {
+    hipLaunchKernelGGL(k64_nodal_vol, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, npx, npy, npz, dec, volp, nvol);
+}
+
 void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
                     const BcDense &bc, const MaterialTable &mt, int scheme, double dt, double mtol, double facl,
-                    uint32_t *err) {
+                    uint32_t *err, const ShiftF64 *shift) {
     if (scheme != 1)
         hipLaunchKernelGGL(k64_vel_plus_dt_acc, dim3(cdiv(3 * g.nodec, 256)), dim3(256), 0, s, 3 * g.nodec, N.nv,
                            (const double *)N.na, dt);
     if (P.n > 0) hipLaunchKernelGGL(k64_g2p_vel, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, scheme, dt);
     hipLaunchKernelGGL(k64_musl, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, mtol, facl);
-    if (P.n > 0) hipLaunchKernelGGL(k64_g2p_update, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, mt, dt, err);
+    const double *geup = nullptr, *b0 = nullptr;
+    if (shift && P.n > 0) {
+        // update_point_coord_and_stress computes the correction before moving
+        // any particle (old positions, old volumes)
+        const int nbe = cdiv(g.nodec, kShiftBlock), nbd = cdiv(P.n, kShiftBlock);
+        hipLaunchKernelGGL(k64_shift_nei, dim3(cdiv(nbe, 64)), dim3(64), 0, s, g.nodec, (const double *)N.nvof, *shift);
+        hipLaunchKernelGGL(k64_shift_geup, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, *shift);
+        hipLaunchKernelGGL(k64_shift_dot, dim3(cdiv(nbd, 64)), dim3(64), 0, s, P.n, P.ld, *shift);
+        hipLaunchKernelGGL(k64_shift_b0, dim3(1), dim3(64), 0, s, nbe, nbd, mtol, *shift);
+        geup = shift->geup;
+        b0 = shift->b0;
+    }
+    if (P.n > 0)
+        hipLaunchKernelGGL(k64_g2p_update, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, mt, dt, err, geup, b0);
 }
 
 int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cell, const double *xyg, int32_t *nc,

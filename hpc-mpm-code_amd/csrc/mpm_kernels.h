@@ -138,9 +138,26 @@ struct NodesF64 {
 void launch_f64_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
                     const BcDense &bc, double fx, double fy, double fz, double mtol, double facl);
 void launch_f64_solve_grid(hipStream_t s, const GridGeom &g, const NodesF64 &N, const BcDense &bc, double mtol);
+// Particle shifting of the reference (DeltaCorrectionParticleShifting), FP64:
+// nvol = nodal volume of the filled grid; nei, geup, partial sums and b0 are
+// scratch. Global sums are taken in blocks of kShiftBlock consecutive entries.
+constexpr int kShiftBlock = 256;
+struct ShiftF64 {
+    const double *nvol;  // [nodec]
+    double *nei;         // [nodec]
+    double *geup;        // [3][ld]
+    double *part_eu;     // [cdiv(nodec, kShiftBlock)]
+    double *part_dot;    // [cdiv(n, kShiftBlock)]
+    double *b0;          // [1]
+};
+struct GaussOffsets {
+    double d[6][3];
+};
+void launch_f64_nodal_vol(hipStream_t s, const GridGeom &g, int npx, int npy, int npz, const GaussOffsets &dec,
+                          double volp, double *nvol);
 void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
                     const BcDense &bc, const MaterialTable &mt, int scheme, double dt, double mtol, double facl,
-                    uint32_t *err);
+                    uint32_t *err, const ShiftF64 *shift = nullptr);
 int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cell, const double *xyg, int32_t *nc,
                      double *sf, double *dsf);
 

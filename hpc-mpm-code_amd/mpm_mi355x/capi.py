@@ -111,6 +111,8 @@ SIGNATURES = {
     "mpm_maksf_batch": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _ip, _dp, _ip, _dp, _dp]),
     "mpm_bin_positions": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _dp, _ip, _ip]),
     "mpm_bandwidth_probe": (C.c_int, [C.c_int64, C.c_int32, _dp]),
+    "mpm_set_shifting": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
+    "mpm_download_nodal_volume": (C.c_int, [C.c_void_p, _dp]),
     # include/mpm/mpm_surrogate.h
     "mpm_mlp_forward": (C.c_int, [_fp, _fp, _fp, C.c_int64, _fp]),
     "mpm_mlp_gradient": (C.c_int, [_fp, _fp, _fp, _fp, C.c_int64, _fp, _fp]),
@@ -284,6 +286,17 @@ class Engine:
             nbc = np.ascontiguousarray(nbc, dtype=np.int32)
             fbc = np.ascontiguousarray(fbc, dtype=np.float64)
             check(lib.mpm_set_bc(self.h, comp, _ptr(nbc, C.c_int32), _ptr(fbc), len(nbc)))
+        if getattr(scene, "shifting", False):
+            self.set_shifting(True, scene.npxye)
+
+    def set_shifting(self, on, npxye=(2, 2, 2)):
+        npx = np.ascontiguousarray(npxye, dtype=np.int32)
+        check(self.lib.mpm_set_shifting(self.h, int(on), _ptr(npx, C.c_int32)))
+
+    def nodal_volume(self):
+        out = np.zeros(self.nodec)
+        check(self.lib.mpm_download_nodal_volume(self.h, _ptr(out)))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

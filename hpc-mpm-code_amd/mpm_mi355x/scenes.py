@@ -62,6 +62,8 @@ class Scene:
     bcs: dict = field(default_factory=dict)   # comp -> (nbc, fbc)
     parts: object = None
     idimc: tuple = (1, 1, 1)                 # 1 = linear tent, 2 = quadratic B-spline (FP64-exact engine)
+    shifting: bool = False                   # reference particle shifting (FP64-exact engine)
+    npxye: tuple = (2, 2, 2)                 # points per cell of the nodal-volume reference (MakNodalVol)
 
     @property
     def dx(self):

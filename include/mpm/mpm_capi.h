@@ -224,6 +224,18 @@ enum {
     MPM_OPT_FUSED_COUNT = 5
 };
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
+/* Reference particle shifting (DeltaCorrectionParticleShifting,
+ * module/material_point.cpp:174-227, applied in update_point_coord_and_stress
+ * var_trans_solid_explicit.cpp:159-176), FP64-exact engine only. on = 1
+ * computes the nodal volume of the filled grid (MakNodalVol,
+ * module/mesh.cpp:160-202) for npxye[3] points per cell and applies the
+ * correction in every later step; on = 0 turns it off (the default). The two
+ * global sums of the correction are taken in blocks of 256 consecutive
+ * entries, then over the blocks (single-rank order; the reference sums per
+ * MPI rank). */
+int mpm_set_shifting(mpm_engine *eng, int32_t on, const int32_t *npxye);
+/* Nodal volume used by the shifting ([nodec]). */
+int mpm_download_nodal_volume(mpm_engine *eng, double *nvol);
 /* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
 int mpm_set_timing(mpm_engine *eng, int32_t enabled);
 

@@ -465,6 +465,103 @@ static void j2_update(const orc_state *st, int pid, const double *prop, const do
 
 /* Node2ParticleS + MUSL + update_point_coord_and_stress,
  * var_trans_solid_explicit.cpp:101-311 (F-bar off, PIC/FLIP). */
+/* MakNodalVol, module/mesh.cpp:160-202; offsets of GaussianDistribution :12-54 */
+void orc_nodal_vol(orc_state *st, const int32_t npxye[3]) {
+    double dec[6][3];
+    memset(dec, 0, sizeof(dec));
+    for (int a = 0; a < 3; a++) {
+        const double h = st->dx[a];
+        switch (npxye[a]) {
+        case 1: dec[0][a] = 0.0; break;
+        case 2: dec[0][a] = -0.25 * h; dec[1][a] = 0.25 * h; break;
+        case 3: dec[0][a] = -1.0 / 3.0 * h; dec[1][a] = 0.0; dec[2][a] = 1.0 / 3.0 * h; break;
+        case 4:
+            dec[0][a] = -0.375 * h; dec[1][a] = -0.125 * h; dec[2][a] = 0.125 * h; dec[3][a] = 0.375 * h;
+            break;
+        case 5:
+            dec[0][a] = -0.4 * h; dec[1][a] = -0.2 * h; dec[2][a] = 0.0; dec[3][a] = 0.2 * h; dec[4][a] = 0.4 * h;
+            break;
+        case 6:
+            dec[0][a] = -5.0 / 12.0 * h; dec[1][a] = -3.0 / 12.0 * h; dec[2][a] = -1.0 / 12.0 * h;
+            dec[3][a] = 1.0 / 12.0 * h; dec[4][a] = 3.0 / 12.0 * h; dec[5][a] = 5.0 / 12.0 * h;
+            break;
+        default: break;
+        }
+    }
+    const double volp = st->dx[0] * st->dx[1] * st->dx[2] / (double)(npxye[0] * npxye[1] * npxye[2]);
+    const int nex = st->nex[0], ney = st->nex[1];
+    int32_t ncm[27];
+    double sf[27], dsf[81];
+    memset(st->nvol, 0, sizeof(double) * st->nodec);
+    for (int m = 0; m < st->nelem; m++) {
+        const int ize = m / (nex * ney);
+        const int iye = (m - ize * (nex * ney)) / nex;
+        const int ixe = m - ize * (nex * ney) - iye * nex;
+        double xye[3], xyp[3];
+        xye[0] = st->xmin[0] + st->dx[0] * ((double)ixe + 0.5);
+        xye[1] = st->xmin[1] + st->dx[1] * ((double)iye + 0.5);
+        xye[2] = st->xmin[2] + st->dx[2] * ((double)ize + 0.5);
+        for (int iz = 0; iz < npxye[2]; iz++) {
+            xyp[2] = xye[2] + dec[iz][2];
+            for (int iy = 0; iy < npxye[1]; iy++) {
+                xyp[1] = xye[1] + dec[iy][1];
+                for (int ix = 0; ix < npxye[0]; ix++) {
+                    xyp[0] = xye[0] + dec[ix][0];
+                    int nenode = orc_maksf(st, m, xyp, ncm, sf, dsf);
+                    for (int ni = 0; ni < nenode; ni++) st->nvol[ncm[ni]] += sf[ni] * volp;
+                }
+            }
+        }
+    }
+}
+
+/* DeltaCorrectionParticleShifting, module/material_point.cpp:174-227 */
+void orc_shift_delta(const orc_state *st, double *delta) {
+    const int nn = st->nodec, num = st->num;
+    double *nei = malloc(sizeof(double) * nn);
+    double *geup = calloc(3 * (size_t)num, sizeof(double));
+    /* excess nodal volume and its squared norm */
+    double eu_norm = 0.0;
+    for (int b0 = 0; b0 < nn; b0 += ORC_SUM_BLOCK) {
+        double s = 0.0;
+        for (int i = b0; i < nn && i < b0 + ORC_SUM_BLOCK; i++) {
+            const double e = -st->nvol[i] + st->nvof[i];
+            nei[i] = e > 0.0 ? e : 0.0;
+            s += nei[i] * nei[i];
+        }
+        eu_norm += s;
+    }
+    /* gradient of the excess volume at the particles */
+    int32_t ncm[27];
+    double sf[27], dsf[81];
+    for (int m = 0; m < st->nelem; m++) {
+        for (int pid = st->idepf[m]; pid != -1; pid = st->idp2p[pid]) {
+            int nenode = orc_maksf(st, m, st->coord + 3 * pid, ncm, sf, dsf);
+            double *g = geup + 3 * pid;
+            for (int ni = 0; ni < nenode; ni++) {
+                const int nid = ncm[ni];
+                g[0] += dsf[3 * ni] * nei[nid];
+                g[1] += dsf[3 * ni + 1] * nei[nid];
+                g[2] += dsf[3 * ni + 2] * nei[nid];
+            }
+        }
+    }
+    double geup_dot = 0.0;
+    for (int b0 = 0; b0 < num; b0 += ORC_SUM_BLOCK) {
+        double s = 0.0;
+        for (int n = b0; n < num && n < b0 + ORC_SUM_BLOCK; n++) {
+            double *g = geup + 3 * n;
+            for (int i = 0; i < 3; i++) g[i] *= 2.0 * st->vol[n];
+            for (int i = 0; i < 3; i++) s += g[i] * g[i];
+        }
+        geup_dot += s;
+    }
+    const double b_0 = (geup_dot < st->mtol) ? 0.0 : (eu_norm / geup_dot);
+    for (int n = 0; n < 3 * num; n++) delta[n] = -b_0 * geup[n];
+    free(nei);
+    free(geup);
+}
+
 void orc_g2p(orc_state *st, double facl) {
     const int nn = st->nodec;
     const double dt = st->dt;
@@ -558,7 +655,13 @@ void orc_g2p(orc_state *st, double facl) {
         }
     }
 
-    /* update_point_coord_and_stress :150-185 (shifting off unless st->shifting) */
+    /* update_point_coord_and_stress :150-185; the shifting correction is
+     * computed first, from the positions and volumes before the update */
+    double *delta = NULL;
+    if (st->shifting && st->nvol) {
+        delta = malloc(sizeof(double) * 3 * (size_t)st->num);
+        orc_shift_delta(st, delta);
+    }
     for (int m = 0; m < st->nelem; m++) {
         for (int pid = st->idepf[m]; pid != -1; pid = st->idp2p[pid]) {
             double xyp[3] = {st->coord[3 * pid], st->coord[3 * pid + 1], st->coord[3 * pid + 2]};
@@ -571,6 +674,8 @@ void orc_g2p(orc_state *st, double facl) {
                 x[1] += sfi * dt * st->nvel[nid + nn];
                 x[2] += sfi * dt * st->nvel[nid + 2 * nn];
             }
+            if (delta)
+                for (int i = 0; i < 3; i++) x[i] += delta[3 * pid + i];
             st->vol[pid] = st->det_def_grad[pid] * st->vol0[pid];
             int mat = st->matid ? st->matid[pid] : 0;
             int model = st->iprop[mat];
@@ -584,6 +689,7 @@ void orc_g2p(orc_state *st, double facl) {
                                   st->det_def_grad[pid], st->det_def_grad[pid]);
         }
     }
+    free(delta);
 }
 
 /* Tait equation of state + Newtonian viscosity (textbook weakly

@@ -129,6 +129,19 @@ int orc_run(orc_state *st, int ista, int iend);
 void orc_bc_set_val(const orc_state *st, int comp_list, int nn, double *var, double facl);
 void orc_bc_set_zero(const orc_state *st, int comp_list, int nn, double *var);
 
+/* Nodal volume of a fully filled grid, MakNodalVol (module/mesh.cpp:160-202):
+ * npxye points per cell at the GaussianDistribution offsets (:12-54), each of
+ * volume dx dy dz / (npx npy npz). Fills st->nvol (nodec). */
+void orc_nodal_vol(orc_state *st, const int32_t npxye[3]);
+
+/* Particle shifting correction, DeltaCorrectionParticleShifting
+ * (module/material_point.cpp:174-227), single rank (dbc = 1). The two global
+ * sums (an MPI_Allreduce there, so their order follows the decomposition)
+ * are taken in blocks of ORC_SUM_BLOCK consecutive entries, each block in
+ * index order, then over the blocks in order. delta: [num][3]. */
+#define ORC_SUM_BLOCK 256
+void orc_shift_delta(const orc_state *st, double *delta);
+
 /* constitutive update, semantics of ConstitutiveModel::UpdateStress */
 void orc_update_stress(int model, double stress[6], const double F[9], const double dF[9], const double *prop,
                        double jac, double jac_bar);

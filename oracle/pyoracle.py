@@ -69,6 +69,8 @@ def load(fast=False):
     lib.orc_bc_set_val.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, _dp, C.c_double]
     lib.orc_bc_set_zero.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, _dp]
     lib.orc_tait_stress.argtypes = [_dp, C.c_double, _dp, C.c_double, _dp]
+    lib.orc_nodal_vol.argtypes = [C.POINTER(OrcState), C.POINTER(C.c_int32)]
+    lib.orc_shift_delta.argtypes = [C.POINTER(OrcState), _dp]
     _fp = C.POINTER(C.c_float)
     lib.orc_mlp_forward.argtypes = [_dp, _dp, _fp, C.c_long, C.c_int, _dp]
     lib.orc_mlp_grad.argtypes = [_dp, _dp, _fp, _fp, C.c_long, C.c_int, _dp]
@@ -181,9 +183,24 @@ class Oracle:
         self.ddg = np.zeros(9 * n)
         for f in ("nmass", "nvof", "nmome", "nforce", "nvel", "naccel", "ddg"):
             setattr(st, f, P(getattr(self, f)))
+        self.nvol = np.zeros(nodec)
+        st.nvol = P(self.nvol)
         self.st = st
         self.nodec = nodec
         self.nelem = nelem
+        if getattr(scene, "shifting", False):
+            self.set_shifting(True, scene.npxye)
+
+    def set_shifting(self, on, npxye=(2, 2, 2)):
+        if on:
+            npx = (C.c_int32 * 3)(*npxye)
+            self.lib.orc_nodal_vol(C.byref(self.st), npx)
+        self.st.shifting = 1 if on else 0
+
+    def shift_delta(self):
+        d = np.zeros((self.parts.n, 3))
+        self.lib.orc_shift_delta(C.byref(self.st), P(d))
+        return d
 
     def bin(self):
         return self.lib.orc_bin(C.byref(self.st))

@@ -404,3 +404,43 @@ def test_reupload_after_steps_matches_fresh_engine(prec):
     b = engine(s, prec, relayout=rel)
     assert b.step(1, 20) == 0
     assert_parts_equal(a.download(s.n), b.download(s.n))
+
+
+# ------------------------------------------------------------------ reference particle shifting
+@pytest.mark.parametrize("idimc,npxye", [((1, 1, 1), (2, 2, 2)), ((2, 2, 2), (2, 2, 2)), ((1, 1, 1), (3, 1, 2))])
+def test_nodal_volume_bitexact(idimc, npxye):
+    s = scenes.tiny(n_cells=8, idimc=idimc)
+    s.shifting, s.npxye = True, npxye
+    print("nodal volume idimc=%s npxye=%s" % (idimc, npxye), flush=True)
+    o = pyoracle.Oracle(s)
+    e = engine(s, capi.PREC_FP64_EXACT)
+    assert np.array_equal(e.nodal_volume(), o.nvol)
+
+
+@pytest.mark.parametrize("scheme", [capi.SCHEME_PIC, capi.SCHEME_FLIP])
+@pytest.mark.parametrize("idimc", [(1, 1, 1), (2, 1, 2)])
+def test_fp64_shifting_run_bitexact(scheme, idimc):
+    s = scenes.tiny(n_cells=8, model=capi.MODEL_LINELAST, props=(1e6, 0.3), scheme=scheme, idimc=idimc)
+    # squeeze the block towards its centre so that nodes hold excess volume
+    c = s.parts.coord.mean(axis=0)
+    s.parts.coord[:] = c + 0.93 * (s.parts.coord - c)
+    s.shifting = True
+    print("shifting scheme=%d idimc=%s n=%d" % (scheme, idimc, s.n), flush=True)
+    o = oracle_run(s, 20)
+    e = engine(s, capi.PREC_FP64_EXACT)
+    assert e.step(1, 20) == 0
+    got = e.download(s.n)
+    assert_parts_equal(got, o.parts)
+    # the correction moved the particles
+    s.shifting = False
+    plain = engine(s, capi.PREC_FP64_EXACT)
+    assert plain.step(1, 20) == 0
+    assert np.max(np.abs(plain.download(s.n).coord - got.coord)) > 1e-9
+
+
+def test_fp32_rejects_shifting():
+    s = scenes.tiny(n_cells=8)
+    e = engine(s, capi.PREC_FP32)
+    with pytest.raises(capi.MPMError) as ei:
+        e.set_shifting(True)
+    assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
