@@ -147,7 +147,8 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
                                                           const uint64_t *__restrict__ bsum,
                                                           uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend,
                                                           uint32_t *__restrict__ active, uint32_t *__restrict__ astart,
-                                                          uint32_t *__restrict__ tiles, TileGeom tg) {
+                                                          uint32_t *__restrict__ tiles, TileGeom tg,
+                                                          uint32_t *__restrict__ occ) {
     const size_t tbase = (size_t)blockIdx.x * kTile;
     uint4 *c4 = (uint4 *)(cnt + tbase);
     uint4 *s4 = (uint4 *)(cstart + tbase);
@@ -178,6 +179,14 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
         s4[idx] = so;
         e4[idx] = so;
         if (any) c4[idx] = make_uint4(0u, 0u, 0u, 0u);
+        // occupancy bits: 8 consecutive lanes hold the 32 cells of one word
+        const int lane = threadIdx.x & 63;
+        uint32_t w = ((vv[0] ? 1u : 0u) | (vv[1] ? 2u : 0u) | (vv[2] ? 4u : 0u) | (vv[3] ? 8u : 0u))
+                     << (4 * (lane & 7));
+        w |= (uint32_t)__shfl_xor((int)w, 1, 64);
+        w |= (uint32_t)__shfl_xor((int)w, 2, 64);
+        w |= (uint32_t)__shfl_xor((int)w, 4, 64);
+        if ((lane & 7) == 0) occ[(tbase + 4 * (size_t)idx) >> 5] = w;
     }
 }
 
@@ -301,7 +310,7 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
     hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
     hipLaunchKernelGGL(k_bin_top, dim3(1), dim3(kBlock), 0, s, b.bsum, nb, b.ctr, b.astart, n, err);
     hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, b.cstart,
-                       b.cend, b.active, b.astart, b.tiles, tg);
+                       b.cend, b.active, b.astart, b.tiles, tg, b.occ);
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, (const uint32_t *)b.cstart,
                            b.cend, b.perm, b.stamp, b.fixlist, b.ctr);

@@ -191,7 +191,9 @@ struct mpm_engine {
     }
 
     BcDense bc() const { return BcDense{d_bcmask, d_bcval}; }
-    CellLists lists() const { return CellLists{perm, bb.cstart, bb.cend, bb.active, bb.astart, bb.ctr, bb.tiles}; }
+    CellLists lists() const {
+        return CellLists{perm, bb.cstart, bb.cend, bb.active, bb.astart, bb.ctr, bb.tiles, bb.occ};
+    }
 };
 
 namespace {
@@ -530,6 +532,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         (rc = e->dalloc(&b.stamp, nelem)) ||
         (rc = e->dalloc(&b.fixlist, nelem)) || (rc = e->dalloc(&b.tiles, (size_t)e->tg.ntiles)) ||
         (rc = e->dalloc(&b.bsum, bin_scan_blocks(e->g.nelem))) || (rc = e->dalloc(&b.ctr, 1)) ||
+        (rc = e->dalloc(&b.occ, ncap / 32)) ||
         (rc = e->dalloc(&e->d_err, 1))) {
         delete e;
         return rc;

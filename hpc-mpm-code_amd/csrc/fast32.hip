@@ -302,7 +302,7 @@ __device__ __forceinline__ bool tile_node(const GridGeom &g, const CellLists &L,
     return true;
 }
 
-__device__ __forceinline__ bool cell_nonempty(const CellLists &L, int c) { return L.cend[c] > L.cstart[c]; }
+__device__ __forceinline__ bool cell_nonempty(const CellLists &L, int c) { return (L.occ[c >> 5] >> (c & 31)) & 1u; }
 
 template <bool DIAG>
 __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, TileGeom tg,

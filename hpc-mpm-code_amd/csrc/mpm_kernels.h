@@ -47,6 +47,7 @@ struct CellLists {
     const uint32_t *astart;  // [n_active + 1]: start of each active cell, then the binned total
     const BinCounters *ctr;
     const uint32_t *tiles;
+    const uint32_t *occ;     // one bit per cell: the cell holds particles
 };
 
 struct TileGeom {
@@ -90,6 +91,7 @@ struct BinBuffers {
     uint32_t *fixlist;  // [nelem]
     uint32_t *perm;     // [n]
     uint32_t *tiles;    // [ntiles]
+    uint32_t *occ;      // [bin_cell_capacity / 32] occupancy bits
     uint64_t *bsum;     // [bin_scan_blocks(nelem)]
     BinCounters *ctr;
 };
