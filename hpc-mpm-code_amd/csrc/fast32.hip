@@ -304,6 +304,31 @@ __device__ __forceinline__ bool tile_node(const GridGeom &g, const CellLists &L,
 
 __device__ __forceinline__ bool cell_nonempty(const CellLists &L, int c) { return (L.occ[c >> 5] >> (c & 31)) & 1u; }
 
+// The <= 8 cells around node (i, j, k) in ascending cell order: cell index
+// (clamped into the grid when outside) and whether it holds particles. All
+// of a node's partial loads can then be issued at once instead of one cell's
+// loads per dependent round trip.
+struct NodeCells {
+    int c[8];
+    int corner[8];
+    bool on[8];
+};
+__device__ __forceinline__ bool node_cells8(const GridGeom &g, const CellLists &L, int i, int j, int k, NodeCells &nc) {
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int di = q & 1, dj = (q >> 1) & 1, dk = q >> 2;  // q ascending = cell index ascending
+        const int ci = i - 1 + di, cj = j - 1 + dj, ck = k - 1 + dk;
+        const bool in = ci >= 0 && ci < g.nex[0] && cj >= 0 && cj < g.nex[1] && ck >= 0 && ck < g.nex[2];
+        const int c = in ? ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1] : 0;
+        nc.c[q] = c;
+        nc.corner[q] = (1 - di) + 2 * (1 - dj) + 4 * (1 - dk);
+        nc.on[q] = in && cell_nonempty(L, c);
+        any = any || nc.on[q];
+    }
+    return any;
+}
+
 template <bool DIAG>
 __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, TileGeom tg,
                                                       const float *__restrict__ part, NodesF32 N, BcDense bc,
@@ -318,22 +343,22 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
         s[f] = 0.f;
         cp[f] = 0.f;
     }
-    bool any = false;
-    for (int ck = k - 1; ck <= k; ck++) {
-        if (ck < 0 || ck >= g.nex[2]) continue;
-        for (int cj = j - 1; cj <= j; cj++) {
-            if (cj < 0 || cj >= g.nex[1]) continue;
-            for (int ci = i - 1; ci <= i; ci++) {
-                if (ci < 0 || ci >= g.nex[0]) continue;
-                int c = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
-                if (!cell_nonempty(L, c)) continue;
-                int corner = (i - ci) + 2 * (j - cj) + 4 * (k - ck);
-                any = true;
+    NodeCells nc;
+    const bool any = node_cells8(g, L, i, j, k, nc);
+    if (any) {
+        float val[8][NQ];
 #pragma unroll
-                for (int f = 0; f < 7; f++) neu_add(s[f], cp[f], part[(size_t)(corner * 7 + f) * ne + c]);
-                if (DIAG) neu_add(s[7], cp[7], part[(size_t)(56 + corner) * ne + c]);
-            }
+        for (int q = 0; q < 8; q++) {
+#pragma unroll
+            for (int f = 0; f < 7; f++) val[q][f] = part[(size_t)(nc.corner[q] * 7 + f) * ne + nc.c[q]];
+            if (DIAG) val[q][7] = part[(size_t)(56 + nc.corner[q]) * ne + nc.c[q]];
         }
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (nc.on[q]) {
+#pragma unroll
+                for (int f = 0; f < NQ; f++) neu_add(s[f], cp[f], val[q][f]);
+            }
     }
     const int nodec = g.nodec;
     if (!any) {
@@ -539,21 +564,20 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, T
     if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
     const size_t ne = (size_t)g.nelem;
     float s[3] = {0.f, 0.f, 0.f}, cp[3] = {0.f, 0.f, 0.f};
-    bool any = false;
-    for (int ck = k - 1; ck <= k; ck++) {
-        if (ck < 0 || ck >= g.nex[2]) continue;
-        for (int cj = j - 1; cj <= j; cj++) {
-            if (cj < 0 || cj >= g.nex[1]) continue;
-            for (int ci = i - 1; ci <= i; ci++) {
-                if (ci < 0 || ci >= g.nex[0]) continue;
-                int c = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
-                if (!cell_nonempty(L, c)) continue;
-                int corner = (i - ci) + 2 * (j - cj) + 4 * (k - ck);
-                any = true;
+    NodeCells nc;
+    const bool any = node_cells8(g, L, i, j, k, nc);
+    if (any) {
+        float val[8][3];
 #pragma unroll
-                for (int f = 0; f < 3; f++) neu_add(s[f], cp[f], part[(size_t)(corner * 3 + f) * ne + c]);
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int f = 0; f < 3; f++) val[q][f] = part[(size_t)(nc.corner[q] * 3 + f) * ne + nc.c[q]];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (nc.on[q]) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) neu_add(s[f], cp[f], val[q][f]);
             }
-        }
     }
     const int nodec = g.nodec;
     if (!any) {
