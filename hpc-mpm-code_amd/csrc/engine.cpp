@@ -130,6 +130,10 @@ struct mpm_engine {
         uint32_t *log_key = nullptr;
         int64_t cap = 0, nsamples = 0, logged_steps = 0;
         unsigned long long *ctr = nullptr;  // evaluated, fallbacks, offered
+        float *queue = nullptr;             // plastic trial states of a step, [qcap][kSurQueueRow]
+        uint32_t *qcount = nullptr;         // [2]
+        int64_t qcap = 0;
+        uint32_t parity = 0;
         uint32_t thr = 0, step_tag = 0;
         bool sorted = false;
         float loss = 0.f, tol = 1e-4f;
@@ -143,6 +147,8 @@ struct mpm_engine {
         ~Surrogate() {
             free_log();
             if (ctr) (void)hipFree(ctr);
+            if (queue) (void)hipFree(queue);
+            if (qcount) (void)hipFree(qcount);
             net.release();
         }
     } sur;
@@ -345,11 +351,16 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
                 sa.step_tag = su.step_tag++;
                 su.logged_steps++;
             }
-        } else if (su.enabled && su.have_net) {
+        } else if (su.enabled && su.have_net && su.queue && su.qcap >= e->n) {
             sa.mode = 2;
             sa.tol = su.tol;
             sa.net = su.net.view();
             sa.net_scaling = su.net.scaling;
+            sa.queue = su.queue;
+            sa.qcount = su.qcount;
+            sa.qcap = (uint32_t)su.qcap;
+            sa.parity = su.parity;
+            su.parity ^= 1u;
         }
         sa.ctr = su.ctr;
         sa.pid = e->pid[e->cur];
@@ -358,6 +369,7 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         e->clear_counts();
         BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->bb.tiles, e->tg.ntiles, e->bb.ctr};
         launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa, bf);
+        if (sa.mode == 2) launch_f32_j2_surrogate(e->stream, e->P32[e->cur], e->mt, sa);
         e->counts_ready = e->fused_count;
         e->keys_valid = true;  // the update kernel writes the new keys
     }
@@ -703,6 +715,18 @@ int mpm_surrogate_enable(mpm_engine *e, int32_t on, double tol) {
     if (on) {
         e->sur.tol = (float)std::min(tol, 1e30);
         e->sur.logging = false;
+        auto &su = e->sur;
+        if (su.qcap < e->n) {
+            if (su.queue) (void)hipFree(su.queue);
+            su.queue = nullptr;
+            su.qcap = 0;
+            MPM_HIP_CHECK(hipMalloc((void **)&su.queue, (size_t)e->n * kSurQueueRow * sizeof(float)));
+            su.qcap = e->n;
+        }
+        if (!su.qcount) MPM_HIP_CHECK(hipMalloc((void **)&su.qcount, 2 * sizeof(uint32_t)));
+        MPM_HIP_CHECK(hipMemsetAsync(su.qcount, 0, 2 * sizeof(uint32_t), e->stream));
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        su.parity = 0;
     }
     return 0;
 }
@@ -865,6 +889,13 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
     e->has_epsp = true;
     int rc = alloc_particles(e, n);
     if (rc) return rc;
+    if (e->sur.enabled && e->sur.qcap < n) {  // the surrogate queue holds every particle
+        if (e->sur.queue) (void)hipFree(e->sur.queue);
+        e->sur.queue = nullptr;
+        e->sur.qcap = 0;
+        MPM_HIP_CHECK(hipMalloc((void **)&e->sur.queue, (size_t)n * kSurQueueRow * sizeof(float)));
+        e->sur.qcap = n;
+    }
     const size_t ld = e->ld;
     auto F_at = [&](int i, int q) -> double {
         if (p->def_grad) return p->def_grad[9 * (size_t)i + q];
@@ -1214,7 +1245,8 @@ int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
     const int B = e->graph_block;
     const bool ramp_over = e->prm.nlstep <= 1 || istep0 > e->prm.nlstep;
     const bool relayout_ok = !(e->prec == MPM_PREC_FP32 && e->relayout_every > 0) || B % (2 * e->relayout_every) == 0;
-    if (B > 0 && !e->timing && ramp_over && relayout_ok && !e->sur.logging && nsteps > B) {
+    const bool parity_ok = !e->sur.enabled || (B % 2) == 0;  // the surrogate queue counters alternate by step
+    if (B > 0 && !e->timing && ramp_over && relayout_ok && parity_ok && !e->sur.logging && nsteps > B) {
         // all but the last step in whole graph blocks; the rest eagerly (the
         // last step may write the diagnostic node fields)
         const int nblk = (nsteps - 1) / B;
@@ -1619,6 +1651,8 @@ int mpm_ensemble_capture(mpm_ensemble *x, int32_t istep0, int32_t block_steps) {
         if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && block_steps % (2 * e->relayout_every) != 0)
             return set_error(MPM_ERR_INVALID_ARGUMENT,
                              "block_steps must be a multiple of twice every scene's relayout interval");
+        if (e->sur.enabled && block_steps % 2 != 0)
+            return set_error(MPM_ERR_INVALID_ARGUMENT, "block_steps must be even when the surrogate is on");
     }
     int rc = ens_prepare(x);
     if (rc) return rc;

@@ -21,6 +21,8 @@
 #include "mpm_common.h"
 #include "mpm_kernels.h"
 
+#include <algorithm>
+
 namespace mpm {
 
 namespace {
@@ -788,18 +790,6 @@ __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9]
     }
 }
 
-template <bool ON>
-struct SurSmem {
-    MlpSmem w;
-    float scaling[64];
-    int wcount[4];
-    __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];  // queue of up to 256 staged rows
-};
-template <>
-struct SurSmem<false> {
-    int unused;
-};
-
 __device__ __forceinline__ uint32_t mix32(uint32_t h) {
     h ^= h >> 16;
     h *= 0x85EBCA6Bu;
@@ -851,7 +841,6 @@ template <int SUR>
 __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
                                                        MaterialTable mt, StepConsts sc, uint32_t *err,
                                                        SurrogateArgs sa, BinFuse bf) {
-    __shared__ SurSmem<SUR == 2> sm;
     const int slot0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (bf.cnt) {
         for (int i = slot0; i < bf.ntiles; i += gridDim.x * blockDim.x) bf.tiles[i] = 0u;
@@ -1005,54 +994,29 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                 }
             }
         } else {
-            // The block's plastic trial states are queued in LDS (block
-            // order) and the network runs on full waves of the queue, so a
-            // block with k plastic states makes ceil(k / 64) wave passes.
-            bool accept = false;
-            if (__syncthreads_or(plastic)) {
-                const int w = threadIdx.x >> 6;
-                mlp_load_smem(sm.w, sa.net);
-                if (threadIdx.x < 64) sm.scaling[threadIdx.x] = sa.net_scaling[threadIdx.x];
-                const unsigned long long m = __ballot(plastic);
-                if (lane == 0) sm.wcount[w] = __popcll(m);
-                __syncthreads();
-                int base = 0, k = 0;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    base += (i < w) ? sm.wcount[i] : 0;
-                    k += sm.wcount[i];
-                }
-                const int qi = base + __popcll(m & ((1ull << lane) - 1ull));
-                if (plastic) {
-                    float x[16], xn[16];
-                    j2_features(t, D, s, x);
-#pragma unroll
-                    for (int f = 0; f < 16; f++) xn[f] = (x[f] - sm.scaling[f]) * sm.scaling[16 + f];
-                    mlp_stage_row(&sm.xs[qi >> 6][(qi & 63) * kMlpIn], xn);
-                }
-                __syncthreads();
-                if (w * 64 < k) mlp_wave_run(sm.w, sm.xs[w], lane);
-                __syncthreads();
-                if (plastic) {
-                    const float *y = (const float *)sm.xs[qi >> 6] + (qi & 63) * 8;
-                    float ss[6];
-#pragma unroll
-                    for (int c = 0; c < 6; c++) ss[c] = t.st[c] - (y[c] * sm.scaling[48 + c] + sm.scaling[32 + c]);
-                    const float dg = y[6] * sm.scaling[54] + sm.scaling[38];
-                    if (j2_residual(t, ss, dg) <= sa.tol) {
-                        accept = true;
-#pragma unroll
-                        for (int c = 0; c < 6; c++) s[c] = ss[c];
-                        ep = t.ep + dg;
-                    }
-                }
-                const unsigned long long rej = __ballot(plastic && !accept);
-                if (lane == 0 && m) {
-                    atomicAdd(&sa.ctr[0], (unsigned long long)__popcll(m));
-                    if (rej) atomicAdd(&sa.ctr[1], (unsigned long long)__popcll(rej));
+            // exact update for every particle; the plastic trial states are
+            // queued for the network pass that follows this kernel
+            float x[16];
+            if (plastic) j2_features(t, D, s, x);
+            if (j2) j2_return(t, s, &ep);
+            if (blockIdx.x == 0 && threadIdx.x == 0) sa.qcount[sa.parity ^ 1u] = 0u;  // next step's counter
+            const unsigned long long m = __ballot(plastic);
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&sa.qcount[sa.parity], (uint32_t)__popcll(m));
+                base = (uint32_t)__shfl((int)base, leader, 64);
+                const uint32_t qi = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (plastic && qi < sa.qcap) {
+                    float4 *row = (float4 *)(sa.queue + (size_t)qi * kSurQueueRow);
+                    row[0] = make_float4(__int_as_float(slot), x[0], x[1], x[2]);
+                    row[1] = make_float4(x[3], x[4], x[5], x[6]);
+                    row[2] = make_float4(x[7], x[8], x[9], x[10]);
+                    row[3] = make_float4(x[11], x[12], t.st[0], t.st[1]);
+                    row[4] = make_float4(t.st[2], t.st[3], t.st[4], t.st[5]);
+                    row[5] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
             }
-            if (j2 && !accept) j2_return(t, s, &ep);
         }
     }
     const uint32_t nk = inside ? (uint32_t)(ci[0] + ci[1] * g.nex[0] + ci[2] * nexy) : (uint32_t)g.nelem;
@@ -1078,6 +1042,87 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     if (!(J > 0.0)) e |= kErrJacobian;
     if (!isfinite(r[0] + r[1] + r[2] + s[0] + s[1] + s[2])) e |= kErrNonFinite;
     if (e) atomicOr(err, e);
+}
+
+// Network pass over the queued plastic J2 trial states of a step: 64 queued
+// states per wave on the BF16 MFMA, accepted where the return-mapping
+// residual is within the tolerance (the exact result written by the update
+// kernel stays otherwise).
+__global__ void __launch_bounds__(256) k32_j2_surrogate(PartF32 P, MaterialTable mt, SurrogateArgs sa) {
+    __shared__ MlpSmem sw;
+    __shared__ float scl[64];
+    __shared__ __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];
+    const uint32_t nq = min(sa.qcount[sa.parity], sa.qcap);
+    if ((uint32_t)blockIdx.x * 256u >= nq) return;  // whole block idle
+    mlp_load_smem(sw, sa.net);
+    if (threadIdx.x < 64) scl[threadIdx.x] = sa.net_scaling[threadIdx.x];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t q0 = ((uint32_t)blockIdx.x * 4u + (uint32_t)w) * 64u; q0 < nq; q0 += gridDim.x * 256u) {
+        const uint32_t qi = q0 + (uint32_t)lane;
+        const bool live = qi < nq;
+        float r[kSurQueueRow];
+        if (live) {
+            const float4 *row = (const float4 *)(sa.queue + (size_t)qi * kSurQueueRow);
+#pragma unroll
+            for (int c = 0; c < kSurQueueRow / 4; c++) {
+                const float4 v = row[c];
+                r[4 * c] = v.x;
+                r[4 * c + 1] = v.y;
+                r[4 * c + 2] = v.z;
+                r[4 * c + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < kSurQueueRow; c++) r[c] = 0.f;
+        }
+        float xn[16];
+#pragma unroll
+        for (int f = 0; f < 16; f++) xn[f] = (f < kMlpInUsed) ? (r[1 + f] - scl[f]) * scl[16 + f] : 0.f;
+        float y[kMlpOutUsed];
+        mlp_wave_infer(sw, xn, y, xs[w], lane);
+        bool accept = false;
+        if (live) {
+            const int slot = __float_as_int(r[0]);
+            const int mat = P.matid ? P.matid[slot] : 0;
+            const MaterialDev &md = mt.m[mat];
+            // trial state from the queued trial stress (same arithmetic as j2_trial)
+            J2Trial t;
+            const float E = (float)md.prop[0], nu = (float)md.prop[1];
+            t.E = E;
+            t.nu = nu;
+            t.sy = (float)md.prop[2];
+            t.Hh = (float)md.prop[3];
+            t.ep = r[13];
+            t.lmd = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
+            t.mu = E / (2.f * (1.f + nu));
+#pragma unroll
+            for (int c = 0; c < 6; c++) t.st[c] = r[14 + c];
+            t.pm = (t.st[0] + t.st[1] + t.st[2]) * (1.f / 3.f);
+            t.d0 = t.st[0] - t.pm;
+            t.d1 = t.st[1] - t.pm;
+            t.d2 = t.st[2] - t.pm;
+            const float j2 = 0.5f * (t.d0 * t.d0 + t.d1 * t.d1 + t.d2 * t.d2) + t.st[3] * t.st[3] +
+                             t.st[4] * t.st[4] + t.st[5] * t.st[5];
+            t.q = sqrtf(3.f * j2);
+            float ss[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) ss[c] = t.st[c] - (y[c] * scl[48 + c] + scl[32 + c]);
+            const float dg = y[6] * scl[54] + scl[38];
+            if (j2_residual(t, ss, dg) <= sa.tol) {
+                accept = true;
+                const size_t ld = P.ld;
+#pragma unroll
+                for (int c = 0; c < 6; c++) P.sig[c * ld + slot] = ss[c];
+                if (P.eps_p) P.eps_p[slot] = t.ep + dg;
+            }
+        }
+        const unsigned long long ev = __ballot(live), rej = __ballot(live && !accept);
+        if (lane == 0) {
+            atomicAdd(&sa.ctr[0], (unsigned long long)__popcll(ev));
+            if (rej) atomicAdd(&sa.ctr[1], (unsigned long long)__popcll(rej));
+        }
+    }
 }
 
 __global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict__ perm, const int *__restrict__ pid_src,
@@ -1172,6 +1217,14 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
     else
         hipLaunchKernelGGL(k32_musl_nodes<false>, dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials,
                            N, bc, c);
+}
+
+void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const MaterialTable &mt, const SurrogateArgs &sa) {
+    if (P.n <= 0) return;
+    // grid over the worst case (every particle plastic), blocks beyond the
+    // queue length leave at once
+    const int blocks = std::min(cdiv(P.n, 256), 2048);
+    hipLaunchKernelGGL(k32_j2_surrogate, dim3(blocks), dim3(256), 0, s, P, mt, sa);
 }
 
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,

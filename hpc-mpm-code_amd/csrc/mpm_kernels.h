@@ -226,13 +226,19 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
                              const StepConsts &c, float *partials, int diag, hipEvent_t mid = nullptr);
 // dF, H, J, V, x, stress and the new cell key of every slot.
 // J2 constitutive surrogate in the FP32 update: mode 0 = exact update only,
-// 1 = exact update and sample logging, 2 = network for plastic trial states
-// with residual check and exact fallback.
+// 1 = exact update and sample logging, 2 = exact update and a queue of the
+// plastic trial states, which launch_f32_j2_surrogate then runs through the
+// network (residual check, exact result kept on rejection).
+constexpr int kSurQueueRow = 24;  // floats per queued state: slot, 13 features + pad (16), trial stress (6), pad
 struct SurrogateArgs {
     int mode;
     float tol;                     // mode 2: accepted residual / yield stress
     MlpParams net;                 // mode 2
     const float *net_scaling;      // mode 2: [in_mu | in_isd | out_mu | out_sd], 64 floats
+    float *queue;                  // mode 2: [qcap][kSurQueueRow]
+    uint32_t *qcount;              // mode 2: two counters, used by alternate steps
+    uint32_t parity;               // mode 2: this step's counter
+    uint32_t qcap;                 // mode 2: queue rows
     unsigned long long *ctr;       // [0] evaluated, [1] fallbacks, [2] samples offered to the log
     float *log_in, *log_tgt;       // mode 1: [cap][16]
     uint32_t *log_key;             // mode 1: step_tag * n + particle id
@@ -245,6 +251,8 @@ struct SurrogateArgs {
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                            const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
                            const SurrogateArgs &sa, const BinFuse &bf);
+// network pass over the queued plastic states of this step (mode 2)
+void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const MaterialTable &mt, const SurrogateArgs &sa);
 void launch_f32_permute(hipStream_t s, const PartF32 &src, const PartF32 &dst, const uint32_t *perm,
                         const int *pid_src, int *pid_dst);
 void launch_iota_u32(hipStream_t s, uint32_t *p, int n);
