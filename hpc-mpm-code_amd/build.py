@@ -26,6 +26,7 @@ UNITS = [
     ("mlp.hip", []),
     ("surrogate.cpp", ["-x", "hip"]),
     ("engine.cpp", ["-x", "hip"]),
+    ("case_io.cpp", ["-x", "c++"]),
 ]
 
 

@@ -1,0 +1,335 @@
+// Reference case files (input.txt, griddata.txt, spdata.txt) for one rank:
+// readers in the layouts the reference's own readers expect, writers in the
+// layouts its generator writes (include/mpm/mpm_case.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm/mpm_capi.h"
+#include "../../include/mpm/mpm_case.h"
+
+namespace mpm {
+int set_error_msg(int code, const char *msg);
+}
+
+struct mpm_case {
+    mpm_case_desc d{};
+    std::vector<mpm_material_desc> mats;
+    std::vector<double> coord, mass, vol0;
+    std::vector<int32_t> id, matid;
+    std::vector<int32_t> nbc[3];
+    std::vector<double> fbc[3];
+    std::vector<int32_t> node_count;  // nelem, node, nodec as written in griddata.txt
+};
+
+namespace {
+
+// Line-oriented reader with the reference's conventions: a header line is
+// skipped whole, values are read with >>, the rest of a value line is skipped.
+struct Reader {
+    std::ifstream in;
+    std::string name;
+    bool ok = true;
+    explicit Reader(const char *path) : in(path), name(path) { ok = in.is_open(); }
+    void skip_line() { in.ignore(1 << 20, '\n'); }
+    template <typename T>
+    void get(T &v) {
+        if (!(in >> v)) ok = false;
+    }
+};
+
+int fail(const std::string &what) { return mpm::set_error_msg(MPM_ERR_INVALID_ARGUMENT, what.c_str()); }
+
+int read_input(mpm_case *c, const char *path) {
+    Reader r(path);
+    if (!r.ok) return fail(std::string("cannot open ") + path);
+    mpm_case_desc &d = c->d;
+    std::string sol;
+    int nr_flag = 0;
+    double gamma_nb = 0, beta_nb = 0;
+    int npropmax = 0;
+    r.skip_line();
+    r.get(sol);
+    r.get(nr_flag);
+    r.get(d.rstflag);
+    r.get(d.step.nlstep);
+    r.skip_line();
+    r.skip_line();
+    r.get(d.ista);
+    r.get(d.iend);
+    r.get(d.iout);
+    r.skip_line();
+    r.skip_line();
+    r.get(d.step.dt);
+    r.get(d.step.mtol);
+    r.get(gamma_nb);
+    r.get(beta_nb);
+    r.skip_line();
+    r.skip_line();
+    for (int i = 0; i < 3; i++) r.get(d.grid.xmin[i]);
+    r.skip_line();
+    r.skip_line();
+    for (int i = 0; i < 3; i++) r.get(d.grid.xmax[i]);
+    r.skip_line();
+    r.skip_line();
+    for (int i = 0; i < 3; i++) r.get(d.grid.nelem[i]);
+    r.skip_line();
+    r.skip_line();
+    for (int i = 0; i < 3; i++) r.get(d.grid.idimc[i]);
+    r.skip_line();
+    r.skip_line();
+    for (int i = 0; i < 3; i++) r.get(d.npxye[i]);
+    r.skip_line();
+    r.skip_line();
+    r.get(d.rho);
+    r.get(d.nmat);
+    r.get(npropmax);
+    r.skip_line();
+    if (!r.ok || d.nmat < 1 || d.nmat > 1024 || npropmax < 0 || npropmax > 8)
+        return fail(std::string("malformed parameters in ") + path);
+    c->mats.assign(d.nmat, mpm_material_desc{});
+    for (int m = 0; m < d.nmat; m++) {
+        int iprop = 0, nprop = 0;
+        r.skip_line();
+        r.get(iprop);
+        r.get(nprop);
+        r.skip_line();
+        r.skip_line();
+        if (nprop < 0 || nprop > 8) return fail(std::string("bad nprop in ") + path);
+        for (int i = 0; i < nprop; i++) r.get(c->mats[m].prop[i]);
+        r.skip_line();
+        c->mats[m].model = iprop;
+        c->mats[m].nprop = nprop;
+    }
+    r.skip_line();
+    for (int i = 0; i < 3; i++) r.get(d.step.bb[i]);
+    if (!r.ok) return fail(std::string("malformed material or gravity block in ") + path);
+    if (sol == "PIC")
+        d.step.scheme = MPM_SCHEME_PIC;
+    else if (sol == "FLIP")
+        d.step.scheme = MPM_SCHEME_FLIP;
+    else
+        return mpm::set_error_msg(MPM_ERR_UNSUPPORTED, ("transfer scheme " + sol + " is not supported").c_str());
+    return 0;
+}
+
+int read_griddata(mpm_case *c, const char *path) {
+    Reader r(path);
+    if (!r.ok) return fail(std::string("cannot open ") + path);
+    int idimc[3], nelem, node, nodec, xyelem[3], xynode[3], xynodec[3];
+    double xmin[3], xmax[3];
+    for (int i = 0; i < 3; i++) r.get(idimc[i]);
+    r.skip_line();
+    for (int i = 0; i < 3; i++) r.get(xmin[i]);
+    r.skip_line();
+    for (int i = 0; i < 3; i++) r.get(xmax[i]);
+    r.skip_line();
+    r.get(nelem);
+    for (int i = 0; i < 3; i++) r.get(xyelem[i]);
+    r.skip_line();
+    r.get(node);
+    for (int i = 0; i < 3; i++) r.get(xynode[i]);
+    r.skip_line();
+    r.get(nodec);
+    for (int i = 0; i < 3; i++) r.get(xynodec[i]);
+    r.skip_line();
+    if (!r.ok) return fail(std::string("malformed mesh header in ") + path);
+    for (int i = 0; i < 3; i++) {
+        c->d.grid.idimc[i] = idimc[i];
+        c->d.grid.xmin[i] = xmin[i];
+        c->d.grid.xmax[i] = xmax[i];
+        c->d.grid.nelem[i] = xyelem[i];
+        if (xynode[i] != xyelem[i] + 1 || xynodec[i] != xyelem[i] + idimc[i])
+            return fail(std::string("inconsistent node counts in ") + path);
+    }
+    c->node_count = {nelem, node, nodec};
+    for (int comp = 0; comp < 3; comp++) {  // BCInput: ibc, then ibc lines "nbc fbc"
+        int ibc = 0;
+        r.get(ibc);
+        r.skip_line();
+        if (!r.ok || ibc < 0 || ibc > nodec) return fail(std::string("malformed BC list in ") + path);
+        c->nbc[comp].resize(ibc);
+        c->fbc[comp].resize(ibc);
+        for (int i = 0; i < ibc; i++) {
+            r.get(c->nbc[comp][i]);
+            r.get(c->fbc[comp][i]);
+            r.skip_line();
+        }
+        if (!r.ok) return fail(std::string("malformed BC entry in ") + path);
+        c->d.ibc[comp] = ibc;
+    }
+    return 0;
+}
+
+int read_spdata(mpm_case *c, const char *path) {
+    Reader r(path);
+    if (!r.ok) return fail(std::string("cannot open ") + path);
+    long long num = 0;
+    r.get(num);
+    r.skip_line();
+    if (!r.ok || num < 0 || num >= (1ll << 31)) return fail(std::string("bad point count in ") + path);
+    c->coord.resize(3 * num);
+    for (long long i = 0; i < num; i++) {
+        for (int a = 0; a < 3; a++) r.get(c->coord[3 * i + a]);
+        r.skip_line();
+    }
+    c->id.resize(num);
+    c->matid.resize(num);
+    c->mass.resize(num);
+    c->vol0.resize(num);
+    for (long long i = 0; i < num; i++) {
+        r.get(c->id[i]);
+        r.get(c->matid[i]);
+        r.get(c->mass[i]);
+        r.get(c->vol0[i]);
+        r.skip_line();
+    }
+    if (!r.ok) return fail(std::string("malformed point data in ") + path);
+    c->d.n = num;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_case_read(const char *input_txt, const char *griddata_txt, const char *spdata_txt, mpm_case **out) {
+    if (!out) return fail("null output");
+    *out = nullptr;
+    auto *c = new mpm_case();
+    c->d.step.mtol = 1e-12;
+    c->d.step.nlstep = 1;
+    int rc = 0;
+    if (!rc && input_txt) rc = read_input(c, input_txt);
+    if (!rc && griddata_txt) rc = read_griddata(c, griddata_txt);
+    if (!rc && spdata_txt) rc = read_spdata(c, spdata_txt);
+    if (!rc && !c->mats.empty())
+        for (int32_t m : c->matid)
+            if (m < 0 || m >= (int)c->mats.size()) {
+                rc = fail("point material id out of range");
+                break;
+            }
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+int mpm_case_describe(const mpm_case *c, mpm_case_desc *out) {
+    if (!c || !out) return fail("null argument");
+    *out = c->d;
+    return 0;
+}
+
+int mpm_case_materials(const mpm_case *c, mpm_material_desc *mats, int32_t max) {
+    if (!c || (max > 0 && !mats)) return fail("null argument");
+    for (int i = 0; i < (int)c->mats.size() && i < max; i++) mats[i] = c->mats[i];
+    return 0;
+}
+
+int mpm_case_particles(const mpm_case *c, mpm_particles_view *v) {
+    if (!c || !v) return fail("null argument");
+    std::memset(v, 0, sizeof *v);
+    v->n = c->d.n;
+    auto *mc = const_cast<mpm_case *>(c);
+    v->coord = mc->coord.data();
+    v->mass = mc->mass.data();
+    v->vol0 = mc->vol0.data();
+    v->matid = mc->matid.data();
+    return 0;
+}
+
+int mpm_case_bc(const mpm_case *c, int32_t comp, const int32_t **nbc, const double **fbc, int32_t *ibc) {
+    if (!c || comp < 0 || comp > 2 || !nbc || !fbc || !ibc) return fail("bad arguments");
+    *nbc = c->nbc[comp].data();
+    *fbc = c->fbc[comp].data();
+    *ibc = (int32_t)c->nbc[comp].size();
+    return 0;
+}
+
+int mpm_case_write(const mpm_case *c, const char *griddata_txt, const char *spdata_txt) {
+    if (!c) return fail("null case");
+    const mpm_case_desc &d = c->d;
+    if (griddata_txt) {
+        std::ofstream o(griddata_txt);
+        if (!o.is_open()) return fail(std::string("cannot write ") + griddata_txt);
+        o.flags(std::ios::right | std::ios::scientific);
+        int nel = 1, nod = 1, noc = 1;
+        for (int i = 0; i < 3; i++) {
+            nel *= d.grid.nelem[i];
+            nod *= d.grid.nelem[i] + 1;
+            noc *= d.grid.nelem[i] + d.grid.idimc[i];
+        }
+        for (int i = 0; i < 3; i++) o << std::setw(10) << d.grid.idimc[i];
+        o << "\n";
+        for (int i = 0; i < 3; i++) o << std::setw(15) << d.grid.xmin[i];
+        o << "\n";
+        for (int i = 0; i < 3; i++) o << std::setw(15) << d.grid.xmax[i];
+        o << "\n";
+        o << std::setw(10) << nel;
+        for (int i = 0; i < 3; i++) o << std::setw(10) << d.grid.nelem[i];
+        o << "\n";
+        o << std::setw(10) << nod;
+        for (int i = 0; i < 3; i++) o << std::setw(10) << d.grid.nelem[i] + 1;
+        o << "\n";
+        o << std::setw(10) << noc;
+        for (int i = 0; i < 3; i++) o << std::setw(10) << d.grid.nelem[i] + d.grid.idimc[i];
+        o << "\n";
+        const char *names[3] = {"usbc", "vsbc", "wsbc"};
+        for (int comp = 0; comp < 3; comp++) {
+            o << std::setw(15) << c->nbc[comp].size() << std::setw(15) << "! ---- " << names[comp] << ".ibc ----"
+              << "\n";
+            for (size_t i = 0; i < c->nbc[comp].size(); i++)
+                o << std::setw(15) << c->nbc[comp][i] << std::setw(15) << c->fbc[comp][i] << "\n";
+        }
+        if (!o.good()) return fail(std::string("write failed: ") + griddata_txt);
+    }
+    if (spdata_txt) {
+        std::ofstream o(spdata_txt);
+        if (!o.is_open()) return fail(std::string("cannot write ") + spdata_txt);
+        o.flags(std::ios::right | std::ios::scientific);
+        o << std::setw(10) << d.n << "\n";
+        for (int64_t i = 0; i < d.n; i++) {
+            for (int a = 0; a < 3; a++) o << std::setw(15) << c->coord[3 * i + a];
+            o << "\n";
+        }
+        for (int64_t i = 0; i < d.n; i++)
+            o << std::setw(10) << c->id[i] << std::setw(10) << c->matid[i] << std::setw(15) << c->mass[i]
+              << std::setw(15) << c->vol0[i] << "\n";
+        if (!o.good()) return fail(std::string("write failed: ") + spdata_txt);
+    }
+    return 0;
+}
+
+int mpm_case_engine(const mpm_case *c, int32_t precision, int32_t relayout_every, int32_t shifting,
+                    mpm_engine **out) {
+    if (!c || !out) return fail("null argument");
+    if (c->mats.empty() || c->d.n <= 0) return fail("case has no materials or no points");
+    mpm_step_params p = c->d.step;
+    p.precision = precision;
+    p.relayout_every = relayout_every;
+    int rc = mpm_engine_create(&c->d.grid, c->mats.data(), (int32_t)c->mats.size(), &p, out);
+    if (rc) return rc;
+    for (int comp = 0; comp < 3 && !rc; comp++)
+        rc = mpm_set_bc(*out, comp, c->nbc[comp].data(), c->fbc[comp].data(), (int32_t)c->nbc[comp].size());
+    mpm_particles_view v;
+    if (!rc) rc = mpm_case_particles(c, &v);
+    if (!rc) rc = mpm_upload_particles(*out, &v);
+    if (!rc && shifting) rc = mpm_set_shifting(*out, 1, c->d.npxye);
+    if (rc) {
+        mpm_engine_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+void mpm_case_free(mpm_case *c) { delete c; }
+
+}  // extern "C"

@@ -86,6 +86,12 @@ class TuneResult(C.Structure):
                 ("scenes", C.c_int32), ("ms_per_step", C.c_double), ("ms_per_step_worst", C.c_double)]
 
 
+class CaseDesc(C.Structure):
+    _fields_ = [("grid", GridDesc), ("step", StepParams), ("ista", C.c_int32), ("iend", C.c_int32),
+                ("iout", C.c_int32), ("rstflag", C.c_int32), ("npxye", C.c_int32 * 3), ("nmat", C.c_int32),
+                ("rho", C.c_double), ("n", C.c_int64), ("ibc", C.c_int32 * 3), ("pad", C.c_int32)]
+
+
 _fp = C.POINTER(C.c_float)
 
 SIGNATURES = {
@@ -133,6 +139,15 @@ SIGNATURES = {
     "mpm_ensemble_launch": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mpm_ensemble_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneResult), C.c_int32,
                                         C.POINTER(C.c_int32)]),
+    # include/mpm/mpm_case.h
+    "mpm_case_read": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mpm_case_describe": (C.c_int, [C.c_void_p, C.POINTER(CaseDesc)]),
+    "mpm_case_materials": (C.c_int, [C.c_void_p, C.POINTER(MaterialDesc), C.c_int32]),
+    "mpm_case_particles": (C.c_int, [C.c_void_p, C.POINTER(ParticlesView)]),
+    "mpm_case_bc": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(_ip), C.POINTER(_dp), C.POINTER(C.c_int32)]),
+    "mpm_case_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "mpm_case_engine": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "mpm_case_free": (None, [C.c_void_p]),
 }
 
 MLP_NPARAMS = 6288
@@ -446,6 +461,86 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+def _path(p):
+    return None if p is None else os.fsencode(p)
+
+
+class Case:
+    """A reference solid case read from the generator's files
+    (include/mpm/mpm_case.h): input.txt parameter block, griddata.txt,
+    spdata.txt. Any of the three may be None."""
+
+    def __init__(self, input_txt=None, griddata_txt=None, spdata_txt=None):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.mpm_case_read(_path(input_txt), _path(griddata_txt), _path(spdata_txt), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mpm_case_free(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def desc(self):
+        d = CaseDesc()
+        check(self.lib.mpm_case_describe(self.h, C.byref(d)))
+        return d
+
+    def materials(self):
+        d = self.desc()
+        m = (MaterialDesc * max(d.nmat, 1))()
+        check(self.lib.mpm_case_materials(self.h, m, d.nmat))
+        return [(m[i].model, [m[i].prop[k] for k in range(m[i].nprop)]) for i in range(d.nmat)]
+
+    def particles(self):
+        v = ParticlesView()
+        check(self.lib.mpm_case_particles(self.h, C.byref(v)))
+        n = int(v.n)
+        arr = lambda ptr, shape: np.ctypeslib.as_array(ptr, shape=shape).copy() if n else np.zeros(shape)
+        matid = np.ctypeslib.as_array(v.matid, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+        return Particles(n, coord=arr(v.coord, (n, 3)), mass=arr(v.mass, (n,)), vol0=arr(v.vol0, (n,)),
+                         matid=matid)
+
+    def bcs(self):
+        out = {}
+        for comp in range(3):
+            nbc, fbc, ibc = _ip(), _dp(), C.c_int32()
+            check(self.lib.mpm_case_bc(self.h, comp, C.byref(nbc), C.byref(fbc), C.byref(ibc)))
+            k = ibc.value
+            out[comp] = ((np.ctypeslib.as_array(nbc, shape=(k,)).copy(), np.ctypeslib.as_array(fbc, shape=(k,)).copy())
+                         if k else (np.zeros(0, np.int32), np.zeros(0)))
+        return out
+
+    def write(self, griddata_txt=None, spdata_txt=None):
+        check(self.lib.mpm_case_write(self.h, _path(griddata_txt), _path(spdata_txt)))
+
+    def engine(self, precision=PREC_FP64_EXACT, relayout_every=0, shifting=False):
+        """Engine built by mpm_case_engine (grid, materials, BCs, points
+        uploaded; reference particle shifting on when asked)."""
+        h = C.c_void_p()
+        check(self.lib.mpm_case_engine(self.h, precision, relayout_every, int(shifting), C.byref(h)))
+        d = self.desc()
+        e = Engine.__new__(Engine)
+        e.lib, e.h, e.scene = self.lib, h, None
+        e.nodec = int(np.prod([d.grid.nelem[a] + d.grid.idimc[a] for a in range(3)]))
+        e.nelem = int(np.prod([d.grid.nelem[a] for a in range(3)]))
+        return e
+
+    def scene(self, name="case"):
+        """The case as a scene description (the form the oracle and the
+        Python-side engine constructor take)."""
+        from . import scenes
+        d = self.desc()
+        g = d.grid
+        s = scenes.Scene(name, tuple(g.xmin), tuple(g.xmax), tuple(g.nelem), self.materials(), d.rho, d.step.dt,
+                         max(d.iend - d.ista + 1, 0), tuple(d.step.bb), mtol=d.step.mtol, scheme=d.step.scheme,
+                         nlstep=d.step.nlstep, bcs=self.bcs(), parts=self.particles(), idimc=tuple(g.idimc),
+                         npxye=tuple(d.npxye))
+        return s
 
 
 def mlp_forward(params, scaling, inp):

@@ -1,0 +1,118 @@
+"""Reference case files through the C ABI (include/mpm/mpm_case.h): host-only
+reading and writing, no GPU call.
+
+The beam block of the reference generator's input.txt (tests/golden/
+beam_input.txt) is turned into griddata.txt / spdata.txt by an independent
+Python restatement of the generator (tests/golden/beam_case.py). The library
+must read back what the generator's formats hold and write byte-identical
+files."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from mpm_mi355x import capi
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+import beam_case  # noqa: E402
+
+pytestmark = pytest.mark.skipif(not os.path.exists(capi.LIB_PATH), reason="engine library not built")
+
+
+@pytest.fixture()
+def beam(tmp_path):
+    c, b = beam_case.make(str(tmp_path))
+    case = capi.Case(beam_case.BEAM_INPUT, str(tmp_path / "griddata.txt"), str(tmp_path / "spdata.txt"))
+    return case, c, b, tmp_path
+
+
+def test_beam_parameters_read_as_written(beam):
+    case, c, b, _ = beam
+    d = case.desc()
+    assert d.step.scheme == capi.SCHEME_FLIP
+    assert (d.rstflag, d.step.nlstep, d.ista, d.iend, d.iout) == (2, 1, 1, 2, 1)
+    assert (d.step.dt, d.step.mtol) == (1.0e-3, 1.0e-12)
+    assert tuple(d.grid.xmin) == (0.0, 0.0, 0.0) and tuple(d.grid.xmax) == (5.0, 0.5, 5.0)
+    assert tuple(d.grid.nelem) == (10, 1, 10) and tuple(d.grid.idimc) == (2, 1, 2)
+    assert tuple(d.npxye) == (4, 1, 4) and d.rho == 2500.0 and d.nmat == 1
+    assert tuple(d.step.bb) == (0.0, 0.0, -9.81)
+    assert case.materials() == [(capi.MODEL_NEOHOOKEAN, [1.0e7, 0.3])]
+
+
+def test_beam_points_and_bcs_closed_form(beam):
+    case, c, b, _ = beam
+    d = case.desc()
+    # Cells i = 0..7 (x <= 4) and k = 7, 8 (z in [3.5, 4.5]), 16 points each.
+    assert d.n == 8 * 2 * 16
+    # nodec = 12 x 2 x 12 = 288; u and w fix the x and z boundary planes
+    # (288 - 10*2*10 interior = 88), v fixes k == 0 and both y planes (all).
+    assert tuple(d.ibc) == (88, 288, 88)
+    p = case.particles()
+    volp = 0.5 * 0.5 * 0.5 / 16
+    assert np.all(p.vol0 == float("%.6e" % volp)) and np.all(p.mass == float("%.6e" % (2500.0 * volp)))
+    assert np.all(p.matid == 0)
+    assert p.coord[:, 0].min() > 0 and p.coord[:, 0].max() < 4.0
+    assert p.coord[:, 2].min() > 3.5 and p.coord[:, 2].max() < 4.5
+    # the file holds 7 significant digits; the reader returns exactly those
+    assert np.array_equal(p.coord, np.array([[float("%.6e" % v) for v in r] for r in b["coord"]]))
+    bc = case.bcs()
+    for comp in range(3):
+        assert np.array_equal(bc[comp][0], np.array(b["bc"][comp]))
+        assert np.all(bc[comp][1] == 0.0)
+
+
+def test_writer_is_byte_identical_to_generator_format(beam):
+    case, c, b, tmp = beam
+    out = tmp / "out"
+    out.mkdir()
+    case.write(str(out / "griddata.txt"), str(out / "spdata.txt"))
+    for f in ("griddata.txt", "spdata.txt"):
+        assert (out / f).read_bytes() == (tmp / f).read_bytes(), f
+
+
+def test_round_trip_through_written_files(beam):
+    case, c, b, tmp = beam
+    out = tmp / "rt"
+    out.mkdir()
+    case.write(str(out / "griddata.txt"), str(out / "spdata.txt"))
+    again = capi.Case(beam_case.BEAM_INPUT, str(out / "griddata.txt"), str(out / "spdata.txt"))
+    p, q = case.particles(), again.particles()
+    assert np.array_equal(p.coord, q.coord) and np.array_equal(p.mass, q.mass)
+    assert tuple(case.desc().ibc) == tuple(again.desc().ibc)
+
+
+def test_scene_view_matches_case(beam):
+    case, c, b, _ = beam
+    s = case.scene()
+    assert s.idimc == (2, 1, 2) and s.npxye == (4, 1, 4) and s.nelem == (10, 1, 10)
+    assert s.parts.n == 256 and s.scheme == capi.SCHEME_FLIP
+    assert s.materials == [(capi.MODEL_NEOHOOKEAN, [1.0e7, 0.3])]
+
+
+def test_missing_and_malformed_files_are_errors(tmp_path):
+    with pytest.raises(capi.MPMError) as e:
+        capi.Case(str(tmp_path / "nope.txt"))
+    assert e.value.code == capi.MPM_ERR_INVALID_ARGUMENT and "cannot open" in str(e.value)
+    bad = tmp_path / "spdata.txt"
+    bad.write_text("         3\n 1.0 2.0 3.0\n")
+    with pytest.raises(capi.MPMError):
+        capi.Case(None, None, str(bad))
+    gd = tmp_path / "griddata.txt"
+    gd.write_text("2 1 2\n0 0 0\n1 1 1\n1 1 1 1\n8 2 2 2\n5 3 2 3\n")  # nodec 3x2x3 inconsistent with idimc
+    with pytest.raises(capi.MPMError):
+        capi.Case(None, str(gd), None)
+
+
+def test_unsupported_transfer_scheme_and_bad_matid(tmp_path):
+    src = open(beam_case.BEAM_INPUT).read().splitlines()
+    src[1] = "APIC  0  2  1"
+    f = tmp_path / "input.txt"
+    f.write_text("\n".join(src) + "\n")
+    with pytest.raises(capi.MPMError) as e:
+        capi.Case(str(f))
+    assert e.value.code == capi.MPM_ERR_UNSUPPORTED
+    sp = tmp_path / "spdata.txt"
+    sp.write_text("%10d\n%15.6e%15.6e%15.6e\n%10d%10d%15.6e%15.6e\n" % (1, 1.0, 0.25, 4.0, 0, 3, 1.0, 1.0))
+    with pytest.raises(capi.MPMError):
+        capi.Case(beam_case.BEAM_INPUT, None, str(sp))

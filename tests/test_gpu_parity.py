@@ -444,3 +444,46 @@ def test_fp32_rejects_shifting():
     with pytest.raises(capi.MPMError) as ei:
         e.set_shifting(True)
     assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
+
+
+# ------------------------------------------------------------------ reference case files
+def _beam_case(tmp_path, iprop=None):
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import beam_case
+    beam_case.make(str(tmp_path))
+    inp = beam_case.BEAM_INPUT
+    if iprop is not None:
+        lines = open(inp).read().splitlines()
+        lines[19] = "%d  2" % iprop
+        inp = str(tmp_path / "input.txt")
+        open(inp, "w").write("\n".join(lines) + "\n")
+    return capi.Case(inp, str(tmp_path / "griddata.txt"), str(tmp_path / "spdata.txt"))
+
+
+@pytest.mark.parametrize("iprop", [capi.MODEL_LINELAST, capi.MODEL_NEOHOOKEAN])
+def test_reference_beam_case_with_shifting(tmp_path, iprop):
+    """The generator's beam (quadratic 2,1,2 stencil, 4x1x4 points per cell,
+    FLIP, gravity, particle shifting) read from its own file formats and run
+    200 steps: bit-exact with the linear elastic model, Neo-Hookean within
+    the device log() tolerance."""
+    case = _beam_case(tmp_path, iprop)
+    s = case.scene()
+    s.shifting = True
+    print("beam case iprop=%d n=%d" % (iprop, s.n), flush=True)
+    o = oracle_run(s, 200)
+    e = case.engine(capi.PREC_FP64_EXACT, shifting=True)
+    assert e.step(1, 200) == 0
+    got = e.download(s.n)
+    if iprop == capi.MODEL_LINELAST:
+        assert_parts_equal(got, o.parts)
+    else:
+        np.testing.assert_allclose(got.coord, o.parts.coord, rtol=0, atol=1e-13)
+        np.testing.assert_allclose(got.stress, o.parts.stress, rtol=1e-9, atol=1e-9 * np.abs(o.parts.stress).max())
+    # the beam sagged under gravity
+    assert got.coord[:, 2].min() < s.parts.coord[:, 2].min() - 1e-6
+    # the case-built engine is the scene-built one
+    e2 = engine(s, capi.PREC_FP64_EXACT)
+    assert e2.step(1, 200) == 0
+    assert_parts_equal(e2.download(s.n), got)
