@@ -109,12 +109,17 @@ int read_input(mpm_case *c, const char *path) {
     r.skip_line();
     for (int i = 0; i < 3; i++) r.get(d.step.bb[i]);
     if (!r.ok) return fail(std::string("malformed material or gravity block in ") + path);
+    // ParseMapScheme (module/dataset.h:21-27)
     if (sol == "PIC")
         d.step.scheme = MPM_SCHEME_PIC;
     else if (sol == "FLIP")
         d.step.scheme = MPM_SCHEME_FLIP;
+    else if (sol == "TPIC")
+        d.step.scheme = MPM_SCHEME_TPIC;
+    else if (sol == "APIC")
+        d.step.scheme = MPM_SCHEME_APIC;
     else
-        return mpm::set_error_msg(MPM_ERR_UNSUPPORTED, ("transfer scheme " + sol + " is not supported").c_str());
+        return fail("unknown mapping scheme '" + sol + "' in " + path);
     return 0;
 }
 

@@ -296,7 +296,7 @@ int do_bin(mpm_engine *e) {
 
 int do_p2g(mpm_engine *e, double facl, int diag, hipEvent_t mid = nullptr) {
     if (e->prec == MPM_PREC_FP64_EXACT) {
-        launch_f64_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.bb[0] * facl,
+        launch_f64_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.scheme, e->prm.bb[0] * facl,
                        e->prm.bb[1] * facl, e->prm.bb[2] * facl, e->prm.mtol, facl);
     } else {
         launch_f32_p2g(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, consts(e, facl),
@@ -402,6 +402,8 @@ int alloc_particles(mpm_engine *e, int n) {
         if (e->has_trac && (rc = e->dalloc(&P.trac, 3 * ld, true))) return rc;
         if (e->has_matid && (rc = e->dalloc(&P.matid, ld, true))) return rc;
         if (e->has_epsp && (rc = e->dalloc(&P.eps_p, ld, true))) return rc;
+        if (e->prm.scheme >= MPM_SCHEME_TPIC && (rc = e->dalloc(&P.C, 9 * ld, true))) return rc;
+        if (e->prm.scheme == MPM_SCHEME_APIC && (rc = e->dalloc(&P.Dinv, 9 * ld, true))) return rc;
         P.key = e->keys64;
         P.ld = ld;
         P.n = n;
@@ -463,6 +465,7 @@ GridGeom make_geom(const mpm_grid_desc *gd) {
     GridGeom g{};
     for (int i = 0; i < 3; i++) {
         g.xmin[i] = gd->xmin[i];
+        g.xmax[i] = gd->xmax[i];
         g.dx[i] = (gd->xmax[i] - gd->xmin[i]) / (double)gd->nelem[i];  // datain_para.cpp:87
         g.nex[i] = gd->nelem[i];
         g.idimc[i] = gd->idimc[i];
@@ -490,8 +493,10 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     int rc = validate_grid(grid, params && params->precision == MPM_PREC_FP64_EXACT);
     if (rc) return rc;
     if (!(params->dt > 0.0)) return set_error(MPM_ERR_INVALID_ARGUMENT, "dt must be positive");
-    if (params->scheme != MPM_SCHEME_PIC && params->scheme != MPM_SCHEME_FLIP)
-        return set_error(MPM_ERR_UNSUPPORTED, "scheme must be PIC or FLIP");
+    if (params->scheme < MPM_SCHEME_PIC || params->scheme > MPM_SCHEME_APIC)
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown transfer scheme");
+    if (params->scheme >= MPM_SCHEME_TPIC && params->precision != MPM_PREC_FP64_EXACT)
+        return set_error(MPM_ERR_UNSUPPORTED, "TPIC/APIC transfers run in the FP64-exact engine only");
     if (params->precision != MPM_PREC_FP64_EXACT && params->precision != MPM_PREC_FP32)
         return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown precision");
     if (params->nlstep < 1) return set_error(MPM_ERR_INVALID_ARGUMENT, "nlstep must be >= 1");
@@ -853,6 +858,43 @@ int mpm_download_nodal_volume(mpm_engine *e, double *nvol) {
     return 0;
 }
 
+// TPIC velocity gradient / APIC B matrix and APIC inverse D, host [n][3][3]
+static int affine_copy(mpm_engine *e, double *cmat, double *dinv, bool to_device) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    const PartF64 &P = e->P64;
+    if (e->prec != MPM_PREC_FP64_EXACT || !P.C)
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "the engine's transfer scheme is not TPIC or APIC");
+    if (dinv && !P.Dinv) return set_error(MPM_ERR_INVALID_ARGUMENT, "inverse D exists under APIC only");
+    const size_t ld = e->ld;
+    const int n = e->n;
+    double *host[2] = {cmat, dinv};
+    double *dev[2] = {P.C, P.Dinv};
+    for (int b = 0; b < 2; b++) {
+        if (!host[b]) continue;
+        std::vector<double> soa(9 * ld);
+        if (to_device) {
+            for (int i = 0; i < n; i++)
+                for (int q = 0; q < 9; q++) soa[q * ld + i] = host[b][9 * (size_t)i + q];
+            if ((rc = h2d(e, dev[b], soa))) return rc;
+            MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        } else {
+            if ((rc = d2h(e, soa, (const double *)dev[b], 9 * ld))) return rc;
+            MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+            for (int i = 0; i < n; i++)
+                for (int q = 0; q < 9; q++) host[b][9 * (size_t)i + q] = soa[q * ld + i];
+        }
+    }
+    return 0;
+}
+
+int mpm_download_affine(mpm_engine *e, double *cmat, double *dinv) { return affine_copy(e, cmat, dinv, false); }
+
+int mpm_upload_affine(mpm_engine *e, const double *cmat, const double *dinv) {
+    if (e) e->drop_graph();
+    return affine_copy(e, const_cast<double *>(cmat), const_cast<double *>(dinv), true);
+}
+
 int mpm_engine_destroy(mpm_engine *eng) {
     if (!eng) return 0;
     (void)hipStreamSynchronize(eng->stream);
@@ -952,6 +994,12 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
             (rc = h2d(e, P.m, m)) || (rc = h2d(e, P.V, V)) || (rc = h2d(e, P.V0, V0)) || (rc = h2d(e, P.J, J)) ||
             (rc = h2d(e, P.eps_p, ep)))
             return rc;
+        // TPIC/APIC matrices start at zero (solid_material_point.cpp:36-43)
+        if (P.C || P.Dinv) {
+            std::vector<double> zero(9 * ld, 0.0);
+            if ((P.C && (rc = h2d(e, P.C, zero))) || (P.Dinv && (rc = h2d(e, P.Dinv, zero)))) return rc;
+            MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        }
         if (e->has_trac) {
             tr.assign(3 * ld, 0.0);
             for (int i = 0; i < n; i++)
@@ -1034,13 +1082,21 @@ int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) {
     const size_t ld = e->ld;
     if (e->prec == MPM_PREC_FP64_EXACT) {
         const PartF64 &P = e->P64;
-        std::vector<double> x, v, sig, F, V, J, ep;
+        std::vector<double> x, v, sig, F, V, J, ep, m, V0, tr;
+        std::vector<int> mid;
         if ((rc = d2h(e, x, P.x, 3 * ld)) || (rc = d2h(e, v, P.v, 3 * ld)) || (rc = d2h(e, sig, P.sig, 6 * ld)) ||
             (rc = d2h(e, F, P.F, 9 * ld)) || (rc = d2h(e, V, P.V, ld)) || (rc = d2h(e, J, P.J, ld)) ||
-            (rc = d2h(e, ep, P.eps_p, ld)))
+            (rc = d2h(e, ep, P.eps_p, ld)) || (rc = d2h(e, m, P.m, ld)) || (rc = d2h(e, V0, P.V0, ld)))
             return rc;
+        if (P.matid && (rc = d2h(e, mid, (const int *)P.matid, ld))) return rc;
+        if (P.trac && (rc = d2h(e, tr, (const double *)P.trac, 3 * ld))) return rc;
         MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
         for (int i = 0; i < n; i++) {
+            if (p->mass) p->mass[i] = m[i];
+            if (p->vol0) p->vol0[i] = V0[i];
+            if (p->matid) p->matid[i] = mid.empty() ? 0 : mid[i];
+            if (p->trac_force)
+                for (int a = 0; a < 3; a++) p->trac_force[3 * (size_t)i + a] = tr.empty() ? 0.0 : tr[a * ld + i];
             for (int a = 0; a < 3; a++) {
                 if (p->coord) p->coord[3 * (size_t)i + a] = x[a * ld + i];
                 if (p->vel) p->vel[3 * (size_t)i + a] = v[a * ld + i];
@@ -1055,14 +1111,17 @@ int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) {
         }
     } else {
         const PartF32 &P = e->P32[e->cur];
-        std::vector<float> rr, v, sig, H, ep;
+        std::vector<float> rr, v, sig, H, ep, m, V0, tr;
         std::vector<double> V, J;
-        std::vector<int> pidv;
+        std::vector<int> pidv, mid;
         std::vector<uint32_t> keys;
         if ((rc = d2h(e, rr, P.r, 3 * ld)) || (rc = d2h(e, v, P.v, 3 * ld)) || (rc = d2h(e, keys, P.key, ld)) ||
             (rc = d2h(e, sig, P.sig, 6 * ld)) || (rc = d2h(e, H, P.H, 9 * ld)) || (rc = d2h(e, V, P.V, ld)) ||
-            (rc = d2h(e, J, P.J, ld)) || (rc = d2h(e, ep, P.eps_p, ld)) || (rc = d2h(e, pidv, e->pid[e->cur], ld)))
+            (rc = d2h(e, J, P.J, ld)) || (rc = d2h(e, ep, P.eps_p, ld)) || (rc = d2h(e, pidv, e->pid[e->cur], ld)) ||
+            (rc = d2h(e, m, P.m, ld)) || (rc = d2h(e, V0, P.V0, ld)))
             return rc;
+        if (P.matid && (rc = d2h(e, mid, (const int *)P.matid, ld))) return rc;
+        if (P.trac && (rc = d2h(e, tr, (const float *)P.trac, 3 * ld))) return rc;
         MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
         const GridGeom &g = e->g;
         for (int s = 0; s < n; s++) {
@@ -1083,6 +1142,11 @@ int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) {
             if (p->vol) p->vol[i] = V[s];
             if (p->det_def_grad) p->det_def_grad[i] = J[s];
             if (p->eps_p && !ep.empty()) p->eps_p[i] = ep[s];
+            if (p->mass) p->mass[i] = m[s];
+            if (p->vol0) p->vol0[i] = V0[s];
+            if (p->matid) p->matid[i] = mid.empty() ? 0 : mid[s];
+            if (p->trac_force)
+                for (int a = 0; a < 3; a++) p->trac_force[3 * i + a] = tr.empty() ? 0.0 : (double)tr[a * ld + s];
         }
     }
     return 0;

@@ -120,9 +120,47 @@ __device__ __forceinline__ int node_cells(const GridGeom &g, int i, int j, int k
     return cnt;
 }
 
+// Control-point coordinate on one axis: DefineCpPos (module/mesh.h:32-47),
+// one rank.
+__device__ __forceinline__ double node_coord(const GridGeom &g, int a, int idx) {
+    if (g.idimc[a] == 1) return g.xmin[a] + g.dx[a] * (double)idx;
+    if (idx == 0) return g.xmin[a];
+    if (idx == g.nxc[a] - 1) return g.xmax[a];
+    return g.xmin[a] + g.dx[a] * ((double)idx - 0.5);
+}
+
+// Velocity particle pid carries to control point (i, j, k) under TPIC / APIC
+// (map_and_interpolate.h:62-78 and :107-131): v_p + M (x_a - x_p), M the
+// velocity gradient (TPIC) or B D^-1 (APIC, summed over k first as there).
+// The node coordinate is xyc[nid][j]; the reference's xyc[j][nid] at :69 and
+// :123 reads out of bounds.
+__device__ __forceinline__ void affine_vel(const GridGeom &g, const PartF64 &P, uint32_t pid, int i, int j, int k,
+                                           const double xyg[3], int scheme, double va[3]) {
+    const size_t ld = P.ld;
+    double d[3] = {node_coord(g, 0, i) - xyg[0], node_coord(g, 1, j) - xyg[1], node_coord(g, 2, k) - xyg[2]};
+    double M[9];
+    for (int q = 0; q < 9; q++) M[q] = P.C[q * ld + pid];
+    if (scheme == 3) {
+        double Di[9], BD[9];
+        for (int q = 0; q < 9; q++) Di[q] = P.Dinv[q * ld + pid];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                double acc = 0.0;
+                for (int q = 0; q < 3; q++) acc += M[3 * r + q] * Di[3 * q + c];
+                BD[3 * r + c] = acc;
+            }
+        for (int q = 0; q < 9; q++) M[q] = BD[q];
+    }
+    for (int r = 0; r < 3; r++) {
+        double t = 0.0;
+        for (int c = 0; c < 3; c++) t += M[3 * r + c] * d[c];
+        va[r] = va[r] + t;
+    }
+}
+
 // Particle2NodeS (var_trans_solid_explicit.cpp:20-96) for one node.
-__global__ void k64_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, double fx, double fy, double fz,
-                        double mtol, double facl) {
+__global__ void k64_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, int scheme, double fx, double fy,
+                        double fz, double mtol, double facl) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodec) return;
     const int nodec = g.nodec;
@@ -152,9 +190,11 @@ __global__ void k64_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense 
             }
             nm += sfi * mp;
             nvof += sfi * vp;
-            p0 += sfi * mp * P.v[pid];
-            p1 += sfi * mp * P.v[ld + pid];
-            p2 += sfi * mp * P.v[2 * ld + pid];
+            double va[3] = {P.v[pid], P.v[ld + pid], P.v[2 * ld + pid]};
+            if (scheme >= 2) affine_vel(g, P, pid, i, j, k, xyg, scheme, va);
+            p0 += sfi * mp * va[0];
+            p1 += sfi * mp * va[1];
+            p2 += sfi * mp * va[2];
             double fi0 = -vp * (s_0 * ds[0] + s_5 * ds[1] + s_4 * ds[2]);
             double fi1 = -vp * (s_5 * ds[0] + s_1 * ds[1] + s_3 * ds[2]);
             double fi2 = -vp * (s_4 * ds[0] + s_3 * ds[1] + s_2 * ds[2]);
@@ -235,6 +275,8 @@ __global__ void k64_g2p_vel(GridGeom g, PartF64 P, NodesF64 N, int scheme, doubl
         v1 = P.v[ld + pid];
         v2 = P.v[2 * ld + pid];
     }
+    // TPIC velocity gradient / APIC B and D, zeroed at :203-209
+    double Cm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, AD[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = 0; k <= g.idimc[2]; k++)
         for (int j = 0; j <= g.idimc[1]; j++)
             for (int i = 0; i <= g.idimc[0]; i++) {
@@ -242,23 +284,57 @@ __global__ void k64_g2p_vel(GridGeom g, PartF64 P, NodesF64 N, int scheme, doubl
                 int nid = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
                 double sfi, ds[3];
                 sf_node(g, xyg, ix, iy, iz, sfi, ds);
+                const double nv[3] = {N.nv[nid], N.nv[nodec + nid], N.nv[2 * nodec + nid]};
                 if (scheme != 1) {
-                    v0 += sfi * N.nv[nid];
-                    v1 += sfi * N.nv[nodec + nid];
-                    v2 += sfi * N.nv[2 * nodec + nid];
+                    v0 += sfi * nv[0];
+                    v1 += sfi * nv[1];
+                    v2 += sfi * nv[2];
                 } else {
                     v0 += sfi * dt * N.na[nid];
                     v1 += sfi * dt * N.na[nodec + nid];
                     v2 += sfi * dt * N.na[2 * nodec + nid];
                 }
+                if (scheme == 2) {
+                    // TPIC::VarGradG2P (map_and_interpolate.h:81-99)
+                    for (int r = 0; r < 3; r++)
+                        for (int c = 0; c < 3; c++) Cm[3 * r + c] += ds[c] * nv[r];
+                } else if (scheme == 3) {
+                    // APIC::VarBmatG2P :134-148, DmatG2P :151-164
+                    double d[3] = {node_coord(g, 0, ix) - xyg[0], node_coord(g, 1, iy) - xyg[1],
+                                   node_coord(g, 2, iz) - xyg[2]};
+                    for (int r = 0; r < 3; r++)
+                        for (int c = 0; c < 3; c++) Cm[3 * r + c] += sfi * nv[r] * d[c];
+                    for (int r = 0; r < 3; r++)
+                        for (int c = 0; c < 3; c++) AD[3 * r + c] += sfi * d[r] * d[c];
+                }
             }
     P.v[pid] = v0;
     P.v[ld + pid] = v1;
     P.v[2 * ld + pid] = v2;
+    if (scheme >= 2)
+        for (int q = 0; q < 9; q++) P.C[q * ld + pid] = Cm[q];
+    if (scheme == 3) {
+        // APIC::InvDmatG2P :167-175 with InvMat3 (cal_mat.h:48-66)
+        double det = AD[0] * (AD[4] * AD[8] - AD[5] * AD[7]) + AD[1] * (AD[5] * AD[6] - AD[3] * AD[8]) +
+                     AD[2] * (AD[3] * AD[7] - AD[4] * AD[6]);
+        det = 1.0 / det;
+        double o[9];
+        o[0] = (AD[4] * AD[8] - AD[5] * AD[7]) * det;
+        o[1] = (AD[2] * AD[7] - AD[1] * AD[8]) * det;
+        o[2] = (AD[1] * AD[5] - AD[2] * AD[4]) * det;
+        o[3] = (AD[5] * AD[6] - AD[3] * AD[8]) * det;
+        o[4] = (AD[0] * AD[8] - AD[2] * AD[6]) * det;
+        o[5] = (AD[2] * AD[3] - AD[0] * AD[5]) * det;
+        o[6] = (AD[3] * AD[7] - AD[4] * AD[6]) * det;
+        o[7] = (AD[1] * AD[6] - AD[0] * AD[7]) * det;
+        o[8] = (AD[0] * AD[4] - AD[1] * AD[3]) * det;
+        for (int q = 0; q < 9; q++) P.Dinv[q * ld + pid] = o[q];
+    }
 }
 
 // MUSL (var_trans_solid_explicit.cpp:101-148) for one node
-__global__ void k64_musl(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, double mtol, double facl) {
+__global__ void k64_musl(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, int scheme, double mtol,
+                         double facl) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodec) return;
     const int nodec = g.nodec;
@@ -278,9 +354,11 @@ __global__ void k64_musl(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense
             double sfi, ds[3];
             sf_node(g, xyg, i, j, k, sfi, ds);
             double mp = P.m[pid];
-            p0 += sfi * mp * P.v[pid];
-            p1 += sfi * mp * P.v[ld + pid];
-            p2 += sfi * mp * P.v[2 * ld + pid];
+            double va[3] = {P.v[pid], P.v[ld + pid], P.v[2 * ld + pid]};
+            if (scheme >= 2) affine_vel(g, P, pid, i, j, k, xyg, scheme, va);
+            p0 += sfi * mp * va[0];
+            p1 += sfi * mp * va[1];
+            p2 += sfi * mp * va[2];
         }
     }
     double nm = N.nm[n];
@@ -548,8 +626,9 @@ __global__ void k64_maksf(GridGeom g, int n, const int32_t *cell, const double *
 }  // namespace
 
 void launch_f64_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
-                    const BcDense &bc, double fx, double fy, double fz, double mtol, double facl) {
-    hipLaunchKernelGGL(k64_p2g, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, fx, fy, fz, mtol, facl);
+                    const BcDense &bc, int scheme, double fx, double fy, double fz, double mtol, double facl) {
+    hipLaunchKernelGGL(k64_p2g, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, scheme, fx, fy, fz, mtol,
+                       facl);
 }
 
 void launch_f64_solve_grid(hipStream_t s, const GridGeom &g, const NodesF64 &N, const BcDense &bc, double mtol) {
@@ -670,7 +749,7 @@ void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const No
         hipLaunchKernelGGL(k64_vel_plus_dt_acc, dim3(cdiv(3 * g.nodec, 256)), dim3(256), 0, s, 3 * g.nodec, N.nv,
                            (const double *)N.na, dt);
     if (P.n > 0) hipLaunchKernelGGL(k64_g2p_vel, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, scheme, dt);
-    hipLaunchKernelGGL(k64_musl, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, mtol, facl);
+    hipLaunchKernelGGL(k64_musl, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, scheme, mtol, facl);
     const double *geup = nullptr, *b0 = nullptr;
     if (shift && P.n > 0) {
         // update_point_coord_and_stress computes the correction before moving

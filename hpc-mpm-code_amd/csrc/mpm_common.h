@@ -12,6 +12,7 @@ namespace mpm {
 // (module/shape_function.cpp:34).
 struct GridGeom {
     double xmin[3];
+    double xmax[3];
     double dx[3];
     int nex[3];
     int nxc[3];

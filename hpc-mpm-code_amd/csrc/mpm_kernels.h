@@ -123,6 +123,8 @@ struct PartF64 {
     int *matid;     // or null
     double *eps_p;  // or null
     double *ddg;    // [9][ld] deformation-gradient increment
+    double *C;      // [9][ld] TPIC velocity gradient / APIC B matrix, row-major (null for PIC/FLIP)
+    double *Dinv;   // [9][ld] APIC inverse D matrix (null otherwise)
     const uint32_t *key;
     size_t ld;
     int n;
@@ -138,7 +140,7 @@ struct NodesF64 {
 };
 
 void launch_f64_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
-                    const BcDense &bc, double fx, double fy, double fz, double mtol, double facl);
+                    const BcDense &bc, int scheme, double fx, double fy, double fz, double mtol, double facl);
 void launch_f64_solve_grid(hipStream_t s, const GridGeom &g, const NodesF64 &N, const BcDense &bc, double mtol);
 // Particle shifting of the reference (DeltaCorrectionParticleShifting), FP64:
 // nvol = nodal volume of the filled grid; nei, geup, partial sums and b0 are

@@ -23,7 +23,7 @@ MPM_ERR_UNSUPPORTED = -6
 MPM_ERR_NO_DEVICE = -7
 MPM_ERR_INTERNAL = -8
 
-SCHEME_PIC, SCHEME_FLIP = 0, 1
+SCHEME_PIC, SCHEME_FLIP, SCHEME_TPIC, SCHEME_APIC = 0, 1, 2, 3
 PREC_FP64_EXACT, PREC_FP32 = 0, 1
 MODEL_RIGID, MODEL_LINELAST, MODEL_NEOHOOKEAN, MODEL_STVK, MODEL_MOONEYRIVLIN, MODEL_J2 = -1, 0, 1, 2, 3, 10
 MODEL_TAIT = 20
@@ -119,6 +119,8 @@ SIGNATURES = {
     "mpm_bandwidth_probe": (C.c_int, [C.c_int64, C.c_int32, _dp]),
     "mpm_set_shifting": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
     "mpm_download_nodal_volume": (C.c_int, [C.c_void_p, _dp]),
+    "mpm_download_affine": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "mpm_upload_affine": (C.c_int, [C.c_void_p, _dp, _dp]),
     # include/mpm/mpm_surrogate.h
     "mpm_mlp_forward": (C.c_int, [_fp, _fp, _fp, C.c_int64, _fp]),
     "mpm_mlp_gradient": (C.c_int, [_fp, _fp, _fp, _fp, C.c_int64, _fp, _fp]),
@@ -307,6 +309,18 @@ class Engine:
     def set_shifting(self, on, npxye=(2, 2, 2)):
         npx = np.ascontiguousarray(npxye, dtype=np.int32)
         check(self.lib.mpm_set_shifting(self.h, int(on), _ptr(npx, C.c_int32)))
+
+    def affine(self, n, dinv=False):
+        """TPIC velocity gradient / APIC B matrix [n][3][3] (and APIC inverse D)."""
+        cm = np.zeros((n, 3, 3))
+        di = np.zeros((n, 3, 3)) if dinv else None
+        check(self.lib.mpm_download_affine(self.h, _ptr(cm), _ptr(di)))
+        return (cm, di) if dinv else cm
+
+    def set_affine(self, cmat, dinv=None):
+        cm = np.ascontiguousarray(cmat, np.float64)
+        di = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
+        check(self.lib.mpm_upload_affine(self.h, _ptr(cm), _ptr(di)))
 
     def nodal_volume(self):
         out = np.zeros(self.nodec)

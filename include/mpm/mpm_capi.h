@@ -41,8 +41,12 @@ typedef enum mpm_status {
     MPM_ERR_INTERNAL = -8       /* an internal consistency check failed (the step placed nothing) */
 } mpm_status;
 
-/* Transfer scheme; same meaning as MapScheme (module/dataset.h:19). */
-enum { MPM_SCHEME_PIC = 0, MPM_SCHEME_FLIP = 1 };
+/* Transfer scheme; same meaning and codes as MapScheme (module/dataset.h:19).
+ * TPIC and APIC (module/map_and_interpolate.h:57-176) run in the FP64-exact
+ * engine only; their P2G reads the node coordinate as xyc[nid][j] where the
+ * reference writes xyc[j][nid] (:69, :123), an out-of-bounds read of its
+ * vector<array<double,3>>. */
+enum { MPM_SCHEME_PIC = 0, MPM_SCHEME_FLIP = 1, MPM_SCHEME_TPIC = 2, MPM_SCHEME_APIC = 3 };
 
 /* Constitutive models; codes -1..3 are the reference's iprop values
  * (module/solid/constitutive_model.cpp:287-305). MPM_MODEL_J2 and
@@ -89,7 +93,7 @@ typedef struct mpm_step_params {
     double dt;
     double mtol;       /* node mass threshold (input.txt) */
     double bb[3];      /* body force per unit mass (gravity) */
-    int32_t scheme;    /* MPM_SCHEME_PIC / MPM_SCHEME_FLIP */
+    int32_t scheme;    /* MPM_SCHEME_* (sp.solswitch) */
     int32_t nlstep;    /* load ramp: facl = istep/nlstep for istep <= nlstep, else 1 */
     int32_t precision; /* MPM_PREC_* */
     int32_t relayout_every; /* FP32: physically reorder particles every k steps (0 = never) */
@@ -169,6 +173,10 @@ int mpm_set_bc(mpm_engine *eng, int32_t comp, const int32_t *nbc, const double *
 /* Copies particles host -> device (replaces InputPointData + the initial
  * state set-up of datain_para.cpp:104-126). */
 int mpm_upload_particles(mpm_engine *eng, const mpm_particles_view *p);
+/* Copies every non-NULL field of the view device -> host, in the original
+ * particle order (mass, vol0, matid and trac_force included, so that the
+ * view re-uploads as a restart). The FP32 engine returns its FP32 copies of
+ * mass, vol0 and trac_force. */
 int mpm_download_particles(mpm_engine *eng, mpm_particles_view *p);
 int mpm_download_nodes(mpm_engine *eng, mpm_nodes_view *nodes);
 
@@ -236,6 +244,13 @@ int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 int mpm_set_shifting(mpm_engine *eng, int32_t on, const int32_t *npxye);
 /* Nodal volume used by the shifting ([nodec]). */
 int mpm_download_nodal_volume(mpm_engine *eng, double *nvol);
+
+/* TPIC / APIC particle matrices, host [n][3][3] row-major: cmat is the TPIC
+ * vel_grad or the APIC vel_Bmat, dinv the APIC inv_Dmat (NULL to skip).
+ * They start at zero on upload (solid_material_point.cpp:36-43); the upload
+ * restores them from a restart (solid_data_io.cpp:88-95). */
+int mpm_download_affine(mpm_engine *eng, double *cmat, double *dinv);
+int mpm_upload_affine(mpm_engine *eng, const double *cmat, const double *dinv);
 /* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
 int mpm_set_timing(mpm_engine *eng, int32_t enabled);
 

@@ -10,9 +10,8 @@
  *                         sum of gradients = 0, linear reproduction, and the
  *                         dN sign rule at x == x_a (shape_function.cpp:45-50)
  *   orc_maksf (idimc=2)   partition of unity, gradient sum, linear
- *                         reproduction in the interior; boundary branches are
- *                         "parity unpinned" (only the reference could pin them,
- *                         and its translation unit needs MPI headers)
+ *                         reproduction in every cell, boundary cells included
+ *                         (with orc_node_coord, below)
  *   orc_cell_of/orc_bin   brute force on tiny grids, faces map to the higher
  *                         cell, linked-list order = stable sort by cell
  *   orc_p2g/orc_g2p       conservation of mass and momentum, sum of internal
@@ -22,6 +21,17 @@
  *                         constitutive_model.cpp compiled under oracle/_ref
  *   orc_bc_set_*          bit-exact against the reference's bc.cpp (oracle/_ref)
  *   orc_det3/orc_inv3     bit-exact against the reference's cal_mat.h templates
+ *   orc_node_coord        linear reproduction sum_a N_a(x) x_a = x and
+ *                         sum_a x_a (x) grad N_a = I in every cell, which
+ *                         also pins the quadratic B-spline boundary branches
+ *                         of orc_maksf (the end control points are the
+ *                         Greville abscissae of the open B-spline)
+ *   TPIC/APIC in orc_p2g/orc_g2p  G2P of a linear nodal field v = A x + b
+ *                         returns C = A (TPIC) and B D^-1 = A (APIC) for a
+ *                         non-symmetric A; P2G of particles carrying
+ *                         v = A x_p + b, C = A returns A x_a + b at every
+ *                         node with mass; the affine term adds no net
+ *                         momentum
  *   orc_j2_return         no reference counterpart: lands on the yield surface,
  *                         dgamma >= 0, elastic steps unchanged, closed-form
  *                         uniaxial hardening response
@@ -194,7 +204,55 @@ static void apply_vel_bc(const orc_state *st, double *nvel, double facl) {
 
 #define MAXNEN 27
 
-/* Particle2NodeS, var_trans_solid_explicit.cpp:20-99 (PIC/FLIP branch). */
+/* DefineCpPos (module/mesh.h:32-47) on one rank (xymin == xyminw):
+ * linear axes put control points on the cell corners; quadratic B-spline
+ * axes put the two end points on the grid ends and the others at the cell
+ * centres. Pinned in tests/test_oracle_affine.py by linear reproduction,
+ * sum_a N_a(x) x_a == x, for both shape functions. */
+double orc_node_coord(const orc_state *st, int axis, int idx) {
+    if (st->idimc[axis] == 1) return st->xmin[axis] + st->dx[axis] * (double)idx;
+    if (idx == 0) return st->xmin[axis];
+    if (idx == st->nxc[axis] - 1) return st->xmax[axis];
+    return st->xmin[axis] + st->dx[axis] * ((double)idx - 0.5);
+}
+
+static void node_offset(const orc_state *st, int nid, const double *xp, double d[3]) {
+    int i = nid % st->nxc[0];
+    int j = (nid / st->nxc[0]) % st->nxc[1];
+    int k = nid / (st->nxc[0] * st->nxc[1]);
+    d[0] = orc_node_coord(st, 0, i) - xp[0];
+    d[1] = orc_node_coord(st, 1, j) - xp[1];
+    d[2] = orc_node_coord(st, 2, k) - xp[2];
+}
+
+/* Velocity carried to node nid by particle pid (module/map_and_interpolate.h
+ * TPIC::VarP2G :62-78, APIC::VarP2G :107-131): v_p + C (x_a - x_p) with C
+ * the TPIC velocity gradient or the APIC product B D^-1 (summed over k, then
+ * over j, as there). The reference indexes the node coordinate as
+ * xyc[j][nid] (:69, :123), out of bounds for vector<array<double,3>>; this
+ * restatement uses xyc[nid][j], the coordinate the APIC G2P side uses. */
+static void affine_vel(const orc_state *st, int pid, int nid, const double *xp, const double *v, double out[3]) {
+    const double *C = st->cmat + 9 * (size_t)pid;
+    double d[3], M[9];
+    node_offset(st, nid, xp, d);
+    if (st->scheme == ORC_APIC) {
+        const double *Di = st->dinv + 9 * (size_t)pid;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double acc = 0.0;
+                for (int k = 0; k < 3; k++) acc += C[3 * i + k] * Di[3 * k + j];
+                M[3 * i + j] = acc;
+            }
+        C = M;
+    }
+    for (int i = 0; i < 3; i++) {
+        double t = 0.0;
+        for (int j = 0; j < 3; j++) t += C[3 * i + j] * d[j];
+        out[i] = v[i] + t;
+    }
+}
+
+/* Particle2NodeS, var_trans_solid_explicit.cpp:20-99 (all four transfer schemes). */
 void orc_p2g(orc_state *st, double facl) {
     const int nn = st->nodec;
     double fx = st->bb[0] * facl, fy = st->bb[1] * facl, fz = st->bb[2] * facl;
@@ -224,9 +282,11 @@ void orc_p2g(orc_state *st, double facl) {
                 double d1 = dsf[3 * ni], d2 = dsf[3 * ni + 1], d3 = dsf[3 * ni + 2];
                 st->nmass[nid] += sfi * mp;
                 st->nvof[nid] += sfi * vp;
-                st->nmome[nid] += sfi * mp * vel[0];
-                st->nmome[nid + nn] += sfi * mp * vel[1];
-                st->nmome[nid + 2 * nn] += sfi * mp * vel[2];
+                double va[3] = {vel[0], vel[1], vel[2]};
+                if (st->scheme >= ORC_TPIC) affine_vel(st, pid, nid, xyp, vel, va);
+                st->nmome[nid] += sfi * mp * va[0];
+                st->nmome[nid + nn] += sfi * mp * va[1];
+                st->nmome[nid + 2 * nn] += sfi * mp * va[2];
                 double fi0 = -vp * (sts[0] * d1 + sts[5] * d2 + sts[4] * d3);
                 double fi1 = -vp * (sts[5] * d1 + sts[1] * d2 + sts[3] * d3);
                 double fi2 = -vp * (sts[4] * d1 + sts[3] * d2 + sts[2] * d3);
@@ -573,39 +633,64 @@ void orc_g2p(orc_state *st, double facl) {
         for (int n = 0; n < 3 * nn; n++) st->nvel[n] += dt * st->naccel[n];
         memset(st->vel, 0, sizeof(double) * 3 * (size_t)st->num);
     }
+    /* :203-209 */
+    if (st->scheme == ORC_TPIC || st->scheme == ORC_APIC) memset(st->cmat, 0, sizeof(double) * 9 * (size_t)st->num);
+    if (st->scheme == ORC_APIC) memset(st->dinv, 0, sizeof(double) * 9 * (size_t)st->num);
     /* :211-240 */
     for (int m = 0; m < st->nelem; m++) {
         for (int pid = st->idepf[m]; pid != -1; pid = st->idp2p[pid]) {
-            int nenode = orc_maksf(st, m, st->coord + 3 * pid, ncm, sf, dsf);
+            const double *xp = st->coord + 3 * pid;
+            int nenode = orc_maksf(st, m, xp, ncm, sf, dsf);
             double *v = st->vel + 3 * pid;
+            double *Cm = st->cmat ? st->cmat + 9 * (size_t)pid : NULL;
+            double ADp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             for (int ni = 0; ni < nenode; ni++) {
                 int nid = ncm[ni];
                 double sfi = sf[ni];
+                const double nv[3] = {st->nvel[nid], st->nvel[nid + nn], st->nvel[nid + 2 * nn]};
                 if (st->scheme != ORC_FLIP) {
-                    v[0] += sfi * st->nvel[nid];
-                    v[1] += sfi * st->nvel[nid + nn];
-                    v[2] += sfi * st->nvel[nid + 2 * nn];
+                    v[0] += sfi * nv[0];
+                    v[1] += sfi * nv[1];
+                    v[2] += sfi * nv[2];
                 } else {
                     v[0] += sfi * dt * st->naccel[nid];
                     v[1] += sfi * dt * st->naccel[nid + nn];
                     v[2] += sfi * dt * st->naccel[nid + 2 * nn];
                 }
+                if (st->scheme == ORC_TPIC) {
+                    /* TPIC::VarGradG2P (map_and_interpolate.h:81-99): C_ij += dN_j v_i */
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) Cm[3 * i + j] += dsf[3 * ni + j] * nv[i];
+                } else if (st->scheme == ORC_APIC) {
+                    /* APIC::VarBmatG2P :134-148 and DmatG2P :151-164 */
+                    double d[3];
+                    node_offset(st, nid, xp, d);
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) Cm[3 * i + j] += sfi * nv[i] * d[j];
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) ADp[3 * i + j] += sfi * d[i] * d[j];
+                }
             }
+            /* APIC::InvDmatG2P :167-175 (InvMat3, cal_mat.h:48-66) */
+            if (st->scheme == ORC_APIC) orc_inv3(ADp, st->dinv + 9 * (size_t)pid);
         }
     }
     /* MUSL :101-148 */
     memset(st->nmome, 0, sizeof(double) * 3 * nn);
     for (int m = 0; m < st->nelem; m++) {
         for (int pid = st->idepf[m]; pid != -1; pid = st->idp2p[pid]) {
-            int nenode = orc_maksf(st, m, st->coord + 3 * pid, ncm, sf, dsf);
+            const double *xp = st->coord + 3 * pid;
+            int nenode = orc_maksf(st, m, xp, ncm, sf, dsf);
             const double mp = st->mass[pid];
             const double *v = st->vel + 3 * pid;
             for (int ni = 0; ni < nenode; ni++) {
                 int nid = ncm[ni];
                 double sfi = sf[ni];
-                st->nmome[nid] += sfi * mp * v[0];
-                st->nmome[nid + nn] += sfi * mp * v[1];
-                st->nmome[nid + 2 * nn] += sfi * mp * v[2];
+                double va[3] = {v[0], v[1], v[2]};
+                if (st->scheme >= ORC_TPIC) affine_vel(st, pid, nid, xp, v, va);
+                st->nmome[nid] += sfi * mp * va[0];
+                st->nmome[nid + nn] += sfi * mp * va[1];
+                st->nmome[nid + 2 * nn] += sfi * mp * va[2];
             }
         }
     }

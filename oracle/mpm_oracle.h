@@ -32,7 +32,7 @@ extern "C" {
 #define ORC_MAXPROP 8
 
 /* Transfer schemes, same meaning as MapScheme in module/dataset.h:19. */
-enum { ORC_PIC = 0, ORC_FLIP = 1 };
+enum { ORC_PIC = 0, ORC_FLIP = 1, ORC_TPIC = 2, ORC_APIC = 3 };
 
 /* Constitutive model codes, same as iprop in module/solid/constitutive_model.cpp:287-305,
  * plus J2 plasticity (no reference counterpart). */
@@ -106,6 +106,12 @@ typedef struct orc_state {
 
     /* scratch for the deformation-gradient increment, [num][9] */
     double *ddg;
+
+    /* affine transfer state, [num][3][3] row-major, zero at start
+     * (solid_material_point.cpp:36-43): TPIC vel_grad, APIC vel_Bmat in
+     * cmat; APIC inv_Dmat in dinv. May be NULL for PIC/FLIP. */
+    double *cmat;
+    double *dinv;
 } orc_state;
 
 /* grid set-up helper: dx = (xmax - xmin) / nex as in datain_para.cpp:87 */
@@ -141,6 +147,9 @@ void orc_nodal_vol(orc_state *st, const int32_t npxye[3]);
  * index order, then over the blocks in order. delta: [num][3]. */
 #define ORC_SUM_BLOCK 256
 void orc_shift_delta(const orc_state *st, double *delta);
+
+/* Control-point coordinate on one axis (DefineCpPos, module/mesh.h:32-47, one rank). */
+double orc_node_coord(const orc_state *st, int axis, int idx);
 
 /* constitutive update, semantics of ConstitutiveModel::UpdateStress */
 void orc_update_stress(int model, double stress[6], const double F[9], const double dF[9], const double *prop,

@@ -19,7 +19,7 @@ REF_LIB = os.path.join(HERE, "_ref", "libmpm_ref.so")
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
-PIC, FLIP = 0, 1
+PIC, FLIP, TPIC, APIC = 0, 1, 2, 3
 
 
 class OrcState(C.Structure):
@@ -36,7 +36,7 @@ class OrcState(C.Structure):
         ("def_grad", _dp), ("det_def_grad", _dp), ("trac_force", _dp), ("matid", _ip), ("eps_p", _dp),
         ("idepf", _ip), ("idp2p", _ip), ("numep", _ip),
         ("nmass", _dp), ("nvof", _dp), ("nmome", _dp), ("nforce", _dp), ("nvel", _dp), ("naccel", _dp),
-        ("nvol", _dp), ("ddg", _dp),
+        ("nvol", _dp), ("ddg", _dp), ("cmat", _dp), ("dinv", _dp),
     ]
 
 
@@ -71,6 +71,8 @@ def load(fast=False):
     lib.orc_tait_stress.argtypes = [_dp, C.c_double, _dp, C.c_double, _dp]
     lib.orc_nodal_vol.argtypes = [C.POINTER(OrcState), C.POINTER(C.c_int32)]
     lib.orc_shift_delta.argtypes = [C.POINTER(OrcState), _dp]
+    lib.orc_node_coord.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int]
+    lib.orc_node_coord.restype = C.c_double
     _fp = C.POINTER(C.c_float)
     lib.orc_mlp_forward.argtypes = [_dp, _dp, _fp, C.c_long, C.c_int, _dp]
     lib.orc_mlp_grad.argtypes = [_dp, _dp, _fp, _fp, C.c_long, C.c_int, _dp]
@@ -185,6 +187,10 @@ class Oracle:
             setattr(st, f, P(getattr(self, f)))
         self.nvol = np.zeros(nodec)
         st.nvol = P(self.nvol)
+        # TPIC velocity gradient / APIC B matrix and APIC inverse D, [n][3][3], zero at start
+        self.cmat = np.zeros((n, 3, 3))
+        self.dinv = np.zeros((n, 3, 3))
+        st.cmat, st.dinv = P(self.cmat), P(self.dinv)
         self.st = st
         self.nodec = nodec
         self.nelem = nelem
@@ -201,6 +207,9 @@ class Oracle:
         d = np.zeros((self.parts.n, 3))
         self.lib.orc_shift_delta(C.byref(self.st), P(d))
         return d
+
+    def node_coord(self, axis, idx):
+        return self.lib.orc_node_coord(C.byref(self.st), axis, idx)
 
     def bin(self):
         return self.lib.orc_bin(C.byref(self.st))

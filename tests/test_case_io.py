@@ -104,14 +104,18 @@ def test_missing_and_malformed_files_are_errors(tmp_path):
         capi.Case(None, str(gd), None)
 
 
-def test_unsupported_transfer_scheme_and_bad_matid(tmp_path):
+def test_transfer_scheme_names_and_bad_matid(tmp_path):
     src = open(beam_case.BEAM_INPUT).read().splitlines()
-    src[1] = "APIC  0  2  1"
     f = tmp_path / "input.txt"
+    for name, code in (("APIC", capi.SCHEME_APIC), ("TPIC", capi.SCHEME_TPIC), ("PIC", capi.SCHEME_PIC)):
+        src[1] = name + "  0  2  1"
+        f.write_text("\n".join(src) + "\n")
+        assert capi.Case(str(f)).desc().step.scheme == code
+    src[1] = "MUSL  0  2  1"   # ParseMapScheme rejects unknown names
     f.write_text("\n".join(src) + "\n")
     with pytest.raises(capi.MPMError) as e:
         capi.Case(str(f))
-    assert e.value.code == capi.MPM_ERR_UNSUPPORTED
+    assert e.value.code == capi.MPM_ERR_INVALID_ARGUMENT and "MUSL" in str(e.value)
     sp = tmp_path / "spdata.txt"
     sp.write_text("%10d\n%15.6e%15.6e%15.6e\n%10d%10d%15.6e%15.6e\n" % (1, 1.0, 0.25, 4.0, 0, 3, 1.0, 1.0))
     with pytest.raises(capi.MPMError):

@@ -487,3 +487,47 @@ def test_reference_beam_case_with_shifting(tmp_path, iprop):
     e2 = engine(s, capi.PREC_FP64_EXACT)
     assert e2.step(1, 200) == 0
     assert_parts_equal(e2.download(s.n), got)
+
+
+# ------------------------------------------------------------------ TPIC / APIC transfers
+@pytest.mark.parametrize("scheme", [capi.SCHEME_TPIC, capi.SCHEME_APIC])
+@pytest.mark.parametrize("idimc", [(1, 1, 1), (2, 1, 2), (2, 2, 2)])
+def test_fp64_affine_transfer_run_bitexact(scheme, idimc):
+    """TPIC / APIC (map_and_interpolate.h:57-176) through P2G, G2P and MUSL:
+    particle state and the particle matrices bit-identical to the oracle."""
+    s = scenes.tiny(n_cells=8, model=capi.MODEL_LINELAST, props=(1e6, 0.3), scheme=scheme, idimc=idimc)
+    print("affine transfer scheme=%d idimc=%s n=%d" % (scheme, idimc, s.n), flush=True)
+    o = oracle_run(s, 30)
+    e = engine(s, capi.PREC_FP64_EXACT)
+    assert e.step(1, 30) == 0
+    assert_parts_equal(e.download(s.n), o.parts)
+    if scheme == capi.SCHEME_TPIC:
+        assert np.array_equal(e.affine(s.n), o.cmat)
+    else:
+        cm, di = e.affine(s.n, dinv=True)
+        assert np.array_equal(cm, o.cmat) and np.array_equal(di, o.dinv)
+    grad = o.cmat if scheme == capi.SCHEME_TPIC else o.cmat @ o.dinv
+    assert np.abs(grad).max() > 0.1  # the rotating block has a velocity gradient
+
+
+def test_affine_state_upload_restores_a_run():
+    """Restart: uploading the particle state and the APIC matrices after 10
+    steps continues exactly as the uninterrupted run."""
+    s = scenes.tiny(n_cells=8, model=capi.MODEL_LINELAST, props=(1e6, 0.3), scheme=capi.SCHEME_APIC)
+    a = engine(s, capi.PREC_FP64_EXACT)
+    assert a.step(1, 10) == 0
+    mid = a.download(s.n)
+    cm, di = a.affine(s.n, dinv=True)
+    assert a.step(11, 10) == 0
+    b = capi.Engine(s, precision=capi.PREC_FP64_EXACT)
+    b.upload(mid)
+    b.set_affine(cm, di)
+    assert b.step(11, 10) == 0
+    assert_parts_equal(b.download(s.n), a.download(s.n))
+
+
+def test_fp32_rejects_affine_transfers():
+    s = scenes.tiny(n_cells=8, scheme=capi.SCHEME_TPIC)
+    with pytest.raises(capi.MPMError) as ei:
+        capi.Engine(s, precision=capi.PREC_FP32)
+    assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
