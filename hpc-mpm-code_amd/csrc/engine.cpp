@@ -103,6 +103,10 @@ struct mpm_engine {
     double *d_geup = nullptr, *d_part_dot = nullptr;  // particle-sized
     int geup_n = 0;
 
+    // F-bar projection (FP64-exact engine)
+    bool fbar = false;
+    double *d_njbar = nullptr;
+
     // whole-step graph: blocks of graph_block steps captured once and replayed
     int graph_block = 0;
     int graph_cur = -1, graph_since = -1;  // particle buffer and relayout counter the graph starts from
@@ -330,7 +334,7 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
             sh = ShiftF64{e->d_nvol, e->d_nei, e->d_geup, e->d_part_eu, e->d_part_dot, e->d_b0};
         }
         launch_f64_g2p(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->mt, e->prm.scheme, e->prm.dt,
-                       e->prm.mtol, facl, e->d_err, e->shifting ? &sh : nullptr);
+                       e->prm.mtol, facl, e->d_err, e->shifting ? &sh : nullptr, e->fbar ? e->d_njbar : nullptr);
         e->keys_valid = false;
     } else {
         StepConsts c = consts(e, facl);
@@ -404,6 +408,9 @@ int alloc_particles(mpm_engine *e, int n) {
         if (e->has_epsp && (rc = e->dalloc(&P.eps_p, ld, true))) return rc;
         if (e->prm.scheme >= MPM_SCHEME_TPIC && (rc = e->dalloc(&P.C, 9 * ld, true))) return rc;
         if (e->prm.scheme == MPM_SCHEME_APIC && (rc = e->dalloc(&P.Dinv, 9 * ld, true))) return rc;
+        if (e->fbar && ((rc = e->dalloc(&P.Fbar, 9 * ld, true)) || (rc = e->dalloc(&P.Jbar, ld, true)) ||
+                        (rc = e->dalloc(&P.detdd, ld, true))))
+            return rc;
         P.key = e->keys64;
         P.ld = ld;
         P.n = n;
@@ -858,6 +865,80 @@ int mpm_download_nodal_volume(mpm_engine *e, double *nvol) {
     return 0;
 }
 
+// F-bar state at the start of a run: Fbar = I, Jbar = 1 (datain_para.cpp:122)
+static int fbar_reset(mpm_engine *e) {
+    const size_t ld = e->ld;
+    std::vector<double> F(9 * ld, 0.0), J(ld, 1.0);
+    for (size_t i = 0; i < ld; i++) F[0 * ld + i] = F[4 * ld + i] = F[8 * ld + i] = 1.0;
+    int rc;
+    if ((rc = h2d(e, e->P64.Fbar, F)) || (rc = h2d(e, e->P64.Jbar, J))) return rc;
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int mpm_set_fbar(mpm_engine *e, int32_t on) {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    if (on && e->prec != MPM_PREC_FP64_EXACT)
+        return set_error(MPM_ERR_UNSUPPORTED, "the F-bar projection runs in the FP64-exact engine only");
+    e->drop_graph();
+    if (!on) {
+        e->fbar = false;
+        return 0;
+    }
+    int rc;
+    if (!e->d_njbar && (rc = e->dalloc(&e->d_njbar, (size_t)e->g.nodec))) return rc;
+    e->fbar = true;
+    if (e->n > 0) {
+        PartF64 &P = e->P64;
+        const size_t ld = e->ld;
+        if (!P.Fbar && ((rc = e->dalloc(&P.Fbar, 9 * ld, true)) || (rc = e->dalloc(&P.Jbar, ld, true)) ||
+                        (rc = e->dalloc(&P.detdd, ld, true))))
+            return rc;
+        if ((rc = fbar_reset(e))) return rc;
+    }
+    return 0;
+}
+
+static int fbar_copy(mpm_engine *e, double *fbar, double *jbar, bool to_device) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    PartF64 &P = e->P64;
+    if (e->prec != MPM_PREC_FP64_EXACT || !P.Fbar) return set_error(MPM_ERR_INVALID_ARGUMENT, "F-bar is off");
+    const size_t ld = e->ld;
+    const int n = e->n;
+    if (fbar) {
+        std::vector<double> soa(9 * ld);
+        if (to_device) {
+            for (int i = 0; i < n; i++)
+                for (int q = 0; q < 9; q++) soa[q * ld + i] = fbar[9 * (size_t)i + q];
+            if ((rc = h2d(e, P.Fbar, soa))) return rc;
+        } else {
+            if ((rc = d2h(e, soa, (const double *)P.Fbar, 9 * ld))) return rc;
+        }
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        if (!to_device)
+            for (int i = 0; i < n; i++)
+                for (int q = 0; q < 9; q++) fbar[9 * (size_t)i + q] = soa[q * ld + i];
+    }
+    if (jbar) {
+        if (to_device)
+            MPM_HIP_CHECK(hipMemcpyAsync(P.Jbar, jbar, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        else
+            MPM_HIP_CHECK(hipMemcpyAsync(jbar, P.Jbar, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    }
+    return 0;
+}
+
+int mpm_download_fbar(mpm_engine *e, double *def_grad_bar, double *det_def_grad_bar) {
+    return fbar_copy(e, def_grad_bar, det_def_grad_bar, false);
+}
+
+int mpm_upload_fbar(mpm_engine *e, const double *def_grad_bar, const double *det_def_grad_bar) {
+    if (e) e->drop_graph();
+    return fbar_copy(e, const_cast<double *>(def_grad_bar), const_cast<double *>(det_def_grad_bar), true);
+}
+
 // TPIC velocity gradient / APIC B matrix and APIC inverse D, host [n][3][3]
 static int affine_copy(mpm_engine *e, double *cmat, double *dinv, bool to_device) {
     int rc = require_particles(e);
@@ -994,6 +1075,7 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
             (rc = h2d(e, P.m, m)) || (rc = h2d(e, P.V, V)) || (rc = h2d(e, P.V0, V0)) || (rc = h2d(e, P.J, J)) ||
             (rc = h2d(e, P.eps_p, ep)))
             return rc;
+        if (P.Fbar && (rc = fbar_reset(e))) return rc;
         // TPIC/APIC matrices start at zero (solid_material_point.cpp:36-43)
         if (P.C || P.Dinv) {
             std::vector<double> zero(9 * ld, 0.0);

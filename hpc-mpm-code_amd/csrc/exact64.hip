@@ -400,7 +400,7 @@ __device__ __forceinline__ void bmat(const double F[9], double be[9]) {
 // ConstitutiveModel::UpdateStress (constitutive_model.cpp:287-305), FP64,
 // same expression order as the reference. J2 has no reference counterpart.
 __device__ void update_stress64(const MaterialDev &md, double *sig, size_t ld, int pid, const double F[9],
-                                const double dF[9], double jac, double *eps_p, double dt) {
+                                const double dF[9], double jac, double jac_bar, double *eps_p, double dt) {
     const double *p = md.prop;
     switch (md.model) {
     case -1:
@@ -428,7 +428,7 @@ __device__ void update_stress64(const MaterialDev &md, double *sig, size_t ld, i
         double yng = p[0], nu = p[1];
         double lmd = nu * yng / ((1.0 + nu) * (1.0 - 2.0 * nu));
         double mu = yng / (2.0 * (1.0 + nu));
-        double lnjac = log(jac);
+        double lnjac = log(jac_bar);
         double be[9], s[9];
         bmat(F, be);
         for (int q = 0; q < 9; q++) {
@@ -532,7 +532,7 @@ __device__ void update_stress64(const MaterialDev &md, double *sig, size_t ld, i
 // Deformation gradient, coordinates, volume and stress
 // (var_trans_solid_explicit.cpp:244-296 and :150-185), one thread per particle.
 __global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err,
-                               const double *__restrict__ geup, const double *__restrict__ b0p) {
+                               const double *__restrict__ geup, const double *__restrict__ b0p, int fbar) {
     int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P.n) return;
     uint32_t m = P.key[pid];
@@ -582,6 +582,10 @@ __global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable 
     }
     double J = det3(F);
     P.J[pid] = J;
+    if (fbar) {  // F-bar: positions, volume and stress follow the projection (k64_fbar_finish)
+        P.detdd[pid] = det3(dd);
+        return;
+    }
     if (geup) {  // shifting correction after the position update (var_trans_solid_explicit.cpp:176)
         const double b0 = *b0p;
         dx0 += -b0 * geup[pid];
@@ -593,8 +597,99 @@ __global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable 
     P.x[2 * ld + pid] = dx2;
     P.V[pid] = J * P.V0[pid];
     int mat = P.matid ? P.matid[pid] : 0;
-    update_stress64(mt.m[mat], P.sig, ld, pid, F, dd, J, P.eps_p, dt);
+    update_stress64(mt.m[mat], P.sig, ld, pid, F, dd, J, J, P.eps_p, dt);
     if (!(J > 0.0)) atomicOr(err, kErrJacobian);
+    if (!isfinite(dx0) || !isfinite(dx1) || !isfinite(dx2)) atomicOr(err, kErrNonFinite);
+}
+
+// cal_def_grad_bar (fbar_projection.cpp:18-48) for one node: sum of
+// N (Jbar_p det dF_p) V_p in the reference's order, divided by nvof.
+__global__ void k64_fbar_nodes(GridGeom g, PartF64 P, NodesF64 N, CellLists L, double mtol, double *njbar) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    const size_t ld = P.ld;
+    int i = n % g.nxc[0];
+    int j = (n / g.nxc[0]) % g.nxc[1];
+    int k = n / (g.nxc[0] * g.nxc[1]);
+    int cells[27];
+    int nc = node_cells(g, i, j, k, cells);
+    double acc = 0.0;
+    for (int q = 0; q < nc; q++) {
+        int c = cells[q];
+        for (uint32_t s = L.cstart[c]; s < L.cend[c]; s++) {
+            uint32_t pid = L.perm[s];
+            double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+            double sfi, ds[3];
+            sf_node(g, xyg, i, j, k, sfi, ds);
+            double jp = P.Jbar[pid] * P.detdd[pid];
+            acc += sfi * jp * P.V[pid];
+        }
+    }
+    double nv = N.nvof[n];
+    njbar[n] = (nv > mtol) ? acc / nv : 0.0;
+}
+
+// cal_def_grad_bar (fbar_projection.cpp:50-91) and then
+// update_point_coord_and_stress (var_trans_solid_explicit.cpp:150-185) with
+// Fbar, Jbar and the scaled increment, one thread per particle.
+__global__ void k64_fbar_finish(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err,
+                                const double *__restrict__ njbar, const double *__restrict__ geup,
+                                const double *__restrict__ b0p) {
+    int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P.n) return;
+    uint32_t m = P.key[pid];
+    if (m >= (uint32_t)g.nelem) return;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    int ixe, iye, ize;
+    cell_ijk(g, (int)m, ixe, iye, ize);
+    double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+    double jbar = 0.0;
+    double dx0 = xyg[0], dx1 = xyg[1], dx2 = xyg[2];
+    for (int k = 0; k <= g.idimc[2]; k++)
+        for (int j = 0; j <= g.idimc[1]; j++)
+            for (int i = 0; i <= g.idimc[0]; i++) {
+                int ix = ixe + i, iy = iye + j, iz = ize + k;
+                int nid = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+                double sfi, ds[3];
+                sf_node(g, xyg, ix, iy, iz, sfi, ds);
+                jbar += sfi * njbar[nid];
+                dx0 += sfi * dt * N.nv[nid];
+                dx1 += sfi * dt * N.nv[nodec + nid];
+                dx2 += sfi * dt * N.nv[2 * nodec + nid];
+            }
+    const double Jp = P.Jbar[pid] * P.detdd[pid];
+    double dd[9], Fb[9], Fn[9];
+    for (int q = 0; q < 9; q++) {
+        dd[q] = P.ddg[q * ld + pid];
+        dd[q] *= cbrt((jbar / Jp));
+        Fb[q] = P.Fbar[q * ld + pid];
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int kk = 0; kk < 3; kk++) acc += dd[3 * i + kk] * Fb[3 * kk + j];
+            Fn[3 * i + j] = acc;
+        }
+    for (int q = 0; q < 9; q++) {
+        P.Fbar[q * ld + pid] = Fn[q];
+        P.ddg[q * ld + pid] = dd[q];
+    }
+    P.Jbar[pid] = jbar;
+    if (geup) {
+        const double b0 = *b0p;
+        dx0 += -b0 * geup[pid];
+        dx1 += -b0 * geup[ld + pid];
+        dx2 += -b0 * geup[2 * ld + pid];
+    }
+    P.x[pid] = dx0;
+    P.x[ld + pid] = dx1;
+    P.x[2 * ld + pid] = dx2;
+    const double J = P.J[pid];
+    P.V[pid] = J * P.V0[pid];
+    int mat = P.matid ? P.matid[pid] : 0;
+    update_stress64(mt.m[mat], P.sig, ld, pid, Fn, dd, J, jbar, P.eps_p, dt);
+    if (!(J > 0.0) || !(jbar > 0.0)) atomicOr(err, kErrJacobian);
     if (!isfinite(dx0) || !isfinite(dx1) || !isfinite(dx2)) atomicOr(err, kErrNonFinite);
 }
 
@@ -744,7 +839,7 @@ void launch_f64_nodal_vol(hipStream_t s, const GridGeom &g, int npx, int npy, in
 
 void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
                     const BcDense &bc, const MaterialTable &mt, int scheme, double dt, double mtol, double facl,
-                    uint32_t *err, const ShiftF64 *shift) {
+                    uint32_t *err, const ShiftF64 *shift, double *njbar) {
     if (scheme != 1)
         hipLaunchKernelGGL(k64_vel_plus_dt_acc, dim3(cdiv(3 * g.nodec, 256)), dim3(256), 0, s, 3 * g.nodec, N.nv,
                            (const double *)N.na, dt);
@@ -762,8 +857,15 @@ void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const No
         geup = shift->geup;
         b0 = shift->b0;
     }
+    const int fbar = njbar != nullptr;
     if (P.n > 0)
-        hipLaunchKernelGGL(k64_g2p_update, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, mt, dt, err, geup, b0);
+        hipLaunchKernelGGL(k64_g2p_update, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, mt, dt, err, geup, b0,
+                           fbar);
+    if (fbar && P.n > 0) {  // var_trans_solid_explicit.cpp:298-301
+        hipLaunchKernelGGL(k64_fbar_nodes, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, mtol, njbar);
+        hipLaunchKernelGGL(k64_fbar_finish, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, mt, dt, err,
+                           (const double *)njbar, geup, b0);
+    }
 }
 
 int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cell, const double *xyg, int32_t *nc,

@@ -125,6 +125,9 @@ struct PartF64 {
     double *ddg;    // [9][ld] deformation-gradient increment
     double *C;      // [9][ld] TPIC velocity gradient / APIC B matrix, row-major (null for PIC/FLIP)
     double *Dinv;   // [9][ld] APIC inverse D matrix (null otherwise)
+    double *Fbar;   // [9][ld] F-bar deformation gradient (null when F-bar is off)
+    double *Jbar;   // [ld] its determinant as projected
+    double *detdd;  // [ld] det of the step's deformation-gradient increment (F-bar scratch)
     const uint32_t *key;
     size_t ld;
     int n;
@@ -161,7 +164,7 @@ void launch_f64_nodal_vol(hipStream_t s, const GridGeom &g, int npx, int npy, in
                           double volp, double *nvol);
 void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
                     const BcDense &bc, const MaterialTable &mt, int scheme, double dt, double mtol, double facl,
-                    uint32_t *err, const ShiftF64 *shift = nullptr);
+                    uint32_t *err, const ShiftF64 *shift = nullptr, double *njbar = nullptr);
 int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cell, const double *xyg, int32_t *nc,
                      double *sf, double *dsf);
 

@@ -121,6 +121,9 @@ SIGNATURES = {
     "mpm_download_nodal_volume": (C.c_int, [C.c_void_p, _dp]),
     "mpm_download_affine": (C.c_int, [C.c_void_p, _dp, _dp]),
     "mpm_upload_affine": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "mpm_set_fbar": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mpm_download_fbar": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "mpm_upload_fbar": (C.c_int, [C.c_void_p, _dp, _dp]),
     # include/mpm/mpm_surrogate.h
     "mpm_mlp_forward": (C.c_int, [_fp, _fp, _fp, C.c_int64, _fp]),
     "mpm_mlp_gradient": (C.c_int, [_fp, _fp, _fp, _fp, C.c_int64, _fp, _fp]),
@@ -305,6 +308,8 @@ class Engine:
             check(lib.mpm_set_bc(self.h, comp, _ptr(nbc, C.c_int32), _ptr(fbc), len(nbc)))
         if getattr(scene, "shifting", False):
             self.set_shifting(True, scene.npxye)
+        if getattr(scene, "fbar", False):
+            self.set_fbar(True)
 
     def set_shifting(self, on, npxye=(2, 2, 2)):
         npx = np.ascontiguousarray(npxye, dtype=np.int32)
@@ -321,6 +326,20 @@ class Engine:
         cm = np.ascontiguousarray(cmat, np.float64)
         di = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
         check(self.lib.mpm_upload_affine(self.h, _ptr(cm), _ptr(di)))
+
+    def set_fbar(self, on=True):
+        check(self.lib.mpm_set_fbar(self.h, int(on)))
+
+    def fbar_state(self, n):
+        """(Fbar [n][3][3], Jbar [n]) of the F-bar projection."""
+        fb = np.zeros((n, 3, 3))
+        jb = np.zeros(n)
+        check(self.lib.mpm_download_fbar(self.h, _ptr(fb), _ptr(jb)))
+        return fb, jb
+
+    def set_fbar_state(self, fbar, jbar):
+        check(self.lib.mpm_upload_fbar(self.h, _ptr(np.ascontiguousarray(fbar, np.float64)),
+                                       _ptr(np.ascontiguousarray(jbar, np.float64))))
 
     def nodal_volume(self):
         out = np.zeros(self.nodec)

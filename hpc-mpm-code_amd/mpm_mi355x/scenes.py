@@ -64,6 +64,7 @@ class Scene:
     idimc: tuple = (1, 1, 1)                 # 1 = linear tent, 2 = quadratic B-spline (FP64-exact engine)
     shifting: bool = False                   # reference particle shifting (FP64-exact engine)
     npxye: tuple = (2, 2, 2)                 # points per cell of the nodal-volume reference (MakNodalVol)
+    fbar: bool = False                       # F-bar projection (FP64-exact engine)
 
     @property
     def dx(self):

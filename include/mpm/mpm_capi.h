@@ -251,6 +251,21 @@ int mpm_download_nodal_volume(mpm_engine *eng, double *nvol);
  * restores them from a restart (solid_data_io.cpp:88-95). */
 int mpm_download_affine(mpm_engine *eng, double *cmat, double *dinv);
 int mpm_upload_affine(mpm_engine *eng, const double *cmat, const double *dinv);
+
+/* F-bar projection (cal_def_grad_bar, module/solid/explicit/fbar_projection.cpp:18-94;
+ * Fbar_flag, off by default in the reference), FP64-exact engine only. After
+ * the deformation-gradient update, N (Jbar det dF) V is gathered to the
+ * nodes, divided by nvof, interpolated back as Jbar; dF is scaled by
+ * cbrt(Jbar / (Jbar_old det dF)) and Fbar = dF Fbar_old. Stress then uses
+ * Fbar, the scaled dF, the true J and Jbar (solid_material_point.cpp:136-152);
+ * the volume stays J V0. The reference passes delta_def_grad in a layout
+ * its declaration does not accept; this is the projection with
+ * delta_def_grad[p][i][j] = dF_ij. Turning it on (or uploading particles
+ * while on) starts from Fbar = I, Jbar = 1. */
+int mpm_set_fbar(mpm_engine *eng, int32_t on);
+/* Fbar [n][3][3] and Jbar [n] (NULL to skip); the upload restores a restart. */
+int mpm_download_fbar(mpm_engine *eng, double *def_grad_bar, double *det_def_grad_bar);
+int mpm_upload_fbar(mpm_engine *eng, const double *def_grad_bar, const double *det_def_grad_bar);
 /* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
 int mpm_set_timing(mpm_engine *eng, int32_t enabled);
 

@@ -32,6 +32,10 @@
  *                         v = A x_p + b, C = A returns A x_a + b at every
  *                         node with mass; the affine term adds no net
  *                         momentum
+ *   orc_fbar              conservation of the volume-weighted J by the
+ *                         nodal projection, reproduction of a constant J,
+ *                         Fbar = F for a homogeneous deformation, and
+ *                         det Fbar = Jbar along a run
  *   orc_j2_return         no reference counterpart: lands on the yield surface,
  *                         dgamma >= 0, elastic steps unchanged, closed-form
  *                         uniaxial hardening response
@@ -737,8 +741,11 @@ void orc_g2p(orc_state *st, double facl) {
                 }
             memcpy(Fo, F, sizeof(F));
             st->det_def_grad[pid] = orc_det3(F);
+            if (st->fbar) st->det_ddg[pid] = orc_det3(dd);
         }
     }
+    /* :298-301 */
+    if (st->fbar) orc_fbar(st);
 
     /* update_point_coord_and_stress :150-185; the shifting correction is
      * computed first, from the positions and volumes before the update */
@@ -769,12 +776,67 @@ void orc_g2p(orc_state *st, double facl) {
                 j2_update(st, pid, prop, st->ddg + 9 * pid);
             else if (model == ORC_TAIT)
                 orc_tait_stress(prop, st->det_def_grad[pid], st->ddg + 9 * pid, dt, st->stress + 6 * pid);
+            else if (st->fbar)  /* F-bar: F and J-bar of the projection, true J (solid_material_point.cpp:143-149) */
+                orc_update_stress(model, st->stress + 6 * pid, st->def_grad_bar + 9 * pid, st->ddg + 9 * pid, prop,
+                                  st->det_def_grad[pid], st->det_def_grad_bar[pid]);
             else
                 orc_update_stress(model, st->stress + 6 * pid, st->def_grad + 9 * pid, st->ddg + 9 * pid, prop,
                                   st->det_def_grad[pid], st->det_def_grad[pid]);
         }
     }
     free(delta);
+}
+
+/* cal_def_grad_bar (module/solid/explicit/fbar_projection.cpp:18-94). The
+ * reference declares delta_def_grad as vector<array<array<double,3>,3>>
+ * ([pid][i][j]) while its caller passes the [i][j][pid] nested vectors built
+ * at var_trans_solid_explicit.cpp:242-244, which does not compile; this is
+ * the projection with delta_def_grad[pid][i][j] = dF_ij of particle pid.
+ * Nodal J-bar: sum of N (Jbar_p det dF_p) V_p over cells ascending and
+ * particles in list order, divided by nvof where nvof > mtol (else 0);
+ * particle J-bar: sum of N nJbar; dF *= cbrt(Jbar / (Jbar_old det dF));
+ * Fbar = dF Fbar_old. Pinned in tests/test_oracle_fbar.py: a homogeneous
+ * deformation gives Fbar = F; det Fbar = Jbar after many steps; the nodal
+ * projection conserves the volume-weighted J. */
+void orc_fbar(orc_state *st) {
+    const int nn = st->nodec;
+    int32_t ncm[MAXNEN];
+    double sf[MAXNEN], dsf[3 * MAXNEN];
+    double *nj = st->njbar;
+    memset(nj, 0, sizeof(double) * nn);
+    for (int m = 0; m < st->nelem; m++) {
+        for (int pid = st->idepf[m]; pid != -1; pid = st->idp2p[pid]) {
+            int nenode = orc_maksf(st, m, st->coord + 3 * pid, ncm, sf, dsf);
+            const double jp = st->det_def_grad_bar[pid] * st->det_ddg[pid];
+            for (int ni = 0; ni < nenode; ni++) nj[ncm[ni]] += sf[ni] * jp * st->vol[pid];
+        }
+    }
+    for (int n = 0; n < nn; n++) {
+        if (st->nvof[n] > st->mtol)
+            nj[n] /= st->nvof[n];
+        else
+            nj[n] = 0.0;
+    }
+    for (int m = 0; m < st->nelem; m++) {
+        for (int pid = st->idepf[m]; pid != -1; pid = st->idp2p[pid]) {
+            int nenode = orc_maksf(st, m, st->coord + 3 * pid, ncm, sf, dsf);
+            double jbar = 0.0;
+            for (int ni = 0; ni < nenode; ni++) jbar += sf[ni] * nj[ncm[ni]];
+            const double J = st->det_def_grad_bar[pid] * st->det_ddg[pid];
+            double *dd = st->ddg + 9 * pid;
+            for (int q = 0; q < 9; q++) dd[q] *= cbrt((jbar / J));
+            double *Fb = st->def_grad_bar + 9 * pid;
+            double Fn[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double acc = 0.0;
+                    for (int k = 0; k < 3; k++) acc += dd[3 * i + k] * Fb[3 * k + j];
+                    Fn[3 * i + j] = acc;
+                }
+            memcpy(Fb, Fn, sizeof(Fn));
+            st->det_def_grad_bar[pid] = jbar;
+        }
+    }
 }
 
 /* Tait equation of state + Newtonian viscosity (textbook weakly

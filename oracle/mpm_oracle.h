@@ -112,6 +112,16 @@ typedef struct orc_state {
      * cmat; APIC inv_Dmat in dinv. May be NULL for PIC/FLIP. */
     double *cmat;
     double *dinv;
+
+    /* F-bar projection (fbar_projection.cpp:18-94), off unless fbar != 0:
+     * def_grad_bar [num][9] (identity at start), det_def_grad_bar [num] (one
+     * at start), nodal scratch njbar [nodec] and per-particle det dF [num]. */
+    int32_t fbar;
+    int32_t pad1;
+    double *def_grad_bar;
+    double *det_def_grad_bar;
+    double *njbar;
+    double *det_ddg;
 } orc_state;
 
 /* grid set-up helper: dx = (xmax - xmin) / nex as in datain_para.cpp:87 */
@@ -147,6 +157,11 @@ void orc_nodal_vol(orc_state *st, const int32_t npxye[3]);
  * index order, then over the blocks in order. delta: [num][3]. */
 #define ORC_SUM_BLOCK 256
 void orc_shift_delta(const orc_state *st, double *delta);
+
+/* F-bar projection of one step (after the deformation-gradient update of
+ * orc_g2p, before the stress update): scales st->ddg and updates
+ * def_grad_bar / det_def_grad_bar. Called by orc_g2p when st->fbar is set. */
+void orc_fbar(orc_state *st);
 
 /* Control-point coordinate on one axis (DefineCpPos, module/mesh.h:32-47, one rank). */
 double orc_node_coord(const orc_state *st, int axis, int idx);

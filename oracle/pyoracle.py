@@ -37,6 +37,8 @@ class OrcState(C.Structure):
         ("idepf", _ip), ("idp2p", _ip), ("numep", _ip),
         ("nmass", _dp), ("nvof", _dp), ("nmome", _dp), ("nforce", _dp), ("nvel", _dp), ("naccel", _dp),
         ("nvol", _dp), ("ddg", _dp), ("cmat", _dp), ("dinv", _dp),
+        ("fbar", C.c_int32), ("pad1", C.c_int32), ("def_grad_bar", _dp), ("det_def_grad_bar", _dp),
+        ("njbar", _dp), ("det_ddg", _dp),
     ]
 
 
@@ -73,6 +75,7 @@ def load(fast=False):
     lib.orc_shift_delta.argtypes = [C.POINTER(OrcState), _dp]
     lib.orc_node_coord.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int]
     lib.orc_node_coord.restype = C.c_double
+    lib.orc_fbar.argtypes = [C.POINTER(OrcState)]
     _fp = C.POINTER(C.c_float)
     lib.orc_mlp_forward.argtypes = [_dp, _dp, _fp, C.c_long, C.c_int, _dp]
     lib.orc_mlp_grad.argtypes = [_dp, _dp, _fp, _fp, C.c_long, C.c_int, _dp]
@@ -191,6 +194,14 @@ class Oracle:
         self.cmat = np.zeros((n, 3, 3))
         self.dinv = np.zeros((n, 3, 3))
         st.cmat, st.dinv = P(self.cmat), P(self.dinv)
+        # F-bar state (identity / one at start) and scratch
+        self.def_grad_bar = np.tile(np.eye(3), (n, 1, 1))
+        self.det_def_grad_bar = np.ones(n)
+        self.njbar = np.zeros(nodec)
+        self.det_ddg = np.zeros(n)
+        for f in ("def_grad_bar", "det_def_grad_bar", "njbar", "det_ddg"):
+            setattr(st, f, P(getattr(self, f)))
+        st.fbar = 1 if getattr(scene, "fbar", False) else 0
         self.st = st
         self.nodec = nodec
         self.nelem = nelem
@@ -210,6 +221,9 @@ class Oracle:
 
     def node_coord(self, axis, idx):
         return self.lib.orc_node_coord(C.byref(self.st), axis, idx)
+
+    def fbar(self):
+        self.lib.orc_fbar(C.byref(self.st))
 
     def bin(self):
         return self.lib.orc_bin(C.byref(self.st))

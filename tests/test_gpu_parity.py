@@ -531,3 +531,59 @@ def test_fp32_rejects_affine_transfers():
     with pytest.raises(capi.MPMError) as ei:
         capi.Engine(s, precision=capi.PREC_FP32)
     assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
+
+
+# ------------------------------------------------------------------ F-bar projection
+@pytest.mark.parametrize("model,props", [(capi.MODEL_NEOHOOKEAN, (5e5, 0.35)), (capi.MODEL_LINELAST, (1e6, 0.3))])
+@pytest.mark.parametrize("idimc,shifting", [((1, 1, 1), False), ((2, 1, 2), False), ((2, 1, 2), True)])
+def test_fp64_fbar_run(model, props, idimc, shifting):
+    """F-bar (fbar_projection.cpp:18-94) over 50 steps against the oracle.
+    The projection calls cbrt (and Neo-Hookean log) from the device math
+    library, so agreement is to the last bits, not bitwise: positions within
+    1e-14, Fbar and Jbar within 1e-12 relative, stress within 1e-9 relative."""
+    s = scenes.tiny(n_cells=8, model=model, props=props, scheme=capi.SCHEME_FLIP, idimc=idimc)
+    s.fbar = True
+    s.shifting = shifting
+    if shifting:  # squeezed block: nodes hold excess volume, the correction moves particles
+        c = s.parts.coord.mean(axis=0)
+        s.parts.coord[:] = c + 0.93 * (s.parts.coord - c)
+    print("fbar model=%d idimc=%s shifting=%d" % (model, idimc, shifting), flush=True)
+    o = oracle_run(s, 50)
+    e = engine(s, capi.PREC_FP64_EXACT)
+    assert e.step(1, 50) == 0
+    got = e.download(s.n)
+    np.testing.assert_allclose(got.coord, o.parts.coord, rtol=0, atol=1e-14)
+    fb, jb = e.fbar_state(s.n)
+    np.testing.assert_allclose(fb, o.def_grad_bar, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(jb, o.det_def_grad_bar, rtol=1e-12)
+    np.testing.assert_allclose(got.stress, o.parts.stress, rtol=0, atol=1e-9 * np.abs(o.parts.stress).max())
+    np.testing.assert_allclose(got.def_grad, o.parts.def_grad, rtol=0, atol=1e-13)
+    # the projection changes the stress
+    s.fbar = False
+    plain = engine(s, capi.PREC_FP64_EXACT)
+    assert plain.step(1, 50) == 0
+    assert np.abs(plain.download(s.n).stress - got.stress).max() > 1e-6 * np.abs(got.stress).max()
+
+
+def test_fbar_state_upload_restores_a_run():
+    s = scenes.tiny(n_cells=8)
+    s.fbar = True
+    a = engine(s, capi.PREC_FP64_EXACT)
+    assert a.step(1, 10) == 0
+    mid = a.download(s.n)
+    fb, jb = a.fbar_state(s.n)
+    assert a.step(11, 10) == 0
+    b = capi.Engine(s, precision=capi.PREC_FP64_EXACT)
+    b.upload(mid)
+    b.set_fbar_state(fb, jb)
+    assert b.step(11, 10) == 0
+    assert_parts_equal(b.download(s.n), a.download(s.n))
+    assert np.array_equal(b.fbar_state(s.n)[1], a.fbar_state(s.n)[1])
+
+
+def test_fp32_rejects_fbar():
+    s = scenes.tiny(n_cells=8)
+    e = engine(s, capi.PREC_FP32)
+    with pytest.raises(capi.MPMError) as ei:
+        e.set_fbar(True)
+    assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
