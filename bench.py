@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--fused-count", type=int, default=1, help="count cells in the update kernel (1) or not (0)")
     ap.add_argument("--graph-block", type=int, default=0,
                     help="steps per recorded HIP graph block (0 = eager launches)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes-per-cell", type=int, default=0, help="override (1, 2, 4, 8); 0 = automatic")
     ap.add_argument("--no-probe", action="store_true", help="skip the copy-bandwidth probe")
@@ -102,21 +102,44 @@ def make_scene(name):
     return scenes.cfg2(ppc=(2, 4, 2))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, nsteps):
-    """FP64 oracle (timing build, -O3 -march=native), one core, on the same
-    workload for a few steps; step loop only."""
+    """The step on the host cores, on the same workload, step loop only: the
+    OpenMP port of the FP64 oracle (oracle/mpm_oracle_omp.c, -O3
+    -march=native, cell colouring) on up to 16 threads, the box's CPU share,
+    and the scalar oracle on one thread for one step beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
-    o = pyoracle.Oracle(scene, fast=True)
+    try:
+        threads = min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        threads = min(16, os.cpu_count() or 1)
+    o = pyoracle.Oracle(scene, fast="omp")
     t0 = time.perf_counter()
-    rc = o.run(1, nsteps)
+    rc = o.run_omp(1, nsteps, threads)
     dt = time.perf_counter() - t0
     if rc != 0:
         return None
-    return {"value": scene.n * nsteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "%s, %d particles, %d steps (FP64 oracle, -O3 -march=native, 1 thread), %.1f s" % (
-                scene.name, scene.n, nsteps, dt)}
+    del o
+    o1 = pyoracle.Oracle(scene, fast=True)
+    t1 = time.perf_counter()
+    rc1 = o1.run(1, 1)
+    dt1 = time.perf_counter() - t1
+    return {"value": scene.n * nsteps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "%s, %d particles, %d steps, FP64 OpenMP port of the step (cell colouring), %d threads, "
+                      "%.1f s" % (scene.name, scene.n, nsteps, threads, dt),
+            "cpu_model": cpu_model(),
+            "single_thread": None if rc1 else {"value": scene.n / dt1, "steps": 1, "seconds": round(dt1, 2)}}
 
 
 def timed_steps(eng, istep0, nsteps):

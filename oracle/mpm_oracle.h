@@ -163,6 +163,11 @@ void orc_shift_delta(const orc_state *st, double *delta);
  * def_grad_bar / det_def_grad_bar. Called by orc_g2p when st->fbar is set. */
 void orc_fbar(orc_state *st);
 
+/* OpenMP port of orc_run for the CPU baseline (mpm_oracle_omp.c, built into
+ * _build/libmpm_oracle_omp.so only): PIC/FLIP, no J2, shifting or F-bar
+ * (returns -1000 for those). nthreads <= 0 keeps the OpenMP default. */
+int orc_run_omp(orc_state *st, int ista, int iend, int nthreads);
+
 /* Control-point coordinate on one axis (DefineCpPos, module/mesh.h:32-47, one rank). */
 double orc_node_coord(const orc_state *st, int axis, int idx);
 

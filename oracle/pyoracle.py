@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "libmpm_oracle.so")
 LIB_FAST = os.path.join(HERE, "_build", "libmpm_oracle_fast.so")
+LIB_OMP = os.path.join(HERE, "_build", "libmpm_oracle_omp.so")  # multithreaded CPU-baseline port
 REF_LIB = os.path.join(HERE, "_ref", "libmpm_ref.so")
 
 _dp = C.POINTER(C.c_double)
@@ -43,11 +44,13 @@ class OrcState(C.Structure):
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE, "oracle", "timing"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "oracle", "timing", "omp"], check=True)
 
 
 def load(fast=False):
-    path = LIB_FAST if fast else LIB
+    """fast=False: parity build; True: timing build; "omp": timing build with
+    the OpenMP port of the step (orc_run_omp)."""
+    path = LIB_OMP if fast == "omp" else (LIB_FAST if fast else LIB)
     if not os.path.exists(path):
         build()
     lib = C.CDLL(path)
@@ -76,6 +79,8 @@ def load(fast=False):
     lib.orc_node_coord.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int]
     lib.orc_node_coord.restype = C.c_double
     lib.orc_fbar.argtypes = [C.POINTER(OrcState)]
+    if fast == "omp":
+        lib.orc_run_omp.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, C.c_int]
     _fp = C.POINTER(C.c_float)
     lib.orc_mlp_forward.argtypes = [_dp, _dp, _fp, C.c_long, C.c_int, _dp]
     lib.orc_mlp_grad.argtypes = [_dp, _dp, _fp, _fp, C.c_long, C.c_int, _dp]
@@ -242,6 +247,10 @@ class Oracle:
 
     def run(self, ista, iend):
         return self.lib.orc_run(C.byref(self.st), ista, iend)
+
+    def run_omp(self, ista, iend, threads=0):
+        """The OpenMP port (needs Oracle(scene, fast="omp"))."""
+        return self.lib.orc_run_omp(C.byref(self.st), ista, iend, threads)
 
     def cell_of(self, x):
         x = np.ascontiguousarray(x, np.float64)
