@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--fused-count", type=int, default=1, help="count cells in the update kernel (1) or not (0)")
     ap.add_argument("--graph-block", type=int, default=0,
                     help="steps per recorded HIP graph block (0 = eager launches)")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes-per-cell", type=int, default=0, help="override (1, 2, 4, 8); 0 = automatic")
     ap.add_argument("--no-probe", action="store_true", help="skip the copy-bandwidth probe")

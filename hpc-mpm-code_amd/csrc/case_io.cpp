@@ -338,3 +338,197 @@ int mpm_case_engine(const mpm_case *c, int32_t precision, int32_t relayout_every
 void mpm_case_free(mpm_case *c) { delete c; }
 
 }  // extern "C"
+
+struct mpm_restart {
+    int32_t scheme = 0;
+    std::vector<double> coord, vel, mass, vol, vol0, stress, F, J, cmat, dinv;
+    std::vector<int32_t> id, matid, surf;
+};
+
+namespace {
+
+// OutputVector rows (module/data_io.h:49-60): width w per value
+void put_rows(std::ofstream &o, const double *a, int64_t n, int per, int w) {
+    for (int64_t i = 0; i < n; i++) {
+        for (int q = 0; q < per; q++) o << std::setw(w) << (a ? a[per * i + q] : 0.0);
+        o << "\n";
+    }
+}
+
+// InputVector rows (module/data_io.h:18-38): per values, then the rest of the line
+bool get_rows(Reader &r, std::vector<double> *a, int64_t n, int per) {
+    if (a) a->resize((size_t)(per * n));
+    for (int64_t i = 0; i < n; i++) {
+        for (int q = 0; q < per; q++) {
+            double v = 0.0;
+            r.get(v);
+            if (a) (*a)[per * i + q] = v;
+        }
+        r.skip_line();
+    }
+    return r.ok;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_restart_save(const char *path, const mpm_particles_view *p, int32_t scheme, const double *cmat,
+                     const double *dinv, const int32_t *id, const int32_t *surf, int32_t digits) {
+    if (!path || !p || p->n < 0 || (p->n > 0 && (!p->coord || !p->mass || !p->vol0)))
+        return fail("restart needs a path and a view with coord, mass, vol0");
+    if (scheme < MPM_SCHEME_PIC || scheme > MPM_SCHEME_APIC) return fail("unknown transfer scheme");
+    if (digits != 0 && (digits < 1 || digits > 17)) return fail("digits must be 0 or 1..17");
+    std::ofstream o;
+    o.flags(std::ios::right | std::ios::scientific);  // solid_data_io.cpp:112-113
+    o.open(path);
+    if (!o.is_open()) return fail(std::string("cannot write ") + path);
+    const int w = digits ? digits + 9 : 15;
+    if (digits) o << std::setprecision(digits - 1);
+    const int64_t n = p->n;
+    o << n << "     --- Number of solid material point --- " << "\n";
+    if (n) {
+        put_rows(o, p->coord, n, 3, w);
+        for (int64_t i = 0; i < n; i++) {
+            double vol = p->vol ? p->vol[i] : p->vol0[i];
+            o << (id ? id[i] : (int32_t)i) << std::setw(15) << (p->matid ? p->matid[i] : 0) << std::setw(15)
+              << (surf ? surf[i] : 0) << std::setw(w) << p->mass[i] << std::setw(w) << p->vol0[i] << std::setw(w)
+              << vol << "\n";
+        }
+        put_rows(o, p->vel, n, 3, w);
+        put_rows(o, nullptr, n, 3, w);  // accel
+        put_rows(o, p->stress, n, 6, w);
+        if (p->def_grad) {
+            put_rows(o, p->def_grad, n, 9, w);
+        } else {
+            std::vector<double> eye(9 * (size_t)n, 0.0);
+            for (int64_t i = 0; i < n; i++) eye[9 * i] = eye[9 * i + 4] = eye[9 * i + 8] = 1.0;
+            put_rows(o, eye.data(), n, 9, w);
+        }
+        if (scheme == MPM_SCHEME_TPIC) {
+            put_rows(o, cmat, n, 9, w);
+            put_rows(o, nullptr, n, 9, w);  // accel_grad
+        } else if (scheme == MPM_SCHEME_APIC) {
+            put_rows(o, cmat, n, 9, w);
+            put_rows(o, nullptr, n, 9, w);  // accel_Bmat
+            put_rows(o, dinv, n, 9, w);
+        }
+    }
+    if (!o.good()) return fail(std::string("write failed: ") + path);
+    return 0;
+}
+
+int mpm_restart_load(const char *path, int32_t scheme, mpm_restart **out) {
+    if (!path || !out) return fail("null argument");
+    *out = nullptr;
+    if (scheme < MPM_SCHEME_PIC || scheme > MPM_SCHEME_APIC) return fail("unknown transfer scheme");
+    Reader r(path);
+    if (!r.ok) return fail(std::string("cannot open ") + path);
+    long long num = 0;
+    r.get(num);
+    r.skip_line();
+    if (!r.ok || num < 0 || num >= (1ll << 31)) return fail(std::string("bad point count in ") + path);
+    auto *R = new mpm_restart();
+    R->scheme = scheme;
+    const int64_t n = num;
+    bool ok = get_rows(r, &R->coord, n, 3);
+    R->id.resize(n);
+    R->matid.resize(n);
+    R->surf.resize(n);
+    R->mass.resize(n);
+    R->vol0.resize(n);
+    R->vol.resize(n);
+    R->J.resize(n);
+    for (int64_t i = 0; ok && i < n; i++) {
+        r.get(R->id[i]);
+        r.get(R->matid[i]);
+        r.get(R->surf[i]);
+        r.get(R->mass[i]);
+        r.get(R->vol0[i]);
+        r.get(R->vol[i]);
+        r.skip_line();
+        R->J[i] = R->vol[i] / R->vol0[i];  // solid_data_io.cpp:79
+        ok = r.ok;
+    }
+    ok = ok && get_rows(r, &R->vel, n, 3) && get_rows(r, nullptr, n, 3) && get_rows(r, &R->stress, n, 6) &&
+         get_rows(r, &R->F, n, 9);
+    if (ok && scheme == MPM_SCHEME_TPIC) ok = get_rows(r, &R->cmat, n, 9) && get_rows(r, nullptr, n, 9);
+    if (ok && scheme == MPM_SCHEME_APIC)
+        ok = get_rows(r, &R->cmat, n, 9) && get_rows(r, nullptr, n, 9) && get_rows(r, &R->dinv, n, 9);
+    if (!ok) {
+        delete R;
+        return fail(std::string("malformed restart file ") + path);
+    }
+    *out = R;
+    return 0;
+}
+
+int mpm_restart_view(const mpm_restart *R, mpm_particles_view *v, const double **cmat, const double **dinv,
+                     const int32_t **id, const int32_t **surf) {
+    if (!R || !v) return fail("null argument");
+    auto *m = const_cast<mpm_restart *>(R);
+    std::memset(v, 0, sizeof *v);
+    v->n = (int64_t)R->mass.size();
+    v->coord = m->coord.data();
+    v->vel = m->vel.data();
+    v->mass = m->mass.data();
+    v->vol = m->vol.data();
+    v->vol0 = m->vol0.data();
+    v->stress = m->stress.data();
+    v->def_grad = m->F.data();
+    v->det_def_grad = m->J.data();
+    v->matid = m->matid.data();
+    if (cmat) *cmat = R->cmat.empty() ? nullptr : R->cmat.data();
+    if (dinv) *dinv = R->dinv.empty() ? nullptr : R->dinv.data();
+    if (id) *id = R->id.data();
+    if (surf) *surf = R->surf.data();
+    return 0;
+}
+
+void mpm_restart_free(mpm_restart *r) { delete r; }
+
+int mpm_restart_write(mpm_engine *eng, const char *path, int32_t digits) {
+    mpm_engine_info info;
+    int rc = mpm_get_engine_info(eng, &info);
+    if (rc) return rc;
+    if (info.n <= 0) return fail("no particles uploaded");
+    const size_t n = (size_t)info.n;
+    std::vector<double> x(3 * n), v(3 * n), m(n), V(n), V0(n), sig(6 * n), F(9 * n), J(n), cm, di;
+    std::vector<int32_t> mid(n);
+    mpm_particles_view p{};
+    p.n = info.n;
+    p.coord = x.data();
+    p.vel = v.data();
+    p.mass = m.data();
+    p.vol = V.data();
+    p.vol0 = V0.data();
+    p.stress = sig.data();
+    p.def_grad = F.data();
+    p.det_def_grad = J.data();
+    p.matid = mid.data();
+    if ((rc = mpm_download_particles(eng, &p))) return rc;
+    if (info.scheme >= MPM_SCHEME_TPIC) {
+        cm.resize(9 * n);
+        if (info.scheme == MPM_SCHEME_APIC) di.resize(9 * n);
+        if ((rc = mpm_download_affine(eng, cm.data(), di.empty() ? nullptr : di.data()))) return rc;
+    }
+    return mpm_restart_save(path, &p, info.scheme, cm.empty() ? nullptr : cm.data(),
+                            di.empty() ? nullptr : di.data(), nullptr, nullptr, digits);
+}
+
+int mpm_restart_read(mpm_engine *eng, const char *path) {
+    mpm_engine_info info;
+    int rc = mpm_get_engine_info(eng, &info);
+    if (rc) return rc;
+    mpm_restart *R = nullptr;
+    if ((rc = mpm_restart_load(path, info.scheme, &R))) return rc;
+    mpm_particles_view p;
+    const double *cm = nullptr, *di = nullptr;
+    rc = mpm_restart_view(R, &p, &cm, &di, nullptr, nullptr);
+    if (!rc) rc = mpm_upload_particles(eng, &p);
+    if (!rc && info.scheme >= MPM_SCHEME_TPIC) rc = mpm_upload_affine(eng, cm, di);
+    mpm_restart_free(R);
+    return rc;
+}
+
+}  // extern "C"

@@ -976,6 +976,18 @@ int mpm_upload_affine(mpm_engine *e, const double *cmat, const double *dinv) {
     return affine_copy(e, const_cast<double *>(cmat), const_cast<double *>(dinv), true);
 }
 
+int mpm_get_engine_info(mpm_engine *e, mpm_engine_info *out) {
+    if (!e || !out) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    out->n = e->n;
+    out->nelem = e->g.nelem;
+    out->nodec = e->g.nodec;
+    out->precision = e->prec;
+    out->scheme = e->prm.scheme;
+    out->shifting = e->shifting ? 1 : 0;
+    out->fbar = e->fbar ? 1 : 0;
+    return 0;
+}
+
 int mpm_engine_destroy(mpm_engine *eng) {
     if (!eng) return 0;
     (void)hipStreamSynchronize(eng->stream);

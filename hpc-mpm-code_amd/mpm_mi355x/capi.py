@@ -86,6 +86,11 @@ class TuneResult(C.Structure):
                 ("scenes", C.c_int32), ("ms_per_step", C.c_double), ("ms_per_step_worst", C.c_double)]
 
 
+class EngineInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nelem", C.c_int32), ("nodec", C.c_int32), ("precision", C.c_int32),
+                ("scheme", C.c_int32), ("shifting", C.c_int32), ("fbar", C.c_int32)]
+
+
 class CaseDesc(C.Structure):
     _fields_ = [("grid", GridDesc), ("step", StepParams), ("ista", C.c_int32), ("iend", C.c_int32),
                 ("iout", C.c_int32), ("rstflag", C.c_int32), ("npxye", C.c_int32 * 3), ("nmat", C.c_int32),
@@ -153,6 +158,14 @@ SIGNATURES = {
     "mpm_case_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "mpm_case_engine": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "mpm_case_free": (None, [C.c_void_p]),
+    "mpm_restart_save": (C.c_int, [C.c_char_p, C.POINTER(ParticlesView), C.c_int32, _dp, _dp, _ip, _ip, C.c_int32]),
+    "mpm_restart_load": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "mpm_restart_view": (C.c_int, [C.c_void_p, C.POINTER(ParticlesView), C.POINTER(_dp), C.POINTER(_dp),
+                                   C.POINTER(_ip), C.POINTER(_ip)]),
+    "mpm_restart_free": (None, [C.c_void_p]),
+    "mpm_restart_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "mpm_restart_read": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "mpm_get_engine_info": (C.c_int, [C.c_void_p, C.POINTER(EngineInfo)]),
 }
 
 MLP_NPARAMS = 6288
@@ -326,6 +339,17 @@ class Engine:
         cm = np.ascontiguousarray(cmat, np.float64)
         di = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
         check(self.lib.mpm_upload_affine(self.h, _ptr(cm), _ptr(di)))
+
+    def info(self):
+        i = EngineInfo()
+        check(self.lib.mpm_get_engine_info(self.h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in EngineInfo._fields_}
+
+    def restart_write(self, path, digits=0):
+        check(self.lib.mpm_restart_write(self.h, _path(path), digits))
+
+    def restart_read(self, path):
+        check(self.lib.mpm_restart_read(self.h, _path(path)))
 
     def set_fbar(self, on=True):
         check(self.lib.mpm_set_fbar(self.h, int(on)))
@@ -574,6 +598,37 @@ class Case:
                          nlstep=d.step.nlstep, bcs=self.bcs(), parts=self.particles(), idimc=tuple(g.idimc),
                          npxye=tuple(d.npxye))
         return s
+
+
+def restart_save(path, parts, scheme=SCHEME_PIC, cmat=None, dinv=None, ids=None, surf=None, digits=0):
+    """Writes a reference restart file (include/mpm/mpm_case.h) from host arrays."""
+    lib = load()
+    ids = None if ids is None else np.ascontiguousarray(ids, np.int32)
+    surf = None if surf is None else np.ascontiguousarray(surf, np.int32)
+    cm = None if cmat is None else np.ascontiguousarray(cmat, np.float64)
+    di = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
+    check(lib.mpm_restart_save(_path(path), C.byref(parts.view()), scheme, _ptr(cm), _ptr(di),
+                               _ptr(ids, C.c_int32), _ptr(surf, C.c_int32), digits))
+
+
+def restart_load(path, scheme=SCHEME_PIC):
+    """Reads a reference restart file: (Particles, cmat or None, dinv or None, ids, surf)."""
+    lib = load()
+    h = C.c_void_p()
+    check(lib.mpm_restart_load(_path(path), scheme, C.byref(h)))
+    try:
+        v = ParticlesView()
+        cm, di, ids, surf = _dp(), _dp(), _ip(), _ip()
+        check(lib.mpm_restart_view(h, C.byref(v), C.byref(cm), C.byref(di), C.byref(ids), C.byref(surf)))
+        n = int(v.n)
+        a = lambda ptr, shape: np.ctypeslib.as_array(ptr, shape=shape).copy() if n and ptr else None
+        parts = Particles(n, coord=a(v.coord, (n, 3)), vel=a(v.vel, (n, 3)), mass=a(v.mass, (n,)),
+                          vol0=a(v.vol0, (n,)), vol=a(v.vol, (n,)), stress=a(v.stress, (n, 6)),
+                          def_grad=a(v.def_grad, (n, 3, 3)), det_def_grad=a(v.det_def_grad, (n,)),
+                          matid=a(v.matid, (n,)))
+        return parts, a(cm, (n, 3, 3)), a(di, (n, 3, 3)), a(ids, (n,)), a(surf, (n,))
+    finally:
+        lib.mpm_restart_free(h)
 
 
 def mlp_forward(params, scaling, inp):

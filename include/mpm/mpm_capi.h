@@ -164,6 +164,16 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
                       const mpm_step_params *params, mpm_engine **out);
 int mpm_engine_destroy(mpm_engine *eng);
 
+/* What an engine holds: particle count (0 before an upload), grid sizes and
+ * the run options. */
+typedef struct mpm_engine_info {
+    int64_t n;
+    int32_t nelem, nodec;
+    int32_t precision, scheme;
+    int32_t shifting, fbar;
+} mpm_engine_info;
+int mpm_get_engine_info(mpm_engine *eng, mpm_engine_info *out);
+
 /* Node-list Dirichlet BC; comp 0/1/2 = usbc/vsbc/wsbc, 3 = rigid_bc (all
  * three components). Semantics of BoundaryCondition::BCSetVal/BCSetZero
  * (module/bc.cpp:45-67): velocity set to fbc*facl, acceleration zeroed.

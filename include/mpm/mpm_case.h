@@ -56,6 +56,31 @@ int mpm_case_engine(const mpm_case *c, int32_t precision, int32_t relayout_every
                     mpm_engine **out);
 void mpm_case_free(mpm_case *c);
 
+/* Restart files (SolidMaterialPointBase::RestartOutput / RestartInput,
+ * module/solid/solid_data_io.cpp:61-141; one rank's <pointfile><rank>_re.txt):
+ * count line; coordinates; "id matid surf mass vol0 vol" per point; vel;
+ * accel; stress; def_grad; then vel_grad, accel_grad (TPIC) or vel_Bmat,
+ * accel_Bmat, inv_Dmat (APIC). det F is read back as vol / vol0. The
+ * explicit step keeps no particle acceleration: it is written as zero and
+ * ignored on reading, as are the acceleration matrices.
+ * digits = 0 writes the reference's layout exactly (setw(15), scientific,
+ * 6 digits: 7 significant digits, as the reference's own restarts);
+ * digits = 17 widens the fields to 26 characters so that a restart
+ * continues bit for bit. */
+typedef struct mpm_restart mpm_restart;
+int mpm_restart_save(const char *path, const mpm_particles_view *p, int32_t scheme, const double *cmat,
+                     const double *dinv, const int32_t *id, const int32_t *surf, int32_t digits);
+int mpm_restart_load(const char *path, int32_t scheme, mpm_restart **out);
+/* Pointers into the restart's arrays (valid until mpm_restart_free): every
+ * field of the view but trac_force and eps_p; cmat/dinv [n][3][3] or NULL
+ * for the scheme; id, surf [n]. */
+int mpm_restart_view(const mpm_restart *r, mpm_particles_view *p, const double **cmat, const double **dinv,
+                     const int32_t **id, const int32_t **surf);
+void mpm_restart_free(mpm_restart *r);
+/* Engine-level: download and save / load and upload (matrices included). */
+int mpm_restart_write(mpm_engine *eng, const char *path, int32_t digits);
+int mpm_restart_read(mpm_engine *eng, const char *path);
+
 #ifdef __cplusplus
 }
 #endif

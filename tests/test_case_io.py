@@ -120,3 +120,75 @@ def test_transfer_scheme_names_and_bad_matid(tmp_path):
     sp.write_text("%10d\n%15.6e%15.6e%15.6e\n%10d%10d%15.6e%15.6e\n" % (1, 1.0, 0.25, 4.0, 0, 3, 1.0, 1.0))
     with pytest.raises(capi.MPMError):
         capi.Case(beam_case.BEAM_INPUT, None, str(sp))
+
+
+# ------------------------------------------------------------------ restart files (solid_data_io.cpp:61-141)
+def _state(n=7, seed=4):
+    rng = np.random.default_rng(seed)
+    return capi.Particles(n, coord=rng.random((n, 3)), vel=rng.normal(size=(n, 3)), mass=1 + rng.random(n),
+                          vol0=1e-3 * (1 + rng.random(n)), vol=1e-3 * (1 + rng.random(n)),
+                          stress=rng.normal(size=(n, 6)) * 1e4, def_grad=np.eye(3) + 0.1 * rng.normal(size=(n, 3, 3)),
+                          matid=rng.integers(0, 3, n))
+
+
+def _reference_layout(p, scheme, cmat, dinv):
+    """RestartOutput's text: ios::right | ios::scientific, default precision."""
+    e = lambda v: "%15.6e" % v
+    rows = lambda a, k: "".join("".join(e(v) for v in r) + "\n" for r in np.asarray(a).reshape(-1, k))
+    n = p.n
+    t = "%d     --- Number of solid material point --- \n" % n
+    t += rows(p.coord, 3)
+    for i in range(n):
+        t += "%d%15d%15d%s%s%s\n" % (i, p.matid[i], 0, e(p.mass[i]), e(p.vol0[i]), e(p.vol[i]))
+    t += rows(p.vel, 3) + rows(np.zeros((n, 3)), 3) + rows(p.stress, 6) + rows(p.def_grad, 9)
+    if scheme == capi.SCHEME_TPIC:
+        t += rows(cmat, 9) + rows(np.zeros((n, 9)), 9)
+    if scheme == capi.SCHEME_APIC:
+        t += rows(cmat, 9) + rows(np.zeros((n, 9)), 9) + rows(dinv, 9)
+    return t
+
+
+@pytest.mark.parametrize("scheme", [capi.SCHEME_FLIP, capi.SCHEME_TPIC, capi.SCHEME_APIC])
+def test_restart_writer_is_the_reference_layout(tmp_path, scheme):
+    p = _state()
+    rng = np.random.default_rng(1)
+    cm, di = rng.normal(size=(p.n, 3, 3)), rng.normal(size=(p.n, 3, 3))
+    f = tmp_path / "sp0_re.txt"
+    capi.restart_save(str(f), p, scheme, cm, di if scheme == capi.SCHEME_APIC else None)
+    assert f.read_text() == _reference_layout(p, scheme, cm, di)
+
+
+@pytest.mark.parametrize("scheme", [capi.SCHEME_PIC, capi.SCHEME_APIC])
+def test_restart_round_trip(tmp_path, scheme):
+    p = _state()
+    rng = np.random.default_rng(2)
+    cm, di = rng.normal(size=(p.n, 3, 3)), rng.normal(size=(p.n, 3, 3))
+    ids, surf = np.arange(p.n)[::-1].copy(), rng.integers(0, 2, p.n)
+    full, short = tmp_path / "full.txt", tmp_path / "short.txt"
+    capi.restart_save(str(full), p, scheme, cm, di, ids, surf, digits=17)
+    capi.restart_save(str(short), p, scheme, cm, di, ids, surf, digits=0)
+    q, qcm, qdi, qid, qsurf = capi.restart_load(str(full), scheme)
+    for fld in ("coord", "vel", "mass", "vol0", "vol", "stress", "def_grad", "matid"):
+        assert np.array_equal(getattr(q, fld), getattr(p, fld)), fld
+    assert np.array_equal(q.det_def_grad, p.vol / p.vol0)  # solid_data_io.cpp:79
+    assert np.array_equal(qid, ids) and np.array_equal(qsurf, surf)
+    if scheme == capi.SCHEME_APIC:
+        assert np.array_equal(qcm, cm) and np.array_equal(qdi, di)
+    else:
+        assert qcm is None and qdi is None
+    r = capi.restart_load(str(short), scheme)[0]
+    np.testing.assert_allclose(r.coord, p.coord, rtol=5e-7)          # 7 significant digits
+    assert not np.array_equal(r.coord, p.coord)
+
+
+def test_restart_errors(tmp_path):
+    with pytest.raises(capi.MPMError):
+        capi.restart_load(str(tmp_path / "missing.txt"))
+    f = tmp_path / "bad.txt"
+    f.write_text("3     --- Number of solid material point --- \n1 2 3\n")
+    with pytest.raises(capi.MPMError):
+        capi.restart_load(str(f))
+    p = _state(3)
+    capi.restart_save(str(f), p, capi.SCHEME_PIC)
+    with pytest.raises(capi.MPMError):   # an APIC reader expects the matrix blocks
+        capi.restart_load(str(f), capi.SCHEME_APIC)

@@ -587,3 +587,26 @@ def test_fp32_rejects_fbar():
     with pytest.raises(capi.MPMError) as ei:
         e.set_fbar(True)
     assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
+
+
+# ------------------------------------------------------------------ restart files
+@pytest.mark.parametrize("scheme", [capi.SCHEME_FLIP, capi.SCHEME_APIC])
+def test_restart_file_continues_a_run(tmp_path, scheme):
+    """RestartOutput / RestartInput layout (solid_data_io.cpp:61-141) written
+    with 17 digits: the run read back continues bit for bit."""
+    s = scenes.tiny(n_cells=8, model=capi.MODEL_LINELAST, props=(1e6, 0.3), scheme=scheme)
+    a = engine(s, capi.PREC_FP64_EXACT)
+    assert a.step(1, 10) == 0
+    f = str(tmp_path / "sp0_re.txt")
+    a.restart_write(f, digits=17)
+    assert a.step(11, 10) == 0
+    b = capi.Engine(s, precision=capi.PREC_FP64_EXACT)
+    b.restart_read(f)
+    assert b.info()["n"] == s.n
+    assert b.step(11, 10) == 0
+    assert_parts_equal(b.download(s.n), a.download(s.n))
+    # the reference's own 7-digit layout restarts close to the run
+    a.restart_write(f, digits=0)
+    c = capi.Engine(s, precision=capi.PREC_FP64_EXACT)
+    c.restart_read(f)
+    np.testing.assert_allclose(c.download(s.n).coord, a.download(s.n).coord, rtol=1e-6)
