@@ -47,7 +47,15 @@ static int bit_length(uint32_t v) {
 
 using namespace mpm;
 
-enum { OPT_FULL_NODE_OUTPUT = 1, OPT_LANES_PER_CELL = 2, OPT_RELAYOUT_EVERY = 3, OPT_GRAPH_BLOCK = 4, OPT_FUSED_COUNT = 5 };
+enum {
+    OPT_FULL_NODE_OUTPUT = 1,
+    OPT_LANES_PER_CELL = 2,
+    OPT_RELAYOUT_EVERY = 3,
+    OPT_GRAPH_BLOCK = 4,
+    OPT_FUSED_COUNT = 5,
+    OPT_CELLS_PREFETCH = 6,
+    OPT_CELLS_GRID = 7
+};
 
 struct mpm_engine {
     GridGeom g{};
@@ -79,6 +87,8 @@ struct mpm_engine {
     NodeTables T{};
     float m_small = 0.f;
     int lanes_per_cell = 1;  // from the particles per non-empty cell at upload
+    int cells_prefetch = 1;
+    int cells_grid = 0;
     int fp32_outside_at_upload = 0;
 
     uint32_t *keys64 = nullptr;  // FP64 mode cell keys by particle
@@ -250,6 +260,8 @@ StepConsts consts(const mpm_engine *e, double facl) {
     c.m_small = e->m_small;
     c.scheme = e->prm.scheme;
     c.lanes_per_cell = e->lanes_per_cell;
+    c.cells_prefetch = e->cells_prefetch;
+    c.cells_grid = e->cells_grid;
     return c;
 }
 
@@ -1529,6 +1541,15 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
         if (e->prec != MPM_PREC_FP32) return value ? set_error(MPM_ERR_UNSUPPORTED, "relayout is an FP32 option") : 0;
         e->relayout_every = (int)value;
         e->steps_since_relayout = 0;
+        return 0;
+    case OPT_CELLS_PREFETCH:
+        e->drop_graph();
+        e->cells_prefetch = value ? 1 : 0;
+        return 0;
+    case OPT_CELLS_GRID:
+        if (value < 0 || value > (1 << 20)) return set_error(MPM_ERR_INVALID_ARGUMENT, "cells grid out of range");
+        e->drop_graph();
+        e->cells_grid = (int)value;
         return 0;
     case OPT_FUSED_COUNT:
         e->clear_counts();

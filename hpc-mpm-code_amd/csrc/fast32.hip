@@ -100,14 +100,27 @@ __device__ __forceinline__ uint32_t first_pos(uint32_t cs, uint32_t lo, int sub)
     return f;
 }
 
-// combine the L lanes of a cell (fixed xor butterfly; every lane ends with the
-// same, order-independent value)
-template <int L>
-__device__ __forceinline__ float lane_combine(float v) {
+// Sums NV values over the L lanes of a cell and leaves lane sub with the
+// totals of entries k L + sub at v[k], k < NV / L. Each level exchanges half
+// of the remaining entries with the lane O apart; a total is
+// ((a0 + a1) + (a2 + a3)) ... in lane order, as a xor butterfly would give.
+template <int O, int L, int N>
+struct ReduceScatter {
+    static __device__ __forceinline__ void run(float *v, int sub) {
+        const bool hi = (sub & O) != 0;
 #pragma unroll
-    for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+        for (int k = 0; k < N / 2; k++) {
+            const float keep = hi ? v[2 * k + 1] : v[2 * k];
+            const float send = hi ? v[2 * k] : v[2 * k + 1];
+            v[k] = keep + __shfl_xor(send, O, 64);
+        }
+        ReduceScatter<O * 2, L, N / 2>::run(v, sub);
+    }
+};
+template <int L, int N>
+struct ReduceScatter<L, L, N> {
+    static __device__ __forceinline__ void run(float *, int) {}
+};
 
 // ------------------------------------------------------------------ P2G
 // Particle fields are gathered (through the sorted order) straight from HBM
@@ -133,7 +146,17 @@ __device__ __forceinline__ void lds_gather2(const double *field_base, uint32_t s
 // wait for every outstanding vector-memory load (including LDS DMA) of this wave
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
 
-template <int LPC, bool DIAG, bool TRAC>
+// slot indices of the chunk starting at sorted position base (nch particles)
+__device__ __forceinline__ void load_slots(const CellLists &L, uint32_t base, int nch, uint32_t slot[kBlk]) {
+#pragma unroll
+    for (int r = 0; r < kBlk; r++) slot[r] = (64 * r + (int)threadIdx.x < nch) ? L.perm[base + 64 * r + threadIdx.x] : 0u;
+}
+
+// PF: software prefetch. The next group's cell ranges are requested at the
+// start of a group and the next chunk's slot indices while the current
+// chunk's fields are in flight, so that a chunk waits for one memory round
+// trip instead of three (setup -> slots -> fields).
+template <int LPC, bool DIAG, bool TRAC, bool PF>
 __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T,
                                                        float *__restrict__ part) {
     constexpr int G = kSeg / LPC;
@@ -146,26 +169,42 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
     const int t = threadIdx.x;
     const size_t ld = P.ld;
     const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
-    for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-        CellInfo si;
-        uint32_t S, E;
-        __syncthreads();
-        group_setup<LPC>(g, L, n_active, grp, si, S, E);
-        __syncthreads();
-        float acc[8][7];
+    int grp = blockIdx.x;
+    if (grp >= ngroups) return;
+    CellInfo si;
+    uint32_t S, E;
+    group_setup<LPC>(g, L, n_active, grp, si, S, E);
+    uint32_t slot[kBlk];
+    if (PF) load_slots(L, S, (int)min((uint32_t)kChunk, E - S), slot);
+    for (; grp < ngroups;) {
+        const int grp_n = grp + (int)gridDim.x;
+        CellInfo si_n;
+        uint32_t S_n = 0, E_n = 0;
+        if (PF && grp_n < ngroups) group_setup<LPC>(g, L, n_active, grp_n, si_n, S_n, E_n);
+        // mass and momentum per corner; internal force as moments of -V sigma
+        // against the weight products (FX[a][j + 2k] = sum S_ax wy wz, FY over
+        // wx wz, FZ over wx wy) so that one multiply-add per particle covers
+        // all corners; corners take them with the sign of dN/dx at the end
+        float acc[8][4];
+        float FX[3][4], FY[3][4], FZ[3][4];
+        float tr[8][3];
         float vof[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) {
 #pragma unroll
-            for (int f = 0; f < 7; f++) acc[q][f] = 0.f;
+            for (int f = 0; f < 4; f++) acc[q][f] = 0.f;
+#pragma unroll
+            for (int a = 0; a < 3; a++) tr[q][a] = 0.f;
             vof[q] = 0.f;
         }
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) FX[a][u] = FY[a][u] = FZ[a][u] = 0.f;
 
         for (uint32_t base = S; base < E; base += kChunk) {
             const int nch = (int)min((uint32_t)kChunk, E - base);
-            uint32_t slot[kBlk];
-#pragma unroll
-            for (int r = 0; r < kBlk; r++) slot[r] = (64 * r + t < nch) ? L.perm[base + 64 * r + t] : 0u;
+            if (!PF) load_slots(L, base, nch, slot);
 #pragma unroll
             for (int r = 0; r < kBlk; r++) {
                 if (64 * r + t < nch) {
@@ -184,6 +223,13 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                         for (int a = 0; a < 3; a++) lds_gather(P.trac + a * ld, sl, stage, 15 + a, r);
                     }
                 }
+            }
+            if (PF) {  // slots of the next chunk: later in this group or the first of the next group
+                const uint32_t nb = base + kChunk;
+                if (nb < E)
+                    load_slots(L, nb, (int)min((uint32_t)kChunk, E - nb), slot);
+                else if (grp_n < ngroups)
+                    load_slots(L, S_n, (int)min((uint32_t)kChunk, E_n - S_n), slot);
             }
             wait_vm_all();
             __syncthreads();
@@ -222,6 +268,17 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                         wxz[u][w] = wx[u] * wz[w];
                         wxy[u][w] = wx[u] * wy[w];
                     }
+                const float Xa[3] = {X0, X1, X2}, Ya[3] = {Y0, Y1, Y2}, Za[3] = {Z0, Z1, Z2};
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+#pragma unroll
+                        for (int w = 0; w < 2; w++) {
+                            FX[a][u + 2 * w] += Xa[a] * wyz[u][w];
+                            FY[a][u + 2 * w] += Ya[a] * wxz[u][w];
+                            FZ[a][u + 2 * w] += Za[a] * wxy[u][w];
+                        }
 #pragma unroll
                 for (int k = 0; k < 2; k++)
 #pragma unroll
@@ -230,45 +287,55 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                         for (int ii = 0; ii < 2; ii++) {
                             const int q = ii + 2 * j + 4 * k;
                             const float N = wx[ii] * wyz[j][k];
-                            const float gx = ii ? wyz[j][k] : -wyz[j][k];
-                            const float gy = j ? wxz[ii][k] : -wxz[ii][k];
-                            const float gz = k ? wxy[ii][j] : -wxy[ii][j];
                             acc[q][0] += N * m;
                             acc[q][1] += N * px;
                             acc[q][2] += N * py;
                             acc[q][3] += N * pz;
-                            acc[q][4] += X0 * gx + Y0 * gy + Z0 * gz;
-                            acc[q][5] += X1 * gx + Y1 * gy + Z1 * gz;
-                            acc[q][6] += X2 * gx + Y2 * gy + Z2 * gz;
                             if (TRAC) {
-                                acc[q][4] += N * tx;
-                                acc[q][5] += N * ty;
-                                acc[q][6] += N * tz;
+                                tr[q][0] += N * tx;
+                                tr[q][1] += N * ty;
+                                tr[q][2] += N * tz;
                             }
                             if (DIAG) vof[q] += N * (float)Vd;
                         }
             }
             __syncthreads();
         }
-        if (LPC > 1) {
+        // partial of entry q * 7 + f (f: m, p, force) and 56 + q (volume)
+        constexpr int NV = DIAG ? 64 : 56;
+        float v[NV];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
+        for (int k = 0; k < 2; k++)
 #pragma unroll
-                for (int f = 0; f < 7; f++) acc[q][f] = lane_combine<LPC>(acc[q][f]);
-                if (DIAG) vof[q] = lane_combine<LPC>(vof[q]);
-            }
-        }
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int ii = 0; ii < 2; ii++) {
+                    const int q = ii + 2 * j + 4 * k;
+#pragma unroll
+                    for (int f = 0; f < 4; f++) v[q * 7 + f] = acc[q][f];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        const float fx_ = ii ? FX[a][j + 2 * k] : -FX[a][j + 2 * k];
+                        const float fy_ = j ? FY[a][ii + 2 * k] : -FY[a][ii + 2 * k];
+                        const float fz_ = k ? FZ[a][ii + 2 * j] : -FZ[a][ii + 2 * j];
+                        v[q * 7 + 4 + a] = (fx_ + fy_) + fz_ + (TRAC ? tr[q][a] : 0.f);
+                    }
+                    if (DIAG) v[56 + q] = vof[q];
+                }
+        ReduceScatter<1, LPC, NV>::run(v, sub);
         if (si.ce > si.cs) {
             const size_t ne = (size_t)g.nelem;
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-#pragma unroll
-                for (int f = 0; f < 7; f++)
-                    if ((q * 7 + f) % LPC == sub) part[(size_t)(q * 7 + f) * ne + si.c] = acc[q][f];
-            if (DIAG) {
-#pragma unroll
-                for (int q = 0; q < 8; q++)
-                    if (q % LPC == sub) part[(size_t)(56 + q) * ne + si.c] = vof[q];
+            for (int k = 0; k < NV / LPC; k++) part[(size_t)(k * LPC + sub) * ne + si.c] = v[k];
+        }
+        grp = grp_n;
+        if (grp < ngroups) {
+            if (PF) {
+                si = si_n;
+                S = S_n;
+                E = E_n;
+            } else {
+                group_setup<LPC>(g, L, n_active, grp, si, S, E);
             }
         }
     }  // groups
@@ -348,12 +415,18 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
     NodeCells nc;
     const bool any = node_cells8(g, L, i, j, k, nc);
     if (any) {
+        // loads only from cells that hold particles: a tile's nodes next to
+        // empty cells fetch nothing for them
         float val[8][NQ];
 #pragma unroll
         for (int q = 0; q < 8; q++) {
 #pragma unroll
-            for (int f = 0; f < 7; f++) val[q][f] = part[(size_t)(nc.corner[q] * 7 + f) * ne + nc.c[q]];
-            if (DIAG) val[q][7] = part[(size_t)(56 + nc.corner[q]) * ne + nc.c[q]];
+            for (int f = 0; f < NQ; f++) val[q][f] = 0.f;
+            if (nc.on[q]) {
+#pragma unroll
+                for (int f = 0; f < 7; f++) val[q][f] = part[(size_t)(nc.corner[q] * 7 + f) * ne + nc.c[q]];
+                if (DIAG) val[q][7] = part[(size_t)(56 + nc.corner[q]) * ne + nc.c[q]];
+            }
         }
 #pragma unroll
         for (int q = 0; q < 8; q++)
@@ -541,19 +614,16 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
             }
             __syncthreads();
         }
-        if (LPC > 1) {
+        float v[24];
 #pragma unroll
-            for (int q = 0; q < 8; q++)
+        for (int q = 0; q < 8; q++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) acc[q][c] = lane_combine<LPC>(acc[q][c]);
-        }
+            for (int c = 0; c < 3; c++) v[q * 3 + c] = acc[q][c];
+        ReduceScatter<1, LPC, 24>::run(v, sub);
         if (ne) {
             const size_t nel = (size_t)g.nelem;
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    if ((q * 3 + c) % LPC == sub) part[(size_t)(q * 3 + c) * nel + si.c] = acc[q][c];
+            for (int k = 0; k < 24 / LPC; k++) part[(size_t)(k * LPC + sub) * nel + si.c] = v[k];
         }
     }  // groups
 }
@@ -571,9 +641,14 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, T
     if (any) {
         float val[8][3];
 #pragma unroll
-        for (int q = 0; q < 8; q++)
+        for (int q = 0; q < 8; q++) {
 #pragma unroll
-            for (int f = 0; f < 3; f++) val[q][f] = part[(size_t)(nc.corner[q] * 3 + f) * ne + nc.c[q]];
+            for (int f = 0; f < 3; f++) val[q][f] = 0.f;
+            if (nc.on[q]) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) val[q][f] = part[(size_t)(nc.corner[q] * 3 + f) * ne + nc.c[q]];
+            }
+        }
 #pragma unroll
         for (int q = 0; q < 8; q++)
             if (nc.on[q]) {
@@ -1155,32 +1230,45 @@ static int cell_groups_grid(const GridGeom &g) {
     return ng < 8192 ? ng : 8192;
 }
 
-template <int LPC>
-static void p2g_cells_launch(hipStream_t s, int ng, bool diag, bool trac, const GridGeom &g, const PartF32 &P,
-                             const CellLists &L, const NodeTables &T, float *partials) {
+template <int LPC, bool PF>
+static void p2g_cells_launch_pf(hipStream_t s, int ng, bool diag, bool trac, const GridGeom &g, const PartF32 &P,
+                                const CellLists &L, const NodeTables &T, float *partials) {
     if (diag) {
         if (trac)
-            hipLaunchKernelGGL((k32_p2g_cells<LPC, true, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
+            hipLaunchKernelGGL((k32_p2g_cells<LPC, true, true, PF>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
         else
-            hipLaunchKernelGGL((k32_p2g_cells<LPC, true, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
+            hipLaunchKernelGGL((k32_p2g_cells<LPC, true, false, PF>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T,
+                               partials);
     } else {
         if (trac)
-            hipLaunchKernelGGL((k32_p2g_cells<LPC, false, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
+            hipLaunchKernelGGL((k32_p2g_cells<LPC, false, true, PF>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T,
+                               partials);
         else
-            hipLaunchKernelGGL((k32_p2g_cells<LPC, false, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials);
+            hipLaunchKernelGGL((k32_p2g_cells<LPC, false, false, PF>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T,
+                               partials);
     }
+}
+
+template <int LPC>
+static void p2g_cells_launch(hipStream_t s, int ng, bool diag, bool trac, bool pf, const GridGeom &g,
+                             const PartF32 &P, const CellLists &L, const NodeTables &T, float *partials) {
+    if (pf)
+        p2g_cells_launch_pf<LPC, true>(s, ng, diag, trac, g, P, L, T, partials);
+    else
+        p2g_cells_launch_pf<LPC, false>(s, ng, diag, trac, g, P, L, T, partials);
 }
 
 void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
                     const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
                     int diag, hipEvent_t mid) {
-    const int ng = cell_groups_grid(g);
+    const int ng = c.cells_grid > 0 ? c.cells_grid : cell_groups_grid(g);
     const bool trac = P.trac != nullptr;
+    const bool pf = c.cells_prefetch != 0;
     switch (c.lanes_per_cell) {
-    case 8: p2g_cells_launch<8>(s, ng, diag, trac, g, P, L, T, partials); break;
-    case 4: p2g_cells_launch<4>(s, ng, diag, trac, g, P, L, T, partials); break;
-    case 2: p2g_cells_launch<2>(s, ng, diag, trac, g, P, L, T, partials); break;
-    default: p2g_cells_launch<1>(s, ng, diag, trac, g, P, L, T, partials); break;
+    case 8: p2g_cells_launch<8>(s, ng, diag, trac, pf, g, P, L, T, partials); break;
+    case 4: p2g_cells_launch<4>(s, ng, diag, trac, pf, g, P, L, T, partials); break;
+    case 2: p2g_cells_launch<2>(s, ng, diag, trac, pf, g, P, L, T, partials); break;
+    default: p2g_cells_launch<1>(s, ng, diag, trac, pf, g, P, L, T, partials); break;
     }
     if (mid) (void)hipEventRecord(mid, s);
     if (diag)

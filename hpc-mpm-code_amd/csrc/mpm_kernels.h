@@ -205,6 +205,8 @@ struct StepConsts {
     float m_small;     // below this node mass the division runs in FP64
     int scheme;
     int lanes_per_cell;  // 1, 2, 4 or 8 lanes fold one cell's particles in the cells kernels
+    int cells_prefetch;  // cells kernels request the next chunk's indices while a chunk loads
+    int cells_grid;      // workgroups of the cells kernels (0 = automatic)
 };
 
 // Grid spacing constants of the FP32 kernels.

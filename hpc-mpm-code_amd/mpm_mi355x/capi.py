@@ -32,6 +32,8 @@ OPT_RELAYOUT_EVERY = 3
 OPT_GRAPH_BLOCK = 4
 OPT_FUSED_COUNT = 5
 OPT_LANES_PER_CELL = 2
+OPT_CELLS_PREFETCH = 6
+OPT_CELLS_GRID = 7
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

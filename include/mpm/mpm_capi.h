@@ -233,13 +233,19 @@ int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
  * block is a multiple of twice the relayout interval. Uploads, options and
  * surrogate changes drop the recorded graph.
  * MPM_OPT_FUSED_COUNT (FP32, default 1): the update kernel counts particles
- * per cell for the next binning, which then skips its count pass. */
+ * per cell for the next binning, which then skips its count pass.
+ * MPM_OPT_CELLS_PREFETCH (FP32, default 1): the P2G cells kernel requests the
+ * next chunk's particle indices and the next group's cell ranges while a
+ * chunk's fields load. MPM_OPT_CELLS_GRID (FP32, default 0 = automatic):
+ * workgroups of the cells kernels. Neither changes results. */
 enum {
     MPM_OPT_FULL_NODE_OUTPUT = 1,
     MPM_OPT_LANES_PER_CELL = 2,
     MPM_OPT_RELAYOUT_EVERY = 3,
     MPM_OPT_GRAPH_BLOCK = 4,
-    MPM_OPT_FUSED_COUNT = 5
+    MPM_OPT_FUSED_COUNT = 5,
+    MPM_OPT_CELLS_PREFETCH = 6,
+    MPM_OPT_CELLS_GRID = 7
 };
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 /* Reference particle shifting (DeltaCorrectionParticleShifting,
