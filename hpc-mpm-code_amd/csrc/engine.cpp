@@ -145,9 +145,10 @@ struct mpm_engine {
         int64_t cap = 0, nsamples = 0, logged_steps = 0;
         unsigned long long *ctr = nullptr;  // evaluated, fallbacks, offered
         float *queue = nullptr;             // plastic trial states of a step, [qcap][kSurQueueRow]
-        uint32_t *qcount = nullptr;         // [2]
-        int64_t qcap = 0;
-        uint32_t parity = 0;
+        uint32_t *bcount = nullptr;         // queued states per region of kSurRegion rows
+        uint32_t *rstat = nullptr;          // per region: [evaluated, rejected], summed over steps
+        int64_t stat_base[2] = {0, 0};      // counts of regions folded in before a reallocation
+        int64_t qcap = 0;                   // rows: regions * kSurRegion, one region per update block
         uint32_t thr = 0, step_tag = 0;
         bool sorted = false;
         float loss = 0.f, tol = 1e-4f;
@@ -162,7 +163,8 @@ struct mpm_engine {
             free_log();
             if (ctr) (void)hipFree(ctr);
             if (queue) (void)hipFree(queue);
-            if (qcount) (void)hipFree(qcount);
+            if (bcount) (void)hipFree(bcount);
+            if (rstat) (void)hipFree(rstat);
             net.release();
         }
     } sur;
@@ -330,6 +332,9 @@ int do_solve(mpm_engine *e) {
     return 0;
 }
 
+// Surrogate queue rows for n particles: one region of kSurRegion rows per update block.
+static int64_t sur_queue_rows(int64_t n) { return (int64_t)cdiv((int)n, kSurRegion) * kSurRegion; }
+
 int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2 = nullptr) {
     if (e->prec == MPM_PREC_FP64_EXACT) {
         ShiftF64 sh{};
@@ -367,16 +372,16 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
                 sa.step_tag = su.step_tag++;
                 su.logged_steps++;
             }
-        } else if (su.enabled && su.have_net && su.queue && su.qcap >= e->n) {
+        } else if (su.enabled && su.have_net && su.queue && su.qcap >= sur_queue_rows(e->n)) {
             sa.mode = 2;
             sa.tol = su.tol;
             sa.net = su.net.view();
             sa.net_scaling = su.net.scaling;
             sa.queue = su.queue;
-            sa.qcount = su.qcount;
+            sa.bcount = su.bcount;
+            sa.rstat = su.rstat;
+            sa.nregions = (uint32_t)cdiv(e->n, kSurRegion);
             sa.qcap = (uint32_t)su.qcap;
-            sa.parity = su.parity;
-            su.parity ^= 1u;
         }
         sa.ctr = su.ctr;
         sa.pid = e->pid[e->cur];
@@ -385,7 +390,7 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         e->clear_counts();
         BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->bb.tiles, e->tg.ntiles, e->bb.ctr};
         launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa, bf);
-        if (sa.mode == 2) launch_f32_j2_surrogate(e->stream, e->P32[e->cur], e->mt, sa);
+        if (sa.mode == 2) launch_f32_j2_surrogate(e->stream, e->P32[e->cur], sa);
         e->counts_ready = e->fused_count;
         e->keys_valid = true;  // the update kernel writes the new keys
     }
@@ -615,6 +620,49 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     return 0;
 }
 
+// Sums the per-region network tallies (evaluated, rejected) on the host.
+static int sur_region_tally(mpm_engine *e, int64_t out[2]) {
+    auto &su = e->sur;
+    out[0] = su.stat_base[0];
+    out[1] = su.stat_base[1];
+    if (!su.rstat) return 0;
+    const size_t nr = (size_t)(su.qcap / kSurRegion);
+    std::vector<uint32_t> h(2 * nr);
+    MPM_HIP_CHECK(hipMemcpyAsync(h.data(), su.rstat, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    for (size_t i = 0; i < nr; i++) {
+        out[0] += h[2 * i];
+        out[1] += h[2 * i + 1];
+    }
+    return 0;
+}
+
+// (Re)allocates the surrogate queue, its per-region counts and tallies for n
+// particles; tallies of the old regions are folded into the base counts.
+static int sur_alloc_queue(mpm_engine *e, int64_t n) {
+    auto &su = e->sur;
+    const int64_t rows = sur_queue_rows(n);
+    if (su.qcap >= rows && su.bcount && su.rstat) return 0;
+    int64_t t[2];
+    int rc = sur_region_tally(e, t);
+    if (rc) return rc;
+    su.stat_base[0] = t[0];
+    su.stat_base[1] = t[1];
+    for (void *p : {(void *)su.queue, (void *)su.bcount, (void *)su.rstat})
+        if (p) (void)hipFree(p);
+    su.queue = nullptr;
+    su.bcount = su.rstat = nullptr;
+    su.qcap = 0;
+    const size_t nr = (size_t)(rows / kSurRegion);
+    MPM_HIP_CHECK(hipMalloc((void **)&su.queue, (size_t)rows * kSurQueueRow * sizeof(float)));
+    MPM_HIP_CHECK(hipMalloc((void **)&su.bcount, nr * sizeof(uint32_t)));
+    MPM_HIP_CHECK(hipMalloc((void **)&su.rstat, 2 * nr * sizeof(uint32_t)));
+    MPM_HIP_CHECK(hipMemsetAsync(su.rstat, 0, 2 * nr * sizeof(uint32_t), e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    su.qcap = rows;
+    return 0;
+}
+
 static int sur_require(mpm_engine *e) {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     if (e->prec != MPM_PREC_FP32)
@@ -739,18 +787,8 @@ int mpm_surrogate_enable(mpm_engine *e, int32_t on, double tol) {
     if (on) {
         e->sur.tol = (float)std::min(tol, 1e30);
         e->sur.logging = false;
-        auto &su = e->sur;
-        if (su.qcap < e->n) {
-            if (su.queue) (void)hipFree(su.queue);
-            su.queue = nullptr;
-            su.qcap = 0;
-            MPM_HIP_CHECK(hipMalloc((void **)&su.queue, (size_t)e->n * kSurQueueRow * sizeof(float)));
-            su.qcap = e->n;
-        }
-        if (!su.qcount) MPM_HIP_CHECK(hipMalloc((void **)&su.qcount, 2 * sizeof(uint32_t)));
-        MPM_HIP_CHECK(hipMemsetAsync(su.qcount, 0, 2 * sizeof(uint32_t), e->stream));
-        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
-        su.parity = 0;
+        rc = sur_alloc_queue(e, e->n);
+        if (rc) return rc;
     }
     return 0;
 }
@@ -763,6 +801,10 @@ int mpm_surrogate_reset_counters(mpm_engine *e) {
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     c[0] = c[1] = 0;  // keep the log count
     MPM_HIP_CHECK(hipMemcpyAsync(e->sur.ctr, c, sizeof c, hipMemcpyHostToDevice, e->stream));
+    auto &su = e->sur;
+    su.stat_base[0] = su.stat_base[1] = 0;
+    if (su.rstat)
+        MPM_HIP_CHECK(hipMemsetAsync(su.rstat, 0, 2 * (size_t)(su.qcap / kSurRegion) * sizeof(uint32_t), e->stream));
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return 0;
 }
@@ -776,8 +818,10 @@ int mpm_surrogate_stats_get(mpm_engine *e, mpm_surrogate_stats *out) {
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     out->samples = (int64_t)std::min<unsigned long long>(c[2], (unsigned long long)e->sur.cap);
     out->logged_steps = e->sur.logged_steps;
-    out->evaluated = (int64_t)c[0];
-    out->fallbacks = (int64_t)c[1];
+    int64_t t[2];
+    if ((rc = sur_region_tally(e, t))) return rc;
+    out->evaluated = (int64_t)c[0] + t[0];
+    out->fallbacks = (int64_t)c[1] + t[1];
     out->loss = e->sur.loss;
     out->tol = e->sur.tol;
     return 0;
@@ -1036,12 +1080,9 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
     e->has_epsp = true;
     int rc = alloc_particles(e, n);
     if (rc) return rc;
-    if (e->sur.enabled && e->sur.qcap < n) {  // the surrogate queue holds every particle
-        if (e->sur.queue) (void)hipFree(e->sur.queue);
-        e->sur.queue = nullptr;
-        e->sur.qcap = 0;
-        MPM_HIP_CHECK(hipMalloc((void **)&e->sur.queue, (size_t)n * kSurQueueRow * sizeof(float)));
-        e->sur.qcap = n;
+    if (e->sur.enabled) {  // the surrogate queue holds every particle
+        rc = sur_alloc_queue(e, n);
+        if (rc) return rc;
     }
     const size_t ld = e->ld;
     auto F_at = [&](int i, int q) -> double {
@@ -1415,8 +1456,7 @@ int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
     const int B = e->graph_block;
     const bool ramp_over = e->prm.nlstep <= 1 || istep0 > e->prm.nlstep;
     const bool relayout_ok = !(e->prec == MPM_PREC_FP32 && e->relayout_every > 0) || B % (2 * e->relayout_every) == 0;
-    const bool parity_ok = !e->sur.enabled || (B % 2) == 0;  // the surrogate queue counters alternate by step
-    if (B > 0 && !e->timing && ramp_over && relayout_ok && parity_ok && !e->sur.logging && nsteps > B) {
+    if (B > 0 && !e->timing && ramp_over && relayout_ok && !e->sur.logging && nsteps > B) {
         // all but the last step in whole graph blocks; the rest eagerly (the
         // last step may write the diagnostic node fields)
         const int nblk = (nsteps - 1) / B;
@@ -1830,8 +1870,6 @@ int mpm_ensemble_capture(mpm_ensemble *x, int32_t istep0, int32_t block_steps) {
         if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && block_steps % (2 * e->relayout_every) != 0)
             return set_error(MPM_ERR_INVALID_ARGUMENT,
                              "block_steps must be a multiple of twice every scene's relayout interval");
-        if (e->sur.enabled && block_steps % 2 != 0)
-            return set_error(MPM_ERR_INVALID_ARGUMENT, "block_steps must be even when the surrogate is on");
     }
     int rc = ens_prepare(x);
     if (rc) return rc;

@@ -1069,19 +1069,27 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                 }
             }
         } else {
-            // exact update for every particle; the plastic trial states are
-            // queued for the network pass that follows this kernel
+            // exact update for every particle; the plastic trial states of this
+            // block are compacted, in slot order, into the block's own queue
+            // region for the network pass that follows this kernel
             float x[16];
             if (plastic) j2_features(t, D, s, x);
             if (j2) j2_return(t, s, &ep);
-            if (blockIdx.x == 0 && threadIdx.x == 0) sa.qcount[sa.parity ^ 1u] = 0u;  // next step's counter
+            __shared__ uint32_t wcnt[kSurRegion / 64];
+            const int w = (int)(threadIdx.x >> 6);
             const unsigned long long m = __ballot(plastic);
-            if (m) {
-                const int leader = __ffsll((unsigned long long)m) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&sa.qcount[sa.parity], (uint32_t)__popcll(m));
-                base = (uint32_t)__shfl((int)base, leader, 64);
-                const uint32_t qi = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wcnt[w] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t off = 0, tot = 0;
+#pragma unroll
+            for (int i = 0; i < kSurRegion / 64; i++) {
+                off += i < w ? wcnt[i] : 0u;
+                tot += wcnt[i];
+            }
+            if (threadIdx.x == 0) sa.bcount[blockIdx.x] = tot;
+            {
+                const uint32_t qi = blockIdx.x * (uint32_t)kSurRegion + off +
+                                    (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 if (plastic && qi < sa.qcap) {
                     float4 *row = (float4 *)(sa.queue + (size_t)qi * kSurQueueRow);
                     row[0] = make_float4(__int_as_float(slot), x[0], x[1], x[2]);
@@ -1089,7 +1097,7 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                     row[2] = make_float4(x[7], x[8], x[9], x[10]);
                     row[3] = make_float4(x[11], x[12], t.st[0], t.st[1]);
                     row[4] = make_float4(t.st[2], t.st[3], t.st[4], t.st[5]);
-                    row[5] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    row[5] = make_float4(t.E, t.nu, t.sy, t.Hh);
                 }
             }
         }
@@ -1119,83 +1127,127 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     if (e) atomicOr(err, e);
 }
 
-// Network pass over the queued plastic J2 trial states of a step: 64 queued
-// states per wave on the BF16 MFMA, accepted where the return-mapping
-// residual is within the tolerance (the exact result written by the update
-// kernel stays otherwise).
-__global__ void __launch_bounds__(256) k32_j2_surrogate(PartF32 P, MaterialTable mt, SurrogateArgs sa) {
+// queue row qi into registers (zeros when not live)
+__device__ __forceinline__ void sur_load_row(const float *queue, uint32_t qi, bool live, float r[kSurQueueRow]) {
+    if (live) {
+        const float4 *row = (const float4 *)(queue + (size_t)qi * kSurQueueRow);
+#pragma unroll
+        for (int c = 0; c < kSurQueueRow / 4; c++) {
+            const float4 v = row[c];
+            r[4 * c] = v.x;
+            r[4 * c + 1] = v.y;
+            r[4 * c + 2] = v.z;
+            r[4 * c + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < kSurQueueRow; c++) r[c] = 0.f;
+    }
+}
+
+// Network pass over the queued plastic J2 trial states of a step. A tile
+// slot is (region, k): rows [64 k, 64 k + 64) of queue region k / 4 (the
+// plastic states of one update block). The blocks are persistent (the
+// network weights are staged in LDS once per block); each wave visits every
+// nw-th tile slot, finds the non-empty ones of its next 64 slots with one
+// load of the region counts per lane, and loads the rows of its next tile
+// while the current tile runs through the network on the BF16 MFMA. A
+// prediction is accepted where the return-mapping residual is within the
+// tolerance; the exact result written by the update kernel stays otherwise.
+// The evaluated / rejected tallies go to per-region counters: a region's
+// tiles may be spread over several waves, so they are added atomically, but
+// to distinct addresses.
+__global__ void __launch_bounds__(256) k32_j2_surrogate(PartF32 P, SurrogateArgs sa) {
     __shared__ MlpSmem sw;
     __shared__ float scl[64];
     __shared__ __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];
-    const uint32_t nq = min(sa.qcount[sa.parity], sa.qcap);
-    if ((uint32_t)blockIdx.x * 256u >= nq) return;  // whole block idle
     mlp_load_smem(sw, sa.net);
     if (threadIdx.x < 64) scl[threadIdx.x] = sa.net_scaling[threadIdx.x];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (uint32_t q0 = ((uint32_t)blockIdx.x * 4u + (uint32_t)w) * 64u; q0 < nq; q0 += gridDim.x * 256u) {
-        const uint32_t qi = q0 + (uint32_t)lane;
-        const bool live = qi < nq;
+    const uint32_t nslots = 4u * sa.nregions;
+    const uint32_t nw = gridDim.x * 4u;
+    const uint32_t gw = blockIdx.x * 4u + (uint32_t)w;
+    for (uint32_t base = gw; base < nslots; base += 64u * nw) {
+        // lane l checks slot base + l nw
+        const uint32_t my = base + (uint32_t)lane * nw;
+        uint32_t c = 0;
+        if (my < nslots) c = min(sa.bcount[my >> 2], (uint32_t)kSurRegion);
+        const int myn = (int)c - 64 * (int)(my & 3u);  // rows of this lane's tile
+        unsigned long long todo = __ballot(myn > 0);
+        if (!todo) continue;
+        int b = __ffsll(todo) - 1;
+        todo &= todo - 1ull;
+        int n_cur = __shfl(myn, b, 64);
+        uint32_t q_cur = ((base + (uint32_t)b * nw) >> 2) * (uint32_t)kSurRegion + 64u * ((base + (uint32_t)b * nw) & 3u);
         float r[kSurQueueRow];
-        if (live) {
-            const float4 *row = (const float4 *)(sa.queue + (size_t)qi * kSurQueueRow);
-#pragma unroll
-            for (int c = 0; c < kSurQueueRow / 4; c++) {
-                const float4 v = row[c];
-                r[4 * c] = v.x;
-                r[4 * c + 1] = v.y;
-                r[4 * c + 2] = v.z;
-                r[4 * c + 3] = v.w;
+        sur_load_row(sa.queue, q_cur + (uint32_t)lane, lane < n_cur, r);
+        for (;;) {
+            // next tile of this batch: its rows are loaded while this one runs
+            const bool more = todo != 0ull;
+            int n_nxt = 0;
+            uint32_t q_nxt = 0;
+            float rn[kSurQueueRow];
+            if (more) {
+                const int bn = __ffsll(todo) - 1;
+                todo &= todo - 1ull;
+                n_nxt = __shfl(myn, bn, 64);
+                const uint32_t sn = base + (uint32_t)bn * nw;
+                q_nxt = (sn >> 2) * (uint32_t)kSurRegion + 64u * (sn & 3u);
             }
-        } else {
+            sur_load_row(sa.queue, q_nxt + (uint32_t)lane, more && lane < n_nxt, rn);
+            const bool live = lane < n_cur;
+            float xn[16];
 #pragma unroll
-            for (int c = 0; c < kSurQueueRow; c++) r[c] = 0.f;
-        }
-        float xn[16];
+            for (int f = 0; f < 16; f++) xn[f] = (f < kMlpInUsed) ? (r[1 + f] - scl[f]) * scl[16 + f] : 0.f;
+            float y[kMlpOutUsed];
+            mlp_wave_infer(sw, xn, y, xs[w], lane);
+            bool accept = false;
+            if (live) {
+                const int slot = __float_as_int(r[0]);
+                // trial state from the queued trial stress (same arithmetic as j2_trial)
+                J2Trial t;
+                const float E = r[20], nu = r[21];
+                t.E = E;
+                t.nu = nu;
+                t.sy = r[22];
+                t.Hh = r[23];
+                t.ep = r[13];
+                t.lmd = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
+                t.mu = E / (2.f * (1.f + nu));
 #pragma unroll
-        for (int f = 0; f < 16; f++) xn[f] = (f < kMlpInUsed) ? (r[1 + f] - scl[f]) * scl[16 + f] : 0.f;
-        float y[kMlpOutUsed];
-        mlp_wave_infer(sw, xn, y, xs[w], lane);
-        bool accept = false;
-        if (live) {
-            const int slot = __float_as_int(r[0]);
-            const int mat = P.matid ? P.matid[slot] : 0;
-            const MaterialDev &md = mt.m[mat];
-            // trial state from the queued trial stress (same arithmetic as j2_trial)
-            J2Trial t;
-            const float E = (float)md.prop[0], nu = (float)md.prop[1];
-            t.E = E;
-            t.nu = nu;
-            t.sy = (float)md.prop[2];
-            t.Hh = (float)md.prop[3];
-            t.ep = r[13];
-            t.lmd = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
-            t.mu = E / (2.f * (1.f + nu));
+                for (int cc = 0; cc < 6; cc++) t.st[cc] = r[14 + cc];
+                t.pm = (t.st[0] + t.st[1] + t.st[2]) * (1.f / 3.f);
+                t.d0 = t.st[0] - t.pm;
+                t.d1 = t.st[1] - t.pm;
+                t.d2 = t.st[2] - t.pm;
+                const float j2 = 0.5f * (t.d0 * t.d0 + t.d1 * t.d1 + t.d2 * t.d2) + t.st[3] * t.st[3] +
+                                 t.st[4] * t.st[4] + t.st[5] * t.st[5];
+                t.q = sqrtf(3.f * j2);
+                float ss[6];
 #pragma unroll
-            for (int c = 0; c < 6; c++) t.st[c] = r[14 + c];
-            t.pm = (t.st[0] + t.st[1] + t.st[2]) * (1.f / 3.f);
-            t.d0 = t.st[0] - t.pm;
-            t.d1 = t.st[1] - t.pm;
-            t.d2 = t.st[2] - t.pm;
-            const float j2 = 0.5f * (t.d0 * t.d0 + t.d1 * t.d1 + t.d2 * t.d2) + t.st[3] * t.st[3] +
-                             t.st[4] * t.st[4] + t.st[5] * t.st[5];
-            t.q = sqrtf(3.f * j2);
-            float ss[6];
+                for (int cc = 0; cc < 6; cc++) ss[cc] = t.st[cc] - (y[cc] * scl[48 + cc] + scl[32 + cc]);
+                const float dg = y[6] * scl[54] + scl[38];
+                if (j2_residual(t, ss, dg) <= sa.tol) {
+                    accept = true;
+                    const size_t ld = P.ld;
 #pragma unroll
-            for (int c = 0; c < 6; c++) ss[c] = t.st[c] - (y[c] * scl[48 + c] + scl[32 + c]);
-            const float dg = y[6] * scl[54] + scl[38];
-            if (j2_residual(t, ss, dg) <= sa.tol) {
-                accept = true;
-                const size_t ld = P.ld;
-#pragma unroll
-                for (int c = 0; c < 6; c++) P.sig[c * ld + slot] = ss[c];
-                if (P.eps_p) P.eps_p[slot] = t.ep + dg;
+                    for (int cc = 0; cc < 6; cc++) P.sig[cc * ld + slot] = ss[cc];
+                    if (P.eps_p) P.eps_p[slot] = t.ep + dg;
+                }
             }
-        }
-        const unsigned long long ev = __ballot(live), rej = __ballot(live && !accept);
-        if (lane == 0) {
-            atomicAdd(&sa.ctr[0], (unsigned long long)__popcll(ev));
-            if (rej) atomicAdd(&sa.ctr[1], (unsigned long long)__popcll(rej));
+            const uint32_t ev = (uint32_t)__popcll(__ballot(live));
+            const uint32_t rj = (uint32_t)__popcll(__ballot(live && !accept));
+            if (lane == 0) {
+                const uint32_t rg = q_cur / (uint32_t)kSurRegion;
+                atomicAdd(&sa.rstat[2 * rg], ev);
+                if (rj) atomicAdd(&sa.rstat[2 * rg + 1], rj);
+            }
+            if (!more) break;
+            n_cur = n_nxt;
+            q_cur = q_nxt;
+#pragma unroll
+            for (int cc = 0; cc < kSurQueueRow; cc++) r[cc] = rn[cc];
         }
     }
 }
@@ -1307,12 +1359,12 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
                            N, bc, c);
 }
 
-void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const MaterialTable &mt, const SurrogateArgs &sa) {
+void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const SurrogateArgs &sa) {
     if (P.n <= 0) return;
-    // grid over the worst case (every particle plastic), blocks beyond the
-    // queue length leave at once
-    const int blocks = std::min(cdiv(P.n, 256), 2048);
-    hipLaunchKernelGGL(k32_j2_surrogate, dim3(blocks), dim3(256), 0, s, P, mt, sa);
+    // persistent blocks, two per CU of the 256 (this kernel's occupancy is
+    // two waves per SIMD)
+    const int blocks = std::min((int)sa.nregions, 512);
+    hipLaunchKernelGGL(k32_j2_surrogate, dim3(blocks), dim3(256), 0, s, P, sa);
 }
 
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,

@@ -235,18 +235,23 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
 // J2 constitutive surrogate in the FP32 update: mode 0 = exact update only,
 // 1 = exact update and sample logging, 2 = exact update and a queue of the
 // plastic trial states, which launch_f32_j2_surrogate then runs through the
-// network (residual check, exact result kept on rejection).
-constexpr int kSurQueueRow = 24;  // floats per queued state: slot, 13 features + pad (16), trial stress (6), pad
+// network (residual check, exact result kept on rejection). Each 256-slot
+// block of the update kernel compacts its plastic states into its own queue
+// region (rows [256 b, 256 b + count_b)) and stores count_b, so the queue is
+// filled without any global atomic.
+constexpr int kSurQueueRow = 24;  // floats per queued state: slot, 13 features, trial stress (6), E, nu, sy, H
+constexpr int kSurRegion = 256;   // queue rows per update-kernel block (= its block size)
 struct SurrogateArgs {
     int mode;
     float tol;                     // mode 2: accepted residual / yield stress
     MlpParams net;                 // mode 2
     const float *net_scaling;      // mode 2: [in_mu | in_isd | out_mu | out_sd], 64 floats
     float *queue;                  // mode 2: [qcap][kSurQueueRow]
-    uint32_t *qcount;              // mode 2: two counters, used by alternate steps
-    uint32_t parity;               // mode 2: this step's counter
-    uint32_t qcap;                 // mode 2: queue rows
-    unsigned long long *ctr;       // [0] evaluated, [1] fallbacks, [2] samples offered to the log
+    uint32_t *bcount;              // mode 2: queued states per region (one region per update block)
+    uint32_t nregions;             // mode 2: update blocks = cdiv(n, kSurRegion)
+    uint32_t *rstat;               // mode 2: per region [evaluated, rejected], summed over steps
+    uint32_t qcap;                 // mode 2: queue rows (>= nregions * kSurRegion)
+    unsigned long long *ctr;       // [2] samples offered to the log ([0], [1]: evaluated, fallbacks of earlier forms)
     float *log_in, *log_tgt;       // mode 1: [cap][16]
     uint32_t *log_key;             // mode 1: step_tag * n + particle id
     uint32_t cap;
@@ -259,7 +264,7 @@ void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, c
                            const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
                            const SurrogateArgs &sa, const BinFuse &bf);
 // network pass over the queued plastic states of this step (mode 2)
-void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const MaterialTable &mt, const SurrogateArgs &sa);
+void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const SurrogateArgs &sa);
 void launch_f32_permute(hipStream_t s, const PartF32 &src, const PartF32 &dst, const uint32_t *perm,
                         const int *pid_src, int *pid_dst);
 void launch_iota_u32(hipStream_t s, uint32_t *p, int n);
