@@ -10,7 +10,9 @@
 //                 cells, its index in the active-cell list
 //   3. finalize   writes cstart/cend (cend starts as a cursor at cstart), the
 //                 active-cell list, the active node-tile flags, and re-zeroes
-//                 the counters for the next step
+//                 the counters for the next step -- or keeps them, when the
+//                 next update kernel adjusts them for the particles that
+//                 change cell (kept counts)
 //   4. scatter    wave-aggregated atomic cursor per run; lanes of a run get
 //                 consecutive positions in slot order. A run that does not
 //                 land at its cell's start means the cell received several
@@ -148,7 +150,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
                                                           uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend,
                                                           uint32_t *__restrict__ active, uint32_t *__restrict__ astart,
                                                           uint32_t *__restrict__ tiles, TileGeom tg,
-                                                          uint32_t *__restrict__ occ) {
+                                                          uint32_t *__restrict__ occ, int keep) {
     const size_t tbase = (size_t)blockIdx.x * kTile;
     uint4 *c4 = (uint4 *)(cnt + tbase);
     uint4 *s4 = (uint4 *)(cstart + tbase);
@@ -178,7 +180,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
         const uint4 so = make_uint4(st[0], st[1], st[2], st[3]);
         s4[idx] = so;
         e4[idx] = so;
-        if (any) c4[idx] = make_uint4(0u, 0u, 0u, 0u);
+        if (any && !keep) c4[idx] = make_uint4(0u, 0u, 0u, 0u);
         // occupancy bits: 8 consecutive lanes hold the 32 cells of one word
         const int lane = threadIdx.x & 63;
         uint32_t w = ((vv[0] ? 1u : 0u) | (vv[1] ? 2u : 0u) | (vv[2] ? 4u : 0u) | (vv[3] ? 8u : 0u))
@@ -298,7 +300,7 @@ void launch_bin_count(hipStream_t s, const GridGeom &g, const TileGeom &tg, cons
 }
 
 int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
-               uint32_t *err, bool counted) {
+               uint32_t *err, bool counted, bool keep) {
     const int nelem = g.nelem;
     const int nb = cdiv(nelem, kTile);
     if (nb > kTile) return -1;
@@ -310,7 +312,7 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
     hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
     hipLaunchKernelGGL(k_bin_top, dim3(1), dim3(kBlock), 0, s, b.bsum, nb, b.ctr, b.astart, n, err);
     hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, b.cstart,
-                       b.cend, b.active, b.astart, b.tiles, tg, b.occ);
+                       b.cend, b.active, b.astart, b.tiles, tg, b.occ, keep ? 1 : 0);
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, (const uint32_t *)b.cstart,
                            b.cend, b.perm, b.stamp, b.fixlist, b.ctr);

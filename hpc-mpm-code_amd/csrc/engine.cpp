@@ -99,7 +99,7 @@ struct mpm_engine {
     uint32_t *d_err = nullptr;
     bool bins_valid = false;
     bool keys_valid = false;
-    bool counts_ready = false;  // per-cell counts of the current keys are in bb.cnt (fused into the update)
+    bool counts_ready = false;  // bb.cnt holds the per-cell counts of the current keys (else it is zero)
     bool fused_count = true;    // FP32: the update kernel counts the new keys for the next binning
     int last_pass = 0;  // 1 = p2g, 2 = g2p
     int steps_since_relayout = 0;
@@ -294,8 +294,11 @@ int do_bin(mpm_engine *e) {
         e->keys_valid = true;  // FP32 keys are uploaded and then kept by the update kernel
     }
     const uint32_t *keys = (e->prec == MPM_PREC_FP64_EXACT) ? e->keys64 : e->P32[e->cur].key;
-    int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, e->d_err, e->counts_ready);
-    e->counts_ready = false;
+    // FP32 with fused counting keeps the counts: the update kernel then moves
+    // only the counts of particles that change cell
+    const bool keep = e->prec == MPM_PREC_FP32 && e->fused_count;
+    int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, e->d_err, e->counts_ready, keep);
+    e->counts_ready = keep;
     if (rc) return set_error(MPM_ERR_HIP, "binning launch failed");
     e->perm = e->bb.perm;
     if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && ++e->steps_since_relayout >= e->relayout_every) {
@@ -385,10 +388,11 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         }
         sa.ctr = su.ctr;
         sa.pid = e->pid[e->cur];
-        // count the new keys for the next binning; counts left from an update
-        // that no binning consumed are cleared first
-        e->clear_counts();
-        BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->bb.tiles, e->tg.ntiles, e->bb.ctr};
+        // count the new keys for the next binning: adjust kept counts of the
+        // current keys, or count from zero
+        if (!e->fused_count) e->clear_counts();
+        BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->counts_ready ? 1 : 0, e->bb.tiles, e->tg.ntiles,
+                   e->bb.ctr};
         launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa, bf);
         if (sa.mode == 2) launch_f32_j2_surrogate(e->stream, e->P32[e->cur], sa);
         e->counts_ready = e->fused_count;
@@ -1592,6 +1596,7 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
         e->cells_grid = (int)value;
         return 0;
     case OPT_FUSED_COUNT:
+        e->drop_graph();  // recorded steps assume the counting mode they were recorded with
         e->clear_counts();
         e->fused_count = value != 0;
         return 0;

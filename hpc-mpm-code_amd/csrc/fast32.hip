@@ -127,20 +127,25 @@ struct ReduceScatter<L, L, N> {
 // into LDS with gfx950's global_load_lds_dword: lane l of a wave writes the
 // dword for chunk particle 64*r + l at stage block r. Nothing is held in
 // VGPRs while the loads are in flight, so a wave keeps all of a chunk's loads
-// outstanding. Each field's 64-particle blocks are 65 dwords apart, which
-// spreads the thread-per-cell reads (stride = particles per cell) over the
-// LDS banks.
+// outstanding. Each field's 64-particle blocks are S = 64 + L dwords apart
+// for L lanes per cell: lane (cell c, sub s) of a 16-point-per-cell group
+// reads particle 16 c + s + L k, which lands in block c / (64 / 16) at
+// bank (L (c / 4) + 16 (c % 4) + s + L k) mod 64 -- a different bank for
+// every lane of the wave (likewise for 8 points per cell).
 constexpr int kBlk = kChunk / 64;
-__device__ __forceinline__ int st_idx(int f, int i) { return (f * kBlk + (i >> 6)) * 65 + (i & 63); }
+template <int S>
+__device__ __forceinline__ int st_idx(int f, int i) { return (f * kBlk + (i >> 6)) * S + (i & 63); }
 
+template <int S>
 __device__ __forceinline__ void lds_gather(const void *field_base, uint32_t slot, float *stage, int f, int r) {
-    __builtin_amdgcn_global_load_lds((const uint32_t *)field_base + slot, stage + st_idx(f, 64 * r), 4, 0, 0);
+    __builtin_amdgcn_global_load_lds((const uint32_t *)field_base + slot, stage + st_idx<S>(f, 64 * r), 4, 0, 0);
 }
 
+template <int S>
 __device__ __forceinline__ void lds_gather2(const double *field_base, uint32_t slot, float *stage, int f, int r) {
     const uint32_t *p = (const uint32_t *)field_base + 2 * (size_t)slot;
-    __builtin_amdgcn_global_load_lds(p, stage + st_idx(f, 64 * r), 4, 0, 0);
-    __builtin_amdgcn_global_load_lds(p + 1, stage + st_idx(f + 1, 64 * r), 4, 0, 0);
+    __builtin_amdgcn_global_load_lds(p, stage + st_idx<S>(f, 64 * r), 4, 0, 0);
+    __builtin_amdgcn_global_load_lds(p + 1, stage + st_idx<S>(f + 1, 64 * r), 4, 0, 0);
 }
 
 // wait for every outstanding vector-memory load (including LDS DMA) of this wave
@@ -162,7 +167,8 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
     constexpr int G = kSeg / LPC;
     // fields: 0-2 r, 3-5 v, 6 m, 7-8 V (lo, hi dword), 9-14 sigma, [15-17 traction]
     constexpr int NF = 15 + (TRAC ? 3 : 0);
-    __shared__ float stage[NF * kBlk * 65];
+    constexpr int SS = 64 + LPC;
+    __shared__ float stage[NF * kBlk * SS];
     const uint32_t n_active = L.ctr->n_active;
     const int ngroups = (int)((n_active + G - 1) / G);
     const int sub = threadIdx.x % LPC;
@@ -211,16 +217,16 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                     const uint32_t sl = slot[r];
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
-                        lds_gather(P.r + a * ld, sl, stage, a, r);
-                        lds_gather(P.v + a * ld, sl, stage, 3 + a, r);
+                        lds_gather<SS>(P.r + a * ld, sl, stage, a, r);
+                        lds_gather<SS>(P.v + a * ld, sl, stage, 3 + a, r);
                     }
-                    lds_gather(P.m, sl, stage, 6, r);
-                    lds_gather2(P.V, sl, stage, 7, r);
+                    lds_gather<SS>(P.m, sl, stage, 6, r);
+                    lds_gather2<SS>(P.V, sl, stage, 7, r);
 #pragma unroll
-                    for (int q = 0; q < 6; q++) lds_gather(P.sig + q * ld, sl, stage, 9 + q, r);
+                    for (int q = 0; q < 6; q++) lds_gather<SS>(P.sig + q * ld, sl, stage, 9 + q, r);
                     if (TRAC) {
 #pragma unroll
-                        for (int a = 0; a < 3; a++) lds_gather(P.trac + a * ld, sl, stage, 15 + a, r);
+                        for (int a = 0; a < 3; a++) lds_gather<SS>(P.trac + a * ld, sl, stage, 15 + a, r);
                     }
                 }
             }
@@ -236,27 +242,27 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
             const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
             for (uint32_t pos = first_pos<LPC>(si.cs, lo, sub); pos < hi; pos += LPC) {
                 const int i = (int)(pos - base);
-                const float fx = stage[st_idx(0, i)];
-                const float fy = stage[st_idx(1, i)];
-                const float fz = stage[st_idx(2, i)];
-                const float m = stage[st_idx(6, i)];
-                const float px = m * stage[st_idx(3, i)], py = m * stage[st_idx(4, i)], pz = m * stage[st_idx(5, i)];
-                const double Vd = __hiloint2double(__float_as_int(stage[st_idx(8, i)]),
-                                                   __float_as_int(stage[st_idx(7, i)]));
+                const float fx = stage[st_idx<SS>(0, i)];
+                const float fy = stage[st_idx<SS>(1, i)];
+                const float fz = stage[st_idx<SS>(2, i)];
+                const float m = stage[st_idx<SS>(6, i)];
+                const float px = m * stage[st_idx<SS>(3, i)], py = m * stage[st_idx<SS>(4, i)], pz = m * stage[st_idx<SS>(5, i)];
+                const double Vd = __hiloint2double(__float_as_int(stage[st_idx<SS>(8, i)]),
+                                                   __float_as_int(stage[st_idx<SS>(7, i)]));
                 const float nV = -(float)Vd;
                 // -V sigma (Voigt xx,yy,zz,yz,xz,xy) scaled by 1/dx of the derivative direction
-                const float s0 = nV * stage[st_idx(9, i)], s1 = nV * stage[st_idx(10, i)];
-                const float s2 = nV * stage[st_idx(11, i)], s3 = nV * stage[st_idx(12, i)];
-                const float s4 = nV * stage[st_idx(13, i)], s5 = nV * stage[st_idx(14, i)];
+                const float s0 = nV * stage[st_idx<SS>(9, i)], s1 = nV * stage[st_idx<SS>(10, i)];
+                const float s2 = nV * stage[st_idx<SS>(11, i)], s3 = nV * stage[st_idx<SS>(12, i)];
+                const float s4 = nV * stage[st_idx<SS>(13, i)], s5 = nV * stage[st_idx<SS>(14, i)];
                 // force on a corner: f_a = sum_b (-V sigma_ab) dN/dx_b, dN/dx = (sx/dx wy wz, sy/dy wx wz, sz/dz wx wy)
                 const float X0 = s0 * inv0, X1 = s5 * inv0, X2 = s4 * inv0;  // b = x: sigma_xx, sigma_yx, sigma_zx
                 const float Y0 = s5 * inv1, Y1 = s1 * inv1, Y2 = s3 * inv1;  // b = y
                 const float Z0 = s4 * inv2, Z1 = s3 * inv2, Z2 = s2 * inv2;  // b = z
                 float tx = 0.f, ty = 0.f, tz = 0.f;
                 if (TRAC) {
-                    tx = stage[st_idx(15, i)];
-                    ty = stage[st_idx(16, i)];
-                    tz = stage[st_idx(17, i)];
+                    tx = stage[st_idx<SS>(15, i)];
+                    ty = stage[st_idx<SS>(16, i)];
+                    tz = stage[st_idx<SS>(17, i)];
                 }
                 const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
                 float wyz[2][2], wxz[2][2], wxy[2][2];
@@ -499,8 +505,9 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
     constexpr int G = kSeg / LPC;
     // staged fields: 0-2 r, 3 m, [4-6 old v for FLIP]; results: G2P velocity per particle
     constexpr int NF = 4 + (FLIP ? 3 : 0);
-    __shared__ float stage[NF * kBlk * 65];
-    __shared__ float vout[3 * kBlk * 65];
+    constexpr int SS = 64 + LPC;
+    __shared__ float stage[NF * kBlk * SS];
+    __shared__ float vout[3 * kBlk * SS];
     const uint32_t n_active = L.ctr->n_active;
     const int ngroups = (int)((n_active + G - 1) / G);
     const int sub = threadIdx.x % LPC;
@@ -558,10 +565,10 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                     const uint32_t sl = slot[r];
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
-                        lds_gather(P.r + a * ld, sl, stage, a, r);
-                        if (FLIP) lds_gather(P.v + a * ld, sl, stage, 4 + a, r);
+                        lds_gather<SS>(P.r + a * ld, sl, stage, a, r);
+                        if (FLIP) lds_gather<SS>(P.v + a * ld, sl, stage, 4 + a, r);
                     }
-                    lds_gather(P.m, sl, stage, 3, r);
+                    lds_gather<SS>(P.m, sl, stage, 3, r);
                 }
             }
             wait_vm_all();
@@ -569,10 +576,10 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
             const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
             for (uint32_t pos = first_pos<LPC>(si.cs, lo, sub); pos < hi; pos += LPC) {
                 const int i = (int)(pos - base);
-                const float fx = stage[st_idx(0, i)];
-                const float fy = stage[st_idx(1, i)];
-                const float fz = stage[st_idx(2, i)];
-                const float m = stage[st_idx(3, i)];
+                const float fx = stage[st_idx<SS>(0, i)];
+                const float fy = stage[st_idx<SS>(1, i)];
+                const float fz = stage[st_idx<SS>(2, i)];
+                const float m = stage[st_idx<SS>(3, i)];
                 const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
                 float Nw[8];
                 float vp[3] = {0.f, 0.f, 0.f};
@@ -588,13 +595,13 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                             for (int c = 0; c < 3; c++) vp[c] += Nw[q] * nvl[q][c];
                         }
                 if (FLIP) {
-                    vp[0] = stage[st_idx(4, i)] + sc.dt * vp[0];
-                    vp[1] = stage[st_idx(5, i)] + sc.dt * vp[1];
-                    vp[2] = stage[st_idx(6, i)] + sc.dt * vp[2];
+                    vp[0] = stage[st_idx<SS>(4, i)] + sc.dt * vp[0];
+                    vp[1] = stage[st_idx<SS>(5, i)] + sc.dt * vp[1];
+                    vp[2] = stage[st_idx<SS>(6, i)] + sc.dt * vp[2];
                 }
-                vout[st_idx(0, i)] = vp[0];
-                vout[st_idx(1, i)] = vp[1];
-                vout[st_idx(2, i)] = vp[2];
+                vout[st_idx<SS>(0, i)] = vp[0];
+                vout[st_idx<SS>(1, i)] = vp[1];
+                vout[st_idx<SS>(2, i)] = vp[2];
                 const float mv[3] = {m * vp[0], m * vp[1], m * vp[2]};
 #pragma unroll
                 for (int q = 0; q < 8; q++)
@@ -607,9 +614,9 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                 const int i = 64 * r + t;
                 if (i < nch) {
                     const uint32_t sl = slot[r];
-                    P.v[sl] = vout[st_idx(0, i)];
-                    P.v[ld + sl] = vout[st_idx(1, i)];
-                    P.v[2 * ld + sl] = vout[st_idx(2, i)];
+                    P.v[sl] = vout[st_idx<SS>(0, i)];
+                    P.v[ld + sl] = vout[st_idx<SS>(1, i)];
+                    P.v[2 * ld + sl] = vout[st_idx<SS>(2, i)];
                 }
             }
             __syncthreads();
@@ -1104,10 +1111,18 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     }
     const uint32_t nk = inside ? (uint32_t)(ci[0] + ci[1] * g.nex[0] + ci[2] * nexy) : (uint32_t)g.nelem;
     if (bf.cnt) {
-        const uint32_t ck = valid ? nk : 0xFFFFFFFFu;
-        int len, head;
-        const bool h = wave_runs(ck, threadIdx.x & 63, &len, &head);
-        if (h && ck < (uint32_t)g.nelem) atomicAdd(&bf.cnt[ck], (uint32_t)len);
+        if (bf.delta) {
+            // kept counts: only a particle that changes cell moves one count
+            if (valid && nk != key) {
+                atomicSub(&bf.cnt[key], 1u);
+                if (nk < (uint32_t)g.nelem) atomicAdd(&bf.cnt[nk], 1u);
+            }
+        } else {
+            const uint32_t ck = valid ? nk : 0xFFFFFFFFu;
+            int len, head;
+            const bool h = wave_runs(ck, threadIdx.x & 63, &len, &head);
+            if (h && ck < (uint32_t)g.nelem) atomicAdd(&bf.cnt[ck], (uint32_t)len);
+        }
     }
     if (!valid) return;
     if (md.model == 10 && P.eps_p) P.eps_p[slot] = ep;

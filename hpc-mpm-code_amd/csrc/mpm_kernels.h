@@ -76,13 +76,14 @@ __device__ __forceinline__ bool wave_runs(uint32_t k, int lane, int *len, int *h
 // pass): counts of the new keys, and the clearing of tile flags and counters.
 struct BinFuse {
     uint32_t *cnt;  // null: not fused
+    int delta;      // 1: cnt holds the counts of the keys being replaced; only moves are applied
     uint32_t *tiles;
     int ntiles;
     BinCounters *ctr;
 };
 
 struct BinBuffers {
-    uint32_t *cnt;      // [bin_cell_capacity], zero between calls
+    uint32_t *cnt;      // [bin_cell_capacity]: zero, or the counts of the current keys (kept counts)
     uint32_t *cstart;   // [bin_cell_capacity]
     uint32_t *cend;     // [bin_cell_capacity]
     uint32_t *active;   // [nelem]
@@ -105,9 +106,12 @@ size_t bin_cell_capacity(int nelem);
 void launch_bin_count(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n,
                       BinBuffers &b);
 // counted: the per-cell counts, cleared tiles and counters were produced by
-// the previous update kernel (fused counting), so the count pass is skipped
+// the previous update kernel (fused counting), so the count pass is skipped.
+// keep: the counts are left in place (they are the counts of keys[], which
+// the next update kernel then only adjusts for the particles that change
+// cell); otherwise they are zeroed.
 int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
-               uint32_t *err, bool counted = false);
+               uint32_t *err, bool counted = false, bool keep = false);
 
 // ------------------------------------------------------ FP64 exact kernels
 struct PartF64 {
