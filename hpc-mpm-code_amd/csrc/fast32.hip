@@ -1172,7 +1172,7 @@ __device__ __forceinline__ void sur_load_row(const float *queue, uint32_t qi, bo
 // The evaluated / rejected tallies go to per-region counters: a region's
 // tiles may be spread over several waves, so they are added atomically, but
 // to distinct addresses.
-__global__ void __launch_bounds__(256) k32_j2_surrogate(PartF32 P, SurrogateArgs sa) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k32_j2_surrogate(PartF32 P, SurrogateArgs sa) {
     __shared__ MlpSmem sw;
     __shared__ float scl[64];
     __shared__ __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];
@@ -1376,9 +1376,16 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
 
 void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const SurrogateArgs &sa) {
     if (P.n <= 0) return;
-    // persistent blocks, two per CU of the 256 (this kernel's occupancy is
-    // two waves per SIMD)
-    const int blocks = std::min((int)sa.nregions, 512);
+    // persistent blocks: as many as are resident at once (queried once)
+    static int resident = 0;
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k32_j2_surrogate, 256, 0);
+        resident = std::max(1, cus) * std::max(1, per_cu);
+    }
+    const int blocks = std::min((int)sa.nregions, resident);
     hipLaunchKernelGGL(k32_j2_surrogate, dim3(blocks), dim3(256), 0, s, P, sa);
 }
 

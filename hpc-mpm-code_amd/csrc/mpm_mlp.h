@@ -33,6 +33,9 @@ __device__ void mlp_publish_param(const MlpParams &p, int i, float v);
 
 typedef short mlp_v4s __attribute__((ext_vector_type(4)));
 typedef float mlp_v4f __attribute__((ext_vector_type(4)));
+typedef uint32_t mlp_v2u __attribute__((ext_vector_type(2)));
+typedef float mlp_v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 mlp_v2bf __attribute__((ext_vector_type(2)));
 
 // Weights of one network staged in LDS by a whole block (bf16 bit patterns,
 // [out][in] row-major) and the FP32 biases.
@@ -51,6 +54,17 @@ __device__ __forceinline__ uint16_t mlp_f2bf(float x) {
     uint32_t u = __float_as_uint(x);
     u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
     return (uint16_t)(u >> 16);
+}
+
+// two floats to packed bf16 (a in the low half), round to nearest even: one
+// v_cvt_pk_bf16_f32 on gfx950; equal to mlp_f2bf for finite values
+__device__ __forceinline__ uint32_t mlp_pk_bf16(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((mlp_v2f){a, b}, mlp_v2bf));
+}
+
+// four floats to the bf16 operand fragment of one MFMA
+__device__ __forceinline__ mlp_v4s mlp_frag_bf16(float a, float b, float c, float d) {
+    return __builtin_bit_cast(mlp_v4s, (mlp_v2u){mlp_pk_bf16(a, b), mlp_pk_bf16(c, d)});
 }
 
 __device__ __forceinline__ void mlp_load_smem(MlpSmem &s, const MlpParams &p) {
@@ -93,8 +107,9 @@ __device__ __forceinline__ void mlp_wave_run(const MlpSmem &s, uint16_t *xs, int
         for (int hb = 0; hb < 4; hb++) {
             const mlp_v4s a = *(const mlp_v4s *)(s.w1 + (hb * 16 + c16) * kMlpIn + q4);
             mlp_v4f acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb[si], mlp_v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 4; r++) h1[hb][r] = (short)mlp_f2bf(fmaxf(acc[r] + s.b1[hb * 16 + q4 + r], 0.f));
+            const float *bb = s.b1 + hb * 16 + q4;
+            h1[hb] = mlp_frag_bf16(fmaxf(acc[0] + bb[0], 0.f), fmaxf(acc[1] + bb[1], 0.f), fmaxf(acc[2] + bb[2], 0.f),
+                                   fmaxf(acc[3] + bb[3], 0.f));
         }
 #pragma unroll
         for (int hb = 0; hb < 4; hb++) {
@@ -104,8 +119,9 @@ __device__ __forceinline__ void mlp_wave_run(const MlpSmem &s, uint16_t *xs, int
                 const mlp_v4s a = *(const mlp_v4s *)(s.w2 + (hb * 16 + c16) * kMlpHid + kb * 16 + q4);
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kb], acc, 0, 0, 0);
             }
-#pragma unroll
-            for (int r = 0; r < 4; r++) h2[hb][r] = (short)mlp_f2bf(fmaxf(acc[r] + s.b2[hb * 16 + q4 + r], 0.f));
+            const float *bb = s.b2 + hb * 16 + q4;
+            h2[hb] = mlp_frag_bf16(fmaxf(acc[0] + bb[0], 0.f), fmaxf(acc[1] + bb[1], 0.f), fmaxf(acc[2] + bb[2], 0.f),
+                                   fmaxf(acc[3] + bb[3], 0.f));
         }
         mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -126,7 +142,7 @@ __device__ __forceinline__ void mlp_stage_row(uint16_t *row, const float xn[kMlp
     uint32_t pk[kMlpIn / 2];
 #pragma unroll
     for (int f = 0; f < kMlpIn / 2; f++)
-        pk[f] = (uint32_t)mlp_f2bf(xn[2 * f]) | ((uint32_t)mlp_f2bf(xn[2 * f + 1]) << 16);
+        pk[f] = mlp_pk_bf16(xn[2 * f], xn[2 * f + 1]);
     uint4 *r4 = (uint4 *)row;
     r4[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
     r4[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
