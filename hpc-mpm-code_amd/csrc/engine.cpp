@@ -97,6 +97,7 @@ struct mpm_engine {
     uint32_t *perm = nullptr;    // bb.perm (or perm_id right after a relayout)
     uint32_t *perm_id = nullptr;
     uint32_t *d_err = nullptr;
+    double *d_ndispl = nullptr;  // [3*nodec] nodal displacement increment of the implicit G2P
     bool bins_valid = false;
     bool keys_valid = false;
     bool counts_ready = false;  // bb.cnt holds the per-cell counts of the current keys (else it is zero)
@@ -1374,6 +1375,87 @@ int mpm_g2p(mpm_engine *e, double facl) {
     if ((rc = upload_bc(e))) return rc;
     clear_err(e);
     if ((rc = do_g2p(e, facl, 1, nullptr))) return rc;
+    return check_err(e);
+}
+
+// ---------------------------------------------------------------- implicit path
+// Particle accelerations of the implicit transfers (FP64-exact engine),
+// allocated as zero on first use after an upload.
+static int implicit_require(mpm_engine *e) {
+    int rc = require_particles(e);
+    if (rc) return rc;
+    if (e->prec != MPM_PREC_FP64_EXACT)
+        return set_error(MPM_ERR_UNSUPPORTED, "the implicit transfers run in the FP64-exact engine");
+    if (e->prm.scheme != MPM_SCHEME_PIC && e->prm.scheme != MPM_SCHEME_FLIP)
+        return set_error(MPM_ERR_UNSUPPORTED, "the implicit transfers support PIC and FLIP");
+    if (!e->P64.a) {
+        if ((rc = e->dalloc(&e->P64.a, 3 * e->ld, true))) return rc;
+        MPM_HIP_CHECK(hipMemsetAsync(e->P64.a, 0, 3 * e->ld * sizeof(double), e->stream));
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    }
+    return 0;
+}
+
+static int accel_copy(mpm_engine *e, double *accel, bool to_device) {
+    if (!accel) return set_error(MPM_ERR_INVALID_ARGUMENT, "null acceleration array");
+    int rc = implicit_require(e);
+    if (rc) return rc;
+    const size_t ld = e->ld;
+    const int n = e->n;
+    std::vector<double> soa(3 * ld);
+    if (to_device) {
+        for (int i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) soa[c * ld + i] = accel[3 * (size_t)i + c];
+        if ((rc = h2d(e, e->P64.a, soa))) return rc;
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    } else {
+        if ((rc = d2h(e, soa, (const double *)e->P64.a, 3 * ld))) return rc;
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        for (int i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) accel[3 * (size_t)i + c] = soa[c * ld + i];
+    }
+    return 0;
+}
+
+int mpm_upload_accel(mpm_engine *e, const double *accel) {
+    if (e) e->drop_graph();
+    return accel_copy(e, const_cast<double *>(accel), true);
+}
+
+int mpm_download_accel(mpm_engine *e, double *accel) { return accel_copy(e, accel, false); }
+
+int mpm_implicit_p2g(mpm_engine *e, double facl) {
+    int rc = implicit_require(e);
+    if (rc) return rc;
+    if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "call mpm_bin before mpm_implicit_p2g");
+    if ((rc = upload_bc(e))) return rc;
+    clear_err(e);
+    launch_f64_implicit_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.mtol, facl);
+    MPM_HIP_CHECK(hipGetLastError());
+    e->last_pass = 1;
+    return check_err(e);
+}
+
+int mpm_implicit_g2p(mpm_engine *e, const double *ndispl, const double *nb_para, double gamma_nb) {
+    int rc = implicit_require(e);
+    if (rc) return rc;
+    if (!ndispl || !nb_para) return set_error(MPM_ERR_INVALID_ARGUMENT, "null ndispl or Newmark coefficients");
+    if (e->shifting || e->fbar)
+        return set_error(MPM_ERR_UNSUPPORTED, "the implicit G2P runs with particle shifting and F-bar off");
+    if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "call mpm_bin before mpm_implicit_g2p");
+    const size_t n3 = 3 * (size_t)e->g.nodec;
+    if (!e->d_ndispl && (rc = e->dalloc(&e->d_ndispl, n3, false))) return rc;
+    MPM_HIP_CHECK(hipMemcpyAsync(e->d_ndispl, ndispl, n3 * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    NewmarkF64 nb{};
+    for (int q = 0; q < 6; q++) nb.c[q] = nb_para[q];
+    e->drop_graph();
+    clear_err(e);
+    launch_f64_implicit_g2p(e->stream, e->g, e->P64, e->N64, e->mt, e->d_ndispl, nb, e->prm.dt, gamma_nb,
+                            e->prm.scheme == MPM_SCHEME_FLIP ? 1 : 0, e->d_err);
+    MPM_HIP_CHECK(hipGetLastError());
+    e->keys_valid = false;  // the positions moved
+    e->bins_valid = false;
+    e->last_pass = 2;
     return check_err(e);
 }
 

@@ -718,6 +718,172 @@ __global__ void k64_maksf(GridGeom g, int n, const int32_t *cell, const double *
             }
 }
 
+// ------------------------------------------------------------------------
+// Implicit-path transfers: the P2G and G2P halves of the reference's
+// implicit solid step (its PETSc solve is the caller's). PIC and FLIP.
+
+// SolidMaterialPoint::Particle2Node (solid/implicit/var_trans_solid_implicit.cpp:20-80)
+// for one node: mass, volume, momentum (VelP2G) and m a (AccelP2G,
+// map_and_interpolate.cpp:12-49) in the reference's order; velocity and
+// acceleration where the mass exceeds mtol; velocity BCs set, acceleration
+// BCs zeroed.
+__global__ void k64_implicit_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, double mtol,
+                                 double facl) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    const int nodec = g.nodec;
+    const size_t ld = P.ld;
+    int i = n % g.nxc[0];
+    int j = (n / g.nxc[0]) % g.nxc[1];
+    int k = n / (g.nxc[0] * g.nxc[1]);
+    int cells[27];
+    int nc = node_cells(g, i, j, k, cells);
+    double nm = 0.0, nvof = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = 0; q < nc; q++) {
+        int c = cells[q];
+        uint32_t s0 = L.cstart[c], s1 = L.cend[c];
+        for (uint32_t s = s0; s < s1; s++) {
+            uint32_t pid = L.perm[s];
+            double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+            double sfi, ds[3];
+            sf_node(g, xyg, i, j, k, sfi, ds);
+            double mp = P.m[pid], vp = P.V[pid];
+            nm += sfi * mp;
+            nvof += sfi * vp;
+            p0 += sfi * mp * P.v[pid];
+            p1 += sfi * mp * P.v[ld + pid];
+            p2 += sfi * mp * P.v[2 * ld + pid];
+            f0 += sfi * mp * P.a[pid];
+            f1 += sfi * mp * P.a[ld + pid];
+            f2 += sfi * mp * P.a[2 * ld + pid];
+        }
+    }
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (nm > mtol) {
+        v0 = p0 / nm;
+        v1 = p1 / nm;
+        v2 = p2 / nm;
+        a0 = f0 / nm;
+        a1 = f1 / nm;
+        a2 = f2 / nm;
+    }
+    apply_vel_bc(bc, n, nodec, facl, v0, v1, v2);
+    uint8_t mk = bc.mask ? bc.mask[n] : 0;
+    if (mk & 1) a0 = 0.0;
+    if (mk & 2) a1 = 0.0;
+    if (mk & 4) a2 = 0.0;
+    N.nm[n] = nm;
+    N.nvof[n] = nvof;
+    N.np[n] = p0;
+    N.np[nodec + n] = p1;
+    N.np[2 * nodec + n] = p2;
+    N.nf[n] = f0;
+    N.nf[nodec + n] = f1;
+    N.nf[2 * nodec + n] = f2;
+    N.nv[n] = v0;
+    N.nv[nodec + n] = v1;
+    N.nv[2 * nodec + n] = v2;
+    N.na[n] = a0;
+    N.na[nodec + n] = a1;
+    N.na[2 * nodec + n] = a2;
+}
+
+// PredictNewmarkBetaVelAndAccel + CommitNodalKinematics
+// (nonlinear_time_integration.cpp:59-77), every component of every node.
+__global__ void k64_newmark_nodes(int n3, const double *__restrict__ ndispl, double *nv, double *na, NewmarkF64 nb) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n3) return;
+    const double u = ndispl[n], v = nv[n], a = na[n];
+    const double vk = nb.c[0] * u - nb.c[1] * v - nb.c[2] * a;
+    const double ak = nb.c[3] * u - nb.c[4] * v - nb.c[5] * a;
+    nv[n] = vk;
+    na[n] = ak;
+}
+
+// Node2Particle (var_trans_solid_implicit.cpp:106-167) after the node update,
+// one thread per particle: dF = I + sum dN (x) ndispl (ComputeDeltaDefGrad,
+// material_point.cpp:127-151, af = 1, the identity added last), F = dF F, J,
+// V = J V0, the stress update; for non-rigid materials displ = sum N ndispl,
+// accel = sum N naccel and (PIC) vel = sum N nvel; then
+// CommitParticleKinematics (nonlinear_time_integration.cpp:79-96).
+__global__ void k64_implicit_g2p(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, const double *__restrict__ ndispl,
+                                 double dt, double gamma_nb, int flip, uint32_t *err) {
+    int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P.n) return;
+    uint32_t m = P.key[pid];
+    if (m >= (uint32_t)g.nelem) return;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    const double af = 1.0;
+    int ixe, iye, ize;
+    cell_ijk(g, (int)m, ixe, iye, ize);
+    double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+    double dd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k <= g.idimc[2]; k++)
+        for (int j = 0; j <= g.idimc[1]; j++)
+            for (int i = 0; i <= g.idimc[0]; i++) {
+                int nid = (ixe + i) + g.nxc[0] * (iye + j) + g.nxc[0] * g.nxc[1] * (ize + k);
+                double sfi, ds[3];
+                sf_node(g, xyg, ixe + i, iye + j, ize + k, sfi, ds);
+                double uu = ndispl[nid], uv = ndispl[nodec + nid], uw = ndispl[2 * nodec + nid];
+                dd[0] += ds[0] * af * uu;
+                dd[1] += ds[1] * af * uu;
+                dd[2] += ds[2] * af * uu;
+                dd[3] += ds[0] * af * uv;
+                dd[4] += ds[1] * af * uv;
+                dd[5] += ds[2] * af * uv;
+                dd[6] += ds[0] * af * uw;
+                dd[7] += ds[1] * af * uw;
+                dd[8] += ds[2] * af * uw;
+            }
+    dd[0] += 1.0;
+    dd[4] += 1.0;
+    dd[8] += 1.0;
+    double Fo[9], F[9];
+    for (int q = 0; q < 9; q++) Fo[q] = P.F[q * ld + pid];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int kk = 0; kk < 3; kk++) acc += dd[3 * i + kk] * Fo[3 * kk + j];
+            F[3 * i + j] = acc;
+        }
+    for (int q = 0; q < 9; q++) {
+        P.F[q * ld + pid] = F[q];
+        P.ddg[q * ld + pid] = dd[q];
+    }
+    const double J = det3(F);
+    P.J[pid] = J;
+    P.V[pid] = J * P.V0[pid];
+    const int mat = P.matid ? P.matid[pid] : 0;
+    const MaterialDev &md = mt.m[mat];
+    update_stress64(md, P.sig, ld, pid, F, dd, J, J, P.eps_p, dt);
+    double du[3] = {0.0, 0.0, 0.0}, ap[3] = {0.0, 0.0, 0.0};
+    double vp[3];
+    for (int c = 0; c < 3; c++) vp[c] = flip ? P.v[c * ld + pid] : 0.0;
+    if (md.model != -1) {
+        for (int k = 0; k <= g.idimc[2]; k++)
+            for (int j = 0; j <= g.idimc[1]; j++)
+                for (int i = 0; i <= g.idimc[0]; i++) {
+                    int nid = (ixe + i) + g.nxc[0] * (iye + j) + g.nxc[0] * g.nxc[1] * (ize + k);
+                    double sfi, ds[3];
+                    sf_node(g, xyg, ixe + i, iye + j, ize + k, sfi, ds);
+                    for (int c = 0; c < 3; c++) du[c] += sfi * ndispl[c * nodec + nid];
+                    for (int c = 0; c < 3; c++) ap[c] += sfi * N.na[c * nodec + nid];
+                    if (!flip)
+                        for (int c = 0; c < 3; c++) vp[c] += sfi * N.nv[c * nodec + nid];
+                }
+    }
+    for (int c = 0; c < 3; c++) {
+        if (flip) vp[c] += dt * ((1.0 - gamma_nb) * P.a[c * ld + pid] + gamma_nb * ap[c]);
+        P.v[c * ld + pid] = vp[c];
+        P.a[c * ld + pid] = ap[c];
+        P.x[c * ld + pid] = xyg[c] + du[c];
+    }
+    if (!(J > 0.0)) atomicOr(err, kErrJacobian);
+    if (!isfinite(xyg[0] + du[0]) || !isfinite(xyg[1] + du[1]) || !isfinite(xyg[2] + du[2]))
+        atomicOr(err, kErrNonFinite);
+}
+
 }  // namespace
 
 void launch_f64_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
@@ -872,6 +1038,21 @@ int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cel
                      double *sf, double *dsf) {
     if (n > 0) hipLaunchKernelGGL(k64_maksf, dim3(cdiv(n, 128)), dim3(128), 0, s, g, n, cell, xyg, nc, sf, dsf);
     return (int)hipGetLastError();
+}
+
+void launch_f64_implicit_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N,
+                              const CellLists &L, const BcDense &bc, double mtol, double facl) {
+    hipLaunchKernelGGL(k64_implicit_p2g, dim3(cdiv(g.nodec, 256)), dim3(256), 0, s, g, P, N, L, bc, mtol, facl);
+}
+
+void launch_f64_implicit_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N,
+                              const MaterialTable &mt, const double *ndispl, const NewmarkF64 &nb, double dt,
+                              double gamma_nb, int flip, uint32_t *err) {
+    const int n3 = 3 * g.nodec;
+    hipLaunchKernelGGL(k64_newmark_nodes, dim3(cdiv(n3, 256)), dim3(256), 0, s, n3, ndispl, N.nv, N.na, nb);
+    if (P.n > 0)
+        hipLaunchKernelGGL(k64_implicit_g2p, dim3(cdiv(P.n, 256)), dim3(256), 0, s, g, P, N, mt, ndispl, dt, gamma_nb,
+                           flip, err);
 }
 
 }  // namespace mpm

@@ -132,6 +132,7 @@ struct PartF64 {
     double *Fbar;   // [9][ld] F-bar deformation gradient (null when F-bar is off)
     double *Jbar;   // [ld] its determinant as projected
     double *detdd;  // [ld] det of the step's deformation-gradient increment (F-bar scratch)
+    double *a;      // [3][ld] particle acceleration of the implicit transfers (null until used)
     const uint32_t *key;
     size_t ld;
     int n;
@@ -171,6 +172,19 @@ void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const No
                     uint32_t *err, const ShiftF64 *shift = nullptr, double *njbar = nullptr);
 int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cell, const double *xyg, int32_t *nc,
                      double *sf, double *dsf);
+// Implicit-path transfers (reference SolidMaterialPoint::Particle2Node /
+// Node2Particle, PIC and FLIP): node mass, volume, momentum, m a, velocity
+// and acceleration with BCs; then, for the nodal displacement increment
+// ndispl ([u|v|w] blocks), the Newmark node update with coefficients
+// NewmarkBetaParaSet's c[0..5] and the particle update.
+struct NewmarkF64 {
+    double c[6];
+};
+void launch_f64_implicit_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N,
+                              const CellLists &L, const BcDense &bc, double mtol, double facl);
+void launch_f64_implicit_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N,
+                              const MaterialTable &mt, const double *ndispl, const NewmarkF64 &nb, double dt,
+                              double gamma_nb, int flip, uint32_t *err);
 
 // ------------------------------------------------------------ FP32 kernels
 struct PartF32 {

@@ -128,6 +128,10 @@ SIGNATURES = {
     "mpm_download_nodal_volume": (C.c_int, [C.c_void_p, _dp]),
     "mpm_download_affine": (C.c_int, [C.c_void_p, _dp, _dp]),
     "mpm_upload_affine": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "mpm_upload_accel": (C.c_int, [C.c_void_p, _dp]),
+    "mpm_download_accel": (C.c_int, [C.c_void_p, _dp]),
+    "mpm_implicit_p2g": (C.c_int, [C.c_void_p, C.c_double]),
+    "mpm_implicit_g2p": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double]),
     "mpm_set_fbar": (C.c_int, [C.c_void_p, C.c_int32]),
     "mpm_download_fbar": (C.c_int, [C.c_void_p, _dp, _dp]),
     "mpm_upload_fbar": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -341,6 +345,25 @@ class Engine:
         cm = np.ascontiguousarray(cmat, np.float64)
         di = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
         check(self.lib.mpm_upload_affine(self.h, _ptr(cm), _ptr(di)))
+
+    def accel(self, n):
+        """Particle accelerations of the implicit transfers, [n][3]."""
+        a = np.zeros((n, 3))
+        check(self.lib.mpm_download_accel(self.h, _ptr(a)))
+        return a
+
+    def set_accel(self, accel):
+        a = np.ascontiguousarray(accel, np.float64)
+        check(self.lib.mpm_upload_accel(self.h, _ptr(a)))
+
+    def implicit_p2g(self, facl=1.0):
+        check(self.lib.mpm_implicit_p2g(self.h, facl))
+
+    def implicit_g2p(self, ndispl, nb_para, gamma_nb):
+        nd = np.ascontiguousarray(ndispl, np.float64)
+        nb = np.ascontiguousarray(nb_para, np.float64)
+        assert nd.size == 3 * self.nodec and nb.size == 6
+        check(self.lib.mpm_implicit_g2p(self.h, _ptr(nd), _ptr(nb), gamma_nb))
 
     def info(self):
         i = EngineInfo()

@@ -268,6 +268,37 @@ int mpm_download_nodal_volume(mpm_engine *eng, double *nvol);
 int mpm_download_affine(mpm_engine *eng, double *cmat, double *dinv);
 int mpm_upload_affine(mpm_engine *eng, const double *cmat, const double *dinv);
 
+/* Implicit-path transfers of the reference's implicit solid step, FP64-exact
+ * engine, PIC and FLIP (MPM_ERR_UNSUPPORTED otherwise); the implicit solve
+ * itself (PETSc there) is the caller's.
+ *
+ * Particle accelerations, host [n][3] (MaterialPoint::accel,
+ * module/material_point.h:113-121); zero after an upload until set.
+ *
+ * mpm_implicit_p2g replaces SolidMaterialPoint::Particle2Node
+ * (module/solid/implicit/var_trans_solid_implicit.cpp:20-80): node mass,
+ * volume, momentum sum N m v (VelP2G) and force sum N m a (AccelP2G,
+ * module/map_and_interpolate.cpp:12-49) in the reference's summation order;
+ * nvel = nmome / nmass, naccel = nforce / nmass where nmass > mtol; velocity
+ * BCs set to fbc * facl, acceleration BCs zeroed (module/bc.cpp:45-67).
+ * Needs a current binning (mpm_bin). Results: mpm_download_nodes.
+ *
+ * mpm_implicit_g2p replaces SolidMaterialPoint::Node2Particle (:106-167)
+ * for the converged nodal displacement increment ndispl ([u|v|w] blocks of
+ * nodec): the Newmark predictor of the node velocity and acceleration
+ * (PredictNewmarkBetaVelAndAccel + CommitNodalKinematics,
+ * module/nonlinear_time_integration.cpp:59-77) with nb_para the six values of
+ * NewmarkBetaParaSet (:43-57); per particle dF = I + sum dN (x) ndispl
+ * (ComputeDeltaDefGrad, module/material_point.cpp:127-151), F = dF F, J,
+ * V = J V0 and the stress update; for non-rigid materials the displacement,
+ * acceleration and (PIC) velocity gathers; CommitParticleKinematics (:79-96)
+ * with gamma_nb. Particle shifting and F-bar must be off. Invalidates the
+ * binning. */
+int mpm_upload_accel(mpm_engine *eng, const double *accel);
+int mpm_download_accel(mpm_engine *eng, double *accel);
+int mpm_implicit_p2g(mpm_engine *eng, double facl);
+int mpm_implicit_g2p(mpm_engine *eng, const double *ndispl, const double *nb_para, double gamma_nb);
+
 /* F-bar projection (cal_def_grad_bar, module/solid/explicit/fbar_projection.cpp:18-94;
  * Fbar_flag, off by default in the reference), FP64-exact engine only. After
  * the deformation-gradient update, N (Jbar det dF) V is gathered to the

@@ -881,3 +881,172 @@ int orc_run(orc_state *st, int ista, int iend) {
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Implicit-path transfers: the P2G and G2P halves of the reference's
+ * implicit solid step, without its PETSc solve (the nodal displacement
+ * increment comes from the caller). PIC and FLIP; particle shifting and
+ * F-bar off.
+ * ------------------------------------------------------------------------ */
+
+/* NewmarkBetaParaSet, module/nonlinear_time_integration.cpp:43-57 */
+void orc_newmark_para(double gamma_nb, double beta_nb, double dt, double nb[6]) {
+    nb[0] = gamma_nb / (beta_nb * dt);
+    nb[1] = gamma_nb / beta_nb - 1.0;
+    nb[2] = dt / 2.0 * (gamma_nb / beta_nb - 2.0);
+    nb[3] = 1.0 / (beta_nb * dt * dt);
+    nb[4] = 1.0 / (beta_nb * dt);
+    nb[5] = 1.0 / (2.0 * beta_nb) - 1.0;
+}
+
+/* SolidMaterialPoint::Particle2Node, module/solid/implicit/var_trans_solid_implicit.cpp:20-80:
+ * nmass, nvof, nmome = sum N m v (VelP2G) and nforce = sum N m a (AccelP2G,
+ * module/map_and_interpolate.cpp:12-49) in the cell / particle-list order;
+ * nvel = nmome / nmass and naccel = nforce / nmass where nmass > mtol (else
+ * 0); velocity BCs set (u, v, w lists, then the rigid list on all three
+ * components), acceleration BCs zeroed. accel: [num][3]. */
+void orc_implicit_p2g(orc_state *st, const double *accel, double facl) {
+    const int nn = st->nodec;
+    int32_t ncm[MAXNEN];
+    double sf[MAXNEN], dsf[3 * MAXNEN];
+    memset(st->nmass, 0, sizeof(double) * nn);
+    memset(st->nvof, 0, sizeof(double) * nn);
+    memset(st->nmome, 0, sizeof(double) * 3 * nn);
+    memset(st->nforce, 0, sizeof(double) * 3 * nn);
+    for (int m = 0; m < st->nelem; m++) {
+        for (int pid = st->idepf[m]; pid != -1; pid = st->idp2p[pid]) {
+            const double *xyp = st->coord + 3 * pid;
+            const double mp = st->mass[pid], vp = st->vol[pid];
+            const double *v = st->vel + 3 * pid, *a = accel + 3 * pid;
+            int nenode = orc_maksf(st, m, xyp, ncm, sf, dsf);
+            for (int ni = 0; ni < nenode; ni++) {
+                int nid = ncm[ni];
+                double sfi = sf[ni];
+                st->nmass[nid] += sfi * mp;
+                st->nvof[nid] += sfi * vp;
+                for (int c = 0; c < 3; c++) st->nmome[nid + c * nn] += sfi * mp * v[c];
+                for (int c = 0; c < 3; c++) st->nforce[nid + c * nn] += sfi * mp * a[c];
+            }
+        }
+    }
+    memset(st->nvel, 0, sizeof(double) * 3 * nn);
+    memset(st->naccel, 0, sizeof(double) * 3 * nn);
+    for (int n = 0; n < nn; n++) {
+        if (st->nmass[n] > st->mtol) {
+            for (int c = 0; c < 3; c++) {
+                st->nvel[n + c * nn] = st->nmome[n + c * nn] / st->nmass[n];
+                st->naccel[n + c * nn] = st->nforce[n + c * nn] / st->nmass[n];
+            }
+        }
+    }
+    apply_vel_bc(st, st->nvel, facl);
+    orc_bc_set_zero(st, 0, 0, st->naccel);
+    orc_bc_set_zero(st, 1, nn, st->naccel);
+    orc_bc_set_zero(st, 2, 2 * nn, st->naccel);
+    orc_bc_set_zero(st, 3, 0, st->naccel);
+    orc_bc_set_zero(st, 3, nn, st->naccel);
+    orc_bc_set_zero(st, 3, 2 * nn, st->naccel);
+}
+
+/* SolidMaterialPoint::Node2Particle, var_trans_solid_implicit.cpp:106-167,
+ * for a converged nodal displacement increment ndispl ([u|v|w] blocks):
+ *   PredictNewmarkBetaVelAndAccel + CommitNodalKinematics
+ *     (nonlinear_time_integration.cpp:59-77): nvel, naccel from ndispl;
+ *   per particle: dF = I + sum dN (x) ndispl (ComputeDeltaDefGrad,
+ *     module/material_point.cpp:127-151 with af = 1, the identity added
+ *     last), F = dF F, J, V = J V0, the stress update; for non-rigid
+ *     materials displ = sum N ndispl, accel = sum N naccel and (not FLIP)
+ *     vel = sum N nvel;
+ *   CommitParticleKinematics (nonlinear_time_integration.cpp:79-96): FLIP
+ *     vel += dt ((1 - gamma) accel_old + gamma accel); x += displ.
+ * accel: [num][3], updated. Returns -1 for TPIC/APIC, -2 with shifting or
+ * F-bar on. */
+int orc_implicit_g2p(orc_state *st, double *accel, const double *ndispl, const double nb[6], double gamma_nb) {
+    if (st->scheme != ORC_PIC && st->scheme != ORC_FLIP) return -1;
+    if (st->shifting || st->fbar) return -2;
+    const int nn = st->nodec, np = st->num;
+    const double dt = st->dt, af = 1.0;
+    const int flip = st->scheme == ORC_FLIP;
+    int32_t ncm[MAXNEN];
+    double sf[MAXNEN], dsf[3 * MAXNEN];
+    for (int n = 0; n < 3 * nn; n++) {
+        const double u = ndispl[n], v = st->nvel[n], a = st->naccel[n];
+        const double vk = nb[0] * u - nb[1] * v - nb[2] * a;
+        const double ak = nb[3] * u - nb[4] * v - nb[5] * a;
+        st->nvel[n] = vk;
+        st->naccel[n] = ak;
+    }
+    double *accel_old = NULL;
+    if (flip) {
+        accel_old = malloc(sizeof(double) * 3 * (size_t)np);
+        memcpy(accel_old, accel, sizeof(double) * 3 * (size_t)np);
+    }
+    memset(accel, 0, sizeof(double) * 3 * (size_t)np);
+    if (!flip) memset(st->vel, 0, sizeof(double) * 3 * (size_t)np);
+    double *displ = calloc(3 * (size_t)np, sizeof(double));
+    for (int m = 0; m < st->nelem; m++) {
+        for (int pid = st->idepf[m]; pid != -1; pid = st->idp2p[pid]) {
+            const double *xp = st->coord + 3 * pid;
+            int nenode = orc_maksf(st, m, xp, ncm, sf, dsf);
+            double *dd = st->ddg + 9 * pid;
+            for (int i = 0; i < 9; i++) dd[i] = 0.0;
+            for (int ni = 0; ni < nenode; ni++) {
+                int nid = ncm[ni];
+                double d1 = dsf[3 * ni], d2 = dsf[3 * ni + 1], d3 = dsf[3 * ni + 2];
+                double uu = ndispl[nid], uv = ndispl[nid + nn], uw = ndispl[nid + 2 * nn];
+                dd[0] += d1 * af * uu;
+                dd[1] += d2 * af * uu;
+                dd[2] += d3 * af * uu;
+                dd[3] += d1 * af * uv;
+                dd[4] += d2 * af * uv;
+                dd[5] += d3 * af * uv;
+                dd[6] += d1 * af * uw;
+                dd[7] += d2 * af * uw;
+                dd[8] += d3 * af * uw;
+            }
+            dd[0] += 1.0;
+            dd[4] += 1.0;
+            dd[8] += 1.0;
+            double *Fo = st->def_grad + 9 * pid;
+            double F[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double acc = 0.0;
+                    for (int k = 0; k < 3; k++) acc += dd[3 * i + k] * Fo[3 * k + j];
+                    F[3 * i + j] = acc;
+                }
+            memcpy(Fo, F, sizeof(F));
+            st->det_def_grad[pid] = orc_det3(F);
+            st->vol[pid] = st->det_def_grad[pid] * st->vol0[pid];
+            int mat = st->matid ? st->matid[pid] : 0;
+            int model = st->iprop[mat];
+            const double *prop = st->mat_prop[mat];
+            if (model == ORC_J2)
+                j2_update(st, pid, prop, dd);
+            else if (model == ORC_TAIT)
+                orc_tait_stress(prop, st->det_def_grad[pid], dd, dt, st->stress + 6 * pid);
+            else
+                orc_update_stress(model, st->stress + 6 * pid, F, dd, prop, st->det_def_grad[pid],
+                                  st->det_def_grad[pid]);
+            if (model != ORC_RIGID) {
+                for (int ni = 0; ni < nenode; ni++) {
+                    int nid = ncm[ni];
+                    double sfi = sf[ni];
+                    for (int c = 0; c < 3; c++) displ[3 * pid + c] += sfi * ndispl[nid + c * nn];
+                    for (int c = 0; c < 3; c++) accel[3 * pid + c] += sfi * st->naccel[nid + c * nn];
+                    if (!flip)
+                        for (int c = 0; c < 3; c++) st->vel[3 * pid + c] += sfi * st->nvel[nid + c * nn];
+                }
+            }
+        }
+    }
+    for (int p = 0; p < np; p++)
+        for (int i = 0; i < 3; i++) {
+            if (flip)
+                st->vel[3 * p + i] += dt * ((1.0 - gamma_nb) * accel_old[3 * p + i] + gamma_nb * accel[3 * p + i]);
+            st->coord[3 * p + i] += displ[3 * p + i];
+        }
+    free(displ);
+    free(accel_old);
+    return 0;
+}

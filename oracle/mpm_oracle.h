@@ -168,6 +168,17 @@ void orc_fbar(orc_state *st);
  * (returns -1000 for those). nthreads <= 0 keeps the OpenMP default. */
 int orc_run_omp(orc_state *st, int ista, int iend, int nthreads);
 
+/* Implicit-path transfers (the P2G / G2P halves of the reference's implicit
+ * solid step; the nodal displacement increment ndispl, [u|v|w] blocks, is
+ * the caller's). Newmark parameters of NewmarkBetaParaSet
+ * (module/nonlinear_time_integration.cpp:43-57); Particle2Node
+ * (module/solid/implicit/var_trans_solid_implicit.cpp:20-80) with particle
+ * accelerations accel [num][3]; Node2Particle (:106-167) for PIC/FLIP
+ * (-1 otherwise; -2 with shifting or F-bar), updating accel. */
+void orc_newmark_para(double gamma_nb, double beta_nb, double dt, double nb_para[6]);
+void orc_implicit_p2g(orc_state *st, const double *accel, double facl);
+int orc_implicit_g2p(orc_state *st, double *accel, const double *ndispl, const double nb_para[6], double gamma_nb);
+
 /* Control-point coordinate on one axis (DefineCpPos, module/mesh.h:32-47, one rank). */
 double orc_node_coord(const orc_state *st, int axis, int idx);
 

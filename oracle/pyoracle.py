@@ -79,6 +79,9 @@ def load(fast=False):
     lib.orc_node_coord.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int]
     lib.orc_node_coord.restype = C.c_double
     lib.orc_fbar.argtypes = [C.POINTER(OrcState)]
+    lib.orc_newmark_para.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+    lib.orc_implicit_p2g.argtypes = [C.POINTER(OrcState), _dp, C.c_double]
+    lib.orc_implicit_g2p.argtypes = [C.POINTER(OrcState), _dp, _dp, _dp, C.c_double]
     if fast == "omp":
         lib.orc_run_omp.argtypes = [C.POINTER(OrcState), C.c_int, C.c_int, C.c_int]
     _fp = C.POINTER(C.c_float)
@@ -89,6 +92,13 @@ def load(fast=False):
 
 
 MLP_NPARAMS = 6288
+
+
+def newmark_para(gamma_nb, beta_nb, dt):
+    """The six Newmark coefficients of NewmarkBetaParaSet."""
+    nb = np.zeros(6)
+    load().orc_newmark_para(gamma_nb, beta_nb, dt, P(nb))
+    return nb
 
 
 def mlp_forward(params, scaling, inp, bf16=True):
@@ -241,6 +251,19 @@ class Oracle:
 
     def g2p(self, facl=1.0):
         self.lib.orc_g2p(C.byref(self.st), facl)
+
+    def implicit_p2g(self, accel, facl=1.0):
+        """Particle2Node of the implicit path; accel: (n, 3) particle accelerations."""
+        accel = np.ascontiguousarray(accel, np.float64)
+        self.lib.orc_implicit_p2g(C.byref(self.st), P(accel), facl)
+
+    def implicit_g2p(self, accel, ndispl, nb_para, gamma_nb):
+        """Node2Particle of the implicit path for nodal displacement ndispl
+        ([u|v|w] blocks); accel ((n, 3), C-contiguous FP64) is updated in place."""
+        assert accel.flags.c_contiguous and accel.dtype == np.float64
+        nd = np.ascontiguousarray(ndispl, np.float64)
+        nb = np.ascontiguousarray(nb_para, np.float64)
+        return self.lib.orc_implicit_g2p(C.byref(self.st), P(accel), P(nd), P(nb), gamma_nb)
 
     def step(self, istep):
         return self.lib.orc_step(C.byref(self.st), istep)

@@ -610,3 +610,63 @@ def test_restart_file_continues_a_run(tmp_path, scheme):
     c = capi.Engine(s, precision=capi.PREC_FP64_EXACT)
     c.restart_read(f)
     np.testing.assert_allclose(c.download(s.n).coord, a.download(s.n).coord, rtol=1e-6)
+
+
+# ------------------------------------------------------------ implicit path
+GAMMA_NB, BETA_NB = 0.6, 0.3025
+
+
+@pytest.mark.parametrize("scheme", [capi.SCHEME_PIC, capi.SCHEME_FLIP])
+@pytest.mark.parametrize("model,props", [(capi.MODEL_LINELAST, (1e6, 0.3)), (capi.MODEL_NEOHOOKEAN, (5e5, 0.35))])
+def test_implicit_transfers_bitexact(scheme, model, props):
+    """Particle2Node / Node2Particle of the reference's implicit path
+    (var_trans_solid_implicit.cpp:20-167) for a caller's displacement
+    increment, three rounds: node fields, particles and accelerations equal
+    the oracle bit for bit (Neo-Hookean stress to the device log's last bits).
+    A third of the particles are rigid; the floor BCs apply."""
+    s = scenes.tiny(n_cells=8, model=model, props=props, scheme=scheme)
+    s.materials = [(model, list(props)), (capi.MODEL_RIGID, [0.0, 0.0])]
+    s.parts.matid = (np.arange(s.n) % 3 == 0).astype(np.int32)
+    rng = np.random.default_rng(11)
+    s.parts.def_grad[:] = np.eye(3) + 1e-3 * rng.standard_normal((s.n, 3, 3))
+    s.parts.det_def_grad[:] = np.linalg.det(s.parts.def_grad)
+    accel = np.ascontiguousarray(rng.standard_normal((s.n, 3)))
+    o = pyoracle.Oracle(s)
+    e = engine(s)
+    e.set_accel(accel)
+    nb = pyoracle.newmark_para(GAMMA_NB, BETA_NB, s.dt)
+    for rnd in range(3):
+        assert o.bin() == 0
+        e.bin()
+        o.implicit_p2g(accel, 0.75)
+        e.implicit_p2g(0.75)
+        nd = e.nodes()
+        for f in ("nmass", "nvof", "nmome", "nforce", "nvel", "naccel"):
+            assert np.array_equal(getattr(nd, f), getattr(o, f)), (rnd, f)
+        ndispl = 1e-5 * rng.standard_normal(3 * o.nodec)
+        assert o.implicit_g2p(accel, ndispl, nb, GAMMA_NB) == 0
+        e.implicit_g2p(ndispl, nb, GAMMA_NB)
+        nd = e.nodes()
+        assert np.array_equal(nd.nvel, o.nvel) and np.array_equal(nd.naccel, o.naccel), rnd
+        got = e.download(s.n)
+        assert_parts_equal(got, o.parts, ("coord", "vel", "def_grad", "det_def_grad", "vol"))
+        assert np.array_equal(e.accel(s.n), accel), rnd
+        if model == capi.MODEL_LINELAST:
+            assert np.array_equal(got.stress, o.parts.stress), rnd
+        else:  # log() of the device math library may differ in the last bit
+            np.testing.assert_allclose(got.stress, o.parts.stress, rtol=1e-12,
+                                       atol=1e-12 * np.abs(o.parts.stress).max())
+
+
+def test_implicit_transfers_reject_fp32_and_affine():
+    s = scenes.tiny(n_cells=8)
+    e = engine(s, capi.PREC_FP32)
+    with pytest.raises(capi.MPMError) as ei:
+        e.implicit_p2g()
+    assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
+    s = scenes.tiny(n_cells=8, scheme=capi.SCHEME_APIC)
+    e = engine(s)
+    e.bin()
+    with pytest.raises(capi.MPMError) as ei:
+        e.implicit_p2g()
+    assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
