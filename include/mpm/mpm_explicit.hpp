@@ -162,6 +162,30 @@ class ExplicitSolid {
     void Node2ParticleS(double facl) { check(mpm_g2p(h_, facl)); }
     // istep0 .. istep0 + nsteps - 1, facl from nlstep as main.cpp:43-48
     void Step(int istep0, int nsteps) { check(mpm_step(h_, istep0, nsteps)); }
+
+    // Implicit path (FP64-exact engine, PIC/FLIP): the transfers of
+    // SolidMaterialPoint::Particle2Node / Node2Particle
+    // (module/solid/implicit/var_trans_solid_implicit.cpp:20-80, :106-167)
+    // around the reference's own implicit solve, which supplies the converged
+    // nodal displacement increment ndispl ([u|v|w] blocks of nodec). nb_para
+    // and gamma_nb are the reference's Newmark values (NewmarkBetaParaSet,
+    // module/nonlinear_time_integration.cpp:43-57). SP::accel is the point
+    // container's vector<array<double,3>> of accelerations.
+    template <class SP>
+    void UploadAccel(SP &sp) {
+        check(mpm_upload_accel(h_, detail::dptr(sp.accel)));
+    }
+    template <class SP>
+    void DownloadAccel(SP &sp) {
+        sp.accel.resize((size_t)n_);
+        check(mpm_download_accel(h_, detail::dptr(sp.accel)));
+    }
+    void Particle2Node(double facl) { check(mpm_implicit_p2g(h_, facl)); }
+    void Node2Particle(const std::vector<double> &ndispl, const std::vector<double> &nb_para, double gamma_nb) {
+        if ((int64_t)ndispl.size() != 3 * nodec_ || nb_para.size() < 6)
+            throw Error(MPM_ERR_INVALID_ARGUMENT, "ndispl needs 3 * nodec values and nb_para 6");
+        check(mpm_implicit_g2p(h_, ndispl.data(), nb_para.data(), gamma_nb));
+    }
     void Sync() { check(mpm_sync(h_)); }
 
     mpm_engine *handle() { return h_; }

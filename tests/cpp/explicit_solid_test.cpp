@@ -7,7 +7,10 @@
 //      particle state and node mass bit-identical to the oracle after 40 steps;
 //   2. linked lists after MeshPointLinklist equal the oracle's;
 //   3. FP32 engine with Step(): positions within 1e-6 (domain length 1);
-//   4. a particle outside the grid raises mpm::Error(MPM_ERR_OUT_OF_GRID).
+//   4. a particle outside the grid raises mpm::Error(MPM_ERR_OUT_OF_GRID);
+//   5. implicit transfers (Particle2Node, Node2Particle for a given nodal
+//      displacement) from the state after step 1: nodes, particles and
+//      accelerations bit-identical to the oracle.
 // Exit status 0 on success.
 #include <array>
 #include <cmath>
@@ -25,6 +28,7 @@ struct Points {  // field names and types of the reference's point container
     std::vector<std::array<double, 3>> coord, vel, trac_force;
     std::vector<std::array<double, 6>> stress;
     std::vector<std::array<std::array<double, 3>, 3>> def_grad;
+    std::vector<std::array<double, 3>> accel;
     std::vector<int> idepf, idp2p, numep;
     std::vector<double> nmass, nvof, nmome, nvel, naccel, nforce;
 };
@@ -237,6 +241,44 @@ int main() {
                 std::printf("out-of-grid particle: %s\n", e.what());
             }
             EXPECT(code == MPM_ERR_OUT_OF_GRID, "expected MPM_ERR_OUT_OF_GRID, got %d", code);
+        }
+        // ---- 5. implicit transfers from the state the oracle reached
+        {
+            Points p = o;  // the oracle's state after the steps above
+            p.accel.assign(n, {0, 0, 0});
+            for (int i = 0; i < n; i++)
+                for (int c = 0; c < 3; c++) p.accel[i][c] = 0.1 * std::sin(0.37 * i + c);
+            std::vector<double> oacc(3 * (size_t)n);
+            std::memcpy(oacc.data(), &p.accel[0][0], oacc.size() * sizeof(double));
+            mpm::ExplicitSolid eng(grid(), mats, cfg);
+            for (int c = 0; c < 3; c++) eng.SetBC(c, fb);
+            eng.UploadPoints(p);
+            eng.UploadAccel(p);
+            eng.MeshPointLinklist();
+            eng.Particle2Node(1.0);
+            orc_bin(&st);
+            orc_implicit_p2g(&st, oacc.data(), 1.0);
+            eng.DownloadNodes(p);
+            const double dnv = maxdiff(p.nvel, st.nvel, 1), dna = maxdiff(p.naccel, st.naccel, 1),
+                         dnf = maxdiff(p.nforce, st.nforce, 1);
+            std::vector<double> ndispl(3 * (size_t)nodec), nb(6);
+            for (size_t q = 0; q < ndispl.size(); q++) ndispl[q] = 1e-6 * std::cos(0.11 * (double)q);
+            orc_newmark_para(0.6, 0.3025, DT, nb.data());
+            eng.Node2Particle(ndispl, nb, 0.6);
+            if (orc_implicit_g2p(&st, oacc.data(), ndispl.data(), nb.data(), 0.6) != 0) {
+                std::printf("FAIL: oracle implicit G2P\n");
+                return 2;
+            }
+            eng.DownloadPoints(p);
+            eng.DownloadAccel(p);
+            const double dx = maxdiff(p.coord, st.coord, 3), dv = maxdiff(p.vel, st.vel, 3),
+                         df = maxdiff(p.def_grad, st.def_grad, 9), ds = maxdiff(p.stress, st.stress, 6),
+                         da = maxdiff(p.accel, oacc.data(), 3);
+            std::printf("implicit transfers: max |diff| nvel %.3g naccel %.3g nforce %.3g coord %.3g vel %.3g F %.3g "
+                        "stress %.3g accel %.3g\n",
+                        dnv, dna, dnf, dx, dv, df, ds, da);
+            EXPECT(dnv == 0 && dna == 0 && dnf == 0 && dx == 0 && dv == 0 && df == 0 && ds == 0 && da == 0,
+                   "implicit transfers not bit-identical");
         }
     } catch (const mpm::Error &e) {
         std::printf("FAIL: %s\n", e.what());
