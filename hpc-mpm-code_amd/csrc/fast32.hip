@@ -75,8 +75,9 @@ __device__ __forceinline__ void group_setup(const GridGeom &g, const CellLists &
     const uint32_t a0 = (uint32_t)grp * G;
     const uint32_t a = a0 + t / L;
     const uint32_t a_end = min(a0 + (uint32_t)G, n_active);
-    S = L_.astart[a0];
-    E = L_.astart[a_end];
+    // uniform: scalar loads, which do not hold up the vector-memory counter
+    S = ((const __attribute__((address_space(4))) uint32_t *)L_.astart)[a0];
+    E = ((const __attribute__((address_space(4))) uint32_t *)L_.astart)[a_end];
     ci.c = -1;
     ci.cs = ci.ce = 0u;
     ci.ix = ci.iy = ci.iz = 0;
@@ -151,10 +152,30 @@ __device__ __forceinline__ void lds_gather2(const double *field_base, uint32_t s
 // wait for every outstanding vector-memory load (including LDS DMA) of this wave
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
 
+// wait until at most the kBlk most recent vector-memory loads of this wave
+// are in flight (vmcnt(4); loads complete in issue order), leaving the slot
+// prefetch of the next chunk outstanding
+static_assert(kBlk == 4, "the wait below encodes vmcnt(4)");
+__device__ __forceinline__ void wait_vm_keep_slots() { __builtin_amdgcn_s_waitcnt(0x0F74); }
+
+// A cells-kernel block is one wave: LDS written by its own DMA is visible
+// after the vmcnt wait, and its own LDS accesses execute in order, so a
+// compiler barrier (no hardware barrier, whose fence would wait for every
+// outstanding load) orders the stage buffer's reuse.
+static_assert(kSeg == 64, "cells kernels assume one wave per block");
+__device__ __forceinline__ void wave_fence() { asm volatile("" ::: "memory"); }
+
 // slot indices of the chunk starting at sorted position base (nch particles)
 __device__ __forceinline__ void load_slots(const CellLists &L, uint32_t base, int nch, uint32_t slot[kBlk]) {
 #pragma unroll
     for (int r = 0; r < kBlk; r++) slot[r] = (64 * r + (int)threadIdx.x < nch) ? L.perm[base + 64 * r + threadIdx.x] : 0u;
+}
+
+// the same for every lane, positions clamped into [base, end): exactly kBlk
+// load instructions, whatever the chunk length
+__device__ __forceinline__ void load_slots_all(const CellLists &L, uint32_t base, uint32_t end, uint32_t slot[kBlk]) {
+#pragma unroll
+    for (int r = 0; r < kBlk; r++) slot[r] = L.perm[min(base + 64u * r + threadIdx.x, end - 1u)];
 }
 
 // PF: software prefetch. The next group's cell ranges are requested at the
@@ -230,15 +251,25 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                     }
                 }
             }
-            if (PF) {  // slots of the next chunk: later in this group or the first of the next group
+            // slots of the next chunk (later in this group or the first of the
+            // next group) load while this chunk's fields are in flight
+            uint32_t slot_n[kBlk];
+            bool pf_out = false;
+            if (PF) {
                 const uint32_t nb = base + kChunk;
-                if (nb < E)
-                    load_slots(L, nb, (int)min((uint32_t)kChunk, E - nb), slot);
-                else if (grp_n < ngroups)
-                    load_slots(L, S_n, (int)min((uint32_t)kChunk, E_n - S_n), slot);
+                if (nb < E) {
+                    load_slots_all(L, nb, E, slot_n);
+                    pf_out = true;
+                } else if (grp_n < ngroups) {
+                    load_slots_all(L, S_n, E_n, slot_n);
+                    pf_out = true;
+                }
             }
-            wait_vm_all();
-            __syncthreads();
+            if (pf_out)
+                wait_vm_keep_slots();
+            else
+                wait_vm_all();
+            wave_fence();
             const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
             for (uint32_t pos = first_pos<LPC>(si.cs, lo, sub); pos < hi; pos += LPC) {
                 const int i = (int)(pos - base);
@@ -305,7 +336,11 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                             if (DIAG) vof[q] += N * (float)Vd;
                         }
             }
-            __syncthreads();
+            wave_fence();
+            if (pf_out) {
+#pragma unroll
+                for (int r = 0; r < kBlk; r++) slot[r] = slot_n[r];
+            }
         }
         // partial of entry q * 7 + f (f: m, p, force) and 56 + q (volume)
         constexpr int NV = DIAG ? 64 : 56;
