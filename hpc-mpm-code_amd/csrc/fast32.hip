@@ -1208,10 +1208,10 @@ __device__ __forceinline__ void sur_load_row(const float *queue, uint32_t qi, bo
 // tiles may be spread over several waves, so they are added atomically, but
 // to distinct addresses.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k32_j2_surrogate(PartF32 P, SurrogateArgs sa) {
-    __shared__ MlpSmem sw;
+    __shared__ MlpInferSmem sw;
     __shared__ float scl[64];
     __shared__ __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];
-    mlp_load_smem(sw, sa.net);
+    mlp_load_infer_smem(sw, sa.net);
     if (threadIdx.x < 64) scl[threadIdx.x] = sa.net_scaling[threadIdx.x];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

@@ -113,9 +113,9 @@ __device__ __forceinline__ void stage_inputs(uint16_t *x, const float *raw, cons
 // One sample per lane; the same wave routine the fused constitutive update uses.
 __global__ void __launch_bounds__(256) k_mlp_infer(MlpParams p, const float *__restrict__ in, float *__restrict__ out,
                                                    int n) {
-    __shared__ MlpSmem sw;
+    __shared__ MlpInferSmem sw;
     __shared__ __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];
-    mlp_load_smem(sw, p);
+    mlp_load_infer_smem(sw, p);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int s0 = (blockIdx.x * 4 + w) * 64; s0 < n; s0 += gridDim.x * 256) {

@@ -46,6 +46,18 @@ struct MlpSmem {
     float b1[kMlpHid], b2[kMlpHid], b3[kMlpOut];
 };
 
+// The same weights staged for inference in MFMA-fragment order: fragment f
+// of a layer holds, for each lane l, the 4 bf16 of the A operand that lane
+// reads (row 16 hb + l % 16, columns 16 kb + 4 (l / 16) .. + 3), 8 bytes per
+// lane contiguous, so a wave reads a fragment with one conflict-free
+// ds_read_b64. Fragments: layer 1 hb (4), layer 2 4 hb + kb (16), layer 3 kb (4).
+struct MlpInferSmem {
+    uint16_t f1[4][64][4];
+    uint16_t f2[16][64][4];
+    uint16_t f3[4][64][4];
+    float b1[kMlpHid], b2[kMlpHid], b3[kMlpOut];
+};
+
 // Per-wave exchange buffer: 64 samples x 16 bf16 inputs, reused for 64 x 8
 // FP32 outputs.
 constexpr int kMlpWaveScratchBytes = 64 * kMlpIn * 2;
@@ -81,6 +93,33 @@ __device__ __forceinline__ void mlp_load_smem(MlpSmem &s, const MlpParams &p) {
     for (int i = threadIdx.x; i < kMlpOut; i += blockDim.x) s.b3[i] = p.b3[i];
 }
 
+__device__ __forceinline__ void mlp_load_infer_smem(MlpInferSmem &s, const MlpParams &p) {
+    constexpr int kUnits = (4 + 16 + 4) * 64;  // (fragment, lane) pairs of 8 bytes
+    for (int u = threadIdx.x; u < kUnits; u += blockDim.x) {
+        const int f = u >> 6, l = u & 63, c16 = l & 15, q4 = 4 * (l >> 4);
+        const uint16_t *src;
+        uint2 *dst;
+        if (f < 4) {  // layer 1: row 16 hb + c16 of [64][16]
+            src = p.w1 + (16 * f + c16) * kMlpIn + q4;
+            dst = (uint2 *)s.f1[f][l];
+        } else if (f < 20) {  // layer 2: fragment 4 hb + kb of [64][64]
+            const int hb = (f - 4) >> 2, kb = (f - 4) & 3;
+            src = p.w2 + (16 * hb + c16) * kMlpHid + 16 * kb + q4;
+            dst = (uint2 *)s.f2[f - 4][l];
+        } else {  // layer 3: fragment kb of [16][64]
+            const int kb = f - 20;
+            src = p.w3 + c16 * kMlpHid + 16 * kb + q4;
+            dst = (uint2 *)s.f3[kb][l];
+        }
+        *dst = *(const uint2 *)src;
+    }
+    for (int i = threadIdx.x; i < kMlpHid; i += blockDim.x) {
+        s.b1[i] = p.b1[i];
+        s.b2[i] = p.b2[i];
+    }
+    for (int i = threadIdx.x; i < kMlpOut; i += blockDim.x) s.b3[i] = p.b3[i];
+}
+
 __device__ __forceinline__ void mlp_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -93,7 +132,7 @@ __device__ __forceinline__ void mlp_wave_sync() {
 // l%16) is exactly the B operand fragment of the next; only the inputs and
 // outputs pass through LDS. Must be called by all 64 lanes of the wave after
 // the rows are visible to the wave.
-__device__ __forceinline__ void mlp_wave_run(const MlpSmem &s, uint16_t *xs, int lane) {
+__device__ __forceinline__ void mlp_wave_run(const MlpInferSmem &s, uint16_t *xs, int lane) {
     const int c16 = lane & 15, q4 = 4 * (lane >> 4);
     mlp_v4s xb[4];
 #pragma unroll
@@ -105,7 +144,7 @@ __device__ __forceinline__ void mlp_wave_run(const MlpSmem &s, uint16_t *xs, int
         mlp_v4s h1[4], h2[4];
 #pragma unroll
         for (int hb = 0; hb < 4; hb++) {
-            const mlp_v4s a = *(const mlp_v4s *)(s.w1 + (hb * 16 + c16) * kMlpIn + q4);
+            const mlp_v4s a = *(const mlp_v4s *)s.f1[hb][lane];
             mlp_v4f acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb[si], mlp_v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
             const float *bb = s.b1 + hb * 16 + q4;
             h1[hb] = mlp_frag_bf16(fmaxf(acc[0] + bb[0], 0.f), fmaxf(acc[1] + bb[1], 0.f), fmaxf(acc[2] + bb[2], 0.f),
@@ -116,7 +155,7 @@ __device__ __forceinline__ void mlp_wave_run(const MlpSmem &s, uint16_t *xs, int
             mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kb = 0; kb < 4; kb++) {
-                const mlp_v4s a = *(const mlp_v4s *)(s.w2 + (hb * 16 + c16) * kMlpHid + kb * 16 + q4);
+                const mlp_v4s a = *(const mlp_v4s *)s.f2[4 * hb + kb][lane];
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kb], acc, 0, 0, 0);
             }
             const float *bb = s.b2 + hb * 16 + q4;
@@ -126,7 +165,7 @@ __device__ __forceinline__ void mlp_wave_run(const MlpSmem &s, uint16_t *xs, int
         mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < 4; kb++) {
-            const mlp_v4s a = *(const mlp_v4s *)(s.w3 + c16 * kMlpHid + kb * 16 + q4);
+            const mlp_v4s a = *(const mlp_v4s *)s.f3[kb][lane];
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[kb], acc, 0, 0, 0);
         }
         if (q4 < 8) {
@@ -149,7 +188,7 @@ __device__ __forceinline__ void mlp_stage_row(uint16_t *row, const float xn[kMlp
 }
 
 // one sample per lane: xn this lane's normalised inputs, y its normalised outputs
-__device__ __forceinline__ void mlp_wave_infer(const MlpSmem &s, const float xn[kMlpIn], float y[kMlpOutUsed],
+__device__ __forceinline__ void mlp_wave_infer(const MlpInferSmem &s, const float xn[kMlpIn], float y[kMlpOutUsed],
                                                uint16_t *xs, int lane) {
     mlp_stage_row(xs + lane * kMlpIn, xn);
     mlp_wave_sync();
