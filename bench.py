@@ -37,16 +37,16 @@ METRIC = "particle-steps/sec on 1 MI355X, surrogate vs plain at \u22641e-3 rel s
 # engine's storage format; the cell-partial buffer between a cells kernel
 # and its node pass is not counted).
 BYTES = {
-    # perm 4 + cell offset 12 + v 12 + m 4 + V(f64) 8 + stress 24
-    "p2g_cells": (64, 0, 0),
+    # perm 4 + cell offset 12 + v 12 + m 4 + V 4 + stress 24
+    "p2g_cells": (60, 0, 0),
     # node m 4 + v 12 + a 12 written
     "p2g_nodes": (0, 28, 0),
     # perm 4 + cell offset 12 + m 4 read, v 12 written; node v, a read 24
     "musl_cells": (32, 24, 0),
     # node m 4 read, MUSL v 12 written
     "musl_nodes": (0, 16, 0),
-    # key 4 + offset 12 + H 36 + V0 4 read; offset 12 + H 36 + stress 24 + J 8 + V 8 + key 4 written; node v 12
-    "update": (148, 12, 0),
+    # key 4 + offset 12 + H 36 + V0 4 read; offset 12 + H 36 + stress 24 + V 4 + key 4 written; node v 12
+    "update": (136, 12, 0),
     # counting sort: keys read twice, perm written (12 per particle); per cell: count read twice,
     # start and end written (16)
     "bin": (12, 0, 16),

@@ -101,6 +101,7 @@ struct mpm_engine {
     bool bins_valid = false;
     bool keys_valid = false;
     bool counts_ready = false;  // bb.cnt holds the per-cell counts of the current keys (else it is zero)
+    bool jv_derived = false;    // FP32: a step ran since the upload; det F and V follow from H at download
     bool fused_count = true;    // FP32: the update kernel counts the new keys for the next binning
     int last_pass = 0;  // 1 = p2g, 2 = g2p
     int steps_since_relayout = 0;
@@ -395,6 +396,7 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->counts_ready ? 1 : 0, e->bb.tiles, e->tg.ntiles,
                    e->bb.ctr};
         launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa, bf);
+        e->jv_derived = true;
         if (sa.mode == 2) launch_f32_j2_surrogate(e->stream, e->P32[e->cur], sa);
         e->counts_ready = e->fused_count;
         e->keys_valid = true;  // the update kernel writes the new keys
@@ -442,8 +444,9 @@ int alloc_particles(mpm_engine *e, int n) {
             PartF32 &P = e->P32[b];
             if ((rc = e->dalloc(&P.r, 3 * ld, true)) || (rc = e->dalloc(&P.v, 3 * ld, true)) || (rc = e->dalloc(&P.m, ld, true)) ||
                 (rc = e->dalloc(&P.V0, ld, true)) || (rc = e->dalloc(&P.V, ld, true)) ||
-                (rc = e->dalloc(&P.J, ld, true)) || (rc = e->dalloc(&P.H, 9 * ld, true)) ||
-                (rc = e->dalloc(&P.sig, 6 * ld, true)) || (rc = e->dalloc(&P.key, ld, true)) ||
+                (rc = e->dalloc(&P.V32, ld, true)) || (rc = e->dalloc(&P.J, ld, true)) ||
+                (rc = e->dalloc(&P.H, 9 * ld, true)) || (rc = e->dalloc(&P.sig, 6 * ld, true)) ||
+                (rc = e->dalloc(&P.key, ld, true)) ||
                 (rc = e->dalloc(&e->pid[b], ld, true)))
                 return rc;
             if (e->has_trac && (rc = e->dalloc(&P.trac, 3 * ld, true))) return rc;
@@ -1163,7 +1166,8 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
             if ((rc = h2d(e, P.matid, mi))) return rc;
         }
     } else {
-        std::vector<float> rr(3 * ld), v(3 * ld, 0.f), sig(6 * ld, 0.f), H(9 * ld), m(ld), V0(ld), ep(ld, 0.f), tr;
+        std::vector<float> rr(3 * ld), v(3 * ld, 0.f), sig(6 * ld, 0.f), H(9 * ld), m(ld), V0(ld), V32(ld), ep(ld, 0.f),
+            tr;
         std::vector<double> V(ld), J(ld);
         std::vector<int> pidv(ld);
         std::vector<uint32_t> keys(ld);
@@ -1196,6 +1200,7 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
             m[i] = (float)p->mass[i];
             V0[i] = (float)p->vol0[i];
             V[i] = p->vol ? p->vol[i] : p->vol0[i];
+            V32[i] = (float)V[i];
             J[i] = p->det_def_grad ? p->det_def_grad[i] : 1.0;
             if (p->eps_p) ep[i] = (float)p->eps_p[i];
             pidv[i] = i;
@@ -1203,8 +1208,8 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
         PartF32 &P = e->P32[0];
         if ((rc = h2d(e, P.r, rr)) || (rc = h2d(e, P.v, v)) || (rc = h2d(e, P.key, keys)) ||
             (rc = h2d(e, P.sig, sig)) || (rc = h2d(e, P.H, H)) || (rc = h2d(e, P.m, m)) || (rc = h2d(e, P.V0, V0)) ||
-            (rc = h2d(e, P.V, V)) || (rc = h2d(e, P.J, J)) || (rc = h2d(e, P.eps_p, ep)) ||
-            (rc = h2d(e, e->pid[0], pidv)))
+            (rc = h2d(e, P.V, V)) || (rc = h2d(e, P.V32, V32)) || (rc = h2d(e, P.J, J)) ||
+            (rc = h2d(e, P.eps_p, ep)) || (rc = h2d(e, e->pid[0], pidv)))
             return rc;
         e->fp32_outside_at_upload = outside;
         if (e->has_trac) {
@@ -1220,6 +1225,7 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
     }
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     e->keys_valid = e->prec == MPM_PREC_FP32;
+    e->jv_derived = false;
     e->clear_counts();
     e->bins_valid = false;
     e->steps_since_relayout = 0;
@@ -1263,6 +1269,7 @@ int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) {
         }
     } else {
         const PartF32 &P = e->P32[e->cur];
+        if (e->jv_derived) launch_f32_refresh_jv(e->stream, e->g, P);
         std::vector<float> rr, v, sig, H, ep, m, V0, tr;
         std::vector<double> V, J;
         std::vector<int> pidv, mid;
@@ -1571,6 +1578,7 @@ int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
             e->graph_cur = cur0;
             e->graph_since = since0;
         }
+        if (nblk > 0 && e->prec == MPM_PREC_FP32) e->jv_derived = true;
         for (int b = 0; b < nblk; b++) MPM_HIP_CHECK(hipGraphLaunch(e->step_exec, e->stream));
         if (e->prec == MPM_PREC_FP64_EXACT) e->keys_valid = false;
         const int done = nblk * B;
@@ -2006,6 +2014,8 @@ int mpm_ensemble_launch(mpm_ensemble *x, int32_t nlaunch, double *elapsed_ms) {
     int rc = ens_prepare(x);
     if (rc) return rc;
     MPM_HIP_CHECK(hipEventRecord(x->t0, x->s0));
+    for (auto *e : x->eng)
+        if (nlaunch > 0 && e->prec == MPM_PREC_FP32) e->jv_derived = true;
     for (int k = 0; k < nlaunch; k++) MPM_HIP_CHECK(hipGraphLaunch(x->exec, x->s0));
     MPM_HIP_CHECK(hipEventRecord(x->t1, x->s0));
     MPM_HIP_CHECK(hipEventSynchronize(x->t1));

@@ -142,13 +142,6 @@ __device__ __forceinline__ void lds_gather(const void *field_base, uint32_t slot
     __builtin_amdgcn_global_load_lds((const uint32_t *)field_base + slot, stage + st_idx<S>(f, 64 * r), 4, 0, 0);
 }
 
-template <int S>
-__device__ __forceinline__ void lds_gather2(const double *field_base, uint32_t slot, float *stage, int f, int r) {
-    const uint32_t *p = (const uint32_t *)field_base + 2 * (size_t)slot;
-    __builtin_amdgcn_global_load_lds(p, stage + st_idx<S>(f, 64 * r), 4, 0, 0);
-    __builtin_amdgcn_global_load_lds(p + 1, stage + st_idx<S>(f + 1, 64 * r), 4, 0, 0);
-}
-
 // wait for every outstanding vector-memory load (including LDS DMA) of this wave
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
 
@@ -186,8 +179,8 @@ template <int LPC, bool DIAG, bool TRAC, bool PF>
 __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T,
                                                        float *__restrict__ part) {
     constexpr int G = kSeg / LPC;
-    // fields: 0-2 r, 3-5 v, 6 m, 7-8 V (lo, hi dword), 9-14 sigma, [15-17 traction]
-    constexpr int NF = 15 + (TRAC ? 3 : 0);
+    // fields: 0-2 r, 3-5 v, 6 m, 7 V, 8-13 sigma, [14-16 traction]
+    constexpr int NF = 14 + (TRAC ? 3 : 0);
     constexpr int SS = 64 + LPC;
     __shared__ float stage[NF * kBlk * SS];
     const uint32_t n_active = L.ctr->n_active;
@@ -242,12 +235,12 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                         lds_gather<SS>(P.v + a * ld, sl, stage, 3 + a, r);
                     }
                     lds_gather<SS>(P.m, sl, stage, 6, r);
-                    lds_gather2<SS>(P.V, sl, stage, 7, r);
+                    lds_gather<SS>(P.V32, sl, stage, 7, r);
 #pragma unroll
-                    for (int q = 0; q < 6; q++) lds_gather<SS>(P.sig + q * ld, sl, stage, 9 + q, r);
+                    for (int q = 0; q < 6; q++) lds_gather<SS>(P.sig + q * ld, sl, stage, 8 + q, r);
                     if (TRAC) {
 #pragma unroll
-                        for (int a = 0; a < 3; a++) lds_gather<SS>(P.trac + a * ld, sl, stage, 15 + a, r);
+                        for (int a = 0; a < 3; a++) lds_gather<SS>(P.trac + a * ld, sl, stage, 14 + a, r);
                     }
                 }
             }
@@ -278,22 +271,21 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                 const float fz = stage[st_idx<SS>(2, i)];
                 const float m = stage[st_idx<SS>(6, i)];
                 const float px = m * stage[st_idx<SS>(3, i)], py = m * stage[st_idx<SS>(4, i)], pz = m * stage[st_idx<SS>(5, i)];
-                const double Vd = __hiloint2double(__float_as_int(stage[st_idx<SS>(8, i)]),
-                                                   __float_as_int(stage[st_idx<SS>(7, i)]));
-                const float nV = -(float)Vd;
+                const float Vp = stage[st_idx<SS>(7, i)];
+                const float nV = -Vp;
                 // -V sigma (Voigt xx,yy,zz,yz,xz,xy) scaled by 1/dx of the derivative direction
-                const float s0 = nV * stage[st_idx<SS>(9, i)], s1 = nV * stage[st_idx<SS>(10, i)];
-                const float s2 = nV * stage[st_idx<SS>(11, i)], s3 = nV * stage[st_idx<SS>(12, i)];
-                const float s4 = nV * stage[st_idx<SS>(13, i)], s5 = nV * stage[st_idx<SS>(14, i)];
+                const float s0 = nV * stage[st_idx<SS>(8, i)], s1 = nV * stage[st_idx<SS>(9, i)];
+                const float s2 = nV * stage[st_idx<SS>(10, i)], s3 = nV * stage[st_idx<SS>(11, i)];
+                const float s4 = nV * stage[st_idx<SS>(12, i)], s5 = nV * stage[st_idx<SS>(13, i)];
                 // force on a corner: f_a = sum_b (-V sigma_ab) dN/dx_b, dN/dx = (sx/dx wy wz, sy/dy wx wz, sz/dz wx wy)
                 const float X0 = s0 * inv0, X1 = s5 * inv0, X2 = s4 * inv0;  // b = x: sigma_xx, sigma_yx, sigma_zx
                 const float Y0 = s5 * inv1, Y1 = s1 * inv1, Y2 = s3 * inv1;  // b = y
                 const float Z0 = s4 * inv2, Z1 = s3 * inv2, Z2 = s2 * inv2;  // b = z
                 float tx = 0.f, ty = 0.f, tz = 0.f;
                 if (TRAC) {
-                    tx = stage[st_idx<SS>(15, i)];
-                    ty = stage[st_idx<SS>(16, i)];
-                    tz = stage[st_idx<SS>(17, i)];
+                    tx = stage[st_idx<SS>(14, i)];
+                    ty = stage[st_idx<SS>(15, i)];
+                    tz = stage[st_idx<SS>(16, i)];
                 }
                 const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
                 float wyz[2][2], wxz[2][2], wxy[2][2];
@@ -333,7 +325,7 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                                 tr[q][1] += N * ty;
                                 tr[q][2] += N * tz;
                             }
-                            if (DIAG) vof[q] += N * (float)Vd;
+                            if (DIAG) vof[q] += N * Vp;
                         }
             }
             wave_fence();
@@ -954,6 +946,17 @@ __device__ __forceinline__ float j2_residual(const J2Trial &t, const float s[6],
     return r / t.sy;
 }
 
+// det(I + H) in FP64 of an FP32 displacement gradient. No contraction, so
+// that the update kernel and the J refresh give the same bits.
+__device__ __forceinline__ double det_I_plus_H(const float H[9]) {
+#pragma clang fp contract(off)
+    double A[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) A[q] = (double)H[q] + ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0);
+    return A[0] * (A[4] * A[8] - A[5] * A[7]) + A[1] * (A[5] * A[6] - A[3] * A[8]) +
+           A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
 template <int SUR>
 __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
                                                        MaterialTable mt, StepConsts sc, uint32_t *err,
@@ -1030,12 +1033,7 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
         for (int b = 0; b < 3; b++)
             Hn[3 * a + b] = (H[3 * a + b] + D[3 * a + b]) +
                             (D[3 * a] * H[b] + D[3 * a + 1] * H[3 + b] + D[3 * a + 2] * H[6 + b]);
-    // J = det(I + H_new) in FP64
-    double A[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) A[q] = (double)Hn[q] + ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0);
-    const double J = A[0] * (A[4] * A[8] - A[5] * A[7]) + A[1] * (A[5] * A[6] - A[3] * A[8]) +
-                     A[2] * (A[3] * A[7] - A[4] * A[6]);
+    const double J = det_I_plus_H(Hn);
     const double V = J * (double)P.V0[slot];
     // position: offset in cell units, carried into the cell index when it leaves [0, 1)
     int ci[3] = {ix, iy, iz};
@@ -1167,8 +1165,12 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     for (int q = 0; q < 6; q++) P.sig[q * ld + slot] = s[q];
 #pragma unroll
     for (int a = 0; a < 3; a++) P.r[a * ld + slot] = r[a];
-    P.J[slot] = J;
-    P.V[slot] = V;
+    P.V32[slot] = (float)V;
+    if (!inside) {
+        // frozen from here on: keep its FP64 det F and volume
+        P.J[slot] = J;
+        P.V[slot] = V;
+    }
     P.key[slot] = nk;
     uint32_t e = 0;
     if (!inside) e |= kErrOutOfGrid;
@@ -1317,6 +1319,7 @@ __global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict
     for (int q = 0; q < 6; q++) dst.sig[q * lt + s] = src.sig[q * ls + o];
     dst.m[s] = src.m[o];
     dst.V0[s] = src.V0[o];
+    dst.V32[s] = src.V32[o];
     dst.V[s] = src.V[o];
     dst.J[s] = src.J[o];
     dst.key[s] = src.key[o];
@@ -1325,7 +1328,22 @@ __global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict
     pid_dst[s] = pid_src[o];
 }
 
+__global__ void k32_refresh_jv(GridGeom g, PartF32 P) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n || P.key[s] >= (uint32_t)g.nelem) return;
+    float H[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) H[q] = P.H[q * P.ld + s];
+    const double J = det_I_plus_H(H);
+    P.J[s] = J;
+    P.V[s] = J * (double)P.V0[s];
+}
+
 }  // namespace
+
+void launch_f32_refresh_jv(hipStream_t s, const GridGeom &g, const PartF32 &P) {
+    if (P.n > 0) hipLaunchKernelGGL(k32_refresh_jv, dim3(cdiv(P.n, 256)), dim3(256), 0, s, g, P);
+}
 
 static int cell_groups_grid(const GridGeom &g) {
     int ng = cdiv(g.nelem, 16);

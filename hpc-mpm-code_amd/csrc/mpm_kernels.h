@@ -192,8 +192,9 @@ struct PartF32 {
     float *v;       // [3][ld]
     float *m;
     float *V0;
-    double *V;      // FP64 volume V = J V0
-    double *J;      // FP64 det F
+    float *V32;     // volume V = J V0 rounded to FP32: written by every update, read by P2G
+    double *V;      // FP64 volume and det F: the uploaded values, and those of a particle when it
+    double *J;      // leaves the grid; for particles inside, launch_f32_refresh_jv derives them from H
     float *H;       // [9][ld] displacement gradient F - I
     float *sig;     // [6][ld]
     float *eps_p;   // J2 equivalent plastic strain (or null)
@@ -283,6 +284,9 @@ void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, c
                            const SurrogateArgs &sa, const BinFuse &bf);
 // network pass over the queued plastic states of this step (mode 2)
 void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const SurrogateArgs &sa);
+// J = det(I + H) and V = J V0 in FP64 (the values the update kernel computes)
+// into P.J / P.V for every particle inside the grid
+void launch_f32_refresh_jv(hipStream_t s, const GridGeom &g, const PartF32 &P);
 void launch_f32_permute(hipStream_t s, const PartF32 &src, const PartF32 &dst, const uint32_t *perm,
                         const int *pid_src, int *pid_dst);
 void launch_iota_u32(hipStream_t s, uint32_t *p, int n);
