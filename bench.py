@@ -45,8 +45,9 @@ BYTES = {
     "musl_cells": (32, 24, 0),
     # node m 4 read, MUSL v 12 written
     "musl_nodes": (0, 16, 0),
-    # key 4 + offset 12 + H 36 + V0 4 read; offset 12 + H 36 + stress 24 + V 4 + key 4 written; node v 12
-    "update": (136, 12, 0),
+    # key 4 + offset 12 + H 36 + V0 4 read; offset 12 + H 36 + stress 24 + V 4 written (the key only for
+    # particles that change cell); node v 12
+    "update": (132, 12, 0),
     # counting sort: keys read twice, perm written (12 per particle); per cell: count read twice,
     # start and end written (16)
     "bin": (12, 0, 16),

@@ -5,14 +5,19 @@
 //
 //   1. count      wave-aggregated atomic increments of a per-cell counter
 //                 (one atomic per run of equal keys in a wave)
-//   2. scan       one exclusive scan of the packed pair (non-empty, count)
-//                 per cell gives each cell's start offset and, for non-empty
-//                 cells, its index in the active-cell list
-//   3. finalize   writes cstart/cend (cend starts as a cursor at cstart), the
-//                 active-cell list, the active node-tile flags, and re-zeroes
-//                 the counters for the next step -- or keeps them, when the
-//                 next update kernel adjusts them for the particles that
-//                 change cell (kept counts)
+//   2. reduce     the packed pair (non-empty, count) summed per tile of 4096
+//                 cells
+//   3. finalize   per tile: the sum of the tile totals before it, then one
+//                 exclusive scan of the packed pairs gives each cell's start
+//                 offset and, for non-empty cells, its index in the
+//                 active-cell list; writes cstart/cend (cend starts as a
+//                 cursor at cstart), the active-cell list, the active
+//                 node-tile flags, and re-zeroes the counters for the next
+//                 step -- or keeps them, when the next update kernel adjusts
+//                 them for the particles that change cell (kept counts).
+//                 A tile that is empty and was empty at the previous binning
+//                 is left as it is: its cells still have cstart == cend and
+//                 clear occupancy bits, which is all an empty cell needs.
 //   4. scatter    wave-aggregated atomic cursor per run; lanes of a run get
 //                 consecutive positions in slot order. A run that does not
 //                 land at its cell's start means the cell received several
@@ -33,6 +38,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
+static_assert(kTile == kBinTile, "a scan block covers one binning tile");
 constexpr int kWaves = kBlock / 64;
 
 __global__ void __launch_bounds__(256) k_bin_count(const uint32_t *__restrict__ keys, int n, int nelem,
@@ -99,97 +105,141 @@ __global__ void __launch_bounds__(kBlock) k_bin_reduce(const uint32_t *__restric
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(kBlock) k_bin_top(uint64_t *__restrict__ bsum, int nb, BinCounters *ctr,
-                                                     uint32_t *__restrict__ astart, int n, uint32_t *err) {
-    const int base = threadIdx.x * kItems;
-    uint64_t v[kItems];
-    uint64_t s = 0;
+// sum over the block (every thread gets it)
+__device__ __forceinline__ uint64_t block_sum64(uint64_t v) {
+    __shared__ uint64_t wsum[kWaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < kItems; i++) {
-        v[i] = (base + i < nb) ? bsum[base + i] : 0ull;
-        s += v[i];
-    }
-    uint64_t tot;
-    uint64_t pre = block_excl_scan64(s, &tot);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    uint64_t tot = 0;
 #pragma unroll
-    for (int i = 0; i < kItems; i++) {
-        if (base + i < nb) bsum[base + i] = pre;
-        pre += v[i];
-    }
-    if (threadIdx.x == 0) {
-        ctr->stamp += 1u;
-        astart[tot >> 32] = (uint32_t)tot;
-        // more counted particles than exist: nothing of this binning may be
-        // used (the later kernels would index past the permutation)
-        const bool bad = (uint64_t)(uint32_t)tot > (uint64_t)n;
-        ctr->bad = bad ? 1u : 0u;
-        ctr->n_active = bad ? 0u : (uint32_t)(tot >> 32);
-        if (bad) atomicOr(err, kErrBinCounts);
-    }
+    for (int i = 0; i < kWaves; i++) tot += wsum[i];
+    __syncthreads();
+    return tot;
 }
 
-__device__ __forceinline__ void mark_cell(const GridGeom &g, const TileGeom &tg, uint32_t *tiles, int c) {
-    const int nex = g.nex[0], nexy = g.nex[0] * g.nex[1];
-    int iz = c / nexy;
-    int iy = (c - iz * nexy) / nex;
-    int ix = c - iz * nexy - iy * nex;
-    // node tiles of the cell's corners (benign identical stores)
+// Node tiles touched by the corners of cell (ix, iy, iz): tile ranges
+// [tx0, tx1] x [ty0, ty1] x [tz0, tz1]; consecutive cells of a row share them
+// except where the row crosses a tile boundary.
+struct TileSet {
+    int tx0, tx1, ty0, ty1, tz0, tz1;
+    __device__ bool operator==(const TileSet &o) const {
+        return tx0 == o.tx0 && tx1 == o.tx1 && ty0 == o.ty0 && ty1 == o.ty1 && tz0 == o.tz0 && tz1 == o.tz1;
+    }
+};
+
+__device__ __forceinline__ TileSet tile_set(int ix, int iy, int iz) {
+    return TileSet{ix / kNodeTileX, (ix + 1) / kNodeTileX, iy / kNodeTileY,
+                   (iy + 1) / kNodeTileY, iz / kNodeTileZ, (iz + 1) / kNodeTileZ};
+}
+
+__device__ __forceinline__ void mark_tiles(const TileGeom &tg, uint32_t *tiles, const TileSet &t) {
+    // (benign identical stores)
 #pragma unroll
     for (int k = 0; k < 2; k++)
 #pragma unroll
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int ii = 0; ii < 2; ii++) {
-                int tx = (ix + ii) / kNodeTileX, ty = (iy + j) / kNodeTileY, tz = (iz + k) / kNodeTileZ;
+                const int tx = ii ? t.tx1 : t.tx0, ty = j ? t.ty1 : t.ty0, tz = k ? t.tz1 : t.tz0;
                 tiles[tx + tg.ntx * (ty + tg.nty * tz)] = 1u;
             }
 }
 
 __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *__restrict__ cnt,
-                                                          const uint64_t *__restrict__ bsum,
-                                                          uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend,
+                                                          const uint64_t *__restrict__ bsum, int nb,
+                                                          uint32_t *__restrict__ tprev, BinCounters *ctr, int n,
+                                                          uint32_t *err, uint32_t *__restrict__ cstart,
+                                                          uint32_t *__restrict__ cend,
                                                           uint32_t *__restrict__ active, uint32_t *__restrict__ astart,
                                                           uint32_t *__restrict__ tiles, TileGeom tg,
                                                           uint32_t *__restrict__ occ, int keep) {
-    const size_t tbase = (size_t)blockIdx.x * kTile;
-    uint4 *c4 = (uint4 *)(cnt + tbase);
-    uint4 *s4 = (uint4 *)(cstart + tbase);
-    uint4 *e4 = (uint4 *)(cend + tbase);
-    uint64_t run = bsum[blockIdx.x];
-    for (int r = 0; r < 4; r++) {
-        const int idx = r * kBlock + threadIdx.x;
-        const uint4 v = c4[idx];
-        uint64_t tot;
-        uint64_t pre = block_excl_scan64(packed4(v), &tot) + run;
-        run += tot;
-        const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
-        uint32_t st[4];
-        bool any = false;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            st[q] = (uint32_t)pre;
-            if (vv[q]) {
-                const int c = (int)(tbase + 4 * idx + q);
-                active[(uint32_t)(pre >> 32)] = (uint32_t)c;
-                astart[(uint32_t)(pre >> 32)] = st[q];
-                mark_cell(g, tg, tiles, c);
-                any = true;
-            }
-            pre += packed(vv[q]);
-        }
-        const uint4 so = make_uint4(st[0], st[1], st[2], st[3]);
-        s4[idx] = so;
-        e4[idx] = so;
-        if (any && !keep) c4[idx] = make_uint4(0u, 0u, 0u, 0u);
-        // occupancy bits: 8 consecutive lanes hold the 32 cells of one word
-        const int lane = threadIdx.x & 63;
-        uint32_t w = ((vv[0] ? 1u : 0u) | (vv[1] ? 2u : 0u) | (vv[2] ? 4u : 0u) | (vv[3] ? 8u : 0u))
-                     << (4 * (lane & 7));
-        w |= (uint32_t)__shfl_xor((int)w, 1, 64);
-        w |= (uint32_t)__shfl_xor((int)w, 2, 64);
-        w |= (uint32_t)__shfl_xor((int)w, 4, 64);
-        if ((lane & 7) == 0) occ[(tbase + 4 * (size_t)idx) >> 5] = w;
+    const int b = blockIdx.x;
+    const uint32_t was = tprev[b];
+    const uint64_t mine = bsum[b];
+    // tile totals before this tile, and of all tiles
+    uint64_t before = 0, all = 0;
+    for (int i = threadIdx.x; i < nb; i += kBlock) {
+        const uint64_t v = bsum[i];
+        all += v;
+        before += i < b ? v : 0ull;
     }
+    before = block_sum64(before);
+    all = block_sum64(all);
+    if (b == nb - 1 && threadIdx.x == 0) {
+        ctr->stamp += 1u;
+        astart[all >> 32] = (uint32_t)all;
+        // more counted particles than exist: nothing of this binning may be
+        // used (the later kernels would index past the permutation)
+        const bool bad = (uint64_t)(uint32_t)all > (uint64_t)n;
+        ctr->bad = bad ? 1u : 0u;
+        ctr->n_active = bad ? 0u : (uint32_t)(all >> 32);
+        if (bad) atomicOr(err, kErrBinCounts);
+    }
+    if (mine == 0ull && was == 0u) return;  // uniform over the block
+    if (threadIdx.x == 0) tprev[b] = mine != 0ull ? 1u : 0u;
+    // thread t owns cells c0 .. c0 + 15
+    const size_t c0 = (size_t)b * kTile + (size_t)kItems * threadIdx.x;
+    uint4 *c4 = (uint4 *)(cnt + c0);
+    uint32_t cc[kItems];
+#pragma unroll
+    for (int r = 0; r < kItems / 4; r++) {
+        const uint4 v = c4[r];
+        cc[4 * r] = v.x;
+        cc[4 * r + 1] = v.y;
+        cc[4 * r + 2] = v.z;
+        cc[4 * r + 3] = v.w;
+    }
+    uint64_t own = 0;
+#pragma unroll
+    for (int q = 0; q < kItems; q++) own += packed(cc[q]);
+    uint64_t tot;
+    uint64_t pre = block_excl_scan64(own, &tot) + before;
+    const int nex = g.nex[0], ney = g.nex[1], nexy = nex * ney;
+    int iz = (int)(c0 / (size_t)nexy);
+    int iy = (int)((c0 - (size_t)iz * nexy) / (size_t)nex);
+    int ix = (int)(c0 - (size_t)iz * nexy - (size_t)iy * nex);
+    TileSet last{-1, -1, -1, -1, -1, -1};
+    uint32_t st[kItems];
+    uint32_t bits = 0;
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+        st[q] = (uint32_t)pre;
+        if (cc[q]) {
+            active[(uint32_t)(pre >> 32)] = (uint32_t)(c0 + q);
+            astart[(uint32_t)(pre >> 32)] = st[q];
+            bits |= 1u << q;
+            const TileSet t = tile_set(ix, iy, iz);
+            if (!(t == last)) {
+                mark_tiles(tg, tiles, t);
+                last = t;
+            }
+        }
+        pre += packed(cc[q]);
+        if (++ix == nex) {
+            ix = 0;
+            if (++iy == ney) {
+                iy = 0;
+                iz++;
+            }
+        }
+    }
+    uint4 *s4 = (uint4 *)(cstart + c0);
+    uint4 *e4 = (uint4 *)(cend + c0);
+#pragma unroll
+    for (int r = 0; r < kItems / 4; r++) {
+        const uint4 so = make_uint4(st[4 * r], st[4 * r + 1], st[4 * r + 2], st[4 * r + 3]);
+        s4[r] = so;
+        e4[r] = so;
+        if (bits && !keep) c4[r] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    // occupancy bits: two threads per 32-cell word
+    static_assert(kItems == 16, "two threads per occupancy word");
+    uint32_t w = bits << (16 * (threadIdx.x & 1));
+    w |= (uint32_t)__shfl_xor((int)w, 1, 64);
+    if ((threadIdx.x & 1) == 0) occ[c0 >> 5] = w;
 }
 
 __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict__ keys, int n, int nelem,
@@ -292,11 +342,16 @@ TileGeom make_tile_geom(const GridGeom &g) {
 size_t bin_scan_blocks(int nelem) { return (size_t)cdiv(nelem, kTile); }
 size_t bin_cell_capacity(int nelem) { return bin_scan_blocks(nelem) * kTile; }
 
+void launch_bin_reduce(hipStream_t s, const GridGeom &g, BinBuffers &b) {
+    hipLaunchKernelGGL(k_bin_reduce, dim3(cdiv(g.nelem, kTile)), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
+}
+
 void launch_bin_count(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n,
                       BinBuffers &b) {
     int pblocks = cdiv(n > 0 ? n : 1, 256);
     if (pblocks > 8192) pblocks = 8192;
     hipLaunchKernelGGL(k_bin_count, dim3(pblocks), dim3(256), 0, s, keys, n, g.nelem, b.cnt, b.tiles, tg.ntiles, b.ctr);
+    launch_bin_reduce(s, g, b);
 }
 
 int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
@@ -309,10 +364,9 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
     if (!counted)
         hipLaunchKernelGGL(k_bin_count, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, b.cnt, b.tiles, tg.ntiles,
                            b.ctr);
-    hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
-    hipLaunchKernelGGL(k_bin_top, dim3(1), dim3(kBlock), 0, s, b.bsum, nb, b.ctr, b.astart, n, err);
-    hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, b.cstart,
-                       b.cend, b.active, b.astart, b.tiles, tg, b.occ, keep ? 1 : 0);
+    if (!counted) hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
+    hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, nb, b.tprev,
+                       b.ctr, n, err, b.cstart, b.cend, b.active, b.astart, b.tiles, tg, b.occ, keep ? 1 : 0);
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, (const uint32_t *)b.cstart,
                            b.cend, b.perm, b.stamp, b.fixlist, b.ctr);

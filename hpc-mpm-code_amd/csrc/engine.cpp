@@ -100,7 +100,8 @@ struct mpm_engine {
     double *d_ndispl = nullptr;  // [3*nodec] nodal displacement increment of the implicit G2P
     bool bins_valid = false;
     bool keys_valid = false;
-    bool counts_ready = false;  // bb.cnt holds the per-cell counts of the current keys (else it is zero)
+    bool counts_ready = false;  // bb.cnt holds the per-cell counts of the current keys and bb.bsum their
+                                // tile totals (else the counts are zero)
     bool jv_derived = false;    // FP32: a step ran since the upload; det F and V follow from H at download
     bool fused_count = true;    // FP32: the update kernel counts the new keys for the next binning
     int last_pass = 0;  // 1 = p2g, 2 = g2p
@@ -393,9 +394,12 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         // count the new keys for the next binning: adjust kept counts of the
         // current keys, or count from zero
         if (!e->fused_count) e->clear_counts();
-        BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->counts_ready ? 1 : 0, e->bb.tiles, e->tg.ntiles,
-                   e->bb.ctr};
+        BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->counts_ready ? 1 : 0, e->bb.bsum, e->bb.tiles,
+                   e->tg.ntiles, e->bb.ctr};
         launch_f32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, sa, bf);
+        // counted from zero: the tile totals of the new counts (kept counts
+        // carry theirs along)
+        if (bf.cnt && !bf.delta) launch_bin_reduce(e->stream, e->g, e->bb);
         e->jv_derived = true;
         if (sa.mode == 2) launch_f32_j2_surrogate(e->stream, e->P32[e->cur], sa);
         e->counts_ready = e->fused_count;
@@ -580,7 +584,8 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         (rc = e->dalloc(&b.active, nelem)) || (rc = e->dalloc(&b.astart, nelem + 1)) ||
         (rc = e->dalloc(&b.stamp, nelem)) ||
         (rc = e->dalloc(&b.fixlist, nelem)) || (rc = e->dalloc(&b.tiles, (size_t)e->tg.ntiles)) ||
-        (rc = e->dalloc(&b.bsum, bin_scan_blocks(e->g.nelem))) || (rc = e->dalloc(&b.ctr, 1)) ||
+        (rc = e->dalloc(&b.bsum, bin_scan_blocks(e->g.nelem))) ||
+        (rc = e->dalloc(&b.tprev, bin_scan_blocks(e->g.nelem))) || (rc = e->dalloc(&b.ctr, 1)) ||
         (rc = e->dalloc(&b.occ, ncap / 32)) ||
         (rc = e->dalloc(&e->d_err, 1))) {
         delete e;
@@ -594,6 +599,8 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     if (he == hipSuccess) he = hipMemset(b.stamp, 0, nelem * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.cstart, 0, ncap * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.cend, 0, ncap * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(b.occ, 0, ncap / 32 * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(b.tprev, 0, bin_scan_blocks(e->g.nelem) * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.ctr, 0, sizeof(BinCounters));
     if (he != hipSuccess) {
         int r = hip_fail(he, "hipMemset", __FILE__, __LINE__);

@@ -946,6 +946,23 @@ __device__ __forceinline__ float j2_residual(const J2Trial &t, const float s[6],
     return r / t.sy;
 }
 
+// tsum[t] += d (modulo 2^64) for the lanes with act set; one atomic per wave
+// when all of them share the tile. Called by every lane of the wave.
+__device__ __forceinline__ void tile_total_add(uint64_t *tsum, bool act, uint32_t t, unsigned long long d) {
+    const unsigned long long m = __ballot(act);
+    if (!m) return;
+    const int f = __ffsll(m) - 1;
+    const uint32_t t0 = (uint32_t)__shfl((int)t, f, 64);
+    if (__ballot(act && t != t0) == 0ull) {
+        unsigned long long sum = act ? d : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+        if ((int)(threadIdx.x & 63) == f) atomicAdd((unsigned long long *)&tsum[t0], sum);
+    } else if (act) {
+        atomicAdd((unsigned long long *)&tsum[t], d);
+    }
+}
+
 // det(I + H) in FP64 of an FP32 displacement gradient. No contraction, so
 // that the update kernel and the J refresh give the same bits.
 __device__ __forceinline__ double det_I_plus_H(const float H[9]) {
@@ -1145,11 +1162,17 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     const uint32_t nk = inside ? (uint32_t)(ci[0] + ci[1] * g.nex[0] + ci[2] * nexy) : (uint32_t)g.nelem;
     if (bf.cnt) {
         if (bf.delta) {
-            // kept counts: only a particle that changes cell moves one count
-            if (valid && nk != key) {
-                atomicSub(&bf.cnt[key], 1u);
-                if (nk < (uint32_t)g.nelem) atomicAdd(&bf.cnt[nk], 1u);
-            }
+            // kept counts: only a particle that changes cell moves one count,
+            // and the packed tile totals (non-empty cells, particles) follow
+            const bool mover = valid && nk != key;
+            const bool lands = mover && nk < (uint32_t)g.nelem;
+            uint32_t o_from = 0u, o_to = 1u;
+            if (mover) o_from = atomicSub(&bf.cnt[key], 1u);
+            if (lands) o_to = atomicAdd(&bf.cnt[nk], 1u);
+            const unsigned long long d_from = 1ull + (o_from == 1u ? (1ull << 32) : 0ull);
+            const unsigned long long d_to = 1ull + (o_to == 0u ? (1ull << 32) : 0ull);
+            tile_total_add(bf.tsum, mover, key >> kBinTileShift, 0ull - d_from);
+            tile_total_add(bf.tsum, lands, nk >> kBinTileShift, d_to);
         } else {
             const uint32_t ck = valid ? nk : 0xFFFFFFFFu;
             int len, head;
@@ -1171,7 +1194,7 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
         P.J[slot] = J;
         P.V[slot] = V;
     }
-    P.key[slot] = nk;
+    if (nk != key) P.key[slot] = nk;  // most particles stay in their cell
     uint32_t e = 0;
     if (!inside) e |= kErrOutOfGrid;
     if (!(J > 0.0)) e |= kErrJacobian;

@@ -51,6 +51,10 @@ enum : uint32_t {
     kErrBinCounts = 16u,    // per-cell counts add up to more than the particle count
 };
 
+// Cells per binning tile (the unit of the per-tile totals of the cell counts).
+constexpr int kBinTileShift = 12;
+constexpr int kBinTile = 1 << kBinTileShift;
+
 // Node tiles of the node passes (x fastest); a tile is processed only when a
 // cell touching one of its nodes holds particles.
 constexpr int kNodeTileX = 64;

@@ -77,6 +77,7 @@ __device__ __forceinline__ bool wave_runs(uint32_t k, int lane, int *len, int *h
 struct BinFuse {
     uint32_t *cnt;  // null: not fused
     int delta;      // 1: cnt holds the counts of the keys being replaced; only moves are applied
+    uint64_t *tsum; // with delta: the tile totals of cnt (BinBuffers::bsum), kept up to date as well
     uint32_t *tiles;
     int ntiles;
     BinCounters *ctr;
@@ -92,8 +93,10 @@ struct BinBuffers {
     uint32_t *fixlist;  // [nelem]
     uint32_t *perm;     // [n]
     uint32_t *tiles;    // [ntiles]
-    uint32_t *occ;      // [bin_cell_capacity / 32] occupancy bits
-    uint64_t *bsum;     // [bin_scan_blocks(nelem)]
+    uint32_t *occ;      // [bin_cell_capacity / 32] occupancy bits (zero at allocation)
+    uint64_t *bsum;     // [bin_scan_blocks(nelem)] packed (non-empty cells, particles) per tile of
+                        // kBinTile cells: written by the reduce pass, or kept with kept counts
+    uint32_t *tprev;    // [bin_scan_blocks(nelem)] tile non-empty at the previous binning (zero at allocation)
     BinCounters *ctr;
 };
 
@@ -102,14 +105,18 @@ size_t bin_scan_blocks(int nelem);
 // per-cell arrays (cnt, cstart, cend) must hold this many entries (padding zeroed)
 size_t bin_cell_capacity(int nelem);
 // Counting-sort binning of keys[0..n) (sentinel nelem = outside the grid).
-// count pass alone (per-cell counts of keys, cleared tiles and counters)
+// count pass alone (per-cell counts of keys, their tile totals, cleared tiles
+// and counters)
 void launch_bin_count(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n,
                       BinBuffers &b);
-// counted: the per-cell counts, cleared tiles and counters were produced by
-// the previous update kernel (fused counting), so the count pass is skipped.
+// tile totals (b.bsum) of the per-cell counts b.cnt
+void launch_bin_reduce(hipStream_t s, const GridGeom &g, BinBuffers &b);
+// counted: the per-cell counts and their tile totals, cleared tiles and
+// counters were produced by the previous update kernel (fused counting), so
+// the count and reduce passes are skipped.
 // keep: the counts are left in place (they are the counts of keys[], which
-// the next update kernel then only adjusts for the particles that change
-// cell); otherwise they are zeroed.
+// the next update kernel then only adjusts, with their tile totals, for the
+// particles that change cell); otherwise they are zeroed.
 int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint32_t *keys, int n, BinBuffers &b,
                uint32_t *err, bool counted = false, bool keep = false);
 
