@@ -242,6 +242,9 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
     if ((threadIdx.x & 1) == 0) occ[c0 >> 5] = w;
 }
 
+// Each wave places 128 consecutive slots as two halves of 64 whose loads and
+// atomics are in flight together; a run of equal keys that crosses from the
+// first half into the second takes one atomic for both parts.
 __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict__ keys, int n, int nelem,
                                                       const uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend,
                                                       uint32_t *__restrict__ perm, uint32_t *__restrict__ stamp_arr,
@@ -249,22 +252,39 @@ __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict_
     const int lane = threadIdx.x & 63;
     const uint32_t stamp = ctr->stamp;
     if (ctr->bad) return;
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const int p = base + threadIdx.x;
-        const uint32_t k = (p < n) ? keys[p] : 0xFFFFFFFFu;
-        int len, head;
-        bool h = wave_runs(k, lane, &len, &head);
-        uint32_t b = 0;
-        const bool valid = k < (uint32_t)nelem;
-        if (h && valid) {
-            b = atomicAdd(&cend[k], (uint32_t)len);
-            if (b != cstart[k]) {
+    const uint32_t un = (uint32_t)nelem;
+    for (int wbase = (blockIdx.x * (int)blockDim.x + (int)threadIdx.x - lane) * 2; wbase < n;
+         wbase += gridDim.x * blockDim.x * 2) {
+        const int p0 = wbase + lane, p1 = wbase + 64 + lane;
+        const uint32_t k0 = (p0 < n) ? keys[p0] : 0xFFFFFFFFu;
+        const uint32_t k1 = (p1 < n) ? keys[p1] : 0xFFFFFFFFu;
+        int len0, head0, len1, head1;
+        const bool h0 = wave_runs(k0, lane, &len0, &head0);
+        const bool h1 = wave_runs(k1, lane, &len1, &head1);
+        const bool cross = (uint32_t)__shfl((int)k0, 63, 64) == (uint32_t)__shfl((int)k1, 0, 64);
+        const int head_last0 = __shfl(head0, 63, 64);  // head of the run that reaches the end of half 0
+        const int len_first1 = __shfl(len1, 0, 64);
+        uint32_t b0 = 0, b1 = 0;
+        if (h0 && k0 < un) {
+            const int add = len0 + ((cross && lane == head_last0) ? len_first1 : 0);
+            b0 = atomicAdd(&cend[k0], (uint32_t)add);
+            if (b0 != cstart[k0]) {
                 // this cell received more than one run: queue it once for the fix-up
-                if (atomicExch(&stamp_arr[k], stamp) != stamp) fixlist[atomicAdd(&ctr->n_fix, 1u)] = k;
+                if (atomicExch(&stamp_arr[k0], stamp) != stamp) fixlist[atomicAdd(&ctr->n_fix, 1u)] = k0;
             }
         }
-        uint32_t bb = (uint32_t)__shfl((int)b, head, 64);
-        if (valid) perm[bb + (uint32_t)(lane - head)] = (uint32_t)p;
+        if (h1 && k1 < un && !(cross && lane == 0)) {
+            b1 = atomicAdd(&cend[k1], (uint32_t)len1);
+            if (b1 != cstart[k1]) {
+                if (atomicExch(&stamp_arr[k1], stamp) != stamp) fixlist[atomicAdd(&ctr->n_fix, 1u)] = k1;
+            }
+        }
+        const uint32_t bb0 = (uint32_t)__shfl((int)b0, head0, 64);
+        // the crossing run continues after its 64 - head_last0 slots in half 0
+        const uint32_t cont = (uint32_t)__shfl((int)b0, head_last0, 64) + (uint32_t)(64 - head_last0);
+        const uint32_t bb1 = (cross && head1 == 0) ? cont : (uint32_t)__shfl((int)b1, head1, 64);
+        if (k0 < un) perm[bb0 + (uint32_t)(lane - head0)] = (uint32_t)p0;
+        if (k1 < un) perm[bb1 + (uint32_t)(lane - head1)] = (uint32_t)p1;
     }
 }
 
@@ -368,7 +388,9 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
     hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, nb, b.tprev,
                        b.ctr, n, err, b.cstart, b.cend, b.active, b.astart, b.tiles, tg, b.occ, keep ? 1 : 0);
     if (n > 0) {
-        hipLaunchKernelGGL(k_bin_scatter, dim3(pblocks), dim3(256), 0, s, keys, n, nelem, (const uint32_t *)b.cstart,
+        int sblocks = cdiv(n, 512);
+        if (sblocks > 8192) sblocks = 8192;
+        hipLaunchKernelGGL(k_bin_scatter, dim3(sblocks), dim3(256), 0, s, keys, n, nelem, (const uint32_t *)b.cstart,
                            b.cend, b.perm, b.stamp, b.fixlist, b.ctr);
         hipLaunchKernelGGL(k_bin_fixup, dim3(256), dim3(256), 0, s, (const uint32_t *)b.fixlist,
                            (const BinCounters *)b.ctr, (const uint32_t *)b.cstart, (const uint32_t *)b.cend, b.perm,
