@@ -243,6 +243,36 @@ def test_fp32_deterministic():
     assert_parts_equal(runs[0], runs[1])
 
 
+def det_cofactor(F):
+    """det of row-major 3x3 matrices (n, 9), first-row cofactor expansion in
+    IEEE double without fused operations (numpy elementwise arithmetic)."""
+    A = F.reshape(-1, 9)
+    return (A[:, 0] * (A[:, 4] * A[:, 8] - A[:, 5] * A[:, 7]) + A[:, 1] * (A[:, 5] * A[:, 6] - A[:, 3] * A[:, 8])
+            + A[:, 2] * (A[:, 3] * A[:, 7] - A[:, 4] * A[:, 6]))
+
+
+def test_fp32_det_and_volume_follow_the_state():
+    """The FP32 engine keeps det F and the FP64 volume only as uploaded and
+    derives them from F at download once steps have run: without a step the
+    uploaded values come back unchanged; after steps (with relayouts) det F is
+    the determinant of the downloaded F bit for bit and V = det F V0."""
+    s = scenes.tiny(n_cells=8)
+    rng = np.random.default_rng(3)
+    s.parts.det_def_grad = 1.0 + 1e-3 * rng.random(s.n)
+    s.parts.vol = s.parts.vol0 * (1.0 + 1e-3 * rng.random(s.n))
+    e = engine(s, capi.PREC_FP32, relayout=3)
+    d0 = e.download(s.n)
+    np.testing.assert_array_equal(d0.det_def_grad, s.parts.det_def_grad)
+    np.testing.assert_array_equal(d0.vol, s.parts.vol)
+    assert e.step(1, 7) == 0
+    d1 = e.download(s.n)
+    e.close()
+    J = det_cofactor(d1.def_grad)
+    np.testing.assert_array_equal(d1.det_def_grad, J)
+    np.testing.assert_array_equal(d1.vol, J * d1.vol0)
+    assert not np.array_equal(d1.det_def_grad, s.parts.det_def_grad)
+
+
 def test_fp32_relayout_changes_only_summation_order():
     s = scenes.tiny(n_cells=16)
     outs = []
