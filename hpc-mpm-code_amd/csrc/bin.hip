@@ -21,10 +21,13 @@
 //   4. scatter    wave-aggregated atomic cursor per run; lanes of a run get
 //                 consecutive positions in slot order. A run that does not
 //                 land at its cell's start means the cell received several
-//                 runs (whose relative order depends on scheduling): that
-//                 cell is queued once for
-//   5. fix-up     sorting its positions by slot index, which restores the
-//                 deterministic stable order.
+//                 runs (whose relative order depends on scheduling): the cell
+//                 is marked with this binning's stamp (a plain store), and
+//   5. fix-up     one thread per active cell sorts the positions of a marked
+//                 cell by slot index, which restores the deterministic stable
+//                 order. No shared queue counter: when many cells are marked
+//                 (particles out of cell order), appends to one counter
+//                 serialised the scatter.
 //
 // Integer atomics only; counts, offsets and the final order do not depend on
 // the order in which waves run.
@@ -49,7 +52,6 @@ __global__ void __launch_bounds__(256) k_bin_count(const uint32_t *__restrict__ 
     for (int i = gid; i < ntiles; i += gridDim.x * blockDim.x) tiles[i] = 0u;
     if (gid == 0) {
         ctr->n_active = 0u;
-        ctr->n_fix = 0u;
     }
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int p = base + threadIdx.x;
@@ -248,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
 __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict__ keys, int n, int nelem,
                                                       const uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend,
                                                       uint32_t *__restrict__ perm, uint32_t *__restrict__ stamp_arr,
-                                                      uint32_t *__restrict__ fixlist, BinCounters *ctr) {
+                                                      const BinCounters *ctr) {
     const int lane = threadIdx.x & 63;
     const uint32_t stamp = ctr->stamp;
     if (ctr->bad) return;
@@ -268,16 +270,11 @@ __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict_
         if (h0 && k0 < un) {
             const int add = len0 + ((cross && lane == head_last0) ? len_first1 : 0);
             b0 = atomicAdd(&cend[k0], (uint32_t)add);
-            if (b0 != cstart[k0]) {
-                // this cell received more than one run: queue it once for the fix-up
-                if (atomicExch(&stamp_arr[k0], stamp) != stamp) fixlist[atomicAdd(&ctr->n_fix, 1u)] = k0;
-            }
+            if (b0 != cstart[k0]) stamp_arr[k0] = stamp;  // several runs: the fix-up sorts this cell
         }
         if (h1 && k1 < un && !(cross && lane == 0)) {
             b1 = atomicAdd(&cend[k1], (uint32_t)len1);
-            if (b1 != cstart[k1]) {
-                if (atomicExch(&stamp_arr[k1], stamp) != stamp) fixlist[atomicAdd(&ctr->n_fix, 1u)] = k1;
-            }
+            if (b1 != cstart[k1]) stamp_arr[k1] = stamp;
         }
         const uint32_t bb0 = (uint32_t)__shfl((int)b0, head0, 64);
         // the crossing run continues after its 64 - head_last0 slots in half 0
@@ -288,15 +285,17 @@ __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict_
     }
 }
 
-// Sorts the positions of each queued cell by slot index.
-__global__ void __launch_bounds__(256) k_bin_fixup(const uint32_t *__restrict__ fixlist, const BinCounters *ctr,
+// Sorts the positions of each cell marked by this binning by slot index.
+__global__ void __launch_bounds__(256) k_bin_fixup(const uint32_t *__restrict__ active,
+                                                    const uint32_t *__restrict__ stamp_arr, const BinCounters *ctr,
                                                     const uint32_t *__restrict__ cstart,
                                                     const uint32_t *__restrict__ cend, uint32_t *__restrict__ perm,
                                                     uint32_t *__restrict__ err) {
     if (ctr->bad) return;
-    const uint32_t nfix = ctr->n_fix;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nfix; i += gridDim.x * blockDim.x) {
-        const uint32_t c = fixlist[i];
+    const uint32_t na = ctr->n_active, stamp = ctr->stamp;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+        const uint32_t c = active[i];
+        if (stamp_arr[c] != stamp) continue;
         const uint32_t s0 = cstart[c], m = cend[c] - s0;
         uint32_t *a = perm + s0;
         if (m <= 16u) {
@@ -391,10 +390,12 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
         int sblocks = cdiv(n, 512);
         if (sblocks > 8192) sblocks = 8192;
         hipLaunchKernelGGL(k_bin_scatter, dim3(sblocks), dim3(256), 0, s, keys, n, nelem, (const uint32_t *)b.cstart,
-                           b.cend, b.perm, b.stamp, b.fixlist, b.ctr);
-        hipLaunchKernelGGL(k_bin_fixup, dim3(256), dim3(256), 0, s, (const uint32_t *)b.fixlist,
-                           (const BinCounters *)b.ctr, (const uint32_t *)b.cstart, (const uint32_t *)b.cend, b.perm,
-                           err);
+                           b.cend, b.perm, b.stamp, (const BinCounters *)b.ctr);
+        int fblocks = cdiv(nelem, 256);
+        if (fblocks > 4096) fblocks = 4096;
+        hipLaunchKernelGGL(k_bin_fixup, dim3(fblocks), dim3(256), 0, s, (const uint32_t *)b.active,
+                           (const uint32_t *)b.stamp, (const BinCounters *)b.ctr, (const uint32_t *)b.cstart,
+                           (const uint32_t *)b.cend, b.perm, err);
     }
     return (int)hipGetLastError();
 }

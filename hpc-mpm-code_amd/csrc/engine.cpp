@@ -583,7 +583,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         (rc = e->dalloc(&b.cnt, ncap)) || (rc = e->dalloc(&b.cstart, ncap)) || (rc = e->dalloc(&b.cend, ncap)) ||
         (rc = e->dalloc(&b.active, nelem)) || (rc = e->dalloc(&b.astart, nelem + 1)) ||
         (rc = e->dalloc(&b.stamp, nelem)) ||
-        (rc = e->dalloc(&b.fixlist, nelem)) || (rc = e->dalloc(&b.tiles, (size_t)e->tg.ntiles)) ||
+        (rc = e->dalloc(&b.tiles, (size_t)e->tg.ntiles)) ||
         (rc = e->dalloc(&b.bsum, bin_scan_blocks(e->g.nelem))) ||
         (rc = e->dalloc(&b.tprev, bin_scan_blocks(e->g.nelem))) || (rc = e->dalloc(&b.ctr, 1)) ||
         (rc = e->dalloc(&b.occ, ncap / 32)) ||

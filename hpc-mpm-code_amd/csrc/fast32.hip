@@ -990,7 +990,6 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
         for (int i = slot0; i < bf.ntiles; i += gridDim.x * blockDim.x) bf.tiles[i] = 0u;
         if (slot0 == 0) {
             bf.ctr->n_active = 0u;
-            bf.ctr->n_fix = 0u;
         }
     }
     // Wave-wide steps (the network, the run-aggregated cell counts) need every

@@ -34,7 +34,6 @@ void launch_keys_f64(hipStream_t s, const GridGeom &g, const double *x, size_t l
 // flags the node tiles next to a non-empty cell.
 struct BinCounters {
     uint32_t n_active;
-    uint32_t n_fix;
     uint32_t stamp;  // binning generation, advanced on the device (graph replays stay distinct)
     uint32_t bad;    // counts inconsistent with the particle count: this binning placed nothing
 };
@@ -89,8 +88,7 @@ struct BinBuffers {
     uint32_t *cend;     // [bin_cell_capacity]
     uint32_t *active;   // [nelem]
     uint32_t *astart;   // [nelem + 1]
-    uint32_t *stamp;    // [nelem], fix-up queue marks
-    uint32_t *fixlist;  // [nelem]
+    uint32_t *stamp;    // [nelem], stamp of the binning that marked the cell for the fix-up
     uint32_t *perm;     // [n]
     uint32_t *tiles;    // [ntiles]
     uint32_t *occ;      // [bin_cell_capacity / 32] occupancy bits (zero at allocation)
