@@ -186,7 +186,10 @@ int mpm_upload_particles(mpm_engine *eng, const mpm_particles_view *p);
 /* Copies every non-NULL field of the view device -> host, in the original
  * particle order (mass, vol0, matid and trac_force included, so that the
  * view re-uploads as a restart). The FP32 engine returns its FP32 copies of
- * mass, vol0 and trac_force. */
+ * mass, vol0 and trac_force. Once steps have run, its det_def_grad and vol
+ * are det(I + H) and det(I + H) vol0 in FP64 from the stored displacement
+ * gradient H, exactly as its update kernel computes them. Particles outside
+ * the grid keep the values they had when they left, or that were uploaded. */
 int mpm_download_particles(mpm_engine *eng, mpm_particles_view *p);
 int mpm_download_nodes(mpm_engine *eng, mpm_nodes_view *nodes);
 
