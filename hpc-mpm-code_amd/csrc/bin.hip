@@ -161,6 +161,8 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
     const int b = blockIdx.x;
     const uint32_t was = tprev[b];
     const uint64_t mine = bsum[b];
+    // a tile left as it is needs no prefix (only the last block forms the totals)
+    if (mine == 0ull && was == 0u && b != nb - 1) return;  // uniform over the block
     // tile totals before this tile, and of all tiles
     uint64_t before = 0, all = 0;
     for (int i = threadIdx.x; i < nb; i += kBlock) {
