@@ -39,10 +39,10 @@ METRIC = "particle-steps/sec on 1 MI355X, surrogate vs plain at \u22641e-3 rel s
 BYTES = {
     # perm 4 + cell offset 12 + v 12 + m 4 + V 4 + stress 24
     "p2g_cells": (60, 0, 0),
-    # node m 4 + v 12 + a 12 written
-    "p2g_nodes": (0, 28, 0),
-    # perm 4 + cell offset 12 + m 4 read, v 12 written; node v, a read 24
-    "musl_cells": (32, 24, 0),
+    # node m 4 + v 12 + a 12 + (v + dt a) 12 written
+    "p2g_nodes": (0, 40, 0),
+    # perm 4 + cell offset 12 + m 4 read, v 12 written; node v + dt a (PIC) or a (FLIP) read 12
+    "musl_cells": (32, 12, 0),
     # node m 4 read, MUSL v 12 written
     "musl_nodes": (0, 16, 0),
     # key 4 + offset 12 + H 36 + V0 4 read; offset 12 + H 36 + stress 24 + V 4 written (the key only for

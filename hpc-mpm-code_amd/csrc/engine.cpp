@@ -623,6 +623,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         if ((rc = e->dalloc(&N.nm, nodec)) || (rc = e->dalloc(&N.nvof, nodec)) || (rc = e->dalloc(&N.np, 3 * nodec)) ||
             (rc = e->dalloc(&N.nf, 3 * nodec)) || (rc = e->dalloc(&N.nv, 3 * nodec)) ||
             (rc = e->dalloc(&N.na, 3 * nodec)) || (rc = e->dalloc(&N.nvm, 3 * nodec)) ||
+            (rc = e->dalloc(&N.nvp, 3 * nodec)) ||
             (rc = e->dalloc(&e->partials, (size_t)kPartialsPerCell * nelem))) {
             delete e;
             return rc;

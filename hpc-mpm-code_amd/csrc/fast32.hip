@@ -518,6 +518,8 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
     for (int c = 0; c < 3; c++) {
         N.nv[c * nodec + n] = v[c];
         N.na[c * nodec + n] = a[c];
+        // formed once here rather than from two loads per corner in each MUSL cell
+        N.nvp[c * nodec + n] = v[c] + sc.dt * a[c];
     }
     if (DIAG) {
         float vof = s[7] + cp[7];
@@ -570,8 +572,7 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                         const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
-                            float a = N.na[c * nodec + nid];
-                            nvl[q][c] = FLIP ? a : (N.nv[c * nodec + nid] + sc.dt * a);
+                            nvl[q][c] = FLIP ? N.na[c * nodec + nid] : N.nvp[c * nodec + nid];
                         }
                     }
         } else {

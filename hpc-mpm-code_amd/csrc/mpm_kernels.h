@@ -215,6 +215,7 @@ struct NodesF32 {
     float *nv;    // [3][nodec] velocity after the P2G grid update (+dt a for PIC)
     float *na;    // [3][nodec] acceleration (FLIP and diagnostics)
     float *nvm;   // [3][nodec] MUSL velocity
+    float *nvp;   // [3][nodec] v + dt a, the node velocity PIC's G2P interpolates
     float *np;    // [3][nodec] momentum (diagnostics, may be null)
     float *nf;    // [3][nodec] force (diagnostics, may be null)
     float *nvof;  // [nodec] volume (diagnostics, may be null)
