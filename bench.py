@@ -59,8 +59,8 @@ BYTES = {
 # profiles/hbm_traffic_cfg2_4M_fp32.md). Reported as roofline.traffic for
 # that workload only.
 MEASURED_TRAFFIC = {
-    "cfg2-3d-neohookean-16ppc": {"update": 573019067, "p2g_cells": 315724920, "musl_cells": 188608442,
-                                 "p2g_nodes": 85645507, "musl_nodes": 40453126},
+    "cfg2-3d-neohookean-16ppc": {"update": 573016077, "p2g_cells": 315724512, "musl_cells": 175602166,
+                                 "p2g_nodes": 89658072, "musl_nodes": 40454217},
 }
 
 
