@@ -421,8 +421,8 @@ int alloc_particles(mpm_engine *e, int n) {
     e->free_particles();
     e->n = n;
     // rows of the structure-of-arrays fields padded to 64 entries, so that
-    // every field starts on a cache-line boundary whatever n is (an
-    // unaligned row costs the update kernel 12% and P2G 25% more time)
+    // every field starts on a cache-line boundary whatever n is (unaligned
+    // rows cost the update kernel 12% more time)
     e->ld = ((size_t)n + 63) & ~(size_t)63;
     const size_t ld = e->ld;
     int rc = 0;
