@@ -222,6 +222,24 @@ def test_fp32_cfg1_1000_steps():
     assert rel(nd.nmome, o.nmome) < 1e-3            # node momentum, relative L2
 
 
+def test_fp32_particle_count_off_row_multiple():
+    """A particle count that is not a multiple of the 64-entry row padding of
+    the particle arrays, with relayouts: the FP32 run follows the oracle."""
+    s = scenes.tiny(n_cells=12)
+    p = s.parts
+    n = p.n - 37
+    assert n % 64 != 0
+    s.parts = capi.Particles(n, coord=p.coord[:n], vel=p.vel[:n], mass=p.mass[:n], vol0=p.vol0[:n])
+    o = oracle_run(s, 20)
+    e = engine(s, capi.PREC_FP32, relayout=3)
+    assert e.step(1, 20) == 0
+    got = e.download(n)
+    assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-7
+    assert rel(got.vel, o.parts.vel) < 1e-4
+    assert rel(got.stress, o.parts.stress) < 1e-3
+    np.testing.assert_array_equal(got.mass, o.parts.mass.astype(np.float32))
+
+
 def test_fp32_particle_leaving_the_grid_is_an_error():
     s = scenes.tiny(n_cells=8)
     s.bcs = {}
