@@ -1496,12 +1496,19 @@ int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *nu
     auto to_pid = [&](uint32_t slot) -> int { return pidv.empty() ? (int)slot : pidv[slot]; };
     if (idp2p)
         for (int i = 0; i < n; i++) idp2p[i] = -1;
+    // The reference's lists run through a cell's particles in index order.
+    // The binning orders a cell by slot, which is index order until the FP32
+    // engine relays the slots out; then each cell is put back in index order.
+    std::vector<int> pids;
     for (int c = 0; c < nelem; c++) {
         uint32_t a = cs[c], b = ce[c];
         if (numep) numep[c] = (int32_t)(b - a);
-        if (idepf) idepf[c] = (b > a) ? to_pid(perm[a]) : -1;
+        pids.clear();
+        for (uint32_t s = a; s < b; s++) pids.push_back(to_pid(perm[s]));
+        if (!pidv.empty()) std::sort(pids.begin(), pids.end());
+        if (idepf) idepf[c] = pids.empty() ? -1 : pids[0];
         if (idp2p)
-            for (uint32_t s = a; s + 1 < b; s++) idp2p[to_pid(perm[s])] = to_pid(perm[s + 1]);
+            for (size_t q = 0; q + 1 < pids.size(); q++) idp2p[pids[q]] = pids[q + 1];
     }
     if (cell_of)
         for (int s = 0; s < n; s++) cell_of[to_pid((uint32_t)s)] = keys[s] >= (uint32_t)nelem ? -1 : (int32_t)keys[s];

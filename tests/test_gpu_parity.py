@@ -240,6 +240,32 @@ def test_fp32_particle_count_off_row_multiple():
     np.testing.assert_array_equal(got.mass, o.parts.mass.astype(np.float32))
 
 
+def test_fp32_kept_counts_under_many_cell_crossings():
+    """A rigid translation fast enough that most particles change cell over
+    the run: the per-cell counts and per-tile totals that the update kernel
+    keeps (instead of recounting) still give, after every step, a binning
+    whose counts and linked lists match the particles' cells exactly."""
+    s = scenes.tiny(n_cells=16, gravity=(0.0, 0.0, 0.0), omega=0.0)
+    s.bcs = {}
+    s.parts.vel[:] = [200.0, 117.0, 153.0]
+    e = engine(s, capi.PREC_FP32, relayout=4)
+    for _ in range(6):
+        assert e.step(1, 9) == 0
+        e.bin()
+        idepf, idp2p, numep, cell = e.bins(s.n)
+        assert np.all(cell >= 0)
+        np.testing.assert_array_equal(numep, np.bincount(cell, minlength=len(numep)))
+        # every cell's list runs through its particles in ascending index order
+        order = np.lexsort((np.arange(s.n), cell))
+        starts = np.r_[True, cell[order][1:] != cell[order][:-1]]
+        np.testing.assert_array_equal(idepf[cell[order][starts]], order[starts])
+        nxt = np.r_[order[1:], -1]
+        nxt[np.r_[~starts[1:], False] == False] = -1
+        np.testing.assert_array_equal(idp2p[order], nxt)
+    got = e.download(s.n)
+    assert np.max(np.abs(got.coord - (s.parts.coord + 54 * s.dt * s.parts.vel))) < 1e-6
+
+
 def test_fp32_particle_leaving_the_grid_is_an_error():
     s = scenes.tiny(n_cells=8)
     s.bcs = {}
