@@ -240,6 +240,19 @@ def test_fp32_particle_count_off_row_multiple():
     np.testing.assert_array_equal(got.mass, o.parts.mass.astype(np.float32))
 
 
+def test_fp32_flip_run_against_oracle():
+    """FLIP in the FP32 engine (its G2P interpolates the node acceleration
+    onto the particle's own velocity), with relayouts, against the oracle."""
+    s = scenes.tiny(n_cells=12, scheme=capi.SCHEME_FLIP)
+    o = oracle_run(s, 20)
+    e = engine(s, capi.PREC_FP32, relayout=4)
+    assert e.step(1, 20) == 0
+    got = e.download(s.n)
+    assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-7
+    assert rel(got.vel, o.parts.vel) < 1e-5
+    assert rel(got.stress, o.parts.stress) < 1e-3
+
+
 def test_fp32_kept_counts_under_many_cell_crossings():
     """A rigid translation fast enough that most particles change cell over
     the run: the per-cell counts and per-tile totals that the update kernel
