@@ -240,6 +240,33 @@ def test_fp32_particle_count_off_row_multiple():
     np.testing.assert_array_equal(got.mass, o.parts.mass.astype(np.float32))
 
 
+def test_fp32_traction_and_two_materials_against_oracle():
+    """The FP32 P2G with traction forces and an update with a material per
+    particle (linear elastic and Neo-Hookean): node fields after one P2G and
+    particles after a short run against the oracle."""
+    s = scenes.tiny(n_cells=12)
+    s.materials = [(capi.MODEL_NEOHOOKEAN, [5e5, 0.35]), (capi.MODEL_LINELAST, [1e6, 0.3])]
+    rng = np.random.default_rng(9)
+    s.parts.matid = (np.arange(s.n) % 2).astype(np.int32)
+    s.parts.trac_force = rng.standard_normal((s.n, 3)) * 1e-2
+    o = pyoracle.Oracle(s)
+    e = engine(s, capi.PREC_FP32)
+    o.bin()
+    e.bin()
+    o.p2g(0.75)
+    e.p2g(0.75)
+    nd = e.nodes()
+    assert rel(nd.nmass, o.nmass) < 1e-6
+    assert rel(nd.nforce, o.nforce) < 1e-5
+    e.close()
+    o = oracle_run(s, 15)
+    e = engine(s, capi.PREC_FP32, relayout=4)
+    assert e.step(1, 15) == 0
+    got = e.download(s.n)
+    assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-7
+    assert rel(got.stress, o.parts.stress) < 1e-3
+
+
 def test_fp32_flip_run_against_oracle():
     """FLIP in the FP32 engine (its G2P interpolates the node acceleration
     onto the particle's own velocity), with relayouts, against the oracle."""
