@@ -854,27 +854,35 @@ __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9]
         s[5] = sg[1];
         break;
     }
-    case 3: {  // Mooney-Rivlin (constitutive_model.cpp:187-285), direct FP32 form
+    case 3: {  // Mooney-Rivlin (constitutive_model.cpp:187-285)
+        // The reference's form 2/J ((c1 + c2 I1) b - c2 b^2) - 2/(3J) (c1 I1 + 2 c2 I2) I + hp I
+        // (b of the isochoric F) subtracts terms of size c1 + c2 down to the stress. Written with
+        // B = b - I, its deviator is 2/J ((c1 + c2 (1 + tr B)) dev B - c2 dev B^2): the same
+        // quantity with nothing cancelling in FP32.
         float c1 = (float)p[0], c2 = (float)p[1], bulk = (float)p[2];
         float Jf = (float)J;
-        float sc = 1.f / cbrtf(Jf);
-        float F[9];
-        for (int q = 0; q < 9; q++) F[q] = sc * (H[q] + ((q == 0 || q == 4 || q == 8) ? 1.f : 0.f));
-        float be[9], be2[9];
+        const float lj = log1pf((float)(J - 1.0));
+        const float E = expm1f(-2.f / 3.f * lj);  // J^(-2/3) - 1
+        float B[9];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                const float HHt = H[3 * a] * H[3 * b] + H[3 * a + 1] * H[3 * b + 1] + H[3 * a + 2] * H[3 * b + 2];
+                B[3 * a + b] = (1.f + E) * (H[3 * a + b] + H[3 * b + a] + HHt) + (a == b ? E : 0.f);
+            }
+        float B2[9];
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++)
-                be[3 * a + b] = F[3 * a] * F[3 * b] + F[3 * a + 1] * F[3 * b + 1] + F[3 * a + 2] * F[3 * b + 2];
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++)
-                be2[3 * a + b] = be[3 * a] * be[b] + be[3 * a + 1] * be[3 + b] + be[3 * a + 2] * be[6 + b];
-        float I1 = be[0] + be[4] + be[8];
-        float I2 = 0.5f * (I1 * I1 - (be2[0] + be2[4] + be2[8]));
-        float hp = bulk * log1pf((float)(J - 1.0)) / Jf;
+                B2[3 * a + b] = B[3 * a] * B[b] + B[3 * a + 1] * B[3 + b] + B[3 * a + 2] * B[6 + b];
+        const float trB = B[0] + B[4] + B[8], trB2 = B2[0] + B2[4] + B2[8];
+        const float k1 = c1 + c2 * (1.f + trB);
+        const float hp = bulk * lj / Jf;
         float sg[9];
-        for (int a = 0; a < 3; a++) {
-            for (int b = 0; b < 3; b++) sg[3 * a + b] = 2.f * ((c1 + c2 * I1) * be[3 * a + b] - c2 * be2[3 * a + b]) / Jf;
-            sg[4 * a] += hp - 2.f / (3.f * Jf) * (c1 * I1 + 2.f * c2 * I2);
-        }
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                const float dB = B[3 * a + b] - (a == b ? trB / 3.f : 0.f);
+                const float dB2 = B2[3 * a + b] - (a == b ? trB2 / 3.f : 0.f);
+                sg[3 * a + b] = 2.f * (k1 * dB - c2 * dB2) / Jf + (a == b ? hp : 0.f);
+            }
         s[0] = sg[0];
         s[1] = sg[4];
         s[2] = sg[8];

@@ -267,6 +267,21 @@ def test_fp32_traction_and_two_materials_against_oracle():
     assert rel(got.stress, o.parts.stress) < 1e-3
 
 
+@pytest.mark.parametrize("model,props", [(capi.MODEL_STVK, (2e6, 0.25)),
+                                         (capi.MODEL_MOONEYRIVLIN, (8e4, 2e4, 5e5))])
+def test_fp32_hyperelastic_models_against_oracle(model, props):
+    """St. Venant-Kirchhoff and Mooney-Rivlin (c1, c2, bulk modulus) in the
+    FP32 update kernel, a short run with relayouts against the oracle."""
+    s = scenes.tiny(n_cells=12, model=model, props=props)
+    o = oracle_run(s, 15)
+    e = engine(s, capi.PREC_FP32, relayout=4)
+    assert e.step(1, 15) == 0
+    got = e.download(s.n)
+    assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-7
+    assert rel(got.stress, o.parts.stress) < 1e-3
+    assert rel(got.def_grad, o.parts.def_grad) < 1e-6
+
+
 def test_fp32_flip_run_against_oracle():
     """FLIP in the FP32 engine (its G2P interpolates the node acceleration
     onto the particle's own velocity), with relayouts, against the oracle."""
