@@ -15,13 +15,14 @@
 
 namespace mpm {
 int set_error_msg(int code, const char *msg);
+int exception_status();
 }
 
 struct mpm_case {
     mpm_case_desc d{};
     std::vector<mpm_material_desc> mats;
     std::vector<double> coord, mass, vol0;
-    std::vector<int32_t> id, matid;
+    std::vector<int32_t> id, matid, surf;
     std::vector<int32_t> nbc[3];
     std::vector<double> fbc[3];
     std::vector<int32_t> node_count;  // nelem, node, nodec as written in griddata.txt
@@ -37,6 +38,16 @@ struct Reader {
     bool ok = true;
     explicit Reader(const char *path) : in(path), name(path) { ok = in.is_open(); }
     void skip_line() { in.ignore(1 << 20, '\n'); }
+    // Bytes in the file: a declared count of n records needs at least
+    // 2 n bytes (one digit and a separator each), so counts above half the
+    // size are rejected before anything is allocated for them.
+    long long size() {
+        const auto here = in.tellg();
+        in.seekg(0, std::ios::end);
+        const long long sz = (long long)in.tellg();
+        in.seekg(here);
+        return sz;
+    }
     template <typename T>
     void get(T &v) {
         if (!(in >> v)) ok = false;
@@ -157,7 +168,8 @@ int read_griddata(mpm_case *c, const char *path) {
         int ibc = 0;
         r.get(ibc);
         r.skip_line();
-        if (!r.ok || ibc < 0 || ibc > nodec) return fail(std::string("malformed BC list in ") + path);
+        if (!r.ok || ibc < 0 || ibc > nodec || ibc > r.size() / 2)
+            return fail(std::string("malformed BC list in ") + path);
         c->nbc[comp].resize(ibc);
         c->fbc[comp].resize(ibc);
         for (int i = 0; i < ibc; i++) {
@@ -177,7 +189,8 @@ int read_spdata(mpm_case *c, const char *path) {
     long long num = 0;
     r.get(num);
     r.skip_line();
-    if (!r.ok || num < 0 || num >= (1ll << 31)) return fail(std::string("bad point count in ") + path);
+    if (!r.ok || num < 0 || num >= (1ll << 31) || num > r.size() / 2)
+        return fail(std::string("bad point count in ") + path);
     c->coord.resize(3 * num);
     for (long long i = 0; i < num; i++) {
         for (int a = 0; a < 3; a++) r.get(c->coord[3 * i + a]);
@@ -185,14 +198,32 @@ int read_spdata(mpm_case *c, const char *path) {
     }
     c->id.resize(num);
     c->matid.resize(num);
+    c->surf.assign(num, 0);
     c->mass.resize(num);
     c->vol0.resize(num);
-    for (long long i = 0; i < num; i++) {
-        r.get(c->id[i]);
-        r.get(c->matid[i]);
-        r.get(c->mass[i]);
-        r.get(c->vol0[i]);
-        r.skip_line();
+    // Point lines: "id matid mass vol0" as the generator writes them
+    // (solid_generator.cpp:203-206) or "id matid surf_point mass vol0" as the
+    // solver's reader takes them (module/solid/solid_data_io.cpp:22). The
+    // layout is fixed by the first line; a line with another column count
+    // is an error.
+    int cols = 0;
+    std::string line;
+    for (long long i = 0; r.ok && i < num; i++) {
+        if (!std::getline(r.in, line)) {
+            r.ok = false;
+            break;
+        }
+        std::istringstream ls(line);
+        std::vector<std::string> tok;
+        for (std::string t; ls >> t;) tok.push_back(t);
+        if (i == 0) cols = (int)tok.size();
+        if ((cols != 4 && cols != 5) || (int)tok.size() != cols)
+            return fail(std::string("point line with an unexpected column count in ") + path);
+        std::istringstream vs(line);
+        vs >> c->id[i] >> c->matid[i];
+        if (cols == 5) vs >> c->surf[i];
+        vs >> c->mass[i] >> c->vol0[i];
+        if (vs.fail()) r.ok = false;
     }
     if (!r.ok) return fail(std::string("malformed point data in ") + path);
     c->d.n = num;
@@ -203,7 +234,7 @@ int read_spdata(mpm_case *c, const char *path) {
 
 extern "C" {
 
-int mpm_case_read(const char *input_txt, const char *griddata_txt, const char *spdata_txt, mpm_case **out) {
+int mpm_case_read(const char *input_txt, const char *griddata_txt, const char *spdata_txt, mpm_case **out) try {
     if (!out) return fail("null output");
     *out = nullptr;
     auto *c = new mpm_case();
@@ -225,21 +256,27 @@ int mpm_case_read(const char *input_txt, const char *griddata_txt, const char *s
     }
     *out = c;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_case_describe(const mpm_case *c, mpm_case_desc *out) {
+int mpm_case_describe(const mpm_case *c, mpm_case_desc *out) try {
     if (!c || !out) return fail("null argument");
     *out = c->d;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_case_materials(const mpm_case *c, mpm_material_desc *mats, int32_t max) {
+int mpm_case_materials(const mpm_case *c, mpm_material_desc *mats, int32_t max) try {
     if (!c || (max > 0 && !mats)) return fail("null argument");
     for (int i = 0; i < (int)c->mats.size() && i < max; i++) mats[i] = c->mats[i];
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_case_particles(const mpm_case *c, mpm_particles_view *v) {
+int mpm_case_particles(const mpm_case *c, mpm_particles_view *v) try {
     if (!c || !v) return fail("null argument");
     std::memset(v, 0, sizeof *v);
     v->n = c->d.n;
@@ -249,17 +286,21 @@ int mpm_case_particles(const mpm_case *c, mpm_particles_view *v) {
     v->vol0 = mc->vol0.data();
     v->matid = mc->matid.data();
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_case_bc(const mpm_case *c, int32_t comp, const int32_t **nbc, const double **fbc, int32_t *ibc) {
+int mpm_case_bc(const mpm_case *c, int32_t comp, const int32_t **nbc, const double **fbc, int32_t *ibc) try {
     if (!c || comp < 0 || comp > 2 || !nbc || !fbc || !ibc) return fail("bad arguments");
     *nbc = c->nbc[comp].data();
     *fbc = c->fbc[comp].data();
     *ibc = (int32_t)c->nbc[comp].size();
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_case_write(const mpm_case *c, const char *griddata_txt, const char *spdata_txt) {
+int mpm_case_write(const mpm_case *c, const char *griddata_txt, const char *spdata_txt) try {
     if (!c) return fail("null case");
     const mpm_case_desc &d = c->d;
     if (griddata_txt) {
@@ -311,10 +352,12 @@ int mpm_case_write(const mpm_case *c, const char *griddata_txt, const char *spda
         if (!o.good()) return fail(std::string("write failed: ") + spdata_txt);
     }
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 int mpm_case_engine(const mpm_case *c, int32_t precision, int32_t relayout_every, int32_t shifting,
-                    mpm_engine **out) {
+                    mpm_engine **out) try {
     if (!c || !out) return fail("null argument");
     if (c->mats.empty() || c->d.n <= 0) return fail("case has no materials or no points");
     mpm_step_params p = c->d.step;
@@ -333,6 +376,8 @@ int mpm_case_engine(const mpm_case *c, int32_t precision, int32_t relayout_every
         *out = nullptr;
     }
     return rc;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 void mpm_case_free(mpm_case *c) { delete c; }
@@ -374,7 +419,7 @@ bool get_rows(Reader &r, std::vector<double> *a, int64_t n, int per) {
 extern "C" {
 
 int mpm_restart_save(const char *path, const mpm_particles_view *p, int32_t scheme, const double *cmat,
-                     const double *dinv, const int32_t *id, const int32_t *surf, int32_t digits) {
+                     const double *dinv, const int32_t *id, const int32_t *surf, int32_t digits) try {
     if (!path || !p || p->n < 0 || (p->n > 0 && (!p->coord || !p->mass || !p->vol0)))
         return fail("restart needs a path and a view with coord, mass, vol0");
     if (scheme < MPM_SCHEME_PIC || scheme > MPM_SCHEME_APIC) return fail("unknown transfer scheme");
@@ -416,9 +461,11 @@ int mpm_restart_save(const char *path, const mpm_particles_view *p, int32_t sche
     }
     if (!o.good()) return fail(std::string("write failed: ") + path);
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_restart_load(const char *path, int32_t scheme, mpm_restart **out) {
+int mpm_restart_load(const char *path, int32_t scheme, mpm_restart **out) try {
     if (!path || !out) return fail("null argument");
     *out = nullptr;
     if (scheme < MPM_SCHEME_PIC || scheme > MPM_SCHEME_APIC) return fail("unknown transfer scheme");
@@ -427,7 +474,8 @@ int mpm_restart_load(const char *path, int32_t scheme, mpm_restart **out) {
     long long num = 0;
     r.get(num);
     r.skip_line();
-    if (!r.ok || num < 0 || num >= (1ll << 31)) return fail(std::string("bad point count in ") + path);
+    if (!r.ok || num < 0 || num >= (1ll << 31) || num > r.size() / 2)
+        return fail(std::string("bad point count in ") + path);
     auto *R = new mpm_restart();
     R->scheme = scheme;
     const int64_t n = num;
@@ -461,10 +509,12 @@ int mpm_restart_load(const char *path, int32_t scheme, mpm_restart **out) {
     }
     *out = R;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 int mpm_restart_view(const mpm_restart *R, mpm_particles_view *v, const double **cmat, const double **dinv,
-                     const int32_t **id, const int32_t **surf) {
+                     const int32_t **id, const int32_t **surf) try {
     if (!R || !v) return fail("null argument");
     auto *m = const_cast<mpm_restart *>(R);
     std::memset(v, 0, sizeof *v);
@@ -483,11 +533,13 @@ int mpm_restart_view(const mpm_restart *R, mpm_particles_view *v, const double *
     if (id) *id = R->id.data();
     if (surf) *surf = R->surf.data();
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 void mpm_restart_free(mpm_restart *r) { delete r; }
 
-int mpm_restart_write(mpm_engine *eng, const char *path, int32_t digits) {
+int mpm_restart_write(mpm_engine *eng, const char *path, int32_t digits) try {
     mpm_engine_info info;
     int rc = mpm_get_engine_info(eng, &info);
     if (rc) return rc;
@@ -514,9 +566,11 @@ int mpm_restart_write(mpm_engine *eng, const char *path, int32_t digits) {
     }
     return mpm_restart_save(path, &p, info.scheme, cm.empty() ? nullptr : cm.data(),
                             di.empty() ? nullptr : di.data(), nullptr, nullptr, digits);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_restart_read(mpm_engine *eng, const char *path) {
+int mpm_restart_read(mpm_engine *eng, const char *path) try {
     mpm_engine_info info;
     int rc = mpm_get_engine_info(eng, &info);
     if (rc) return rc;
@@ -529,6 +583,8 @@ int mpm_restart_read(mpm_engine *eng, const char *path) {
     if (!rc && info.scheme >= MPM_SCHEME_TPIC) rc = mpm_upload_affine(eng, cm, di);
     mpm_restart_free(R);
     return rc;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 }  // extern "C"

@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -26,6 +28,20 @@ static int set_error(int code, const std::string &msg) {
 }
 
 int set_error_msg(int code, const char *msg) { return set_error(code, msg); }
+
+// Status of an exception caught at the C boundary (called from a catch
+// block): no C++ exception leaves an entry point.
+int exception_status() {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return set_error(MPM_ERR_HOST_MEMORY, "host memory allocation failed");
+    } catch (const std::exception &ex) {
+        return set_error(MPM_ERR_INTERNAL, std::string("exception: ") + ex.what());
+    } catch (...) {
+        return set_error(MPM_ERR_INTERNAL, "unknown exception");
+    }
+}
 
 int hip_fail(hipError_t e, const char *what, const char *file, int line) {
     char buf[512];
@@ -307,6 +323,7 @@ int do_bin(mpm_engine *e) {
     if (e->prec == MPM_PREC_FP32 && e->relayout_every > 0 && ++e->steps_since_relayout >= e->relayout_every) {
         // physically reorder particle slots into the sorted order
         const int o = 1 - e->cur;
+        if (!e->P32[o].r || !e->pid[o]) return set_error(MPM_ERR_INTERNAL, "relayout without a second particle buffer");
         launch_f32_permute(e->stream, e->P32[e->cur], e->P32[o], e->perm, e->pid[e->cur], e->pid[o]);
         e->cur = o;
         launch_iota_u32(e->stream, e->perm_id, n);
@@ -417,6 +434,36 @@ int require_particles(mpm_engine *e) {
     return 0;
 }
 
+// One FP32 particle buffer (slot arrays of every field the upload carries);
+// buffer 1 is the target of the relayout permutation.
+int alloc_f32_buffer(mpm_engine *e, int b) {
+    const size_t ld = e->ld;
+    PartF32 &P = e->P32[b];
+    int rc = 0;
+    if ((rc = e->dalloc(&P.r, 3 * ld, true)) || (rc = e->dalloc(&P.v, 3 * ld, true)) || (rc = e->dalloc(&P.m, ld, true)) ||
+        (rc = e->dalloc(&P.V0, ld, true)) || (rc = e->dalloc(&P.V, ld, true)) ||
+        (rc = e->dalloc(&P.V32, ld, true)) || (rc = e->dalloc(&P.J, ld, true)) ||
+        (rc = e->dalloc(&P.H, 9 * ld, true)) || (rc = e->dalloc(&P.sig, 6 * ld, true)) ||
+        (rc = e->dalloc(&P.key, ld, true)) || (rc = e->dalloc(&e->pid[b], ld, true)))
+        return rc;
+    if (e->has_trac && (rc = e->dalloc(&P.trac, 3 * ld, true))) return rc;
+    if (e->has_matid && (rc = e->dalloc(&P.matid, ld, true))) return rc;
+    if (e->has_epsp && (rc = e->dalloc(&P.eps_p, ld, true))) return rc;
+    P.ld = ld;
+    P.n = e->n;
+    return 0;
+}
+
+// A relayout needs the second FP32 buffer. An engine whose particles were
+// uploaded with relayout off has only the first; allocate the second before
+// the interval becomes positive.
+int ensure_relayout_buffer(mpm_engine *e) {
+    if (e->prec != MPM_PREC_FP32 || e->n <= 0) return 0;
+    const int o = 1 - e->cur;
+    if (e->P32[o].r && e->pid[o]) return 0;
+    return alloc_f32_buffer(e, o);
+}
+
 int alloc_particles(mpm_engine *e, int n) {
     e->free_particles();
     e->n = n;
@@ -447,21 +494,8 @@ int alloc_particles(mpm_engine *e, int n) {
         P.n = n;
     } else {
         int nbuf = e->relayout_every > 0 ? 2 : 1;
-        for (int b = 0; b < nbuf; b++) {
-            PartF32 &P = e->P32[b];
-            if ((rc = e->dalloc(&P.r, 3 * ld, true)) || (rc = e->dalloc(&P.v, 3 * ld, true)) || (rc = e->dalloc(&P.m, ld, true)) ||
-                (rc = e->dalloc(&P.V0, ld, true)) || (rc = e->dalloc(&P.V, ld, true)) ||
-                (rc = e->dalloc(&P.V32, ld, true)) || (rc = e->dalloc(&P.J, ld, true)) ||
-                (rc = e->dalloc(&P.H, 9 * ld, true)) || (rc = e->dalloc(&P.sig, 6 * ld, true)) ||
-                (rc = e->dalloc(&P.key, ld, true)) ||
-                (rc = e->dalloc(&e->pid[b], ld, true)))
-                return rc;
-            if (e->has_trac && (rc = e->dalloc(&P.trac, 3 * ld, true))) return rc;
-            if (e->has_matid && (rc = e->dalloc(&P.matid, ld, true))) return rc;
-            if (e->has_epsp && (rc = e->dalloc(&P.eps_p, ld, true))) return rc;
-            P.ld = ld;
-            P.n = n;
-        }
+        for (int b = 0; b < nbuf; b++)
+            if ((rc = alloc_f32_buffer(e, b))) return rc;
         e->cur = 0;
         if ((rc = e->dalloc(&e->perm_id, ld, true))) return rc;
     }
@@ -524,7 +558,7 @@ const char *mpm_version(void) { return "mpm-mi355x 0.1.0 gfx950"; }
 const char *mpm_last_error(void) { return g_last_error.c_str(); }
 
 int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, int32_t nmat,
-                      const mpm_step_params *params, mpm_engine **out) {
+                      const mpm_step_params *params, mpm_engine **out) try {
     if (!grid || !params || !out || (nmat > 0 && !mats))
         return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
@@ -637,6 +671,8 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     }
     *out = e;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 // Sums the per-region network tallies (evaluated, rejected) on the host.
@@ -697,7 +733,7 @@ static int sur_require(mpm_engine *e) {
     return 0;
 }
 
-int mpm_surrogate_collect(mpm_engine *e, int32_t on, int64_t capacity, double rate) {
+int mpm_surrogate_collect(mpm_engine *e, int32_t on, int64_t capacity, double rate) try {
     int rc = sur_require(e);
     if (rc) return rc;
     e->drop_graph();
@@ -724,6 +760,8 @@ int mpm_surrogate_collect(mpm_engine *e, int32_t on, int64_t capacity, double ra
     su.sorted = false;
     su.logging = true;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 static int sur_samples(mpm_engine *e, int64_t *n) {
@@ -773,7 +811,7 @@ static int sur_sort(mpm_engine *e) {
     return 0;
 }
 
-int mpm_surrogate_fit(mpm_engine *e, const mpm_train_params *tp, float *final_loss) {
+int mpm_surrogate_fit(mpm_engine *e, const mpm_train_params *tp, float *final_loss) try {
     int rc = sur_require(e);
     if (rc) return rc;
     e->drop_graph();
@@ -793,9 +831,11 @@ int mpm_surrogate_fit(mpm_engine *e, const mpm_train_params *tp, float *final_lo
     su.have_net = true;
     if (final_loss) *final_loss = su.loss;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_surrogate_enable(mpm_engine *e, int32_t on, double tol) {
+int mpm_surrogate_enable(mpm_engine *e, int32_t on, double tol) try {
     int rc = sur_require(e);
     if (rc) return rc;
     e->drop_graph();
@@ -810,9 +850,11 @@ int mpm_surrogate_enable(mpm_engine *e, int32_t on, double tol) {
         if (rc) return rc;
     }
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_surrogate_reset_counters(mpm_engine *e) {
+int mpm_surrogate_reset_counters(mpm_engine *e) try {
     int rc = sur_require(e);
     if (rc) return rc;
     unsigned long long c[4];
@@ -826,9 +868,11 @@ int mpm_surrogate_reset_counters(mpm_engine *e) {
         MPM_HIP_CHECK(hipMemsetAsync(su.rstat, 0, 2 * (size_t)(su.qcap / kSurRegion) * sizeof(uint32_t), e->stream));
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_surrogate_stats_get(mpm_engine *e, mpm_surrogate_stats *out) {
+int mpm_surrogate_stats_get(mpm_engine *e, mpm_surrogate_stats *out) try {
     int rc = sur_require(e);
     if (rc) return rc;
     if (!out) return set_error(MPM_ERR_INVALID_ARGUMENT, "null output");
@@ -844,16 +888,20 @@ int mpm_surrogate_stats_get(mpm_engine *e, mpm_surrogate_stats *out) {
     out->loss = e->sur.loss;
     out->tol = e->sur.tol;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_surrogate_get_params(mpm_engine *e, float *params, float *scaling) {
+int mpm_surrogate_get_params(mpm_engine *e, float *params, float *scaling) try {
     int rc = sur_require(e);
     if (rc) return rc;
     if (!e->sur.have_net) return set_error(MPM_ERR_INVALID_ARGUMENT, "no trained or loaded network");
     return mlp_get(e->sur.net, params, scaling, e->stream);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_surrogate_set_params(mpm_engine *e, const float *params, const float *scaling) {
+int mpm_surrogate_set_params(mpm_engine *e, const float *params, const float *scaling) try {
     int rc = sur_require(e);
     if (rc) return rc;
     e->drop_graph();
@@ -862,9 +910,11 @@ int mpm_surrogate_set_params(mpm_engine *e, const float *params, const float *sc
     if ((rc = mlp_set(e->sur.net, params, scaling, e->stream))) return rc;
     e->sur.have_net = true;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_surrogate_download_samples(mpm_engine *e, float *in, float *tgt, int64_t max, int64_t *n) {
+int mpm_surrogate_download_samples(mpm_engine *e, float *in, float *tgt, int64_t max, int64_t *n) try {
     int rc = sur_require(e);
     if (rc) return rc;
     if (!n || max < 0 || (max > 0 && (!in || !tgt))) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
@@ -886,9 +936,11 @@ int mpm_surrogate_download_samples(mpm_engine *e, float *in, float *tgt, int64_t
     }
     *n = k;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_set_shifting(mpm_engine *e, int32_t on, const int32_t *npxye) {
+int mpm_set_shifting(mpm_engine *e, int32_t on, const int32_t *npxye) try {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     if (on && e->prec != MPM_PREC_FP64_EXACT)
         return set_error(MPM_ERR_UNSUPPORTED, "particle shifting runs in the FP64-exact engine only");
@@ -930,14 +982,18 @@ int mpm_set_shifting(mpm_engine *e, int32_t on, const int32_t *npxye) {
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     e->shifting = true;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_download_nodal_volume(mpm_engine *e, double *nvol) {
+int mpm_download_nodal_volume(mpm_engine *e, double *nvol) try {
     if (!e || !nvol) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     if (!e->d_nvol) return set_error(MPM_ERR_INVALID_ARGUMENT, "no nodal volume (call mpm_set_shifting)");
     MPM_HIP_CHECK(hipMemcpyAsync(nvol, e->d_nvol, sizeof(double) * e->g.nodec, hipMemcpyDeviceToHost, e->stream));
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 // F-bar state at the start of a run: Fbar = I, Jbar = 1 (datain_para.cpp:122)
@@ -951,7 +1007,7 @@ static int fbar_reset(mpm_engine *e) {
     return 0;
 }
 
-int mpm_set_fbar(mpm_engine *e, int32_t on) {
+int mpm_set_fbar(mpm_engine *e, int32_t on) try {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     if (on && e->prec != MPM_PREC_FP64_EXACT)
         return set_error(MPM_ERR_UNSUPPORTED, "the F-bar projection runs in the FP64-exact engine only");
@@ -972,6 +1028,8 @@ int mpm_set_fbar(mpm_engine *e, int32_t on) {
         if ((rc = fbar_reset(e))) return rc;
     }
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 static int fbar_copy(mpm_engine *e, double *fbar, double *jbar, bool to_device) {
@@ -1005,13 +1063,17 @@ static int fbar_copy(mpm_engine *e, double *fbar, double *jbar, bool to_device) 
     return 0;
 }
 
-int mpm_download_fbar(mpm_engine *e, double *def_grad_bar, double *det_def_grad_bar) {
+int mpm_download_fbar(mpm_engine *e, double *def_grad_bar, double *det_def_grad_bar) try {
     return fbar_copy(e, def_grad_bar, det_def_grad_bar, false);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_upload_fbar(mpm_engine *e, const double *def_grad_bar, const double *det_def_grad_bar) {
+int mpm_upload_fbar(mpm_engine *e, const double *def_grad_bar, const double *det_def_grad_bar) try {
     if (e) e->drop_graph();
     return fbar_copy(e, const_cast<double *>(def_grad_bar), const_cast<double *>(det_def_grad_bar), true);
+} catch (...) {
+    return mpm::exception_status();
 }
 
 // TPIC velocity gradient / APIC B matrix and APIC inverse D, host [n][3][3]
@@ -1044,14 +1106,20 @@ static int affine_copy(mpm_engine *e, double *cmat, double *dinv, bool to_device
     return 0;
 }
 
-int mpm_download_affine(mpm_engine *e, double *cmat, double *dinv) { return affine_copy(e, cmat, dinv, false); }
-
-int mpm_upload_affine(mpm_engine *e, const double *cmat, const double *dinv) {
-    if (e) e->drop_graph();
-    return affine_copy(e, const_cast<double *>(cmat), const_cast<double *>(dinv), true);
+int mpm_download_affine(mpm_engine *e, double *cmat, double *dinv) try {
+    return affine_copy(e, cmat, dinv, false);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_get_engine_info(mpm_engine *e, mpm_engine_info *out) {
+int mpm_upload_affine(mpm_engine *e, const double *cmat, const double *dinv) try {
+    if (e) e->drop_graph();
+    return affine_copy(e, const_cast<double *>(cmat), const_cast<double *>(dinv), true);
+} catch (...) {
+    return mpm::exception_status();
+}
+
+int mpm_get_engine_info(mpm_engine *e, mpm_engine_info *out) try {
     if (!e || !out) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     out->n = e->n;
     out->nelem = e->g.nelem;
@@ -1061,16 +1129,20 @@ int mpm_get_engine_info(mpm_engine *e, mpm_engine_info *out) {
     out->shifting = e->shifting ? 1 : 0;
     out->fbar = e->fbar ? 1 : 0;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_engine_destroy(mpm_engine *eng) {
+int mpm_engine_destroy(mpm_engine *eng) try {
     if (!eng) return 0;
     (void)hipStreamSynchronize(eng->stream);
     delete eng;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_set_bc(mpm_engine *e, int32_t comp, const int32_t *nbc, const double *fbc, int32_t ibc) {
+int mpm_set_bc(mpm_engine *e, int32_t comp, const int32_t *nbc, const double *fbc, int32_t ibc) try {
     if (!e || comp < 0 || comp > 3 || ibc < 0 || (ibc > 0 && !nbc))
         return set_error(MPM_ERR_INVALID_ARGUMENT, "bad BC arguments");
     for (int i = 0; i < ibc; i++)
@@ -1082,9 +1154,11 @@ int mpm_set_bc(mpm_engine *e, int32_t comp, const int32_t *nbc, const double *fb
         e->fbc[comp].assign(ibc, 0.0);
     e->bc_dirty = true;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
+int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) try {
     if (e) e->drop_graph();
     if (!e || !p || p->n <= 0 || !p->coord || !p->mass || !p->vol0)
         return set_error(MPM_ERR_INVALID_ARGUMENT, "particle view needs n > 0, coord, mass, vol0");
@@ -1241,9 +1315,11 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) {
     e->bins_valid = false;
     e->steps_since_relayout = 0;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) {
+int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) try {
     int rc = require_particles(e);
     if (rc) return rc;
     if (!p || p->n != e->n) return set_error(MPM_ERR_INVALID_ARGUMENT, "view size differs from the engine's");
@@ -1320,9 +1396,11 @@ int mpm_download_particles(mpm_engine *e, mpm_particles_view *p) {
         }
     }
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_download_nodes(mpm_engine *e, mpm_nodes_view *nv) {
+int mpm_download_nodes(mpm_engine *e, mpm_nodes_view *nv) try {
     if (!e || !nv) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     if (nv->nodec != e->g.nodec) return set_error(MPM_ERR_INVALID_ARGUMENT, "nodec differs from the engine's");
     const size_t nodec = (size_t)e->g.nodec;
@@ -1357,17 +1435,21 @@ int mpm_download_nodes(mpm_engine *e, mpm_nodes_view *nv) {
         }
     }
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_bin(mpm_engine *e) {
+int mpm_bin(mpm_engine *e) try {
     int rc = require_particles(e);
     if (rc) return rc;
     clear_err(e);
     if ((rc = do_bin(e))) return rc;
     return check_err(e);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_p2g(mpm_engine *e, double facl) {
+int mpm_p2g(mpm_engine *e, double facl) try {
     int rc = require_particles(e);
     if (rc) return rc;
     if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "call mpm_bin before mpm_p2g");
@@ -1375,18 +1457,22 @@ int mpm_p2g(mpm_engine *e, double facl) {
     if ((rc = do_p2g(e, facl, 1))) return rc;
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_solve_grid(mpm_engine *e) {
+int mpm_solve_grid(mpm_engine *e) try {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     int rc = upload_bc(e);
     if (rc) return rc;
     if ((rc = do_solve(e))) return rc;
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_g2p(mpm_engine *e, double facl) {
+int mpm_g2p(mpm_engine *e, double facl) try {
     int rc = require_particles(e);
     if (rc) return rc;
     if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "call mpm_bin/mpm_p2g before mpm_g2p");
@@ -1394,6 +1480,8 @@ int mpm_g2p(mpm_engine *e, double facl) {
     clear_err(e);
     if ((rc = do_g2p(e, facl, 1, nullptr))) return rc;
     return check_err(e);
+} catch (...) {
+    return mpm::exception_status();
 }
 
 // ---------------------------------------------------------------- implicit path
@@ -1435,14 +1523,20 @@ static int accel_copy(mpm_engine *e, double *accel, bool to_device) {
     return 0;
 }
 
-int mpm_upload_accel(mpm_engine *e, const double *accel) {
+int mpm_upload_accel(mpm_engine *e, const double *accel) try {
     if (e) e->drop_graph();
     return accel_copy(e, const_cast<double *>(accel), true);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_download_accel(mpm_engine *e, double *accel) { return accel_copy(e, accel, false); }
+int mpm_download_accel(mpm_engine *e, double *accel) try {
+    return accel_copy(e, accel, false);
+} catch (...) {
+    return mpm::exception_status();
+}
 
-int mpm_implicit_p2g(mpm_engine *e, double facl) {
+int mpm_implicit_p2g(mpm_engine *e, double facl) try {
     int rc = implicit_require(e);
     if (rc) return rc;
     if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "call mpm_bin before mpm_implicit_p2g");
@@ -1452,9 +1546,11 @@ int mpm_implicit_p2g(mpm_engine *e, double facl) {
     MPM_HIP_CHECK(hipGetLastError());
     e->last_pass = 1;
     return check_err(e);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_implicit_g2p(mpm_engine *e, const double *ndispl, const double *nb_para, double gamma_nb) {
+int mpm_implicit_g2p(mpm_engine *e, const double *ndispl, const double *nb_para, double gamma_nb) try {
     int rc = implicit_require(e);
     if (rc) return rc;
     if (!ndispl || !nb_para) return set_error(MPM_ERR_INVALID_ARGUMENT, "null ndispl or Newmark coefficients");
@@ -1475,9 +1571,11 @@ int mpm_implicit_g2p(mpm_engine *e, const double *ndispl, const double *nb_para,
     e->bins_valid = false;
     e->last_pass = 2;
     return check_err(e);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *numep, int32_t *cell_of) {
+int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *numep, int32_t *cell_of) try {
     int rc = require_particles(e);
     if (rc) return rc;
     if (!e->bins_valid) return set_error(MPM_ERR_INVALID_ARGUMENT, "no valid binning (call mpm_bin)");
@@ -1513,6 +1611,8 @@ int mpm_download_bins(mpm_engine *e, int32_t *idepf, int32_t *idp2p, int32_t *nu
     if (cell_of)
         for (int s = 0; s < n; s++) cell_of[to_pid((uint32_t)s)] = keys[s] >= (uint32_t)nelem ? -1 : (int32_t)keys[s];
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 // Before recording a graph: make the per-cell counts of the current keys
@@ -1549,7 +1649,7 @@ static int enqueue_steps(mpm_engine *e, int32_t istep0, int32_t nsteps, bool dia
     return 0;
 }
 
-int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
+int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) try {
     int rc = require_particles(e);
     if (rc) return rc;
     if (nsteps < 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "nsteps < 0");
@@ -1632,15 +1732,19 @@ int mpm_step(mpm_engine *e, int32_t istep0, int32_t nsteps) {
         e->kt = kt;
     }
     return check_err(e);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_sync(mpm_engine *e) {
+int mpm_sync(mpm_engine *e) try {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_get_kernel_times(mpm_engine *e, mpm_kernel_times *out) {
+int mpm_get_kernel_times(mpm_engine *e, mpm_kernel_times *out) try {
     if (!e || !out) return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     *out = e->kt;
     if (e->bins_valid || e->n > 0) {
@@ -1670,15 +1774,19 @@ int mpm_get_kernel_times(mpm_engine *e, mpm_kernel_times *out) {
         out->active_nodes = na;
     }
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_set_timing(mpm_engine *e, int32_t enabled) {
+int mpm_set_timing(mpm_engine *e, int32_t enabled) try {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     e->timing = enabled != 0;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
+int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) try {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     e->drop_graph();
     switch (key) {
@@ -1691,6 +1799,10 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
     case OPT_RELAYOUT_EVERY:
         if (value < 0 || value > (1 << 20)) return set_error(MPM_ERR_INVALID_ARGUMENT, "relayout interval out of range");
         if (e->prec != MPM_PREC_FP32) return value ? set_error(MPM_ERR_UNSUPPORTED, "relayout is an FP32 option") : 0;
+        if (value > 0) {
+            int rc = ensure_relayout_buffer(e);
+            if (rc) return rc;
+        }
         e->relayout_every = (int)value;
         e->steps_since_relayout = 0;
         return 0;
@@ -1714,10 +1826,12 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) {
         return 0;
     default: return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown option");
     }
+} catch (...) {
+    return mpm::exception_status();
 }
 
 int mpm_maksf_batch(const mpm_grid_desc *grid, int64_t n, const int32_t *cell, const double *xyg, int32_t *nc,
-                    double *sf, double *dsf) {
+                    double *sf, double *dsf) try {
     if (!grid || n < 0 || (n > 0 && (!cell || !xyg || !nc || !sf || !dsf)))
         return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     int rc = validate_grid(grid, true);
@@ -1755,9 +1869,11 @@ int mpm_maksf_batch(const mpm_grid_desc *grid, int64_t n, const int32_t *cell, c
     if (he != hipSuccess) rc = hip_fail(he, "mpm_maksf_batch", __FILE__, __LINE__);
     cleanup();
     return rc;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, int32_t *cell_of, int32_t *order) {
+int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, int32_t *cell_of, int32_t *order) try {
     if (!grid || n < 0 || (n > 0 && (!xyg || !cell_of || !order)))
         return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     int rc = validate_grid(grid, false);
@@ -1811,9 +1927,11 @@ int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, i
         order[i] = (int32_t)vh[i];
     }
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_bandwidth_probe(int64_t nbytes, int32_t iters, double *gbps) {
+int mpm_bandwidth_probe(int64_t nbytes, int32_t iters, double *gbps) try {
     if (nbytes < 4096 || iters < 1 || !gbps) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad probe arguments");
     const size_t n16 = (size_t)nbytes / 16;
     void *a = nullptr, *b = nullptr;
@@ -1845,6 +1963,8 @@ int mpm_bandwidth_probe(int64_t nbytes, int32_t iters, double *gbps) {
     if (e1) (void)hipEventDestroy(e1);
     if (s) (void)hipStreamDestroy(s);
     return rc;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 }  // extern "C"
@@ -1929,7 +2049,7 @@ int ens_eager(mpm_ensemble *x, const std::vector<int> &idx, int32_t istep0, int3
 
 extern "C" {
 
-int mpm_ensemble_create(mpm_engine *const *engines, int32_t n, const int32_t *bucket, mpm_ensemble **out) {
+int mpm_ensemble_create(mpm_engine *const *engines, int32_t n, const int32_t *bucket, mpm_ensemble **out) try {
     if (!engines || n <= 0 || !out) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad ensemble arguments");
     *out = nullptr;
     auto *x = new mpm_ensemble();
@@ -1955,16 +2075,20 @@ int mpm_ensemble_create(mpm_engine *const *engines, int32_t n, const int32_t *bu
     }
     *out = x;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_ensemble_destroy(mpm_ensemble *x) {
+int mpm_ensemble_destroy(mpm_ensemble *x) try {
     if (!x) return 0;
     if (x->s0) (void)hipStreamSynchronize(x->s0);
     delete x;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_ensemble_step(mpm_ensemble *x, int32_t istep0, int32_t nsteps, double *elapsed_ms) {
+int mpm_ensemble_step(mpm_ensemble *x, int32_t istep0, int32_t nsteps, double *elapsed_ms) try {
     if (!x || nsteps < 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
     int rc = ens_prepare(x);
     if (rc) return rc;
@@ -1972,9 +2096,11 @@ int mpm_ensemble_step(mpm_ensemble *x, int32_t istep0, int32_t nsteps, double *e
     for (size_t i = 0; i < all.size(); i++) all[i] = (int)i;
     if ((rc = ens_eager(x, all, istep0, nsteps, elapsed_ms))) return rc;
     return ens_check(x);
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_ensemble_capture(mpm_ensemble *x, int32_t istep0, int32_t block_steps) {
+int mpm_ensemble_capture(mpm_ensemble *x, int32_t istep0, int32_t block_steps) try {
     if (!x || block_steps <= 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
     for (auto *e : x->eng) {
         if (istep0 <= e->prm.nlstep && e->prm.nlstep > 1)
@@ -2024,9 +2150,11 @@ int mpm_ensemble_capture(mpm_ensemble *x, int32_t istep0, int32_t block_steps) {
     x->block_steps = block_steps;
     // the capture did not run anything: the state is where it was
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
-int mpm_ensemble_launch(mpm_ensemble *x, int32_t nlaunch, double *elapsed_ms) {
+int mpm_ensemble_launch(mpm_ensemble *x, int32_t nlaunch, double *elapsed_ms) try {
     if (!x || nlaunch < 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
     if (!x->exec) return set_error(MPM_ERR_INVALID_ARGUMENT, "no captured graph");
     int rc = ens_prepare(x);
@@ -2045,10 +2173,12 @@ int mpm_ensemble_launch(mpm_ensemble *x, int32_t nlaunch, double *elapsed_ms) {
         e->keys_valid = true;
     }
     return ens_check(x);
+} catch (...) {
+    return mpm::exception_status();
 }
 
 int mpm_ensemble_autotune(mpm_ensemble *x, int32_t istep0, int32_t trial_steps, mpm_tune_result *results,
-                          int32_t max_results, int32_t *nresults) {
+                          int32_t max_results, int32_t *nresults) try {
     if (!x || trial_steps <= 0 || (max_results > 0 && !results) || !nresults)
         return set_error(MPM_ERR_INVALID_ARGUMENT, "bad arguments");
     int rc = ens_prepare(x);
@@ -2066,6 +2196,9 @@ int mpm_ensemble_autotune(mpm_ensemble *x, int32_t istep0, int32_t trial_steps, 
                 idx.push_back((int)i);
                 f32 = f32 && x->eng[i]->prec == MPM_PREC_FP32;
             }
+        if (f32)
+            for (int i : idx)
+                if ((rc = ensure_relayout_buffer(x->eng[i]))) return rc;
         double best = 1e300, worst = 0.0;
         int best_l = 1, best_r = 0, step = istep0;
         for (int li = 0; li < 3; li++)
@@ -2109,6 +2242,8 @@ int mpm_ensemble_autotune(mpm_ensemble *x, int32_t istep0, int32_t trial_steps, 
     }
     *nresults = std::min(nres, max_results);
     return ens_check(x);
+} catch (...) {
+    return mpm::exception_status();
 }
 
 }  // extern "C"

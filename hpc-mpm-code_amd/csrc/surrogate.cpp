@@ -277,7 +277,7 @@ int check_rows(const void *a, const void *b, int64_t n) {
 
 extern "C" {
 
-int mpm_mlp_forward(const float *params, const float *scaling, const float *in, int64_t n, float *out) {
+int mpm_mlp_forward(const float *params, const float *scaling, const float *in, int64_t n, float *out) try {
     if (!params || !scaling) return set_error_msg(MPM_ERR_INVALID_ARGUMENT, "null argument");
     int rc = check_rows(in, out, n);
     if (rc || n == 0) return rc;
@@ -299,10 +299,12 @@ int mpm_mlp_forward(const float *params, const float *scaling, const float *in, 
     d.release();
     if (e != hipSuccess) return hip_fail(e, "mpm_mlp_forward", __FILE__, __LINE__);
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 int mpm_mlp_gradient(const float *params, const float *scaling, const float *in, const float *tgt, int64_t n,
-                     float *grad, float *loss) {
+                     float *grad, float *loss) try {
     if (!params || !scaling || !grad) return set_error_msg(MPM_ERR_INVALID_ARGUMENT, "null argument");
     int rc = check_rows(in, tgt, n);
     if (rc) return rc;
@@ -331,10 +333,12 @@ int mpm_mlp_gradient(const float *params, const float *scaling, const float *in,
     if (e != hipSuccess) return hip_fail(e, "mpm_mlp_gradient", __FILE__, __LINE__);
     if (loss) *loss = lh;
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 int mpm_mlp_train(float *params, const float *scaling, const float *in, const float *tgt, int64_t n,
-                  const mpm_train_params *tp, float *loss_per_epoch) {
+                  const mpm_train_params *tp, float *loss_per_epoch) try {
     if (!params || !scaling || !tp) return set_error_msg(MPM_ERR_INVALID_ARGUMENT, "null argument");
     int rc = check_rows(in, tgt, n);
     if (rc) return rc;
@@ -360,6 +364,8 @@ int mpm_mlp_train(float *params, const float *scaling, const float *in, const fl
     if (rc) return rc;
     if (loss_per_epoch) std::memcpy(loss_per_epoch, el.data(), sizeof(float) * el.size());
     return 0;
+} catch (...) {
+    return mpm::exception_status();
 }
 
 }  // extern "C"

@@ -12,6 +12,7 @@
 namespace mpm {
 
 int set_error_msg(int code, const char *msg);
+int exception_status();
 
 struct MlpDev {
     float *master = nullptr;   // FP32 parameters [kMlpNParams]

@@ -22,6 +22,7 @@ MPM_ERR_HIP = -5
 MPM_ERR_UNSUPPORTED = -6
 MPM_ERR_NO_DEVICE = -7
 MPM_ERR_INTERNAL = -8
+MPM_ERR_HOST_MEMORY = -9
 
 SCHEME_PIC, SCHEME_FLIP, SCHEME_TPIC, SCHEME_APIC = 0, 1, 2, 3
 PREC_FP64_EXACT, PREC_FP32 = 0, 1

@@ -38,7 +38,8 @@ typedef enum mpm_status {
     MPM_ERR_HIP = -5,           /* a HIP runtime call failed */
     MPM_ERR_UNSUPPORTED = -6,   /* option not implemented for this precision/shape */
     MPM_ERR_NO_DEVICE = -7,
-    MPM_ERR_INTERNAL = -8       /* an internal consistency check failed (the step placed nothing) */
+    MPM_ERR_INTERNAL = -8,      /* an internal consistency check failed (the step placed nothing) */
+    MPM_ERR_HOST_MEMORY = -9    /* a host allocation failed */
 } mpm_status;
 
 /* Transfer scheme; same meaning and codes as MapScheme (module/dataset.h:19).

@@ -579,6 +579,23 @@ void orc_nodal_vol(orc_state *st, const int32_t npxye[3]) {
     }
 }
 
+/* Planted mistakes for the tests that pin orc_shift_delta (finite-difference
+ * gradient of the excess-volume norm and the Gauss-Newton step length): the
+ * parity build leaves ORC_SHIFT_MUTANT undefined; _build/libmpm_oracle_mut1.so
+ * drops the factor 2 of the gradient, _build/libmpm_oracle_mut2.so takes the
+ * initial volume vol0 where the reference takes vol. Those tests must fail on
+ * both. */
+#if defined(ORC_SHIFT_MUTANT) && ORC_SHIFT_MUTANT == 1
+#define ORC_SHIFT_GRAD_FACTOR 1.0
+#else
+#define ORC_SHIFT_GRAD_FACTOR 2.0
+#endif
+#if defined(ORC_SHIFT_MUTANT) && ORC_SHIFT_MUTANT == 2
+#define ORC_SHIFT_VOL vol0
+#else
+#define ORC_SHIFT_VOL vol
+#endif
+
 /* DeltaCorrectionParticleShifting, module/material_point.cpp:174-227 */
 void orc_shift_delta(const orc_state *st, double *delta) {
     const int nn = st->nodec, num = st->num;
@@ -615,7 +632,7 @@ void orc_shift_delta(const orc_state *st, double *delta) {
         double s = 0.0;
         for (int n = b0; n < num && n < b0 + ORC_SUM_BLOCK; n++) {
             double *g = geup + 3 * n;
-            for (int i = 0; i < 3; i++) g[i] *= 2.0 * st->vol[n];
+            for (int i = 0; i < 3; i++) g[i] *= ORC_SHIFT_GRAD_FACTOR * st->ORC_SHIFT_VOL[n];
             for (int i = 0; i < 3; i++) s += g[i] * g[i];
         }
         geup_dot += s;

@@ -47,10 +47,21 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE, "oracle", "timing", "omp"], check=True)
 
 
+def mutant_path(k):
+    """Parity build with planted mistake k in orc_shift_delta (1: no factor 2
+    in the gradient, 2: vol0 for vol), for tests that must reject it."""
+    return os.path.join(HERE, "_build", "libmpm_oracle_mut%d.so" % k)
+
+
 def load(fast=False):
     """fast=False: parity build; True: timing build; "omp": timing build with
-    the OpenMP port of the step (orc_run_omp)."""
-    path = LIB_OMP if fast == "omp" else (LIB_FAST if fast else LIB)
+    the OpenMP port of the step (orc_run_omp); ("mutant", k): mutant_path(k)."""
+    if isinstance(fast, tuple) and fast[0] == "mutant":
+        path = mutant_path(fast[1])
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", HERE, "mutants"], check=True)
+    else:
+        path = LIB_OMP if fast == "omp" else (LIB_FAST if fast else LIB)
     if not os.path.exists(path):
         build()
     lib = C.CDLL(path)

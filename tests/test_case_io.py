@@ -192,3 +192,52 @@ def test_restart_errors(tmp_path):
     capi.restart_save(str(f), p, capi.SCHEME_PIC)
     with pytest.raises(capi.MPMError):   # an APIC reader expects the matrix blocks
         capi.restart_load(str(f), capi.SCHEME_APIC)
+
+
+def _spdata_lines(path):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    n = int(lines[0].split()[0])
+    return lines, n
+
+
+def test_spdata_with_the_solver_readers_five_columns(beam):
+    """The solver's reader takes 'id matid surf_point mass vol0'
+    (module/solid/solid_data_io.cpp:22); a five-column file gives the same
+    mass and vol0 as the generator's four-column one."""
+    case, c, b, tmp = beam
+    lines, n = _spdata_lines(tmp / "spdata.txt")
+    five = lines[: 1 + n]
+    for l in lines[1 + n: 1 + 2 * n]:
+        t = l.split()
+        five.append("%10s%10s%10d%15s%15s" % (t[0], t[1], 1, t[2], t[3]))
+    (tmp / "sp5.txt").write_text("\n".join(five) + "\n")
+    again = capi.Case(beam_case.BEAM_INPUT, str(tmp / "griddata.txt"), str(tmp / "sp5.txt"))
+    p, q = case.particles(), again.particles()
+    assert np.array_equal(p.mass, q.mass) and np.array_equal(p.vol0, q.vol0)
+    assert np.array_equal(p.matid, q.matid) and np.array_equal(p.coord, q.coord)
+
+
+@pytest.mark.parametrize("cols", [3, 6])
+def test_spdata_with_another_column_count_is_rejected(beam, cols):
+    case, c, b, tmp = beam
+    lines, n = _spdata_lines(tmp / "spdata.txt")
+    bad = lines[: 1 + n] + [" ".join(["1"] * cols)] * n
+    (tmp / "bad.txt").write_text("\n".join(bad) + "\n")
+    with pytest.raises(capi.MPMError) as ei:
+        capi.Case(beam_case.BEAM_INPUT, str(tmp / "griddata.txt"), str(tmp / "bad.txt"))
+    assert ei.value.code == capi.MPM_ERR_INVALID_ARGUMENT
+
+
+def test_declared_counts_beyond_the_file_size_are_rejected(beam, tmp_path):
+    """A corrupt count line cannot make the reader allocate for 2^31 points."""
+    case, c, b, tmp = beam
+    (tmp / "huge.txt").write_text("2000000000\n0.1 0.1 0.1\n")
+    with pytest.raises(capi.MPMError):
+        capi.Case(beam_case.BEAM_INPUT, str(tmp / "griddata.txt"), str(tmp / "huge.txt"))
+    (tmp / "huge_re.txt").write_text("2000000000\n")
+    import ctypes as C
+    lib = capi.load()
+    h = C.c_void_p()
+    rc = lib.mpm_restart_load(str(tmp / "huge_re.txt").encode(), capi.SCHEME_PIC, C.byref(h))
+    assert rc == capi.MPM_ERR_INVALID_ARGUMENT and not h.value

@@ -99,3 +99,29 @@ def test_atlas_matches_individual_scenes():
         assert np.array_equal(got.coord[sl] - np.asarray(origin), ref.coord), sc.name
         for f in ("vel", "stress", "def_grad", "det_def_grad", "vol"):
             assert np.array_equal(getattr(got, f)[sl], getattr(ref, f)), (sc.name, f)
+
+
+def test_relayout_turned_on_after_upload():
+    """An engine uploaded with relayout off gets its second particle buffer
+    when the option turns relayout on; the run then equals one that had
+    relayout on from the upload."""
+    s = scenes.tiny(n_cells=10)
+    e = capi.Engine(s, precision=capi.PREC_FP32, relayout_every=0)
+    e.upload(s.parts)
+    e.set_option(capi.OPT_RELAYOUT_EVERY, 2)
+    assert e.step(1, 9) == 0
+    got = e.download(s.n)
+    e.close()
+    ref = single(s, capi.PREC_FP32, 2, 9)
+    for f in ("coord", "vel", "stress", "def_grad"):
+        assert np.array_equal(getattr(got, f), getattr(ref, f)), f
+
+
+def test_autotune_of_an_ensemble_created_without_relayout():
+    scs = [scenes.cfg4_scene(k) for k in KS[:3]]
+    ens = capi.Ensemble(scs, precision=capi.PREC_FP32, relayout_every=0)
+    ens.step(1, 2)
+    res = ens.autotune(3, trial_steps=2)
+    assert len(res) == 1
+    ens.step(3 + 9 * 2, 16)
+    ens.close()

@@ -267,18 +267,23 @@ def test_fp32_traction_and_two_materials_against_oracle():
     assert rel(got.stress, o.parts.stress) < 1e-3
 
 
-@pytest.mark.parametrize("model,props", [(capi.MODEL_STVK, (2e6, 0.25)),
-                                         (capi.MODEL_MOONEYRIVLIN, (8e4, 2e4, 5e5))])
-def test_fp32_hyperelastic_models_against_oracle(model, props):
+@pytest.mark.parametrize("model,props,stress_tol", [(capi.MODEL_STVK, (2e6, 0.25), 1e-3),
+                                                    (capi.MODEL_MOONEYRIVLIN, (8e4, 2e4, 5e5), 1e-4)])
+def test_fp32_hyperelastic_models_against_oracle(model, props, stress_tol):
     """St. Venant-Kirchhoff and Mooney-Rivlin (c1, c2, bulk modulus) in the
-    FP32 update kernel, a short run with relayouts against the oracle."""
+    FP32 update kernel, a short run with relayouts against the oracle. The
+    Mooney-Rivlin bound is ten times below the 1.1e-3 that the reference's
+    cancelling form gives in FP32 over these 15 steps, so only the form
+    built on B = b - I passes it."""
     s = scenes.tiny(n_cells=12, model=model, props=props)
     o = oracle_run(s, 15)
     e = engine(s, capi.PREC_FP32, relayout=4)
     assert e.step(1, 15) == 0
     got = e.download(s.n)
+    err = rel(got.stress, o.parts.stress)
+    print("model %d: stress error %.3e" % (model, err), flush=True)
     assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-7
-    assert rel(got.stress, o.parts.stress) < 1e-3
+    assert err < stress_tol
     assert rel(got.def_grad, o.parts.def_grad) < 1e-6
 
 

@@ -2,7 +2,16 @@
 module/material_point.cpp:174-227) and nodal volume (MakNodalVol,
 module/mesh.cpp:160-202), which need MPI to build in the reference: closed
 forms of the nodal volume, no shift for the undisturbed particle lattice,
-descent of the excess-volume norm, mirror symmetry."""
+descent of the excess-volume norm, mirror symmetry, and the correction
+itself from the mathematics it implements.
+
+The correction is one Gauss-Newton step on the excess-volume norm
+E(x) = sum_a max(0, nvof_a - nvol_a)^2 with nvof_a = sum_p N_a(x_p) V_p:
+geup_p = 2 V_p sum_a grad N_a(x_p) nei_a is dE/dx_p at fixed V_p, and
+delta_p = -E geup_p / sum_q |geup_q|^2, so that sum_p delta_p . geup_p = -E.
+The finite-difference tests below predict delta from E alone; the oracle
+builds with a planted mistake (no factor 2 in geup, vol0 for vol) fail
+them."""
 import numpy as np
 import pytest
 
@@ -102,3 +111,105 @@ def test_shift_is_mirror_symmetric():
         j = key[q]
         assert abs(d[j, 0] + d[i, 0]) <= tol
         assert abs(d[j, 1] - d[i, 1]) <= tol and abs(d[j, 2] - d[i, 2]) <= tol
+
+
+def squeezed_scene(idimc, seed=11):
+    """A block squeezed towards its centre, jittered, with particle volumes
+    V != V0, so that every term of the correction is exercised."""
+    s = lattice_scene(8, (2, 2, 2), lo=(2, 2, 2), hi=(6, 6, 6), jitter=0.12, seed=seed)
+    c = np.array([0.5, 0.5, 0.5])
+    s.parts.coord[:] = c + 0.9 * (s.parts.coord - c)
+    rng = np.random.default_rng(seed + 1)
+    s.parts.vol[:] = s.parts.vol0 * rng.uniform(0.7, 1.3, s.n)
+    s.idimc = idimc
+    return s
+
+
+def fd_gradient(o, x0, eps):
+    """Central differences of E with respect to every particle coordinate."""
+    g = np.zeros_like(x0)
+    for p in range(len(x0)):
+        for a in range(3):
+            x = x0.copy()
+            x[p, a] += eps
+            ep = excess(o, x)
+            x[p, a] -= 2 * eps
+            em = excess(o, x)
+            g[p, a] = (ep - em) / (2 * eps)
+    return g
+
+
+def gauss_newton_mismatch(idimc, fast=False):
+    s = squeezed_scene(idimc)
+    o = pyoracle.Oracle(s, fast=fast)
+    x0 = o.parts.coord.copy()
+    e0 = excess(o, x0)                # bins and nvof at x0
+    d = o.shift_delta()
+    h = 1.0 / 8
+    g = fd_gradient(o, x0, 1e-7 * h)
+    pred = -e0 * g / np.sum(g * g)
+    ident = float(np.sum(d * g))      # sum_p delta_p . dE/dx_p
+    return e0, d, pred, ident
+
+
+@pytest.mark.parametrize("idimc", [(1, 1, 1), (2, 2, 2)])
+def test_shift_delta_is_the_gauss_newton_step_of_the_excess_norm(idimc):
+    e0, d, pred, ident = gauss_newton_mismatch(idimc)
+    assert e0 > 0 and np.max(np.abs(d)) > 0
+    # the step length: sum_p delta_p . dE/dx_p = -E
+    assert abs(ident + e0) <= 1e-6 * e0, (ident, e0)
+    # the step itself, particle by particle
+    assert np.max(np.abs(d - pred)) <= 1e-6 * np.max(np.abs(pred))
+
+
+@pytest.mark.parametrize("mutant", [1, 2])
+def test_planted_mistakes_in_the_correction_are_caught(mutant):
+    """The same check on oracle builds with a planted mistake: 1 drops the
+    factor 2 of the gradient (the step doubles), 2 takes vol0 for vol."""
+    e0, d, pred, ident = gauss_newton_mismatch((1, 1, 1), fast=("mutant", mutant))
+    err = np.max(np.abs(d - pred)) / np.max(np.abs(pred))
+    assert err > 1e-2, err
+    if mutant == 1:
+        assert abs(ident + 2 * e0) <= 1e-6 * e0
+
+
+@pytest.mark.parametrize("idimc", [(1, 1, 1), (2, 2, 2), (2, 1, 2)])
+def test_maksf_gradient_is_the_derivative_of_the_weights(idimc):
+    """dsf = d sf / dx by central differences inside the cells, for the tent
+    functions and the quadratic B-spline, boundary cells included."""
+    import ctypes as C
+    nelem = (6, 5, 7)
+    st = pyoracle.OrcState()
+    lo, hi = (0.0, -0.5, 1.0), (1.2, 0.5, 2.4)
+    for i in range(3):
+        st.xmin[i], st.xmax[i], st.nex[i], st.idimc[i] = lo[i], hi[i], nelem[i], idimc[i]
+    orc = pyoracle.load()
+    orc.orc_grid_init(C.byref(st))
+    rng = np.random.default_rng(5)
+    n = 3000
+    ijk = np.stack([rng.integers(0, nelem[a], n) for a in range(3)], 1)
+    cells = (ijk[:, 0] + ijk[:, 1] * nelem[0] + ijk[:, 2] * nelem[0] * nelem[1]).astype(np.int32)
+    h = np.array([st.dx[a] for a in range(3)])
+    frac = rng.uniform(0.01, 0.99, (n, 3))
+    x = np.array(lo) + (ijk + frac) * h
+    nen = int(np.prod(np.array(idimc) + 1))
+
+    def maksf(xx):
+        nc = np.zeros((n, nen), np.int32)
+        sf = np.zeros((n, nen))
+        dsf = np.zeros((n, nen, 3))
+        orc.orc_maksf_batch(C.byref(st), n, pyoracle.P(cells, C.c_int32), pyoracle.P(np.ascontiguousarray(xx)),
+                            pyoracle.P(nc, C.c_int32), pyoracle.P(sf), pyoracle.P(dsf))
+        return nc, sf, dsf
+
+    nc, sf, dsf = maksf(x)
+    for a in range(3):
+        eps = 1e-6 * h[a]
+        xp, xm = x.copy(), x.copy()
+        xp[:, a] += eps
+        xm[:, a] -= eps
+        ncp, sfp, _ = maksf(xp)
+        ncm, sfm, _ = maksf(xm)
+        assert np.array_equal(ncp, nc) and np.array_equal(ncm, nc)
+        fd = (sfp - sfm) / (2 * eps)
+        assert np.max(np.abs(fd - dsf[:, :, a])) <= 1e-7 / h[a], a
