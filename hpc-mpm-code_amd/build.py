@@ -21,6 +21,7 @@ COMMON = [
 UNITS = [
     ("sort.hip", []),
     ("bin.hip", []),
+    ("noalloc.hip", []),
     ("exact64.hip", ["-ffp-contract=off"]),
     ("fast32.hip", []),
     ("mlp.hip", []),

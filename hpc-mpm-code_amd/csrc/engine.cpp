@@ -50,6 +50,8 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
     return MPM_ERR_HIP;
 }
 
+static_assert(MPM_NOALLOC_MAX_POSITIONS == kNoAllocMaxPositions, "allocation-free binning limit");
+
 static int bit_length(uint32_t v) {
     int b = 0;
     while (v) {
@@ -1926,6 +1928,23 @@ int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, i
         cell_of[i] = kh[i] >= (uint32_t)g.nelem ? -1 : (int32_t)kh[i];
         order[i] = (int32_t)vh[i];
     }
+    return 0;
+} catch (...) {
+    return mpm::exception_status();
+}
+
+int mpm_bin_positions_noalloc(const mpm_grid_desc *grid, int64_t n, const double *xyg, int32_t *cell_of,
+                              int32_t *order, void *stream) try {
+    if (!grid || n < 0 || (n > 0 && (!xyg || !cell_of || !order)))
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_grid(grid, false);
+    if (rc) return rc;
+    if (n > MPM_NOALLOC_MAX_POSITIONS)
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "more positions than the allocation-free binning holds");
+    if (n == 0) return 0;
+    const GridGeom g = make_geom(grid);
+    const int e = noalloc_bin_positions(g, (int)n, xyg, cell_of, order, (hipStream_t)stream);
+    if (e) return hip_fail((hipError_t)e, "mpm_bin_positions_noalloc", __FILE__, __LINE__);
     return 0;
 } catch (...) {
     return mpm::exception_status();

@@ -28,6 +28,13 @@ void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
 void launch_keys_f64(hipStream_t s, const GridGeom &g, const double *x, size_t ld, int n, uint32_t *keys,
                      uint32_t *err);
 
+// Allocation-free binning of device AoS positions (noalloc.hip): all
+// intermediates live in a fixed file-scope device array sized for
+// kNoAllocMaxPositions positions (about 21 MB per GPU). Asynchronous on s.
+constexpr int kNoAllocMaxPositions = 1 << 20;
+int noalloc_bin_positions(const GridGeom &g, int n, const double *xyg, int32_t *cell_of, int32_t *order,
+                          hipStream_t s);
+
 // Cell lists produced by the binning: particles of cell c are
 // perm[cstart[c] .. cend[c]) (cstart == cend for an empty cell); active[] lists
 // the non-empty cells in ascending order, ctr->n_active of them; tiles[]

@@ -23,6 +23,7 @@ MPM_ERR_UNSUPPORTED = -6
 MPM_ERR_NO_DEVICE = -7
 MPM_ERR_INTERNAL = -8
 MPM_ERR_HOST_MEMORY = -9
+NOALLOC_MAX_POSITIONS = 1048576  # MPM_NOALLOC_MAX_POSITIONS
 
 SCHEME_PIC, SCHEME_FLIP, SCHEME_TPIC, SCHEME_APIC = 0, 1, 2, 3
 PREC_FP64_EXACT, PREC_FP32 = 0, 1
@@ -124,6 +125,8 @@ SIGNATURES = {
     "mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "mpm_maksf_batch": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _ip, _dp, _ip, _dp, _dp]),
     "mpm_bin_positions": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _dp, _ip, _ip]),
+    "mpm_bin_positions_noalloc": (C.c_int, [C.POINTER(GridDesc), C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]),
     "mpm_bandwidth_probe": (C.c_int, [C.c_int64, C.c_int32, _dp]),
     "mpm_set_shifting": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
     "mpm_download_nodal_volume": (C.c_int, [C.c_void_p, _dp]),

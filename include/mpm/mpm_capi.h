@@ -331,6 +331,23 @@ int mpm_maksf_batch(const mpm_grid_desc *grid, int64_t n, const int32_t *cell, c
  * particle indices sorted by (cell, index). */
 int mpm_bin_positions(const mpm_grid_desc *grid, int64_t n, const double *xyg, int32_t *cell_of, int32_t *order);
 
+/* Allocation-free variant of mpm_bin_positions; no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same stable cell-key sort, with every intermediate in a fixed device
+ * array of the library (about 21 MB per GPU, held until the process ends),
+ * and no hipMalloc, hipFree, memset or synchronisation.
+ * xyg: DEVICE pointer to [n][3] FP64 positions; cell_of, order: DEVICE
+ * pointers to [n] int32 outputs, same meaning as in mpm_bin_positions.
+ * stream: the hipStream_t to enqueue on (NULL = the null stream); the call
+ * returns once the work is enqueued. n <= MPM_NOALLOC_MAX_POSITIONS, else
+ * MPM_ERR_INVALID_ARGUMENT. All calls on one GPU share that GPU's array:
+ * calls that overlap on one GPU (other streams, other threads) race and must
+ * be serialised by the caller. */
+#define MPM_NOALLOC_MAX_POSITIONS 1048576
+int mpm_bin_positions_noalloc(const mpm_grid_desc *grid, int64_t n, const double *xyg, int32_t *cell_of,
+                              int32_t *order, void *stream);
+
 /* Measurement helper with no counterpart in the reference: copies nbytes
  * device-to-device iters times with a plain 16-byte-per-lane copy kernel and
  * returns the achieved bandwidth (bytes read + written per second / 1e9).
