@@ -1,4 +1,5 @@
-"""Builds the MI355X engine library libmpm_mi355x.so in-tree with hipcc.
+"""Builds the MI355X engine library libmpm_mi355x.so in-tree with hipcc, and
+the reference-signature host shim libmpm_refshim.so with g++.
 
 Every HIP translation unit is compiled for gfx950 only. exact64.hip is built
 with -ffp-contract=off (it reproduces the reference's FP64 arithmetic
@@ -38,6 +39,23 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+SHIM_OUT = os.path.join(HERE, "libmpm_refshim.so")
+SHIM_INC = os.path.join(os.path.dirname(HERE), "include", "mpm_ref")
+
+
+def build_shim(verbose=False):
+    """The reference-signature host shim (CPU FP64, include/mpm_ref/): plain
+    C++17 with -ffp-contract=off, as the reference's own x86-64 build."""
+    tmp = SHIM_OUT + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-I" + SHIM_INC,
+           "-shared", "-o", tmp, os.path.join(HERE, "shim", "ref_shim.cpp")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, SHIM_OUT)
+    return SHIM_OUT
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
@@ -52,6 +70,7 @@ def build(verbose=False):
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
+    build_shim(verbose)
     return OUT
 
 
