@@ -25,6 +25,7 @@ UNITS = [
     ("noalloc.hip", []),
     ("exact64.hip", ["-ffp-contract=off"]),
     ("fast32.hip", []),
+    ("stencil32.hip", []),
     ("mlp.hip", []),
     ("surrogate.cpp", ["-x", "hip"]),
     ("engine.cpp", ["-x", "hip"]),

@@ -132,9 +132,11 @@ struct TileSet {
     }
 };
 
-__device__ __forceinline__ TileSet tile_set(int ix, int iy, int iz) {
-    return TileSet{ix / kNodeTileX, (ix + 1) / kNodeTileX, iy / kNodeTileY,
-                   (iy + 1) / kNodeTileY, iz / kNodeTileZ, (iz + 1) / kNodeTileZ};
+// (a cell's stencil spans nodes ix .. ix + idimc; with idimc <= 2 and tiles of
+// at least 2 nodes per axis that is at most two tiles per axis)
+__device__ __forceinline__ TileSet tile_set(const GridGeom &g, int ix, int iy, int iz) {
+    return TileSet{ix / kNodeTileX, (ix + g.idimc[0]) / kNodeTileX, iy / kNodeTileY,
+                   (iy + g.idimc[1]) / kNodeTileY, iz / kNodeTileZ, (iz + g.idimc[2]) / kNodeTileZ};
 }
 
 __device__ __forceinline__ void mark_tiles(const TileGeom &tg, uint32_t *tiles, const TileSet &t) {
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
             active[(uint32_t)(pre >> 32)] = (uint32_t)(c0 + q);
             astart[(uint32_t)(pre >> 32)] = st[q];
             bits |= 1u << q;
-            const TileSet t = tile_set(ix, iy, iz);
+            const TileSet t = tile_set(g, ix, iy, iz);
             if (!(t == last)) {
                 mark_tiles(tg, tiles, t);
                 last = t;

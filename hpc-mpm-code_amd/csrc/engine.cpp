@@ -79,6 +79,8 @@ struct mpm_engine {
     GridGeom g{};
     mpm_step_params prm{};
     MaterialTable mt{};
+    std::vector<MaterialDev> mat_host;  // host copy of the device material table
+    int mat_dt = 0;                     // FP32: per-material time steps (mpm_set_material_dt)
     int prec = MPM_PREC_FP64_EXACT;
     hipStream_t stream = nullptr;
     int device = 0;
@@ -102,6 +104,7 @@ struct mpm_engine {
     int cur = 0;
     NodesF32 N32{};
     float *partials = nullptr;
+    bool stencil = false;  // FP32: the stencil kernels (quadratic B-spline on some axis, or TPIC / APIC)
     NodeTables T{};
     float m_small = 0.f;
     int lanes_per_cell = 1;  // from the particles per non-empty cell at upload
@@ -171,6 +174,7 @@ struct mpm_engine {
         int64_t stat_base[2] = {0, 0};      // counts of regions folded in before a reallocation
         int64_t qcap = 0;                   // rows: regions * kSurRegion, one region per update block
         uint32_t thr = 0, step_tag = 0;
+        int kind = kSurJ2;  // the model the network stands in for (from the materials)
         bool sorted = false;
         float loss = 0.f, tol = 1e-4f;
         void free_log() {
@@ -285,6 +289,7 @@ StepConsts consts(const mpm_engine *e, double facl) {
     c.lanes_per_cell = e->lanes_per_cell;
     c.cells_prefetch = e->cells_prefetch;
     c.cells_grid = e->cells_grid;
+    c.mat_dt = e->mat_dt;
     return c;
 }
 
@@ -341,6 +346,9 @@ int do_p2g(mpm_engine *e, double facl, int diag, hipEvent_t mid = nullptr) {
     if (e->prec == MPM_PREC_FP64_EXACT) {
         launch_f64_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.scheme, e->prm.bb[0] * facl,
                        e->prm.bb[1] * facl, e->prm.bb[2] * facl, e->prm.mtol, facl);
+    } else if (e->stencil) {
+        launch_s32_p2g(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, consts(e, facl),
+                       e->partials, diag, mid);
     } else {
         launch_f32_p2g(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, consts(e, facl),
                        e->partials, diag, mid);
@@ -378,10 +386,23 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         launch_f64_g2p(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->mt, e->prm.scheme, e->prm.dt,
                        e->prm.mtol, facl, e->d_err, e->shifting ? &sh : nullptr, e->fbar ? e->d_njbar : nullptr);
         e->keys_valid = false;
+    } else if (e->stencil) {
+        StepConsts c = consts(e, facl);
+        launch_s32_g2p_vel_musl(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, c,
+                                e->partials, diag, mid2);
+        if (mid) MPM_HIP_CHECK(hipEventRecord(mid, e->stream));
+        if (!e->fused_count) e->clear_counts();
+        BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->counts_ready ? 1 : 0, e->bb.bsum, e->bb.tiles,
+                   e->tg.ntiles, e->bb.ctr};
+        launch_s32_g2p_update(e->stream, e->g, e->P32[e->cur], e->N32, e->T, e->mt, c, e->d_err, bf);
+        if (bf.cnt && !bf.delta) launch_bin_reduce(e->stream, e->g, e->bb);
+        e->jv_derived = true;
+        e->counts_ready = e->fused_count;
+        e->keys_valid = true;
     } else {
         StepConsts c = consts(e, facl);
         launch_f32_g2p_vel_musl(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, c,
-                                e->partials, diag, mid2);
+                                e->partials, diag, mid2, &e->mt);
         if (mid) MPM_HIP_CHECK(hipEventRecord(mid, e->stream));
         SurrogateArgs sa{};
         auto &su = e->sur;
@@ -389,6 +410,7 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
             // the sample key step_tag * n + id must fit 32 bits
             if ((uint64_t)(su.step_tag + 1) * (uint64_t)e->n <= 0xFFFFFFFFull) {
                 sa.mode = 1;
+                sa.kind = su.kind;
                 sa.log_in = su.log_in;
                 sa.log_tgt = su.log_tgt;
                 sa.log_key = su.log_key;
@@ -399,6 +421,7 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
             }
         } else if (su.enabled && su.have_net && su.queue && su.qcap >= sur_queue_rows(e->n)) {
             sa.mode = 2;
+            sa.kind = su.kind;
             sa.tol = su.tol;
             sa.net = su.net.view();
             sa.net_scaling = su.net.scaling;
@@ -451,6 +474,8 @@ int alloc_f32_buffer(mpm_engine *e, int b) {
     if (e->has_trac && (rc = e->dalloc(&P.trac, 3 * ld, true))) return rc;
     if (e->has_matid && (rc = e->dalloc(&P.matid, ld, true))) return rc;
     if (e->has_epsp && (rc = e->dalloc(&P.eps_p, ld, true))) return rc;
+    if (e->prm.scheme >= MPM_SCHEME_TPIC && (rc = e->dalloc(&P.C, 9 * ld, true))) return rc;
+    if (e->prm.scheme == MPM_SCHEME_APIC && (rc = e->dalloc(&P.Dinv, 9 * ld, true))) return rc;
     P.ld = ld;
     P.n = e->n;
     return 0;
@@ -525,10 +550,8 @@ int validate_grid(const mpm_grid_desc *g, bool allow_quadratic) {
         if (g->nelem[i] <= 0) return set_error(MPM_ERR_INVALID_ARGUMENT, "nelem must be positive");
         if (!(g->xmax[i] > g->xmin[i])) return set_error(MPM_ERR_INVALID_ARGUMENT, "xmax must exceed xmin");
         if (g->idimc[i] != 1 && !(allow_quadratic && g->idimc[i] == 2))
-            return set_error(MPM_ERR_UNSUPPORTED, allow_quadratic
-                                                      ? "shape-function order must be 1 or 2"
-                                                      : "the FP32 engine takes the linear shape function only "
-                                                        "(idimc = 1); the FP64-exact engine also takes 2");
+            return set_error(MPM_ERR_UNSUPPORTED, allow_quadratic ? "shape-function order must be 1 or 2"
+                                                                  : "shape-function order must be 1 here");
     }
     double ne = (double)g->nelem[0] * g->nelem[1] * g->nelem[2];
     double nn = (double)(g->nelem[0] + g->idimc[0]) * (g->nelem[1] + g->idimc[1]) * (g->nelem[2] + g->idimc[2]);
@@ -565,13 +588,18 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         return set_error(MPM_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (nmat < 1 || nmat > kMaxMat) return set_error(MPM_ERR_INVALID_ARGUMENT, "nmat must be in 1..1024");
-    int rc = validate_grid(grid, params && params->precision == MPM_PREC_FP64_EXACT);
+    int rc = validate_grid(grid, true);
     if (rc) return rc;
     if (!(params->dt > 0.0)) return set_error(MPM_ERR_INVALID_ARGUMENT, "dt must be positive");
     if (params->scheme < MPM_SCHEME_PIC || params->scheme > MPM_SCHEME_APIC)
         return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown transfer scheme");
-    if (params->scheme >= MPM_SCHEME_TPIC && params->precision != MPM_PREC_FP64_EXACT)
-        return set_error(MPM_ERR_UNSUPPORTED, "TPIC/APIC transfers run in the FP64-exact engine only");
+    {
+        const GridGeom g0 = make_geom(grid);
+        if (params->precision == MPM_PREC_FP32 && !s32_supported(g0))
+            return set_error(MPM_ERR_UNSUPPORTED,
+                             "the FP32 engine takes idimc (1,1,1), (2,2,2) or (2,1,2); the FP64-exact engine any of 1 "
+                             "and 2 per axis");
+    }
     if (params->precision != MPM_PREC_FP64_EXACT && params->precision != MPM_PREC_FP32)
         return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown precision");
     if (params->nlstep < 1) return set_error(MPM_ERR_INVALID_ARGUMENT, "nlstep must be >= 1");
@@ -587,12 +615,14 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     e->g = make_geom(grid);
     e->prm = *params;
     e->prec = params->precision;
+    e->stencil = e->prec == MPM_PREC_FP32 &&
+                 (grid->idimc[0] != 1 || grid->idimc[1] != 1 || grid->idimc[2] != 1 || params->scheme >= MPM_SCHEME_TPIC);
     e->relayout_every = params->precision == MPM_PREC_FP32 ? std::max(0, (int)params->relayout_every) : 0;
     e->mt.nmat = nmat;
     std::vector<MaterialDev> mh(nmat);
     for (int i = 0; i < nmat; i++) {
         mh[i].model = mats[i].model;
-        mh[i].pad = 0;
+        mh[i].dt = 0.f;
         for (int k = 0; k < 8; k++) mh[i].prop[k] = mats[i].prop[k];
     }
     hipError_t he = hipGetDevice(&e->device);
@@ -605,6 +635,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         }
         he = hipMemcpy(dm, mh.data(), sizeof(MaterialDev) * nmat, hipMemcpyHostToDevice);
         e->mt.m = dm;
+        e->mat_host = mh;
     }
     if (he == hipSuccess) he = hipGetDevice(&e->device);
     if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
@@ -660,7 +691,9 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
             (rc = e->dalloc(&N.nf, 3 * nodec)) || (rc = e->dalloc(&N.nv, 3 * nodec)) ||
             (rc = e->dalloc(&N.na, 3 * nodec)) || (rc = e->dalloc(&N.nvm, 3 * nodec)) ||
             (rc = e->dalloc(&N.nvp, 3 * nodec)) ||
-            (rc = e->dalloc(&e->partials, (size_t)kPartialsPerCell * nelem))) {
+            (rc = e->dalloc(&e->partials, (size_t)(e->stencil ? std::max(kPartialsPerCell, s32_partials_per_cell(g))
+                                                                  : kPartialsPerCell) *
+                                              nelem))) {
             delete e;
             return rc;
         }
@@ -724,6 +757,9 @@ static int sur_require(mpm_engine *e) {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
     if (e->prec != MPM_PREC_FP32)
         return set_error(MPM_ERR_UNSUPPORTED, "the constitutive surrogate runs in the FP32 engine only");
+    if (e->stencil)
+        return set_error(MPM_ERR_UNSUPPORTED,
+                         "the constitutive surrogate runs with the tent functions and PIC / FLIP only");
     if (!e->sur.ctr) {
         MPM_HIP_CHECK(hipMalloc((void **)&e->sur.ctr, 4 * sizeof(unsigned long long)));
         MPM_HIP_CHECK(hipMemsetAsync(e->sur.ctr, 0, 4 * sizeof(unsigned long long), e->stream));
@@ -732,6 +768,15 @@ static int sur_require(mpm_engine *e) {
         int rc = e->sur.net.init();
         if (rc) return rc;
     }
+    // J2 plasticity if any material has it, else the Tait pressure closure
+    bool j2 = false, tait = false;
+    for (const MaterialDev &m : e->mat_host) {
+        j2 = j2 || m.model == MPM_MODEL_J2;
+        tait = tait || m.model == MPM_MODEL_TAIT;
+    }
+    if (!j2 && !tait)
+        return set_error(MPM_ERR_UNSUPPORTED, "the surrogate stands in for J2 plasticity or the Tait pressure");
+    e->sur.kind = j2 ? kSurJ2 : kSurTait;
     return 0;
 }
 
@@ -889,6 +934,7 @@ int mpm_surrogate_stats_get(mpm_engine *e, mpm_surrogate_stats *out) try {
     out->fallbacks = (int64_t)c[1] + t[1];
     out->loss = e->sur.loss;
     out->tol = e->sur.tol;
+    out->offered = (int64_t)c[2];
     return 0;
 } catch (...) {
     return mpm::exception_status();
@@ -1079,9 +1125,42 @@ int mpm_upload_fbar(mpm_engine *e, const double *def_grad_bar, const double *det
 }
 
 // TPIC velocity gradient / APIC B matrix and APIC inverse D, host [n][3][3]
+// FP32 engine: slot arrays in the engine's current particle order, host
+// arrays in the original particle order (slot s holds particle pid[s]).
+static int affine_copy_f32(mpm_engine *e, double *cmat, double *dinv, bool to_device) {
+    const PartF32 &P = e->P32[e->cur];
+    if (!P.C) return set_error(MPM_ERR_INVALID_ARGUMENT, "the engine's transfer scheme is not TPIC or APIC");
+    if (dinv && !P.Dinv) return set_error(MPM_ERR_INVALID_ARGUMENT, "inverse D exists under APIC only");
+    const size_t ld = e->ld;
+    const int n = e->n;
+    int rc = 0;
+    std::vector<int> pidv;
+    if ((rc = d2h(e, pidv, (const int *)e->pid[e->cur], ld))) return rc;
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    double *host[2] = {cmat, dinv};
+    float *dev[2] = {P.C, P.Dinv};
+    for (int b = 0; b < 2; b++) {
+        if (!host[b]) continue;
+        std::vector<float> soa(9 * ld, 0.f);
+        if (to_device) {
+            for (int s = 0; s < n; s++)
+                for (int q = 0; q < 9; q++) soa[q * ld + s] = (float)host[b][9 * (size_t)pidv[s] + q];
+            if ((rc = h2d(e, dev[b], soa))) return rc;
+            MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        } else {
+            if ((rc = d2h(e, soa, (const float *)dev[b], 9 * ld))) return rc;
+            MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+            for (int s = 0; s < n; s++)
+                for (int q = 0; q < 9; q++) host[b][9 * (size_t)pidv[s] + q] = (double)soa[q * ld + s];
+        }
+    }
+    return 0;
+}
+
 static int affine_copy(mpm_engine *e, double *cmat, double *dinv, bool to_device) {
     int rc = require_particles(e);
     if (rc) return rc;
+    if (e->prec == MPM_PREC_FP32) return affine_copy_f32(e, cmat, dinv, to_device);
     const PartF64 &P = e->P64;
     if (e->prec != MPM_PREC_FP64_EXACT || !P.C)
         return set_error(MPM_ERR_INVALID_ARGUMENT, "the engine's transfer scheme is not TPIC or APIC");
@@ -1299,6 +1378,12 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) try {
             (rc = h2d(e, P.eps_p, ep)) || (rc = h2d(e, e->pid[0], pidv)))
             return rc;
         e->fp32_outside_at_upload = outside;
+        // TPIC/APIC matrices start at zero (solid_material_point.cpp:36-43)
+        if (P.C || P.Dinv) {
+            std::vector<float> zero(9 * ld, 0.f);
+            if ((P.C && (rc = h2d(e, P.C, zero))) || (P.Dinv && (rc = h2d(e, P.Dinv, zero)))) return rc;
+            MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+        }
         if (e->has_trac) {
             tr.assign(3 * ld, 0.f);
             for (int i = 0; i < n; i++)
@@ -1828,6 +1913,27 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) try {
         return 0;
     default: return set_error(MPM_ERR_INVALID_ARGUMENT, "unknown option");
     }
+} catch (...) {
+    return mpm::exception_status();
+}
+
+int mpm_set_material_dt(mpm_engine *e, const double *dt) try {
+    if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
+    if (dt && (e->prec != MPM_PREC_FP32 || e->stencil))
+        return set_error(MPM_ERR_UNSUPPORTED,
+                         "per-material time steps run in the FP32 engine with the tent functions and PIC / FLIP");
+    std::vector<MaterialDev> mh = e->mat_host;
+    for (size_t i = 0; i < mh.size(); i++) {
+        if (dt && !(dt[i] > 0.0 && dt[i] < 1e30)) return set_error(MPM_ERR_INVALID_ARGUMENT, "time steps must be positive");
+        mh[i].dt = dt ? (float)dt[i] : 0.f;
+    }
+    e->drop_graph();
+    MPM_HIP_CHECK(hipMemcpyAsync((void *)e->mt.m, mh.data(), sizeof(MaterialDev) * mh.size(), hipMemcpyHostToDevice,
+                                 e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->mat_host = mh;
+    e->mat_dt = dt ? 1 : 0;
+    return 0;
 } catch (...) {
     return mpm::exception_status();
 }

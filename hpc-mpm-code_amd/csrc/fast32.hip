@@ -18,8 +18,7 @@
 // applies the grid update (velocity, acceleration, BCs) in the same pass.
 // The MUSL remap uses the same structure, with the G2P velocity of every
 // particle computed from the 8 node values of its cell held in registers.
-#include "mpm_common.h"
-#include "mpm_kernels.h"
+#include "fast32_device.h"
 
 #include <algorithm>
 
@@ -27,153 +26,6 @@ namespace mpm {
 
 namespace {
 
-constexpr int kSeg = 64;      // cells per row segment = threads per workgroup
-constexpr int kChunk = 256;   // particles staged per LDS pass
-
-// q = a / n, rem = a % n for a < 2^24 via the float reciprocal, corrected to exact
-__device__ __forceinline__ int fdivmod(int a, int n, float inv_n, int &rem) {
-    int q = (int)((float)a * inv_n);
-    int r = a - q * n;
-    if (r < 0) {
-        q--;
-        r += n;
-    } else if (r >= n) {
-        q++;
-        r -= n;
-    }
-    rem = r;
-    return q;
-}
-
-// Neumaier (improved Kahan) compensated sum.
-__device__ __forceinline__ void neu_add(float &s, float &c, float x) {
-    float t = s + x;
-    if (fabsf(s) >= fabsf(x))
-        c += (s - t) + x;
-    else
-        c += (x - t) + s;
-    s = t;
-}
-
-// A group is 64 consecutive entries of the active-cell list (one per thread).
-// Because the list is in ascending cell order and skips only empty cells, the
-// particles of a group are one contiguous range [S, E) of the sorted order.
-struct CellInfo {
-    int ix, iy, iz, c;
-    uint32_t cs, ce;
-};
-
-// A group is G = 64 / L consecutive entries of the active-cell list; L lanes
-// work on each cell (lane = cell-in-group * L + sub). Because the list is in
-// ascending cell order and skips only empty cells, the particles of a group
-// are one contiguous range [S, E) of the sorted order.
-template <int L>
-__device__ __forceinline__ void group_setup(const GridGeom &g, const CellLists &L_, uint32_t n_active, int grp,
-                                            CellInfo &ci, uint32_t &S, uint32_t &E) {
-    constexpr int G = kSeg / L;
-    const int t = threadIdx.x;
-    const uint32_t a0 = (uint32_t)grp * G;
-    const uint32_t a = a0 + t / L;
-    const uint32_t a_end = min(a0 + (uint32_t)G, n_active);
-    // uniform: scalar loads, which do not hold up the vector-memory counter
-    S = ((const __attribute__((address_space(4))) uint32_t *)L_.astart)[a0];
-    E = ((const __attribute__((address_space(4))) uint32_t *)L_.astart)[a_end];
-    ci.c = -1;
-    ci.cs = ci.ce = 0u;
-    ci.ix = ci.iy = ci.iz = 0;
-    if (a < n_active) {
-        ci.c = (int)L_.active[a];
-        ci.cs = L_.astart[a];
-        ci.ce = L_.astart[a + 1];
-        const int nexy = g.nex[0] * g.nex[1];
-        ci.iz = ci.c / nexy;
-        const int r = ci.c - ci.iz * nexy;
-        ci.iy = r / g.nex[0];
-        ci.ix = r - ci.iy * g.nex[0];
-    }
-}
-
-// first position >= lo of the arithmetic sequence cs + sub + k L
-template <int L>
-__device__ __forceinline__ uint32_t first_pos(uint32_t cs, uint32_t lo, int sub) {
-    uint32_t f = cs + (uint32_t)sub;
-    if (f < lo) f += ((lo - f + L - 1) / L) * L;
-    return f;
-}
-
-// Sums NV values over the L lanes of a cell and leaves lane sub with the
-// totals of entries k L + sub at v[k], k < NV / L. Each level exchanges half
-// of the remaining entries with the lane O apart; a total is
-// ((a0 + a1) + (a2 + a3)) ... in lane order, as a xor butterfly would give.
-template <int O, int L, int N>
-struct ReduceScatter {
-    static __device__ __forceinline__ void run(float *v, int sub) {
-        const bool hi = (sub & O) != 0;
-#pragma unroll
-        for (int k = 0; k < N / 2; k++) {
-            const float keep = hi ? v[2 * k + 1] : v[2 * k];
-            const float send = hi ? v[2 * k] : v[2 * k + 1];
-            v[k] = keep + __shfl_xor(send, O, 64);
-        }
-        ReduceScatter<O * 2, L, N / 2>::run(v, sub);
-    }
-};
-template <int L, int N>
-struct ReduceScatter<L, L, N> {
-    static __device__ __forceinline__ void run(float *, int) {}
-};
-
-// ------------------------------------------------------------------ P2G
-// Particle fields are gathered (through the sorted order) straight from HBM
-// into LDS with gfx950's global_load_lds_dword: lane l of a wave writes the
-// dword for chunk particle 64*r + l at stage block r. Nothing is held in
-// VGPRs while the loads are in flight, so a wave keeps all of a chunk's loads
-// outstanding. Each field's 64-particle blocks are S = 64 + L dwords apart
-// for L lanes per cell: lane (cell c, sub s) of a 16-point-per-cell group
-// reads particle 16 c + s + L k, which lands in block c / (64 / 16) at
-// bank (L (c / 4) + 16 (c % 4) + s + L k) mod 64 -- a different bank for
-// every lane of the wave (likewise for 8 points per cell).
-constexpr int kBlk = kChunk / 64;
-template <int S>
-__device__ __forceinline__ int st_idx(int f, int i) { return (f * kBlk + (i >> 6)) * S + (i & 63); }
-
-// cache policy of the LDS DMA: 0, or kNT for a stream read once per step
-// (non-temporal, the gfx9 SLC / gfx940+ NT bit), which keeps L2 for the data
-// the next kernels re-read
-constexpr int kNT = 2;
-template <int S, int CPOL = 0>
-__device__ __forceinline__ void lds_gather(const void *field_base, uint32_t slot, float *stage, int f, int r) {
-    __builtin_amdgcn_global_load_lds((const uint32_t *)field_base + slot, stage + st_idx<S>(f, 64 * r), 4, 0, CPOL);
-}
-
-// wait for every outstanding vector-memory load (including LDS DMA) of this wave
-__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
-
-// wait until at most the kBlk most recent vector-memory loads of this wave
-// are in flight (vmcnt(4); loads complete in issue order), leaving the slot
-// prefetch of the next chunk outstanding
-static_assert(kBlk == 4, "the wait below encodes vmcnt(4)");
-__device__ __forceinline__ void wait_vm_keep_slots() { __builtin_amdgcn_s_waitcnt(0x0F74); }
-
-// A cells-kernel block is one wave: LDS written by its own DMA is visible
-// after the vmcnt wait, and its own LDS accesses execute in order, so a
-// compiler barrier (no hardware barrier, whose fence would wait for every
-// outstanding load) orders the stage buffer's reuse.
-static_assert(kSeg == 64, "cells kernels assume one wave per block");
-__device__ __forceinline__ void wave_fence() { asm volatile("" ::: "memory"); }
-
-// slot indices of the chunk starting at sorted position base (nch particles)
-__device__ __forceinline__ void load_slots(const CellLists &L, uint32_t base, int nch, uint32_t slot[kBlk]) {
-#pragma unroll
-    for (int r = 0; r < kBlk; r++) slot[r] = (64 * r + (int)threadIdx.x < nch) ? L.perm[base + 64 * r + threadIdx.x] : 0u;
-}
-
-// the same for every lane, positions clamped into [base, end): exactly kBlk
-// load instructions, whatever the chunk length
-__device__ __forceinline__ void load_slots_all(const CellLists &L, uint32_t base, uint32_t end, uint32_t slot[kBlk]) {
-#pragma unroll
-    for (int r = 0; r < kBlk; r++) slot[r] = L.perm[min(base + 64u * r + threadIdx.x, end - 1u)];
-}
 
 // PF: software prefetch. The next group's cell ranges are requested at the
 // start of a group and the next chunk's slot indices while the current
@@ -380,37 +232,6 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
     }  // groups
 }
 
-__device__ __forceinline__ void node_ijk(const GridGeom &g, int n, int &i, int &j, int &k) {
-    i = n % g.nxc[0];
-    j = (n / g.nxc[0]) % g.nxc[1];
-    k = n / (g.nxc[0] * g.nxc[1]);
-}
-
-__device__ __forceinline__ float safe_div(float p, float m, float m_small) {
-    if (m >= m_small) return p / m;
-    return (float)((double)p / (double)m);
-}
-
-// Node of this thread in the node tile of this block; false when the tile has
-// no non-empty cell next to it (all threads of the block agree) or the node
-// lies outside the grid.
-__device__ __forceinline__ bool tile_node(const GridGeom &g, const CellLists &L, const TileGeom &tg, bool all_tiles,
-                                          int &n, int &i, int &j, int &k) {
-    const int tile = blockIdx.x;
-    if (!all_tiles && L.tiles[tile] == 0u) return false;
-    const int tx = tile % tg.ntx;
-    const int ty = (tile / tg.ntx) % tg.nty;
-    const int tz = tile / (tg.ntx * tg.nty);
-    const int l = threadIdx.x;
-    i = tx * kNodeTileX + (l % kNodeTileX);
-    j = ty * kNodeTileY + (l / kNodeTileX) % kNodeTileY;
-    k = tz * kNodeTileZ + l / (kNodeTileX * kNodeTileY);
-    if (i >= g.nxc[0] || j >= g.nxc[1] || k >= g.nxc[2]) return false;
-    n = i + g.nxc[0] * (j + g.nxc[1] * k);
-    return true;
-}
-
-__device__ __forceinline__ bool cell_nonempty(const CellLists &L, int c) { return (L.occ[c >> 5] >> (c & 31)) & 1u; }
 
 // The <= 8 cells around node (i, j, k) in ascending cell order: cell index
 // (clamped into the grid when outside) and whether it holds particles. All
@@ -534,12 +355,15 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
 }
 
 // ------------------------------------------------------ G2P velocity + MUSL
-template <int LPC, bool FLIP>
+// MDT: per-material time steps (each particle takes its material's dt); the
+// PIC node velocity v + dt a is then formed per particle from v and a.
+template <int LPC, bool FLIP, bool MDT>
 __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T, NodesF32 N,
-                                                        StepConsts sc, float *__restrict__ part) {
+                                                        StepConsts sc, float *__restrict__ part, MaterialTable mt) {
     constexpr int G = kSeg / LPC;
-    // staged fields: 0-2 r, 3 m, [4-6 old v for FLIP]; results: G2P velocity per particle
-    constexpr int NF = 4 + (FLIP ? 3 : 0);
+    // staged fields: 0-2 r, 3 m, [4-6 old v for FLIP], [matid for MDT]; results: G2P velocity per particle
+    constexpr int F_MAT = 4 + (FLIP ? 3 : 0);
+    constexpr int NF = F_MAT + (MDT ? 1 : 0);
     constexpr int SS = 64 + LPC;
     __shared__ float stage[NF * kBlk * SS];
     __shared__ float vout[3 * kBlk * SS];
@@ -559,7 +383,9 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
 
         // node values of this cell: PIC uses v + dt a (var_trans_solid_explicit.cpp:194-200),
         // FLIP uses a
-        float nvl[8][3];
+        // (MDT PIC: v here, a in nal)
+        constexpr bool TWO = MDT && !FLIP;
+        float nvl[8][3], nal[TWO ? 8 : 1][3];
         if (ne) {
             const int n0 = si.ix + g.nxc[0] * si.iy + g.nxc[0] * g.nxc[1] * si.iz;
 #pragma unroll
@@ -572,14 +398,18 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                         const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
-                            nvl[q][c] = FLIP ? N.na[c * nodec + nid] : N.nvp[c * nodec + nid];
+                            nvl[q][c] = FLIP ? N.na[c * nodec + nid] : (MDT ? N.nv : N.nvp)[c * nodec + nid];
+                            if constexpr (TWO) nal[q][c] = N.na[c * nodec + nid];
                         }
                     }
         } else {
 #pragma unroll
             for (int q = 0; q < 8; q++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) nvl[q][c] = 0.f;
+                for (int c = 0; c < 3; c++) {
+                    nvl[q][c] = 0.f;
+                    if constexpr (TWO) nal[q][c] = 0.f;
+                }
         }
 
         float acc[8][3];
@@ -603,6 +433,7 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                         if (FLIP) lds_gather<SS>(P.v + a * ld, sl, stage, 4 + a, r);
                     }
                     lds_gather<SS>(P.m, sl, stage, 3, r);
+                    if (MDT && P.matid) lds_gather<SS>(P.matid, sl, stage, F_MAT, r);
                 }
             }
             wait_vm_all();
@@ -616,7 +447,7 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                 const float m = stage[st_idx<SS>(3, i)];
                 const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
                 float Nw[8];
-                float vp[3] = {0.f, 0.f, 0.f};
+                float vp[3] = {0.f, 0.f, 0.f}, ap[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int k = 0; k < 2; k++)
 #pragma unroll
@@ -626,12 +457,21 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
                             const int q = ii + 2 * j + 4 * k;
                             Nw[q] = wx[ii] * (wy[j] * wz[k]);
 #pragma unroll
-                            for (int c = 0; c < 3; c++) vp[c] += Nw[q] * nvl[q][c];
+                            for (int c = 0; c < 3; c++) {
+                                vp[c] += Nw[q] * nvl[q][c];
+                                if constexpr (TWO) ap[c] += Nw[q] * nal[TWO ? q : 0][c];
+                            }
                         }
+                float dtp = sc.dt;
+                // (an engine uploaded without material ids has material 0 only)
+                if constexpr (MDT) dtp = mt.m[P.matid ? __float_as_int(stage[st_idx<SS>(F_MAT, i)]) : 0].dt;
                 if (FLIP) {
-                    vp[0] = stage[st_idx<SS>(4, i)] + sc.dt * vp[0];
-                    vp[1] = stage[st_idx<SS>(5, i)] + sc.dt * vp[1];
-                    vp[2] = stage[st_idx<SS>(6, i)] + sc.dt * vp[2];
+                    vp[0] = stage[st_idx<SS>(4, i)] + dtp * vp[0];
+                    vp[1] = stage[st_idx<SS>(5, i)] + dtp * vp[1];
+                    vp[2] = stage[st_idx<SS>(6, i)] + dtp * vp[2];
+                } else if (TWO) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) vp[c] += dtp * ap[c];
                 }
                 vout[st_idx<SS>(0, i)] = vp[0];
                 vout[st_idx<SS>(1, i)] = vp[1];
@@ -725,270 +565,6 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, T
     }
 }
 
-// ------------------------------------------------------ particle update
-// Stress update in FP32 from H = F - I and the increment D = dF - I
-// (semantics of ConstitutiveModel::UpdateStress, constitutive_model.cpp:287-305,
-// rewritten without cancellation against the identity).
-// J2 plasticity with linear isotropic hardening, small-increment form:
-// elastic predictor sigma_tr = sigma_n + C:de, radial return onto
-// q = sy + H eps_p (Simo & Hughes 1998, Box 3.2).
-struct J2Trial {
-    float st[6];          // trial stress
-    float pm, d0, d1, d2;  // trial pressure and normal deviators
-    float q, f;           // von Mises stress and yield function of the trial state
-    float mu, lmd, E, nu, sy, Hh, ep;
-};
-
-__device__ __forceinline__ void j2_trial(const double *p, const float D[9], const float s[6], float ep, J2Trial &t) {
-    const float E = (float)p[0], nu = (float)p[1];
-    t.E = E;
-    t.nu = nu;
-    t.sy = (float)p[2];
-    t.Hh = (float)p[3];
-    t.ep = ep;
-    t.lmd = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
-    t.mu = E / (2.f * (1.f + nu));
-    const float lmd = t.lmd, mu = t.mu;
-    float de[6] = {D[0], D[4], D[8], D[5] + D[7], D[6] + D[2], D[1] + D[3]};
-    float tr = de[0] + de[1] + de[2];
-    t.st[0] = s[0] + lmd * tr + 2.f * mu * de[0];
-    t.st[1] = s[1] + lmd * tr + 2.f * mu * de[1];
-    t.st[2] = s[2] + lmd * tr + 2.f * mu * de[2];
-    t.st[3] = s[3] + mu * de[3];
-    t.st[4] = s[4] + mu * de[4];
-    t.st[5] = s[5] + mu * de[5];
-    t.pm = (t.st[0] + t.st[1] + t.st[2]) * (1.f / 3.f);
-    t.d0 = t.st[0] - t.pm;
-    t.d1 = t.st[1] - t.pm;
-    t.d2 = t.st[2] - t.pm;
-    float j2 = 0.5f * (t.d0 * t.d0 + t.d1 * t.d1 + t.d2 * t.d2) + t.st[3] * t.st[3] + t.st[4] * t.st[4] +
-               t.st[5] * t.st[5];
-    t.q = sqrtf(3.f * j2);
-    t.f = t.q - (t.sy + t.Hh * ep);
-}
-
-// exact return (or the elastic trial state); eps_p receives the new value
-__device__ __forceinline__ void j2_return(const J2Trial &t, float s[6], float *eps_p) {
-    if (t.f <= 0.f) {
-        for (int c = 0; c < 6; c++) s[c] = t.st[c];
-    } else {
-        float dg = t.f / (3.f * t.mu + t.Hh);
-        float scl = 1.f - 3.f * t.mu * dg / t.q;
-        s[0] = t.pm + t.d0 * scl;
-        s[1] = t.pm + t.d1 * scl;
-        s[2] = t.pm + t.d2 * scl;
-        s[3] = t.st[3] * scl;
-        s[4] = t.st[4] * scl;
-        s[5] = t.st[5] * scl;
-        if (eps_p) *eps_p = t.ep + dg;
-    }
-}
-
-__device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9], const float D[9], double J,
-                                         float s[6], float *eps_p, float inv_dt) {
-    const double *p = md.prop;
-    switch (md.model) {
-    case -1:
-        for (int q = 0; q < 6; q++) s[q] = 0.f;
-        break;
-    case 0: {  // hypoelastic increment, dF - I = D
-        float E = (float)p[0], nu = (float)p[1];
-        float C0 = (E * (1.f - nu)) / ((1.f - 2.f * nu) * (1.f + nu));
-        float C1 = (E * nu) / ((1.f - 2.f * nu) * (1.f + nu));
-        float C2 = E / (2.f * (1.f + nu));
-        float e0 = D[0], e1 = D[4], e2 = D[8];
-        s[0] += C0 * e0 + C1 * (e1 + e2);
-        s[1] += C0 * e1 + C1 * (e0 + e2);
-        s[2] += C0 * e2 + C1 * (e0 + e1);
-        s[3] += C2 * (D[5] + D[7]);
-        s[4] += C2 * (D[6] + D[2]);
-        s[5] += C2 * (D[1] + D[3]);
-        break;
-    }
-    case 1: {  // Neo-Hookean: sigma = (mu (b - I) + lambda ln J I) / J, b - I = H + H^T + H H^T
-        float E = (float)p[0], nu = (float)p[1];
-        float lmd = nu * E / ((1.f + nu) * (1.f - 2.f * nu));
-        float mu = E / (2.f * (1.f + nu));
-        float lnj = log1pf((float)(J - 1.0));
-        float invJ = (float)(1.0 / J);
-        float B[6];
-        // B = H + H^T + H H^T in Voigt order xx,yy,zz,yz,xz,xy
-        B[0] = 2.f * H[0] + (H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
-        B[1] = 2.f * H[4] + (H[3] * H[3] + H[4] * H[4] + H[5] * H[5]);
-        B[2] = 2.f * H[8] + (H[6] * H[6] + H[7] * H[7] + H[8] * H[8]);
-        B[3] = (H[5] + H[7]) + (H[3] * H[6] + H[4] * H[7] + H[5] * H[8]);
-        B[4] = (H[2] + H[6]) + (H[0] * H[6] + H[1] * H[7] + H[2] * H[8]);
-        B[5] = (H[1] + H[3]) + (H[0] * H[3] + H[1] * H[4] + H[2] * H[5]);
-        float vol = lmd * lnj;
-        s[0] = (mu * B[0] + vol) * invJ;
-        s[1] = (mu * B[1] + vol) * invJ;
-        s[2] = (mu * B[2] + vol) * invJ;
-        s[3] = mu * B[3] * invJ;
-        s[4] = mu * B[4] * invJ;
-        s[5] = mu * B[5] * invJ;
-        break;
-    }
-    case 2: {  // St.Venant-Kirchhoff: sigma = (mu (B + B^2) + lambda trE (I + B)) / J with b = I + B
-        float E = (float)p[0], nu = (float)p[1];
-        float lmd = nu * E / ((1.f + nu) * (1.f - 2.f * nu));
-        float mu = E / (2.f * (1.f + nu));
-        float invJ = (float)(1.0 / J);
-        float Bm[9];
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++)
-                Bm[3 * a + b] = H[3 * a + b] + H[3 * b + a] +
-                                (H[3 * a] * H[3 * b] + H[3 * a + 1] * H[3 * b + 1] + H[3 * a + 2] * H[3 * b + 2]);
-        float trE = 0.5f * (Bm[0] + Bm[4] + Bm[8]);
-        float sg[9];
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) {
-                float b2 = Bm[3 * a] * Bm[b] + Bm[3 * a + 1] * Bm[3 + b] + Bm[3 * a + 2] * Bm[6 + b];
-                float eye = (a == b) ? 1.f : 0.f;
-                sg[3 * a + b] = (mu * (Bm[3 * a + b] + b2) + lmd * trE * (eye + Bm[3 * a + b])) * invJ;
-            }
-        s[0] = sg[0];
-        s[1] = sg[4];
-        s[2] = sg[8];
-        s[3] = sg[5];
-        s[4] = sg[2];
-        s[5] = sg[1];
-        break;
-    }
-    case 3: {  // Mooney-Rivlin (constitutive_model.cpp:187-285)
-        // The reference's form 2/J ((c1 + c2 I1) b - c2 b^2) - 2/(3J) (c1 I1 + 2 c2 I2) I + hp I
-        // (b of the isochoric F) subtracts terms of size c1 + c2 down to the stress. Written with
-        // B = b - I, its deviator is 2/J ((c1 + c2 (1 + tr B)) dev B - c2 dev B^2): the same
-        // quantity with nothing cancelling in FP32.
-        float c1 = (float)p[0], c2 = (float)p[1], bulk = (float)p[2];
-        float Jf = (float)J;
-        const float lj = log1pf((float)(J - 1.0));
-        const float E = expm1f(-2.f / 3.f * lj);  // J^(-2/3) - 1
-        float B[9];
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) {
-                const float HHt = H[3 * a] * H[3 * b] + H[3 * a + 1] * H[3 * b + 1] + H[3 * a + 2] * H[3 * b + 2];
-                B[3 * a + b] = (1.f + E) * (H[3 * a + b] + H[3 * b + a] + HHt) + (a == b ? E : 0.f);
-            }
-        float B2[9];
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++)
-                B2[3 * a + b] = B[3 * a] * B[b] + B[3 * a + 1] * B[3 + b] + B[3 * a + 2] * B[6 + b];
-        const float trB = B[0] + B[4] + B[8], trB2 = B2[0] + B2[4] + B2[8];
-        const float k1 = c1 + c2 * (1.f + trB);
-        const float hp = bulk * lj / Jf;
-        float sg[9];
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) {
-                const float dB = B[3 * a + b] - (a == b ? trB / 3.f : 0.f);
-                const float dB2 = B2[3 * a + b] - (a == b ? trB2 / 3.f : 0.f);
-                sg[3 * a + b] = 2.f * (k1 * dB - c2 * dB2) / Jf + (a == b ? hp : 0.f);
-            }
-        s[0] = sg[0];
-        s[1] = sg[4];
-        s[2] = sg[8];
-        s[3] = sg[5];
-        s[4] = sg[2];
-        s[5] = sg[1];
-        break;
-    }
-    case 20: {  // Tait equation of state + Newtonian viscosity; p without cancellation near J = 1
-        const float rho0 = (float)p[0], c = (float)p[1], gam = (float)p[2], mu = (float)p[3];
-        const float B = rho0 * c * c / gam;
-        const float pr = B * expm1f(-gam * log1pf((float)(J - 1.0)));
-        const float tr = (D[0] + D[4] + D[8]) * (1.f / 3.f);
-        const float m2 = 2.f * mu * inv_dt, m1 = mu * inv_dt;
-        s[0] = -pr + m2 * (D[0] - tr);
-        s[1] = -pr + m2 * (D[4] - tr);
-        s[2] = -pr + m2 * (D[8] - tr);
-        s[3] = m1 * (D[5] + D[7]);
-        s[4] = m1 * (D[6] + D[2]);
-        s[5] = m1 * (D[1] + D[3]);
-        break;
-    }
-    case 10: {  // J2 radial return, small-increment form
-        J2Trial t;
-        j2_trial(p, D, s, eps_p ? *eps_p : 0.f, t);
-        j2_return(t, s, eps_p);
-        break;
-    }
-    default: break;
-    }
-}
-
-__device__ __forceinline__ uint32_t mix32(uint32_t h) {
-    h ^= h >> 16;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    h *= 0xC2B2AE35u;
-    h ^= h >> 16;
-    return h;
-}
-
-// 13 surrogate inputs: strain increment (engineering shear), elastic strain
-// of the old stress, old equivalent plastic strain
-__device__ __forceinline__ void j2_features(const J2Trial &t, const float D[9], const float s_old[6], float x[16]) {
-    x[0] = D[0];
-    x[1] = D[4];
-    x[2] = D[8];
-    x[3] = D[5] + D[7];
-    x[4] = D[6] + D[2];
-    x[5] = D[1] + D[3];
-    const float tr = s_old[0] + s_old[1] + s_old[2];
-    const float a = (1.f + t.nu) / t.E, b = t.nu / t.E, im = 1.f / t.mu;
-#pragma unroll
-    for (int c = 0; c < 3; c++) x[6 + c] = a * s_old[c] - b * tr;
-#pragma unroll
-    for (int c = 3; c < 6; c++) x[6 + c] = s_old[c] * im;
-    x[12] = t.ep;
-    x[13] = x[14] = x[15] = 0.f;
-}
-
-// Residual of the discrete return-mapping equations at a predicted state
-// (stress s, plastic increment dg), relative to the yield stress:
-// max(|q(s) - sy - H (ep + dg)|, max_c |s_c - s_rr(dg)_c|) / sy, where
-// s_rr(dg) is the radial-return stress for that increment.
-__device__ __forceinline__ float j2_residual(const J2Trial &t, const float s[6], float dg) {
-    if (!(dg >= 0.f)) return 1e30f;
-    const float pm = (s[0] + s[1] + s[2]) * (1.f / 3.f);
-    const float e0 = s[0] - pm, e1 = s[1] - pm, e2 = s[2] - pm;
-    const float j2 = 0.5f * (e0 * e0 + e1 * e1 + e2 * e2) + s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
-    const float q = sqrtf(3.f * j2);
-    float r = fabsf(q - (t.sy + t.Hh * (t.ep + dg)));
-    const float scl = 1.f - 3.f * t.mu * dg / t.q;
-    const float rr[6] = {t.pm + t.d0 * scl, t.pm + t.d1 * scl, t.pm + t.d2 * scl,
-                         t.st[3] * scl,     t.st[4] * scl,     t.st[5] * scl};
-#pragma unroll
-    for (int c = 0; c < 6; c++) r = fmaxf(r, fabsf(s[c] - rr[c]));
-    return r / t.sy;
-}
-
-// tsum[t] += d (modulo 2^64) for the lanes with act set; one atomic per wave
-// when all of them share the tile. Called by every lane of the wave.
-__device__ __forceinline__ void tile_total_add(uint64_t *tsum, bool act, uint32_t t, unsigned long long d) {
-    const unsigned long long m = __ballot(act);
-    if (!m) return;
-    const int f = __ffsll(m) - 1;
-    const uint32_t t0 = (uint32_t)__shfl((int)t, f, 64);
-    if (__ballot(act && t != t0) == 0ull) {
-        unsigned long long sum = act ? d : 0ull;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-        if ((int)(threadIdx.x & 63) == f) atomicAdd((unsigned long long *)&tsum[t0], sum);
-    } else if (act) {
-        atomicAdd((unsigned long long *)&tsum[t], d);
-    }
-}
-
-// det(I + H) in FP64 of an FP32 displacement gradient. No contraction, so
-// that the update kernel and the J refresh give the same bits.
-__device__ __forceinline__ double det_I_plus_H(const float H[9]) {
-#pragma clang fp contract(off)
-    double A[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) A[q] = (double)H[q] + ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0);
-    return A[0] * (A[4] * A[8] - A[5] * A[7]) + A[1] * (A[5] * A[6] - A[3] * A[8]) +
-           A[2] * (A[3] * A[7] - A[4] * A[6]);
-}
 
 template <int SUR>
 __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
@@ -1052,9 +628,12 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                 dxp[1] += Nw * v1;
                 dxp[2] += Nw * v2;
             }
+    const int mat = P.matid ? P.matid[slot] : 0;
+    const MaterialDev &md = mt.m[mat];
+    const float dtp = sc.mat_dt ? md.dt : sc.dt;
     float D[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) D[q] = sc.dt * Dv[q];
+    for (int q = 0; q < 9; q++) D[q] = dtp * Dv[q];
     // H_new = (I + D)(I + H) - I = H + D + D H
     float H[9], Hn[9];
 #pragma unroll
@@ -1072,7 +651,7 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     bool inside = true;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        float ra = r[a] + (sc.dt * dxp[a]) * T.inv_dx[a];
+        float ra = r[a] + (dtp * dxp[a]) * T.inv_dx[a];
         const float fl = floorf(ra);
         ra -= fl;  // exact for |fl| <= 1 (Sterbenz)
         ci[a] += (int)fl;
@@ -1083,8 +662,6 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
         r[a] = ra;
         inside = inside && ci[a] >= 0 && ci[a] < g.nex[a];
     }
-    const int mat = P.matid ? P.matid[slot] : 0;
-    const MaterialDev &md = mt.m[mat];
     // only the rate forms (linear elastic increment, J2) read the old stress
     const bool rate_form = (md.model == 0 || md.model == 10);
     float s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -1093,17 +670,25 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
         for (int q = 0; q < 6; q++) s[q] = P.sig[q * ld + slot];
     }
     float ep = (md.model == 10 && P.eps_p) ? P.eps_p[slot] : 0.f;
+    bool defer = false;  // the stress (and plastic strain) of this slot is written by the network pass
     if constexpr (SUR == 0) {
-        stress32(md, Hn, D, J, s, &ep, 1.f / sc.dt);
+        stress32(md, Hn, D, J, s, &ep, 1.f / dtp);
     } else {
         const bool j2 = md.model == 10;
+        const bool tait = md.model == 20;
         J2Trial t;
+        // the network's candidates: J2 plastic trial states, or Tait particles
         bool plastic = false;
+        float pex = 0.f, Bt = 0.f;  // Tait: exact pressure and its scale B
         if (j2) {
             j2_trial(md.prop, D, s, ep, t);
-            plastic = valid && t.f > 0.f;
+            plastic = valid && t.f > 0.f && sa.kind == kSurJ2;
         } else {
-            stress32(md, Hn, D, J, s, &ep, 1.f / sc.dt);
+            stress32(md, Hn, D, J, s, &ep, 1.f / dtp);
+            if (tait && sa.kind == kSurTait) {
+                plastic = valid;
+                pex = tait_pressure(md.prop, J, &Bt);
+            }
         }
         const int lane = threadIdx.x & 63;
         if constexpr (SUR == 1) {
@@ -1122,14 +707,20 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                 if (want) {
                     const unsigned long long idx = base + __popcll(m & ((1ull << lane) - 1ull));
                     if (idx < sa.cap) {
-                        float x[16];
-                        j2_features(t, D, s_old, x);
-                        float y[16];
+                        float x[16], y[16];
+                        if (sa.kind == kSurTait) {
+                            tait_features(J, D, dtp, x);
+                            y[0] = pex;
 #pragma unroll
-                        for (int c = 0; c < 6; c++) y[c] = t.st[c] - s[c];
-                        y[6] = ep - t.ep;
+                            for (int c = 1; c < 16; c++) y[c] = 0.f;
+                        } else {
+                            j2_features(t, D, s_old, x);
 #pragma unroll
-                        for (int c = 7; c < 16; c++) y[c] = 0.f;
+                            for (int c = 0; c < 6; c++) y[c] = t.st[c] - s[c];
+                            y[6] = ep - t.ep;
+#pragma unroll
+                            for (int c = 7; c < 16; c++) y[c] = 0.f;
+                        }
                         float4 *xi = (float4 *)(sa.log_in + idx * 16), *yo = (float4 *)(sa.log_tgt + idx * 16);
 #pragma unroll
                         for (int c = 0; c < 4; c++) {
@@ -1141,12 +732,22 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                 }
             }
         } else {
-            // exact update for every particle; the plastic trial states of this
-            // block are compacted, in slot order, into the block's own queue
-            // region for the network pass that follows this kernel
+            // the candidates of this block are compacted, in slot order, into
+            // the block's own queue region for the network pass that follows
+            // this kernel. J2: the plastic corrector is left to that pass (the
+            // network where its residual is within the tolerance, the exact
+            // return otherwise), so the stress of a queued state is not
+            // stored here. Tait: the exact stress is stored and its pressure
+            // judges the network's.
             float x[16];
-            if (plastic) j2_features(t, D, s, x);
-            if (j2) j2_return(t, s, &ep);
+            if (plastic) {
+                if (sa.kind == kSurTait)
+                    tait_features(J, D, dtp, x);
+                else
+                    j2_features(t, D, s, x);
+            }
+            if (j2 && !plastic) j2_return(t, s, &ep);  // elastic trial state (or a J2 state the network skips)
+            defer = j2 && plastic;
             __shared__ uint32_t wcnt[kSurRegion / 64];
             const int w = (int)(threadIdx.x >> 6);
             const unsigned long long m = __ballot(plastic);
@@ -1167,9 +768,13 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                     row[0] = make_float4(__int_as_float(slot), x[0], x[1], x[2]);
                     row[1] = make_float4(x[3], x[4], x[5], x[6]);
                     row[2] = make_float4(x[7], x[8], x[9], x[10]);
-                    row[3] = make_float4(x[11], x[12], t.st[0], t.st[1]);
-                    row[4] = make_float4(t.st[2], t.st[3], t.st[4], t.st[5]);
-                    row[5] = make_float4(t.E, t.nu, t.sy, t.Hh);
+                    if (sa.kind == kSurTait) {
+                        row[3] = make_float4(x[11], x[12], pex, Bt);
+                    } else {
+                        row[3] = make_float4(x[11], x[12], t.st[0], t.st[1]);
+                        row[4] = make_float4(t.st[2], t.st[3], t.st[4], t.st[5]);
+                        row[5] = make_float4(t.E, t.nu, t.sy, t.Hh);
+                    }
                 }
             }
         }
@@ -1196,14 +801,16 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
         }
     }
     if (!valid) return;
-    if (md.model == 10 && P.eps_p) P.eps_p[slot] = ep;
+    if (md.model == 10 && P.eps_p && !defer) P.eps_p[slot] = ep;
     // streamed out with non-temporal stores: the particle state is next read
     // a whole step later, and stores that stay dirty in L2 slow the next
     // kernels (P2G cells 77 -> 66 us on the 4M-particle cube)
 #pragma unroll
     for (int q = 0; q < 9; q++) __builtin_nontemporal_store(Hn[q], &P.H[q * ld + slot]);
+    if (!defer) {
 #pragma unroll
-    for (int q = 0; q < 6; q++) __builtin_nontemporal_store(s[q], &P.sig[q * ld + slot]);
+        for (int q = 0; q < 6; q++) __builtin_nontemporal_store(s[q], &P.sig[q * ld + slot]);
+    }
 #pragma unroll
     for (int a = 0; a < 3; a++) __builtin_nontemporal_store(r[a], &P.r[a * ld + slot]);
     __builtin_nontemporal_store((float)V, &P.V32[slot]);
@@ -1238,19 +845,22 @@ __device__ __forceinline__ void sur_load_row(const float *queue, uint32_t qi, bo
     }
 }
 
-// Network pass over the queued plastic J2 trial states of a step. A tile
+// Network pass over the states queued by the update kernel of a step. A tile
 // slot is (region, k): rows [64 k, 64 k + 64) of queue region k / 4 (the
-// plastic states of one update block). The blocks are persistent (the
+// queued states of one update block). The blocks are persistent (the
 // network weights are staged in LDS once per block); each wave visits every
 // nw-th tile slot, finds the non-empty ones of its next 64 slots with one
 // load of the region counts per lane, and loads the rows of its next tile
-// while the current tile runs through the network on the BF16 MFMA. A
-// prediction is accepted where the return-mapping residual is within the
-// tolerance; the exact result written by the update kernel stays otherwise.
-// The evaluated / rejected tallies go to per-region counters: a region's
-// tiles may be spread over several waves, so they are added atomically, but
-// to distinct addresses.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k32_j2_surrogate(PartF32 P, SurrogateArgs sa) {
+// while the current tile runs through the network on the BF16 MFMA.
+// J2: a prediction is accepted where the return-mapping residual is within
+// the tolerance; otherwise the exact radial return of the queued trial state
+// is computed here (the update kernel left these stresses to this pass).
+// Tait: the network's pressure replaces the exact one where they agree
+// within the tolerance; otherwise the exact stress the update kernel wrote
+// stays. The evaluated / rejected tallies go to per-region counters: a
+// region's tiles may be spread over several waves, so they are added
+// atomically, but to distinct addresses.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k32_surrogate_pass(PartF32 P, SurrogateArgs sa) {
     __shared__ MlpInferSmem sw;
     __shared__ float scl[64];
     __shared__ __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];
@@ -1298,35 +908,39 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             bool accept = false;
             if (live) {
                 const int slot = __float_as_int(r[0]);
-                // trial state from the queued trial stress (same arithmetic as j2_trial)
-                J2Trial t;
-                const float E = r[20], nu = r[21];
-                t.E = E;
-                t.nu = nu;
-                t.sy = r[22];
-                t.Hh = r[23];
-                t.ep = r[13];
-                t.lmd = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
-                t.mu = E / (2.f * (1.f + nu));
+                const size_t ld = P.ld;
+                if (sa.kind == kSurTait) {
+                    // pressure closure: the exact Tait pressure judges the
+                    // network's; an accepted one replaces it in the stored stress
+                    const float pex = r[14], B = r[15];
+                    const float pnet = y[0] * scl[48] + scl[32];
+                    if (fabsf(pnet - pex) <= sa.tol * (fabsf(pex) + 1e-6f * B)) {
+                        accept = true;
+                        const float d = pex - pnet;
 #pragma unroll
-                for (int cc = 0; cc < 6; cc++) t.st[cc] = r[14 + cc];
-                t.pm = (t.st[0] + t.st[1] + t.st[2]) * (1.f / 3.f);
-                t.d0 = t.st[0] - t.pm;
-                t.d1 = t.st[1] - t.pm;
-                t.d2 = t.st[2] - t.pm;
-                const float j2 = 0.5f * (t.d0 * t.d0 + t.d1 * t.d1 + t.d2 * t.d2) + t.st[3] * t.st[3] +
-                                 t.st[4] * t.st[4] + t.st[5] * t.st[5];
-                t.q = sqrtf(3.f * j2);
-                float ss[6];
+                        for (int cc = 0; cc < 3; cc++) P.sig[cc * ld + slot] += d;
+                    }
+                } else {
+                    // trial state from the queued trial stress (the update kernel's arithmetic)
+                    J2Trial t;
+                    t.E = r[20];
+                    t.nu = r[21];
+                    t.sy = r[22];
+                    t.Hh = r[23];
+                    t.ep = r[13];
 #pragma unroll
-                for (int cc = 0; cc < 6; cc++) ss[cc] = t.st[cc] - (y[cc] * scl[48 + cc] + scl[32 + cc]);
-                const float dg = y[6] * scl[54] + scl[38];
-                if (j2_residual(t, ss, dg) <= sa.tol) {
-                    accept = true;
-                    const size_t ld = P.ld;
+                    for (int cc = 0; cc < 6; cc++) t.st[cc] = r[14 + cc];
+                    j2_trial_derived(t);
+                    float ss[6];
+#pragma unroll
+                    for (int cc = 0; cc < 6; cc++) ss[cc] = t.st[cc] - (y[cc] * scl[48 + cc] + scl[32 + cc]);
+                    const float dg = y[6] * scl[54] + scl[38];
+                    float epn = t.ep + dg;
+                    accept = j2_residual(t, ss, dg) <= sa.tol;
+                    if (!accept) j2_return(t, ss, &epn);  // fallback: the exact radial return
 #pragma unroll
                     for (int cc = 0; cc < 6; cc++) P.sig[cc * ld + slot] = ss[cc];
-                    if (P.eps_p) P.eps_p[slot] = t.ep + dg;
+                    if (P.eps_p) P.eps_p[slot] = epn;
                 }
             }
             const uint32_t ev = (uint32_t)__popcll(__ballot(live));
@@ -1366,6 +980,10 @@ __global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict
     dst.key[s] = src.key[o];
     if (src.eps_p) dst.eps_p[s] = src.eps_p[o];
     if (src.matid) dst.matid[s] = src.matid[o];
+    for (int q = 0; q < 9; q++) {
+        if (src.C) dst.C[q * lt + s] = src.C[q * ls + o];
+        if (src.Dinv) dst.Dinv[q * lt + s] = src.Dinv[q * ls + o];
+    }
     pid_dst[s] = pid_src[o];
 }
 
@@ -1440,24 +1058,37 @@ void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const No
                            N, bc, c);
 }
 
+template <int LPC, bool MDT>
+static void musl_cells_launch_mdt(hipStream_t s, int ng, const GridGeom &g, const PartF32 &P, const CellLists &L,
+                                  const NodeTables &T, const NodesF32 &N, const StepConsts &c, float *partials,
+                                  const MaterialTable &mt) {
+    if (c.scheme == 1)
+        hipLaunchKernelGGL((k32_musl_cells<LPC, true, MDT>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials,
+                           mt);
+    else
+        hipLaunchKernelGGL((k32_musl_cells<LPC, false, MDT>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials,
+                           mt);
+}
+
 template <int LPC>
 static void musl_cells_launch(hipStream_t s, int ng, const GridGeom &g, const PartF32 &P, const CellLists &L,
-                              const NodeTables &T, const NodesF32 &N, const StepConsts &c, float *partials) {
-    if (c.scheme == 1)
-        hipLaunchKernelGGL((k32_musl_cells<LPC, true>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
+                              const NodeTables &T, const NodesF32 &N, const StepConsts &c, float *partials,
+                              const MaterialTable *mt) {
+    if (c.mat_dt && mt)
+        musl_cells_launch_mdt<LPC, true>(s, ng, g, P, L, T, N, c, partials, *mt);
     else
-        hipLaunchKernelGGL((k32_musl_cells<LPC, false>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
+        musl_cells_launch_mdt<LPC, false>(s, ng, g, P, L, T, N, c, partials, MaterialTable{nullptr, 0});
 }
 
 void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                              const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
-                             const StepConsts &c, float *partials, int diag, hipEvent_t mid) {
+                             const StepConsts &c, float *partials, int diag, hipEvent_t mid, const MaterialTable *mt) {
     const int ng = cell_groups_grid(g);
     switch (c.lanes_per_cell) {
-    case 8: musl_cells_launch<8>(s, ng, g, P, L, T, N, c, partials); break;
-    case 4: musl_cells_launch<4>(s, ng, g, P, L, T, N, c, partials); break;
-    case 2: musl_cells_launch<2>(s, ng, g, P, L, T, N, c, partials); break;
-    default: musl_cells_launch<1>(s, ng, g, P, L, T, N, c, partials); break;
+    case 8: musl_cells_launch<8>(s, ng, g, P, L, T, N, c, partials, mt); break;
+    case 4: musl_cells_launch<4>(s, ng, g, P, L, T, N, c, partials, mt); break;
+    case 2: musl_cells_launch<2>(s, ng, g, P, L, T, N, c, partials, mt); break;
+    default: musl_cells_launch<1>(s, ng, g, P, L, T, N, c, partials, mt); break;
     }
     if (mid) (void)hipEventRecord(mid, s);
     if (diag)
@@ -1476,11 +1107,11 @@ void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const SurrogateArg
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k32_j2_surrogate, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k32_surrogate_pass, 256, 0);
         resident = std::max(1, cus) * std::max(1, per_cu);
     }
     const int blocks = std::min((int)sa.nregions, resident);
-    hipLaunchKernelGGL(k32_j2_surrogate, dim3(blocks), dim3(256), 0, s, P, sa);
+    hipLaunchKernelGGL(k32_surrogate_pass, dim3(blocks), dim3(256), 0, s, P, sa);
 }
 
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,

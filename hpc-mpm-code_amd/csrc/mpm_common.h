@@ -30,7 +30,7 @@ struct BcDense {
 
 struct MaterialDev {
     int model;
-    int pad;
+    float dt;  // time step of the particles of this material when per-material steps are on, else 0
     double prop[8];
 };
 

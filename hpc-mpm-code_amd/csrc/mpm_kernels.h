@@ -213,6 +213,8 @@ struct PartF32 {
     float *trac;    // [3][ld] or null
     int *matid;     // or null
     uint32_t *key;  // cell of each slot
+    float *C;       // [9][ld] TPIC velocity gradient / APIC B matrix, row-major (null for PIC/FLIP)
+    float *Dinv;    // [9][ld] APIC inverse D matrix (null otherwise)
     size_t ld;
     int n;
 };
@@ -239,6 +241,7 @@ struct StepConsts {
     int lanes_per_cell;  // 1, 2, 4 or 8 lanes fold one cell's particles in the cells kernels
     int cells_prefetch;  // cells kernels request the next chunk's indices while a chunk loads
     int cells_grid;      // workgroups of the cells kernels (0 = automatic)
+    int mat_dt;          // 1: each particle advances by its material's time step (MaterialDev::dt)
 };
 
 // Grid spacing constants of the FP32 kernels.
@@ -262,7 +265,24 @@ void launch_f32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const No
 // G2P velocity fused with the MUSL momentum remap + MUSL node pass.
 void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                              const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
+                             const StepConsts &c, float *partials, int diag, hipEvent_t mid = nullptr,
+                             const MaterialTable *mt = nullptr);
+// Stencil kernels (stencil32.hip): 2 or 3 control points per axis (idimc 1 or
+// 2; combinations (1,1,1), (2,2,2) and (2,1,2)), all four transfer schemes.
+// The engine uses them for the quadratic B-spline and for TPIC / APIC; the
+// tent-function PIC / FLIP step keeps the kernels above.
+bool s32_supported(const GridGeom &g);
+int s32_partials_per_cell(const GridGeom &g);
+void launch_s32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                    const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
+                    int diag, hipEvent_t mid = nullptr);
+void launch_s32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
+                             const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
                              const StepConsts &c, float *partials, int diag, hipEvent_t mid = nullptr);
+void launch_s32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
+                           const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
+                           const BinFuse &bf);
+
 // dF, H, J, V, x, stress and the new cell key of every slot.
 // J2 constitutive surrogate in the FP32 update: mode 0 = exact update only,
 // 1 = exact update and sample logging, 2 = exact update and a queue of the
@@ -271,10 +291,17 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
 // block of the update kernel compacts its plastic states into its own queue
 // region (rows [256 b, 256 b + count_b)) and stores count_b, so the queue is
 // filled without any global atomic.
-constexpr int kSurQueueRow = 24;  // floats per queued state: slot, 13 features, trial stress (6), E, nu, sy, H
+// floats per queued state: slot, 13 features, then J2: trial stress (6), E, nu, sy, H;
+// Tait: exact pressure, B = rho0 c^2 / gamma
+constexpr int kSurQueueRow = 24;
 constexpr int kSurRegion = 256;   // queue rows per update-kernel block (= its block size)
+// Models the surrogate stands in for: the J2 plastic corrector, or the Tait
+// pressure (a closure p(J, rate of volume) with the exact equation of state
+// as fallback and judge).
+enum { kSurJ2 = 0, kSurTait = 1 };
 struct SurrogateArgs {
     int mode;
+    int kind;                      // kSurJ2 / kSurTait
     float tol;                     // mode 2: accepted residual / yield stress
     MlpParams net;                 // mode 2
     const float *net_scaling;      // mode 2: [in_mu | in_isd | out_mu | out_sd], 64 floats

@@ -1,0 +1,920 @@
+// FP32 explicit-MPM step kernels for stencils of 2 or 3 control points per
+// axis: the open quadratic B-spline (idimc = 2, module/shape_function.cpp:
+// 53-104) on any axis, the tent functions (idimc = 1) on the others, and the
+// affine transfers TPIC and APIC (module/map_and_interpolate.h:57-176) with
+// either. The linear PIC / FLIP step keeps its tuned kernels (fast32.hip).
+//
+// Same structure and determinism as fast32.hip: a one-wave workgroup owns 64
+// consecutive non-empty cells (one lane per cell), stages their particles
+// (one contiguous range of the cell-sorted order) through LDS with the LDS
+// DMA, and folds each cell's particles in sorted order into per-cell corner
+// partials, NX NY NZ corners per cell. The node passes add, for each control
+// point, the partials of the cells whose stencil holds it, in ascending cell
+// order with Neumaier-compensated FP32 sums, and apply the grid update. The
+// update kernel gathers the NX NY NZ node velocities of each particle in slot
+// order. No float atomics anywhere; runs are bit-reproducible.
+//
+// Positions are cell index + FP32 offset r in [0, 1) in cell units, so the
+// weights are evaluated from r without cancellation: for the quadratic
+// B-spline the three interior pieces are 0.5 (1 - r)^2, 0.75 - (r - 1/2)^2 and
+// 0.5 r^2, and the clamped pieces of the first two and last two control
+// points are the reference's table rewritten in r.
+#include "fast32_device.h"
+
+#include <algorithm>
+
+namespace mpm {
+
+namespace {
+
+// Weights of the control points c + o (o < W) of a particle at offset r of
+// cell c on an axis of nex cells: w (N), d (dN/dr: times 1/dx gives dN/dx),
+// x (control-point coordinate minus the particle's, in cell units; the
+// control points are DefineCpPos's: the nodes for the tent functions, the
+// Greville abscissae of the open B-spline -- domain ends for the first and
+// last, cell centres in between, module/mesh.h:32-47).
+template <int W>
+struct AxisW {
+    float w[W], d[W], x[W];
+};
+
+template <int W>
+__device__ __forceinline__ void axis_weights(int c, float r, int nex, AxisW<W> &A) {
+    if constexpr (W == 2) {
+        A.w[0] = 1.f - r;
+        A.w[1] = r;
+        A.d[0] = -1.f;  // at r == 0 the left node takes the right-hand slope, as shape_function.cpp:45-50
+        A.d[1] = 1.f;
+        A.x[0] = -r;
+        A.x[1] = 1.f - r;
+    } else {
+        const float s = 1.f - r, h = r - 0.5f;
+        A.w[0] = 0.5f * s * s;
+        A.d[0] = -s;
+        A.x[0] = -0.5f - r;
+        A.w[1] = 0.75f - h * h;
+        A.d[1] = -2.f * h;
+        A.x[1] = 0.5f - r;
+        A.w[2] = 0.5f * r * r;
+        A.d[2] = r;
+        A.x[2] = 1.5f - r;
+        // clamped pieces, with the reference's precedence: global index 0,
+        // then 1, then the last, then the one before it
+        const int last = nex + 1;
+#pragma unroll
+        for (int o = 0; o < 3; o++) {
+            const int gi = c + o;
+            if (gi == 0) {
+                A.w[o] = s * s;
+                A.d[o] = -2.f * s;
+                A.x[o] = -r;
+            } else if (gi == 1) {
+                if (o == 1) {
+                    A.w[o] = r * (2.f - 1.5f * r);
+                    A.d[o] = 2.f - 3.f * r;
+                }
+            } else if (gi == last) {
+                A.w[o] = r * r;
+                A.d[o] = 2.f * r;
+                A.x[o] = (float)(nex - c) - r;
+            } else if (gi == last - 1) {
+                if (o == 1) {
+                    A.w[o] = 0.5f + r * (1.f - 1.5f * r);
+                    A.d[o] = 1.f - 3.f * r;
+                }
+            }
+        }
+    }
+}
+
+// 3x3 inverse through FP64 (the APIC D matrix is about dx^2 / 4 I).
+__device__ __forceinline__ void inv3_f(const float a[9], float o[9]) {
+    double A[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) A[q] = (double)a[q];
+    const double c0 = A[4] * A[8] - A[5] * A[7], c1 = A[5] * A[6] - A[3] * A[8], c2 = A[3] * A[7] - A[4] * A[6];
+    const double id = 1.0 / (A[0] * c0 + A[1] * c1 + A[2] * c2);
+    o[0] = (float)(c0 * id);
+    o[1] = (float)((A[2] * A[7] - A[1] * A[8]) * id);
+    o[2] = (float)((A[1] * A[5] - A[2] * A[4]) * id);
+    o[3] = (float)(c1 * id);
+    o[4] = (float)((A[0] * A[8] - A[2] * A[6]) * id);
+    o[5] = (float)((A[2] * A[3] - A[0] * A[5]) * id);
+    o[6] = (float)(c2 * id);
+    o[7] = (float)((A[1] * A[6] - A[0] * A[7]) * id);
+    o[8] = (float)((A[0] * A[4] - A[1] * A[3]) * id);
+}
+
+constexpr int kStageS = 65;  // stage block stride (64 + 1: conflict-free lane-per-cell reads)
+
+// Affine transfer matrix of a particle in cell units of the offset: Mh[a][b]
+// = M_ab dx_b, with M the TPIC velocity gradient or the APIC B D^-1, so that
+// M (x_a - x_p) = sum_b Mh[a][b] x_b for the cell-unit offsets x_b.
+template <int AFF>
+__device__ __forceinline__ void affine_matrix(const float *stage, int i, int fC, const float dx[3], float Mh[3][3]) {
+    float M[9];
+    if constexpr (AFF == 1) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) M[q] = stage[st_idx<kStageS>(fC + q, i)];
+    } else {
+        float B[9], Di[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            B[q] = stage[st_idx<kStageS>(fC + q, i)];
+            Di[q] = stage[st_idx<kStageS>(fC + 9 + q, i)];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) M[3 * a + b] = B[3 * a] * Di[b] + B[3 * a + 1] * Di[3 + b] + B[3 * a + 2] * Di[6 + b];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) Mh[a][b] = M[3 * a + b] * dx[b];
+}
+
+// ------------------------------------------------------------------ P2G
+// AFF: 0 none, 1 TPIC, 2 APIC. Partials per cell: corner q (= i + NX (j +
+// NY k), stencil order k, j, i as MakSf) x {m, px, py, pz, fx, fy, fz} at
+// q * 7 + f, then the corner volumes at 7 NN + q when diag is set.
+template <int NX, int NY, int NZ, int AFF>
+__global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T,
+                                                      float *__restrict__ part, int diag) {
+    constexpr int NN = NX * NY * NZ;
+    // staged fields: 0-2 r, 3-5 v, 6 m, 7 V, 8-13 sigma, 14-16 traction, then C (9) and D^-1 (9)
+    constexpr int F_C = 17;
+    constexpr int NF = F_C + (AFF >= 1 ? 9 : 0) + (AFF == 2 ? 9 : 0);
+    __shared__ float stage[NF * kBlk * kStageS];
+    const uint32_t n_active = L.ctr->n_active;
+    const int ngroups = (int)((n_active + kSeg - 1) / kSeg);
+    const int t = threadIdx.x;
+    const size_t ld = P.ld;
+    const bool trac = P.trac != nullptr;
+    const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
+    const float dxs[3] = {T.dx[0], T.dx[1], T.dx[2]};
+    for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        CellInfo si;
+        uint32_t S, E;
+        wave_fence();
+        group_setup<1>(g, L, n_active, grp, si, S, E);
+        float acc[NN][7];
+        float vof[NN];
+#pragma unroll
+        for (int q = 0; q < NN; q++) {
+#pragma unroll
+            for (int f = 0; f < 7; f++) acc[q][f] = 0.f;
+            vof[q] = 0.f;
+        }
+        for (uint32_t base = S; base < E; base += kChunk) {
+            const int nch = (int)min((uint32_t)kChunk, E - base);
+            uint32_t slot[kBlk];
+            load_slots(L, base, nch, slot);
+#pragma unroll
+            for (int r = 0; r < kBlk; r++) {
+                if (64 * r + t < nch) {
+                    const uint32_t sl = slot[r];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        lds_gather<kStageS, kNT>(P.r + a * ld, sl, stage, a, r);
+                        lds_gather<kStageS, kNT>(P.v + a * ld, sl, stage, 3 + a, r);
+                    }
+                    lds_gather<kStageS, kNT>(P.m, sl, stage, 6, r);
+                    lds_gather<kStageS, kNT>(P.V32, sl, stage, 7, r);
+#pragma unroll
+                    for (int q = 0; q < 6; q++) lds_gather<kStageS, kNT>(P.sig + q * ld, sl, stage, 8 + q, r);
+                    if (trac) {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) lds_gather<kStageS, kNT>(P.trac + a * ld, sl, stage, 14 + a, r);
+                    }
+                    if constexpr (AFF >= 1) {
+#pragma unroll
+                        for (int q = 0; q < 9; q++) lds_gather<kStageS, kNT>(P.C + q * ld, sl, stage, F_C + q, r);
+                    }
+                    if constexpr (AFF == 2) {
+#pragma unroll
+                        for (int q = 0; q < 9; q++)
+                            lds_gather<kStageS, kNT>(P.Dinv + q * ld, sl, stage, F_C + 9 + q, r);
+                    }
+                }
+            }
+            wait_vm_all();
+            wave_fence();
+            const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
+            for (uint32_t pos = lo; pos < hi; pos++) {
+                const int i = (int)(pos - base);
+                AxisW<NX> Ax;
+                AxisW<NY> Ay;
+                AxisW<NZ> Az;
+                axis_weights<NX>(si.ix, stage[st_idx<kStageS>(0, i)], g.nex[0], Ax);
+                axis_weights<NY>(si.iy, stage[st_idx<kStageS>(1, i)], g.nex[1], Ay);
+                axis_weights<NZ>(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], Az);
+                const float m = stage[st_idx<kStageS>(6, i)];
+                const float mv[3] = {m * stage[st_idx<kStageS>(3, i)], m * stage[st_idx<kStageS>(4, i)],
+                                     m * stage[st_idx<kStageS>(5, i)]};
+                const float Vp = stage[st_idx<kStageS>(7, i)];
+                const float nV = -Vp;
+                float sg[6];
+#pragma unroll
+                for (int q = 0; q < 6; q++) sg[q] = nV * stage[st_idx<kStageS>(8 + q, i)];
+                // -V sigma_ab / dx_b (Voigt xx,yy,zz,yz,xz,xy): f_a = sum_b (.)_ab dN/dr_b
+                const float X[3] = {sg[0] * inv0, sg[5] * inv0, sg[4] * inv0};
+                const float Y[3] = {sg[5] * inv1, sg[1] * inv1, sg[3] * inv1};
+                const float Z[3] = {sg[4] * inv2, sg[3] * inv2, sg[2] * inv2};
+                float tr[3] = {0.f, 0.f, 0.f};
+                if (trac) {
+#pragma unroll
+                    for (int a = 0; a < 3; a++) tr[a] = stage[st_idx<kStageS>(14 + a, i)];
+                }
+                // momentum per corner: m v + m M (x_a - x_p), split by axis
+                float px[3][NX], py[3][NY], pz[3][NZ];
+                if constexpr (AFF >= 1) {
+                    float Mh[3][3];
+                    affine_matrix<AFF>(stage, i, F_C, dxs, Mh);
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+#pragma unroll
+                        for (int u = 0; u < NX; u++) px[a][u] = mv[a] + (m * Mh[a][0]) * Ax.x[u];
+#pragma unroll
+                        for (int u = 0; u < NY; u++) py[a][u] = (m * Mh[a][1]) * Ay.x[u];
+#pragma unroll
+                        for (int u = 0; u < NZ; u++) pz[a][u] = (m * Mh[a][2]) * Az.x[u];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < NZ; k++)
+#pragma unroll
+                    for (int j = 0; j < NY; j++) {
+                        const float wyz = Ay.w[j] * Az.w[k];
+                        const float dyz = Ay.d[j] * Az.w[k];
+                        const float wydz = Ay.w[j] * Az.d[k];
+#pragma unroll
+                        for (int ii = 0; ii < NX; ii++) {
+                            const int q = ii + NX * (j + NY * k);
+                            const float N = Ax.w[ii] * wyz;
+                            const float gx = Ax.d[ii] * wyz, gy = Ax.w[ii] * dyz, gz = Ax.w[ii] * wydz;
+                            acc[q][0] += N * m;
+#pragma unroll
+                            for (int a = 0; a < 3; a++) {
+                                const float va = (AFF >= 1) ? (px[a][ii] + py[a][j]) + pz[a][k] : mv[a];
+                                acc[q][1 + a] += N * va;
+                                acc[q][4 + a] += (X[a] * gx + Y[a] * gy) + (Z[a] * gz + N * tr[a]);
+                            }
+                            vof[q] += N * Vp;
+                        }
+                    }
+            }
+            wave_fence();
+        }
+        if (si.ce > si.cs) {
+            const size_t ne = (size_t)g.nelem;
+#pragma unroll
+            for (int q = 0; q < NN; q++) {
+#pragma unroll
+                for (int f = 0; f < 7; f++) part[(size_t)(q * 7 + f) * ne + si.c] = acc[q][f];
+                if (diag) part[(size_t)(7 * NN + q) * ne + si.c] = vof[q];
+            }
+        }
+    }
+}
+
+// Cells whose stencil holds control point (i, j, k), in ascending cell
+// order: cell (i - NX + 1 + di, ...) for di < NX, the point being its corner
+// (NX - 1 - di, ...). Visits f(cell, corner) for the cells inside the grid
+// that hold particles.
+template <int NX, int NY, int NZ, typename F>
+__device__ __forceinline__ bool for_node_cells(const GridGeom &g, const CellLists &L, int i, int j, int k, F f) {
+    bool any = false;
+#pragma unroll
+    for (int dk = 0; dk < NZ; dk++) {
+        const int ck = k - (NZ - 1) + dk;
+        if (ck < 0 || ck >= g.nex[2]) continue;
+#pragma unroll
+        for (int dj = 0; dj < NY; dj++) {
+            const int cj = j - (NY - 1) + dj;
+            if (cj < 0 || cj >= g.nex[1]) continue;
+#pragma unroll
+            for (int di = 0; di < NX; di++) {
+                const int ci = i - (NX - 1) + di;
+                if (ci < 0 || ci >= g.nex[0]) continue;
+                const int c = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
+                if (!cell_nonempty(L, c)) continue;
+                any = true;
+                f(c, (NX - 1 - di) + NX * ((NY - 1 - dj) + NY * (NZ - 1 - dk)));
+            }
+        }
+    }
+    return any;
+}
+
+template <int NX, int NY, int NZ, bool DIAG>
+__global__ void __launch_bounds__(256) ks_p2g_nodes(GridGeom g, CellLists L, TileGeom tg, const float *__restrict__ part,
+                                                     NodesF32 N, BcDense bc, StepConsts sc) {
+    constexpr int NN = NX * NY * NZ;
+    int n, i, j, k;
+    if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
+    const size_t ne = (size_t)g.nelem;
+    constexpr int NQ = DIAG ? 8 : 7;
+    float s[NQ], cp[NQ];
+#pragma unroll
+    for (int f = 0; f < NQ; f++) s[f] = cp[f] = 0.f;
+    const bool any = for_node_cells<NX, NY, NZ>(g, L, i, j, k, [&](int c, int q) {
+        float val[NQ];
+#pragma unroll
+        for (int f = 0; f < 7; f++) val[f] = part[(size_t)(q * 7 + f) * ne + c];
+        if (DIAG) val[7] = part[(size_t)(7 * NN + q) * ne + c];
+#pragma unroll
+        for (int f = 0; f < NQ; f++) neu_add(s[f], cp[f], val[f]);
+    });
+    const int nodec = g.nodec;
+    if (!any) {
+        if (DIAG) {
+            N.nm[n] = 0.f;
+            N.nvof[n] = 0.f;
+            for (int c = 0; c < 3; c++) {
+                N.nv[c * nodec + n] = 0.f;
+                N.na[c * nodec + n] = 0.f;
+                N.np[c * nodec + n] = 0.f;
+                N.nf[c * nodec + n] = 0.f;
+                N.nvm[c * nodec + n] = 0.f;
+                N.nvp[c * nodec + n] = 0.f;
+            }
+        }
+        return;
+    }
+    const float m = s[0] + cp[0];
+    float p[3], f[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) p[c] = s[1 + c] + cp[1 + c];
+    f[0] = (s[4] + cp[4]) + m * sc.bx;
+    f[1] = (s[5] + cp[5]) + m * sc.by;
+    f[2] = (s[6] + cp[6]) + m * sc.bz;
+    float v[3] = {0.f, 0.f, 0.f}, a[3] = {0.f, 0.f, 0.f};
+    if ((double)m > sc.mtol) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v[c] = safe_div(p[c], m, sc.m_small);
+            a[c] = safe_div(f[c], m, sc.m_small);
+        }
+    }
+    const uint8_t mk = bc.mask ? bc.mask[n] : 0;
+    if (mk) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            if (mk & (1 << c)) {
+                v[c] = (float)(bc.val[c * nodec + n] * sc.facl);
+                a[c] = 0.f;
+            }
+    }
+    N.nm[n] = m;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        N.nv[c * nodec + n] = v[c];
+        N.na[c * nodec + n] = a[c];
+        N.nvp[c * nodec + n] = v[c] + sc.dt * a[c];
+    }
+    if (DIAG) {
+        float vof = s[7] + cp[7];
+        if ((double)vof < sc.mtol) vof = 0.f;
+        N.nvof[n] = vof;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            N.np[c * nodec + n] = p[c];
+            N.nf[c * nodec + n] = f[c];
+        }
+    }
+}
+
+// ------------------------------------------------------ G2P velocity + MUSL
+// SCH: 0 PIC, 1 FLIP, 2 TPIC, 3 APIC. G2P velocity from the cell's node
+// values (PIC family: v + dt a, FLIP: the particle's own velocity plus dt a),
+// the TPIC velocity gradient C_ab = sum dN/dx_b v_a (VarGradG2P) or the APIC
+// B_ab = sum N v_a (x_a - x_p)_b and D = sum N (x_a - x_p)(x_a - x_p)^T
+// (VarBmatG2P, DmatG2P, InvDmatG2P), then the MUSL momentum partials
+// N m (v + M (x_a - x_p)) of the corners.
+template <int NX, int NY, int NZ, int SCH>
+__global__ void __launch_bounds__(kSeg) ks_musl_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T, NodesF32 N,
+                                                       StepConsts sc, float *__restrict__ part) {
+    constexpr int NN = NX * NY * NZ;
+    constexpr bool FLIP = SCH == 1;
+    constexpr int AFF = SCH >= 2 ? SCH - 1 : 0;
+    // staged: 0-2 r, 3 m, [4-6 old v for FLIP]; results: v (3), then C or B (9), then D^-1 (9)
+    constexpr int NF = 4 + (FLIP ? 3 : 0);
+    constexpr int NO = 3 + (AFF >= 1 ? 9 : 0) + (AFF == 2 ? 9 : 0);
+    __shared__ float stage[NF * kBlk * kStageS];
+    __shared__ float vout[NO * kBlk * kStageS];
+    const uint32_t n_active = L.ctr->n_active;
+    const int ngroups = (int)((n_active + kSeg - 1) / kSeg);
+    const int t = threadIdx.x;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    const float inv[3] = {T.inv_dx[0], T.inv_dx[1], T.inv_dx[2]};
+    const float dxs[3] = {T.dx[0], T.dx[1], T.dx[2]};
+    for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        CellInfo si;
+        uint32_t S, E;
+        wave_fence();
+        group_setup<1>(g, L, n_active, grp, si, S, E);
+        const bool ne = si.ce > si.cs;
+        float nvl[NN][3];
+        if (ne) {
+            const int n0 = si.ix + g.nxc[0] * si.iy + g.nxc[0] * g.nxc[1] * si.iz;
+#pragma unroll
+            for (int k = 0; k < NZ; k++)
+#pragma unroll
+                for (int j = 0; j < NY; j++)
+#pragma unroll
+                    for (int i = 0; i < NX; i++) {
+                        const int q = i + NX * (j + NY * k);
+                        const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
+#pragma unroll
+                        for (int c = 0; c < 3; c++) nvl[q][c] = FLIP ? N.na[c * nodec + nid] : N.nvp[c * nodec + nid];
+                    }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NN; q++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) nvl[q][c] = 0.f;
+        }
+        float acc[NN][3];
+#pragma unroll
+        for (int q = 0; q < NN; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[q][c] = 0.f;
+
+        for (uint32_t base = S; base < E; base += kChunk) {
+            const int nch = (int)min((uint32_t)kChunk, E - base);
+            uint32_t slot[kBlk];
+            load_slots(L, base, nch, slot);
+#pragma unroll
+            for (int r = 0; r < kBlk; r++) {
+                if (64 * r + t < nch) {
+                    const uint32_t sl = slot[r];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        lds_gather<kStageS>(P.r + a * ld, sl, stage, a, r);
+                        if (FLIP) lds_gather<kStageS>(P.v + a * ld, sl, stage, 4 + a, r);
+                    }
+                    lds_gather<kStageS>(P.m, sl, stage, 3, r);
+                }
+            }
+            wait_vm_all();
+            wave_fence();
+            const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
+            for (uint32_t pos = lo; pos < hi; pos++) {
+                const int i = (int)(pos - base);
+                AxisW<NX> Ax;
+                AxisW<NY> Ay;
+                AxisW<NZ> Az;
+                axis_weights<NX>(si.ix, stage[st_idx<kStageS>(0, i)], g.nex[0], Ax);
+                axis_weights<NY>(si.iy, stage[st_idx<kStageS>(1, i)], g.nex[1], Ay);
+                axis_weights<NZ>(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], Az);
+                const float m = stage[st_idx<kStageS>(3, i)];
+                float vp[3] = {0.f, 0.f, 0.f};
+                float Cm[9];
+#pragma unroll
+                for (int q = 0; q < 9; q++) Cm[q] = 0.f;
+#pragma unroll
+                for (int k = 0; k < NZ; k++)
+#pragma unroll
+                    for (int j = 0; j < NY; j++)
+#pragma unroll
+                        for (int ii = 0; ii < NX; ii++) {
+                            const int q = ii + NX * (j + NY * k);
+                            const float wyz = Ay.w[j] * Az.w[k];
+                            const float Nw = Ax.w[ii] * wyz;
+#pragma unroll
+                            for (int c = 0; c < 3; c++) vp[c] += Nw * nvl[q][c];
+                            if constexpr (AFF == 1) {
+                                // C_ab += dN/dx_b v_a
+                                const float gr[3] = {Ax.d[ii] * wyz * inv[0], Ax.w[ii] * (Ay.d[j] * Az.w[k]) * inv[1],
+                                                     Ax.w[ii] * (Ay.w[j] * Az.d[k]) * inv[2]};
+#pragma unroll
+                                for (int a = 0; a < 3; a++)
+#pragma unroll
+                                    for (int b = 0; b < 3; b++) Cm[3 * a + b] += gr[b] * nvl[q][a];
+                            } else if constexpr (AFF == 2) {
+                                // B_ab += N v_a (x_a - x_p)_b
+                                const float dd[3] = {Ax.x[ii] * dxs[0], Ay.x[j] * dxs[1], Az.x[k] * dxs[2]};
+#pragma unroll
+                                for (int a = 0; a < 3; a++) {
+                                    const float nv = Nw * nvl[q][a];
+#pragma unroll
+                                    for (int b = 0; b < 3; b++) Cm[3 * a + b] += nv * dd[b];
+                                }
+                            }
+                        }
+                if (FLIP) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) vp[c] = stage[st_idx<kStageS>(4 + c, i)] + sc.dt * vp[c];
+                }
+                float Mh[3][3];
+                float Di[9];
+                if constexpr (AFF == 2) {
+                    // D = sum N (x_a - x_p)(x_a - x_p)^T factorises over the axes:
+                    // D_aa = dx_a^2 S2_a prod S0, D_ab = dx_a dx_b S1_a S1_b S0_c
+                    float s0[3] = {0.f, 0.f, 0.f}, s1[3] = {0.f, 0.f, 0.f}, s2[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int u = 0; u < NX; u++) {
+                        s0[0] += Ax.w[u];
+                        s1[0] += Ax.w[u] * Ax.x[u];
+                        s2[0] += Ax.w[u] * Ax.x[u] * Ax.x[u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < NY; u++) {
+                        s0[1] += Ay.w[u];
+                        s1[1] += Ay.w[u] * Ay.x[u];
+                        s2[1] += Ay.w[u] * Ay.x[u] * Ay.x[u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < NZ; u++) {
+                        s0[2] += Az.w[u];
+                        s1[2] += Az.w[u] * Az.x[u];
+                        s2[2] += Az.w[u] * Az.x[u] * Az.x[u];
+                    }
+                    float D[9];
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) {
+                            const int c3 = 3 - a - b;  // the third axis when a != b
+                            D[3 * a + b] = (a == b) ? dxs[a] * dxs[a] * s2[a] * s0[(a + 1) % 3] * s0[(a + 2) % 3]
+                                                    : dxs[a] * dxs[b] * s1[a] * s1[b] * s0[c3];
+                        }
+                    inv3_f(D, Di);
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++)
+                            Mh[a][b] = (Cm[3 * a] * Di[b] + Cm[3 * a + 1] * Di[3 + b] + Cm[3 * a + 2] * Di[6 + b]) *
+                                       dxs[b];
+                } else if constexpr (AFF == 1) {
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) Mh[a][b] = Cm[3 * a + b] * dxs[b];
+                }
+                vout[st_idx<kStageS>(0, i)] = vp[0];
+                vout[st_idx<kStageS>(1, i)] = vp[1];
+                vout[st_idx<kStageS>(2, i)] = vp[2];
+                if constexpr (AFF >= 1) {
+#pragma unroll
+                    for (int q = 0; q < 9; q++) vout[st_idx<kStageS>(3 + q, i)] = Cm[q];
+                }
+                if constexpr (AFF == 2) {
+#pragma unroll
+                    for (int q = 0; q < 9; q++) vout[st_idx<kStageS>(12 + q, i)] = Di[q];
+                }
+                const float mv[3] = {m * vp[0], m * vp[1], m * vp[2]};
+                float px[3][NX], py[3][NY], pz[3][NZ];
+                if constexpr (AFF >= 1) {
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+#pragma unroll
+                        for (int u = 0; u < NX; u++) px[a][u] = mv[a] + (m * Mh[a][0]) * Ax.x[u];
+#pragma unroll
+                        for (int u = 0; u < NY; u++) py[a][u] = (m * Mh[a][1]) * Ay.x[u];
+#pragma unroll
+                        for (int u = 0; u < NZ; u++) pz[a][u] = (m * Mh[a][2]) * Az.x[u];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < NZ; k++)
+#pragma unroll
+                    for (int j = 0; j < NY; j++)
+#pragma unroll
+                        for (int ii = 0; ii < NX; ii++) {
+                            const int q = ii + NX * (j + NY * k);
+                            const float Nw = Ax.w[ii] * (Ay.w[j] * Az.w[k]);
+#pragma unroll
+                            for (int a = 0; a < 3; a++) {
+                                const float va = (AFF >= 1) ? (px[a][ii] + py[a][j]) + pz[a][k] : mv[a];
+                                acc[q][a] += Nw * va;
+                            }
+                        }
+            }
+            wave_fence();
+#pragma unroll
+            for (int r = 0; r < kBlk; r++) {
+                const int i = 64 * r + t;
+                if (i < nch) {
+                    const uint32_t sl = slot[r];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) __builtin_nontemporal_store(vout[st_idx<kStageS>(c, i)], &P.v[c * ld + sl]);
+                    if constexpr (AFF >= 1) {
+#pragma unroll
+                        for (int q = 0; q < 9; q++) P.C[q * ld + sl] = vout[st_idx<kStageS>(3 + q, i)];
+                    }
+                    if constexpr (AFF == 2) {
+#pragma unroll
+                        for (int q = 0; q < 9; q++) P.Dinv[q * ld + sl] = vout[st_idx<kStageS>(12 + q, i)];
+                    }
+                }
+            }
+            wave_fence();
+        }
+        if (ne) {
+            const size_t nel = (size_t)g.nelem;
+#pragma unroll
+            for (int q = 0; q < NN; q++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) part[(size_t)(q * 3 + c) * nel + si.c] = acc[q][c];
+        }
+    }
+}
+
+template <int NX, int NY, int NZ, bool DIAG>
+__global__ void __launch_bounds__(256) ks_musl_nodes(GridGeom g, CellLists L, TileGeom tg,
+                                                      const float *__restrict__ part, NodesF32 N, BcDense bc,
+                                                      StepConsts sc) {
+    int n, i, j, k;
+    if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
+    const size_t ne = (size_t)g.nelem;
+    float s[3] = {0.f, 0.f, 0.f}, cp[3] = {0.f, 0.f, 0.f};
+    const bool any = for_node_cells<NX, NY, NZ>(g, L, i, j, k, [&](int c, int q) {
+        float val[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) val[f] = part[(size_t)(q * 3 + f) * ne + c];
+#pragma unroll
+        for (int f = 0; f < 3; f++) neu_add(s[f], cp[f], val[f]);
+    });
+    const int nodec = g.nodec;
+    if (!any) {
+        if (DIAG)
+            for (int c = 0; c < 3; c++) N.nvm[c * nodec + n] = 0.f;
+        return;
+    }
+    const float m = N.nm[n];
+    float v[3] = {0.f, 0.f, 0.f};
+    const float p[3] = {s[0] + cp[0], s[1] + cp[1], s[2] + cp[2]};
+    if ((double)m > sc.mtol) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = safe_div(p[c], m, sc.m_small);
+    }
+    const uint8_t mk = bc.mask ? bc.mask[n] : 0;
+    if (mk) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            if (mk & (1 << c)) v[c] = (float)(bc.val[c * nodec + n] * sc.facl);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) N.nvm[c * nodec + n] = v[c];
+    if (DIAG) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) N.np[c * nodec + n] = p[c];
+    }
+}
+
+// ------------------------------------------------------ particle update
+// dF - I = dt sum v_a (x) grad N_a over the stencil, H, J (FP64), x, V,
+// stress and the new cell key, in slot order; fused cell counting for the
+// next binning as in the tent-function update kernel.
+template <int NX, int NY, int NZ>
+__global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
+                                                      MaterialTable mt, StepConsts sc, uint32_t *err, BinFuse bf) {
+    const int slot0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bf.cnt) {
+        for (int i = slot0; i < bf.ntiles; i += gridDim.x * blockDim.x) bf.tiles[i] = 0u;
+        if (slot0 == 0) bf.ctr->n_active = 0u;
+    }
+    // every lane takes part in the wave-wide counting; invalid lanes compute
+    // on slot 0 / cell 0 and store nothing
+    bool valid = slot0 < P.n;
+    const int slot = valid ? slot0 : 0;
+    uint32_t key = P.key[slot];
+    if (key >= (uint32_t)g.nelem) valid = false;
+    if (!valid) key = 0;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    const int nexy = g.nex[0] * g.nex[1];
+    int rem, ix, iy;
+    const int iz = fdivmod((int)key, nexy, T.inv_nexy, rem);
+    iy = fdivmod(rem, g.nex[0], T.inv_nex, ix);
+    float r[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r[a] = P.r[a * ld + slot];
+    AxisW<NX> Ax;
+    AxisW<NY> Ay;
+    AxisW<NZ> Az;
+    axis_weights<NX>(ix, r[0], g.nex[0], Ax);
+    axis_weights<NY>(iy, r[1], g.nex[1], Ay);
+    axis_weights<NZ>(iz, r[2], g.nex[2], Az);
+    const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
+    const int n0 = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+    float Dv[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) Dv[q] = 0.f;
+    float dxp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < NZ; k++)
+#pragma unroll
+        for (int j = 0; j < NY; j++)
+#pragma unroll
+            for (int i = 0; i < NX; i++) {
+                const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
+                const float v0 = N.nvm[nid], v1 = N.nvm[nodec + nid], v2 = N.nvm[2 * nodec + nid];
+                const float wyz = Ay.w[j] * Az.w[k];
+                const float Nw = Ax.w[i] * wyz;
+                const float gx = (Ax.d[i] * inv0) * wyz;
+                const float gy = (Ay.d[j] * inv1) * (Ax.w[i] * Az.w[k]);
+                const float gz = (Az.d[k] * inv2) * (Ax.w[i] * Ay.w[j]);
+                Dv[0] += v0 * gx;
+                Dv[1] += v0 * gy;
+                Dv[2] += v0 * gz;
+                Dv[3] += v1 * gx;
+                Dv[4] += v1 * gy;
+                Dv[5] += v1 * gz;
+                Dv[6] += v2 * gx;
+                Dv[7] += v2 * gy;
+                Dv[8] += v2 * gz;
+                dxp[0] += Nw * v0;
+                dxp[1] += Nw * v1;
+                dxp[2] += Nw * v2;
+            }
+    float D[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) D[q] = sc.dt * Dv[q];
+    float H[9], Hn[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) H[q] = __builtin_nontemporal_load(&P.H[q * ld + slot]);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            Hn[3 * a + b] = (H[3 * a + b] + D[3 * a + b]) +
+                            (D[3 * a] * H[b] + D[3 * a + 1] * H[3 + b] + D[3 * a + 2] * H[6 + b]);
+    const double J = det_I_plus_H(Hn);
+    const double V = J * (double)P.V0[slot];
+    int ci[3] = {ix, iy, iz};
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float ra = r[a] + (sc.dt * dxp[a]) * T.inv_dx[a];
+        const float fl = floorf(ra);
+        ra -= fl;
+        ci[a] += (int)fl;
+        if (ra >= 1.f) {
+            ra = 0.f;
+            ci[a] += 1;
+        }
+        r[a] = ra;
+        inside = inside && ci[a] >= 0 && ci[a] < g.nex[a];
+    }
+    const int mat = P.matid ? P.matid[slot] : 0;
+    const MaterialDev &md = mt.m[mat];
+    const bool rate_form = (md.model == 0 || md.model == 10);
+    float s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (rate_form) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) s[q] = P.sig[q * ld + slot];
+    }
+    float ep = (md.model == 10 && P.eps_p) ? P.eps_p[slot] : 0.f;
+    stress32(md, Hn, D, J, s, &ep, 1.f / sc.dt);
+    const uint32_t nk = inside ? (uint32_t)(ci[0] + ci[1] * g.nex[0] + ci[2] * nexy) : (uint32_t)g.nelem;
+    if (bf.cnt) {
+        if (bf.delta) {
+            const bool mover = valid && nk != key;
+            const bool lands = mover && nk < (uint32_t)g.nelem;
+            uint32_t o_from = 0u, o_to = 1u;
+            if (mover) o_from = atomicSub(&bf.cnt[key], 1u);
+            if (lands) o_to = atomicAdd(&bf.cnt[nk], 1u);
+            const unsigned long long d_from = 1ull + (o_from == 1u ? (1ull << 32) : 0ull);
+            const unsigned long long d_to = 1ull + (o_to == 0u ? (1ull << 32) : 0ull);
+            tile_total_add(bf.tsum, mover, key >> kBinTileShift, 0ull - d_from);
+            tile_total_add(bf.tsum, lands, nk >> kBinTileShift, d_to);
+        } else {
+            const uint32_t ck = valid ? nk : 0xFFFFFFFFu;
+            int len, head;
+            const bool h = wave_runs(ck, threadIdx.x & 63, &len, &head);
+            if (h && ck < (uint32_t)g.nelem) atomicAdd(&bf.cnt[ck], (uint32_t)len);
+        }
+    }
+    if (!valid) return;
+    if (md.model == 10 && P.eps_p) P.eps_p[slot] = ep;
+#pragma unroll
+    for (int q = 0; q < 9; q++) __builtin_nontemporal_store(Hn[q], &P.H[q * ld + slot]);
+#pragma unroll
+    for (int q = 0; q < 6; q++) __builtin_nontemporal_store(s[q], &P.sig[q * ld + slot]);
+#pragma unroll
+    for (int a = 0; a < 3; a++) __builtin_nontemporal_store(r[a], &P.r[a * ld + slot]);
+    __builtin_nontemporal_store((float)V, &P.V32[slot]);
+    if (!inside) {
+        P.J[slot] = J;
+        P.V[slot] = V;
+    }
+    if (nk != key) P.key[slot] = nk;
+    uint32_t e = 0;
+    if (!inside) e |= kErrOutOfGrid;
+    if (!(J > 0.0)) e |= kErrJacobian;
+    if (!isfinite(r[0] + r[1] + r[2] + s[0] + s[1] + s[2])) e |= kErrNonFinite;
+    if (e) atomicOr(err, e);
+}
+
+int cell_groups(const GridGeom &g) {
+    const int ng = cdiv(g.nelem, 64);
+    return ng < 8192 ? ng : 8192;
+}
+
+template <int NX, int NY, int NZ>
+void p2g_launch(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
+                int diag, hipEvent_t mid) {
+    const int ng = c.cells_grid > 0 ? c.cells_grid : cell_groups(g);
+    if (c.scheme == 2)
+        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 1>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
+    else if (c.scheme == 3)
+        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 2>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
+    else
+        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 0>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
+    if (mid) (void)hipEventRecord(mid, s);
+    if (diag)
+        hipLaunchKernelGGL((ks_p2g_nodes<NX, NY, NZ, true>), dim3(tg.ntiles), dim3(256), 0, s, g, L, tg,
+                           (const float *)partials, N, bc, c);
+    else
+        hipLaunchKernelGGL((ks_p2g_nodes<NX, NY, NZ, false>), dim3(tg.ntiles), dim3(256), 0, s, g, L, tg,
+                           (const float *)partials, N, bc, c);
+}
+
+template <int NX, int NY, int NZ>
+void musl_launch(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                 const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
+                 int diag, hipEvent_t mid) {
+    const int ng = c.cells_grid > 0 ? c.cells_grid : cell_groups(g);
+    switch (c.scheme) {
+    case 1:
+        hipLaunchKernelGGL((ks_musl_cells<NX, NY, NZ, 1>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
+        break;
+    case 2:
+        hipLaunchKernelGGL((ks_musl_cells<NX, NY, NZ, 2>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
+        break;
+    case 3:
+        hipLaunchKernelGGL((ks_musl_cells<NX, NY, NZ, 3>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
+        break;
+    default:
+        hipLaunchKernelGGL((ks_musl_cells<NX, NY, NZ, 0>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, N, c, partials);
+        break;
+    }
+    if (mid) (void)hipEventRecord(mid, s);
+    if (diag)
+        hipLaunchKernelGGL((ks_musl_nodes<NX, NY, NZ, true>), dim3(tg.ntiles), dim3(256), 0, s, g, L, tg,
+                           (const float *)partials, N, bc, c);
+    else
+        hipLaunchKernelGGL((ks_musl_nodes<NX, NY, NZ, false>), dim3(tg.ntiles), dim3(256), 0, s, g, L, tg,
+                           (const float *)partials, N, bc, c);
+}
+
+// stencil widths (points per axis) of the supported idimc combinations:
+// 0 = (1,1,1), 1 = (2,2,2), 2 = (2,1,2); -1 otherwise
+int stencil_kind(const GridGeom &g) {
+    const int a = g.idimc[0], b = g.idimc[1], c = g.idimc[2];
+    if (a == 1 && b == 1 && c == 1) return 0;
+    if (a == 2 && b == 2 && c == 2) return 1;
+    if (a == 2 && b == 1 && c == 2) return 2;
+    return -1;
+}
+
+}  // namespace
+
+bool s32_supported(const GridGeom &g) { return stencil_kind(g) >= 0; }
+
+int s32_partials_per_cell(const GridGeom &g) {
+    const int nn = (g.idimc[0] + 1) * (g.idimc[1] + 1) * (g.idimc[2] + 1);
+    return 8 * nn;  // 7 P2G quantities + volume per corner (MUSL uses 3)
+}
+
+void launch_s32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                    const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
+                    int diag, hipEvent_t mid) {
+    switch (stencil_kind(g)) {
+    case 0: p2g_launch<2, 2, 2>(s, g, P, N, L, bc, T, tg, c, partials, diag, mid); break;
+    case 1: p2g_launch<3, 3, 3>(s, g, P, N, L, bc, T, tg, c, partials, diag, mid); break;
+    case 2: p2g_launch<3, 2, 3>(s, g, P, N, L, bc, T, tg, c, partials, diag, mid); break;
+    default: break;
+    }
+}
+
+void launch_s32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
+                             const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
+                             const StepConsts &c, float *partials, int diag, hipEvent_t mid) {
+    switch (stencil_kind(g)) {
+    case 0: musl_launch<2, 2, 2>(s, g, P, N, L, bc, T, tg, c, partials, diag, mid); break;
+    case 1: musl_launch<3, 3, 3>(s, g, P, N, L, bc, T, tg, c, partials, diag, mid); break;
+    case 2: musl_launch<3, 2, 3>(s, g, P, N, L, bc, T, tg, c, partials, diag, mid); break;
+    default: break;
+    }
+}
+
+void launch_s32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
+                           const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
+                           const BinFuse &bf) {
+    if (P.n <= 0) return;
+    const dim3 grid(cdiv(P.n, 256)), block(256);
+    switch (stencil_kind(g)) {
+    case 0: hipLaunchKernelGGL((ks_g2p_update<2, 2, 2>), grid, block, 0, s, g, P, N, T, mt, c, err, bf); break;
+    case 1: hipLaunchKernelGGL((ks_g2p_update<3, 3, 3>), grid, block, 0, s, g, P, N, T, mt, c, err, bf); break;
+    case 2: hipLaunchKernelGGL((ks_g2p_update<3, 2, 3>), grid, block, 0, s, g, P, N, T, mt, c, err, bf); break;
+    default: break;
+    }
+}
+
+}  // namespace mpm

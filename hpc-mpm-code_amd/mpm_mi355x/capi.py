@@ -82,7 +82,7 @@ class TrainParams(C.Structure):
 
 class SurrogateStats(C.Structure):
     _fields_ = [("samples", C.c_int64), ("logged_steps", C.c_int64), ("evaluated", C.c_int64),
-                ("fallbacks", C.c_int64), ("loss", C.c_float), ("tol", C.c_float)]
+                ("fallbacks", C.c_int64), ("loss", C.c_float), ("tol", C.c_float), ("offered", C.c_int64)]
 
 
 class TuneResult(C.Structure):
@@ -125,6 +125,7 @@ SIGNATURES = {
     "mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "mpm_maksf_batch": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _ip, _dp, _ip, _dp, _dp]),
     "mpm_bin_positions": (C.c_int, [C.POINTER(GridDesc), C.c_int64, _dp, _ip, _ip]),
+    "mpm_set_material_dt": (C.c_int, [C.c_void_p, _dp]),
     "mpm_bin_positions_noalloc": (C.c_int, [C.POINTER(GridDesc), C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),
     "mpm_bandwidth_probe": (C.c_int, [C.c_int64, C.c_int32, _dp]),
@@ -333,6 +334,8 @@ class Engine:
             self.set_shifting(True, scene.npxye)
         if getattr(scene, "fbar", False):
             self.set_fbar(True)
+        if getattr(scene, "mat_dt", None) is not None:
+            self.set_material_dt(scene.mat_dt)
 
     def set_shifting(self, on, npxye=(2, 2, 2)):
         npx = np.ascontiguousarray(npxye, dtype=np.int32)
@@ -445,6 +448,11 @@ class Engine:
 
     def set_timing(self, on=True):
         check(self.lib.mpm_set_timing(self.h, 1 if on else 0))
+
+    def set_material_dt(self, dt):
+        """Per-material time steps (None restores the common dt)."""
+        d = None if dt is None else np.ascontiguousarray(dt, np.float64)
+        check(self.lib.mpm_set_material_dt(self.h, _ptr(d)))
 
     def set_option(self, key, value):
         check(self.lib.mpm_set_option(self.h, key, value))

@@ -65,6 +65,7 @@ class Scene:
     shifting: bool = False                   # reference particle shifting (FP64-exact engine)
     npxye: tuple = (2, 2, 2)                 # points per cell of the nodal-volume reference (MakNodalVol)
     fbar: bool = False                       # F-bar projection (FP64-exact engine)
+    mat_dt: list = None                      # per-material time steps (FP32 engine), or None: dt for all
 
     @property
     def dx(self):
@@ -337,14 +338,16 @@ def cfg4(n_scenes=256, seed=0x5EED0004):
     return sc, bucket
 
 
-def atlas(scene_list, name="atlas"):
+def atlas(scene_list, name="atlas", per_scene_dt=True):
     """Packs 2D slab scenes of one grid (nelem (G, 1, G), same spacing, density
     and gravity) into one grid so that a single engine advances them all:
     scene k sits in tile (k mod T, k div T) of (G+1) x (G+1) cells, one empty
     gap cell between tiles so that no node is shared; each scene keeps its
     material (matid = k), its floor (the tile's bottom node row: u = w = 0)
-    and v = 0 everywhere. All scenes take the smallest of their time steps.
-    Returns (scene, layout) with layout[k] = (first particle, count, origin)."""
+    and v = 0 everywhere. Each scene keeps its own CFL time step as its
+    material's time step (per_scene_dt, FP32 engine), or all take the
+    smallest. Returns (scene, layout) with layout[k] = (first particle,
+    count, origin)."""
     G = scene_list[0].nelem[0]
     h = scene_list[0].dx[0]
     for sc in scene_list:
@@ -377,17 +380,18 @@ def atlas(scene_list, name="atlas"):
     allv = node_lists(nelem, lambda i, j, k_, nx, ny, nz: np.ones_like(i, dtype=bool))
     mats = [sc.materials[0] for sc in scene_list]
     dt = min(sc.dt for sc in scene_list)
+    mat_dt = [sc.dt for sc in scene_list] if per_scene_dt else None
     return (Scene(name, xmin, xmax, nelem, mats, scene_list[0].rho, dt, scene_list[0].steps, scene_list[0].bb,
-                  bcs={0: _bc(floor), 1: _bc(allv), 2: _bc(floor)}, parts=parts), layout)
+                  bcs={0: _bc(floor), 1: _bc(allv), 2: _bc(floor)}, parts=parts, mat_dt=mat_dt), layout)
 
 
-def cfg4_atlases(n_scenes=256, seed=0x5EED0004):
+def cfg4_atlases(n_scenes=256, seed=0x5EED0004, per_scene_dt=True):
     """The ensemble packed per grid-size bucket: [(atlas scene, layout, member scene indices)]."""
     sc, bucket = cfg4(n_scenes, seed)
     out = []
     for b in range(3):
         idx = [i for i, x in enumerate(bucket) if x == b]
         if idx:
-            a, lay = atlas([sc[i] for i in idx], "cfg4-atlas-%d" % (64 << b))
+            a, lay = atlas([sc[i] for i in idx], "cfg4-atlas-%d" % (64 << b), per_scene_dt)
             out.append((a, lay, idx))
     return out

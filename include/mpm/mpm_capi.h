@@ -317,6 +317,14 @@ int mpm_set_fbar(mpm_engine *eng, int32_t on);
 /* Fbar [n][3][3] and Jbar [n] (NULL to skip); the upload restores a restart. */
 int mpm_download_fbar(mpm_engine *eng, double *def_grad_bar, double *det_def_grad_bar);
 int mpm_upload_fbar(mpm_engine *eng, const double *def_grad_bar, const double *det_def_grad_bar);
+/* Per-material time steps, no counterpart in the reference (one dt per run
+ * there): each particle advances by dt[matid] (nmat values, > 0) in place of
+ * the step parameters' dt, which batched scenes packed into one grid use to
+ * keep their own CFL steps (one material per scene). FP32 engine with the
+ * tent functions and PIC / FLIP only (MPM_ERR_UNSUPPORTED otherwise); the
+ * node velocity v + dt a of PIC's G2P is then formed per particle. NULL
+ * restores the common dt. */
+int mpm_set_material_dt(mpm_engine *eng, const double *dt);
 /* Enables/disables per-kernel HIP-event timing inside mpm_step (off by default). */
 int mpm_set_timing(mpm_engine *eng, int32_t enabled);
 

@@ -1,23 +1,33 @@
 /*
- * Learned constitutive surrogate for the J2 return mapping: C ABI.
+ * Learned constitutive surrogate: C ABI.
  *
  * No counterpart in the reference (module/solid/constitutive_model.cpp has no
- * plasticity and no learned model). The surrogate stands in for the plastic
- * corrector of the J2 radial return inside the engine's stress update
- * (model MPM_MODEL_J2 only; elastic trial states keep the exact update):
+ * plasticity, no fluid equation of state and no learned model). One network
+ * stands in for one part of the engine's stress update, chosen by the
+ * engine's materials: the plastic corrector of the J2 radial return when a
+ * material is MPM_MODEL_J2, else the Tait pressure of MPM_MODEL_TAIT
+ * materials (a pressure closure). Elastic J2 trial states and every other
+ * model keep the exact update.
  *
+ * J2 rows:
  *   inputs  (13) : strain increment (6, engineering shear, Voigt xx,yy,zz,
  *                  yz,xz,xy), elastic strain of the old stress C^-1 sigma_n (6),
  *                  equivalent plastic strain eps_p (1)
  *   outputs (7)  : plastic stress correction sigma_trial - sigma_{n+1} (6) and
  *                  plastic strain increment (1)
- *   network      : 16 -> 64 -> 64 -> 16 (13 inputs and 7 outputs used), ReLU,
- *                  BF16 weights/activations, FP32 accumulation on MFMA.
- *
- * A prediction is accepted only when the discrete return-mapping equations
- * hold within `tol` (relative to the yield stress): the yield condition at
- * the predicted state, and the radial-return stress for the predicted
- * plastic increment. Otherwise that particle takes the exact return mapping.
+ *   A prediction is accepted only when the discrete return-mapping equations
+ *   hold within `tol` (relative to the yield stress): the yield condition at
+ *   the predicted state, and the radial-return stress for the predicted
+ *   plastic increment. Otherwise that particle takes the exact return
+ *   mapping, which the network pass then computes.
+ * Tait rows (pressure closure p(J, rate)):
+ *   inputs  (2 of 13): J - 1 and the volumetric strain rate tr(dF - I) / dt
+ *   output  (1 of 7) : pressure
+ *   The exact Tait pressure is the fallback and the judge: a prediction is
+ *   accepted where |p_net - p| <= tol (|p| + 1e-6 B), B = rho0 c^2 / gamma,
+ *   and then replaces p in the stress.
+ * Network: 16 -> 64 -> 64 -> 16, ReLU, BF16 weights/activations, FP32
+ * accumulation on MFMA; unused inputs and outputs are zero.
  *
  * Parameter vector (FP32, MPM_MLP_NPARAMS values):
  *   [w1 64x16 | b1 64 | w2 64x64 | b2 64 | w3 16x64 | b3 16], weights [out][in].
@@ -56,6 +66,7 @@ typedef struct mpm_surrogate_stats {
     int64_t fallbacks;   /* of those, rejected by the residual check */
     float loss;          /* final training loss (normalised units) */
     float tol;
+    int64_t offered;     /* states selected for the log since collect (samples = min(offered, capacity)) */
 } mpm_surrogate_stats;
 
 /* Standalone network calls on host buffers (device memory is allocated per
@@ -70,8 +81,10 @@ int mpm_mlp_gradient(const float *params, const float *scaling, const float *in,
 int mpm_mlp_train(float *params, const float *scaling, const float *in, const float *tgt, int64_t n,
                   const mpm_train_params *tp, float *loss_per_epoch);
 
-/* Engine hooks (FP32 engine only; MPM_ERR_UNSUPPORTED otherwise).
- * collect: while on, every mpm_step logs the J2 plastic trial states of a
+/* Engine hooks (FP32 engine with the tent functions and PIC / FLIP, and a J2
+ * or Tait material; MPM_ERR_UNSUPPORTED otherwise).
+ * collect: while on, every mpm_step logs the J2 plastic trial states (or the
+ * Tait states) of a
  * deterministic subset of particles (a hash of particle id and step below
  * `rate`) into a device buffer of `capacity` samples. on = 0 stops logging
  * and keeps the samples; a new on = 1 discards them. */
@@ -79,8 +92,8 @@ int mpm_surrogate_collect(mpm_engine *e, int32_t on, int64_t capacity, double ra
 /* Sorts the samples by (step, particle id), derives the scaling, initialises
  * and trains the network on the GPU. */
 int mpm_surrogate_fit(mpm_engine *e, const mpm_train_params *tp, float *final_loss);
-/* Route J2 plastic trial states through the network (on = 1) with residual
- * tolerance tol; on = 0 restores the exact update. */
+/* Route J2 plastic trial states (or Tait pressures) through the network
+ * (on = 1) with tolerance tol; on = 0 restores the exact update. */
 int mpm_surrogate_enable(mpm_engine *e, int32_t on, double tol);
 int mpm_surrogate_stats_get(mpm_engine *e, mpm_surrogate_stats *out);
 int mpm_surrogate_reset_counters(mpm_engine *e);

@@ -75,12 +75,14 @@ def test_failing_scene_is_named():
     ens.close()
 
 
-def test_atlas_matches_individual_scenes():
+@pytest.mark.parametrize("per_scene_dt", [False, True])
+def test_atlas_matches_individual_scenes(per_scene_dt):
     # scenes of one grid size packed into one engine give, scene by scene,
-    # the bits of separate engines run with the same time step
+    # the bits of separate engines run with the same time step: the atlas's
+    # smallest, or (per-material steps) each scene's own
     members = [scenes.cfg4_scene(k) for k in (4, 7, 8)]   # 128 x 128 grids
-    a, layout = scenes.atlas(members)
-    print("atlas %s n=%d dt=%.3e" % (a.nelem, a.n, a.dt), flush=True)
+    a, layout = scenes.atlas(members, per_scene_dt=per_scene_dt)
+    print("atlas %s n=%d dt=%.3e per-scene %s" % (a.nelem, a.n, a.dt, per_scene_dt), flush=True)
     e = capi.Engine(a, precision=capi.PREC_FP32, relayout_every=16)
     e.upload(a.parts)
     e.set_option(capi.OPT_LANES_PER_CELL, 2)
@@ -88,7 +90,10 @@ def test_atlas_matches_individual_scenes():
     got = e.download(a.n)
     e.close()
     for sc, (first, cnt, origin) in zip(members, layout):
-        sc.dt = a.dt
+        if per_scene_dt:
+            sc.mat_dt = [sc.dt]
+        else:
+            sc.dt = a.dt
         s1 = capi.Engine(sc, precision=capi.PREC_FP32, relayout_every=16)
         s1.upload(sc.parts)
         s1.set_option(capi.OPT_LANES_PER_CELL, 2)

@@ -1,0 +1,167 @@
+"""The configured workloads at their full sizes on the GPU.
+
+- cfg2, the 3D Neo-Hookean cube on 128^3 cells, both point counts (4,194,304
+  at 2x4x2 points per cell and 2,097,152 at the stated 2x2x2): the FP64-exact
+  engine against the scalar oracle over the first steps, and the FP32 engine
+  against the FP64-exact engine after the configured 1000 steps (position,
+  stress, velocity, det F and node momentum errors bounded);
+- cfg3, the J2 column collapse on 192x96x96 cells (2,430,000 points): the
+  FP64-exact engine bit for bit against the oracle (plastic return mapping
+  included), and FP32 against FP64 over the 200 steps in which the surrogate's
+  samples are logged;
+- cfg5, the Tait dam break on 256x128x128 cells (4,194,304 points): FP32
+  against FP64 over the first 100 steps, as configured, and the FP64-exact
+  engine against the oracle;
+- cfg4, the 256-scene ensemble: every scene of the three atlas engines, each
+  advancing by its own CFL time step, against the oracle run of that scene
+  alone.
+
+The FP64-exact engine equals the oracle bit for bit where no libm function is
+on the path (linear elastic, J2); Neo-Hookean and Tait call log / expm1 /
+log1p, which the device library rounds differently in the last bits, so
+those comparisons carry ulp-scale tolerances. FP32 bounds are written per
+test.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from mpm_mi355x import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run(scene, prec, steps, relayout=0):
+    e = capi.Engine(scene, precision=prec, relayout_every=relayout)
+    e.upload(scene.parts)
+    assert e.step(1, steps) == 0
+    p, nd = e.download(scene.n), e.nodes()
+    e.close()
+    return p, nd
+
+
+def fp32_errors(scene, steps):
+    """FP32 engine against the FP64-exact engine after `steps` steps."""
+    print("%s: %d particles, %d steps, FP64-exact" % (scene.name, scene.n, steps), flush=True)
+    p64, n64 = run(scene, capi.PREC_FP64_EXACT, steps)
+    print("%s: FP32" % scene.name, flush=True)
+    p32, n32 = run(scene, capi.PREC_FP32, steps, relayout=32)
+    err = {
+        "position": float(np.max(np.abs(p32.coord - p64.coord))),
+        "stress": rel(p32.stress, p64.stress),
+        "velocity": rel(p32.vel, p64.vel),
+        "J": float(np.max(np.abs(p32.det_def_grad - p64.det_def_grad))),
+        "node_momentum": rel(n32.nmome, n64.nmome),
+    }
+    print("  " + ", ".join("%s %.2e" % kv for kv in err.items()), flush=True)
+    return err, p32, p64
+
+
+def oracle_steps(scene, steps):
+    print("%s: oracle, %d steps" % (scene.name, steps), flush=True)
+    o = pyoracle.Oracle(scene)
+    assert o.run(1, steps) == 0
+    print("%s: FP64-exact engine" % scene.name, flush=True)
+    got, nd = run(scene, capi.PREC_FP64_EXACT, steps)
+    return o, got, nd
+
+
+# ------------------------------------------------------------------ cfg2
+@pytest.mark.parametrize("ppc", [(2, 4, 2), (2, 2, 2)])
+def test_cfg2_fp64_engine_against_oracle(ppc):
+    s = scenes.cfg2(ppc=ppc)
+    o, got, nd = oracle_steps(s, 2)
+    assert np.array_equal(nd.nmass, o.nmass)
+    np.testing.assert_allclose(got.coord, o.parts.coord, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(got.vel, o.parts.vel, rtol=0, atol=1e-12 * np.abs(o.parts.vel).max())
+    np.testing.assert_allclose(got.stress, o.parts.stress, rtol=0, atol=1e-10 * np.abs(o.parts.stress).max())
+    assert np.array_equal(got.def_grad, o.parts.def_grad) or np.allclose(got.def_grad, o.parts.def_grad,
+                                                                           rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("ppc", [(2, 4, 2), (2, 2, 2)])
+def test_cfg2_fp32_after_1000_steps(ppc):
+    """The north-star accuracy statement: positions within 1e-5 of the
+    domain after the 1000 configured steps; stress within 1e-4 relative,
+    velocity and node momentum within 1e-5, det F within 1e-6."""
+    s = scenes.cfg2(ppc=ppc)
+    err, _, _ = fp32_errors(s, 1000)
+    assert err["position"] < 1e-5
+    assert err["stress"] < 1e-4
+    assert err["velocity"] < 1e-5 and err["node_momentum"] < 1e-5
+    assert err["J"] < 1e-6
+
+
+# ------------------------------------------------------------------ cfg3
+def test_cfg3_fp64_engine_bitexact_against_oracle():
+    s = scenes.cfg3()
+    o, got, nd = oracle_steps(s, 3)
+    for f in ("coord", "vel", "stress", "def_grad", "det_def_grad", "vol", "eps_p"):
+        assert np.array_equal(getattr(got, f), getattr(o.parts, f)), f
+    assert np.array_equal(nd.nmass, o.nmass)
+
+
+def test_cfg3_fp32_over_the_logging_window():
+    """200 steps of the J2 column (the steps whose plastic states train the
+    surrogate): FP32 against FP64."""
+    s = scenes.cfg3()
+    err, p32, p64 = fp32_errors(s, 200)
+    ep = rel(p32.eps_p, p64.eps_p)
+    print("  eps_p %.2e (max %.3e)" % (ep, float(p64.eps_p.max())), flush=True)
+    assert p64.eps_p.max() > 0.0
+    assert err["position"] < 1e-5
+    assert err["stress"] < 1e-3
+    assert err["velocity"] < 1e-4 and err["node_momentum"] < 1e-4
+    assert ep < 1e-2
+
+
+# ------------------------------------------------------------------ cfg5
+def test_cfg5_fp32_first_100_steps():
+    s = scenes.cfg5()
+    err, _, _ = fp32_errors(s, 100)
+    assert err["position"] < 1e-5
+    assert err["stress"] < 1e-3
+    assert err["velocity"] < 1e-4 and err["node_momentum"] < 1e-4
+    assert err["J"] < 1e-6
+
+
+def test_cfg5_fp64_engine_against_oracle():
+    s = scenes.cfg5()
+    o, got, nd = oracle_steps(s, 2)
+    assert np.array_equal(nd.nmass, o.nmass)
+    np.testing.assert_allclose(got.coord, o.parts.coord, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(got.stress, o.parts.stress, rtol=0, atol=1e-10 * np.abs(o.parts.stress).max())
+
+
+# ------------------------------------------------------------------ cfg4
+def test_cfg4_every_scene_against_the_oracle():
+    """All 256 scenes in their three atlas engines, each scene at its own CFL
+    time step, 20 steps, against the oracle run of the scene alone. Bounds:
+    position 1e-7 of the unit domain, stress 1e-3 relative, velocity 1e-4."""
+    steps = 20
+    sc, _ = scenes.cfg4(256)
+    worst = {"position": 0.0, "stress": 0.0, "velocity": 0.0}
+    checked = 0
+    for a, layout, idx in scenes.cfg4_atlases(256):
+        print("%s: %d scenes, %d particles" % (a.name, len(idx), a.n), flush=True)
+        e = capi.Engine(a, precision=capi.PREC_FP32, relayout_every=16)
+        e.upload(a.parts)
+        assert e.step(1, steps) == 0
+        got = e.download(a.n)
+        e.close()
+        for k, (first, cnt, origin) in zip(idx, layout):
+            o = pyoracle.Oracle(sc[k])
+            assert o.run(1, steps) == 0
+            sl = slice(first, first + cnt)
+            errs = {"position": float(np.max(np.abs(got.coord[sl] - np.asarray(origin) - o.parts.coord))),
+                    "stress": rel(got.stress[sl], o.parts.stress), "velocity": rel(got.vel[sl], o.parts.vel)}
+            for f, v in errs.items():
+                worst[f] = max(worst[f], v)
+            assert errs["position"] < 1e-7 and errs["stress"] < 1e-3 and errs["velocity"] < 1e-4, (sc[k].name, errs)
+            checked += 1
+    print("  %d scenes; worst %s" % (checked, ", ".join("%s %.2e" % kv for kv in worst.items())), flush=True)
+    assert checked == 256

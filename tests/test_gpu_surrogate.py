@@ -205,3 +205,82 @@ def test_network_alone_error_is_measured(trained):
     assert st["fallbacks"] == 0 and st["evaluated"] > 0
     assert np.all(np.isfinite(b.stress))
     assert err < 2.0
+
+
+# ------------------------------------------------- Tait pressure closure (cfg5)
+TAIT = scenes.TAIT_PROPS  # rho0, c, gamma, mu
+
+
+@pytest.fixture(scope="module")
+def tait_trained():
+    """The network trained as a pressure closure p(J, rate of volume) on the
+    states of a small dam break (the Tait material selects the closure)."""
+    s = scenes.tiny_tait(n_cells=16)
+    e = fp32_engine(s)
+    e.surrogate_collect(True, capacity=1 << 19, rate=1.0)
+    assert e.step(1, 80) == 0
+    st = e.surrogate_stats()
+    loss = e.surrogate_fit(epochs=60, batch=2048, lr=3e-3, lr_final=3e-5, seed=3)
+    xin, tgt = e.surrogate_samples(1 << 19)
+    params, scaling = e.surrogate_params()
+    e.close()
+    print("tait closure: %d samples, loss %.3g" % (st["samples"], loss), flush=True)
+    return {"loss": loss, "x": xin, "t": tgt, "params": params, "scaling": scaling, "samples": st["samples"]}
+
+
+def test_tait_samples_follow_the_equation_of_state(tait_trained):
+    """Logged rows: inputs J - 1 and the volumetric rate, target the exact
+    pressure B ((1/J)^gamma - 1), B = rho0 c^2 / gamma (closed form in FP64;
+    the kernel evaluates it in FP32 without cancellation)."""
+    x, t = tait_trained["x"], tait_trained["t"]
+    rho0, c, gam, _ = TAIT
+    B = rho0 * c * c / gam
+    J = 1.0 + x[:, 0].astype(np.float64)
+    p = B * (J ** (-gam) - 1.0)
+    assert len(x) == tait_trained["samples"] > 10000
+    assert np.max(np.abs(t[:, 0] - p)) <= 1e-4 * max(np.max(np.abs(p)), 1e-6 * B)
+    assert np.all(t[:, 1:] == 0.0) and np.all(x[:, 2:] == 0.0) and np.all(np.isfinite(x[:, 1]))
+
+
+def tait_pair(params, scaling, tol, nsteps=60):
+    s = scenes.tiny_tait(n_cells=16)
+    plain = fp32_engine(s)
+    assert plain.step(1, nsteps) == 0
+    a = plain.download(s.n)
+    plain.close()
+    sur = fp32_engine(s)
+    sur.surrogate_set_params(params, scaling)
+    sur.surrogate_enable(True, tol)
+    assert sur.step(1, nsteps) == 0
+    b = sur.download(s.n)
+    st = sur.surrogate_stats()
+    sur.close()
+    return a, b, st
+
+
+def test_tait_closure_tolerance_zero_is_the_plain_method(tait_trained):
+    a, b, st = tait_pair(tait_trained["params"], tait_trained["scaling"], 0.0)
+    assert st["evaluated"] > 0 and st["fallbacks"] == st["evaluated"]
+    for f in ("coord", "vel", "stress", "def_grad"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+@pytest.mark.parametrize("tol", [1e-3, 1e-2])
+def test_tait_closure_stays_within_its_tolerance(tait_trained, tol):
+    """The exact Tait pressure judges every prediction: accepted pressures
+    are within tol of it (relative), so the stress error against the plain
+    method stays at the tolerance's level; the trained network is accepted
+    for a share of the particles that an untrained one never reaches."""
+    a, b, st = tait_pair(tait_trained["params"], tait_trained["scaling"], tol)
+    err = np.linalg.norm(b.stress - a.stress) / np.linalg.norm(a.stress)
+    acc = 1.0 - st["fallbacks"] / max(1, st["evaluated"])
+    print("tait tol %g: evaluated %d, accepted %.4f, rel stress err %.3g" % (tol, st["evaluated"], acc, err),
+          flush=True)
+    assert st["evaluated"] > 0
+    assert err <= 3 * tol
+    rng = np.random.default_rng(5)
+    untrained = (0.05 * rng.standard_normal(len(tait_trained["params"]))).astype(np.float32)
+    _, _, st0 = tait_pair(untrained, tait_trained["scaling"], tol)
+    acc0 = 1.0 - st0["fallbacks"] / max(1, st0["evaluated"])
+    print("  untrained network accepted %.4f" % acc0, flush=True)
+    assert acc > acc0
