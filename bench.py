@@ -69,7 +69,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="cfg2-4M", choices=["cfg1", "cfg2", "cfg2-4M", "cfg3", "cfg5"])
+    ap.add_argument("--config", default="cfg2-4M",
+                    choices=["cfg1", "cfg2", "cfg2-4M", "cfg2-q", "cfg3", "cfg5", "cfg5-16M"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--relayout", type=int, default=32)
     ap.add_argument("--fused-count", type=int, default=1, help="count cells in the update kernel (1) or not (0)")
@@ -91,7 +92,12 @@ def parse():
     ap.add_argument("--sur-epochs", type=int, default=40)
     ap.add_argument("--sur-batch", type=int, default=4096)
     ap.add_argument("--sur-tol", type=float, default=1e-4, help="the stated fallback tolerance")
-    ap.add_argument("--sur-sweep", default="1e-3,1e-2,1e30", help="further tolerances reported")
+    ap.add_argument("--sur-sweep", default="3e-4,1e-3,3e-3,1e-2,3e-2,1e-1,1e30", help="further tolerances reported")
+    ap.add_argument("--no-closure", action="store_true", help="skip the cfg5 pressure-closure measurement")
+    ap.add_argument("--closure-collect", type=int, default=100)
+    ap.add_argument("--closure-steps", type=int, default=100)
+    ap.add_argument("--no-long-run", action="store_true",
+                    help="skip the mean over the configured 1000 steps and the figure after 1500 steps")
     return ap.parse_args()
 
 
@@ -102,8 +108,12 @@ def make_scene(name):
         return scenes.cfg3()
     if name == "cfg5":
         return scenes.cfg5()
+    if name == "cfg5-16M":
+        return scenes.cfg5(ppc=(4, 2, 4))
     if name == "cfg2":
         return scenes.cfg2(ppc=(2, 2, 2))
+    if name == "cfg2-q":  # the cfg2 geometry on the quadratic B-spline (27 control points per particle)
+        return scenes.cfg2(ppc=(2, 2, 2), idimc=(2, 2, 2))
     return scenes.cfg2(ppc=(2, 4, 2))
 
 
@@ -126,9 +136,10 @@ def cpu_baseline(scene, nsteps):
     import pyoracle
 
     try:
-        threads = min(16, len(os.sched_getaffinity(0)))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        threads = min(16, os.cpu_count() or 1)
+        affinity = os.cpu_count() or 1
+    threads = min(16, affinity)
     o = pyoracle.Oracle(scene, fast="omp")
     t0 = time.perf_counter()
     rc = o.run_omp(1, nsteps, threads)
@@ -143,7 +154,7 @@ def cpu_baseline(scene, nsteps):
     return {"value": scene.n * nsteps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": "%s, %d particles, %d steps, FP64 OpenMP port of the step (cell colouring), %d threads, "
                       "%.1f s" % (scene.name, scene.n, nsteps, threads, dt),
-            "cpu_model": cpu_model(),
+            "cpu_model": cpu_model(), "host_cores": os.cpu_count(), "cores_available_to_this_process": affinity,
             "single_thread": None if rc1 else {"value": scene.n / dt1, "steps": 1, "seconds": round(dt1, 2)}}
 
 
@@ -155,27 +166,42 @@ def timed_steps(eng, istep0, nsteps):
     return time.perf_counter() - t0
 
 
-def surrogate_vs_plain(a):
-    """J2 column (192x96x96, 2.43 M particles): the plain method logs plastic
-    trial states for the first steps, the network is fitted on the GPU, then
-    both methods advance the same state; stress error is that of the
-    surrogate run against the plain run (Frobenius over all particles)."""
-    scene = scenes.cfg3()
+def sample_rate(scene, a, collect_steps, target):
+    """Logging rate that makes `target` samples over the first collect_steps
+    steps: a counting pass (log capacity 1, every candidate offered) gives
+    the number of candidate states (J2 plastic trial states, or Tait
+    states) over those steps."""
+    e = capi.Engine(scene, precision=capi.PREC_FP32, relayout_every=a.relayout)
+    e.upload(scene.parts)
+    e.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
+    e.surrogate_collect(True, capacity=1, rate=1.0)
+    capi.check(e.step(1, collect_steps))
+    offered = e.surrogate_stats()["offered"]
+    e.close()
+    return min(1.0, target / max(1.0, float(offered))), offered
+
+
+def surrogate_study(a, scene, collect_steps, timed_steps_n, tols, epochs, seed):
+    """Plain steps log candidate states (rate set so the log holds
+    a.sur_samples of them), the network is fitted on the GPU, then the plain
+    method and the surrogate at each tolerance advance the same state; the
+    stress error is that of the surrogate run against the plain run
+    (Frobenius over all particles)."""
     n = scene.n
+    rate, offered = sample_rate(scene, a, collect_steps, a.sur_samples)
     eng = capi.Engine(scene, precision=capi.PREC_FP32, relayout_every=a.relayout)
     eng.upload(scene.parts)
     eng.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
-    rate = min(1.0, a.sur_samples / float(n * a.sur_collect))
     eng.surrogate_collect(True, capacity=a.sur_samples, rate=rate)
-    t_collect = timed_steps(eng, 1, a.sur_collect)
+    t_collect = timed_steps(eng, 1, collect_steps)
     t0 = time.perf_counter()
-    loss = eng.surrogate_fit(epochs=a.sur_epochs, batch=a.sur_batch, lr=2e-3, lr_final=5e-5, seed=1)
+    loss = eng.surrogate_fit(epochs=epochs, batch=a.sur_batch, lr=2e-3, lr_final=5e-5, seed=seed)
     t_fit = time.perf_counter() - t0
     st = eng.surrogate_stats()
     params, scaling = eng.surrogate_params()
     state = eng.download(n)
     eng.close()
-    i0 = 1 + a.sur_collect
+    i0 = 1 + collect_steps
 
     def advance(tol):
         e = capi.Engine(scene, precision=capi.PREC_FP32, relayout_every=a.relayout)
@@ -185,36 +211,72 @@ def surrogate_vs_plain(a):
             e.surrogate_set_params(params, scaling)
             e.surrogate_enable(True, tol)
         e.set_timing(True)
-        wall = timed_steps(e, i0, a.sur_steps)
+        wall = timed_steps(e, i0, timed_steps_n)
         kt = e.kernel_times()
         ss = e.surrogate_stats() if tol is not None else None
         out = e.download(n)
         e.close()
-        upd = round(1e3 * kt["g2p_upd_ms"] / a.sur_steps, 2)
-        if ss is not None:
-            ss["update_us"] = upd
-        return n * a.sur_steps / wall, ss, out.stress, upd
+        upd = round(1e3 * kt["g2p_upd_ms"] / timed_steps_n, 2)
+        return n * timed_steps_n / wall, ss, out.stress, upd
 
     plain_rate, _, sp, plain_upd = advance(None)
     runs = []
-    tols = [a.sur_tol] + [float(t) for t in a.sur_sweep.split(",") if t]
     for tol in tols:
-        rate_s, ss, su, _ = advance(tol)
+        rate_s, ss, su, upd = advance(tol)
         runs.append({"tol": tol, "particle_steps_per_s": rate_s, "ratio": rate_s / plain_rate,
                      "rel_stress_err": float(np.linalg.norm(su - sp) / np.linalg.norm(sp)),
                      "max_abs_stress_err": float(np.max(np.abs(su - sp))), "evaluated": ss["evaluated"],
-                     "fallback_rate": ss["fallbacks"] / max(1, ss["evaluated"]), "update_us": ss["update_us"]})
-    head = runs[0]
+                     "fallback_rate": ss["fallbacks"] / max(1, ss["evaluated"]), "update_us": upd})
+    # the calibrated tolerance: the largest whose stress error stays within
+    # 1e-3 of the plain method (lowest fallback rate at that error bound)
+    ok = [r for r in runs if r["rel_stress_err"] <= 1e-3]
+    cal = max(ok, key=lambda r: r["tol"]) if ok else None
     return {
         "workload": scene.name, "particles": n, "grid": "x".join(map(str, scene.nelem)),
-        "plain_particle_steps_per_s": plain_rate, "surrogate_particle_steps_per_s": head["particle_steps_per_s"],
-        "plain_update_us": plain_upd, "surrogate_update_us": head["update_us"],
-        "ratio": head["ratio"], "rel_stress_err": head["rel_stress_err"], "tol": head["tol"],
-        "fallback_rate": head["fallback_rate"], "sweep": runs[1:],
-        "samples": st["samples"], "sample_rate": rate, "collect_steps": a.sur_collect,
-        "collect_s": round(t_collect, 3), "fit_s": round(t_fit, 3), "fit_epochs": a.sur_epochs,
-        "fit_batch": a.sur_batch, "fit_loss": loss, "timed_steps": a.sur_steps,
+        "plain_particle_steps_per_s": plain_rate, "plain_update_us": plain_upd,
+        "candidates_in_collect_window": offered, "samples": st["samples"], "sample_rate": rate,
+        "collect_steps": collect_steps, "collect_s": round(t_collect, 3), "fit_s": round(t_fit, 3),
+        "fit_epochs": epochs, "fit_batch": a.sur_batch, "fit_loss": loss, "timed_steps": timed_steps_n,
+        "runs": runs, "calibrated": cal,
     }
+
+
+def surrogate_vs_plain(a):
+    """J2 column (192x96x96, 2.43 M particles): the surrogate for the plastic
+    corrector against the plain return mapping, at the stated tolerance and
+    a sweep."""
+    tols = [a.sur_tol] + [float(t) for t in a.sur_sweep.split(",") if t]
+    r = surrogate_study(a, scenes.cfg3(), a.sur_collect, a.sur_steps, tols, a.sur_epochs, 1)
+    head = r["runs"][0]
+    r.update({"surrogate_particle_steps_per_s": head["particle_steps_per_s"], "ratio": head["ratio"],
+              "rel_stress_err": head["rel_stress_err"], "tol": head["tol"], "fallback_rate": head["fallback_rate"],
+              "surrogate_update_us": head["update_us"]})
+    return r
+
+
+def closure_vs_plain(a):
+    """cfg5 dam break (4.19 M particles): the learned pressure closure
+    p(J, rate) against the exact Tait pressure, which also judges it."""
+    tols = [1e-3, 3e-3, 1e-2, 3e-2, 1e30]
+    return surrogate_study(a, scenes.cfg5(), a.closure_collect, a.closure_steps, tols, a.sur_epochs, 2)
+
+
+def long_run(a, scene_name):
+    """Mean ms per step over the configured 1000 steps from the initial
+    state (relayouts included), and the rate after 1500 steps, when the
+    particles have left their initial lattice."""
+    sc = make_scene(scene_name)
+    e = capi.Engine(sc, precision=capi.PREC_FP32, relayout_every=a.relayout)
+    e.upload(sc.parts)
+    e.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
+    t1000 = timed_steps(e, 1, 1000)
+    capi.check(e.step(1001, 500))
+    t_late = timed_steps(e, 1501, 300)
+    e.close()
+    return {"workload": sc.name, "steps_1_1000_ms_per_step": 1e3 * t1000 / 1000,
+            "steps_1_1000_particle_steps_per_s": sc.n * 1000 / t1000,
+            "steps_1501_1800_ms_per_step": 1e3 * t_late / 300,
+            "steps_1501_1800_particle_steps_per_s": sc.n * 300 / t_late}
 
 
 def accuracy_run(a, scene_name):
@@ -259,16 +321,19 @@ def ensemble_run(a):
     ens = capi.Ensemble([x[0] for x in atl], bucket=list(range(len(atl))), precision=capi.PREC_FP32,
                         relayout_every=16)
     t_setup = time.perf_counter() - t_setup
+    # 500 steps: 2, the autotuner's 9 x 4, then eager steps up to the first
+    # whole number of 32-step graph blocks that ends at step 500
+    block = 32
     step = 1
     ens.step(step, 2)
     step += 2
     tune = ens.autotune(step, trial_steps=4)
     step += 9 * 4
-    t_eager = ens.step(step, 32)
-    step += 32
-    block = 32
+    n_eager = (a.ens_steps - (step - 1)) % block or block
+    t_eager = ens.step(step, n_eager)
+    step += n_eager
     ens.capture(step, block)
-    nlaunch = max(1, (a.ens_steps - (step - 1)) // block)
+    nlaunch = (a.ens_steps - (step - 1)) // block
     t_graph = ens.launch(nlaunch)
     step += nlaunch * block
     ens.close()
@@ -276,7 +341,8 @@ def ensemble_run(a):
     for r, (sc, lay, idx) in zip(tune, atl):
         G = int(sc.name.rsplit("-", 1)[1])
         per_bucket.append({"grid": "%dx%d" % (G, G),
-                           "scenes": len(idx), "particles": sc.n, "atlas_cells": sc.nelem_total, "dt": sc.dt,
+                           "scenes": len(idx), "particles": sc.n, "atlas_cells": sc.nelem_total,
+                           "dt_per_scene_range": [min(sc.mat_dt), max(sc.mat_dt)] if sc.mat_dt else [sc.dt, sc.dt],
                            "lanes_per_cell": r["lanes_per_cell"], "relayout_every": r["relayout_every"],
                            "particle_steps_per_s_eager": sc.n / (r["ms_per_step"] * 1e-3),
                            "autotune_gain": r["ms_per_step_worst"] / r["ms_per_step"]})
@@ -291,8 +357,9 @@ def ensemble_run(a):
             "layout": "one atlas engine per grid-size bucket", "setup_s": round(t_setup, 2),
             "graph_block_steps": block, "graph_launches": nlaunch,
             "particle_steps_per_s_graph": ntot * nlaunch * block / (t_graph * 1e-3),
-            "particle_steps_per_s_eager": ntot * 32 / (t_eager * 1e-3),
-            "ms_per_step_graph": t_graph / (nlaunch * block), "ms_per_step_eager": t_eager / 32,
+            "particle_steps_per_s_eager": ntot * n_eager / (t_eager * 1e-3),
+            "ms_per_step_graph": t_graph / (nlaunch * block), "ms_per_step_eager": t_eager / n_eager,
+            "time_step": "each scene its own CFL dt (per-material time steps)",
             "separate_engines_particle_steps_per_s_graph": ntot * 64 / (t_sep * 1e-3),
             "per_bucket": per_bucket}
 
@@ -367,9 +434,15 @@ def main():
     ens = None
     if not a.no_ensemble:
         ens = ensemble_run(a)
+    clo = None
+    if not a.no_closure:
+        clo = closure_vs_plain(a)
     acc = None
     if a.accuracy_steps > 0:
         acc = accuracy_run(a, a.config)
+    lr = None
+    if not a.no_long_run and prec == capi.PREC_FP32:
+        lr = long_run(a, a.config)
 
     cpu = None
     if not a.no_cpu:
@@ -391,15 +464,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": scene.name, "particles": n, "grid": "x".join(map(str, scene.nelem)),
                    "active_cells": kt["active_cells"], "active_nodes": nact, "method": "plain (no surrogate)",
-                   "scheme": "PIC/MUSL", "relayout_every": a.relayout, "graph_block": a.graph_block,
+                   "scheme": "PIC/MUSL", "shape_function": "quadratic B-spline %s" % (scene.idimc,)
+                   if tuple(scene.idimc) != (1, 1, 1) else "linear tent", "relayout_every": a.relayout,
+                   "graph_block": a.graph_block,
                    "kernel_timing_steps": kt_steps},
         "roofline": roofline,
         "phases": phases,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "surrogate": sur,
+        "closure": clo,
         "ensemble": ens,
         "accuracy": acc,
+        "long_run": lr,
     }
     print(json.dumps(out), flush=True)
 

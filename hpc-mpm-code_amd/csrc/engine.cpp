@@ -343,6 +343,7 @@ int do_bin(mpm_engine *e) {
 }
 
 int do_p2g(mpm_engine *e, double facl, int diag, hipEvent_t mid = nullptr) {
+    if (e->shifting) diag = 1;  // the shifting correction reads the P2G nodal volume
     if (e->prec == MPM_PREC_FP64_EXACT) {
         launch_f64_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.scheme, e->prm.bb[0] * facl,
                        e->prm.bb[1] * facl, e->prm.bb[2] * facl, e->prm.mtol, facl);
@@ -368,20 +369,28 @@ int do_solve(mpm_engine *e) {
 // Surrogate queue rows for n particles: one region of kSurRegion rows per update block.
 static int64_t sur_queue_rows(int64_t n) { return (int64_t)cdiv((int)n, kSurRegion) * kSurRegion; }
 
+// Particle-sized buffers of the shifting correction (gradient by particle
+// or slot, per-block sums), sized for the current upload.
+int shift_buffers(mpm_engine *e, ShiftF64 *sh) {
+    if (e->geup_n != e->n) {
+        for (double **p : {&e->d_geup, &e->d_part_dot}) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+        MPM_HIP_CHECK(hipMalloc((void **)&e->d_geup, 3 * (size_t)e->ld * sizeof(double)));
+        MPM_HIP_CHECK(hipMalloc((void **)&e->d_part_dot, (size_t)cdiv(e->n, kShiftBlock) * sizeof(double)));
+        e->geup_n = e->n;
+    }
+    *sh = ShiftF64{e->d_nvol, e->d_nei, e->d_geup, e->d_part_eu, e->d_part_dot, e->d_b0};
+    return 0;
+}
+
 int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2 = nullptr) {
     if (e->prec == MPM_PREC_FP64_EXACT) {
         ShiftF64 sh{};
         if (e->shifting) {
-            if (e->geup_n != e->n) {
-                for (double **p : {&e->d_geup, &e->d_part_dot}) {
-                    if (*p) (void)hipFree(*p);
-                    *p = nullptr;
-                }
-                MPM_HIP_CHECK(hipMalloc((void **)&e->d_geup, 3 * (size_t)e->ld * sizeof(double)));
-                MPM_HIP_CHECK(hipMalloc((void **)&e->d_part_dot, (size_t)cdiv(e->n, kShiftBlock) * sizeof(double)));
-                e->geup_n = e->n;
-            }
-            sh = ShiftF64{e->d_nvol, e->d_nei, e->d_geup, e->d_part_eu, e->d_part_dot, e->d_b0};
+            int rc = shift_buffers(e, &sh);
+            if (rc) return rc;
         }
         launch_f64_g2p(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->mt, e->prm.scheme, e->prm.dt,
                        e->prm.mtol, facl, e->d_err, e->shifting ? &sh : nullptr, e->fbar ? e->d_njbar : nullptr);
@@ -390,6 +399,14 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         StepConsts c = consts(e, facl);
         launch_s32_g2p_vel_musl(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, c,
                                 e->partials, diag, mid2);
+        if (e->shifting) {
+            ShiftF64 sh{};
+            int rc = shift_buffers(e, &sh);
+            if (rc) return rc;
+            launch_s32_shift(e->stream, e->g, e->P32[e->cur], e->N32, e->T, sh, e->prm.mtol);
+            c.sh_geup = sh.geup;
+            c.sh_b0 = sh.b0;
+        }
         if (mid) MPM_HIP_CHECK(hipEventRecord(mid, e->stream));
         if (!e->fused_count) e->clear_counts();
         BinFuse bf{e->fused_count ? e->bb.cnt : nullptr, e->counts_ready ? 1 : 0, e->bb.bsum, e->bb.tiles,
@@ -403,6 +420,14 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
         StepConsts c = consts(e, facl);
         launch_f32_g2p_vel_musl(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, c,
                                 e->partials, diag, mid2, &e->mt);
+        if (e->shifting) {
+            ShiftF64 sh{};
+            int rc = shift_buffers(e, &sh);
+            if (rc) return rc;
+            launch_s32_shift(e->stream, e->g, e->P32[e->cur], e->N32, e->T, sh, e->prm.mtol);
+            c.sh_geup = sh.geup;
+            c.sh_b0 = sh.b0;
+        }
         if (mid) MPM_HIP_CHECK(hipEventRecord(mid, e->stream));
         SurrogateArgs sa{};
         auto &su = e->sur;
@@ -990,8 +1015,6 @@ int mpm_surrogate_download_samples(mpm_engine *e, float *in, float *tgt, int64_t
 
 int mpm_set_shifting(mpm_engine *e, int32_t on, const int32_t *npxye) try {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
-    if (on && e->prec != MPM_PREC_FP64_EXACT)
-        return set_error(MPM_ERR_UNSUPPORTED, "particle shifting runs in the FP64-exact engine only");
     e->drop_graph();
     if (!on) {
         e->shifting = false;

@@ -649,9 +649,11 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     // position: offset in cell units, carried into the cell index when it leaves [0, 1)
     int ci[3] = {ix, iy, iz};
     bool inside = true;
+    const double b0 = sc.sh_geup ? *sc.sh_b0 : 0.0;  // particle shifting: x += -b0 geup
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float ra = r[a] + (dtp * dxp[a]) * T.inv_dx[a];
+        if (sc.sh_geup) ra += (float)(-b0 * sc.sh_geup[a * ld + slot] * (double)T.inv_dx[a]);
         const float fl = floorf(ra);
         ra -= fl;  // exact for |fl| <= 1 (Sterbenz)
         ci[a] += (int)fl;
@@ -914,7 +916,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                     // network's; an accepted one replaces it in the stored stress
                     const float pex = r[14], B = r[15];
                     const float pnet = y[0] * scl[48] + scl[32];
-                    if (fabsf(pnet - pex) <= sa.tol * (fabsf(pex) + 1e-6f * B)) {
+                    if (fabsf(pnet - pex) < sa.tol * (fabsf(pex) + 1e-6f * B)) {
                         accept = true;
                         const float d = pex - pnet;
 #pragma unroll

@@ -242,6 +242,8 @@ struct StepConsts {
     int cells_prefetch;  // cells kernels request the next chunk's indices while a chunk loads
     int cells_grid;      // workgroups of the cells kernels (0 = automatic)
     int mat_dt;          // 1: each particle advances by its material's time step (MaterialDev::dt)
+    const double *sh_geup;  // particle shifting (or null): [3][ld] gradient of the excess volume by slot
+    const double *sh_b0;    // and the step length b0; the update adds -b0 geup to the position
 };
 
 // Grid spacing constants of the FP32 kernels.
@@ -279,6 +281,13 @@ void launch_s32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const No
 void launch_s32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                              const CellLists &L, const BcDense &bc, const NodeTables &T, const TileGeom &tg,
                              const StepConsts &c, float *partials, int diag, hipEvent_t mid = nullptr);
+// Particle shifting correction of the reference (DeltaCorrectionParticleShifting)
+// for the FP32 state, before the update kernel: excess nodal volume from the
+// P2G volume nvof (FP32) and the nodal volume of the filled grid (FP64), its
+// gradient at every slot's position (2 V sum grad N nei, FP64 sums over the
+// stencil), the two blocked global sums and b0 into S.
+void launch_s32_shift(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const NodeTables &T,
+                      const ShiftF64 &S, double mtol);
 void launch_s32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                            const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
                            const BinFuse &bf);

@@ -747,9 +747,11 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
     const double V = J * (double)P.V0[slot];
     int ci[3] = {ix, iy, iz};
     bool inside = true;
+    const double b0 = sc.sh_geup ? *sc.sh_b0 : 0.0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float ra = r[a] + (sc.dt * dxp[a]) * T.inv_dx[a];
+        if (sc.sh_geup) ra += (float)(-b0 * sc.sh_geup[a * ld + slot] * (double)T.inv_dx[a]);
         const float fl = floorf(ra);
         ra -= fl;
         ci[a] += (int)fl;
@@ -808,6 +810,83 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
     if (!(J > 0.0)) e |= kErrJacobian;
     if (!isfinite(r[0] + r[1] + r[2] + s[0] + s[1] + s[2])) e |= kErrNonFinite;
     if (e) atomicOr(err, e);
+}
+
+// ------------------------------------------------------ particle shifting
+// (module/material_point.cpp:174-227; global sums in blocks of kShiftBlock
+// consecutive entries, then over the blocks in order, as the FP64 engine)
+__global__ void ks_shift_nei(int nodec, const float *__restrict__ nvof, ShiftF64 S) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = b * kShiftBlock;
+    if (i0 >= nodec) return;
+    double acc = 0.0;
+    for (int i = i0; i < nodec && i < i0 + kShiftBlock; i++) {
+        const double e = (double)nvof[i] - S.nvol[i];
+        const double ne = e > 0.0 ? e : 0.0;
+        S.nei[i] = ne;
+        acc += ne * ne;
+    }
+    S.part_eu[b] = acc;
+}
+
+template <int NX, int NY, int NZ>
+__global__ void __launch_bounds__(256) ks_shift_geup(GridGeom g, PartF32 P, NodeTables T, ShiftF64 S) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= P.n) return;
+    const size_t ld = P.ld;
+    const uint32_t key = P.key[slot];
+    double gr[3] = {0.0, 0.0, 0.0};
+    if (key < (uint32_t)g.nelem) {
+        const int nexy = g.nex[0] * g.nex[1];
+        int rem, ix;
+        const int iz = fdivmod((int)key, nexy, T.inv_nexy, rem);
+        const int iy = fdivmod(rem, g.nex[0], T.inv_nex, ix);
+        AxisW<NX> Ax;
+        AxisW<NY> Ay;
+        AxisW<NZ> Az;
+        axis_weights<NX>(ix, P.r[slot], g.nex[0], Ax);
+        axis_weights<NY>(iy, P.r[ld + slot], g.nex[1], Ay);
+        axis_weights<NZ>(iz, P.r[2 * ld + slot], g.nex[2], Az);
+        const int n0 = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+#pragma unroll
+        for (int k = 0; k < NZ; k++)
+#pragma unroll
+            for (int j = 0; j < NY; j++)
+#pragma unroll
+                for (int i = 0; i < NX; i++) {
+                    const double ne = S.nei[n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1]];
+                    gr[0] += (double)(Ax.d[i] * T.inv_dx[0] * (Ay.w[j] * Az.w[k])) * ne;
+                    gr[1] += (double)(Ay.d[j] * T.inv_dx[1] * (Ax.w[i] * Az.w[k])) * ne;
+                    gr[2] += (double)(Az.d[k] * T.inv_dx[2] * (Ax.w[i] * Ay.w[j])) * ne;
+                }
+        const double s2 = 2.0 * (double)P.V32[slot];
+#pragma unroll
+        for (int c = 0; c < 3; c++) gr[c] *= s2;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) S.geup[c * ld + slot] = gr[c];
+}
+
+__global__ void ks_shift_dot(int n, size_t ld, ShiftF64 S) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = b * kShiftBlock;
+    if (i0 >= n) return;
+    double acc = 0.0;
+    for (int i = i0; i < n && i < i0 + kShiftBlock; i++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double v = S.geup[c * ld + i];
+            acc += v * v;
+        }
+    S.part_dot[b] = acc;
+}
+
+__global__ void ks_shift_b0(int nbe, int nbd, double mtol, ShiftF64 S) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double eu = 0.0, dot = 0.0;
+    for (int b = 0; b < nbe; b++) eu += S.part_eu[b];
+    for (int b = 0; b < nbd; b++) dot += S.part_dot[b];
+    *S.b0 = (dot < mtol) ? 0.0 : (eu / dot);
 }
 
 int cell_groups(const GridGeom &g) {
@@ -902,6 +981,22 @@ void launch_s32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
     case 2: musl_launch<3, 2, 3>(s, g, P, N, L, bc, T, tg, c, partials, diag, mid); break;
     default: break;
     }
+}
+
+void launch_s32_shift(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const NodeTables &T,
+                      const ShiftF64 &S, double mtol) {
+    if (P.n <= 0) return;
+    const int nbe = cdiv(g.nodec, kShiftBlock), nbd = cdiv(P.n, kShiftBlock);
+    hipLaunchKernelGGL(ks_shift_nei, dim3(cdiv(nbe, 64)), dim3(64), 0, s, g.nodec, (const float *)N.nvof, S);
+    const dim3 grid(cdiv(P.n, 256)), block(256);
+    switch (stencil_kind(g)) {
+    case 0: hipLaunchKernelGGL((ks_shift_geup<2, 2, 2>), grid, block, 0, s, g, P, T, S); break;
+    case 1: hipLaunchKernelGGL((ks_shift_geup<3, 3, 3>), grid, block, 0, s, g, P, T, S); break;
+    case 2: hipLaunchKernelGGL((ks_shift_geup<3, 2, 3>), grid, block, 0, s, g, P, T, S); break;
+    default: break;
+    }
+    hipLaunchKernelGGL(ks_shift_dot, dim3(cdiv(nbd, 64)), dim3(64), 0, s, P.n, P.ld, S);
+    hipLaunchKernelGGL(ks_shift_b0, dim3(1), dim3(64), 0, s, nbe, nbd, mtol, S);
 }
 
 void launch_s32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,

@@ -144,7 +144,7 @@ def cfg1(steps=1000, seed=0x5EED0001):
                  (0.0, 0.0, -9.81), bcs={0: _bc(sides), 1: _bc(allv), 2: _bc(bottom)}, parts=parts)
 
 
-def cfg2(steps=1000, ppc=(2, 2, 2), n_cells=128, seed=0x5EED0002):
+def cfg2(steps=1000, ppc=(2, 2, 2), n_cells=128, seed=0x5EED0002, idimc=(1, 1, 1)):
     """3D Neo-Hookean cube: 128^3 cells on the unit cube, cube
     [0.25,0.75]x[0,0.5]x[0.25,0.75]; ppc (2,2,2) gives 2,097,152 particles
     (geometry as stated), (2,4,2) gives 4,194,304. E=5e5, nu=0.35, rho=1000,
@@ -164,11 +164,13 @@ def cfg2(steps=1000, ppc=(2, 2, 2), n_cells=128, seed=0x5EED0002):
     vel[:, 2] = -omega * (coord[:, 0] - 0.5)
     vel += uniform(seed, 3 * n, -1e-3, 1e-3).reshape(n, 3)
     parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
-    floor = node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0)
+    floor = node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0, idimc)
     bcs = {0: _bc(floor), 1: _bc(floor), 2: _bc(floor)}
     name = "cfg2-3d-neohookean-%dppc" % int(np.prod(ppc))
+    if tuple(idimc) != (1, 1, 1):
+        name += "-bspline%d%d%d" % tuple(idimc)
     return Scene(name, xmin, xmax, nelem, [(capi.MODEL_NEOHOOKEAN, [5e5, 0.35])], rho, 5e-6, steps,
-                 (0.0, -9.81, 0.0), bcs=bcs, parts=parts)
+                 (0.0, -9.81, 0.0), bcs=bcs, parts=parts, idimc=tuple(idimc))
 
 
 def tiny(n_cells=8, ppc=(2, 2, 2), model=capi.MODEL_NEOHOOKEAN, props=(5e5, 0.35), scheme=capi.SCHEME_PIC,
@@ -254,17 +256,18 @@ def _walls(nelem, idimc=(1, 1, 1)):
     return {0: _bc(xw), 1: _bc(yw), 2: _bc(zw)}
 
 
-def cfg5(steps=2000, seed=0x5EED0005):
+def cfg5(steps=2000, seed=0x5EED0005, ppc=(2, 2, 2)):
     """Dam break, weakly compressible Tait fluid (rho0 = 1000, c = 30,
     gamma = 7, mu = 1e-3): 256x128x128 cells of h = 1/128 on [0,2]x[0,1]x[0,1];
     water column [0,0.5]x[0,0.5]x[0,1] = 64x64x128 cells at 2x2x2 points per
-    cell = 4,194,304 particles; gravity -y; free-slip walls; dt = 2e-5."""
+    cell = 4,194,304 particles (ppc (4,2,4): 32 per cell, 16,777,216, the
+    stated count); gravity -y; free-slip walls; dt = 2e-5."""
     h = 1.0 / 128.0
     nelem = (256, 128, 128)
     xmin, xmax = (0.0, 0.0, 0.0), (2.0, 1.0, 1.0)
-    coord = seed_block(xmin, (h, h, h), nelem, (0, 0, 0), (64, 64, 128), (2, 2, 2))
+    coord = seed_block(xmin, (h, h, h), nelem, (0, 0, 0), (64, 64, 128), ppc)
     n = len(coord)
-    volp = h ** 3 / 8.0
+    volp = h ** 3 / float(np.prod(ppc))
     rho = TAIT_PROPS[0]
     vel = uniform(seed, 3 * n, -1e-3, 1e-3).reshape(n, 3)
     parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))

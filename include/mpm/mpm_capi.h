@@ -254,7 +254,8 @@ enum {
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 /* Reference particle shifting (DeltaCorrectionParticleShifting,
  * module/material_point.cpp:174-227, applied in update_point_coord_and_stress
- * var_trans_solid_explicit.cpp:159-176), FP64-exact engine only. on = 1
+ * var_trans_solid_explicit.cpp:159-176), both engines (the FP32 engine forms
+ * the excess volume from its FP32 P2G volume and sums in FP64). on = 1
  * computes the nodal volume of the filled grid (MakNodalVol,
  * module/mesh.cpp:160-202) for npxye[3] points per cell and applies the
  * correction in every later step; on = 0 turns it off (the default). The two

@@ -50,7 +50,7 @@ int mpm_case_bc(const mpm_case *c, int32_t comp, const int32_t **nbc, const doub
 /* Writes griddata.txt / spdata.txt in the generator's formats. */
 int mpm_case_write(const mpm_case *c, const char *griddata_txt, const char *spdata_txt);
 /* Engine for the case: grid, materials, BCs and points uploaded; with
- * shifting = 1 (FP64-exact only) the reference's particle shifting is on,
+ * shifting = 1 the reference's particle shifting is on,
  * with the case's points per cell for the nodal volume. */
 int mpm_case_engine(const mpm_case *c, int32_t precision, int32_t relayout_every, int32_t shifting,
                     mpm_engine **out);

@@ -626,12 +626,33 @@ def test_fp64_shifting_run_bitexact(scheme, idimc):
     assert np.max(np.abs(plain.download(s.n).coord - got.coord)) > 1e-9
 
 
-def test_fp32_rejects_shifting():
-    s = scenes.tiny(n_cells=8)
-    e = engine(s, capi.PREC_FP32)
-    with pytest.raises(capi.MPMError) as ei:
-        e.set_shifting(True)
-    assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
+@pytest.mark.parametrize("scheme", [capi.SCHEME_PIC, capi.SCHEME_FLIP])
+@pytest.mark.parametrize("idimc", [(1, 1, 1), (2, 1, 2)])
+def test_fp32_shifting_run(scheme, idimc):
+    """The reference's particle shifting in the FP32 engine (nodal volume of
+    the filled grid in FP64, excess volume from the FP32 P2G volume, FP64
+    gradient and global sums, the correction added to the cell offsets), on
+    a squeezed block with relayouts, against the oracle over 20 steps.
+    Bounds: position 1e-7 of the domain, stress 1e-3 relative; the shift
+    itself (shifted minus unshifted positions) within 1e-3 relative."""
+    s = scenes.tiny(n_cells=8, model=capi.MODEL_LINELAST, props=(1e6, 0.3), scheme=scheme, idimc=idimc)
+    c = s.parts.coord.mean(axis=0)
+    s.parts.coord[:] = c + 0.93 * (s.parts.coord - c)
+    s.shifting = True
+    print("fp32 shifting scheme=%d idimc=%s n=%d" % (scheme, idimc, s.n), flush=True)
+    o = oracle_run(s, 20)
+    e = engine(s, capi.PREC_FP32, relayout=4)
+    assert e.step(1, 20) == 0
+    got = e.download(s.n)
+    s.shifting = False
+    o0 = oracle_run(s, 20)
+    shift_ref = o.parts.coord - o0.parts.coord
+    shift_got = got.coord - o0.parts.coord
+    errs = (np.max(np.abs(got.coord - o.parts.coord)), rel(got.stress, o.parts.stress), rel(shift_got, shift_ref))
+    print("  position %.2e stress %.2e shift %.2e (max shift %.2e)" % (errs + (np.abs(shift_ref).max(),)),
+          flush=True)
+    assert np.abs(shift_ref).max() > 1e-6
+    assert errs[0] < 1e-7 and errs[1] < 1e-3 and errs[2] < 1e-3
 
 
 # ------------------------------------------------------------------ reference case files
@@ -679,17 +700,18 @@ def test_reference_beam_case_with_shifting(tmp_path, iprop):
 
 def test_fp32_reference_beam_case():
     """The generator's beam (quadratic 2,1,2 stencil, 4x1x4 points per cell,
-    FLIP, gravity) read from its own files, in the FP32 engine over 200 steps
-    against the oracle (particle shifting off: the FP32 engine does not run
-    it). Bounds: position 1e-6 of the 5 m domain, stress 1e-3 relative."""
+    FLIP, gravity, particle shifting, as the reference's step runs it) read
+    from its own files, in the FP32 engine over 200 steps against the oracle.
+    Bounds: position 1e-6 of the 5 m domain, stress 1e-3 relative."""
     import tempfile
     with tempfile.TemporaryDirectory() as d:
         import pathlib
         case = _beam_case(pathlib.Path(d))
         s = case.scene()
+        s.shifting = True
         print("fp32 beam n=%d" % s.n, flush=True)
         o = oracle_run(s, 200)
-        e = case.engine(capi.PREC_FP32, relayout_every=8)
+        e = case.engine(capi.PREC_FP32, relayout_every=8, shifting=True)
         assert e.step(1, 200) == 0
         got = e.download(s.n)
     errs = (np.max(np.abs(got.coord - o.parts.coord)), rel(got.stress, o.parts.stress))
