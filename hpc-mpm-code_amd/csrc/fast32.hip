@@ -766,13 +766,15 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                 const uint32_t qi = blockIdx.x * (uint32_t)kSurRegion + off +
                                     (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 if (plastic && qi < sa.qcap) {
-                    float4 *row = (float4 *)(sa.queue + (size_t)qi * kSurQueueRow);
-                    row[0] = make_float4(__int_as_float(slot), x[0], x[1], x[2]);
-                    row[1] = make_float4(x[3], x[4], x[5], x[6]);
-                    row[2] = make_float4(x[7], x[8], x[9], x[10]);
                     if (sa.kind == kSurTait) {
-                        row[3] = make_float4(x[11], x[12], pex, Bt);
+                        float4 *row = (float4 *)(sa.queue + (size_t)qi * kSurRowTait);
+                        row[0] = make_float4(__int_as_float(slot), x[0], x[1], pex);
+                        row[1] = make_float4(Bt, 0.f, 0.f, 0.f);
                     } else {
+                        float4 *row = (float4 *)(sa.queue + (size_t)qi * kSurRowJ2);
+                        row[0] = make_float4(__int_as_float(slot), x[0], x[1], x[2]);
+                        row[1] = make_float4(x[3], x[4], x[5], x[6]);
+                        row[2] = make_float4(x[7], x[8], x[9], x[10]);
                         row[3] = make_float4(x[11], x[12], t.st[0], t.st[1]);
                         row[4] = make_float4(t.st[2], t.st[3], t.st[4], t.st[5]);
                         row[5] = make_float4(t.E, t.nu, t.sy, t.Hh);
@@ -830,11 +832,12 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
 }
 
 // queue row qi into registers (zeros when not live)
-__device__ __forceinline__ void sur_load_row(const float *queue, uint32_t qi, bool live, float r[kSurQueueRow]) {
+template <int RF>
+__device__ __forceinline__ void sur_load_row(const float *queue, uint32_t qi, bool live, float r[RF]) {
     if (live) {
-        const float4 *row = (const float4 *)(queue + (size_t)qi * kSurQueueRow);
+        const float4 *row = (const float4 *)(queue + (size_t)qi * RF);
 #pragma unroll
-        for (int c = 0; c < kSurQueueRow / 4; c++) {
+        for (int c = 0; c < RF / 4; c++) {
             const float4 v = row[c];
             r[4 * c] = v.x;
             r[4 * c + 1] = v.y;
@@ -843,7 +846,7 @@ __device__ __forceinline__ void sur_load_row(const float *queue, uint32_t qi, bo
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < kSurQueueRow; c++) r[c] = 0.f;
+        for (int c = 0; c < RF; c++) r[c] = 0.f;
     }
 }
 
@@ -862,7 +865,10 @@ __device__ __forceinline__ void sur_load_row(const float *queue, uint32_t qi, bo
 // stays. The evaluated / rejected tallies go to per-region counters: a
 // region's tiles may be spread over several waves, so they are added
 // atomically, but to distinct addresses.
+template <int KIND>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k32_surrogate_pass(PartF32 P, SurrogateArgs sa) {
+    constexpr int RF = KIND == kSurTait ? kSurRowTait : kSurRowJ2;
+    constexpr int NIN = KIND == kSurTait ? 2 : kMlpInUsed;  // features in a row
     __shared__ MlpInferSmem sw;
     __shared__ float scl[64];
     __shared__ __align__(16) uint16_t xs[4][kMlpWaveScratchBytes / 2];
@@ -885,14 +891,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         todo &= todo - 1ull;
         int n_cur = __shfl(myn, b, 64);
         uint32_t q_cur = ((base + (uint32_t)b * nw) >> 2) * (uint32_t)kSurRegion + 64u * ((base + (uint32_t)b * nw) & 3u);
-        float r[kSurQueueRow];
-        sur_load_row(sa.queue, q_cur + (uint32_t)lane, lane < n_cur, r);
+        float r[RF];
+        sur_load_row<RF>(sa.queue, q_cur + (uint32_t)lane, lane < n_cur, r);
         for (;;) {
             // next tile of this batch: its rows are loaded while this one runs
             const bool more = todo != 0ull;
             int n_nxt = 0;
             uint32_t q_nxt = 0;
-            float rn[kSurQueueRow];
+            float rn[RF];
             if (more) {
                 const int bn = __ffsll(todo) - 1;
                 todo &= todo - 1ull;
@@ -900,21 +906,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                 const uint32_t sn = base + (uint32_t)bn * nw;
                 q_nxt = (sn >> 2) * (uint32_t)kSurRegion + 64u * (sn & 3u);
             }
-            sur_load_row(sa.queue, q_nxt + (uint32_t)lane, more && lane < n_nxt, rn);
+            sur_load_row<RF>(sa.queue, q_nxt + (uint32_t)lane, more && lane < n_nxt, rn);
             const bool live = lane < n_cur;
             float xn[16];
 #pragma unroll
-            for (int f = 0; f < 16; f++) xn[f] = (f < kMlpInUsed) ? (r[1 + f] - scl[f]) * scl[16 + f] : 0.f;
+            for (int f = 0; f < 16; f++)  // a row holds NIN features; the rest are zero
+                xn[f] = (f < kMlpInUsed) ? ((f < NIN ? r[1 + f] : 0.f) - scl[f]) * scl[16 + f] : 0.f;
             float y[kMlpOutUsed];
             mlp_wave_infer(sw, xn, y, xs[w], lane);
             bool accept = false;
             if (live) {
                 const int slot = __float_as_int(r[0]);
                 const size_t ld = P.ld;
-                if (sa.kind == kSurTait) {
+                if constexpr (KIND == kSurTait) {
                     // pressure closure: the exact Tait pressure judges the
                     // network's; an accepted one replaces it in the stored stress
-                    const float pex = r[14], B = r[15];
+                    const float pex = r[3], B = r[4];
                     const float pnet = y[0] * scl[48] + scl[32];
                     if (fabsf(pnet - pex) < sa.tol * (fabsf(pex) + 1e-6f * B)) {
                         accept = true;
@@ -956,7 +963,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             n_cur = n_nxt;
             q_cur = q_nxt;
 #pragma unroll
-            for (int cc = 0; cc < kSurQueueRow; cc++) r[cc] = rn[cc];
+            for (int cc = 0; cc < RF; cc++) r[cc] = rn[cc];
         }
     }
 }
@@ -1104,16 +1111,23 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
 void launch_f32_j2_surrogate(hipStream_t s, const PartF32 &P, const SurrogateArgs &sa) {
     if (P.n <= 0) return;
     // persistent blocks: as many as are resident at once (queried once)
-    static int resident = 0;
-    if (resident == 0) {
+    static int resident[2] = {0, 0};
+    const bool tait = sa.kind == kSurTait;
+    if (resident[tait] == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k32_surrogate_pass, 256, 0);
-        resident = std::max(1, cus) * std::max(1, per_cu);
+        if (tait)
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k32_surrogate_pass<kSurTait>, 256, 0);
+        else
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k32_surrogate_pass<kSurJ2>, 256, 0);
+        resident[tait] = std::max(1, cus) * std::max(1, per_cu);
     }
-    const int blocks = std::min((int)sa.nregions, resident);
-    hipLaunchKernelGGL(k32_surrogate_pass, dim3(blocks), dim3(256), 0, s, P, sa);
+    const int blocks = std::min((int)sa.nregions, resident[tait]);
+    if (tait)
+        hipLaunchKernelGGL(k32_surrogate_pass<kSurTait>, dim3(blocks), dim3(256), 0, s, P, sa);
+    else
+        hipLaunchKernelGGL(k32_surrogate_pass<kSurJ2>, dim3(blocks), dim3(256), 0, s, P, sa);
 }
 
 void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,

@@ -300,9 +300,12 @@ void launch_s32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, c
 // block of the update kernel compacts its plastic states into its own queue
 // region (rows [256 b, 256 b + count_b)) and stores count_b, so the queue is
 // filled without any global atomic.
-// floats per queued state: slot, 13 features, then J2: trial stress (6), E, nu, sy, H;
-// Tait: exact pressure, B = rho0 c^2 / gamma
-constexpr int kSurQueueRow = 24;
+// floats per queued state. J2: slot, 13 features, trial stress (6), E, nu,
+// sy, H. Tait: slot, the 2 features, exact pressure, B = rho0 c^2 / gamma,
+// zero padding. kSurQueueRow is the larger (the queue's allocation).
+constexpr int kSurRowJ2 = 24;
+constexpr int kSurRowTait = 8;
+constexpr int kSurQueueRow = kSurRowJ2;
 constexpr int kSurRegion = 256;   // queue rows per update-kernel block (= its block size)
 // Models the surrogate stands in for: the J2 plastic corrector, or the Tait
 // pressure (a closure p(J, rate of volume) with the exact equation of state

@@ -46,15 +46,21 @@ struct MlpSmem {
     float b1[kMlpHid], b2[kMlpHid], b3[kMlpOut];
 };
 
-// The same weights staged for inference in MFMA-fragment order: fragment f
-// of a layer holds, for each lane l, the 4 bf16 of the A operand that lane
-// reads (row 16 hb + l % 16, columns 16 kb + 4 (l / 16) .. + 3), 8 bytes per
-// lane contiguous, so a wave reads a fragment with one conflict-free
-// ds_read_b64. Fragments: layer 1 hb (4), layer 2 4 hb + kb (16), layer 3 kb (4).
-struct MlpInferSmem {
+// The same weights staged for inference in MFMA-fragment order, each lane's
+// A operand contiguous so that a wave reads a fragment with one
+// conflict-free LDS read. Layer 1 (K = 16) runs on the 16x16x16 BF16 MFMA:
+// fragment hb holds, for lane l, row 16 hb + l % 16, columns 4 (l / 16) .. + 3.
+// Layers 2 and 3 (K = 64) run on gfx950's 16x16x32 BF16 MFMA as two 32-wide
+// K blocks. Their K order is permuted so that the accumulator fragments of
+// the previous layer are the B operand as they stand: in K block kb, lane
+// group q = l / 16 supplies slots 8 q .. 8 q + 7 = hidden units
+// 16 (2 kb) + 4 q .. + 3 and 16 (2 kb + 1) + 4 q .. + 3, which are exactly
+// the rows that lane holds of hidden blocks 2 kb and 2 kb + 1; the weights'
+// columns follow the same permutation.
+struct __attribute__((aligned(16))) MlpInferSmem {
     uint16_t f1[4][64][4];
-    uint16_t f2[16][64][4];
-    uint16_t f3[4][64][4];
+    uint16_t f2[4][2][64][8];  // [output block][K block][lane][8]
+    uint16_t f3[2][64][8];     // [K block][lane][8]
     float b1[kMlpHid], b2[kMlpHid], b3[kMlpOut];
 };
 
@@ -93,25 +99,22 @@ __device__ __forceinline__ void mlp_load_smem(MlpSmem &s, const MlpParams &p) {
     for (int i = threadIdx.x; i < kMlpOut; i += blockDim.x) s.b3[i] = p.b3[i];
 }
 
+// hidden unit of slot j (0..7) of lane group q in K block kb (the permutation above)
+__device__ __forceinline__ int mlp_kperm(int kb, int q, int j) { return 16 * (2 * kb + (j >> 2)) + 4 * q + (j & 3); }
+
 __device__ __forceinline__ void mlp_load_infer_smem(MlpInferSmem &s, const MlpParams &p) {
-    constexpr int kUnits = (4 + 16 + 4) * 64;  // (fragment, lane) pairs of 8 bytes
-    for (int u = threadIdx.x; u < kUnits; u += blockDim.x) {
+    for (int u = threadIdx.x; u < 4 * 64; u += blockDim.x) {  // layer 1: row 16 hb + c16 of [64][16]
         const int f = u >> 6, l = u & 63, c16 = l & 15, q4 = 4 * (l >> 4);
-        const uint16_t *src;
-        uint2 *dst;
-        if (f < 4) {  // layer 1: row 16 hb + c16 of [64][16]
-            src = p.w1 + (16 * f + c16) * kMlpIn + q4;
-            dst = (uint2 *)s.f1[f][l];
-        } else if (f < 20) {  // layer 2: fragment 4 hb + kb of [64][64]
-            const int hb = (f - 4) >> 2, kb = (f - 4) & 3;
-            src = p.w2 + (16 * hb + c16) * kMlpHid + 16 * kb + q4;
-            dst = (uint2 *)s.f2[f - 4][l];
-        } else {  // layer 3: fragment kb of [16][64]
-            const int kb = f - 20;
-            src = p.w3 + c16 * kMlpHid + 16 * kb + q4;
-            dst = (uint2 *)s.f3[kb][l];
-        }
-        *dst = *(const uint2 *)src;
+        *(uint2 *)s.f1[f][l] = *(const uint2 *)(p.w1 + (16 * f + c16) * kMlpIn + q4);
+    }
+    for (int u = threadIdx.x; u < (8 + 2) * 64; u += blockDim.x) {
+        const int f = u >> 6, l = u & 63, c16 = l & 15, q = l >> 4;
+        const int kb = f < 8 ? (f & 1) : f - 8;
+        const uint16_t *row = f < 8 ? p.w2 + (16 * (f >> 1) + c16) * kMlpHid : p.w3 + c16 * kMlpHid;
+        uint16_t *dst = f < 8 ? s.f2[f >> 1][kb][l] : s.f3[kb][l];
+        // two runs of 4 consecutive columns
+        *(uint2 *)dst = *(const uint2 *)(row + mlp_kperm(kb, q, 0));
+        *(uint2 *)(dst + 4) = *(const uint2 *)(row + mlp_kperm(kb, q, 4));
     }
     for (int i = threadIdx.x; i < kMlpHid; i += blockDim.x) {
         s.b1[i] = p.b1[i];
@@ -120,13 +123,23 @@ __device__ __forceinline__ void mlp_load_infer_smem(MlpInferSmem &s, const MlpPa
     for (int i = threadIdx.x; i < kMlpOut; i += blockDim.x) s.b3[i] = p.b3[i];
 }
 
+typedef __bf16 mlp_v8bf __attribute__((ext_vector_type(8)));
+typedef short mlp_v8s __attribute__((ext_vector_type(8)));
+
+// B operand of a 32-wide K block from two 16-row accumulator fragments
+__device__ __forceinline__ mlp_v8bf mlp_cat_frag(mlp_v4s lo, mlp_v4s hi) {
+    const mlp_v8s v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    return __builtin_bit_cast(mlp_v8bf, v);
+}
+
 __device__ __forceinline__ void mlp_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
 
 // Forward pass of the 64 samples staged in xs ([64][16] bf16 normalised
-// inputs) on the 16x16x16 BF16 MFMA; on return xs holds [64][8] FP32
+// inputs) on the BF16 MFMA (16x16x16 for the first layer, 16x16x32 for the
+// others); on return xs holds [64][8] FP32
 // normalised outputs. The layers run transposed (hidden x samples) so that
 // the accumulator fragment of one layer (lane l: rows 4(l/16)+r, column
 // l%16) is exactly the B operand fragment of the next; only the inputs and
@@ -154,9 +167,9 @@ __device__ __forceinline__ void mlp_wave_run(const MlpInferSmem &s, uint16_t *xs
         for (int hb = 0; hb < 4; hb++) {
             mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-                const mlp_v4s a = *(const mlp_v4s *)s.f2[4 * hb + kb][lane];
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kb], acc, 0, 0, 0);
+            for (int kb = 0; kb < 2; kb++) {
+                const mlp_v8bf a = *(const mlp_v8bf *)s.f2[hb][kb][lane];
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, mlp_cat_frag(h1[2 * kb], h1[2 * kb + 1]), acc, 0, 0, 0);
             }
             const float *bb = s.b2 + hb * 16 + q4;
             h2[hb] = mlp_frag_bf16(fmaxf(acc[0] + bb[0], 0.f), fmaxf(acc[1] + bb[1], 0.f), fmaxf(acc[2] + bb[2], 0.f),
@@ -164,9 +177,9 @@ __device__ __forceinline__ void mlp_wave_run(const MlpInferSmem &s, uint16_t *xs
         }
         mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
-            const mlp_v4s a = *(const mlp_v4s *)s.f3[kb][lane];
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[kb], acc, 0, 0, 0);
+        for (int kb = 0; kb < 2; kb++) {
+            const mlp_v8bf a = *(const mlp_v8bf *)s.f3[kb][lane];
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, mlp_cat_frag(h2[2 * kb], h2[2 * kb + 1]), acc, 0, 0, 0);
         }
         if (q4 < 8) {
 #pragma unroll
