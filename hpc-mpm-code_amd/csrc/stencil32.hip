@@ -59,30 +59,32 @@ __device__ __forceinline__ void axis_weights(int c, float r, int nex, AxisW<W> &
         A.d[2] = r;
         A.x[2] = 1.5f - r;
         // clamped pieces, with the reference's precedence: global index 0,
-        // then 1, then the last, then the one before it
+        // then 1, then the last, then the one before it (selects, not
+        // branches: the cell, and so the piece, is fixed per lane)
         const int last = nex + 1;
 #pragma unroll
         for (int o = 0; o < 3; o++) {
             const int gi = c + o;
-            if (gi == 0) {
-                A.w[o] = s * s;
-                A.d[o] = -2.f * s;
-                A.x[o] = -r;
-            } else if (gi == 1) {
-                if (o == 1) {
-                    A.w[o] = r * (2.f - 1.5f * r);
-                    A.d[o] = 2.f - 3.f * r;
-                }
-            } else if (gi == last) {
-                A.w[o] = r * r;
-                A.d[o] = 2.f * r;
-                A.x[o] = (float)(nex - c) - r;
-            } else if (gi == last - 1) {
-                if (o == 1) {
-                    A.w[o] = 0.5f + r * (1.f - 1.5f * r);
-                    A.d[o] = 1.f - 3.f * r;
-                }
+            const bool e0 = gi == 0;
+            const bool e1 = !e0 && gi == 1;
+            const bool eL = !e0 && !e1 && gi == last;
+            const bool eL1 = !e0 && !e1 && !eL && gi == last - 1;
+            float w = A.w[o], d = A.d[o], x = A.x[o];
+            if (o == 1) {
+                w = e1 ? r * (2.f - 1.5f * r) : w;
+                d = e1 ? 2.f - 3.f * r : d;
+                w = eL1 ? 0.5f + r * (1.f - 1.5f * r) : w;
+                d = eL1 ? 1.f - 3.f * r : d;
             }
+            w = e0 ? s * s : w;
+            d = e0 ? -2.f * s : d;
+            x = e0 ? -r : x;
+            w = eL ? r * r : w;
+            d = eL ? 2.f * r : d;
+            x = eL ? (float)(nex - c) - r : x;
+            A.w[o] = w;
+            A.d[o] = d;
+            A.x[o] = x;
         }
     }
 }
@@ -134,21 +136,41 @@ __device__ __forceinline__ void affine_matrix(const float *stage, int i, int fC,
         for (int b = 0; b < 3; b++) Mh[a][b] = M[3 * a + b] * dx[b];
 }
 
+// entry i of a W-entry array in registers, without a dynamic register index
+template <int W>
+__device__ __forceinline__ float pick(const float (&a)[W], int i) {
+    float v = a[0];
+#pragma unroll
+    for (int o = 1; o < W; o++) v = (i == o) ? a[o] : v;
+    return v;
+}
+
 // ------------------------------------------------------------------ P2G
 // AFF: 0 none, 1 TPIC, 2 APIC. Partials per cell: corner q (= i + NX (j +
 // NY k), stencil order k, j, i as MakSf) x {m, px, py, pz, fx, fy, fz} at
 // q * 7 + f, then the corner volumes at 7 NN + q when diag is set.
-template <int NX, int NY, int NZ, int AFF>
+// LZ lanes per cell (1 or NZ): with LZ = NZ each lane folds the cell's
+// particles into one z-layer of the corners (lane = layer * G + cell of the
+// group, G = 64 / NZ cells per group), which cuts the accumulators of the
+// 27-point stencil from 216 to 72 registers; every corner's sum is the same
+// sequence of additions either way.
+template <int NX, int NY, int NZ, int AFF, int LZ>
 __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T,
                                                       float *__restrict__ part, int diag) {
+    static_assert(LZ == 1 || LZ == NZ, "a lane holds all z-layers of a cell or one");
+    constexpr int KL = NZ / LZ;       // z-layers per lane
+    constexpr int G = kSeg / LZ;      // cells per group
     constexpr int NN = NX * NY * NZ;
+    constexpr int NL = NX * NY * KL;  // corners per lane
     // staged fields: 0-2 r, 3-5 v, 6 m, 7 V, 8-13 sigma, 14-16 traction, then C (9) and D^-1 (9)
     constexpr int F_C = 17;
     constexpr int NF = F_C + (AFF >= 1 ? 9 : 0) + (AFF == 2 ? 9 : 0);
     __shared__ float stage[NF * kBlk * kStageS];
     const uint32_t n_active = L.ctr->n_active;
-    const int ngroups = (int)((n_active + kSeg - 1) / kSeg);
+    const int ngroups = (int)((n_active + G - 1) / G);
     const int t = threadIdx.x;
+    const int k0 = (LZ == 1) ? 0 : t / G;  // first z-layer of this lane (LZ for an idle lane)
+    const int u = (LZ == 1) ? t : t - k0 * G;
     const size_t ld = P.ld;
     const bool trac = P.trac != nullptr;
     const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
@@ -157,11 +179,30 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
         CellInfo si;
         uint32_t S, E;
         wave_fence();
-        group_setup<1>(g, L, n_active, grp, si, S, E);
-        float acc[NN][7];
-        float vof[NN];
+        {
+            const uint32_t a0 = (uint32_t)grp * G;
+            const uint32_t a = a0 + (uint32_t)u;
+            const uint32_t a_end = min(a0 + (uint32_t)G, n_active);
+            S = ((const __attribute__((address_space(4))) uint32_t *)L.astart)[a0];
+            E = ((const __attribute__((address_space(4))) uint32_t *)L.astart)[a_end];
+            si.c = -1;
+            si.cs = si.ce = 0u;
+            si.ix = si.iy = si.iz = 0;
+            if (k0 < LZ && a < a_end) {
+                si.c = (int)L.active[a];
+                si.cs = L.astart[a];
+                si.ce = L.astart[a + 1];
+                const int nexy = g.nex[0] * g.nex[1];
+                si.iz = si.c / nexy;
+                const int r = si.c - si.iz * nexy;
+                si.iy = r / g.nex[0];
+                si.ix = r - si.iy * g.nex[0];
+            }
+        }
+        float acc[NL][7];
+        float vof[NL];
 #pragma unroll
-        for (int q = 0; q < NN; q++) {
+        for (int q = 0; q < NL; q++) {
 #pragma unroll
             for (int f = 0; f < 7; f++) acc[q][f] = 0.f;
             vof[q] = 0.f;
@@ -205,10 +246,17 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                 const int i = (int)(pos - base);
                 AxisW<NX> Ax;
                 AxisW<NY> Ay;
-                AxisW<NZ> Az;
+                AxisW<NZ> Azf;
                 axis_weights<NX>(si.ix, stage[st_idx<kStageS>(0, i)], g.nex[0], Ax);
                 axis_weights<NY>(si.iy, stage[st_idx<kStageS>(1, i)], g.nex[1], Ay);
-                axis_weights<NZ>(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], Az);
+                axis_weights<NZ>(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], Azf);
+                AxisW<KL> Az;  // this lane's z-layers
+#pragma unroll
+                for (int kl = 0; kl < KL; kl++) {
+                    Az.w[kl] = (LZ == 1) ? Azf.w[kl] : pick<NZ>(Azf.w, k0 + kl);
+                    Az.d[kl] = (LZ == 1) ? Azf.d[kl] : pick<NZ>(Azf.d, k0 + kl);
+                    Az.x[kl] = (LZ == 1) ? Azf.x[kl] : pick<NZ>(Azf.x, k0 + kl);
+                }
                 const float m = stage[st_idx<kStageS>(6, i)];
                 const float mv[3] = {m * stage[st_idx<kStageS>(3, i)], m * stage[st_idx<kStageS>(4, i)],
                                      m * stage[st_idx<kStageS>(5, i)]};
@@ -227,22 +275,22 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                     for (int a = 0; a < 3; a++) tr[a] = stage[st_idx<kStageS>(14 + a, i)];
                 }
                 // momentum per corner: m v + m M (x_a - x_p), split by axis
-                float px[3][NX], py[3][NY], pz[3][NZ];
+                float px[3][NX], py[3][NY], pz[3][KL];
                 if constexpr (AFF >= 1) {
                     float Mh[3][3];
                     affine_matrix<AFF>(stage, i, F_C, dxs, Mh);
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
 #pragma unroll
-                        for (int u = 0; u < NX; u++) px[a][u] = mv[a] + (m * Mh[a][0]) * Ax.x[u];
+                        for (int o = 0; o < NX; o++) px[a][o] = mv[a] + (m * Mh[a][0]) * Ax.x[o];
 #pragma unroll
-                        for (int u = 0; u < NY; u++) py[a][u] = (m * Mh[a][1]) * Ay.x[u];
+                        for (int o = 0; o < NY; o++) py[a][o] = (m * Mh[a][1]) * Ay.x[o];
 #pragma unroll
-                        for (int u = 0; u < NZ; u++) pz[a][u] = (m * Mh[a][2]) * Az.x[u];
+                        for (int o = 0; o < KL; o++) pz[a][o] = (m * Mh[a][2]) * Az.x[o];
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < NZ; k++)
+                for (int k = 0; k < KL; k++)
 #pragma unroll
                     for (int j = 0; j < NY; j++) {
                         const float wyz = Ay.w[j] * Az.w[k];
@@ -267,12 +315,19 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
             wave_fence();
         }
         if (si.ce > si.cs) {
-            const size_t ne = (size_t)g.nelem;
+            // the cell count is re-read here (an empty asm makes it opaque) so
+            // that the compiler does not keep the 8 NL store addresses live in
+            // registers across the whole kernel
+            int nel = g.nelem;
+            asm volatile("" : "+s"(nel));
+            const size_t ne = (size_t)nel;
+            float *pc = part + (size_t)(NX * NY * k0) * 7 * ne + si.c;
+            float *pv = part + (size_t)(7 * NN + NX * NY * k0) * ne + si.c;
 #pragma unroll
-            for (int q = 0; q < NN; q++) {
+            for (int ql = 0; ql < NL; ql++) {
 #pragma unroll
-                for (int f = 0; f < 7; f++) part[(size_t)(q * 7 + f) * ne + si.c] = acc[q][f];
-                if (diag) part[(size_t)(7 * NN + q) * ne + si.c] = vof[q];
+                for (int f = 0; f < 7; f++) pc[(size_t)(ql * 7 + f) * ne] = acc[ql][f];
+                if (diag) pv[(size_t)ql * ne] = vof[ql];
             }
         }
     }
@@ -889,8 +944,8 @@ __global__ void ks_shift_b0(int nbe, int nbd, double mtol, ShiftF64 S) {
     *S.b0 = (dot < mtol) ? 0.0 : (eu / dot);
 }
 
-int cell_groups(const GridGeom &g) {
-    const int ng = cdiv(g.nelem, 64);
+int cell_groups(const GridGeom &g, int cells_per_group = 64) {
+    const int ng = cdiv(g.nelem, cells_per_group);
     return ng < 8192 ? ng : 8192;
 }
 
@@ -898,13 +953,15 @@ template <int NX, int NY, int NZ>
 void p2g_launch(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
                 const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
                 int diag, hipEvent_t mid) {
-    const int ng = c.cells_grid > 0 ? c.cells_grid : cell_groups(g);
+    // the quadratic z-axis takes one z-layer per lane (3 lanes per cell)
+    constexpr int LZ = (NZ == 3 && NX * NY * NZ >= 18) ? NZ : 1;
+    const int ng = c.cells_grid > 0 ? c.cells_grid : cell_groups(g, kSeg / LZ);
     if (c.scheme == 2)
-        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 1>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
+        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 1, LZ>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
     else if (c.scheme == 3)
-        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 2>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
+        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 2, LZ>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
     else
-        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 0>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
+        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 0, LZ>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
     if (mid) (void)hipEventRecord(mid, s);
     if (diag)
         hipLaunchKernelGGL((ks_p2g_nodes<NX, NY, NZ, true>), dim3(tg.ntiles), dim3(256), 0, s, g, L, tg,
