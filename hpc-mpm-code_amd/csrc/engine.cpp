@@ -105,6 +105,7 @@ struct mpm_engine {
     NodesF32 N32{};
     float *partials = nullptr;
     bool stencil = false;  // FP32: the stencil kernels (quadratic B-spline on some axis, or TPIC / APIC)
+    bool stencil_base = false;  // the choice by shape functions and scheme (F-bar also takes the stencil kernels)
     NodeTables T{};
     float m_small = 0.f;
     int lanes_per_cell = 1;  // from the particles per non-empty cell at upload
@@ -343,7 +344,7 @@ int do_bin(mpm_engine *e) {
 }
 
 int do_p2g(mpm_engine *e, double facl, int diag, hipEvent_t mid = nullptr) {
-    if (e->shifting) diag = 1;  // the shifting correction reads the P2G nodal volume
+    if (e->shifting || e->fbar) diag = 1;  // shifting and F-bar read the P2G nodal volume
     if (e->prec == MPM_PREC_FP64_EXACT) {
         launch_f64_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.scheme, e->prm.bb[0] * facl,
                        e->prm.bb[1] * facl, e->prm.bb[2] * facl, e->prm.mtol, facl);
@@ -406,6 +407,10 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
             launch_s32_shift(e->stream, e->g, e->P32[e->cur], e->N32, e->T, sh, e->prm.mtol);
             c.sh_geup = sh.geup;
             c.sh_b0 = sh.b0;
+        }
+        if (e->fbar) {  // var_trans_solid_explicit.cpp:298-302
+            launch_s32_fbar(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->T, e->tg, c, e->partials);
+            c.fbar = 1;
         }
         if (mid) MPM_HIP_CHECK(hipEventRecord(mid, e->stream));
         if (!e->fused_count) e->clear_counts();
@@ -484,6 +489,17 @@ int require_particles(mpm_engine *e) {
     return 0;
 }
 
+// F-bar arrays of one FP32 particle buffer: Fbar - I and Jbar - 1 (moved by
+// the relayout), the step's increment and Jbar det dF - 1.
+int alloc_f32_fbar(mpm_engine *e, PartF32 &P) {
+    const size_t ld = e->ld;
+    int rc = 0;
+    if (!P.Hb && ((rc = e->dalloc(&P.Hb, 9 * ld, true)) || (rc = e->dalloc(&P.jb, ld, true)) ||
+                  (rc = e->dalloc(&P.Dd, 9 * ld, true)) || (rc = e->dalloc(&P.jm, ld, true))))
+        return rc;
+    return 0;
+}
+
 // One FP32 particle buffer (slot arrays of every field the upload carries);
 // buffer 1 is the target of the relayout permutation.
 int alloc_f32_buffer(mpm_engine *e, int b) {
@@ -501,6 +517,7 @@ int alloc_f32_buffer(mpm_engine *e, int b) {
     if (e->has_epsp && (rc = e->dalloc(&P.eps_p, ld, true))) return rc;
     if (e->prm.scheme >= MPM_SCHEME_TPIC && (rc = e->dalloc(&P.C, 9 * ld, true))) return rc;
     if (e->prm.scheme == MPM_SCHEME_APIC && (rc = e->dalloc(&P.Dinv, 9 * ld, true))) return rc;
+    if (e->fbar && (rc = alloc_f32_fbar(e, P))) return rc;
     P.ld = ld;
     P.n = e->n;
     return 0;
@@ -642,6 +659,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     e->prec = params->precision;
     e->stencil = e->prec == MPM_PREC_FP32 &&
                  (grid->idimc[0] != 1 || grid->idimc[1] != 1 || grid->idimc[2] != 1 || params->scheme >= MPM_SCHEME_TPIC);
+    e->stencil_base = e->stencil;
     e->relayout_every = params->precision == MPM_PREC_FP32 ? std::max(0, (int)params->relayout_every) : 0;
     e->mt.nmat = nmat;
     std::vector<MaterialDev> mh(nmat);
@@ -1078,16 +1096,40 @@ static int fbar_reset(mpm_engine *e) {
     return 0;
 }
 
+// FP32 F-bar state at the start of a run: Fbar - I = 0, Jbar - 1 = 0, in
+// the current particle buffer
+static int fbar_reset32(mpm_engine *e) {
+    const PartF32 &P = e->P32[e->cur];
+    MPM_HIP_CHECK(hipMemsetAsync(P.Hb, 0, 9 * e->ld * sizeof(float), e->stream));
+    MPM_HIP_CHECK(hipMemsetAsync(P.jb, 0, e->ld * sizeof(float), e->stream));
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
 int mpm_set_fbar(mpm_engine *e, int32_t on) try {
     if (!e) return set_error(MPM_ERR_INVALID_ARGUMENT, "null engine");
-    if (on && e->prec != MPM_PREC_FP64_EXACT)
-        return set_error(MPM_ERR_UNSUPPORTED, "the F-bar projection runs in the FP64-exact engine only");
+    if (on && e->prec == MPM_PREC_FP32 && (e->sur.enabled || e->sur.logging || e->mat_dt))
+        return set_error(MPM_ERR_UNSUPPORTED,
+                         "the FP32 F-bar projection runs without the constitutive surrogate and per-material time steps");
     e->drop_graph();
     if (!on) {
         e->fbar = false;
+        e->stencil = e->stencil_base;
         return 0;
     }
     int rc;
+    if (e->prec == MPM_PREC_FP32) {
+        // the stencil kernels carry the projection (the tent functions too)
+        if (!e->N32.nje && (rc = e->dalloc(&e->N32.nje, (size_t)e->g.nodec))) return rc;
+        e->fbar = true;
+        e->stencil = true;
+        if (e->n > 0) {
+            for (int b = 0; b < 2; b++)
+                if (e->P32[b].r && (rc = alloc_f32_fbar(e, e->P32[b]))) return rc;
+            if ((rc = fbar_reset32(e))) return rc;
+        }
+        return 0;
+    }
     if (!e->d_njbar && (rc = e->dalloc(&e->d_njbar, (size_t)e->g.nodec))) return rc;
     e->fbar = true;
     if (e->n > 0) {
@@ -1103,9 +1145,40 @@ int mpm_set_fbar(mpm_engine *e, int32_t on) try {
     return mpm::exception_status();
 }
 
+// FP32: slot s holds particle pid[s]; Fbar = I + Hb, Jbar = 1 + jb.
+static int fbar_copy32(mpm_engine *e, double *fbar, double *jbar, bool to_device) {
+    int rc = 0;
+    const PartF32 &P = e->P32[e->cur];
+    if (!P.Hb) return set_error(MPM_ERR_INVALID_ARGUMENT, "F-bar is off");
+    const size_t ld = e->ld;
+    const int n = e->n;
+    std::vector<int> pidv;
+    std::vector<float> hb, jb;
+    if ((rc = d2h(e, pidv, e->pid[e->cur], ld)) || (rc = d2h(e, hb, (const float *)P.Hb, 9 * ld)) ||
+        (rc = d2h(e, jb, (const float *)P.jb, ld)))
+        return rc;
+    MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    for (int s = 0; s < n; s++) {
+        const size_t i = (size_t)pidv[s];
+        for (int q = 0; q < 9; q++) {
+            const double eye = (q == 0 || q == 4 || q == 8) ? 1.0 : 0.0;
+            if (fbar && to_device) hb[q * ld + s] = (float)(fbar[9 * i + q] - eye);
+            if (fbar && !to_device) fbar[9 * i + q] = (double)hb[q * ld + s] + eye;
+        }
+        if (jbar && to_device) jb[s] = (float)(jbar[i] - 1.0);
+        if (jbar && !to_device) jbar[i] = 1.0 + (double)jb[s];
+    }
+    if (to_device) {
+        if ((fbar && (rc = h2d(e, P.Hb, hb))) || (jbar && (rc = h2d(e, P.jb, jb)))) return rc;
+        MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
+    }
+    return 0;
+}
+
 static int fbar_copy(mpm_engine *e, double *fbar, double *jbar, bool to_device) {
     int rc = require_particles(e);
     if (rc) return rc;
+    if (e->prec == MPM_PREC_FP32) return fbar_copy32(e, fbar, jbar, to_device);
     PartF64 &P = e->P64;
     if (e->prec != MPM_PREC_FP64_EXACT || !P.Fbar) return set_error(MPM_ERR_INVALID_ARGUMENT, "F-bar is off");
     const size_t ld = e->ld;
@@ -1407,6 +1480,7 @@ int mpm_upload_particles(mpm_engine *e, const mpm_particles_view *p) try {
             if ((P.C && (rc = h2d(e, P.C, zero))) || (P.Dinv && (rc = h2d(e, P.Dinv, zero)))) return rc;
             MPM_HIP_CHECK(hipStreamSynchronize(e->stream));
         }
+        if (P.Hb && (rc = fbar_reset32(e))) return rc;
         if (e->has_trac) {
             tr.assign(3 * ld, 0.f);
             for (int i = 0; i < n; i++)

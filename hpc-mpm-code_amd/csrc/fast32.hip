@@ -992,7 +992,9 @@ __global__ void k32_permute(PartF32 src, PartF32 dst, const uint32_t *__restrict
     for (int q = 0; q < 9; q++) {
         if (src.C) dst.C[q * lt + s] = src.C[q * ls + o];
         if (src.Dinv) dst.Dinv[q * lt + s] = src.Dinv[q * ls + o];
+        if (src.Hb) dst.Hb[q * lt + s] = src.Hb[q * ls + o];
     }
+    if (src.jb) dst.jb[s] = src.jb[o];
     pid_dst[s] = pid_src[o];
 }
 

@@ -271,8 +271,11 @@ __device__ __forceinline__ float tait_pressure(const double *p, double J, float 
     return B * expm1f(-gam * log1pf((float)(J - 1.0)));
 }
 
+// Jl is the determinant in the Neo-Hookean log term: J, or Jbar under the
+// F-bar projection (which passes H = Fbar - I and the scaled increment; the
+// other determinants stay det F, as ConstitutiveModel with det_def_grad_bar).
 __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9], const float D[9], double J,
-                                         float s[6], float *eps_p, float inv_dt) {
+                                         double Jl, float s[6], float *eps_p, float inv_dt) {
     const double *p = md.prop;
     switch (md.model) {
     case -1:
@@ -296,7 +299,7 @@ __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9]
         float E = (float)p[0], nu = (float)p[1];
         float lmd = nu * E / ((1.f + nu) * (1.f - 2.f * nu));
         float mu = E / (2.f * (1.f + nu));
-        float lnj = log1pf((float)(J - 1.0));
+        float lnj = log1pf((float)(Jl - 1.0));
         float invJ = (float)(1.0 / J);
         float B[6];
         // B = H + H^T + H H^T in Voigt order xx,yy,zz,yz,xz,xy
@@ -399,6 +402,11 @@ __device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9]
     }
     default: break;
     }
+}
+
+__device__ __forceinline__ void stress32(const MaterialDev &md, const float H[9], const float D[9], double J,
+                                         float s[6], float *eps_p, float inv_dt) {
+    stress32(md, H, D, J, J, s, eps_p, inv_dt);
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t h) {

@@ -215,6 +215,13 @@ struct PartF32 {
     uint32_t *key;  // cell of each slot
     float *C;       // [9][ld] TPIC velocity gradient / APIC B matrix, row-major (null for PIC/FLIP)
     float *Dinv;    // [9][ld] APIC inverse D matrix (null otherwise)
+    // F-bar projection (null when off): the projected gradient as Fbar - I
+    // and Jbar - 1, kept from step to step; within a step, the increment
+    // dF - I and Jbar det dF - 1 that the projection's node average reads
+    float *Hb;      // [9][ld]
+    float *jb;
+    float *Dd;      // [9][ld]
+    float *jm;
     size_t ld;
     int n;
 };
@@ -228,6 +235,7 @@ struct NodesF32 {
     float *np;    // [3][nodec] momentum (diagnostics, may be null)
     float *nf;    // [3][nodec] force (diagnostics, may be null)
     float *nvof;  // [nodec] volume (diagnostics, may be null)
+    float *nje;   // [nodec] F-bar: node average of Jbar det dF, minus 1 (-1 where nvof <= mtol)
 };
 
 struct StepConsts {
@@ -244,6 +252,7 @@ struct StepConsts {
     int mat_dt;          // 1: each particle advances by its material's time step (MaterialDev::dt)
     const double *sh_geup;  // particle shifting (or null): [3][ld] gradient of the excess volume by slot
     const double *sh_b0;    // and the step length b0; the update adds -b0 geup to the position
+    int fbar;               // F-bar projection: the update reads the increment of the F-bar pass
 };
 
 // Grid spacing constants of the FP32 kernels.
@@ -288,6 +297,13 @@ void launch_s32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
 // stencil), the two blocked global sums and b0 into S.
 void launch_s32_shift(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const NodeTables &T,
                       const ShiftF64 &S, double mtol);
+// F-bar projection (fbar_projection.cpp:18-94), before the update: the
+// increment dF - I and F of every particle, then the node average of
+// Jbar det dF weighted by N V (cell partials and a node pass, as P2G); the
+// update (StepConsts::fbar) interpolates it back, scales the increment and
+// advances Fbar and Jbar.
+void launch_s32_fbar(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                     const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials);
 void launch_s32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                            const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
                            const BinFuse &bf);

@@ -724,7 +724,13 @@ __global__ void __launch_bounds__(256) ks_musl_nodes(GridGeom g, CellLists L, Ti
 // dF - I = dt sum v_a (x) grad N_a over the stencil, H, J (FP64), x, V,
 // stress and the new cell key, in slot order; fused cell counting for the
 // next binning as in the tent-function update kernel.
-template <int NX, int NY, int NZ>
+// FB (F-bar projection): F and the increment come from ks_fbar_grad; the
+// node average of Jbar det dF is interpolated here (Jbar = sum N nJbar,
+// fbar_projection.cpp:53-64), the increment is scaled by (Jbar / (Jbar_old
+// det dF))^(1/3) (:66-71), Fbar advances by it (:73-86), and the stress
+// follows Fbar, Jbar and the scaled increment (update_point_coord_and_stress
+// with def_grad_bar, var_trans_solid_explicit.cpp:298-302).
+template <int NX, int NY, int NZ, bool FB>
 __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
                                                       MaterialTable mt, StepConsts sc, uint32_t *err, BinFuse bf) {
     const int slot0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -760,6 +766,7 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
 #pragma unroll
     for (int q = 0; q < 9; q++) Dv[q] = 0.f;
     float dxp[3] = {0.f, 0.f, 0.f};
+    float je = 0.f;  // FB: Jbar - 1 = sum N (nJbar - 1)
 #pragma unroll
     for (int k = 0; k < NZ; k++)
 #pragma unroll
@@ -770,35 +777,73 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
                 const float v0 = N.nvm[nid], v1 = N.nvm[nodec + nid], v2 = N.nvm[2 * nodec + nid];
                 const float wyz = Ay.w[j] * Az.w[k];
                 const float Nw = Ax.w[i] * wyz;
-                const float gx = (Ax.d[i] * inv0) * wyz;
-                const float gy = (Ay.d[j] * inv1) * (Ax.w[i] * Az.w[k]);
-                const float gz = (Az.d[k] * inv2) * (Ax.w[i] * Ay.w[j]);
-                Dv[0] += v0 * gx;
-                Dv[1] += v0 * gy;
-                Dv[2] += v0 * gz;
-                Dv[3] += v1 * gx;
-                Dv[4] += v1 * gy;
-                Dv[5] += v1 * gz;
-                Dv[6] += v2 * gx;
-                Dv[7] += v2 * gy;
-                Dv[8] += v2 * gz;
+                if constexpr (FB) {
+                    je += Nw * N.nje[nid];
+                } else {
+                    const float gx = (Ax.d[i] * inv0) * wyz;
+                    const float gy = (Ay.d[j] * inv1) * (Ax.w[i] * Az.w[k]);
+                    const float gz = (Az.d[k] * inv2) * (Ax.w[i] * Ay.w[j]);
+                    Dv[0] += v0 * gx;
+                    Dv[1] += v0 * gy;
+                    Dv[2] += v0 * gz;
+                    Dv[3] += v1 * gx;
+                    Dv[4] += v1 * gy;
+                    Dv[5] += v1 * gz;
+                    Dv[6] += v2 * gx;
+                    Dv[7] += v2 * gy;
+                    Dv[8] += v2 * gz;
+                }
                 dxp[0] += Nw * v0;
                 dxp[1] += Nw * v1;
                 dxp[2] += Nw * v2;
             }
-    float D[9];
+    float D[9], Hn[9];
+    if constexpr (FB) {
 #pragma unroll
-    for (int q = 0; q < 9; q++) D[q] = sc.dt * Dv[q];
-    float H[9], Hn[9];
+        for (int q = 0; q < 9; q++) {
+            D[q] = P.Dd[q * ld + slot];
+            Hn[q] = P.H[q * ld + slot];  // F - I, advanced by ks_fbar_grad
+        }
+    } else {
 #pragma unroll
-    for (int q = 0; q < 9; q++) H[q] = __builtin_nontemporal_load(&P.H[q * ld + slot]);
+        for (int q = 0; q < 9; q++) D[q] = sc.dt * Dv[q];
+        float H[9];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+        for (int q = 0; q < 9; q++) H[q] = __builtin_nontemporal_load(&P.H[q * ld + slot]);
 #pragma unroll
-        for (int b = 0; b < 3; b++)
-            Hn[3 * a + b] = (H[3 * a + b] + D[3 * a + b]) +
-                            (D[3 * a] * H[b] + D[3 * a + 1] * H[3 + b] + D[3 * a + 2] * H[6 + b]);
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+                Hn[3 * a + b] = (H[3 * a + b] + D[3 * a + b]) +
+                                (D[3 * a] * H[b] + D[3 * a + 1] * H[3 + b] + D[3 * a + 2] * H[6 + b]);
+    }
     const double J = det_I_plus_H(Hn);
+    // F-bar: the scaled increment Ds and Hbar = Fbar - I it advances (the
+    // stress then follows these two and Jbar)
+    float Hs[9];
+    double Jl = J;
+    if constexpr (FB) {
+        const double jm = (double)P.jm[slot];  // Jbar_old det dF - 1
+        const double x = ((double)je - jm) / (1.0 + jm);
+        const double c = cbrt(1.0 + x);
+        const double cm1 = c - 1.0;
+        float Hb[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            D[q] = (float)(c * (double)D[q] + ((q == 0 || q == 4 || q == 8) ? cm1 : 0.0));
+            Hb[q] = P.Hb[q * ld + slot];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+                Hs[3 * a + b] = (Hb[3 * a + b] + D[3 * a + b]) +
+                                (D[3 * a] * Hb[b] + D[3 * a + 1] * Hb[3 + b] + D[3 * a + 2] * Hb[6 + b]);
+        Jl = 1.0 + (double)je;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; q++) Hs[q] = Hn[q];
+    }
     const double V = J * (double)P.V0[slot];
     int ci[3] = {ix, iy, iz};
     bool inside = true;
@@ -826,7 +871,7 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
         for (int q = 0; q < 6; q++) s[q] = P.sig[q * ld + slot];
     }
     float ep = (md.model == 10 && P.eps_p) ? P.eps_p[slot] : 0.f;
-    stress32(md, Hn, D, J, s, &ep, 1.f / sc.dt);
+    stress32(md, Hs, D, J, Jl, s, &ep, 1.f / sc.dt);
     const uint32_t nk = inside ? (uint32_t)(ci[0] + ci[1] * g.nex[0] + ci[2] * nexy) : (uint32_t)g.nelem;
     if (bf.cnt) {
         if (bf.delta) {
@@ -848,8 +893,14 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
     }
     if (!valid) return;
     if (md.model == 10 && P.eps_p) P.eps_p[slot] = ep;
+    if constexpr (FB) {
 #pragma unroll
-    for (int q = 0; q < 9; q++) __builtin_nontemporal_store(Hn[q], &P.H[q * ld + slot]);
+        for (int q = 0; q < 9; q++) P.Hb[q * ld + slot] = Hs[q];
+        P.jb[slot] = je;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; q++) __builtin_nontemporal_store(Hn[q], &P.H[q * ld + slot]);
+    }
 #pragma unroll
     for (int q = 0; q < 6; q++) __builtin_nontemporal_store(s[q], &P.sig[q * ld + slot]);
 #pragma unroll
@@ -862,9 +913,158 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
     if (nk != key) P.key[slot] = nk;
     uint32_t e = 0;
     if (!inside) e |= kErrOutOfGrid;
-    if (!(J > 0.0)) e |= kErrJacobian;
+    if (!(J > 0.0) || !(Jl > 0.0)) e |= kErrJacobian;
     if (!isfinite(r[0] + r[1] + r[2] + s[0] + s[1] + s[2])) e |= kErrNonFinite;
     if (e) atomicOr(err, e);
+}
+
+// ------------------------------------------------------ F-bar projection
+// (fbar_projection.cpp:18-94 with the increment of
+// var_trans_solid_explicit.cpp:244-296.) One thread per slot: dF - I, the
+// new F (stored as H) and Jbar_old det dF - 1 for the node average.
+template <int NX, int NY, int NZ>
+__global__ void __launch_bounds__(256) ks_fbar_grad(GridGeom g, PartF32 P, NodesF32 N, NodeTables T, StepConsts sc) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= P.n) return;
+    const uint32_t key = P.key[slot];
+    if (key >= (uint32_t)g.nelem) return;
+    const size_t ld = P.ld;
+    const int nodec = g.nodec;
+    const int nexy = g.nex[0] * g.nex[1];
+    int rem, ix, iy;
+    const int iz = fdivmod((int)key, nexy, T.inv_nexy, rem);
+    iy = fdivmod(rem, g.nex[0], T.inv_nex, ix);
+    AxisW<NX> Ax;
+    AxisW<NY> Ay;
+    AxisW<NZ> Az;
+    axis_weights<NX>(ix, P.r[slot], g.nex[0], Ax);
+    axis_weights<NY>(iy, P.r[ld + slot], g.nex[1], Ay);
+    axis_weights<NZ>(iz, P.r[2 * ld + slot], g.nex[2], Az);
+    const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
+    const int n0 = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+    float Dv[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) Dv[q] = 0.f;
+#pragma unroll
+    for (int k = 0; k < NZ; k++)
+#pragma unroll
+        for (int j = 0; j < NY; j++)
+#pragma unroll
+            for (int i = 0; i < NX; i++) {
+                const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
+                const float v0 = N.nvm[nid], v1 = N.nvm[nodec + nid], v2 = N.nvm[2 * nodec + nid];
+                const float wyz = Ay.w[j] * Az.w[k];
+                const float gx = (Ax.d[i] * inv0) * wyz;
+                const float gy = (Ay.d[j] * inv1) * (Ax.w[i] * Az.w[k]);
+                const float gz = (Az.d[k] * inv2) * (Ax.w[i] * Ay.w[j]);
+                Dv[0] += v0 * gx;
+                Dv[1] += v0 * gy;
+                Dv[2] += v0 * gz;
+                Dv[3] += v1 * gx;
+                Dv[4] += v1 * gy;
+                Dv[5] += v1 * gz;
+                Dv[6] += v2 * gx;
+                Dv[7] += v2 * gy;
+                Dv[8] += v2 * gz;
+            }
+    float D[9], H[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        D[q] = sc.dt * Dv[q];
+        H[q] = P.H[q * ld + slot];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            P.H[(3 * a + b) * ld + slot] = (H[3 * a + b] + D[3 * a + b]) +
+                                           (D[3 * a] * H[b] + D[3 * a + 1] * H[3 + b] + D[3 * a + 2] * H[6 + b]);
+#pragma unroll
+    for (int q = 0; q < 9; q++) P.Dd[q * ld + slot] = D[q];
+    const double dm = det_I_plus_H(D) - 1.0, jb = (double)P.jb[slot];
+    P.jm[slot] = (float)(jb + dm + jb * dm);
+}
+
+// Cell partials of sum N (Jbar det dF - 1) V over each corner (the cell's
+// particles in sorted order), one lane per cell, particles staged through
+// LDS as in the P2G cells kernel.
+template <int NX, int NY, int NZ>
+__global__ void __launch_bounds__(kSeg) ks_fbar_cells(GridGeom g, PartF32 P, CellLists L, float *__restrict__ part) {
+    constexpr int NN = NX * NY * NZ;
+    __shared__ float stage[5 * kBlk * kStageS];  // r (3), Jbar det dF - 1, V
+    const uint32_t n_active = L.ctr->n_active;
+    const int ngroups = (int)((n_active + kSeg - 1) / kSeg);
+    const int t = threadIdx.x;
+    const size_t ld = P.ld;
+    for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        CellInfo si;
+        uint32_t S, E;
+        wave_fence();
+        group_setup<1>(g, L, n_active, grp, si, S, E);
+        float acc[NN];
+#pragma unroll
+        for (int q = 0; q < NN; q++) acc[q] = 0.f;
+        for (uint32_t base = S; base < E; base += kChunk) {
+            const int nch = (int)min((uint32_t)kChunk, E - base);
+            uint32_t slot[kBlk];
+            load_slots(L, base, nch, slot);
+#pragma unroll
+            for (int r = 0; r < kBlk; r++) {
+                if (64 * r + t < nch) {
+                    const uint32_t sl = slot[r];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) lds_gather<kStageS>(P.r + a * ld, sl, stage, a, r);
+                    lds_gather<kStageS>(P.jm, sl, stage, 3, r);
+                    lds_gather<kStageS>(P.V32, sl, stage, 4, r);
+                }
+            }
+            wait_vm_all();
+            wave_fence();
+            const uint32_t lo = max(si.cs, base), hi = min(si.ce, base + (uint32_t)nch);
+            for (uint32_t pos = lo; pos < hi; pos++) {
+                const int i = (int)(pos - base);
+                AxisW<NX> Ax;
+                AxisW<NY> Ay;
+                AxisW<NZ> Az;
+                axis_weights<NX>(si.ix, stage[st_idx<kStageS>(0, i)], g.nex[0], Ax);
+                axis_weights<NY>(si.iy, stage[st_idx<kStageS>(1, i)], g.nex[1], Ay);
+                axis_weights<NZ>(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], Az);
+                const float jv = stage[st_idx<kStageS>(3, i)] * stage[st_idx<kStageS>(4, i)];
+#pragma unroll
+                for (int k = 0; k < NZ; k++)
+#pragma unroll
+                    for (int j = 0; j < NY; j++) {
+                        const float wyz = Ay.w[j] * Az.w[k];
+#pragma unroll
+                        for (int ii = 0; ii < NX; ii++) acc[ii + NX * (j + NY * k)] += (Ax.w[ii] * wyz) * jv;
+                    }
+            }
+            wave_fence();
+        }
+        if (si.ce > si.cs) {
+            const size_t ne = (size_t)g.nelem;
+#pragma unroll
+            for (int q = 0; q < NN; q++) part[(size_t)q * ne + si.c] = acc[q];
+        }
+    }
+}
+
+// Node average nJbar - 1 = sum N (Jbar det dF - 1) V / nvof where nvof > mtol,
+// and -1 (nJbar = 0) elsewhere (fbar_projection.cpp:44-51). The sum over a
+// node's cells is Neumaier-compensated in ascending cell order.
+template <int NX, int NY, int NZ>
+__global__ void __launch_bounds__(256) ks_fbar_nodes(GridGeom g, CellLists L, TileGeom tg, const float *__restrict__ part,
+                                                      NodesF32 N, StepConsts sc) {
+    int n, i, j, k;
+    if (!tile_node(g, L, tg, false, n, i, j, k)) return;
+    const size_t ne = (size_t)g.nelem;
+    float sm = 0.f, cp = 0.f;
+    const bool any = for_node_cells<NX, NY, NZ>(g, L, i, j, k, [&](int c, int q) {
+        neu_add(sm, cp, part[(size_t)q * ne + c]);
+    });
+    if (!any) return;
+    const float nv = N.nvof[n];
+    N.nje[n] = ((double)nv > sc.mtol) ? (float)((double)(sm + cp) / (double)nv) : -1.f;
 }
 
 // ------------------------------------------------------ particle shifting
@@ -1056,15 +1256,49 @@ void launch_s32_shift(hipStream_t s, const GridGeom &g, const PartF32 &P, const 
     hipLaunchKernelGGL(ks_shift_b0, dim3(1), dim3(64), 0, s, nbe, nbd, mtol, S);
 }
 
+namespace {
+
+template <int NX, int NY, int NZ>
+void fbar_launch(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                 const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials) {
+    hipLaunchKernelGGL((ks_fbar_grad<NX, NY, NZ>), dim3(cdiv(P.n, 256)), dim3(256), 0, s, g, P, N, T, c);
+    const int ng = c.cells_grid > 0 ? c.cells_grid : cell_groups(g);
+    hipLaunchKernelGGL((ks_fbar_cells<NX, NY, NZ>), dim3(ng), dim3(kSeg), 0, s, g, P, L, partials);
+    hipLaunchKernelGGL((ks_fbar_nodes<NX, NY, NZ>), dim3(tg.ntiles), dim3(256), 0, s, g, L, tg, (const float *)partials,
+                       N, c);
+}
+
+template <int NX, int NY, int NZ>
+void update_launch(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const NodeTables &T,
+                   const MaterialTable &mt, const StepConsts &c, uint32_t *err, const BinFuse &bf) {
+    const dim3 grid(cdiv(P.n, 256)), block(256);
+    if (c.fbar)
+        hipLaunchKernelGGL((ks_g2p_update<NX, NY, NZ, true>), grid, block, 0, s, g, P, N, T, mt, c, err, bf);
+    else
+        hipLaunchKernelGGL((ks_g2p_update<NX, NY, NZ, false>), grid, block, 0, s, g, P, N, T, mt, c, err, bf);
+}
+
+}  // namespace
+
+void launch_s32_fbar(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
+                     const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials) {
+    if (P.n <= 0) return;
+    switch (stencil_kind(g)) {
+    case 0: fbar_launch<2, 2, 2>(s, g, P, N, L, T, tg, c, partials); break;
+    case 1: fbar_launch<3, 3, 3>(s, g, P, N, L, T, tg, c, partials); break;
+    case 2: fbar_launch<3, 2, 3>(s, g, P, N, L, T, tg, c, partials); break;
+    default: break;
+    }
+}
+
 void launch_s32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N,
                            const NodeTables &T, const MaterialTable &mt, const StepConsts &c, uint32_t *err,
                            const BinFuse &bf) {
     if (P.n <= 0) return;
-    const dim3 grid(cdiv(P.n, 256)), block(256);
     switch (stencil_kind(g)) {
-    case 0: hipLaunchKernelGGL((ks_g2p_update<2, 2, 2>), grid, block, 0, s, g, P, N, T, mt, c, err, bf); break;
-    case 1: hipLaunchKernelGGL((ks_g2p_update<3, 3, 3>), grid, block, 0, s, g, P, N, T, mt, c, err, bf); break;
-    case 2: hipLaunchKernelGGL((ks_g2p_update<3, 2, 3>), grid, block, 0, s, g, P, N, T, mt, c, err, bf); break;
+    case 0: update_launch<2, 2, 2>(s, g, P, N, T, mt, c, err, bf); break;
+    case 1: update_launch<3, 3, 3>(s, g, P, N, T, mt, c, err, bf); break;
+    case 2: update_launch<3, 2, 3>(s, g, P, N, T, mt, c, err, bf); break;
     default: break;
     }
 }

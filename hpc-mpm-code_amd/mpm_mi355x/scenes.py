@@ -61,10 +61,10 @@ class Scene:
     nlstep: int = 1
     bcs: dict = field(default_factory=dict)   # comp -> (nbc, fbc)
     parts: object = None
-    idimc: tuple = (1, 1, 1)                 # 1 = linear tent, 2 = quadratic B-spline (FP64-exact engine)
-    shifting: bool = False                   # reference particle shifting (FP64-exact engine)
+    idimc: tuple = (1, 1, 1)                 # 1 = linear tent, 2 = quadratic B-spline (open, per axis)
+    shifting: bool = False                   # reference particle shifting
     npxye: tuple = (2, 2, 2)                 # points per cell of the nodal-volume reference (MakNodalVol)
-    fbar: bool = False                       # F-bar projection (FP64-exact engine)
+    fbar: bool = False                       # F-bar projection
     mat_dt: list = None                      # per-material time steps (FP32 engine), or None: dt for all
 
     @property

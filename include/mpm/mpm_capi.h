@@ -305,7 +305,7 @@ int mpm_implicit_p2g(mpm_engine *eng, double facl);
 int mpm_implicit_g2p(mpm_engine *eng, const double *ndispl, const double *nb_para, double gamma_nb);
 
 /* F-bar projection (cal_def_grad_bar, module/solid/explicit/fbar_projection.cpp:18-94;
- * Fbar_flag, off by default in the reference), FP64-exact engine only. After
+ * Fbar_flag, off by default in the reference), both engines. After
  * the deformation-gradient update, N (Jbar det dF) V is gathered to the
  * nodes, divided by nvof, interpolated back as Jbar; dF is scaled by
  * cbrt(Jbar / (Jbar_old det dF)) and Fbar = dF Fbar_old. Stress then uses
@@ -313,7 +313,11 @@ int mpm_implicit_g2p(mpm_engine *eng, const double *ndispl, const double *nb_par
  * the volume stays J V0. The reference passes delta_def_grad in a layout
  * its declaration does not accept; this is the projection with
  * delta_def_grad[p][i][j] = dF_ij. Turning it on (or uploading particles
- * while on) starts from Fbar = I, Jbar = 1. */
+ * while on) starts from Fbar = I, Jbar = 1. The FP32 engine keeps Fbar - I
+ * and Jbar - 1 in FP32, averages (Jbar det dF - 1) V at the nodes (nodes
+ * with nvof <= mtol take Jbar = 0, as the reference), runs the projection in
+ * its stencil kernels (the tent functions included) and refuses it while the
+ * constitutive surrogate or per-material time steps are on. */
 int mpm_set_fbar(mpm_engine *eng, int32_t on);
 /* Fbar [n][3][3] and Jbar [n] (NULL to skip); the upload restores a restart. */
 int mpm_download_fbar(mpm_engine *eng, double *def_grad_bar, double *det_def_grad_bar);

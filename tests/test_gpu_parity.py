@@ -842,9 +842,70 @@ def test_fbar_state_upload_restores_a_run():
     assert np.array_equal(b.fbar_state(s.n)[1], a.fbar_state(s.n)[1])
 
 
-def test_fp32_rejects_fbar():
+@pytest.mark.parametrize("model,props", [(capi.MODEL_NEOHOOKEAN, (5e5, 0.35)), (capi.MODEL_LINELAST, (1e6, 0.3))])
+@pytest.mark.parametrize("idimc,shifting", [((1, 1, 1), False), ((2, 2, 2), False), ((2, 1, 2), False),
+                                            ((2, 1, 2), True)])
+def test_fp32_fbar_run(model, props, idimc, shifting):
+    """The F-bar projection in the FP32 engine (Fbar - I and Jbar - 1 in FP32,
+    the node average of (Jbar det dF - 1) V, the FP64 cube-root scaling), 50
+    steps with relayouts every 4 steps, against the oracle. Measured over the
+    eight cases: position 1.8e-7 of the unit domain (the FP32 offsets over 50
+    steps), Fbar 1e-9, Jbar 1.2e-10, stress 7.4e-7 relative, and the
+    projection's own effect on the stress (F-bar run minus plain run, 1-5% of
+    the stress) 4e-5 relative of the oracle's. Bounds: 5e-7, 3e-9, 5e-10,
+    3e-6 and 2e-4."""
+    s = scenes.tiny(n_cells=8, model=model, props=props, scheme=capi.SCHEME_FLIP, idimc=idimc)
+    s.fbar = True
+    s.shifting = shifting
+    if shifting:
+        c = s.parts.coord.mean(axis=0)
+        s.parts.coord[:] = c + 0.93 * (s.parts.coord - c)
+    print("fp32 fbar model=%d idimc=%s shifting=%d" % (model, idimc, shifting), flush=True)
+    o = oracle_run(s, 50)
+    e = engine(s, capi.PREC_FP32, relayout=4)
+    assert e.step(1, 50) == 0
+    got = e.download(s.n)
+    fb, jb = e.fbar_state(s.n)
+    s.fbar = False
+    o0 = oracle_run(s, 50)
+    eff_ref = o.parts.stress - o0.parts.stress
+    eff_got = got.stress - o0.parts.stress
+    errs = (float(np.max(np.abs(got.coord - o.parts.coord))), float(np.max(np.abs(fb - o.def_grad_bar))),
+            float(np.max(np.abs(jb - o.det_def_grad_bar))), rel(got.stress, o.parts.stress), rel(eff_got, eff_ref))
+    print("  position %.2e Fbar %.2e Jbar %.2e stress %.2e effect %.2e (effect/stress %.2e)"
+          % (errs + (np.linalg.norm(eff_ref) / np.linalg.norm(o.parts.stress),)), flush=True)
+    assert np.linalg.norm(eff_ref) > 1e-4 * np.linalg.norm(o.parts.stress)
+    assert errs[0] < 5e-7 and errs[1] < 3e-9 and errs[2] < 5e-10 and errs[3] < 3e-6 and errs[4] < 2e-4
+
+
+def test_fp32_fbar_state_round_trip_and_restart():
+    """The FP32 F-bar state downloads and uploads unchanged (to FP32
+    rounding), and a run continued from a restart stays with the
+    uninterrupted one."""
+    s = scenes.tiny(n_cells=8, model=capi.MODEL_NEOHOOKEAN, props=(5e5, 0.35))
+    s.fbar = True
+    a = engine(s, capi.PREC_FP32)
+    assert a.step(1, 10) == 0
+    mid = a.download(s.n)
+    fb, jb = a.fbar_state(s.n)
+    assert np.abs(jb - 1.0).max() > 0.0
+    assert a.step(11, 10) == 0
+    b = capi.Engine(s, precision=capi.PREC_FP32)
+    b.upload(mid)
+    b.set_fbar_state(fb, jb)
+    fb2, jb2 = b.fbar_state(s.n)
+    np.testing.assert_allclose(fb2, fb, rtol=0, atol=1e-7)
+    np.testing.assert_allclose(jb2, jb, rtol=0, atol=1e-7)
+    assert b.step(11, 10) == 0
+    ga, gb = a.download(s.n), b.download(s.n)
+    assert np.max(np.abs(ga.coord - gb.coord)) < 1e-7
+    assert rel(gb.stress, ga.stress) < 1e-5
+
+
+def test_fp32_fbar_refuses_per_material_time_steps():
     s = scenes.tiny(n_cells=8)
     e = engine(s, capi.PREC_FP32)
+    e.set_material_dt([s.dt])
     with pytest.raises(capi.MPMError) as ei:
         e.set_fbar(True)
     assert ei.value.code == capi.MPM_ERR_UNSUPPORTED
