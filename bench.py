@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="cfg2-4M",
-                    choices=["cfg1", "cfg2", "cfg2-4M", "cfg2-q", "cfg3", "cfg5", "cfg5-16M"])
+                    choices=["cfg1", "cfg2", "cfg2-4M", "cfg2-q", "cfg2-q212", "cfg2-q4M", "cfg3", "cfg5", "cfg5-16M"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--relayout", type=int, default=32)
     ap.add_argument("--fused-count", type=int, default=1, help="count cells in the update kernel (1) or not (0)")
@@ -114,6 +114,10 @@ def make_scene(name):
         return scenes.cfg2(ppc=(2, 2, 2))
     if name == "cfg2-q":  # the cfg2 geometry on the quadratic B-spline (27 control points per particle)
         return scenes.cfg2(ppc=(2, 2, 2), idimc=(2, 2, 2))
+    if name == "cfg2-q212":  # idimc 2 1 2, the shape functions of the reference's shipped cases (18 points)
+        return scenes.cfg2(ppc=(2, 2, 2), idimc=(2, 1, 2))
+    if name == "cfg2-q4M":  # the quadratic B-spline at 2x4x2 points per cell (4,194,304 particles)
+        return scenes.cfg2(ppc=(2, 4, 2), idimc=(2, 2, 2))
     return scenes.cfg2(ppc=(2, 4, 2))
 
 

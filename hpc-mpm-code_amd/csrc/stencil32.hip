@@ -89,6 +89,36 @@ __device__ __forceinline__ void axis_weights(int c, float r, int nex, AxisW<W> &
     }
 }
 
+// One control point o (0, 1 or 2, the same for a lane's whole particle loop)
+// of the quadratic B-spline, with axis_weights<3>'s expressions: the weight,
+// its r-derivative and the offset, by selects only (an indexed register array
+// would be placed in scratch memory).
+__device__ __forceinline__ void axis_weight_at(int c, float r, int nex, int o, float &w, float &d, float &x) {
+    const float s = 1.f - r, h = r - 0.5f;
+    const float w0 = 0.5f * s * s, w1 = 0.75f - h * h, w2 = 0.5f * r * r;
+    const float d0 = -s, d1 = -2.f * h, d2 = r;
+    const float x0 = -0.5f - r, x1 = 0.5f - r, x2 = 1.5f - r;
+    w = o == 0 ? w0 : (o == 1 ? w1 : w2);
+    d = o == 0 ? d0 : (o == 1 ? d1 : d2);
+    x = o == 0 ? x0 : (o == 1 ? x1 : x2);
+    const int last = nex + 1;
+    const int gi = c + o;
+    const bool e0 = gi == 0;
+    const bool e1 = !e0 && gi == 1;
+    const bool eL = !e0 && !e1 && gi == last;
+    const bool eL1 = !e0 && !e1 && !eL && gi == last - 1;
+    w = (o == 1 && e1) ? r * (2.f - 1.5f * r) : w;
+    d = (o == 1 && e1) ? 2.f - 3.f * r : d;
+    w = (o == 1 && eL1) ? 0.5f + r * (1.f - 1.5f * r) : w;
+    d = (o == 1 && eL1) ? 1.f - 3.f * r : d;
+    w = e0 ? s * s : w;
+    d = e0 ? -2.f * s : d;
+    x = e0 ? -r : x;
+    w = eL ? r * r : w;
+    d = eL ? 2.f * r : d;
+    x = eL ? (float)(nex - c) - r : x;
+}
+
 // 3x3 inverse through FP64 (the APIC D matrix is about dx^2 / 4 I).
 __device__ __forceinline__ void inv3_f(const float a[9], float o[9]) {
     double A[9];
@@ -134,15 +164,6 @@ __device__ __forceinline__ void affine_matrix(const float *stage, int i, int fC,
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) Mh[a][b] = M[3 * a + b] * dx[b];
-}
-
-// entry i of a W-entry array in registers, without a dynamic register index
-template <int W>
-__device__ __forceinline__ float pick(const float (&a)[W], int i) {
-    float v = a[0];
-#pragma unroll
-    for (int o = 1; o < W; o++) v = (i == o) ? a[o] : v;
-    return v;
 }
 
 // ------------------------------------------------------------------ P2G
@@ -246,16 +267,14 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                 const int i = (int)(pos - base);
                 AxisW<NX> Ax;
                 AxisW<NY> Ay;
-                AxisW<NZ> Azf;
+                AxisW<KL> Az;  // this lane's z-layers
                 axis_weights<NX>(si.ix, stage[st_idx<kStageS>(0, i)], g.nex[0], Ax);
                 axis_weights<NY>(si.iy, stage[st_idx<kStageS>(1, i)], g.nex[1], Ay);
-                axis_weights<NZ>(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], Azf);
-                AxisW<KL> Az;  // this lane's z-layers
-#pragma unroll
-                for (int kl = 0; kl < KL; kl++) {
-                    Az.w[kl] = (LZ == 1) ? Azf.w[kl] : pick<NZ>(Azf.w, k0 + kl);
-                    Az.d[kl] = (LZ == 1) ? Azf.d[kl] : pick<NZ>(Azf.d, k0 + kl);
-                    Az.x[kl] = (LZ == 1) ? Azf.x[kl] : pick<NZ>(Azf.x, k0 + kl);
+                if constexpr (LZ == 1) {
+                    axis_weights<NZ>(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], Az);
+                } else {
+                    static_assert(NZ == 3 && KL == 1, "one quadratic z-layer per lane");
+                    axis_weight_at(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], k0, Az.w[0], Az.d[0], Az.x[0]);
                 }
                 const float m = stage[st_idx<kStageS>(6, i)];
                 const float mv[3] = {m * stage[st_idx<kStageS>(3, i)], m * stage[st_idx<kStageS>(4, i)],
