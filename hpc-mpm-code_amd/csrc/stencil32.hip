@@ -308,6 +308,10 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                         for (int o = 0; o < KL; o++) pz[a][o] = (m * Mh[a][2]) * Az.x[o];
                     }
                 }
+                // Tensor-product form: with the y and z factors of a corner row
+                // folded once per row, a corner costs one FMA per quantity for
+                // mass, momentum and volume and two for each force component:
+                // f_a = dN/dr_x (X_a wy wz) + N_x (Y_a dy wz + Z_a wy dz + t_a wy wz)
 #pragma unroll
                 for (int k = 0; k < KL; k++)
 #pragma unroll
@@ -315,19 +319,29 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                         const float wyz = Ay.w[j] * Az.w[k];
                         const float dyz = Ay.d[j] * Az.w[k];
                         const float wydz = Ay.w[j] * Az.d[k];
+                        float Fx[3], Fw[3], Mv[3];
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            Fx[a] = X[a] * wyz;
+                            Fw[a] = Y[a] * dyz + (Z[a] * wydz + tr[a] * wyz);
+                            Mv[a] = mv[a] * wyz;
+                        }
+                        const float Mw = m * wyz, Vw = Vp * wyz;
 #pragma unroll
                         for (int ii = 0; ii < NX; ii++) {
                             const int q = ii + NX * (j + NY * k);
-                            const float N = Ax.w[ii] * wyz;
-                            const float gx = Ax.d[ii] * wyz, gy = Ax.w[ii] * dyz, gz = Ax.w[ii] * wydz;
-                            acc[q][0] += N * m;
+                            const float wx = Ax.w[ii];
+                            acc[q][0] += wx * Mw;
 #pragma unroll
                             for (int a = 0; a < 3; a++) {
-                                const float va = (AFF >= 1) ? (px[a][ii] + py[a][j]) + pz[a][k] : mv[a];
-                                acc[q][1 + a] += N * va;
-                                acc[q][4 + a] += (X[a] * gx + Y[a] * gy) + (Z[a] * gz + N * tr[a]);
+                                if constexpr (AFF >= 1)
+                                    acc[q][1 + a] += (wx * wyz) * ((px[a][ii] + py[a][j]) + pz[a][k]);
+                                else
+                                    acc[q][1 + a] += wx * Mv[a];
+                                acc[q][4 + a] += Ax.d[ii] * Fx[a];
+                                acc[q][4 + a] += wx * Fw[a];
                             }
-                            vof[q] += N * Vp;
+                            vof[q] += wx * Vw;
                         }
                     }
             }
