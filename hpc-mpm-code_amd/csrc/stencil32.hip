@@ -800,36 +800,45 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
     for (int q = 0; q < 9; q++) Dv[q] = 0.f;
     float dxp[3] = {0.f, 0.f, 0.f};
     float je = 0.f;  // FB: Jbar - 1 = sum N (nJbar - 1)
+    // tensor-product form: along each node row (fixed j, k) the x sums
+    // sum_i v N_x and sum_i v dN_x/dr first, then the row's y and z factors
 #pragma unroll
     for (int k = 0; k < NZ; k++)
 #pragma unroll
-        for (int j = 0; j < NY; j++)
+        for (int j = 0; j < NY; j++) {
+            float rw[3] = {0.f, 0.f, 0.f}, rd[3] = {0.f, 0.f, 0.f}, rj = 0.f;
 #pragma unroll
             for (int i = 0; i < NX; i++) {
                 const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
-                const float v0 = N.nvm[nid], v1 = N.nvm[nodec + nid], v2 = N.nvm[2 * nodec + nid];
-                const float wyz = Ay.w[j] * Az.w[k];
-                const float Nw = Ax.w[i] * wyz;
-                if constexpr (FB) {
-                    je += Nw * N.nje[nid];
-                } else {
-                    const float gx = (Ax.d[i] * inv0) * wyz;
-                    const float gy = (Ay.d[j] * inv1) * (Ax.w[i] * Az.w[k]);
-                    const float gz = (Az.d[k] * inv2) * (Ax.w[i] * Ay.w[j]);
-                    Dv[0] += v0 * gx;
-                    Dv[1] += v0 * gy;
-                    Dv[2] += v0 * gz;
-                    Dv[3] += v1 * gx;
-                    Dv[4] += v1 * gy;
-                    Dv[5] += v1 * gz;
-                    Dv[6] += v2 * gx;
-                    Dv[7] += v2 * gy;
-                    Dv[8] += v2 * gz;
+                const float v[3] = {N.nvm[nid], N.nvm[nodec + nid], N.nvm[2 * nodec + nid]};
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    rw[a] += Ax.w[i] * v[a];
+                    if constexpr (!FB) rd[a] += Ax.d[i] * v[a];
                 }
-                dxp[0] += Nw * v0;
-                dxp[1] += Nw * v1;
-                dxp[2] += Nw * v2;
+                if constexpr (FB) rj += Ax.w[i] * N.nje[nid];
             }
+            const float wyz = Ay.w[j] * Az.w[k];
+#pragma unroll
+            for (int a = 0; a < 3; a++) dxp[a] += wyz * rw[a];
+            if constexpr (FB) {
+                je += wyz * rj;
+            } else {
+                const float dyz = Ay.d[j] * Az.w[k], wydz = Ay.w[j] * Az.d[k];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    Dv[3 * a] += wyz * rd[a];
+                    Dv[3 * a + 1] += dyz * rw[a];
+                    Dv[3 * a + 2] += wydz * rw[a];
+                }
+            }
+        }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        Dv[3 * a] *= inv0;
+        Dv[3 * a + 1] *= inv1;
+        Dv[3 * a + 2] *= inv2;
+    }
     float D[9], Hn[9];
     if constexpr (FB) {
 #pragma unroll
