@@ -141,6 +141,9 @@ struct mpm_engine {
     // F-bar projection (FP64-exact engine)
     bool fbar = false;
     double *d_njbar = nullptr;
+    // FP64-exact engine: per-particle values of the two-pass P2G / MUSL sums
+    // (allocated with the particles; null: one-pass gathers)
+    double *d_contrib = nullptr;
 
     // whole-step graph: blocks of graph_block steps captured once and replayed
     int graph_block = 0;
@@ -224,6 +227,7 @@ struct mpm_engine {
         bb.perm = nullptr;
         perm = nullptr;
         perm_id = nullptr;
+        d_contrib = nullptr;
         n = 0;
         bins_valid = keys_valid = false;
     }
@@ -343,11 +347,15 @@ int do_bin(mpm_engine *e) {
     return 0;
 }
 
+// Scratch of the FP64 two-pass sums (allocated with the particles; null when
+// that allocation failed: the one-pass gathers run, with the same results).
+double *f64_contrib(mpm_engine *e) { return e->d_contrib; }
+
 int do_p2g(mpm_engine *e, double facl, int diag, hipEvent_t mid = nullptr) {
     if (e->shifting || e->fbar) diag = 1;  // shifting and F-bar read the P2G nodal volume
     if (e->prec == MPM_PREC_FP64_EXACT) {
         launch_f64_p2g(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->prm.scheme, e->prm.bb[0] * facl,
-                       e->prm.bb[1] * facl, e->prm.bb[2] * facl, e->prm.mtol, facl);
+                       e->prm.bb[1] * facl, e->prm.bb[2] * facl, e->prm.mtol, facl, f64_contrib(e));
     } else if (e->stencil) {
         launch_s32_p2g(e->stream, e->g, e->P32[e->cur], e->N32, e->lists(), e->bc(), e->T, e->tg, consts(e, facl),
                        e->partials, diag, mid);
@@ -394,7 +402,8 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
             if (rc) return rc;
         }
         launch_f64_g2p(e->stream, e->g, e->P64, e->N64, e->lists(), e->bc(), e->mt, e->prm.scheme, e->prm.dt,
-                       e->prm.mtol, facl, e->d_err, e->shifting ? &sh : nullptr, e->fbar ? e->d_njbar : nullptr);
+                       e->prm.mtol, facl, e->d_err, e->shifting ? &sh : nullptr, e->fbar ? e->d_njbar : nullptr,
+                       f64_contrib(e));
         e->keys_valid = false;
     } else if (e->stencil) {
         StepConsts c = consts(e, facl);
@@ -561,6 +570,14 @@ int alloc_particles(mpm_engine *e, int n) {
         P.key = e->keys64;
         P.ld = ld;
         P.n = n;
+        // optional: a failed allocation leaves the one-pass gathers in use
+        const size_t cbytes = f64_contrib_doubles(e->g, ld) * sizeof(double);
+        if (hipMalloc((void **)&e->d_contrib, cbytes) == hipSuccess) {
+            e->part_allocs.push_back((void *)e->d_contrib);
+        } else {
+            (void)hipGetLastError();
+            e->d_contrib = nullptr;
+        }
     } else {
         int nbuf = e->relayout_every > 0 ? 2 : 1;
         for (int b = 0; b < nbuf; b++)

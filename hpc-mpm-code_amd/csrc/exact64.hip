@@ -159,7 +159,7 @@ __device__ __forceinline__ void affine_vel(const GridGeom &g, const PartF64 &P, 
 }
 
 // Particle2NodeS (var_trans_solid_explicit.cpp:20-96) for one node.
-__global__ void k64_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, int scheme, double fx, double fy,
+__global__ void __launch_bounds__(128) k64_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, int scheme, double fx, double fy,
                         double fz, double mtol, double facl) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodec) return;
@@ -227,6 +227,184 @@ __global__ void k64_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense 
     N.nv[2 * nodec + n] = v2;
 }
 
+// ------------------------------------------------------------------
+// The same P2G and MUSL sums in two passes. The per-node gathers above
+// evaluate each particle's weights once for every node of its stencil and
+// read its fields from scattered addresses; here one thread per binned
+// particle (sorted position s) forms the values a node adds for it -- the
+// same expressions as above, so the same doubles -- and stores them at
+// [s][corner][quantity]; a node then adds them in the same order (cells
+// ascending, particles in list order) from contiguous memory. The sums are
+// bit-identical to the single-pass gathers and to the reference's scatter.
+constexpr int kP2GQ = 8;   // nm, nvof, p (3), f (3)
+constexpr int kMuslQ = 3;  // p (3)
+
+__device__ __forceinline__ int stencil_corners(const GridGeom &g) {
+    return (g.idimc[0] + 1) * (g.idimc[1] + 1) * (g.idimc[2] + 1);
+}
+
+template <bool MUSL>
+__global__ void __launch_bounds__(128) k64_contrib(GridGeom g, PartF64 P, CellLists L, int scheme, double fx, double fy, double fz,
+                            double *__restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = L.astart[L.ctr->n_active];
+    if (s >= total) return;
+    constexpr int Q = MUSL ? kMuslQ : kP2GQ;
+    const uint32_t pid = L.perm[s];
+    const size_t ld = P.ld;
+    const int nen = stencil_corners(g);
+    int ixe, iye, ize;
+    cell_ijk(g, (int)P.key[pid], ixe, iye, ize);
+    double xyg[3] = {P.x[pid], P.x[ld + pid], P.x[2 * ld + pid]};
+    const double mp = P.m[pid];
+    double vp = 0.0, s_0 = 0.0, s_1 = 0.0, s_2 = 0.0, s_3 = 0.0, s_4 = 0.0, s_5 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (!MUSL) {
+        vp = P.V[pid];
+        s_0 = P.sig[pid];
+        s_1 = P.sig[ld + pid];
+        s_2 = P.sig[2 * ld + pid];
+        s_3 = P.sig[3 * ld + pid];
+        s_4 = P.sig[4 * ld + pid];
+        s_5 = P.sig[5 * ld + pid];
+        if (P.trac) {
+            t0 = P.trac[pid];
+            t1 = P.trac[ld + pid];
+            t2 = P.trac[2 * ld + pid];
+        }
+    }
+    double *o = out + (size_t)s * nen * Q;
+    int q = 0;
+    for (int dk = 0; dk <= g.idimc[2]; dk++)
+        for (int dj = 0; dj <= g.idimc[1]; dj++)
+            for (int di = 0; di <= g.idimc[0]; di++, q++) {
+                const int i = ixe + di, j = iye + dj, k = ize + dk;
+                double sfi, ds[3];
+                sf_node(g, xyg, i, j, k, sfi, ds);
+                double va[3] = {P.v[pid], P.v[ld + pid], P.v[2 * ld + pid]};
+                if (scheme >= 2) affine_vel(g, P, pid, i, j, k, xyg, scheme, va);
+                double *c = o + q * Q;
+                if (MUSL) {
+                    c[0] = sfi * mp * va[0];
+                    c[1] = sfi * mp * va[1];
+                    c[2] = sfi * mp * va[2];
+                } else {
+                    c[0] = sfi * mp;
+                    c[1] = sfi * vp;
+                    c[2] = sfi * mp * va[0];
+                    c[3] = sfi * mp * va[1];
+                    c[4] = sfi * mp * va[2];
+                    const double fi0 = -vp * (s_0 * ds[0] + s_5 * ds[1] + s_4 * ds[2]);
+                    const double fi1 = -vp * (s_5 * ds[0] + s_1 * ds[1] + s_3 * ds[2]);
+                    const double fi2 = -vp * (s_4 * ds[0] + s_3 * ds[1] + s_2 * ds[2]);
+                    const double fe0 = sfi * (mp * fx + t0);
+                    const double fe1 = sfi * (mp * fy + t1);
+                    const double fe2 = sfi * (mp * fz + t2);
+                    c[5] = fi0 + fe0;
+                    c[6] = fi1 + fe1;
+                    c[7] = fi2 + fe2;
+                }
+            }
+}
+
+// Adds the stored values of node n in the reference's order: cells around the
+// node ascending (k, j, i), each cell's particles in list order.
+template <int Q, typename F>
+__device__ __forceinline__ void gather_contrib(const GridGeom &g, const CellLists &L, const double *__restrict__ in,
+                                               int i, int j, int k, F add) {
+    const int r0 = g.idimc[0], r1 = g.idimc[1], r2 = g.idimc[2];
+    const int nen = (r0 + 1) * (r1 + 1) * (r2 + 1);
+    for (int ck = k - r2; ck <= k; ck++) {
+        if (ck < 0 || ck >= g.nex[2]) continue;
+        for (int cj = j - r1; cj <= j; cj++) {
+            if (cj < 0 || cj >= g.nex[1]) continue;
+            for (int ci = i - r0; ci <= i; ci++) {
+                if (ci < 0 || ci >= g.nex[0]) continue;
+                const int c = ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1];
+                const int corner = (i - ci) + (r0 + 1) * ((j - cj) + (r1 + 1) * (k - ck));
+                const uint32_t s0 = L.cstart[c], s1 = L.cend[c];
+                // four particles' values loaded together, then added in order
+                uint32_t s = s0;
+                for (; s + 4 <= s1; s += 4) {
+                    double v[4][Q];
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+#pragma unroll
+                        for (int q = 0; q < Q; q++) v[u][q] = in[((size_t)(s + u) * nen + corner) * Q + q];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) add(v[u]);
+                }
+                for (; s < s1; s++) add(in + ((size_t)s * nen + corner) * Q);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(128) k64_p2g_gather(GridGeom g, NodesF64 N, CellLists L, BcDense bc, double mtol, double facl,
+                               const double *__restrict__ in) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    const int nodec = g.nodec;
+    const int i = n % g.nxc[0], j = (n / g.nxc[0]) % g.nxc[1], k = n / (g.nxc[0] * g.nxc[1]);
+    double nm = 0.0, nvof = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    gather_contrib<kP2GQ>(g, L, in, i, j, k, [&](const double *c) {
+        nm += c[0];
+        nvof += c[1];
+        p0 += c[2];
+        p1 += c[3];
+        p2 += c[4];
+        f0 += c[5];
+        f1 += c[6];
+        f2 += c[7];
+    });
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (nm > mtol) {
+        v0 = p0 / nm;
+        v1 = p1 / nm;
+        v2 = p2 / nm;
+    }
+    if (nvof < mtol) nvof = 0.0;
+    apply_vel_bc(bc, n, nodec, facl, v0, v1, v2);
+    N.nm[n] = nm;
+    N.nvof[n] = nvof;
+    N.np[n] = p0;
+    N.np[nodec + n] = p1;
+    N.np[2 * nodec + n] = p2;
+    N.nf[n] = f0;
+    N.nf[nodec + n] = f1;
+    N.nf[2 * nodec + n] = f2;
+    N.nv[n] = v0;
+    N.nv[nodec + n] = v1;
+    N.nv[2 * nodec + n] = v2;
+}
+
+__global__ void __launch_bounds__(128) k64_musl_gather(GridGeom g, NodesF64 N, CellLists L, BcDense bc, double mtol, double facl,
+                                const double *__restrict__ in) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodec) return;
+    const int nodec = g.nodec;
+    const int i = n % g.nxc[0], j = (n / g.nxc[0]) % g.nxc[1], k = n / (g.nxc[0] * g.nxc[1]);
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    gather_contrib<kMuslQ>(g, L, in, i, j, k, [&](const double *c) {
+        p0 += c[0];
+        p1 += c[1];
+        p2 += c[2];
+    });
+    const double nm = N.nm[n];
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (nm > mtol) {
+        v0 = p0 / nm;
+        v1 = p1 / nm;
+        v2 = p2 / nm;
+    }
+    apply_vel_bc(bc, n, nodec, facl, v0, v1, v2);
+    N.np[n] = p0;
+    N.np[nodec + n] = p1;
+    N.np[2 * nodec + n] = p2;
+    N.nv[n] = v0;
+    N.nv[nodec + n] = v1;
+    N.nv[2 * nodec + n] = v2;
+}
+
 // SolveSolid (solve_solid_explicit.cpp:8-31)
 __global__ void k64_solve_grid(GridGeom g, NodesF64 N, BcDense bc, double mtol) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,7 +433,7 @@ __global__ void k64_vel_plus_dt_acc(int n3, double *nv, const double *na, double
 }
 
 // var_trans_solid_explicit.cpp:202-240 (PIC / FLIP velocity G2P)
-__global__ void k64_g2p_vel(GridGeom g, PartF64 P, NodesF64 N, int scheme, double dt) {
+__global__ void __launch_bounds__(128) k64_g2p_vel(GridGeom g, PartF64 P, NodesF64 N, int scheme, double dt) {
     int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P.n) return;
     uint32_t m = P.key[pid];
@@ -333,7 +511,7 @@ __global__ void k64_g2p_vel(GridGeom g, PartF64 P, NodesF64 N, int scheme, doubl
 }
 
 // MUSL (var_trans_solid_explicit.cpp:101-148) for one node
-__global__ void k64_musl(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, int scheme, double mtol,
+__global__ void __launch_bounds__(128) k64_musl(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, int scheme, double mtol,
                          double facl) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodec) return;
@@ -531,7 +709,7 @@ __device__ void update_stress64(const MaterialDev &md, double *sig, size_t ld, i
 
 // Deformation gradient, coordinates, volume and stress
 // (var_trans_solid_explicit.cpp:244-296 and :150-185), one thread per particle.
-__global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err,
+__global__ void __launch_bounds__(128) k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err,
                                const double *__restrict__ geup, const double *__restrict__ b0p, int fbar) {
     int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P.n) return;
@@ -604,7 +782,7 @@ __global__ void k64_g2p_update(GridGeom g, PartF64 P, NodesF64 N, MaterialTable 
 
 // cal_def_grad_bar (fbar_projection.cpp:18-48) for one node: sum of
 // N (Jbar_p det dF_p) V_p in the reference's order, divided by nvof.
-__global__ void k64_fbar_nodes(GridGeom g, PartF64 P, NodesF64 N, CellLists L, double mtol, double *njbar) {
+__global__ void __launch_bounds__(128) k64_fbar_nodes(GridGeom g, PartF64 P, NodesF64 N, CellLists L, double mtol, double *njbar) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodec) return;
     const size_t ld = P.ld;
@@ -632,7 +810,7 @@ __global__ void k64_fbar_nodes(GridGeom g, PartF64 P, NodesF64 N, CellLists L, d
 // cal_def_grad_bar (fbar_projection.cpp:50-91) and then
 // update_point_coord_and_stress (var_trans_solid_explicit.cpp:150-185) with
 // Fbar, Jbar and the scaled increment, one thread per particle.
-__global__ void k64_fbar_finish(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err,
+__global__ void __launch_bounds__(128) k64_fbar_finish(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, double dt, uint32_t *err,
                                 const double *__restrict__ njbar, const double *__restrict__ geup,
                                 const double *__restrict__ b0p) {
     int pid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -693,7 +871,7 @@ __global__ void k64_fbar_finish(GridGeom g, PartF64 P, NodesF64 N, MaterialTable
     if (!isfinite(dx0) || !isfinite(dx1) || !isfinite(dx2)) atomicOr(err, kErrNonFinite);
 }
 
-__global__ void k64_maksf(GridGeom g, int n, const int32_t *cell, const double *xyg, int32_t *nc, double *sf,
+__global__ void __launch_bounds__(128) k64_maksf(GridGeom g, int n, const int32_t *cell, const double *xyg, int32_t *nc, double *sf,
                           double *dsf) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -727,7 +905,7 @@ __global__ void k64_maksf(GridGeom g, int n, const int32_t *cell, const double *
 // map_and_interpolate.cpp:12-49) in the reference's order; velocity and
 // acceleration where the mass exceeds mtol; velocity BCs set, acceleration
 // BCs zeroed.
-__global__ void k64_implicit_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, double mtol,
+__global__ void __launch_bounds__(256) k64_implicit_p2g(GridGeom g, PartF64 P, NodesF64 N, CellLists L, BcDense bc, double mtol,
                                  double facl) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodec) return;
@@ -806,7 +984,7 @@ __global__ void k64_newmark_nodes(int n3, const double *__restrict__ ndispl, dou
 // V = J V0, the stress update; for non-rigid materials displ = sum N ndispl,
 // accel = sum N naccel and (PIC) vel = sum N nvel; then
 // CommitParticleKinematics (nonlinear_time_integration.cpp:79-96).
-__global__ void k64_implicit_g2p(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, const double *__restrict__ ndispl,
+__global__ void __launch_bounds__(256) k64_implicit_g2p(GridGeom g, PartF64 P, NodesF64 N, MaterialTable mt, const double *__restrict__ ndispl,
                                  double dt, double gamma_nb, int flip, uint32_t *err) {
     int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P.n) return;
@@ -886,8 +1064,21 @@ __global__ void k64_implicit_g2p(GridGeom g, PartF64 P, NodesF64 N, MaterialTabl
 
 }  // namespace
 
+size_t f64_contrib_doubles(const GridGeom &g, size_t ld) {
+    return ld * (size_t)((g.idimc[0] + 1) * (g.idimc[1] + 1) * (g.idimc[2] + 1)) * kP2GQ;
+}
+
 void launch_f64_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
-                    const BcDense &bc, int scheme, double fx, double fy, double fz, double mtol, double facl) {
+                    const BcDense &bc, int scheme, double fx, double fy, double fz, double mtol, double facl,
+                    double *contrib) {
+    if (contrib) {
+        if (P.n > 0)
+            hipLaunchKernelGGL(k64_contrib<false>, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, L, scheme, fx, fy, fz,
+                               contrib);
+        hipLaunchKernelGGL(k64_p2g_gather, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, N, L, bc, mtol, facl,
+                           (const double *)contrib);
+        return;
+    }
     hipLaunchKernelGGL(k64_p2g, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, scheme, fx, fy, fz, mtol,
                        facl);
 }
@@ -899,7 +1090,7 @@ void launch_f64_solve_grid(hipStream_t s, const GridGeom &g, const NodesF64 &N, 
 // MakNodalVol (module/mesh.cpp:160-202) for one node: cells around it in
 // ascending order, points of each cell in (z, y, x) order, as the reference's
 // scatter adds them.
-__global__ void k64_nodal_vol(GridGeom g, int npx, int npy, int npz, GaussOffsets dec, double volp,
+__global__ void __launch_bounds__(128) k64_nodal_vol(GridGeom g, int npx, int npy, int npz, GaussOffsets dec, double volp,
                               double *__restrict__ nvol) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodec) return;
@@ -946,7 +1137,7 @@ __global__ void k64_shift_nei(int nodec, const double *__restrict__ nvof, ShiftF
 }
 
 // gradient of nei at the particle (old position), times 2 V (old volume)
-__global__ void k64_shift_geup(GridGeom g, PartF64 P, ShiftF64 S) {
+__global__ void __launch_bounds__(128) k64_shift_geup(GridGeom g, PartF64 P, ShiftF64 S) {
     const int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P.n) return;
     const size_t ld = P.ld;
@@ -1005,12 +1196,20 @@ void launch_f64_nodal_vol(hipStream_t s, const GridGeom &g, int npx, int npy, in
 
 void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
                     const BcDense &bc, const MaterialTable &mt, int scheme, double dt, double mtol, double facl,
-                    uint32_t *err, const ShiftF64 *shift, double *njbar) {
+                    uint32_t *err, const ShiftF64 *shift, double *njbar, double *contrib) {
     if (scheme != 1)
         hipLaunchKernelGGL(k64_vel_plus_dt_acc, dim3(cdiv(3 * g.nodec, 256)), dim3(256), 0, s, 3 * g.nodec, N.nv,
                            (const double *)N.na, dt);
     if (P.n > 0) hipLaunchKernelGGL(k64_g2p_vel, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, N, scheme, dt);
-    hipLaunchKernelGGL(k64_musl, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, scheme, mtol, facl);
+    if (contrib) {
+        if (P.n > 0)
+            hipLaunchKernelGGL(k64_contrib<true>, dim3(cdiv(P.n, 128)), dim3(128), 0, s, g, P, L, scheme, 0.0, 0.0,
+                               0.0, contrib);
+        hipLaunchKernelGGL(k64_musl_gather, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, N, L, bc, mtol, facl,
+                           (const double *)contrib);
+    } else {
+        hipLaunchKernelGGL(k64_musl, dim3(cdiv(g.nodec, 128)), dim3(128), 0, s, g, P, N, L, bc, scheme, mtol, facl);
+    }
     const double *geup = nullptr, *b0 = nullptr;
     if (shift && P.n > 0) {
         // update_point_coord_and_stress computes the correction before moving

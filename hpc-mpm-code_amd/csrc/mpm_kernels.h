@@ -159,8 +159,12 @@ struct NodesF64 {
     double *na;
 };
 
+// contrib: scratch of f64_contrib_doubles(g, ld) doubles for the two-pass
+// sums (bit-identical to the one-pass gathers used when it is null)
+size_t f64_contrib_doubles(const GridGeom &g, size_t ld);
 void launch_f64_p2g(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
-                    const BcDense &bc, int scheme, double fx, double fy, double fz, double mtol, double facl);
+                    const BcDense &bc, int scheme, double fx, double fy, double fz, double mtol, double facl,
+                    double *contrib = nullptr);
 void launch_f64_solve_grid(hipStream_t s, const GridGeom &g, const NodesF64 &N, const BcDense &bc, double mtol);
 // Particle shifting of the reference (DeltaCorrectionParticleShifting), FP64:
 // nvol = nodal volume of the filled grid; nei, geup, partial sums and b0 are
@@ -181,7 +185,7 @@ void launch_f64_nodal_vol(hipStream_t s, const GridGeom &g, int npx, int npy, in
                           double volp, double *nvol);
 void launch_f64_g2p(hipStream_t s, const GridGeom &g, const PartF64 &P, const NodesF64 &N, const CellLists &L,
                     const BcDense &bc, const MaterialTable &mt, int scheme, double dt, double mtol, double facl,
-                    uint32_t *err, const ShiftF64 *shift = nullptr, double *njbar = nullptr);
+                    uint32_t *err, const ShiftF64 *shift = nullptr, double *njbar = nullptr, double *contrib = nullptr);
 int launch_f64_maksf(hipStream_t s, const GridGeom &g, int n, const int32_t *cell, const double *xyg, int32_t *nc,
                      double *sf, double *dsf);
 // Implicit-path transfers (reference SolidMaterialPoint::Particle2Node /
