@@ -878,6 +878,42 @@ def test_fp32_fbar_run(model, props, idimc, shifting):
     assert errs[0] < 5e-7 and errs[1] < 3e-9 and errs[2] < 5e-10 and errs[3] < 3e-6 and errs[4] < 2e-4
 
 
+@pytest.mark.parametrize("scheme", [capi.SCHEME_TPIC, capi.SCHEME_APIC])
+@pytest.mark.parametrize("idimc", [(1, 1, 1), (2, 1, 2)])
+def test_fp32_options_together(scheme, idimc):
+    """The FP32 stencil kernels with every option at once, as a reference case
+    might set them: an affine transfer, particle shifting, F-bar, two
+    materials (Neo-Hookean, linear elastic) and traction, 30 steps with
+    relayouts every 4 steps, against the oracle. Measured over the four
+    cases: position 6e-8 of the domain, velocity 2e-7, stress 4e-7, Fbar
+    4e-10 and the particle matrices 3.4e-7 relative. Bounds: 2e-7, 1e-6,
+    2e-6, 2e-9 and 2e-6."""
+    s = scenes.tiny(n_cells=8, scheme=scheme, idimc=idimc)
+    s.materials = [(capi.MODEL_NEOHOOKEAN, [5e5, 0.35]), (capi.MODEL_LINELAST, [1e6, 0.3])]
+    rng = np.random.default_rng(17)
+    s.parts.matid = (np.arange(s.n) % 2).astype(np.int32)
+    s.parts.trac_force = rng.standard_normal((s.n, 3)) * 1e-2
+    c = s.parts.coord.mean(axis=0)
+    s.parts.coord[:] = c + 0.93 * (s.parts.coord - c)
+    s.shifting = True
+    s.fbar = True
+    print("fp32 all options scheme=%d idimc=%s n=%d" % (scheme, idimc, s.n), flush=True)
+    o = oracle_run(s, 30)
+    e = engine(s, capi.PREC_FP32, relayout=4)
+    assert e.step(1, 30) == 0
+    got = e.download(s.n)
+    fb, _ = e.fbar_state(s.n)
+    if scheme == capi.SCHEME_TPIC:
+        mat = rel(e.affine(s.n), o.cmat)
+    else:
+        cm, di = e.affine(s.n, dinv=True)
+        mat = max(rel(cm, o.cmat), rel(di, o.dinv))
+    errs = (float(np.max(np.abs(got.coord - o.parts.coord))), rel(got.vel, o.parts.vel), rel(got.stress, o.parts.stress),
+            float(np.max(np.abs(fb - o.def_grad_bar))), mat)
+    print("  position %.2e velocity %.2e stress %.2e Fbar %.2e matrices %.2e" % errs, flush=True)
+    assert errs[0] < 2e-7 and errs[1] < 1e-6 and errs[2] < 2e-6 and errs[3] < 2e-9 and errs[4] < 2e-6
+
+
 def test_fp32_fbar_state_round_trip_and_restart():
     """The FP32 F-bar state downloads and uploads unchanged (to FP32
     rounding), and a run continued from a restart stays with the
