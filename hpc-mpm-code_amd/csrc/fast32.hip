@@ -582,22 +582,29 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
     // and store nothing.
     bool valid = slot0 < P.n;
     const int slot = valid ? slot0 : 0;
+    const size_t ld = P.ld;
+    // The particle's own streams do not depend on its cell key: they are
+    // issued with the key, so their latency overlaps the key -> node gather chain.
+    const int mat = P.matid ? P.matid[slot] : 0;
     uint32_t key = P.key[slot];
+    float r[3], H[9];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r[a] = P.r[a * ld + slot];
+#pragma unroll
+    for (int q = 0; q < 9; q++) H[q] = __builtin_nontemporal_load(&P.H[q * ld + slot]);  // read once
+    const float V0p = P.V0[slot];
     if (key >= (uint32_t)g.nelem) valid = false;
     if (!valid) key = 0;
-    const size_t ld = P.ld;
     const int nodec = g.nodec;
     const int nexy = g.nex[0] * g.nex[1];
     int rem, ix, iy;
     const int iz = fdivmod((int)key, nexy, T.inv_nexy, rem);
     iy = fdivmod(rem, g.nex[0], T.inv_nex, ix);
-    float r[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) r[a] = P.r[a * ld + slot];
     const float fx = r[0], fy = r[1], fz = r[2];
     const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
     const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
     const int n0 = ix + g.nxc[0] * iy + g.nxc[0] * g.nxc[1] * iz;
+    const MaterialDev md = mt.m[mat];  // copied before the node gather: its loads overlap the gather's
     float Dv[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) Dv[q] = 0.f;
@@ -628,16 +635,12 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                 dxp[1] += Nw * v1;
                 dxp[2] += Nw * v2;
             }
-    const int mat = P.matid ? P.matid[slot] : 0;
-    const MaterialDev &md = mt.m[mat];
     const float dtp = sc.mat_dt ? md.dt : sc.dt;
     float D[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) D[q] = dtp * Dv[q];
     // H_new = (I + D)(I + H) - I = H + D + D H
-    float H[9], Hn[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) H[q] = __builtin_nontemporal_load(&P.H[q * ld + slot]);  // read once
+    float Hn[9];
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -645,7 +648,7 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
             Hn[3 * a + b] = (H[3 * a + b] + D[3 * a + b]) +
                             (D[3 * a] * H[b] + D[3 * a + 1] * H[3 + b] + D[3 * a + 2] * H[6 + b]);
     const double J = det_I_plus_H(Hn);
-    const double V = J * (double)P.V0[slot];
+    const double V = J * (double)V0p;
     // position: offset in cell units, carried into the cell index when it leaves [0, 1)
     int ci[3] = {ix, iy, iz};
     bool inside = true;

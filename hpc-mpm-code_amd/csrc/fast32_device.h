@@ -17,17 +17,12 @@ constexpr int kChunk = 256;   // particles staged per LDS pass
 
 // q = a / n, rem = a % n for a < 2^24 via the float reciprocal, corrected to exact
 __device__ __forceinline__ int fdivmod(int a, int n, float inv_n, int &rem) {
-    int q = (int)((float)a * inv_n);
-    int r = a - q * n;
-    if (r < 0) {
-        q--;
-        r += n;
-    } else if (r >= n) {
-        q++;
-        r -= n;
-    }
-    rem = r;
-    return q;
+    const int q = (int)((float)a * inv_n);
+    const int r = a - q * n;
+    // selects, not branches: keeps the caller's loads in one basic block
+    const bool lo = r < 0, hi = r >= n;
+    rem = lo ? r + n : (hi ? r - n : r);
+    return lo ? q - 1 : (hi ? q + 1 : q);
 }
 
 // Neumaier (improved Kahan) compensated sum.
