@@ -159,7 +159,8 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
                                                           uint32_t *__restrict__ cend,
                                                           uint32_t *__restrict__ active, uint32_t *__restrict__ astart,
                                                           uint32_t *__restrict__ tiles, TileGeom tg,
-                                                          uint32_t *__restrict__ occ, int keep) {
+                                                          uint32_t *__restrict__ occ, uint32_t *__restrict__ gs,
+                                                          uint32_t gmask, int keep) {
     const int b = blockIdx.x;
     const uint32_t was = tprev[b];
     const uint64_t mine = bsum[b];
@@ -209,14 +210,16 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
     int ix = (int)(c0 - (size_t)iz * nexy - (size_t)iy * nex);
     TileSet last{-1, -1, -1, -1, -1, -1};
     uint32_t st[kItems];
-    uint32_t bits = 0;
+    uint32_t bits = 0, gbits = 0;
 #pragma unroll
     for (int q = 0; q < kItems; q++) {
         st[q] = (uint32_t)pre;
         if (cc[q]) {
-            active[(uint32_t)(pre >> 32)] = (uint32_t)(c0 + q);
-            astart[(uint32_t)(pre >> 32)] = st[q];
+            const uint32_t a = (uint32_t)(pre >> 32);
+            active[a] = (uint32_t)(c0 + q);
+            astart[a] = st[q];
             bits |= 1u << q;
+            if ((a & gmask) == 0u) gbits |= 1u << q;
             const TileSet t = tile_set(g, ix, iy, iz);
             if (!(t == last)) {
                 mark_tiles(tg, tiles, t);
@@ -241,11 +244,16 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
         e4[r] = so;
         if (bits && !keep) c4[r] = make_uint4(0u, 0u, 0u, 0u);
     }
-    // occupancy bits: two threads per 32-cell word
+    // occupancy and group-start bits: two threads per 32-cell word
     static_assert(kItems == 16, "two threads per occupancy word");
     uint32_t w = bits << (16 * (threadIdx.x & 1));
+    uint32_t gw = gbits << (16 * (threadIdx.x & 1));
     w |= (uint32_t)__shfl_xor((int)w, 1, 64);
-    if ((threadIdx.x & 1) == 0) occ[c0 >> 5] = w;
+    gw |= (uint32_t)__shfl_xor((int)gw, 1, 64);
+    if ((threadIdx.x & 1) == 0) {
+        occ[c0 >> 5] = w;
+        gs[c0 >> 5] = gw;
+    }
 }
 
 // Each wave places 128 consecutive slots as two halves of 64 whose loads and
@@ -389,7 +397,8 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
                            b.ctr);
     if (!counted) hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
     hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, nb, b.tprev,
-                       b.ctr, n, err, b.cstart, b.cend, b.active, b.astart, b.tiles, tg, b.occ, keep ? 1 : 0);
+                       b.ctr, n, err, b.cstart, b.cend, b.active, b.astart, b.tiles, tg, b.occ, b.gs, b.gmask,
+                       keep ? 1 : 0);
     if (n > 0) {
         int sblocks = cdiv(n, 512);
         if (sblocks > 8192) sblocks = 8192;

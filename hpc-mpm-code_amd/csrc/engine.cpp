@@ -72,7 +72,8 @@ enum {
     OPT_GRAPH_BLOCK = 4,
     OPT_FUSED_COUNT = 5,
     OPT_CELLS_PREFETCH = 6,
-    OPT_CELLS_GRID = 7
+    OPT_CELLS_GRID = 7,
+    OPT_X_MERGE = 8
 };
 
 struct mpm_engine {
@@ -109,6 +110,7 @@ struct mpm_engine {
     NodeTables T{};
     float m_small = 0.f;
     int lanes_per_cell = 1;  // from the particles per non-empty cell at upload
+    bool x_merge = true;     // FP32 tent cells kernels add x-neighbour corner partials within a group
     int cells_prefetch = 1;
     int cells_grid = 0;
     int fp32_outside_at_upload = 0;
@@ -244,7 +246,7 @@ struct mpm_engine {
 
     BcDense bc() const { return BcDense{d_bcmask, d_bcval}; }
     CellLists lists() const {
-        return CellLists{perm, bb.cstart, bb.cend, bb.active, bb.astart, bb.ctr, bb.tiles, bb.occ};
+        return CellLists{perm, bb.cstart, bb.cend, bb.active, bb.astart, bb.ctr, bb.tiles, bb.occ, bb.gs, bb.gmask};
     }
 };
 
@@ -328,6 +330,8 @@ int do_bin(mpm_engine *e) {
     // FP32 with fused counting keeps the counts: the update kernel then moves
     // only the counts of particles that change cell
     const bool keep = e->prec == MPM_PREC_FP32 && e->fused_count;
+    // group-start bits for the tent cells kernels' x-merge: 64 / lanes-per-cell cells per group
+    e->bb.gmask = e->x_merge ? (uint32_t)(kGroupLanes / std::max(1, e->lanes_per_cell)) - 1u : 0u;
     int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, e->d_err, e->counts_ready, keep);
     e->counts_ready = keep;
     if (rc) return set_error(MPM_ERR_HIP, "binning launch failed");
@@ -716,7 +720,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         (rc = e->dalloc(&b.tiles, (size_t)e->tg.ntiles)) ||
         (rc = e->dalloc(&b.bsum, bin_scan_blocks(e->g.nelem))) ||
         (rc = e->dalloc(&b.tprev, bin_scan_blocks(e->g.nelem))) || (rc = e->dalloc(&b.ctr, 1)) ||
-        (rc = e->dalloc(&b.occ, ncap / 32)) ||
+        (rc = e->dalloc(&b.occ, ncap / 32)) || (rc = e->dalloc(&b.gs, ncap / 32)) ||
         (rc = e->dalloc(&e->d_err, 1))) {
         delete e;
         return rc;
@@ -730,6 +734,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     if (he == hipSuccess) he = hipMemset(b.cstart, 0, ncap * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.cend, 0, ncap * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.occ, 0, ncap / 32 * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(b.gs, 0, ncap / 32 * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.tprev, 0, bin_scan_blocks(e->g.nelem) * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.ctr, 0, sizeof(BinCounters));
     if (he != hipSuccess) {
@@ -2006,6 +2011,9 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) try {
         }
         e->relayout_every = (int)value;
         e->steps_since_relayout = 0;
+        return 0;
+    case OPT_X_MERGE:
+        e->x_merge = value != 0;
         return 0;
     case OPT_CELLS_PREFETCH:
         e->drop_graph();

@@ -9,13 +9,16 @@
 // vectors.
 //
 // P2G is deterministic without float atomics: a 64-thread workgroup owns a
-// row segment of 64 consecutive cells along x. Their particles are one
-// contiguous range of the cell-sorted order, which is staged through LDS in
-// chunks with coalesced loads; each thread then folds the particles of its
-// own cell, in sorted order, into the 8 corner partials of that cell
-// (registers). A node pass adds the partials of the <= 8 cells around each
-// node in ascending cell order with Neumaier-compensated FP32 sums and
-// applies the grid update (velocity, acceleration, BCs) in the same pass.
+// group of 64 / L consecutive non-empty cells (L lanes per cell). Their
+// particles are one contiguous range of the cell-sorted order, which is
+// staged through LDS in chunks with coalesced loads; the lanes of a cell then
+// fold its particles, in sorted order, into the 8 corner partials of that
+// cell (registers). Where the previous cell of the group is the cell's left
+// neighbour, the two corner partials of each node between them are added in
+// the group, so that node pass reads one value for them. A node pass adds the
+// partials around each node in ascending cell order with Neumaier-compensated
+// FP32 sums and applies the grid update (velocity, acceleration, BCs) in the
+// same pass.
 // The MUSL remap uses the same structure, with the G2P velocity of every
 // particle computed from the 8 node values of its cell held in registers.
 #include "fast32_device.h"
@@ -26,6 +29,88 @@ namespace mpm {
 
 namespace {
 
+// Corner partials of the cells kernels. A cell's values are grouped by
+// corner group g = y-side + 2 z-side and x-side `side` (0: the cell's left
+// corners, 1: its right ones), NQ quantities each. part_entry orders them so
+// that after ReduceScatter the lane of sub s holds whole (group, side) runs in
+// consecutive registers: L <= 4 lanes per cell hold 4 / L groups with both
+// sides (group s 4 / L + gl, run start (side 4 / L + gl) NQ), L = 8 holds side
+// s & 1 of group s >> 1 (run start 0).
+template <int LPC>
+__host__ __device__ constexpr int groups_per_lane() { return LPC <= 4 ? 4 / LPC : 1; }
+template <int LPC, int NQ>
+__device__ __forceinline__ constexpr int part_entry(int g, int side, int f) {
+    constexpr int GPL = groups_per_lane<LPC>();
+    return LPC <= 4 ? LPC * ((side * GPL + g % GPL) * NQ + f) + g / GPL : 8 * f + 2 * g + side;
+}
+
+// Stored layout: slot side * 4 + g, then the cell, then the NQ quantities
+// padded to NQP = 8 (P2G) or 4 (MUSL) floats, so that a cell's run of one
+// slot is stored and loaded with 16-byte accesses.
+template <int NQ>
+__host__ __device__ constexpr int padded_nq() { return NQ <= 4 ? 4 : 8; }
+
+// Stores a cell's corner partials after ReduceScatter, merged along x. A cell
+// whose left neighbour is the previous cell of its group adds that cell's
+// right-corner runs into its own left-corner ones (the lanes that hold them
+// are L apart, or L - 1 for L = 8). Stored: the left-corner runs of every
+// non-empty cell, and the right-corner runs only of a cell whose right
+// neighbour did not take them (empty, past the row's end, or the first cell
+// of the next group) -- the cases in which the node passes read them
+// (node_rows4). With L.gmask == 0 no cell merges and every run is stored.
+template <int LPC, int NQ>
+__device__ __forceinline__ void xmerge_store(float *v, int sub, const CellInfo &si, const CellLists &L, int nex0,
+                                             float *__restrict__ part, size_t ne) {
+    constexpr int NQP = padded_nq<NQ>();
+    constexpr int GPL = groups_per_lane<LPC>();
+    const int lane = threadIdx.x;
+    const bool xm = L.gmask != 0u;
+    const int pc = __shfl_up(si.c, LPC, 64);    // previous cell of the group
+    const int nc = __shfl_down(si.c, LPC, 64);  // next cell of the group
+    const bool merge = xm && lane >= LPC && si.c > 0 && pc == si.c - 1 && si.ix > 0;
+    const bool keep_right = !(xm && lane < kSeg - LPC && nc == si.c + 1 && si.ix + 1 < nex0);
+    const bool on = si.ce > si.cs;
+    if constexpr (LPC <= 4) {
+#pragma unroll
+        for (int gl = 0; gl < GPL; gl++)
+#pragma unroll
+            for (int f = 0; f < NQ; f++) {
+                const float up = __shfl_up(v[(GPL + gl) * NQ + f], LPC, 64);
+                if (merge) v[gl * NQ + f] += up;
+            }
+        if (on) {
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                if (side == 1 && !keep_right) break;
+#pragma unroll
+                for (int gl = 0; gl < GPL; gl++) {
+                    const int g = sub * GPL + gl;
+                    float4 *dst = (float4 *)(part + ((size_t)(side * 4 + g) * ne + si.c) * NQP);
+                    const float *run = v + (side * GPL + gl) * NQ;
+#pragma unroll
+                    for (int h = 0; h < NQP / 4; h++)
+                        dst[h] = make_float4(run[4 * h], 4 * h + 1 < NQ ? run[4 * h + 1] : 0.f,
+                                             4 * h + 2 < NQ ? run[4 * h + 2] : 0.f,
+                                             4 * h + 3 < NQ ? run[4 * h + 3] : 0.f);
+                }
+            }
+        }
+    } else {
+        const int side = sub & 1, g = sub >> 1;
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            const float up = __shfl_up(v[f], LPC - 1, 64);
+            if (merge && side == 0) v[f] += up;
+        }
+        if (on && (side == 0 || keep_right)) {
+            float4 *dst = (float4 *)(part + ((size_t)(side * 4 + g) * ne + si.c) * NQP);
+#pragma unroll
+            for (int h = 0; h < NQP / 4; h++)
+                dst[h] = make_float4(v[4 * h], 4 * h + 1 < NQ ? v[4 * h + 1] : 0.f, 4 * h + 2 < NQ ? v[4 * h + 2] : 0.f,
+                                     4 * h + 3 < NQ ? v[4 * h + 3] : 0.f);
+        }
+    }
+}
 
 // PF: software prefetch. The next group's cell ranges are requested at the
 // start of a group and the next chunk's slot indices while the current
@@ -192,8 +277,9 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                 for (int r = 0; r < kBlk; r++) slot[r] = slot_n[r];
             }
         }
-        // partial of entry q * 7 + f (f: m, p, force) and 56 + q (volume)
-        constexpr int NV = DIAG ? 64 : 56;
+        // quantities per corner: m, p, force (f = 0..6) and the volume (7)
+        constexpr int NQ = DIAG ? 8 : 7;
+        constexpr int NV = 8 * NQ;
         float v[NV];
 #pragma unroll
         for (int k = 0; k < 2; k++)
@@ -203,22 +289,18 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
                 for (int ii = 0; ii < 2; ii++) {
                     const int q = ii + 2 * j + 4 * k;
 #pragma unroll
-                    for (int f = 0; f < 4; f++) v[q * 7 + f] = acc[q][f];
+                    for (int f = 0; f < 4; f++) v[part_entry<LPC, NQ>(j + 2 * k, ii, f)] = acc[q][f];
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
                         const float fx_ = ii ? FX[a][j + 2 * k] : -FX[a][j + 2 * k];
                         const float fy_ = j ? FY[a][ii + 2 * k] : -FY[a][ii + 2 * k];
                         const float fz_ = k ? FZ[a][ii + 2 * j] : -FZ[a][ii + 2 * j];
-                        v[q * 7 + 4 + a] = (fx_ + fy_) + fz_ + (TRAC ? tr[q][a] : 0.f);
+                        v[part_entry<LPC, NQ>(j + 2 * k, ii, 4 + a)] = (fx_ + fy_) + fz_ + (TRAC ? tr[q][a] : 0.f);
                     }
-                    if (DIAG) v[56 + q] = vof[q];
+                    if (DIAG) v[part_entry<LPC, NQ>(j + 2 * k, ii, 7)] = vof[q];
                 }
         ReduceScatter<1, LPC, NV>::run(v, sub);
-        if (si.ce > si.cs) {
-            const size_t ne = (size_t)g.nelem;
-#pragma unroll
-            for (int k = 0; k < NV / LPC; k++) part[(size_t)(k * LPC + sub) * ne + si.c] = v[k];
-        }
+        xmerge_store<LPC, NQ>(v, sub, si, L, g.nex[0], part, (size_t)g.nelem);
         grp = grp_n;
         if (grp < ngroups) {
             if (PF) {
@@ -233,29 +315,72 @@ __global__ void __launch_bounds__(kSeg) k32_p2g_cells(GridGeom g, PartF32 P, Cel
 }
 
 
-// The <= 8 cells around node (i, j, k) in ascending cell order: cell index
-// (clamped into the grid when outside) and whether it holds particles. All
-// of a node's partial loads can then be issued at once instead of one cell's
-// loads per dependent round trip.
-struct NodeCells {
-    int c[8];
-    int corner[8];
-    bool on[8];
+// The <= 4 cell rows around node (i, j, k) in ascending cell order, each with
+// the cell left (i - 1) and right (i) of the node, and which of their partials
+// to read: the right cell's left-corner entries (with the left cell's
+// right-corner ones added when the cells kernel merged the pair: both
+// non-empty and the right cell not the first of its group) and, when the pair
+// was not merged, the left cell's right-corner entries. The occupancy and
+// group-start words are loaded from clamped addresses, all at once.
+struct NodeRows {
+    int cl[4], cr[4];
+    bool rl[4], rr[4];
 };
-__device__ __forceinline__ bool node_cells8(const GridGeom &g, const CellLists &L, int i, int j, int k, NodeCells &nc) {
+__device__ __forceinline__ bool node_rows4(const GridGeom &g, const CellLists &L, int i, int j, int k, NodeRows &nr) {
     bool any = false;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int di = q & 1, dj = (q >> 1) & 1, dk = q >> 2;  // q ascending = cell index ascending
-        const int ci = i - 1 + di, cj = j - 1 + dj, ck = k - 1 + dk;
-        const bool in = ci >= 0 && ci < g.nex[0] && cj >= 0 && cj < g.nex[1] && ck >= 0 && ck < g.nex[2];
-        const int c = in ? ci + cj * g.nex[0] + ck * g.nex[0] * g.nex[1] : 0;
-        nc.c[q] = c;
-        nc.corner[q] = (1 - di) + 2 * (1 - dj) + 4 * (1 - dk);
-        nc.on[q] = in && cell_nonempty(L, c);
-        any = any || nc.on[q];
+    for (int r = 0; r < 4; r++) {
+        const int cj = j - 1 + (r & 1), ck = k - 1 + (r >> 1);
+        const bool row = cj >= 0 && cj < g.nex[1] && ck >= 0 && ck < g.nex[2];
+        const int base = row ? (cj + ck * g.nex[1]) * g.nex[0] : 0;
+        const bool in_l = row && i >= 1;  // a node has i <= nex
+        const bool in_r = row && i < g.nex[0];
+        const int cl = in_l ? base + i - 1 : 0, cr = in_r ? base + i : 0;
+        const uint32_t wl = L.occ[cl >> 5], wr = L.occ[cr >> 5], gr = L.gs[cr >> 5];
+        const bool on_l = in_l && ((wl >> (cl & 31)) & 1u);
+        const bool on_r = in_r && ((wr >> (cr & 31)) & 1u);
+        const bool merged = on_l && on_r && !((gr >> (cr & 31)) & 1u);
+        nr.cl[r] = cl;
+        nr.cr[r] = cr;
+        nr.rl[r] = on_l && !merged;
+        nr.rr[r] = on_r;
+        any = any || on_l || on_r;
     }
     return any;
+}
+
+// Sum over the rows around a node of NQ partial quantities, in ascending
+// cell order, with Neumaier sums (16-byte loads of the stored runs).
+template <int NQ>
+__device__ __forceinline__ void node_partial_sums(const NodeRows &nr, const float *__restrict__ part, size_t ne,
+                                                  float *s, float *cp) {
+    constexpr int NQP = padded_nq<NQ>();
+    float4 val[4][2][NQP / 4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        // the node is in corner group (1 - dj) + 2 (1 - dk) of the cells of row r
+        const int g = (1 - (r & 1)) + 2 * (1 - (r >> 1));
+        const float4 *pl = (const float4 *)(part + ((size_t)(4 + g) * ne + nr.cl[r]) * NQP);
+        const float4 *pr = (const float4 *)(part + ((size_t)g * ne + nr.cr[r]) * NQP);
+#pragma unroll
+        for (int h = 0; h < NQP / 4; h++) {
+            val[r][0][h] = make_float4(0.f, 0.f, 0.f, 0.f);
+            val[r][1][h] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (nr.rl[r]) val[r][0][h] = pl[h];
+            if (nr.rr[r]) val[r][1][h] = pr[h];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            if (!(side ? nr.rr[r] : nr.rl[r])) continue;
+#pragma unroll
+            for (int f = 0; f < NQ; f++) {
+                const float4 q = val[r][side][f >> 2];
+                neu_add(s[f], cp[f], (f & 3) == 0 ? q.x : (f & 3) == 1 ? q.y : (f & 3) == 2 ? q.z : q.w);
+            }
+        }
 }
 
 template <bool DIAG>
@@ -272,29 +397,11 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
         s[f] = 0.f;
         cp[f] = 0.f;
     }
-    NodeCells nc;
-    const bool any = node_cells8(g, L, i, j, k, nc);
-    if (any) {
-        // loads only from cells that hold particles: a tile's nodes next to
-        // empty cells fetch nothing for them
-        float val[8][NQ];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-#pragma unroll
-            for (int f = 0; f < NQ; f++) val[q][f] = 0.f;
-            if (nc.on[q]) {
-#pragma unroll
-                for (int f = 0; f < 7; f++) val[q][f] = part[(size_t)(nc.corner[q] * 7 + f) * ne + nc.c[q]];
-                if (DIAG) val[q][7] = part[(size_t)(56 + nc.corner[q]) * ne + nc.c[q]];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (nc.on[q]) {
-#pragma unroll
-                for (int f = 0; f < NQ; f++) neu_add(s[f], cp[f], val[q][f]);
-            }
-    }
+    NodeRows nr;
+    // loads only from cells that hold particles: a tile's nodes next to
+    // empty cells fetch nothing for them
+    const bool any = node_rows4(g, L, i, j, k, nr);
+    if (any) node_partial_sums<NQ>(nr, part, ne, s, cp);
     const int nodec = g.nodec;
     if (!any) {
         if (DIAG) {
@@ -496,17 +603,14 @@ __global__ void __launch_bounds__(kSeg) k32_musl_cells(GridGeom g, PartF32 P, Ce
             }
             __syncthreads();
         }
+        // corner (i, j, k) = x-side i of corner group j + 2 k; momentum component c
         float v[24];
 #pragma unroll
         for (int q = 0; q < 8; q++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) v[q * 3 + c] = acc[q][c];
+            for (int c = 0; c < 3; c++) v[part_entry<LPC, 3>(q >> 1, q & 1, c)] = acc[q][c];
         ReduceScatter<1, LPC, 24>::run(v, sub);
-        if (ne) {
-            const size_t nel = (size_t)g.nelem;
-#pragma unroll
-            for (int k = 0; k < 24 / LPC; k++) part[(size_t)(k * LPC + sub) * nel + si.c] = v[k];
-        }
+        xmerge_store<LPC, 3>(v, sub, si, L, g.nex[0], part, (size_t)g.nelem);
     }  // groups
 }
 
@@ -518,26 +622,9 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, T
     if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
     const size_t ne = (size_t)g.nelem;
     float s[3] = {0.f, 0.f, 0.f}, cp[3] = {0.f, 0.f, 0.f};
-    NodeCells nc;
-    const bool any = node_cells8(g, L, i, j, k, nc);
-    if (any) {
-        float val[8][3];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-#pragma unroll
-            for (int f = 0; f < 3; f++) val[q][f] = 0.f;
-            if (nc.on[q]) {
-#pragma unroll
-                for (int f = 0; f < 3; f++) val[q][f] = part[(size_t)(nc.corner[q] * 3 + f) * ne + nc.c[q]];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (nc.on[q]) {
-#pragma unroll
-                for (int f = 0; f < 3; f++) neu_add(s[f], cp[f], val[q][f]);
-            }
-    }
+    NodeRows nr;
+    const bool any = node_rows4(g, L, i, j, k, nr);
+    if (any) node_partial_sums<3>(nr, part, ne, s, cp);
     const int nodec = g.nodec;
     if (!any) {
         if (DIAG)

@@ -186,6 +186,7 @@ __device__ __forceinline__ bool tile_node(const GridGeom &g, const CellLists &L,
 }
 
 __device__ __forceinline__ bool cell_nonempty(const CellLists &L, int c) { return (L.occ[c >> 5] >> (c & 31)) & 1u; }
+static_assert(kSeg == kGroupLanes, "the binning's group-start bits assume one wave per group");
 
 // ------------------------------------------------------ particle update
 // Stress update in FP32 from H = F - I and the increment D = dF - I

@@ -54,6 +54,8 @@ struct CellLists {
     const BinCounters *ctr;
     const uint32_t *tiles;
     const uint32_t *occ;     // one bit per cell: the cell holds particles
+    const uint32_t *gs;      // one bit per cell: the cell starts a group of the FP32 tent cells kernels
+    uint32_t gmask;          // cells per group minus 1 of the binning that wrote gs; 0: no x-merge
 };
 
 struct TileGeom {
@@ -99,6 +101,9 @@ struct BinBuffers {
     uint32_t *perm;     // [n]
     uint32_t *tiles;    // [ntiles]
     uint32_t *occ;      // [bin_cell_capacity / 32] occupancy bits (zero at allocation)
+    uint32_t *gs;       // [bin_cell_capacity / 32] group-start bits (zero at allocation): set for the
+                        // active cells whose active-list index is a multiple of gmask + 1
+    uint32_t gmask;     // cells per group of the FP32 tent cells kernels, minus 1 (a power of 2, minus 1)
     uint64_t *bsum;     // [bin_scan_blocks(nelem)] packed (non-empty cells, particles) per tile of
                         // kBinTile cells: written by the reduce pass, or kept with kept counts
     uint32_t *tprev;    // [bin_scan_blocks(nelem)] tile non-empty at the previous binning (zero at allocation)
@@ -267,10 +272,16 @@ struct NodeTables {
     float inv_nexy;
 };
 
-// Partials buffer: 64 floats per cell: 8 corners x {m, px, py, pz, fx, fy, fz}
-// (MUSL uses the first 24 as 8 corners x {px, py, pz}), then 8 corner volumes
-// in diagnostic mode.
+// Partials buffer of the tent kernels: up to 64 floats per cell, as 8 slots
+// (x-side of the corner * 4 + y-side + 2 z-side), then the cell, then the
+// quantities padded to 8 floats for P2G ({m, px, py, pz, fx, fy, fz}, and the
+// corner volume in diagnostic mode) or to 4 for MUSL ({px, py, pz}).
+// Left-corner slots hold the x-merged sums; right-corner slots are stored only
+// where no merge took them.
 constexpr int kPartialsPerCell = 64;
+// lanes of one FP32 tent cells-kernel group (one wave): a group holds
+// kGroupLanes / lanes-per-cell consecutive active cells
+constexpr int kGroupLanes = 64;
 
 // P2G: corner partials per cell + node pass with the grid update (v = p/m,
 // a = f/m, BCs). diag != 0 also writes momentum, force and nodal volume.

@@ -775,18 +775,31 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
     // on slot 0 / cell 0 and store nothing
     bool valid = slot0 < P.n;
     const int slot = valid ? slot0 : 0;
+    const size_t ld = P.ld;
+    // The particle's own streams do not depend on its cell key: they are
+    // issued with the key (the material index first), so their latency
+    // overlaps the key -> node gather chain.
+    const int mat = P.matid ? P.matid[slot] : 0;
     uint32_t key = P.key[slot];
+    float r[3], H[9];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r[a] = P.r[a * ld + slot];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        if constexpr (FB)
+            H[q] = P.H[q * ld + slot];  // F - I, advanced by ks_fbar_grad
+        else
+            H[q] = __builtin_nontemporal_load(&P.H[q * ld + slot]);
+    }
+    const float V0p = P.V0[slot];
     if (key >= (uint32_t)g.nelem) valid = false;
     if (!valid) key = 0;
-    const size_t ld = P.ld;
     const int nodec = g.nodec;
     const int nexy = g.nex[0] * g.nex[1];
     int rem, ix, iy;
     const int iz = fdivmod((int)key, nexy, T.inv_nexy, rem);
     iy = fdivmod(rem, g.nex[0], T.inv_nex, ix);
-    float r[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) r[a] = P.r[a * ld + slot];
+    const MaterialDev md = mt.m[mat];  // copied before the node gather: its loads overlap the gather's
     AxisW<NX> Ax;
     AxisW<NY> Ay;
     AxisW<NZ> Az;
@@ -844,14 +857,11 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             D[q] = P.Dd[q * ld + slot];
-            Hn[q] = P.H[q * ld + slot];  // F - I, advanced by ks_fbar_grad
+            Hn[q] = H[q];
         }
     } else {
 #pragma unroll
         for (int q = 0; q < 9; q++) D[q] = sc.dt * Dv[q];
-        float H[9];
-#pragma unroll
-        for (int q = 0; q < 9; q++) H[q] = __builtin_nontemporal_load(&P.H[q * ld + slot]);
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -886,7 +896,7 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
 #pragma unroll
         for (int q = 0; q < 9; q++) Hs[q] = Hn[q];
     }
-    const double V = J * (double)P.V0[slot];
+    const double V = J * (double)V0p;
     int ci[3] = {ix, iy, iz};
     bool inside = true;
     const double b0 = sc.sh_geup ? *sc.sh_b0 : 0.0;
@@ -904,8 +914,6 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
         r[a] = ra;
         inside = inside && ci[a] >= 0 && ci[a] < g.nex[a];
     }
-    const int mat = P.matid ? P.matid[slot] : 0;
-    const MaterialDev &md = mt.m[mat];
     const bool rate_form = (md.model == 0 || md.model == 10);
     float s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (rate_form) {

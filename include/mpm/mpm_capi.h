@@ -241,7 +241,13 @@ int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
  * MPM_OPT_CELLS_PREFETCH (FP32, default 1): the P2G cells kernel requests the
  * next chunk's particle indices and the next group's cell ranges while a
  * chunk's fields load. MPM_OPT_CELLS_GRID (FP32, default 0 = automatic):
- * workgroups of the cells kernels. Neither changes results. */
+ * workgroups of the cells kernels. Neither changes results.
+ * MPM_OPT_X_MERGE (FP32, default 1): the tent cells kernels add the corner
+ * partials of x-neighbouring cells of one group before storing them (half
+ * the partials written and read). It changes the FP32 summation order, which
+ * then depends on how the non-empty cells fall into groups: 0 makes every
+ * node sum independent of the grouping (as needed for bit-equal results of
+ * scenes packed into one grid and run alone). */
 enum {
     MPM_OPT_FULL_NODE_OUTPUT = 1,
     MPM_OPT_LANES_PER_CELL = 2,
@@ -249,7 +255,8 @@ enum {
     MPM_OPT_GRAPH_BLOCK = 4,
     MPM_OPT_FUSED_COUNT = 5,
     MPM_OPT_CELLS_PREFETCH = 6,
-    MPM_OPT_CELLS_GRID = 7
+    MPM_OPT_CELLS_GRID = 7,
+    MPM_OPT_X_MERGE = 8
 };
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 /* Reference particle shifting (DeltaCorrectionParticleShifting,

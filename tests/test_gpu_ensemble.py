@@ -75,17 +75,29 @@ def test_failing_scene_is_named():
     ens.close()
 
 
+# max |difference| / max |value| of an atlas scene against its own engine
+# after 40 steps with the x-merge on (rounding of the node pair sums only)
+# (measured: coord 6e-10, vel 4.6e-7, stress 6.9e-5, F, det F and V 1.2e-10)
+ATLAS_XMERGE_BOUND = {"coord": 3e-9, "vel": 3e-6, "stress": 5e-4, "def_grad": 1e-9, "det_def_grad": 1e-9,
+                      "vol": 1e-9}
+
+
+@pytest.mark.parametrize("x_merge", [0, 1])
 @pytest.mark.parametrize("per_scene_dt", [False, True])
-def test_atlas_matches_individual_scenes(per_scene_dt):
+def test_atlas_matches_individual_scenes(per_scene_dt, x_merge):
     # scenes of one grid size packed into one engine give, scene by scene,
     # the bits of separate engines run with the same time step: the atlas's
-    # smallest, or (per-material steps) each scene's own
+    # smallest, or (per-material steps) each scene's own. With the x-merge of
+    # the cells kernels on, a node's pair sums follow the grouping of the
+    # non-empty cells, which the other scenes of the atlas shift: the results
+    # then agree to FP32 rounding (bounds below, 40 steps).
     members = [scenes.cfg4_scene(k) for k in (4, 7, 8)]   # 128 x 128 grids
     a, layout = scenes.atlas(members, per_scene_dt=per_scene_dt)
-    print("atlas %s n=%d dt=%.3e per-scene %s" % (a.nelem, a.n, a.dt, per_scene_dt), flush=True)
+    print("atlas %s n=%d dt=%.3e per-scene %s x-merge %d" % (a.nelem, a.n, a.dt, per_scene_dt, x_merge), flush=True)
     e = capi.Engine(a, precision=capi.PREC_FP32, relayout_every=16)
     e.upload(a.parts)
     e.set_option(capi.OPT_LANES_PER_CELL, 2)
+    e.set_option(capi.OPT_X_MERGE, x_merge)
     assert e.step(1, 40) == 0
     got = e.download(a.n)
     e.close()
@@ -97,13 +109,21 @@ def test_atlas_matches_individual_scenes(per_scene_dt):
         s1 = capi.Engine(sc, precision=capi.PREC_FP32, relayout_every=16)
         s1.upload(sc.parts)
         s1.set_option(capi.OPT_LANES_PER_CELL, 2)
+        s1.set_option(capi.OPT_X_MERGE, x_merge)
         assert s1.step(1, 40) == 0
         ref = s1.download(sc.n)
         s1.close()
         sl = slice(first, first + cnt)
-        assert np.array_equal(got.coord[sl] - np.asarray(origin), ref.coord), sc.name
-        for f in ("vel", "stress", "def_grad", "det_def_grad", "vol"):
-            assert np.array_equal(getattr(got, f)[sl], getattr(ref, f)), (sc.name, f)
+        pairs = [("coord", got.coord[sl] - np.asarray(origin), ref.coord)]
+        pairs += [(f, getattr(got, f)[sl], getattr(ref, f)) for f in ("vel", "stress", "def_grad", "det_def_grad", "vol")]
+        for f, x, y in pairs:
+            if not x_merge:
+                assert np.array_equal(x, y), (sc.name, f)
+        if x_merge:
+            diffs = {f: float(np.max(np.abs(x - y))) / max(float(np.max(np.abs(y))), 1e-30) for f, x, y in pairs}
+            print(sc.name, " ".join("%s %.2e" % kv for kv in diffs.items()), flush=True)
+            for f, d in diffs.items():
+                assert d < ATLAS_XMERGE_BOUND[f], (sc.name, f, d)
 
 
 def test_relayout_turned_on_after_upload():

@@ -411,15 +411,18 @@ def test_engine_bins_crowded_cells(prec):
 
 
 def test_fp32_lanes_per_cell_variants_agree():
-    """1, 2, 4 or 8 lanes per cell change only the FP32 summation order."""
+    """1, 2, 4 or 8 lanes per cell, with the x-merge of the corner partials on
+    and off, change only the FP32 summation order."""
     s = scenes.tiny(n_cells=16, ppc=(2, 4, 2))
     o = oracle_run(s, 20)
     outs = []
     for lpc in (1, 2, 4, 8):
-        e = engine(s, capi.PREC_FP32)
-        e.set_option(capi.OPT_LANES_PER_CELL, lpc)
-        assert e.step(1, 20) == 0
-        outs.append(e.download(s.n))
+        for xm in (1, 0):
+            e = engine(s, capi.PREC_FP32)
+            e.set_option(capi.OPT_LANES_PER_CELL, lpc)
+            e.set_option(capi.OPT_X_MERGE, xm)
+            assert e.step(1, 20) == 0
+            outs.append(e.download(s.n))
     for g in outs:
         # offset resolution h * 2^-24 = 3.7e-9 on this 16^3 grid, 20 steps
         assert np.max(np.abs(g.coord - o.parts.coord)) < 5e-8
