@@ -59,8 +59,8 @@ BYTES = {
 # profiles/hbm_traffic_cfg2_4M_fp32.md). Reported as roofline.traffic for
 # that workload only.
 MEASURED_TRAFFIC = {
-    "cfg2-3d-neohookean-16ppc": {"update": 573016077, "p2g_cells": 315724512, "musl_cells": 175602166,
-                                 "p2g_nodes": 89658072, "musl_nodes": 40454217},
+    "cfg2-3d-neohookean-16ppc": {"update": 573025201, "p2g_cells": 291576928, "musl_cells": 168262136,
+                                 "p2g_nodes": 54555437, "musl_nodes": 30520608},
 }
 
 
@@ -413,6 +413,10 @@ def main():
         gbs = byts / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
         kernels[k] = {"avg_us": round(avg_ms * 1e3, 2), "alg_bytes": byts, "GBps": round(gbs, 1),
                       "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        measured = MEASURED_TRAFFIC.get(scene.name, {}).get(k)
+        if measured:
+            kernels[k]["traffic"] = measured
+            kernels[k]["traffic_over_alg"] = round(measured / byts, 3)
     dom = max((k for k in kernels if k != "bin"), key=lambda k: kernels[k]["avg_us"])
     d = kernels[dom]
     traffic = MEASURED_TRAFFIC.get(scene.name, {}).get(dom)
@@ -424,8 +428,11 @@ def main():
     def phase(keys):
         b = sum(kernels[k]["alg_bytes"] for k in keys)
         us = sum(kernels[k]["avg_us"] for k in keys)
-        return {"avg_us": round(us, 2), "GBps": round(b / (us * 1e-6) / 1e9, 1),
-                "frac": round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+        out = {"avg_us": round(us, 2), "GBps": round(b / (us * 1e-6) / 1e9, 1),
+               "frac": round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+        if all("traffic" in kernels[k] for k in keys):
+            out["traffic_over_alg"] = round(sum(kernels[k]["traffic"] for k in keys) / b, 3)
+        return out
 
     phases = {"p2g": phase(["p2g_cells", "p2g_nodes"]), "g2p": phase(["musl_cells", "musl_nodes", "update"])}
 
