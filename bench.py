@@ -92,7 +92,8 @@ def parse():
     ap.add_argument("--sur-epochs", type=int, default=40)
     ap.add_argument("--sur-batch", type=int, default=4096)
     ap.add_argument("--sur-tol", type=float, default=1e-4, help="the stated fallback tolerance")
-    ap.add_argument("--sur-sweep", default="3e-4,1e-3,3e-3,1e-2,3e-2,1e-1,1e30", help="further tolerances reported")
+    ap.add_argument("--sur-sweep", default="1.5e-4,2e-4,3e-4,1e-3,3e-3,1e-2,1e-1,1e30",
+                    help="further tolerances reported")
     ap.add_argument("--no-closure", action="store_true", help="skip the cfg5 pressure-closure measurement")
     ap.add_argument("--closure-collect", type=int, default=100)
     ap.add_argument("--closure-steps", type=int, default=100)
@@ -207,10 +208,11 @@ def surrogate_study(a, scene, collect_steps, timed_steps_n, tols, epochs, seed):
     eng.close()
     i0 = 1 + collect_steps
 
-    def advance(tol):
+    def advance(tol, fused=True):
         e = capi.Engine(scene, precision=capi.PREC_FP32, relayout_every=a.relayout)
         e.upload(state)
         e.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
+        e.set_option(capi.OPT_SURROGATE_FUSED, int(fused))
         if tol is not None:
             e.surrogate_set_params(params, scaling)
             e.surrogate_enable(True, tol)
@@ -225,8 +227,10 @@ def surrogate_study(a, scene, collect_steps, timed_steps_n, tols, epochs, seed):
 
     plain_rate, _, sp, plain_upd = advance(None)
     runs = []
+    fused_su = {}
     for tol in tols:
         rate_s, ss, su, upd = advance(tol)
+        fused_su[tol] = su
         runs.append({"tol": tol, "particle_steps_per_s": rate_s, "ratio": rate_s / plain_rate,
                      "rel_stress_err": float(np.linalg.norm(su - sp) / np.linalg.norm(sp)),
                      "max_abs_stress_err": float(np.max(np.abs(su - sp))), "evaluated": ss["evaluated"],
@@ -235,13 +239,21 @@ def surrogate_study(a, scene, collect_steps, timed_steps_n, tols, epochs, seed):
     # 1e-3 of the plain method (lowest fallback rate at that error bound)
     ok = [r for r in runs if r["rel_stress_err"] <= 1e-3]
     cal = max(ok, key=lambda r: r["tol"]) if ok else None
+    # the same tolerance with the network as a separate pass over queued
+    # candidates (the fused kernel's results are the same bit for bit)
+    ref_tol = (cal or runs[0])["tol"]
+    q_rate, q_ss, q_su, q_upd = advance(ref_tol, fused=False)
+    fused_run = next(r for r in runs if r["tol"] == ref_tol)
+    queued = {"tol": ref_tol, "particle_steps_per_s": q_rate, "ratio": q_rate / plain_rate, "update_us": q_upd,
+              "fused_update_us": fused_run["update_us"], "fused_ratio": fused_run["ratio"],
+              "same_stress_as_fused": bool(np.array_equal(q_su, fused_su[ref_tol]))}
     return {
         "workload": scene.name, "particles": n, "grid": "x".join(map(str, scene.nelem)),
         "plain_particle_steps_per_s": plain_rate, "plain_update_us": plain_upd,
         "candidates_in_collect_window": offered, "samples": st["samples"], "sample_rate": rate,
         "collect_steps": collect_steps, "collect_s": round(t_collect, 3), "fit_s": round(t_fit, 3),
         "fit_epochs": epochs, "fit_batch": a.sur_batch, "fit_loss": loss, "timed_steps": timed_steps_n,
-        "runs": runs, "calibrated": cal,
+        "runs": runs, "calibrated": cal, "network_as_separate_pass": queued,
     }
 
 

@@ -73,7 +73,8 @@ enum {
     OPT_FUSED_COUNT = 5,
     OPT_CELLS_PREFETCH = 6,
     OPT_CELLS_GRID = 7,
-    OPT_X_MERGE = 8
+    OPT_X_MERGE = 8,
+    OPT_SURROGATE_FUSED = 9
 };
 
 struct mpm_engine {
@@ -181,6 +182,7 @@ struct mpm_engine {
         int64_t qcap = 0;                   // rows: regions * kSurRegion, one region per update block
         uint32_t thr = 0, step_tag = 0;
         int kind = kSurJ2;  // the model the network stands in for (from the materials)
+        bool fused = true;  // the network runs inside the update kernel (else a queue and a network pass)
         bool sorted = false;
         float loss = 0.f, tol = 1e-4f;
         void free_log() {
@@ -463,7 +465,7 @@ int do_g2p(mpm_engine *e, double facl, int diag, hipEvent_t mid, hipEvent_t mid2
                 su.logged_steps++;
             }
         } else if (su.enabled && su.have_net && su.queue && su.qcap >= sur_queue_rows(e->n)) {
-            sa.mode = 2;
+            sa.mode = su.fused ? 3 : 2;
             sa.kind = su.kind;
             sa.tol = su.tol;
             sa.net = su.net.view();
@@ -2014,6 +2016,9 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) try {
         return 0;
     case OPT_X_MERGE:
         e->x_merge = value != 0;
+        return 0;
+    case OPT_SURROGATE_FUSED:
+        e->sur.fused = value != 0;
         return 0;
     case OPT_CELLS_PREFETCH:
         e->drop_graph();

@@ -24,6 +24,7 @@
 #include "fast32_device.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace mpm {
 
@@ -653,8 +654,19 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, T
 }
 
 
+// Mode 3 of the update kernel: the network weights in MFMA-fragment order,
+// the feature scaling, and one 64-row exchange tile per wave.
+struct SurFusedSmem {
+    MlpInferSmem sw;
+    float scl[64];
+    __attribute__((aligned(16))) uint16_t xs[4][kMlpWaveScratchBytes / 2];
+};
+
+// SUR: 0 plain; 1 plain with sample logging; 2 queues the network's
+// candidates for k32_surrogate_pass; 3 runs the network here, on the rows of
+// each wave's own lanes (no queue), with the same arithmetic as mode 2.
 template <int SUR>
-__global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k32_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
                                                        MaterialTable mt, StepConsts sc, uint32_t *err,
                                                        SurrogateArgs sa, BinFuse bf) {
     const int slot0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -663,6 +675,12 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
         if (slot0 == 0) {
             bf.ctr->n_active = 0u;
         }
+    }
+    __shared__ std::conditional_t<SUR == 3, SurFusedSmem, int> fsm;
+    if constexpr (SUR == 3) {
+        // staged while the particle streams load; first read after the trial state
+        mlp_load_infer_smem(fsm.sw, sa.net);
+        if (threadIdx.x < 64) fsm.scl[threadIdx.x] = sa.net_scaling[threadIdx.x];
     }
     // Wave-wide steps (the network, the run-aggregated cell counts) need every
     // lane: no lane leaves early; invalid lanes compute on slot 0 / cell 0
@@ -823,6 +841,57 @@ __global__ void __launch_bounds__(256) k32_g2p_update(GridGeom g, PartF32 P, Nod
                     }
                 }
             }
+        } else if constexpr (SUR == 3) {
+            // the wave's candidates run through the network here, one 64-row
+            // tile per wave (rows of non-candidate lanes are computed and
+            // ignored); each output depends only on its own row, so an
+            // accepted prediction equals the one the network pass of mode 2
+            // makes for the same state
+            __syncthreads();  // weights staged (every lane of the block gets here)
+            const unsigned long long m = __ballot(plastic);
+            bool accept = false;
+            if (m) {
+                float x[16];
+                if (sa.kind == kSurTait)
+                    tait_features(J, D, dtp, x);
+                else
+                    j2_features(t, D, s, x);
+                float xn[16];
+#pragma unroll
+                for (int f = 0; f < 16; f++) xn[f] = (plastic && f < kMlpInUsed) ? (x[f] - fsm.scl[f]) * fsm.scl[16 + f] : 0.f;
+                float y[kMlpOutUsed];
+                mlp_wave_infer(fsm.sw, xn, y, fsm.xs[threadIdx.x >> 6], lane);
+                if (plastic) {
+                    if (sa.kind == kSurTait) {
+                        // the exact pressure judges the network's; an accepted one replaces it
+                        const float pnet = y[0] * fsm.scl[48] + fsm.scl[32];
+                        if (fabsf(pnet - pex) < sa.tol * (fabsf(pex) + 1e-6f * Bt)) {
+                            accept = true;
+                            const float d = pex - pnet;
+#pragma unroll
+                            for (int cc = 0; cc < 3; cc++) s[cc] += d;
+                        }
+                    } else {
+                        float ss[6];
+#pragma unroll
+                        for (int cc = 0; cc < 6; cc++) ss[cc] = t.st[cc] - (y[cc] * fsm.scl[48 + cc] + fsm.scl[32 + cc]);
+                        const float dg = y[6] * fsm.scl[54] + fsm.scl[38];
+                        float epn = t.ep + dg;
+                        accept = j2_residual(t, ss, dg) <= sa.tol;
+                        if (!accept) j2_return(t, ss, &epn);  // fallback: the exact radial return
+#pragma unroll
+                        for (int cc = 0; cc < 6; cc++) s[cc] = ss[cc];
+                        ep = epn;
+                    }
+                }
+                const uint32_t ev = (uint32_t)__popcll(m);
+                const uint32_t rj = (uint32_t)__popcll(__ballot(plastic && !accept));
+                if (lane == 0) {
+                    atomicAdd(&sa.rstat[2 * blockIdx.x], ev);
+                    if (rj) atomicAdd(&sa.rstat[2 * blockIdx.x + 1], rj);
+                }
+            }
+            if (j2 && !plastic) j2_return(t, s, &ep);  // elastic trial state
         } else {
             // the candidates of this block are compacted, in slot order, into
             // the block's own queue region for the network pass that follows
@@ -1231,6 +1300,8 @@ void launch_f32_g2p_update(hipStream_t s, const GridGeom &g, const PartF32 &P, c
         hipLaunchKernelGGL(k32_g2p_update<1>, grid, block, 0, s, g, P, N, T, mt, c, err, sa, bf);
     else if (sa.mode == 2)
         hipLaunchKernelGGL(k32_g2p_update<2>, grid, block, 0, s, g, P, N, T, mt, c, err, sa, bf);
+    else if (sa.mode == 3)
+        hipLaunchKernelGGL(k32_g2p_update<3>, grid, block, 0, s, g, P, N, T, mt, c, err, sa, bf);
     else
         hipLaunchKernelGGL(k32_g2p_update<0>, grid, block, 0, s, g, P, N, T, mt, c, err, sa, bf);
 }

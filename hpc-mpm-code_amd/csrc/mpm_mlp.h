@@ -85,6 +85,19 @@ __device__ __forceinline__ mlp_v4s mlp_frag_bf16(float a, float b, float c, floa
     return __builtin_bit_cast(mlp_v4s, (mlp_v2u){mlp_pk_bf16(a, b), mlp_pk_bf16(c, d)});
 }
 
+// ReLU of an accumulator fragment, as the bf16 operand fragment of the next
+// layer: rounded first, then the negative halves cleared by a packed 16-bit
+// integer max (a bf16 with its sign bit set is a negative int16, and the
+// order of non-negative bf16 patterns is their integer order), the same
+// values as max(x, 0) before rounding
+__device__ __forceinline__ mlp_v4s mlp_relu_frag_bf16(mlp_v4f acc) {
+    typedef short v2s __attribute__((ext_vector_type(2)));
+    const v2s z = {0, 0};
+    const v2s lo = __builtin_elementwise_max(__builtin_bit_cast(v2s, mlp_pk_bf16(acc[0], acc[1])), z);
+    const v2s hi = __builtin_elementwise_max(__builtin_bit_cast(v2s, mlp_pk_bf16(acc[2], acc[3])), z);
+    return __builtin_bit_cast(mlp_v4s, (mlp_v2u){__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi)});
+}
+
 __device__ __forceinline__ void mlp_load_smem(MlpSmem &s, const MlpParams &p) {
     for (int i = threadIdx.x; i < kMlpHid * kMlpIn / 4; i += blockDim.x)
         ((uint2 *)s.w1)[i] = ((const uint2 *)p.w1)[i];
@@ -155,36 +168,30 @@ __device__ __forceinline__ void mlp_wave_run(const MlpInferSmem &s, uint16_t *xs
 #pragma unroll
     for (int si = 0; si < 4; si++) {
         mlp_v4s h1[4], h2[4];
+        // each accumulator starts at the bias of its rows
 #pragma unroll
         for (int hb = 0; hb < 4; hb++) {
             const mlp_v4s a = *(const mlp_v4s *)s.f1[hb][lane];
-            mlp_v4f acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb[si], mlp_v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            const float *bb = s.b1 + hb * 16 + q4;
-            h1[hb] = mlp_frag_bf16(fmaxf(acc[0] + bb[0], 0.f), fmaxf(acc[1] + bb[1], 0.f), fmaxf(acc[2] + bb[2], 0.f),
-                                   fmaxf(acc[3] + bb[3], 0.f));
+            const mlp_v4f b0 = *(const mlp_v4f *)(s.b1 + hb * 16 + q4);
+            h1[hb] = mlp_relu_frag_bf16(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb[si], b0, 0, 0, 0));
         }
 #pragma unroll
         for (int hb = 0; hb < 4; hb++) {
-            mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
+            mlp_v4f acc = *(const mlp_v4f *)(s.b2 + hb * 16 + q4);
 #pragma unroll
             for (int kb = 0; kb < 2; kb++) {
                 const mlp_v8bf a = *(const mlp_v8bf *)s.f2[hb][kb][lane];
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, mlp_cat_frag(h1[2 * kb], h1[2 * kb + 1]), acc, 0, 0, 0);
             }
-            const float *bb = s.b2 + hb * 16 + q4;
-            h2[hb] = mlp_frag_bf16(fmaxf(acc[0] + bb[0], 0.f), fmaxf(acc[1] + bb[1], 0.f), fmaxf(acc[2] + bb[2], 0.f),
-                                   fmaxf(acc[3] + bb[3], 0.f));
+            h2[hb] = mlp_relu_frag_bf16(acc);
         }
-        mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
+        mlp_v4f acc = *(const mlp_v4f *)(s.b3 + q4);
 #pragma unroll
         for (int kb = 0; kb < 2; kb++) {
             const mlp_v8bf a = *(const mlp_v8bf *)s.f3[kb][lane];
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, mlp_cat_frag(h2[2 * kb], h2[2 * kb + 1]), acc, 0, 0, 0);
         }
-        if (q4 < 8) {
-#pragma unroll
-            for (int r = 0; r < 4; r++) ys[(si * 16 + c16) * 8 + q4 + r] = acc[r] + s.b3[q4 + r];
-        }
+        if (q4 < 8) *(mlp_v4f *)(ys + (si * 16 + c16) * 8 + q4) = acc;
     }
     mlp_wave_sync();
 }

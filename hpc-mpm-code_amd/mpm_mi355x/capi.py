@@ -37,6 +37,7 @@ OPT_LANES_PER_CELL = 2
 OPT_CELLS_PREFETCH = 6
 OPT_CELLS_GRID = 7
 OPT_X_MERGE = 8
+OPT_SURROGATE_FUSED = 9
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

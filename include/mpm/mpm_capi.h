@@ -247,7 +247,10 @@ int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
  * the partials written and read). It changes the FP32 summation order, which
  * then depends on how the non-empty cells fall into groups: 0 makes every
  * node sum independent of the grouping (as needed for bit-equal results of
- * scenes packed into one grid and run alone). */
+ * scenes packed into one grid and run alone). MPM_OPT_SURROGATE_FUSED
+ * (default 1): the constitutive surrogate's network runs inside the update
+ * kernel on each wave's own candidate rows; 0 queues the candidates and runs
+ * them through a separate network pass. Both give the same results. */
 enum {
     MPM_OPT_FULL_NODE_OUTPUT = 1,
     MPM_OPT_LANES_PER_CELL = 2,
@@ -256,7 +259,8 @@ enum {
     MPM_OPT_FUSED_COUNT = 5,
     MPM_OPT_CELLS_PREFETCH = 6,
     MPM_OPT_CELLS_GRID = 7,
-    MPM_OPT_X_MERGE = 8
+    MPM_OPT_X_MERGE = 8,
+    MPM_OPT_SURROGATE_FUSED = 9
 };
 int mpm_set_option(mpm_engine *eng, int32_t key, int64_t value);
 /* Reference particle shifting (DeltaCorrectionParticleShifting,

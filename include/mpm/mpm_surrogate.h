@@ -19,7 +19,10 @@
  *   hold within `tol` (relative to the yield stress): the yield condition at
  *   the predicted state, and the radial-return stress for the predicted
  *   plastic increment. Otherwise that particle takes the exact return
- *   mapping, which the network pass then computes.
+ *   mapping. The network runs inside the update kernel on each wave's own
+ *   candidates (MPM_OPT_SURROGATE_FUSED = 1, the default) or in a pass over
+ *   the candidates queued by the update kernel (0); both give the same
+ *   stresses.
  * Tait rows (pressure closure p(J, rate)):
  *   inputs  (2 of 13): J - 1 and the volumetric strain rate tr(dF - I) / dt
  *   output  (1 of 7) : pressure

@@ -156,13 +156,14 @@ def test_fit_reduces_the_loss(trained):
     assert trained["loss"] < 0.1
 
 
-def run_pair(params, scaling, tol, nsteps=40):
+def run_pair(params, scaling, tol, nsteps=40, fused=True):
     s = j2_scene()
     plain = fp32_engine(s)
     assert plain.step(1, nsteps) == 0
     a = plain.download(s.n)
     plain.close()
     sur = fp32_engine(s)
+    sur.set_option(capi.OPT_SURROGATE_FUSED, int(fused))
     sur.surrogate_set_params(params, scaling)
     sur.surrogate_enable(True, tol)
     assert sur.step(1, nsteps) == 0
@@ -192,6 +193,22 @@ def test_residual_tolerance_bounds_the_stress_error(trained):
           flush=True)
     assert st["evaluated"] > 0
     assert err <= 1e-3
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e30])
+def test_fused_network_equals_the_network_pass(trained, tol):
+    """The network inside the update kernel (one tile of the wave's own
+    lanes) and the separate pass over the queued states give the same
+    stresses bit for bit: each output of the MFMA forward pass depends only on
+    its own row, and the judge and fallback are the same arithmetic."""
+    _, a, sa = run_pair(trained["params"], trained["scaling"], tol, fused=True)
+    _, b, sb = run_pair(trained["params"], trained["scaling"], tol, fused=False)
+    print("tol %g: fused evaluated %d fallbacks %d; queued evaluated %d fallbacks %d" % (
+        tol, sa["evaluated"], sa["fallbacks"], sb["evaluated"], sb["fallbacks"]), flush=True)
+    assert sa["evaluated"] > 0
+    assert (sa["evaluated"], sa["fallbacks"]) == (sb["evaluated"], sb["fallbacks"])
+    for f in ("coord", "vel", "stress", "eps_p", "def_grad"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
 
 
 def test_network_alone_error_is_measured(trained):
@@ -242,13 +259,14 @@ def test_tait_samples_follow_the_equation_of_state(tait_trained):
     assert np.all(t[:, 1:] == 0.0) and np.all(x[:, 2:] == 0.0) and np.all(np.isfinite(x[:, 1]))
 
 
-def tait_pair(params, scaling, tol, nsteps=60):
+def tait_pair(params, scaling, tol, nsteps=60, fused=True):
     s = scenes.tiny_tait(n_cells=16)
     plain = fp32_engine(s)
     assert plain.step(1, nsteps) == 0
     a = plain.download(s.n)
     plain.close()
     sur = fp32_engine(s)
+    sur.set_option(capi.OPT_SURROGATE_FUSED, int(fused))
     sur.surrogate_set_params(params, scaling)
     sur.surrogate_enable(True, tol)
     assert sur.step(1, nsteps) == 0
@@ -284,3 +302,15 @@ def test_tait_closure_stays_within_its_tolerance(tait_trained, tol):
     acc0 = 1.0 - st0["fallbacks"] / max(1, st0["evaluated"])
     print("  untrained network accepted %.4f" % acc0, flush=True)
     assert acc > acc0
+
+
+def test_tait_fused_network_equals_the_network_pass(tait_trained):
+    """Pressure closure: the network inside the update kernel and the
+    separate pass over the queued rows accept the same predictions and give
+    the same stresses bit for bit."""
+    _, a, sa = tait_pair(tait_trained["params"], tait_trained["scaling"], 1e-2, fused=True)
+    _, b, sb = tait_pair(tait_trained["params"], tait_trained["scaling"], 1e-2, fused=False)
+    assert sa["evaluated"] > 0 and sa["fallbacks"] < sa["evaluated"]
+    assert (sa["evaluated"], sa["fallbacks"]) == (sb["evaluated"], sb["fallbacks"])
+    for f in ("coord", "vel", "stress", "def_grad"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
