@@ -160,7 +160,8 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
                                                           uint32_t *__restrict__ active, uint32_t *__restrict__ astart,
                                                           uint32_t *__restrict__ tiles, TileGeom tg,
                                                           uint32_t *__restrict__ occ, uint32_t *__restrict__ gs,
-                                                          uint32_t gmask, int keep) {
+                                                          uint32_t gsize, uint32_t *__restrict__ gs2,
+                                                          uint32_t gsize2, int keep) {
     const int b = blockIdx.x;
     const uint32_t was = tprev[b];
     const uint64_t mine = bsum[b];
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
     int ix = (int)(c0 - (size_t)iz * nexy - (size_t)iy * nex);
     TileSet last{-1, -1, -1, -1, -1, -1};
     uint32_t st[kItems];
-    uint32_t bits = 0, gbits = 0;
+    uint32_t bits = 0, gbits = 0, gbits2 = 0;
 #pragma unroll
     for (int q = 0; q < kItems; q++) {
         st[q] = (uint32_t)pre;
@@ -219,7 +220,8 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
             active[a] = (uint32_t)(c0 + q);
             astart[a] = st[q];
             bits |= 1u << q;
-            if ((a & gmask) == 0u) gbits |= 1u << q;
+            if (a % gsize == 0u) gbits |= 1u << q;
+            if (gsize2 && a % gsize2 == 0u) gbits2 |= 1u << q;
             const TileSet t = tile_set(g, ix, iy, iz);
             if (!(t == last)) {
                 mark_tiles(tg, tiles, t);
@@ -248,11 +250,14 @@ __global__ void __launch_bounds__(kBlock) k_bin_finalize(GridGeom g, uint32_t *_
     static_assert(kItems == 16, "two threads per occupancy word");
     uint32_t w = bits << (16 * (threadIdx.x & 1));
     uint32_t gw = gbits << (16 * (threadIdx.x & 1));
+    uint32_t gw2 = gbits2 << (16 * (threadIdx.x & 1));
     w |= (uint32_t)__shfl_xor((int)w, 1, 64);
     gw |= (uint32_t)__shfl_xor((int)gw, 1, 64);
+    gw2 |= (uint32_t)__shfl_xor((int)gw2, 1, 64);
     if ((threadIdx.x & 1) == 0) {
         occ[c0 >> 5] = w;
         gs[c0 >> 5] = gw;
+        if (gsize2) gs2[c0 >> 5] = gw2;
     }
 }
 
@@ -397,8 +402,8 @@ int launch_bin(hipStream_t s, const GridGeom &g, const TileGeom &tg, const uint3
                            b.ctr);
     if (!counted) hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kBlock), 0, s, (const uint32_t *)b.cnt, b.bsum);
     hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(kBlock), 0, s, g, b.cnt, (const uint64_t *)b.bsum, nb, b.tprev,
-                       b.ctr, n, err, b.cstart, b.cend, b.active, b.astart, b.tiles, tg, b.occ, b.gs, b.gmask,
-                       keep ? 1 : 0);
+                       b.ctr, n, err, b.cstart, b.cend, b.active, b.astart, b.tiles, tg, b.occ, b.gs, b.gsize > 0 ? b.gsize : 1u,
+                       b.gs2, b.gsize2, keep ? 1 : 0);
     if (n > 0) {
         int sblocks = cdiv(n, 512);
         if (sblocks > 8192) sblocks = 8192;

@@ -248,7 +248,8 @@ struct mpm_engine {
 
     BcDense bc() const { return BcDense{d_bcmask, d_bcval}; }
     CellLists lists() const {
-        return CellLists{perm, bb.cstart, bb.cend, bb.active, bb.astart, bb.ctr, bb.tiles, bb.occ, bb.gs, bb.gmask};
+        return CellLists{perm,  bb.cstart, bb.cend, bb.active, bb.astart, bb.ctr,   bb.tiles,
+                         bb.occ, bb.gs,     bb.gsize, bb.gs2,    bb.gsize2};
     }
 };
 
@@ -332,8 +333,16 @@ int do_bin(mpm_engine *e) {
     // FP32 with fused counting keeps the counts: the update kernel then moves
     // only the counts of particles that change cell
     const bool keep = e->prec == MPM_PREC_FP32 && e->fused_count;
-    // group-start bits for the tent cells kernels' x-merge: 64 / lanes-per-cell cells per group
-    e->bb.gmask = e->x_merge ? (uint32_t)(kGroupLanes / std::max(1, e->lanes_per_cell)) - 1u : 0u;
+    // group-start bits for the cells kernels' x-merge: the tent kernels group 64 / lanes-per-cell
+    // cells (P2G and MUSL alike); the stencil kernels 64 / z-layer lanes per cell in P2G and 64 in MUSL
+    const bool xm = e->x_merge && e->prec == MPM_PREC_FP32;
+    if (e->stencil) {
+        e->bb.gsize = xm ? (uint32_t)s32_p2g_cells_per_group(e->g) : 1u;
+        e->bb.gsize2 = xm ? (uint32_t)kGroupLanes : 1u;
+    } else {
+        e->bb.gsize = xm ? (uint32_t)(kGroupLanes / std::max(1, e->lanes_per_cell)) : 1u;
+        e->bb.gsize2 = 0u;
+    }
     int rc = launch_bin(e->stream, e->g, e->tg, keys, n, e->bb, e->d_err, e->counts_ready, keep);
     e->counts_ready = keep;
     if (rc) return set_error(MPM_ERR_HIP, "binning launch failed");
@@ -723,6 +732,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
         (rc = e->dalloc(&b.bsum, bin_scan_blocks(e->g.nelem))) ||
         (rc = e->dalloc(&b.tprev, bin_scan_blocks(e->g.nelem))) || (rc = e->dalloc(&b.ctr, 1)) ||
         (rc = e->dalloc(&b.occ, ncap / 32)) || (rc = e->dalloc(&b.gs, ncap / 32)) ||
+        (rc = e->dalloc(&b.gs2, ncap / 32)) ||
         (rc = e->dalloc(&e->d_err, 1))) {
         delete e;
         return rc;
@@ -737,6 +747,7 @@ int mpm_engine_create(const mpm_grid_desc *grid, const mpm_material_desc *mats, 
     if (he == hipSuccess) he = hipMemset(b.cend, 0, ncap * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.occ, 0, ncap / 32 * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.gs, 0, ncap / 32 * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(b.gs2, 0, ncap / 32 * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.tprev, 0, bin_scan_blocks(e->g.nelem) * sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(b.ctr, 0, sizeof(BinCounters));
     if (he != hipSuccess) {

@@ -65,7 +65,7 @@ __device__ __forceinline__ void xmerge_store(float *v, int sub, const CellInfo &
     constexpr int NQP = padded_nq<NQ>();
     constexpr int GPL = groups_per_lane<LPC>();
     const int lane = threadIdx.x;
-    const bool xm = L.gmask != 0u;
+    const bool xm = L.gsize > 1u;
     const int pc = __shfl_up(si.c, LPC, 64);    // previous cell of the group
     const int nc = __shfl_down(si.c, LPC, 64);  // next cell of the group
     const bool merge = xm && lane >= LPC && si.c > 0 && pc == si.c - 1 && si.ix > 0;

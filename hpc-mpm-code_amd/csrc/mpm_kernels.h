@@ -54,8 +54,10 @@ struct CellLists {
     const BinCounters *ctr;
     const uint32_t *tiles;
     const uint32_t *occ;     // one bit per cell: the cell holds particles
-    const uint32_t *gs;      // one bit per cell: the cell starts a group of the FP32 tent cells kernels
-    uint32_t gmask;          // cells per group minus 1 of the binning that wrote gs; 0: no x-merge
+    const uint32_t *gs;      // one bit per cell: the cell starts a group of the FP32 cells kernels (P2G)
+    uint32_t gsize;          // cells per group of the binning that wrote gs; 1: every cell starts one (no x-merge)
+    const uint32_t *gs2;     // the same for the MUSL groups of the stencil kernels (64 cells)
+    uint32_t gsize2;
 };
 
 struct TileGeom {
@@ -102,8 +104,10 @@ struct BinBuffers {
     uint32_t *tiles;    // [ntiles]
     uint32_t *occ;      // [bin_cell_capacity / 32] occupancy bits (zero at allocation)
     uint32_t *gs;       // [bin_cell_capacity / 32] group-start bits (zero at allocation): set for the
-                        // active cells whose active-list index is a multiple of gmask + 1
-    uint32_t gmask;     // cells per group of the FP32 tent cells kernels, minus 1 (a power of 2, minus 1)
+                        // active cells whose active-list index is a multiple of gsize
+    uint32_t gsize;     // cells per group of the FP32 P2G cells kernels (1: no x-merge)
+    uint32_t *gs2;      // [bin_cell_capacity / 32] the same for gsize2 (not written when gsize2 == 0)
+    uint32_t gsize2;    // cells per group of the stencil MUSL cells kernels (0: unused)
     uint64_t *bsum;     // [bin_scan_blocks(nelem)] packed (non-empty cells, particles) per tile of
                         // kBinTile cells: written by the reduce pass, or kept with kept counts
     uint32_t *tprev;    // [bin_scan_blocks(nelem)] tile non-empty at the previous binning (zero at allocation)
@@ -299,6 +303,8 @@ void launch_f32_g2p_vel_musl(hipStream_t s, const GridGeom &g, const PartF32 &P,
 // tent-function PIC / FLIP step keeps the kernels above.
 bool s32_supported(const GridGeom &g);
 int s32_partials_per_cell(const GridGeom &g);
+// cells per group of the stencil P2G cells kernel (64 / lanes per cell)
+int s32_p2g_cells_per_group(const GridGeom &g);
 void launch_s32_p2g(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF32 &N, const CellLists &L,
                     const BcDense &bc, const NodeTables &T, const TileGeom &tg, const StepConsts &c, float *partials,
                     int diag, hipEvent_t mid = nullptr);

@@ -166,6 +166,86 @@ __device__ __forceinline__ void affine_matrix(const float *stage, int i, int fC,
         for (int b = 0; b < 3; b++) Mh[a][b] = M[3 * a + b] * dx[b];
 }
 
+// ------------------------------------------------------------------ x-merge
+// Cells that follow each other along x share NX - 1 of their NX corner
+// columns. In a cells kernel the lanes of one z-layer hold the consecutive
+// non-empty cells of a group (lane = layer * G + u). A cell is linked to its
+// left neighbour when that neighbour is the previous cell of its group; its
+// column values become S[NX - 1] = col[NX - 1] and S[ii] = col[ii] + S[ii + 1]
+// of the left neighbour (ii < NX - 1), so S[0] holds everything the cells of
+// its run up to it add to the control point of its column 0. A cell stores
+// S[0], and S[ii >= 1] only when its right neighbour does not take them (it
+// is empty, past the row's end, or starts the next group). A control point's
+// row then reads S[0] of the cell of its column 0, S[d] of the cell d to the
+// left where the cell right of that one did not take it (stencil_row_reads).
+// The node passes learn the links from the occupancy bits and the binning's
+// group-start bits of the same grouping; a group size of 1 links nothing.
+struct XLinks {
+    bool link, keep_right;
+};
+template <int G>
+__device__ __forceinline__ XLinks x_links(const CellInfo &si, int u, int nex0, bool xm) {
+    const int pc = __shfl_up(si.c, 1, 64);
+    const int nc = __shfl_down(si.c, 1, 64);
+    XLinks x;
+    x.link = xm && u >= 1 && si.c > 0 && pc == si.c - 1 && si.ix > 0;
+    x.keep_right = !(xm && u + 1 < G && nc == si.c + 1 && si.ix + 1 < nex0);
+    return x;
+}
+
+// S[ii] += S[ii + 1] of the lane below (the left neighbour) where linked,
+// for one row of NX corners of NQ quantities (called by all lanes)
+template <int NX, int NQ>
+__device__ __forceinline__ void x_merge_row(float (&S)[NX][NQ], bool link) {
+#pragma unroll
+    for (int ii = NX - 2; ii >= 0; ii--)
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            const float up = __shfl_up(S[ii + 1][f], 1, 64);
+            if (link) S[ii][f] += up;
+        }
+}
+
+// stores NQ quantities padded to NQP floats at slot q of cell c ([q][cell][NQP])
+template <int NQ, int NQP>
+__device__ __forceinline__ void store_run(float *__restrict__ part, size_t ne, int q, int c, const float (&v)[NQ]) {
+    float4 *dst = (float4 *)(part + ((size_t)q * ne + (size_t)c) * NQP);
+#pragma unroll
+    for (int h = 0; h < NQP / 4; h++)
+        dst[h] = make_float4(4 * h < NQ ? v[4 * h] : 0.f, 4 * h + 1 < NQ ? v[4 * h + 1] : 0.f,
+                             4 * h + 2 < NQ ? v[4 * h + 2] : 0.f, 4 * h + 3 < NQ ? v[4 * h + 3] : 0.f);
+}
+
+// Which stored runs control point i of one cell row adds: rd[d] for the cell
+// i - d (its column d), from the occupancy and group-start bits of the cells
+// i - NX + 1 .. i of the row (base = the row's first cell).
+template <int NX>
+__device__ __forceinline__ bool stencil_row_reads(const uint32_t *occ, const uint32_t *gs, int nex, int base, int i,
+                                                  bool (&rd)[NX]) {
+    const int xlo = max(i - (NX - 1), 0), xhi = min(i, nex - 1);
+    if (xlo > xhi) {  // past the row's end: no cell of this row holds the point
+#pragma unroll
+        for (int d = 0; d < NX; d++) rd[d] = false;
+        return false;
+    }
+    const int clo = base + xlo, chi = base + xhi;
+    const uint32_t o0 = occ[clo >> 5], o1 = occ[chi >> 5], g0 = gs[clo >> 5], g1 = gs[chi >> 5];
+    auto bit = [&](uint32_t w0, uint32_t w1, int c) -> bool {
+        return (((c >> 5) == (clo >> 5) ? w0 : w1) >> (c & 31)) & 1u;
+    };
+    bool any = false;
+#pragma unroll
+    for (int d = 0; d < NX; d++) {
+        const int cx = i - d;
+        const bool on = cx >= xlo && cx <= xhi && bit(o0, o1, base + cx);
+        any = any || on;
+        // taken by the right neighbour: it holds particles and does not start a group
+        const bool taken = d > 0 && cx + 1 <= xhi && bit(o0, o1, base + cx + 1) && !bit(g0, g1, base + cx + 1);
+        rd[d] = on && !taken;
+    }
+    return any;
+}
+
 // ------------------------------------------------------------------ P2G
 // AFF: 0 none, 1 TPIC, 2 APIC. Partials per cell: corner q (= i + NX (j +
 // NY k), stencil order k, j, i as MakSf) x {m, px, py, pz, fx, fy, fz} at
@@ -347,22 +427,36 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
             }
             wave_fence();
         }
-        if (si.ce > si.cs) {
-            // the cell count is re-read here (an empty asm makes it opaque) so
-            // that the compiler does not keep the 8 NL store addresses live in
-            // registers across the whole kernel
-            int nel = g.nelem;
-            asm volatile("" : "+s"(nel));
-            const size_t ne = (size_t)nel;
-            float *pc = part + (size_t)(NX * NY * k0) * 7 * ne + si.c;
-            float *pv = part + (size_t)(7 * NN + NX * NY * k0) * ne + si.c;
+        // corner rows merged along x, then stored as 32-byte runs (m, p, f, volume)
+        const XLinks xl = x_links<G>(si, u, g.nex[0], L.gsize > 1u);
+        const bool on = si.ce > si.cs;
+        // the cell count is re-read here (an empty asm makes it opaque) so
+        // that the compiler does not keep the store addresses live in
+        // registers across the whole kernel
+        int nel = g.nelem;
+        asm volatile("" : "+s"(nel));
+        const size_t ne = (size_t)nel;
 #pragma unroll
-            for (int ql = 0; ql < NL; ql++) {
+        for (int kl = 0; kl < KL; kl++)
 #pragma unroll
-                for (int f = 0; f < 7; f++) pc[(size_t)(ql * 7 + f) * ne] = acc[ql][f];
-                if (diag) pv[(size_t)ql * ne] = vof[ql];
+            for (int j = 0; j < NY; j++) {
+                float S[NX][8];
+#pragma unroll
+                for (int ii = 0; ii < NX; ii++) {
+                    const int ql = ii + NX * (j + NY * kl);
+#pragma unroll
+                    for (int f = 0; f < 7; f++) S[ii][f] = acc[ql][f];
+                    S[ii][7] = vof[ql];
+                }
+                x_merge_row<NX, 8>(S, xl.link);
+                if (on) {
+#pragma unroll
+                    for (int ii = 0; ii < NX; ii++)
+                        if (ii == 0 || xl.keep_right)
+                            store_run<8, 8>(part, ne, ii + NX * (j + NY * (k0 + kl)), si.c, S[ii]);
+                }
             }
-        }
+        (void)diag;
     }
 }
 
@@ -395,6 +489,48 @@ __device__ __forceinline__ bool for_node_cells(const GridGeom &g, const CellList
     return any;
 }
 
+// Sum over the cell rows around control point (i, j, k), in ascending row
+// order, of the stored runs (NQ of NQP floats) the x-merge leaves for it,
+// with Neumaier sums; false when no cell around holds particles.
+template <int NX, int NY, int NZ, int NQ, int NQP>
+__device__ __forceinline__ bool stencil_node_sums(const GridGeom &g, const uint32_t *occ, const uint32_t *gs, int i,
+                                                  int j, int k, const float *__restrict__ part, size_t ne, float *s,
+                                                  float *cp) {
+    bool any = false;
+#pragma unroll
+    for (int dk = 0; dk < NZ; dk++) {
+        const int ck = k - (NZ - 1) + dk;
+        if (ck < 0 || ck >= g.nex[2]) continue;
+#pragma unroll
+        for (int dj = 0; dj < NY; dj++) {
+            const int cj = j - (NY - 1) + dj;
+            if (cj < 0 || cj >= g.nex[1]) continue;
+            const int base = (cj + ck * g.nex[1]) * g.nex[0];
+            bool rd[NX];
+            any = stencil_row_reads<NX>(occ, gs, g.nex[0], base, i, rd) || any;
+            const int qyz = NX * ((NY - 1 - dj) + NY * (NZ - 1 - dk));
+            float4 v[NX][NQP / 4];
+#pragma unroll
+            for (int d = 0; d < NX; d++) {
+                const float4 *src = (const float4 *)(part + ((size_t)(d + qyz) * ne + (size_t)(base + i - d)) * NQP);
+#pragma unroll
+                for (int h = 0; h < NQP / 4; h++) v[d][h] = rd[d] ? src[h] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            // ascending cell order within the row: column NX - 1 of the leftmost cell first
+#pragma unroll
+            for (int d = NX - 1; d >= 0; d--) {
+                if (!rd[d]) continue;
+#pragma unroll
+                for (int f = 0; f < NQ; f++) {
+                    const float4 q = v[d][f >> 2];
+                    neu_add(s[f], cp[f], (f & 3) == 0 ? q.x : (f & 3) == 1 ? q.y : (f & 3) == 2 ? q.z : q.w);
+                }
+            }
+        }
+    }
+    return any;
+}
+
 template <int NX, int NY, int NZ, bool DIAG>
 __global__ void __launch_bounds__(256) ks_p2g_nodes(GridGeom g, CellLists L, TileGeom tg, const float *__restrict__ part,
                                                      NodesF32 N, BcDense bc, StepConsts sc) {
@@ -406,14 +542,7 @@ __global__ void __launch_bounds__(256) ks_p2g_nodes(GridGeom g, CellLists L, Til
     float s[NQ], cp[NQ];
 #pragma unroll
     for (int f = 0; f < NQ; f++) s[f] = cp[f] = 0.f;
-    const bool any = for_node_cells<NX, NY, NZ>(g, L, i, j, k, [&](int c, int q) {
-        float val[NQ];
-#pragma unroll
-        for (int f = 0; f < 7; f++) val[f] = part[(size_t)(q * 7 + f) * ne + c];
-        if (DIAG) val[7] = part[(size_t)(7 * NN + q) * ne + c];
-#pragma unroll
-        for (int f = 0; f < NQ; f++) neu_add(s[f], cp[f], val[f]);
-    });
+    const bool any = stencil_node_sums<NX, NY, NZ, NQ, 8>(g, L.occ, L.gs, i, j, k, part, ne, s, cp);
     const int nodec = g.nodec;
     if (!any) {
         if (DIAG) {
@@ -701,13 +830,25 @@ __global__ void __launch_bounds__(kSeg) ks_musl_cells(GridGeom g, PartF32 P, Cel
             }
             wave_fence();
         }
-        if (ne) {
-            const size_t nel = (size_t)g.nelem;
+        // corner rows merged along x (the MUSL grouping: one lane per cell), 16-byte runs
+        const XLinks xl = x_links<kSeg>(si, t, g.nex[0], L.gsize2 > 1u);
+        const size_t nel = (size_t)g.nelem;
 #pragma unroll
-            for (int q = 0; q < NN; q++)
+        for (int k = 0; k < NZ; k++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) part[(size_t)(q * 3 + c) * nel + si.c] = acc[q][c];
-        }
+            for (int j = 0; j < NY; j++) {
+                float S[NX][3];
+#pragma unroll
+                for (int ii = 0; ii < NX; ii++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) S[ii][c] = acc[ii + NX * (j + NY * k)][c];
+                x_merge_row<NX, 3>(S, xl.link);
+                if (ne) {
+#pragma unroll
+                    for (int ii = 0; ii < NX; ii++)
+                        if (ii == 0 || xl.keep_right) store_run<3, 4>(part, nel, ii + NX * (j + NY * k), si.c, S[ii]);
+                }
+            }
     }
 }
 
@@ -719,13 +860,7 @@ __global__ void __launch_bounds__(256) ks_musl_nodes(GridGeom g, CellLists L, Ti
     if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
     const size_t ne = (size_t)g.nelem;
     float s[3] = {0.f, 0.f, 0.f}, cp[3] = {0.f, 0.f, 0.f};
-    const bool any = for_node_cells<NX, NY, NZ>(g, L, i, j, k, [&](int c, int q) {
-        float val[3];
-#pragma unroll
-        for (int f = 0; f < 3; f++) val[f] = part[(size_t)(q * 3 + f) * ne + c];
-#pragma unroll
-        for (int f = 0; f < 3; f++) neu_add(s[f], cp[f], val[f]);
-    });
+    const bool any = stencil_node_sums<NX, NY, NZ, 3, 4>(g, L.occ, L.gs2, i, j, k, part, ne, s, cp);
     const int nodec = g.nodec;
     if (!any) {
         if (DIAG)
@@ -1262,6 +1397,12 @@ int stencil_kind(const GridGeom &g) {
 }  // namespace
 
 bool s32_supported(const GridGeom &g) { return stencil_kind(g) >= 0; }
+
+int s32_p2g_cells_per_group(const GridGeom &g) {
+    // p2g_launch: the quadratic z-axis of a stencil of 18 or more points takes one z-layer per lane
+    const int nn = (g.idimc[0] + 1) * (g.idimc[1] + 1) * (g.idimc[2] + 1);
+    return (g.idimc[2] == 2 && nn >= 18) ? kSeg / 3 : kSeg;
+}
 
 int s32_partials_per_cell(const GridGeom &g) {
     const int nn = (g.idimc[0] + 1) * (g.idimc[1] + 1) * (g.idimc[2] + 1);

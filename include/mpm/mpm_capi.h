@@ -242,9 +242,10 @@ int mpm_get_kernel_times(mpm_engine *eng, mpm_kernel_times *out);
  * next chunk's particle indices and the next group's cell ranges while a
  * chunk's fields load. MPM_OPT_CELLS_GRID (FP32, default 0 = automatic):
  * workgroups of the cells kernels. Neither changes results.
- * MPM_OPT_X_MERGE (FP32, default 1): the tent cells kernels add the corner
+ * MPM_OPT_X_MERGE (FP32, default 1): the cells kernels add the corner
  * partials of x-neighbouring cells of one group before storing them (half
- * the partials written and read). It changes the FP32 summation order, which
+ * the partials written and read for the tent functions, about a third for
+ * the quadratic B-spline). It changes the FP32 summation order, which
  * then depends on how the non-empty cells fall into groups: 0 makes every
  * node sum independent of the grouping (as needed for bit-equal results of
  * scenes packed into one grid and run alone). MPM_OPT_SURROGATE_FUSED
