@@ -450,6 +450,9 @@ def main():
 
     copy_gbps = None if a.no_probe else round(capi.bandwidth_probe(1 << 30, 20), 1)
     roofline["copy_ceiling_GBps"] = copy_gbps
+    # the update kernel's own stream mix (15 fields read, 19 written) as a
+    # plain streaming kernel: the ceiling for a kernel that moves bytes so
+    roofline["stream_mix_ceiling_GBps"] = None if a.no_probe else round(capi.stream_mix_probe(1 << 22, 20), 1)
 
     sur = None
     if not a.no_surrogate:

@@ -2194,6 +2194,47 @@ int mpm_bin_positions_noalloc(const mpm_grid_desc *grid, int64_t n, const double
     return mpm::exception_status();
 }
 
+int mpm_stream_mix_probe(int64_t n, int32_t iters, double *gbps) try {
+    if (n < 256 || n > (int64_t)1 << 28 || iters < 1 || !gbps)
+        return set_error(MPM_ERR_INVALID_ARGUMENT, "bad probe arguments");
+    // four buffer sets used in turn: more than the last-level cache holds, so
+    // every launch streams from HBM
+    constexpr int kSets = 4;
+    float *p[kSets] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    const size_t nf = stream_mix_floats((size_t)n);
+    hipError_t he = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    for (int k = 0; k < kSets && he == hipSuccess; k++) {
+        he = hipMalloc((void **)&p[k], nf * sizeof(float));
+        if (he == hipSuccess) he = hipMemsetAsync(p[k], 0, nf * sizeof(float), s);
+    }
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he == hipSuccess) {
+        for (int k = 0; k < kSets; k++) launch_stream_mix(s, p[k], (size_t)n);  // warm-up
+        he = hipEventRecord(e0, s);
+    }
+    if (he == hipSuccess) {
+        for (int i = 0; i < iters; i++) launch_stream_mix(s, p[i % kSets], (size_t)n);
+        he = hipEventRecord(e1, s);
+    }
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+    if (he != hipSuccess) rc = hip_fail(he, "mpm_stream_mix_probe", __FILE__, __LINE__);
+    else *gbps = stream_mix_bytes((size_t)n) * iters / (ms * 1e-3) / 1e9;
+    for (float *q : p)
+        if (q) (void)hipFree(q);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
+} catch (...) {
+    return mpm::exception_status();
+}
+
 int mpm_bandwidth_probe(int64_t nbytes, int32_t iters, double *gbps) try {
     if (nbytes < 4096 || iters < 1 || !gbps) return set_error(MPM_ERR_INVALID_ARGUMENT, "bad probe arguments");
     const size_t n16 = (size_t)nbytes / 16;

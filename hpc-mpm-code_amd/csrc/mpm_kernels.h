@@ -18,6 +18,11 @@ struct SortBuffers {
 
 size_t radix_hist_entries(int n);
 void launch_copy16(hipStream_t s, const void *src, void *dst, size_t n16);
+// particle-update stream mix over n particles (HBM probe): floats of one
+// buffer set, bytes moved per launch, launch
+size_t stream_mix_floats(size_t n);
+double stream_mix_bytes(size_t n);
+void launch_stream_mix(hipStream_t s, float *p, size_t n);
 int excl_scan_u32(hipStream_t s, uint32_t *data, int n, uint32_t *bsum);
 // Stable LSD radix sort of (keys_in[i], i) over nbits key bits (standalone
 // binning entry point). keys_in is not

@@ -131,6 +131,7 @@ SIGNATURES = {
     "mpm_bin_positions_noalloc": (C.c_int, [C.POINTER(GridDesc), C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),
     "mpm_bandwidth_probe": (C.c_int, [C.c_int64, C.c_int32, _dp]),
+    "mpm_stream_mix_probe": (C.c_int, [C.c_int64, C.c_int32, _dp]),
     "mpm_set_shifting": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
     "mpm_download_nodal_volume": (C.c_int, [C.c_void_p, _dp]),
     "mpm_download_affine": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -735,4 +736,12 @@ def bandwidth_probe(nbytes=1 << 30, iters=20):
     """Achieved copy bandwidth in GB/s (read + write) of a plain copy kernel."""
     out = C.c_double()
     check(load().mpm_bandwidth_probe(int(nbytes), int(iters), C.byref(out)))
+    return out.value
+
+
+def stream_mix_probe(n=1 << 22, iters=20):
+    """Achieved GB/s (read + write) of the particle update's stream mix over
+    n particles (15 fields read, 19 written, structure of arrays)."""
+    out = C.c_double()
+    check(load().mpm_stream_mix_probe(int(n), int(iters), C.byref(out)))
     return out.value

@@ -374,11 +374,19 @@ int mpm_bin_positions_noalloc(const mpm_grid_desc *grid, int64_t n, const double
                               int32_t *order, void *stream);
 
 /* Measurement helper with no counterpart in the reference: copies nbytes
- * device-to-device iters times with a plain 16-byte-per-lane copy kernel and
- * returns the achieved bandwidth (bytes read + written per second / 1e9).
- * Gives the measured HBM ceiling next to the vendor peak, and the reference
- * kernel for scaling rocprof byte counters. */
+ * device-to-device iters times with a plain 16-byte-per-lane copy kernel (one
+ * word per thread) and returns the achieved bandwidth (bytes read + written
+ * per second / 1e9). Gives the measured HBM ceiling next to the vendor peak,
+ * and the reference kernel for scaling rocprof byte counters. */
 int mpm_bandwidth_probe(int64_t nbytes, int32_t iters, double *gbps);
+/* Measurement helper with no counterpart in the reference: the stream mix of
+ * the FP32 particle update over n particles (15 four-byte fields read, 13 of
+ * them written back and 6 more written, each field its own array, one
+ * particle per thread, non-temporal stores), run iters times over four buffer
+ * sets in turn (4 x 84 n bytes of device memory, more than the last-level
+ * cache) and returned as achieved GB/s of bytes read + written. The ceiling of
+ * a kernel that moves its bytes this way. n in [256, 2^28]. */
+int mpm_stream_mix_probe(int64_t n, int32_t iters, double *gbps);
 
 #ifdef __cplusplus
 }
