@@ -513,6 +513,69 @@ def test_fp32_quadratic_passes_against_oracle():
     assert rel(nd.nvof, o.nvof) < 1e-6
 
 
+def scattered(n_cells=16, n=3000, idimc=(1, 1, 1), scheme=capi.SCHEME_PIC, seed=11):
+    """Irregular occupancy: particles at random positions of two overlapping
+    boxes, 0 to about 6 per cell, so that the non-empty cells form runs of
+    every length with empty cells between them, and the cells kernels'
+    groups start anywhere in a row (the x-merge of the corner partials sees
+    every case of linked, unlinked and group-start neighbours)."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n_cells
+    a = rng.uniform((0.1, 0.0, 0.2), (0.6, 0.5, 0.7), (n // 2, 3))
+    b = rng.uniform((0.35, 0.05, 0.1), (0.9, 0.3, 0.45), (n - n // 2, 3))
+    coord = np.concatenate([a, b])
+    rho, volp = 1000.0, h ** 3 / 4.0
+    vel = rng.uniform(-0.05, 0.05, (n, 3))
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    parts.stress[:] = rng.uniform(-1e3, 1e3, parts.stress.shape)
+    nelem = (n_cells,) * 3
+    floor = scenes.node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0, idimc)
+    bcs = {0: scenes._bc(floor), 1: scenes._bc(floor), 2: scenes._bc(floor)}
+    return scenes.Scene("scattered", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), nelem, [(capi.MODEL_LINELAST, [1e6, 0.3])],
+                        rho, 1e-5, 10, (0.0, -9.81, 0.0), scheme=scheme, bcs=bcs, parts=parts, idimc=tuple(idimc))
+
+
+@pytest.mark.parametrize("idimc,lanes", [((1, 1, 1), 1), ((1, 1, 1), 2), ((1, 1, 1), 4), ((1, 1, 1), 8),
+                                         ((2, 2, 2), 0), ((2, 1, 2), 0)])
+def test_fp32_scattered_cells_p2g_and_musl_against_oracle(idimc, lanes):
+    """One P2G and one G2P velocity + MUSL pass on irregular occupancy (runs
+    of non-empty cells of every length, groups starting anywhere): every
+    node's mass, momentum, force, volume and MUSL velocity against the
+    oracle's, node by node, with and without the x-merge of the corner
+    partials (a missed or doubled corner would be off by a whole particle's
+    contribution at that node)."""
+    s = scattered(idimc=idimc)
+    o = pyoracle.Oracle(s)
+    assert o.bin() == 0
+    o.p2g(1.0)
+    # P2G node fields (the MUSL pass of g2p recomputes the node momentum)
+    p2g = {f: getattr(o, f).copy() for f in ("nmass", "nvof", "nmome", "nforce")}
+    o.solve_grid()
+    o.g2p(1.0)
+    for xm in (1, 0):
+        e = engine(s, capi.PREC_FP32)
+        e.set_option(capi.OPT_X_MERGE, xm)
+        if lanes:
+            e.set_option(capi.OPT_LANES_PER_CELL, lanes)
+        e.bin()
+        e.p2g(1.0)
+        nd = e.nodes()
+        for f, tol in (("nmass", 1e-6), ("nvof", 1e-6), ("nmome", 1e-5), ("nforce", 1e-4)):
+            err = np.max(np.abs(getattr(nd, f) - p2g[f])) / np.max(np.abs(p2g[f]))
+            assert err < tol, (f, err)
+        e.solve_grid()
+        e.g2p(1.0)
+        got = e.download(s.n)
+        nd = e.nodes()
+        print("idimc %s lanes %d x-merge %d: velocity %.2e node velocity %.2e position %.2e" % (
+            idimc, lanes, xm, rel(got.vel, o.parts.vel), rel(nd.nvel, o.nvel),
+            np.max(np.abs(got.coord - o.parts.coord))), flush=True)
+        assert rel(got.vel, o.parts.vel) < 1e-5
+        assert np.max(np.abs(nd.nvel - o.nvel)) < 1e-5 * np.max(np.abs(o.nvel))
+        assert np.max(np.abs(got.coord - o.parts.coord)) < 1e-7
+        e.close()
+
+
 def test_fp32_stencil_run_is_deterministic():
     s = scenes.tiny(n_cells=10, idimc=(2, 2, 2), scheme=capi.SCHEME_APIC)
     outs = []
