@@ -21,6 +21,8 @@
 // points are the reference's table rewritten in r.
 #include "fast32_device.h"
 
+#include <type_traits>
+
 #include <algorithm>
 
 namespace mpm {
@@ -901,6 +903,10 @@ __global__ void __launch_bounds__(256) ks_musl_nodes(GridGeom g, CellLists L, Ti
 template <int NX, int NY, int NZ, bool FB>
 __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, NodesF32 N, NodeTables T,
                                                       MaterialTable mt, StepConsts sc, uint32_t *err, BinFuse bf) {
+    // stencils of 18 or more points gather through a per-wave LDS tile of node values
+    constexpr bool kTileGather = NX * NY * NZ >= 18;
+    constexpr int kTileX = 32;
+    __shared__ std::conditional_t<kTileGather, float4[4][kTileX * NY * NZ], int> ntile;
     const int slot0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (bf.cnt) {
         for (int i = slot0; i < bf.ntiles; i += gridDim.x * blockDim.x) bf.tiles[i] = 0u;
@@ -949,38 +955,87 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
     float dxp[3] = {0.f, 0.f, 0.f};
     float je = 0.f;  // FB: Jbar - 1 = sum N (nJbar - 1)
     // tensor-product form: along each node row (fixed j, k) the x sums
-    // sum_i v N_x and sum_i v dN_x/dr first, then the row's y and z factors
+    // sum_i v N_x and sum_i v dN_x/dr first, then the row's y and z factors;
+    // nodev(i, j, k) gives the node's MUSL velocity and (F-bar) nJbar - 1
+    auto accumulate = [&](auto nodev) {
 #pragma unroll
-    for (int k = 0; k < NZ; k++)
+        for (int k = 0; k < NZ; k++)
 #pragma unroll
-        for (int j = 0; j < NY; j++) {
-            float rw[3] = {0.f, 0.f, 0.f}, rd[3] = {0.f, 0.f, 0.f}, rj = 0.f;
+            for (int j = 0; j < NY; j++) {
+                float rw[3] = {0.f, 0.f, 0.f}, rd[3] = {0.f, 0.f, 0.f}, rj = 0.f;
 #pragma unroll
-            for (int i = 0; i < NX; i++) {
-                const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
-                const float v[3] = {N.nvm[nid], N.nvm[nodec + nid], N.nvm[2 * nodec + nid]};
+                for (int i = 0; i < NX; i++) {
+                    const float4 v4 = nodev(i, j, k);
+                    const float v[3] = {v4.x, v4.y, v4.z};
 #pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    rw[a] += Ax.w[i] * v[a];
-                    if constexpr (!FB) rd[a] += Ax.d[i] * v[a];
+                    for (int a = 0; a < 3; a++) {
+                        rw[a] += Ax.w[i] * v[a];
+                        if constexpr (!FB) rd[a] += Ax.d[i] * v[a];
+                    }
+                    if constexpr (FB) rj += Ax.w[i] * v4.w;
                 }
-                if constexpr (FB) rj += Ax.w[i] * N.nje[nid];
-            }
-            const float wyz = Ay.w[j] * Az.w[k];
+                const float wyz = Ay.w[j] * Az.w[k];
 #pragma unroll
-            for (int a = 0; a < 3; a++) dxp[a] += wyz * rw[a];
-            if constexpr (FB) {
-                je += wyz * rj;
-            } else {
-                const float dyz = Ay.d[j] * Az.w[k], wydz = Ay.w[j] * Az.d[k];
+                for (int a = 0; a < 3; a++) dxp[a] += wyz * rw[a];
+                if constexpr (FB) {
+                    je += wyz * rj;
+                } else {
+                    const float dyz = Ay.d[j] * Az.w[k], wydz = Ay.w[j] * Az.d[k];
 #pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    Dv[3 * a] += wyz * rd[a];
-                    Dv[3 * a + 1] += dyz * rw[a];
-                    Dv[3 * a + 2] += wydz * rw[a];
+                    for (int a = 0; a < 3; a++) {
+                        Dv[3 * a] += wyz * rd[a];
+                        Dv[3 * a + 1] += dyz * rw[a];
+                        Dv[3 * a + 2] += wydz * rw[a];
+                    }
                 }
             }
+    };
+    if constexpr (kTileGather) {
+        // The wave stages the node values of a run of cells in one cell row
+        // (the lanes of the lowest pending lane's row whose stencils fit in
+        // kTileX control points from the run's first cell) in its LDS tile
+        // with a few coalesced loads per lane; those lanes then read their
+        // NX NY NZ nodes from the tile. Repeated until every particle of the
+        // wave is done: once after a relayout, when the wave's particles sit in
+        // a few consecutive cells.
+        float4 *tl = ntile[threadIdx.x >> 6];
+        const int lane = threadIdx.x & 63;
+        const int row = valid ? iy + g.nex[1] * iz : -1;
+        unsigned long long pending = __ballot(valid);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const int rowL = __shfl(row, leader, 64);
+            const bool cand = ((pending >> lane) & 1ull) && row == rowL;
+            int xl = cand ? ix : 0x7fffffff, xh = cand ? ix : -1;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                xl = min(xl, __shfl_xor(xl, o, 64));
+                xh = max(xh, __shfl_xor(xh, o, 64));
+            }
+            const int x0 = xl;
+            const int xn = min(xh, x0 + kTileX - NX) - x0 + NX;  // control points staged along x
+            const bool mine = cand && ix - x0 <= kTileX - NX;
+            const int iyL = __shfl(iy, leader, 64), izL = __shfl(iz, leader, 64);
+            const int cnt = xn * NY * NZ;
+            for (int e = lane; e < cnt; e += 64) {
+                const int xx = e % xn, yz = e / xn;
+                const int nid = (x0 + xx) + g.nxc[0] * (iyL + yz % NY) + g.nxc[0] * g.nxc[1] * (izL + yz / NY);
+                tl[e] = make_float4(N.nvm[nid], N.nvm[nodec + nid], N.nvm[2 * nodec + nid], FB ? N.nje[nid] : 0.f);
+            }
+            wave_fence();  // the tile is written before any lane reads it (one wave: LDS in order)
+            if (mine) {
+                const int xo = ix - x0;
+                accumulate([&](int i, int j, int k) { return tl[(k * NY + j) * xn + xo + i]; });
+            }
+            pending &= ~__ballot(mine);
+            wave_fence();  // every read of this tile precedes the next staging
         }
+    } else {
+        accumulate([&](int i, int j, int k) {
+            const int nid = n0 + i + j * g.nxc[0] + k * g.nxc[0] * g.nxc[1];
+            return make_float4(N.nvm[nid], N.nvm[nodec + nid], N.nvm[2 * nodec + nid], FB ? N.nje[nid] : 0.f);
+        });
+    }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         Dv[3 * a] *= inv0;
