@@ -97,6 +97,8 @@ def parse():
     ap.add_argument("--no-closure", action="store_true", help="skip the cfg5 pressure-closure measurement")
     ap.add_argument("--closure-collect", type=int, default=100)
     ap.add_argument("--closure-steps", type=int, default=100)
+    ap.add_argument("--no-quadratic", action="store_true",
+                    help="skip the reference-shape-function (idimc 2 1 2) workload")
     ap.add_argument("--no-long-run", action="store_true",
                     help="skip the mean over the configured 1000 steps and the figure after 1500 steps")
     return ap.parse_args()
@@ -275,6 +277,29 @@ def closure_vs_plain(a):
     p(J, rate) against the exact Tait pressure, which also judges it."""
     tols = [1e-3, 3e-3, 1e-2, 3e-2, 1e30]
     return surrogate_study(a, scenes.cfg5(), a.closure_collect, a.closure_steps, tols, a.sur_epochs, 2)
+
+
+def quadratic_run(a, name="cfg2-q212", steps=100, warmup=20):
+    """The reference's own shape functions (idimc 2 1 2: quadratic B-spline
+    in x and z, tent in y; data/generate/solid/input.txt) on the cfg2 cube,
+    2,097,152 particles, FP32, same kernels and options as the headline;
+    per-kernel times with HIP events."""
+    sc = make_scene(name)
+    e = capi.Engine(sc, precision=capi.PREC_FP32, relayout_every=a.relayout)
+    e.upload(sc.parts)
+    e.set_option(capi.OPT_FULL_NODE_OUTPUT, 0)
+    capi.check(e.step(1, warmup))
+    t = timed_steps(e, 1 + warmup, steps)
+    e.set_timing(True)
+    capi.check(e.step(1 + warmup + steps, 20))
+    e.set_timing(False)
+    kt = e.kernel_times()
+    e.close()
+    keys = {"bin": "bin_ms", "p2g_cells": "p2g_cells_ms", "p2g_nodes": "p2g_nodes_ms", "musl_cells": "musl_cells_ms",
+            "musl_nodes": "musl_nodes_ms", "update": "g2p_upd_ms"}
+    return {"workload": sc.name, "idimc": list(sc.idimc), "particles": sc.n, "steps": steps,
+            "ms_per_step": 1e3 * t / steps, "particle_steps_per_s": sc.n * steps / t,
+            "kernels_avg_us": {k: round(1e3 * kt[v] / 20, 2) for k, v in keys.items()}}
 
 
 def long_run(a, scene_name):
@@ -466,6 +491,9 @@ def main():
     acc = None
     if a.accuracy_steps > 0:
         acc = accuracy_run(a, a.config)
+    quad = None
+    if not a.no_quadratic and prec == capi.PREC_FP32 and a.config == "cfg2-4M":
+        quad = quadratic_run(a)
     lr = None
     if not a.no_long_run and prec == capi.PREC_FP32:
         lr = long_run(a, a.config)
@@ -503,6 +531,7 @@ def main():
         "ensemble": ens,
         "accuracy": acc,
         "long_run": lr,
+        "reference_shape_functions": quad,
     }
     print(json.dumps(out), flush=True)
 
