@@ -1147,6 +1147,7 @@ int mpm_set_fbar(mpm_engine *e, int32_t on) try {
         return set_error(MPM_ERR_UNSUPPORTED,
                          "the FP32 F-bar projection runs without the constitutive surrogate and per-material time steps");
     e->drop_graph();
+    e->bins_valid = false;  // the kernels (and their cell groups) may change
     if (!on) {
         e->fbar = false;
         e->stencil = e->stencil_base;
@@ -2014,6 +2015,7 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) try {
         if (value != 1 && value != 2 && value != 4 && value != 8)
             return set_error(MPM_ERR_INVALID_ARGUMENT, "lanes per cell must be 1, 2, 4 or 8");
         e->lanes_per_cell = (int)value;
+        e->bins_valid = false;  // the group-start bits of the binning follow the cells kernels' groups
         return 0;
     case OPT_RELAYOUT_EVERY:
         if (value < 0 || value > (1 << 20)) return set_error(MPM_ERR_INVALID_ARGUMENT, "relayout interval out of range");
@@ -2027,6 +2029,7 @@ int mpm_set_option(mpm_engine *e, int32_t key, int64_t value) try {
         return 0;
     case OPT_X_MERGE:
         e->x_merge = value != 0;
+        e->bins_valid = false;  // the group-start bits are written by the binning
         return 0;
     case OPT_SURROGATE_FUSED:
         e->sur.fused = value != 0;

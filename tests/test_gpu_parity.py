@@ -576,6 +576,29 @@ def test_fp32_scattered_cells_p2g_and_musl_against_oracle(idimc, lanes):
         e.close()
 
 
+def test_cell_grouping_options_invalidate_the_binning():
+    """The binning writes the group-start bits of the cells kernels' groups,
+    so an option that changes the groups (lanes per cell, the x-merge, F-bar's
+    switch to the stencil kernels) makes the binning stale: a pass then asks
+    for a new one instead of reading mismatched bits."""
+    s = scattered(idimc=(1, 1, 1))
+    for key, value in ((capi.OPT_LANES_PER_CELL, 4), (capi.OPT_X_MERGE, 0)):
+        e = engine(s, capi.PREC_FP32)
+        e.bin()
+        e.set_option(key, value)
+        with pytest.raises(capi.MPMError):
+            e.p2g(1.0)
+        e.bin()
+        e.p2g(1.0)
+        e.close()
+    e = engine(s, capi.PREC_FP32)
+    e.bin()
+    e.set_fbar(True)
+    with pytest.raises(capi.MPMError):
+        e.p2g(1.0)
+    e.close()
+
+
 def test_fp32_stencil_run_is_deterministic():
     s = scenes.tiny(n_cells=10, idimc=(2, 2, 2), scheme=capi.SCHEME_APIC)
     outs = []
