@@ -62,8 +62,11 @@ __device__ __forceinline__ void axis_weights(int c, float r, int nex, AxisW<W> &
         A.x[2] = 1.5f - r;
         // clamped pieces, with the reference's precedence: global index 0,
         // then 1, then the last, then the one before it (selects, not
-        // branches: the cell, and so the piece, is fixed per lane)
+        // branches: the cell, and so the piece, is fixed per lane). Cells 2 ..
+        // nex - 3 have none: a lane's cell is fixed, so interior waves skip
+        // the selects.
         const int last = nex + 1;
+        if (c >= 2 && c <= nex - 3) return;
 #pragma unroll
         for (int o = 0; o < 3; o++) {
             const int gi = c + o;
@@ -103,6 +106,7 @@ __device__ __forceinline__ void axis_weight_at(int c, float r, int nex, int o, f
     w = o == 0 ? w0 : (o == 1 ? w1 : w2);
     d = o == 0 ? d0 : (o == 1 ? d1 : d2);
     x = o == 0 ? x0 : (o == 1 ? x1 : x2);
+    if (c >= 2 && c <= nex - 3) return;  // no clamped piece in this cell
     const int last = nex + 1;
     const int gi = c + o;
     const bool e0 = gi == 0;
