@@ -114,16 +114,17 @@ struct ReduceScatter<L, L, N> {
 // bank (L (c / 4) + 16 (c % 4) + s + L k) mod 64 -- a different bank for
 // every lane of the wave (likewise for 8 points per cell).
 constexpr int kBlk = kChunk / 64;
-template <int S>
-__device__ __forceinline__ int st_idx(int f, int i) { return (f * kBlk + (i >> 6)) * S + (i & 63); }
+// B: 64-particle blocks per chunk (kBlk unless a kernel stages shorter chunks)
+template <int S, int B = kBlk>
+__device__ __forceinline__ int st_idx(int f, int i) { return (f * B + (i >> 6)) * S + (i & 63); }
 
 // cache policy of the LDS DMA: 0, or kNT for a stream read once per step
 // (non-temporal, the gfx9 SLC / gfx940+ NT bit), which keeps L2 for the data
 // the next kernels re-read
 constexpr int kNT = 2;
-template <int S, int CPOL = 0>
+template <int S, int CPOL = 0, int B = kBlk>
 __device__ __forceinline__ void lds_gather(const void *field_base, uint32_t slot, float *stage, int f, int r) {
-    __builtin_amdgcn_global_load_lds((const uint32_t *)field_base + slot, stage + st_idx<S>(f, 64 * r), 4, 0, CPOL);
+    __builtin_amdgcn_global_load_lds((const uint32_t *)field_base + slot, stage + st_idx<S, B>(f, 64 * r), 4, 0, CPOL);
 }
 
 // wait for every outstanding vector-memory load (including LDS DMA) of this wave
@@ -143,9 +144,10 @@ static_assert(kSeg == 64, "cells kernels assume one wave per block");
 __device__ __forceinline__ void wave_fence() { asm volatile("" ::: "memory"); }
 
 // slot indices of the chunk starting at sorted position base (nch particles)
-__device__ __forceinline__ void load_slots(const CellLists &L, uint32_t base, int nch, uint32_t slot[kBlk]) {
+template <int B = kBlk>
+__device__ __forceinline__ void load_slots(const CellLists &L, uint32_t base, int nch, uint32_t slot[B]) {
 #pragma unroll
-    for (int r = 0; r < kBlk; r++) slot[r] = (64 * r + (int)threadIdx.x < nch) ? L.perm[base + 64 * r + threadIdx.x] : 0u;
+    for (int r = 0; r < B; r++) slot[r] = (64 * r + (int)threadIdx.x < nch) ? L.perm[base + 64 * r + threadIdx.x] : 0u;
 }
 
 // the same for every lane, positions clamped into [base, end): exactly kBlk

@@ -148,18 +148,18 @@ constexpr int kStageS = 65;  // stage block stride (64 + 1: conflict-free lane-p
 // Affine transfer matrix of a particle in cell units of the offset: Mh[a][b]
 // = M_ab dx_b, with M the TPIC velocity gradient or the APIC B D^-1, so that
 // M (x_a - x_p) = sum_b Mh[a][b] x_b for the cell-unit offsets x_b.
-template <int AFF>
+template <int AFF, int NB = kBlk>
 __device__ __forceinline__ void affine_matrix(const float *stage, int i, int fC, const float dx[3], float Mh[3][3]) {
     float M[9];
     if constexpr (AFF == 1) {
 #pragma unroll
-        for (int q = 0; q < 9; q++) M[q] = stage[st_idx<kStageS>(fC + q, i)];
+        for (int q = 0; q < 9; q++) M[q] = stage[st_idx<kStageS, NB>(fC + q, i)];
     } else {
         float B[9], Di[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            B[q] = stage[st_idx<kStageS>(fC + q, i)];
-            Di[q] = stage[st_idx<kStageS>(fC + 9 + q, i)];
+            B[q] = stage[st_idx<kStageS, NB>(fC + q, i)];
+            Di[q] = stage[st_idx<kStageS, NB>(fC + 9 + q, i)];
         }
 #pragma unroll
         for (int a = 0; a < 3; a++)
@@ -272,7 +272,12 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
     // staged fields: 0-2 r, 3-5 v, 6 m, 7 V, 8-13 sigma, 14-16 traction, then C (9) and D^-1 (9)
     constexpr int F_C = 17;
     constexpr int NF = F_C + (AFF >= 1 ? 9 : 0) + (AFF == 2 ? 9 : 0);
-    __shared__ float stage[NF * kBlk * kStageS];
+    // chunk of PB 64-particle blocks: three for the layered stencils (a group
+    // of 21 cells holds 168 particles at 8 per cell), which keeps the stage
+    // small enough for three waves per SIMD
+    constexpr int PB = (LZ > 1) ? 3 : kBlk;
+    constexpr int PCH = 64 * PB;
+    __shared__ float stage[NF * PB * kStageS];
     const uint32_t n_active = L.ctr->n_active;
     const int ngroups = (int)((n_active + G - 1) / G);
     const int t = threadIdx.x;
@@ -314,35 +319,35 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
             for (int f = 0; f < 7; f++) acc[q][f] = 0.f;
             vof[q] = 0.f;
         }
-        for (uint32_t base = S; base < E; base += kChunk) {
-            const int nch = (int)min((uint32_t)kChunk, E - base);
-            uint32_t slot[kBlk];
-            load_slots(L, base, nch, slot);
+        for (uint32_t base = S; base < E; base += PCH) {
+            const int nch = (int)min((uint32_t)PCH, E - base);
+            uint32_t slot[PB];
+            load_slots<PB>(L, base, nch, slot);
 #pragma unroll
-            for (int r = 0; r < kBlk; r++) {
+            for (int r = 0; r < PB; r++) {
                 if (64 * r + t < nch) {
                     const uint32_t sl = slot[r];
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
-                        lds_gather<kStageS, kNT>(P.r + a * ld, sl, stage, a, r);
-                        lds_gather<kStageS, kNT>(P.v + a * ld, sl, stage, 3 + a, r);
+                        lds_gather<kStageS, kNT, PB>(P.r + a * ld, sl, stage, a, r);
+                        lds_gather<kStageS, kNT, PB>(P.v + a * ld, sl, stage, 3 + a, r);
                     }
-                    lds_gather<kStageS, kNT>(P.m, sl, stage, 6, r);
-                    lds_gather<kStageS, kNT>(P.V32, sl, stage, 7, r);
+                    lds_gather<kStageS, kNT, PB>(P.m, sl, stage, 6, r);
+                    lds_gather<kStageS, kNT, PB>(P.V32, sl, stage, 7, r);
 #pragma unroll
-                    for (int q = 0; q < 6; q++) lds_gather<kStageS, kNT>(P.sig + q * ld, sl, stage, 8 + q, r);
+                    for (int q = 0; q < 6; q++) lds_gather<kStageS, kNT, PB>(P.sig + q * ld, sl, stage, 8 + q, r);
                     if (trac) {
 #pragma unroll
-                        for (int a = 0; a < 3; a++) lds_gather<kStageS, kNT>(P.trac + a * ld, sl, stage, 14 + a, r);
+                        for (int a = 0; a < 3; a++) lds_gather<kStageS, kNT, PB>(P.trac + a * ld, sl, stage, 14 + a, r);
                     }
                     if constexpr (AFF >= 1) {
 #pragma unroll
-                        for (int q = 0; q < 9; q++) lds_gather<kStageS, kNT>(P.C + q * ld, sl, stage, F_C + q, r);
+                        for (int q = 0; q < 9; q++) lds_gather<kStageS, kNT, PB>(P.C + q * ld, sl, stage, F_C + q, r);
                     }
                     if constexpr (AFF == 2) {
 #pragma unroll
                         for (int q = 0; q < 9; q++)
-                            lds_gather<kStageS, kNT>(P.Dinv + q * ld, sl, stage, F_C + 9 + q, r);
+                            lds_gather<kStageS, kNT, PB>(P.Dinv + q * ld, sl, stage, F_C + 9 + q, r);
                     }
                 }
             }
@@ -354,22 +359,22 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                 AxisW<NX> Ax;
                 AxisW<NY> Ay;
                 AxisW<KL> Az;  // this lane's z-layers
-                axis_weights<NX>(si.ix, stage[st_idx<kStageS>(0, i)], g.nex[0], Ax);
-                axis_weights<NY>(si.iy, stage[st_idx<kStageS>(1, i)], g.nex[1], Ay);
+                axis_weights<NX>(si.ix, stage[st_idx<kStageS, PB>(0, i)], g.nex[0], Ax);
+                axis_weights<NY>(si.iy, stage[st_idx<kStageS, PB>(1, i)], g.nex[1], Ay);
                 if constexpr (LZ == 1) {
-                    axis_weights<NZ>(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], Az);
+                    axis_weights<NZ>(si.iz, stage[st_idx<kStageS, PB>(2, i)], g.nex[2], Az);
                 } else {
                     static_assert(NZ == 3 && KL == 1, "one quadratic z-layer per lane");
-                    axis_weight_at(si.iz, stage[st_idx<kStageS>(2, i)], g.nex[2], k0, Az.w[0], Az.d[0], Az.x[0]);
+                    axis_weight_at(si.iz, stage[st_idx<kStageS, PB>(2, i)], g.nex[2], k0, Az.w[0], Az.d[0], Az.x[0]);
                 }
-                const float m = stage[st_idx<kStageS>(6, i)];
-                const float mv[3] = {m * stage[st_idx<kStageS>(3, i)], m * stage[st_idx<kStageS>(4, i)],
-                                     m * stage[st_idx<kStageS>(5, i)]};
-                const float Vp = stage[st_idx<kStageS>(7, i)];
+                const float m = stage[st_idx<kStageS, PB>(6, i)];
+                const float mv[3] = {m * stage[st_idx<kStageS, PB>(3, i)], m * stage[st_idx<kStageS, PB>(4, i)],
+                                     m * stage[st_idx<kStageS, PB>(5, i)]};
+                const float Vp = stage[st_idx<kStageS, PB>(7, i)];
                 const float nV = -Vp;
                 float sg[6];
 #pragma unroll
-                for (int q = 0; q < 6; q++) sg[q] = nV * stage[st_idx<kStageS>(8 + q, i)];
+                for (int q = 0; q < 6; q++) sg[q] = nV * stage[st_idx<kStageS, PB>(8 + q, i)];
                 // -V sigma_ab / dx_b (Voigt xx,yy,zz,yz,xz,xy): f_a = sum_b (.)_ab dN/dr_b
                 const float X[3] = {sg[0] * inv0, sg[5] * inv0, sg[4] * inv0};
                 const float Y[3] = {sg[5] * inv1, sg[1] * inv1, sg[3] * inv1};
@@ -377,13 +382,13 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                 float tr[3] = {0.f, 0.f, 0.f};
                 if (trac) {
 #pragma unroll
-                    for (int a = 0; a < 3; a++) tr[a] = stage[st_idx<kStageS>(14 + a, i)];
+                    for (int a = 0; a < 3; a++) tr[a] = stage[st_idx<kStageS, PB>(14 + a, i)];
                 }
                 // momentum per corner: m v + m M (x_a - x_p), split by axis
                 float px[3][NX], py[3][NY], pz[3][KL];
                 if constexpr (AFF >= 1) {
                     float Mh[3][3];
-                    affine_matrix<AFF>(stage, i, F_C, dxs, Mh);
+                    affine_matrix<AFF, PB>(stage, i, F_C, dxs, Mh);
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
 #pragma unroll
