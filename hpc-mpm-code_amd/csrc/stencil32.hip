@@ -949,7 +949,12 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
     int rem, ix, iy;
     const int iz = fdivmod((int)key, nexy, T.inv_nexy, rem);
     iy = fdivmod(rem, g.nex[0], T.inv_nex, ix);
-    const MaterialDev md = mt.m[mat];  // copied before the node gather: its loads overlap the gather's
+    // The material is copied before the node gather, so that its loads
+    // overlap the gather's, except for the 27-point stencil: there its
+    // registers would cost a wave per SIMD.
+    constexpr bool kLateMat = NX * NY * NZ == 27;
+    MaterialDev md;
+    if constexpr (!kLateMat) md = mt.m[mat];
     AxisW<NX> Ax;
     AxisW<NY> Ay;
     AxisW<NZ> Az;
@@ -1096,6 +1101,7 @@ __global__ void __launch_bounds__(256) ks_g2p_update(GridGeom g, PartF32 P, Node
         for (int q = 0; q < 9; q++) Hs[q] = Hn[q];
     }
     const double V = J * (double)V0p;
+    if constexpr (kLateMat) md = mt.m[mat];
     int ci[3] = {ix, iy, iz};
     bool inside = true;
     const double b0 = sc.sh_geup ? *sc.sh_b0 : 0.0;
