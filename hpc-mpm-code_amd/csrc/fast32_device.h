@@ -173,11 +173,16 @@ __device__ __forceinline__ float safe_div(float p, float m, float m_small) {
 // lies outside the grid.
 __device__ __forceinline__ bool tile_node(const GridGeom &g, const CellLists &L, const TileGeom &tg, bool all_tiles,
                                           int &n, int &i, int &j, int &k) {
-    const int tile = blockIdx.x;
+    // Blocks take the tiles with x slowest. Workgroups go round the 8 XCDs in
+    // block order; with x fastest, a particle region one tile wide in x puts
+    // every busy block on ntx / gcd(ntx, 8) of them (two for ntx = 4).
+    const int nyz = tg.nty * tg.ntz;
+    const int tx = (int)blockIdx.x / nyz;
+    const int ryz = (int)blockIdx.x - tx * nyz;
+    const int ty = ryz % tg.nty;
+    const int tz = ryz / tg.nty;
+    const int tile = tx + tg.ntx * (ty + tg.nty * tz);
     if (!all_tiles && L.tiles[tile] == 0u) return false;
-    const int tx = tile % tg.ntx;
-    const int ty = (tile / tg.ntx) % tg.nty;
-    const int tz = tile / (tg.ntx * tg.nty);
     const int l = threadIdx.x;
     i = tx * kNodeTileX + (l % kNodeTileX);
     j = ty * kNodeTileY + (l / kNodeTileX) % kNodeTileY;
