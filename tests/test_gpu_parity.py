@@ -513,6 +513,50 @@ def test_fp32_quadratic_passes_against_oracle():
     assert rel(nd.nvof, o.nvof) < 1e-6
 
 
+@pytest.mark.parametrize("idimc", [(2, 2, 2), (2, 1, 2)])
+def test_fp32_quadratic_passes_at_16_points_per_cell(idimc):
+    """At 16 points per cell a 21-cell group of the layered P2G holds 336
+    particles, more than one 192-particle chunk, and a 64-cell MUSL group
+    more than four 256-particle chunks: one P2G and one G2P velocity + MUSL
+    pass against the oracle, node by node, and an APIC run (affine fields
+    staged per chunk) against the oracle over 10 steps."""
+    s = scenes.tiny(n_cells=8, ppc=(2, 4, 2), model=capi.MODEL_LINELAST, props=(1e6, 0.3), idimc=idimc)
+    rng = np.random.default_rng(5)
+    s.parts.stress[:] = rng.uniform(-1e3, 1e3, s.parts.stress.shape)
+    o = pyoracle.Oracle(s)
+    assert o.bin() == 0
+    o.p2g(1.0)
+    p2g = {f: getattr(o, f).copy() for f in ("nmass", "nvof", "nmome", "nforce")}
+    o.solve_grid()
+    o.g2p(1.0)
+    e = engine(s, capi.PREC_FP32)
+    e.bin()
+    e.p2g(1.0)
+    nd = e.nodes()
+    for f, tol in (("nmass", 1e-6), ("nvof", 1e-6), ("nmome", 1e-5), ("nforce", 1e-4)):
+        err = np.max(np.abs(getattr(nd, f) - p2g[f])) / np.max(np.abs(p2g[f]))
+        assert err < tol, (f, err)
+    e.solve_grid()
+    e.g2p(1.0)
+    got = e.download(s.n)
+    nd = e.nodes()
+    assert rel(got.vel, o.parts.vel) < 1e-5
+    assert np.max(np.abs(nd.nvel - o.nvel)) < 1e-5 * np.max(np.abs(o.nvel))
+    e.close()
+    a = scenes.tiny(n_cells=8, ppc=(2, 4, 2), model=capi.MODEL_LINELAST, props=(1e6, 0.3), scheme=capi.SCHEME_APIC,
+                    idimc=idimc)
+    print("fp32 quadratic 16 ppc idimc=%s n=%d" % (idimc, a.n), flush=True)
+    oa = oracle_run(a, 10)
+    e = engine(a, capi.PREC_FP32, relayout=4)
+    assert e.step(1, 10) == 0
+    got = e.download(a.n)
+    cm, di = e.affine(a.n, dinv=True)
+    errs = (np.max(np.abs(got.coord - oa.parts.coord)), rel(got.vel, oa.parts.vel), rel(got.stress, oa.parts.stress),
+            max(rel(cm, oa.cmat), rel(di, oa.dinv)))
+    print("  position %.2e velocity %.2e stress %.2e matrices %.2e" % errs, flush=True)
+    assert errs[0] < 1e-7 and errs[1] < 1e-5 and errs[2] < 1e-3 and errs[3] < 1e-4
+
+
 def scattered(n_cells=16, n=3000, idimc=(1, 1, 1), scheme=capi.SCHEME_PIC, seed=11):
     """Irregular occupancy: particles at random positions of two overlapping
     boxes, 0 to about 6 per cell, so that the non-empty cells form runs of
