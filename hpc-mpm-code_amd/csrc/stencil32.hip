@@ -200,13 +200,14 @@ __device__ __forceinline__ XLinks x_links(const CellInfo &si, int u, int nex0, b
 }
 
 // S[ii] += S[ii + 1] of the lane below (the left neighbour) where linked,
-// for one row of NX corners of NQ quantities (called by all lanes)
-template <int NX, int NQ>
+// for one row of NX corners of NQ quantities, the first NM of which are
+// merged (called by all lanes)
+template <int NX, int NQ, int NM = NQ>
 __device__ __forceinline__ void x_merge_row(float (&S)[NX][NQ], bool link) {
 #pragma unroll
     for (int ii = NX - 2; ii >= 0; ii--)
 #pragma unroll
-        for (int f = 0; f < NQ; f++) {
+        for (int f = 0; f < NM; f++) {
             const float up = __shfl_up(S[ii + 1][f], 1, 64);
             if (link) S[ii][f] += up;
         }
@@ -261,16 +262,20 @@ __device__ __forceinline__ bool stencil_row_reads(const uint32_t *occ, const uin
 // group, G = 64 / NZ cells per group), which cuts the accumulators of the
 // 27-point stencil from 216 to 72 registers; every corner's sum is the same
 // sequence of additions either way.
-template <int NX, int NY, int NZ, int AFF, int LZ>
+// LEAN: no traction and no diagnostic output (the node pass then adds 7
+// quantities): the corner volumes and the traction term are left out of the
+// particle loop and the traction fields out of the stage; the volume slot of
+// the stored runs holds 0.
+template <int NX, int NY, int NZ, int AFF, int LZ, bool LEAN>
 __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, CellLists L, NodeTables T,
-                                                      float *__restrict__ part, int diag) {
+                                                      float *__restrict__ part) {
     static_assert(LZ == 1 || LZ == NZ, "a lane holds all z-layers of a cell or one");
     constexpr int KL = NZ / LZ;       // z-layers per lane
     constexpr int G = kSeg / LZ;      // cells per group
     constexpr int NN = NX * NY * NZ;
     constexpr int NL = NX * NY * KL;  // corners per lane
-    // staged fields: 0-2 r, 3-5 v, 6 m, 7 V, 8-13 sigma, 14-16 traction, then C (9) and D^-1 (9)
-    constexpr int F_C = 17;
+    // staged fields: 0-2 r, 3-5 v, 6 m, 7 V, 8-13 sigma, [14-16 traction], then C (9) and D^-1 (9)
+    constexpr int F_C = LEAN ? 14 : 17;
     constexpr int NF = F_C + (AFF >= 1 ? 9 : 0) + (AFF == 2 ? 9 : 0);
     // chunk of PB 64-particle blocks: three for the layered stencils (a group
     // of 21 cells holds 168 particles at 8 per cell), which keeps the stage
@@ -284,7 +289,7 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
     const int k0 = (LZ == 1) ? 0 : t / G;  // first z-layer of this lane (LZ for an idle lane)
     const int u = (LZ == 1) ? t : t - k0 * G;
     const size_t ld = P.ld;
-    const bool trac = P.trac != nullptr;
+    const bool trac = !LEAN && P.trac != nullptr;
     const float inv0 = T.inv_dx[0], inv1 = T.inv_dx[1], inv2 = T.inv_dx[2];
     const float dxs[3] = {T.dx[0], T.dx[1], T.dx[2]};
     for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
@@ -414,7 +419,9 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
 #pragma unroll
                         for (int a = 0; a < 3; a++) {
                             Fx[a] = X[a] * wyz;
-                            Fw[a] = Y[a] * dyz + (Z[a] * wydz + tr[a] * wyz);
+                            // explicit FMAs: the lean form is the full one with tr = 0
+                            const float zw = Z[a] * wydz;
+                            Fw[a] = __builtin_fmaf(Y[a], dyz, LEAN ? zw : __builtin_fmaf(tr[a], wyz, zw));
                             Mv[a] = mv[a] * wyz;
                         }
                         const float Mw = m * wyz, Vw = Vp * wyz;
@@ -432,7 +439,7 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                                 acc[q][4 + a] += Ax.d[ii] * Fx[a];
                                 acc[q][4 + a] += wx * Fw[a];
                             }
-                            vof[q] += wx * Vw;
+                            if constexpr (!LEAN) vof[q] += wx * Vw;
                         }
                     }
             }
@@ -457,9 +464,9 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                     const int ql = ii + NX * (j + NY * kl);
 #pragma unroll
                     for (int f = 0; f < 7; f++) S[ii][f] = acc[ql][f];
-                    S[ii][7] = vof[ql];
+                    S[ii][7] = LEAN ? 0.f : vof[ql];
                 }
-                x_merge_row<NX, 8>(S, xl.link);
+                x_merge_row<NX, 8, LEAN ? 7 : 8>(S, xl.link);
                 if (on) {
 #pragma unroll
                     for (int ii = 0; ii < NX; ii++)
@@ -467,7 +474,6 @@ __global__ void __launch_bounds__(kSeg) ks_p2g_cells(GridGeom g, PartF32 P, Cell
                             store_run<8, 8>(part, ne, ii + NX * (j + NY * (k0 + kl)), si.c, S[ii]);
                 }
             }
-        (void)diag;
     }
 }
 
@@ -1411,12 +1417,23 @@ void p2g_launch(hipStream_t s, const GridGeom &g, const PartF32 &P, const NodesF
     // the quadratic z-axis takes one z-layer per lane (3 lanes per cell)
     constexpr int LZ = (NZ == 3 && NX * NY * NZ >= 18) ? NZ : 1;
     const int ng = c.cells_grid > 0 ? c.cells_grid : cell_groups(g, kSeg / LZ);
-    if (c.scheme == 2)
-        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 1, LZ>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
-    else if (c.scheme == 3)
-        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 2, LZ>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
-    else
-        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, 0, LZ>), dim3(ng), dim3(kSeg), 0, s, g, P, L, T, partials, diag);
+    const bool lean = !diag && P.trac == nullptr;
+    auto cells = [&](auto aff, auto lean_c) {
+        hipLaunchKernelGGL((ks_p2g_cells<NX, NY, NZ, decltype(aff)::value, LZ, decltype(lean_c)::value>), dim3(ng),
+                           dim3(kSeg), 0, s, g, P, L, T, partials);
+    };
+    using A0 = std::integral_constant<int, 0>;
+    using A1 = std::integral_constant<int, 1>;
+    using A2 = std::integral_constant<int, 2>;
+    if (lean) {
+        if (c.scheme == 2) cells(A1{}, std::true_type{});
+        else if (c.scheme == 3) cells(A2{}, std::true_type{});
+        else cells(A0{}, std::true_type{});
+    } else {
+        if (c.scheme == 2) cells(A1{}, std::false_type{});
+        else if (c.scheme == 3) cells(A2{}, std::false_type{});
+        else cells(A0{}, std::false_type{});
+    }
     if (mid) (void)hipEventRecord(mid, s);
     if (diag)
         hipLaunchKernelGGL((ks_p2g_nodes<NX, NY, NZ, true>), dim3(tg.ntiles), dim3(256), 0, s, g, L, tg,
