@@ -338,14 +338,16 @@ __device__ __forceinline__ bool node_rows4(const GridGeom &g, const CellLists &L
         const bool in_r = row && i < g.nex[0];
         const int cl = in_l ? base + i - 1 : 0, cr = in_r ? base + i : 0;
         const uint32_t wl = L.occ[cl >> 5], wr = L.occ[cr >> 5], gr = L.gs[cr >> 5];
-        const bool on_l = in_l && ((wl >> (cl & 31)) & 1u);
-        const bool on_r = in_r && ((wr >> (cr & 31)) & 1u);
-        const bool merged = on_l && on_r && !((gr >> (cr & 31)) & 1u);
+        // bitwise, not short-circuit: a branch would let the compiler sink a
+        // word's load into it and wait for each row in turn
+        const bool on_l = in_l & (bool)((wl >> (cl & 31)) & 1u);
+        const bool on_r = in_r & (bool)((wr >> (cr & 31)) & 1u);
+        const bool merged = on_l & on_r & !((gr >> (cr & 31)) & 1u);
         nr.cl[r] = cl;
         nr.cr[r] = cr;
-        nr.rl[r] = on_l && !merged;
+        nr.rl[r] = on_l & !merged;
         nr.rr[r] = on_r;
-        any = any || on_l || on_r;
+        any = any | on_l | on_r;
     }
     return any;
 }
@@ -361,14 +363,16 @@ __device__ __forceinline__ void node_partial_sums(const NodeRows &nr, const floa
     for (int r = 0; r < 4; r++) {
         // the node is in corner group (1 - dj) + 2 (1 - dk) of the cells of row r
         const int g = (1 - (r & 1)) + 2 * (1 - (r >> 1));
-        const float4 *pl = (const float4 *)(part + ((size_t)(4 + g) * ne + nr.cl[r]) * NQP);
-        const float4 *pr = (const float4 *)(part + ((size_t)g * ne + nr.cr[r]) * NQP);
+        // a run that is not added reads the buffer's first run (one line for
+        // the whole wave), so that every load is issued without a branch
+        const size_t ol = nr.rl[r] ? ((size_t)(4 + g) * ne + nr.cl[r]) * NQP : 0;
+        const size_t orr = nr.rr[r] ? ((size_t)g * ne + nr.cr[r]) * NQP : 0;
+        const float4 *pl = (const float4 *)(part + ol);
+        const float4 *pr = (const float4 *)(part + orr);
 #pragma unroll
         for (int h = 0; h < NQP / 4; h++) {
-            val[r][0][h] = make_float4(0.f, 0.f, 0.f, 0.f);
-            val[r][1][h] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (nr.rl[r]) val[r][0][h] = pl[h];
-            if (nr.rr[r]) val[r][1][h] = pr[h];
+            val[r][0][h] = pl[h];
+            val[r][1][h] = pr[h];
         }
     }
 #pragma unroll

@@ -183,7 +183,7 @@ __device__ __forceinline__ void affine_matrix(const float *stage, int i, int fC,
 // S[0], and S[ii >= 1] only when its right neighbour does not take them (it
 // is empty, past the row's end, or starts the next group). A control point's
 // row then reads S[0] of the cell of its column 0, S[d] of the cell d to the
-// left where the cell right of that one did not take it (stencil_row_reads).
+// left where the cell right of that one did not take it (row_reads).
 // The node passes learn the links from the occupancy bits and the binning's
 // group-start bits of the same grouping; a group size of 1 links nothing.
 struct XLinks {
@@ -225,29 +225,44 @@ __device__ __forceinline__ void store_run(float *__restrict__ part, size_t ne, i
 
 // Which stored runs control point i of one cell row adds: rd[d] for the cell
 // i - d (its column d), from the occupancy and group-start bits of the cells
-// i - NX + 1 .. i of the row (base = the row's first cell).
+// i - NX + 1 .. i of the row (base = the row's first cell). The words are
+// loaded by row_words for every row of a control point before any is decoded
+// (one round trip for all rows); a row outside the grid, or past the row's
+// end, reads word 0 and decodes to no runs.
+struct RowWords {
+    uint32_t o0, o1, g0, g1;
+    int clo, xlo, xhi;
+    bool ok;
+};
 template <int NX>
-__device__ __forceinline__ bool stencil_row_reads(const uint32_t *occ, const uint32_t *gs, int nex, int base, int i,
-                                                  bool (&rd)[NX]) {
-    const int xlo = max(i - (NX - 1), 0), xhi = min(i, nex - 1);
-    if (xlo > xhi) {  // past the row's end: no cell of this row holds the point
-#pragma unroll
-        for (int d = 0; d < NX; d++) rd[d] = false;
-        return false;
-    }
-    const int clo = base + xlo, chi = base + xhi;
-    const uint32_t o0 = occ[clo >> 5], o1 = occ[chi >> 5], g0 = gs[clo >> 5], g1 = gs[chi >> 5];
+__device__ __forceinline__ RowWords row_words(const uint32_t *occ, const uint32_t *gs, int nex, int base, int i,
+                                              bool row_ok) {
+    RowWords w;
+    w.xlo = max(i - (NX - 1), 0);
+    w.xhi = min(i, nex - 1);
+    w.ok = row_ok && w.xlo <= w.xhi;
+    w.clo = w.ok ? base + w.xlo : 0;
+    const int chi = w.ok ? base + w.xhi : 0;
+    w.o0 = occ[w.clo >> 5];
+    w.o1 = occ[chi >> 5];
+    w.g0 = gs[w.clo >> 5];
+    w.g1 = gs[chi >> 5];
+    return w;
+}
+template <int NX>
+__device__ __forceinline__ bool row_reads(const RowWords &w, int base, int i, bool (&rd)[NX]) {
     auto bit = [&](uint32_t w0, uint32_t w1, int c) -> bool {
-        return (((c >> 5) == (clo >> 5) ? w0 : w1) >> (c & 31)) & 1u;
+        return (((c >> 5) == (w.clo >> 5) ? w0 : w1) >> (c & 31)) & 1u;
     };
     bool any = false;
 #pragma unroll
     for (int d = 0; d < NX; d++) {
         const int cx = i - d;
-        const bool on = cx >= xlo && cx <= xhi && bit(o0, o1, base + cx);
+        const bool on = w.ok && cx >= w.xlo && cx <= w.xhi && bit(w.o0, w.o1, base + cx);
         any = any || on;
         // taken by the right neighbour: it holds particles and does not start a group
-        const bool taken = d > 0 && cx + 1 <= xhi && bit(o0, o1, base + cx + 1) && !bit(g0, g1, base + cx + 1);
+        const bool taken =
+            d > 0 && cx + 1 <= w.xhi && bit(w.o0, w.o1, base + cx + 1) && !bit(w.g0, w.g1, base + cx + 1);
         rd[d] = on && !taken;
     }
     return any;
@@ -513,37 +528,51 @@ template <int NX, int NY, int NZ, int NQ, int NQP>
 __device__ __forceinline__ bool stencil_node_sums(const GridGeom &g, const uint32_t *occ, const uint32_t *gs, int i,
                                                   int j, int k, const float *__restrict__ part, size_t ne, float *s,
                                                   float *cp) {
-    bool any = false;
+    // every row's occupancy and group-start words first (one round trip)
+    RowWords w[NZ][NY];
+    int base[NZ][NY];
 #pragma unroll
     for (int dk = 0; dk < NZ; dk++) {
         const int ck = k - (NZ - 1) + dk;
-        if (ck < 0 || ck >= g.nex[2]) continue;
 #pragma unroll
         for (int dj = 0; dj < NY; dj++) {
             const int cj = j - (NY - 1) + dj;
-            if (cj < 0 || cj >= g.nex[1]) continue;
-            const int base = (cj + ck * g.nex[1]) * g.nex[0];
-            bool rd[NX];
-            any = stencil_row_reads<NX>(occ, gs, g.nex[0], base, i, rd) || any;
+            const bool ok = ck >= 0 && ck < g.nex[2] && cj >= 0 && cj < g.nex[1];
+            base[dk][dj] = ok ? (cj + ck * g.nex[1]) * g.nex[0] : 0;
+            w[dk][dj] = row_words<NX>(occ, gs, g.nex[0], base[dk][dj], i, ok);
+        }
+    }
+    bool any = false;
+    // then the runs, one z-layer of rows per round trip; a run that is not
+    // added reads the buffer's first run (one line for the whole wave)
+#pragma unroll
+    for (int dk = 0; dk < NZ; dk++) {
+        bool rd[NY][NX];
+        float4 v[NY][NX][NQP / 4];
+#pragma unroll
+        for (int dj = 0; dj < NY; dj++) {
+            any = row_reads<NX>(w[dk][dj], base[dk][dj], i, rd[dj]) || any;
             const int qyz = NX * ((NY - 1 - dj) + NY * (NZ - 1 - dk));
-            float4 v[NX][NQP / 4];
 #pragma unroll
             for (int d = 0; d < NX; d++) {
-                const float4 *src = (const float4 *)(part + ((size_t)(d + qyz) * ne + (size_t)(base + i - d)) * NQP);
+                const size_t off = rd[dj][d] ? ((size_t)(d + qyz) * ne + (size_t)(base[dk][dj] + i - d)) * NQP : 0;
+                const float4 *src = (const float4 *)(part + off);
 #pragma unroll
-                for (int h = 0; h < NQP / 4; h++) v[d][h] = rd[d] ? src[h] : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int h = 0; h < NQP / 4; h++) v[dj][d][h] = src[h];
             }
-            // ascending cell order within the row: column NX - 1 of the leftmost cell first
+        }
+        // ascending cell order within each row: column NX - 1 of the leftmost cell first
+#pragma unroll
+        for (int dj = 0; dj < NY; dj++)
 #pragma unroll
             for (int d = NX - 1; d >= 0; d--) {
-                if (!rd[d]) continue;
+                if (!rd[dj][d]) continue;
 #pragma unroll
                 for (int f = 0; f < NQ; f++) {
-                    const float4 q = v[d][f >> 2];
+                    const float4 q = v[dj][d][f >> 2];
                     neu_add(s[f], cp[f], (f & 3) == 0 ? q.x : (f & 3) == 1 ? q.y : (f & 3) == 2 ? q.z : q.w);
                 }
             }
-        }
     }
     return any;
 }

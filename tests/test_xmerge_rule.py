@@ -88,7 +88,7 @@ def test_full_rows_store_about_half():
 
 # ---------------------------------------------------------------- stencils
 # The stencil kernels (hpc-mpm-code_amd/csrc/stencil32.hip: x_links,
-# x_merge_row, stencil_row_reads) apply the same rule to NX corner columns
+# x_merge_row, row_reads) apply the same rule to NX corner columns
 # (NX = 2 or 3, the quadratic B-spline) with any group size (21 cells in the
 # quadratic P2G, one z-layer of the stencil per lane): S[NX-1] = col[NX-1],
 # S[ii] = col[ii] + S[ii+1] of a linked left neighbour; S[0] always stored,
