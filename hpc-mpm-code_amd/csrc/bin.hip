@@ -283,16 +283,26 @@ __global__ void __launch_bounds__(256) k_bin_scatter(const uint32_t *__restrict_
         const bool cross = (uint32_t)__shfl((int)k0, 63, 64) == (uint32_t)__shfl((int)k1, 0, 64);
         const int head_last0 = __shfl(head0, 63, 64);  // head of the run that reaches the end of half 0
         const int len_first1 = __shfl(len1, 0, 64);
-        uint32_t b0 = 0, b1 = 0;
-        if (h0 && k0 < un) {
+        // both halves' atomics (and the cell starts they are compared with)
+        // are issued before either result is used
+        uint32_t b0 = 0, b1 = 0, s0 = 0, s1 = 0;
+        const bool a0 = h0 && k0 < un;
+        const bool a1 = h1 && k1 < un && !(cross && lane == 0);
+        if (a0) {
             const int add = len0 + ((cross && lane == head_last0) ? len_first1 : 0);
             b0 = atomicAdd(&cend[k0], (uint32_t)add);
-            if (b0 != cstart[k0]) stamp_arr[k0] = stamp;  // several runs: the fix-up sorts this cell
+            s0 = cstart[k0];
         }
-        if (h1 && k1 < un && !(cross && lane == 0)) {
+        if (a1) {
             b1 = atomicAdd(&cend[k1], (uint32_t)len1);
-            if (b1 != cstart[k1]) stamp_arr[k1] = stamp;
+            s1 = cstart[k1];
         }
+        // (an empty asm holds the compares until both atomics are issued: the
+        // compiler would otherwise move the first compare, and its wait,
+        // into the first branch)
+        asm volatile("" : "+v"(b0), "+v"(s0), "+v"(b1), "+v"(s1));
+        if (a0 && b0 != s0) stamp_arr[k0] = stamp;  // several runs: the fix-up sorts this cell
+        if (a1 && b1 != s1) stamp_arr[k1] = stamp;
         const uint32_t bb0 = (uint32_t)__shfl((int)b0, head0, 64);
         // the crossing run continues after its 64 - head_last0 slots in half 0
         const uint32_t cont = (uint32_t)__shfl((int)b0, head_last0, 64) + (uint32_t)(64 - head_last0);
