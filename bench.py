@@ -59,8 +59,8 @@ BYTES = {
 # profiles/hbm_traffic_cfg2_4M_fp32.md). Reported as roofline.traffic for
 # that workload only.
 MEASURED_TRAFFIC = {
-    "cfg2-3d-neohookean-16ppc": {"update": 573025201, "p2g_cells": 291576928, "musl_cells": 168262136,
-                                 "p2g_nodes": 54555437, "musl_nodes": 30520608},
+    "cfg2-3d-neohookean-16ppc": {"update": 573017160, "p2g_cells": 291583874, "musl_cells": 168259878,
+                                 "p2g_nodes": 54485120, "musl_nodes": 30084090},
 }
 
 
