@@ -332,10 +332,10 @@ __device__ __forceinline__ bool node_rows4(const GridGeom &g, const CellLists &L
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int cj = j - 1 + (r & 1), ck = k - 1 + (r >> 1);
-        const bool row = cj >= 0 && cj < g.nex[1] && ck >= 0 && ck < g.nex[2];
+        const bool row = (cj >= 0) & (cj < g.nex[1]) & (ck >= 0) & (ck < g.nex[2]);
         const int base = row ? (cj + ck * g.nex[1]) * g.nex[0] : 0;
-        const bool in_l = row && i >= 1;  // a node has i <= nex
-        const bool in_r = row && i < g.nex[0];
+        const bool in_l = row & (i >= 1);  // a node has i <= nex
+        const bool in_r = row & (i < g.nex[0]);
         const int cl = in_l ? base + i - 1 : 0, cr = in_r ? base + i : 0;
         const uint32_t wl = L.occ[cl >> 5], wr = L.occ[cr >> 5], gr = L.gs[cr >> 5];
         // bitwise, not short-circuit: a branch would let the compiler sink a
@@ -395,6 +395,7 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
     int n, i, j, k;
     if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
     const size_t ne = (size_t)g.nelem;
+    const int nodec = g.nodec;
     constexpr int NQ = DIAG ? 8 : 7;
     float s[NQ], cp[NQ];
 #pragma unroll
@@ -402,26 +403,16 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
         s[f] = 0.f;
         cp[f] = 0.f;
     }
+    // Straight-line up to the stores, so that no load waits behind the sums:
+    // the boundary mask is read with the occupancy words and the prescribed
+    // velocities with the runs (see node_bc); a node next to no particle sums
+    // nothing (its runs read the buffer's first one and are skipped).
+    const uint32_t mk = bc.mask ? bc.mask[n] : 0u;
     NodeRows nr;
-    // loads only from cells that hold particles: a tile's nodes next to
-    // empty cells fetch nothing for them
     const bool any = node_rows4(g, L, i, j, k, nr);
-    if (any) node_partial_sums<NQ>(nr, part, ne, s, cp);
-    const int nodec = g.nodec;
-    if (!any) {
-        if (DIAG) {
-            N.nm[n] = 0.f;
-            N.nvof[n] = 0.f;
-            for (int c = 0; c < 3; c++) {
-                N.nv[c * nodec + n] = 0.f;
-                N.na[c * nodec + n] = 0.f;
-                N.np[c * nodec + n] = 0.f;
-                N.nf[c * nodec + n] = 0.f;
-                N.nvm[c * nodec + n] = 0.f;
-            }
-        }
-        return;
-    }
+    double bv[3];
+    node_bc(bc, n, nodec, mk, bv);
+    node_partial_sums<NQ>(nr, part, ne, s, cp);
     float m = s[0] + cp[0];
     float p[3], f[3];
 #pragma unroll
@@ -437,14 +428,25 @@ __global__ void __launch_bounds__(256) k32_p2g_nodes(GridGeom g, CellLists L, Ti
             a[c] = safe_div(f[c], m, sc.m_small);
         }
     }
-    uint8_t mk = bc.mask ? bc.mask[n] : 0;
-    if (mk) {
 #pragma unroll
-        for (int c = 0; c < 3; c++)
-            if (mk & (1 << c)) {
-                v[c] = (float)(bc.val[c * nodec + n] * sc.facl);
-                a[c] = 0.f;
+    for (int c = 0; c < 3; c++)
+        if (mk & (1u << c)) {
+            v[c] = (float)(bv[c] * sc.facl);
+            a[c] = 0.f;
+        }
+    if (!any) {
+        if (DIAG) {
+            N.nm[n] = 0.f;
+            N.nvof[n] = 0.f;
+            for (int c = 0; c < 3; c++) {
+                N.nv[c * nodec + n] = 0.f;
+                N.na[c * nodec + n] = 0.f;
+                N.np[c * nodec + n] = 0.f;
+                N.nf[c * nodec + n] = 0.f;
+                N.nvm[c * nodec + n] = 0.f;
             }
+        }
+        return;
     }
     N.nm[n] = m;
 #pragma unroll
@@ -626,28 +628,30 @@ __global__ void __launch_bounds__(256) k32_musl_nodes(GridGeom g, CellLists L, T
     int n, i, j, k;
     if (!tile_node(g, L, tg, DIAG, n, i, j, k)) return;
     const size_t ne = (size_t)g.nelem;
+    const int nodec = g.nodec;
     float s[3] = {0.f, 0.f, 0.f}, cp[3] = {0.f, 0.f, 0.f};
+    // straight-line up to the stores, as in k32_p2g_nodes: the node mass and
+    // the boundary mask travel with the occupancy words
+    const float m = N.nm[n];
+    const uint32_t mk = bc.mask ? bc.mask[n] : 0u;
     NodeRows nr;
     const bool any = node_rows4(g, L, i, j, k, nr);
-    if (any) node_partial_sums<3>(nr, part, ne, s, cp);
-    const int nodec = g.nodec;
-    if (!any) {
-        if (DIAG)
-            for (int c = 0; c < 3; c++) N.nvm[c * nodec + n] = 0.f;
-        return;
-    }
-    float m = N.nm[n];
+    double bv[3];
+    node_bc(bc, n, nodec, mk, bv);
+    node_partial_sums<3>(nr, part, ne, s, cp);
     float v[3] = {0.f, 0.f, 0.f};
     float p[3] = {s[0] + cp[0], s[1] + cp[1], s[2] + cp[2]};
     if ((double)m > sc.mtol) {
 #pragma unroll
         for (int c = 0; c < 3; c++) v[c] = safe_div(p[c], m, sc.m_small);
     }
-    uint8_t mk = bc.mask ? bc.mask[n] : 0;
-    if (mk) {
 #pragma unroll
-        for (int c = 0; c < 3; c++)
-            if (mk & (1 << c)) v[c] = (float)(bc.val[c * nodec + n] * sc.facl);
+    for (int c = 0; c < 3; c++)
+        if (mk & (1u << c)) v[c] = (float)(bv[c] * sc.facl);
+    if (!any) {
+        if (DIAG)
+            for (int c = 0; c < 3; c++) N.nvm[c * nodec + n] = 0.f;
+        return;
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) N.nvm[c * nodec + n] = v[c];

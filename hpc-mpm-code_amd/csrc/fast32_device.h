@@ -192,6 +192,15 @@ __device__ __forceinline__ bool tile_node(const GridGeom &g, const CellLists &L,
     return true;
 }
 
+// Prescribed velocities of the masked components of node n (bv[c] for mask
+// bit c; an unmasked component reads the table's first value, one line for
+// the whole wave), loaded without a branch so that they travel with the node
+// pass's run loads instead of after its sums.
+__device__ __forceinline__ void node_bc(const BcDense &bc, int n, int nodec, uint32_t mk, double (&bv)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) bv[c] = bc.mask ? bc.val[((mk >> c) & 1u) ? (size_t)c * nodec + n : 0] : 0.0;
+}
+
 __device__ __forceinline__ bool cell_nonempty(const CellLists &L, int c) { return (L.occ[c >> 5] >> (c & 31)) & 1u; }
 static_assert(kSeg == kGroupLanes, "the binning's group-start bits assume one wave per group");
 
