@@ -76,6 +76,7 @@ def parse():
     ap.add_argument("--fused-count", type=int, default=1, help="count cells in the update kernel (1) or not (0)")
     ap.add_argument("--graph-block", type=int, default=0,
                     help="steps per recorded HIP graph block (0 = eager launches)")
+    ap.add_argument("--cells-grid", type=int, default=0, help="workgroups of the cells kernels (0 = automatic)")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes-per-cell", type=int, default=0, help="override (1, 2, 4, 8); 0 = automatic")
@@ -420,6 +421,8 @@ def main():
         eng.set_option(capi.OPT_LANES_PER_CELL, a.lanes_per_cell)
     if a.graph_block:
         eng.set_option(capi.OPT_GRAPH_BLOCK, a.graph_block)
+    if a.cells_grid:
+        eng.set_option(capi.OPT_CELLS_GRID, a.cells_grid)
     eng.set_option(capi.OPT_FUSED_COUNT, a.fused_count)
     capi.check(eng.step(1, a.warmup))
     capi.check(eng.lib.mpm_sync(eng.h))
