@@ -269,9 +269,10 @@ __device__ __forceinline__ bool row_reads(const RowWords &w, int base, int i, bo
 }
 
 // ------------------------------------------------------------------ P2G
-// AFF: 0 none, 1 TPIC, 2 APIC. Partials per cell: corner q (= i + NX (j +
-// NY k), stencil order k, j, i as MakSf) x {m, px, py, pz, fx, fy, fz} at
-// q * 7 + f, then the corner volumes at 7 NN + q when diag is set.
+// AFF: 0 none, 1 TPIC, 2 APIC. Partials: one 32-byte run {m, px, py, pz,
+// fx, fy, fz, volume} per (corner q, cell c) at (q * nelem + c) * 8, corner
+// q = i + NX (j + NY k) in stencil order k, j, i as MakSf; columns 1 and 2
+// of a row only where the x-merge below leaves them to the cell.
 // LZ lanes per cell (1 or NZ): with LZ = NZ each lane folds the cell's
 // particles into one z-layer of the corners (lane = layer * G + cell of the
 // group, G = 64 / NZ cells per group), which cuts the accumulators of the
