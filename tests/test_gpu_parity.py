@@ -620,6 +620,54 @@ def test_fp32_scattered_cells_p2g_and_musl_against_oracle(idimc, lanes):
         e.close()
 
 
+def wide(idimc=(1, 1, 1), x_hi=0.9, n=2500, seed=17):
+    """A grid 100 cells long in x (101 nodes: two node tiles along x) and 8
+    across, with particles at random positions in x < x_hi: the node passes'
+    blocks take their tiles with x slowest, and a region inside the first
+    x-tile (x_hi = 0.5) or across both (0.9) must give every node its sums."""
+    rng = np.random.default_rng(seed)
+    nelem = (100, 8, 8)
+    h = 0.01
+    coord = rng.uniform((0.0, 0.0, 0.0), (x_hi, 0.06, 0.08), (n, 3))
+    rho, volp = 1000.0, h ** 3 / 4.0
+    vel = rng.uniform(-0.05, 0.05, (n, 3))
+    parts = capi.Particles(n, coord=coord, vel=vel, mass=np.full(n, rho * volp), vol0=np.full(n, volp))
+    parts.stress[:] = rng.uniform(-1e3, 1e3, parts.stress.shape)
+    floor = scenes.node_lists(nelem, lambda i, j, k, nx, ny, nz: j == 0, idimc)
+    bcs = {0: scenes._bc(floor), 1: scenes._bc(floor), 2: scenes._bc(floor)}
+    return scenes.Scene("wide", (0.0, 0.0, 0.0), (1.0, 0.08, 0.08), nelem, [(capi.MODEL_LINELAST, [1e6, 0.3])],
+                        rho, 1e-5, 10, (0.0, -9.81, 0.0), bcs=bcs, parts=parts, idimc=tuple(idimc))
+
+
+@pytest.mark.parametrize("idimc", [(1, 1, 1), (2, 2, 2), (2, 1, 2)])
+@pytest.mark.parametrize("x_hi", [0.5, 0.9])
+def test_node_tiles_along_x_against_oracle(idimc, x_hi):
+    """One P2G and one G2P velocity + MUSL pass on a grid with two node tiles
+    along x: every node's mass, momentum, force, volume and MUSL velocity
+    against the oracle's, node by node."""
+    s = wide(idimc=idimc, x_hi=x_hi)
+    o = pyoracle.Oracle(s)
+    assert o.bin() == 0
+    o.p2g(1.0)
+    p2g = {f: getattr(o, f).copy() for f in ("nmass", "nvof", "nmome", "nforce")}
+    o.solve_grid()
+    o.g2p(1.0)
+    e = engine(s, capi.PREC_FP32)
+    e.bin()
+    e.p2g(1.0)
+    nd = e.nodes()
+    for f, tol in (("nmass", 1e-6), ("nvof", 1e-6), ("nmome", 1e-5), ("nforce", 1e-4)):
+        err = np.max(np.abs(getattr(nd, f) - p2g[f])) / np.max(np.abs(p2g[f]))
+        assert err < tol, (f, err)
+    e.solve_grid()
+    e.g2p(1.0)
+    got = e.download(s.n)
+    nd = e.nodes()
+    assert rel(got.vel, o.parts.vel) < 1e-5
+    assert np.max(np.abs(nd.nvel - o.nvel)) < 1e-5 * np.max(np.abs(o.nvel))
+    e.close()
+
+
 def test_cell_grouping_options_invalidate_the_binning():
     """The binning writes the group-start bits of the cells kernels' groups,
     so an option that changes the groups (lanes per cell, the x-merge, F-bar's
