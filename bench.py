@@ -72,7 +72,7 @@ def parse():
     ap.add_argument("--config", default="cfg2-4M",
                     choices=["cfg1", "cfg2", "cfg2-4M", "cfg2-q", "cfg2-q212", "cfg2-q4M", "cfg3", "cfg5", "cfg5-16M"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--relayout", type=int, default=32)
+    ap.add_argument("--relayout", type=int, default=64, help="relayout interval in steps (0 = never)")
     ap.add_argument("--fused-count", type=int, default=1, help="count cells in the update kernel (1) or not (0)")
     ap.add_argument("--graph-block", type=int, default=0,
                     help="steps per recorded HIP graph block (0 = eager launches)")
