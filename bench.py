@@ -275,7 +275,7 @@ def surrogate_vs_plain(a):
 
 def closure_vs_plain(a):
     """cfg5 dam break (4.19 M particles): the learned pressure closure
-    p(J, rate) against the exact Tait pressure, which also judges it."""
+    p(J, rate) against the exact Tait pressure, which also checks it."""
     tols = [1e-3, 3e-3, 1e-2, 3e-2, 1e30]
     return surrogate_study(a, scenes.cfg5(), a.closure_collect, a.closure_steps, tols, a.sur_epochs, 2)
 
@@ -409,8 +409,8 @@ def ensemble_run(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: this engine has no multi-GPU path (see DESIGN.md for the sharding "
-              "design); rerun with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: this engine has no multi-GPU path; rerun with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     scene = make_scene(a.config)
     prec = capi.PREC_FP32 if a.precision == "fp32" else capi.PREC_FP64_EXACT

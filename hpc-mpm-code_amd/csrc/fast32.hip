@@ -871,7 +871,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 mlp_wave_infer(fsm.sw, xn, y, fsm.xs[threadIdx.x >> 6], lane);
                 if (plastic) {
                     if (sa.kind == kSurTait) {
-                        // the exact pressure judges the network's; an accepted one replaces it
+                        // the exact pressure checks the network's; an accepted one replaces it
                         const float pnet = y[0] * fsm.scl[48] + fsm.scl[32];
                         if (fabsf(pnet - pex) < sa.tol * (fabsf(pex) + 1e-6f * Bt)) {
                             accept = true;
@@ -907,7 +907,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             // network where its residual is within the tolerance, the exact
             // return otherwise), so the stress of a queued state is not
             // stored here. Tait: the exact stress is stored and its pressure
-            // judges the network's.
+            // checks the network's.
             float x[16];
             if (plastic) {
                 if (sa.kind == kSurTait)
@@ -1086,7 +1086,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                 const int slot = __float_as_int(r[0]);
                 const size_t ld = P.ld;
                 if constexpr (KIND == kSurTait) {
-                    // pressure closure: the exact Tait pressure judges the
+                    // pressure closure: the exact Tait pressure checks the
                     // network's; an accepted one replaces it in the stored stress
                     const float pex = r[3], B = r[4];
                     const float pnet = y[0] * scl[48] + scl[32];

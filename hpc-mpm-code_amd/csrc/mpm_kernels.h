@@ -351,7 +351,7 @@ constexpr int kSurQueueRow = kSurRowJ2;
 constexpr int kSurRegion = 256;   // queue rows per update-kernel block (= its block size)
 // Models the surrogate stands in for: the J2 plastic corrector, or the Tait
 // pressure (a closure p(J, rate of volume) with the exact equation of state
-// as fallback and judge).
+// as fallback and check).
 enum { kSurJ2 = 0, kSurTait = 1 };
 struct SurrogateArgs {
     int mode;

@@ -26,7 +26,7 @@
  * Tait rows (pressure closure p(J, rate)):
  *   inputs  (2 of 13): J - 1 and the volumetric strain rate tr(dF - I) / dt
  *   output  (1 of 7) : pressure
- *   The exact Tait pressure is the fallback and the judge: a prediction is
+ *   The exact Tait pressure is the fallback and the check: a prediction is
  *   accepted where |p_net - p| < tol (|p| + 1e-6 B), B = rho0 c^2 / gamma,
  *   and then replaces p in the stress.
  * Network: 16 -> 64 -> 64 -> 16, ReLU, BF16 weights/activations, FP32

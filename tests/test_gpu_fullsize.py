@@ -85,7 +85,7 @@ def test_cfg2_fp64_engine_against_oracle(ppc):
 
 @pytest.mark.parametrize("ppc", [(2, 4, 2), (2, 2, 2)])
 def test_cfg2_fp32_after_1000_steps(ppc):
-    """The north-star accuracy statement: positions within 1e-5 of the
+    """The stated accuracy bound of this workload: positions within 1e-5 of the
     domain after the 1000 configured steps; stress within 1e-4 relative,
     velocity and node momentum within 1e-5, det F within 1e-6."""
     s = scenes.cfg2(ppc=ppc)

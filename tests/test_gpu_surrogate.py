@@ -200,7 +200,7 @@ def test_fused_network_equals_the_network_pass(trained, tol):
     """The network inside the update kernel (one tile of the wave's own
     lanes) and the separate pass over the queued states give the same
     stresses bit for bit: each output of the MFMA forward pass depends only on
-    its own row, and the judge and fallback are the same arithmetic."""
+    its own row, and the check and fallback are the same arithmetic."""
     _, a, sa = run_pair(trained["params"], trained["scaling"], tol, fused=True)
     _, b, sb = run_pair(trained["params"], trained["scaling"], tol, fused=False)
     print("tol %g: fused evaluated %d fallbacks %d; queued evaluated %d fallbacks %d" % (
@@ -285,7 +285,7 @@ def test_tait_closure_tolerance_zero_is_the_plain_method(tait_trained):
 
 @pytest.mark.parametrize("tol", [1e-3, 1e-2])
 def test_tait_closure_stays_within_its_tolerance(tait_trained, tol):
-    """The exact Tait pressure judges every prediction: accepted pressures
+    """The exact Tait pressure checks every prediction: accepted pressures
     are within tol of it (relative), so the stress error against the plain
     method stays at the tolerance's level; the trained network is accepted
     for a share of the particles that an untrained one never reaches."""
